@@ -1,0 +1,196 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/liboracle.so) and of the
+reference build (oracle/_ref/libsimcref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the timed CPU baseline.  The
+product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+MAP_DIR = os.path.join(REPO, "data", "maps")
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libsimcref.so")
+
+H_NCOUNT = 22
+S_NCOUNT = 52
+
+HMS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "slit_oct", "Q1_in", "Q1_mid", "Q1_out", "Q2_in",
+    "Q2_mid", "Q2_out", "Q3_in", "Q3_mid", "Q3_out", "D1_in", "D1_out", "hut", "dc1", "dc2",
+    "scin", "cal", "successes", "coll"]
+SHMS_COUNTERS = [
+    "trials", "successes", "targ_hor", "targ_vert", "targ_oct", "slit_hor", "slit_vert",
+    "slit_oct", "HB_in", "HB_men", "HB_mex", "HB_out", "Q1_in", "Q1_men", "Q1_mid", "Q1_mex",
+    "Q1_out", "Q2_in", "Q2_men", "Q2_mid", "Q2_mex", "Q2_out", "Q3_in", "Q3_men", "Q3_mid",
+    "Q3_mex", "Q3_out", "D1_in", "D1_flr", "D1_men", "D1_mid1", "D1_mid2", "D1_mid3",
+    "D1_mid4", "D1_mid5", "D1_mid6", "D1_mid7", "D1_mex", "D1_out", "BP_in", "BP_out", "hut",
+    "dc1", "dc2", "cer1", "cer2", "s1", "s2", "s3", "cal", "stop_id", "coll"]
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class Settings(C.Structure):
+    _fields_ = [("p_spec", C.c_double), ("m2", C.c_double), ("ctau", C.c_double),
+                ("ms_flag", C.c_int), ("wcs_flag", C.c_int), ("decay_flag", C.c_int),
+                ("using_coll", C.c_int), ("fit_f32", C.c_int), ("wire_sigma", C.c_double)]
+
+
+class EventsIn(C.Structure):
+    _fields_ = [(n, _dp) for n in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")]
+
+
+class EventsOut(C.Structure):
+    _fields_ = [(n, _dp) for n in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp",
+                                   "dy_fp", "pathlen", "resmult", "m2")] + \
+        [("ok", _u8p), ("stop", _u8p), ("used", _i32p)]
+
+
+_lib = None
+_ref = None
+_maps = None
+
+
+def build():
+    """Compile liboracle.so (and oracle/_ref when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if os.path.isdir(os.environ.get("SIMC_REFERENCE", "/root/reference")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref",
+                        "REF=" + os.environ.get("SIMC_REFERENCE", "/root/reference")],
+                       check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.or_maps_load.restype = C.c_void_p
+        L.or_maps_load.argtypes = [C.c_char_p]
+        L.or_run.restype = C.c_int
+        L.or_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(Settings), C.c_int64,
+                             C.POINTER(EventsIn), C.POINTER(EventsOut), C.c_int, _dp,
+                             C.c_int64, C.c_uint64, C.c_uint64, C.c_int, _i64p]
+        L.or_powi.restype = C.c_double
+        L.or_powi.argtypes = [C.c_double, C.c_int]
+        L.or_philox_uniform.restype = C.c_double
+        L.or_philox_uniform.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64]
+        L.or_philox4x32_10.restype = None
+        L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32)]
+        L.or_pion_coll_absorb.restype = C.c_double
+        L.or_pion_coll_absorb.argtypes = [C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def maps():
+    global _maps
+    if _maps is None:
+        p = lib().or_maps_load(MAP_DIR.encode())
+        if not p:
+            raise RuntimeError("oracle: cannot load COSY maps from " + MAP_DIR)
+        _maps = p
+    return _maps
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_LIB):
+            return None
+        L = C.CDLL(REF_LIB)
+        L.ref_run.restype = C.c_int
+        L.ref_run.argtypes = ([C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_int, C.c_int64] + [_dp] * 7 + [_dp] * 11 +
+                              [_u8p, _i32p, _dp, C.c_int64, C.c_char_p, _i64p])
+        _ref = L
+    return _ref
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _prep_outputs(n, ev):
+    out = {k: np.array(ev[k], dtype=np.float64, copy=True)
+           for k in ("dpp", "y", "dxdz", "dydz")}
+    for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2"):
+        out[k] = np.zeros(n)
+    out["ok"] = np.zeros(n, np.uint8)
+    out["stop"] = np.zeros(n, np.uint8)
+    out["used"] = np.zeros(n, np.int32)
+    return out
+
+
+def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, coll=False,
+        fit_f32=True, wire_sigma=0.0, rng="philox", tape=None, stride=0, seed=0, event0=0,
+        nthreads=None):
+    """Run the oracle over SoA events (dict of float64 arrays: dpp, x, y, z, dxdz,
+    dydz, fry).  arm: 1 HMS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'."""
+    L = lib()
+    n = len(events["dpp"])
+    ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
+          ("dpp", "x", "y", "z", "dxdz", "dydz", "fry") if k in events}
+    ein = EventsIn(*[_p(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
+    out = _prep_outputs(n, ev)
+    eout = EventsOut(*[_p(out[k]) for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp",
+                                             "y_fp", "dy_fp", "pathlen", "resmult", "m2")],
+                     _p(out["ok"], _u8p), _p(out["stop"], _u8p), _p(out["used"], _i32p))
+    S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll), int(fit_f32),
+                 wire_sigma)
+    kind = {"tape": 0, "philox": 1, "ranlux": 2}[rng]
+    cnt = np.zeros(64, np.int64)
+    tp = None
+    if kind == 0:
+        tape = np.ascontiguousarray(tape, dtype=np.float64)
+        tp = _p(tape)
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    rc = L.or_run(maps(), arm, C.byref(S), n, C.byref(ein), C.byref(eout), kind, tp,
+                  int(stride), int(seed), int(event0), int(nthreads), _p(cnt, _i64p))
+    out["rc"] = rc
+    out["counters"] = cnt[: H_NCOUNT if arm == 1 else S_NCOUNT]
+    return out
+
+
+def ref_run(arm, events, *, p_spec, m2, tape, stride, ms=True, wcs=True, decay=False,
+            ctau=0.0, coll=False):
+    """Run the reference Fortran (flang build) over the same events on a tape."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    n = len(events["dpp"])
+    ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
+          ("dpp", "x", "y", "z", "dxdz", "dydz", "fry") if k in events}
+    out = _prep_outputs(n, ev)
+    tape = np.ascontiguousarray(tape, dtype=np.float64)
+    cnt = np.zeros(64, np.int64)
+    rc = L.ref_run(arm, p_spec, m2, int(ms), int(wcs), int(decay), ctau, int(coll), n,
+                   *[_p(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")],
+                   *[_p(out[k]) for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp",
+                                          "y_fp", "dy_fp", "pathlen", "resmult", "m2")],
+                   _p(out["ok"], _u8p), _p(out["used"], _i32p), _p(tape), int(stride),
+                   MAP_DIR.encode(), _p(cnt, _i64p))
+    out["rc"] = rc
+    out["counters"] = cnt[: H_NCOUNT if arm == 1 else S_NCOUNT]
+    return out
+
+
+def philox_block(ctr, key):
+    a = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox4x32_10(a, k, o)
+    return list(o)

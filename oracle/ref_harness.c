@@ -1,0 +1,121 @@
+/*
+ * ref_harness.c -- drives the reference Fortran hot path (compiled by
+ * oracle/Makefile from the reference's own sources into oracle/_ref/) event
+ * by event.  TEST INFRASTRUCTURE ONLY.
+ *
+ * Two routines of the path cannot be compiled from the reference tree in
+ * this image (cern/lfit.f and cern/ranlux.f include CERNLIB headers that the
+ * tree does not ship), so this harness supplies:
+ *   grnd_()  -- the uniform source, read from a per-event tape (the reference
+ *               draws from RANLUX through call_ranlux.f:58-73);
+ *   lfit_()  -- the oracle's restatement of cern/lfit.f:11-60.
+ * Everything else (mc_hms, mc_hms_hut, mc_hms_recon, mc_shms, mc_shms_hut,
+ * mc_shms_recon, transp/transp_init, project, musc, musc_ext, rotate_*,
+ * gauss1, enerloss_new, loren, the collimator routines) is the reference's
+ * own code.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "simc_oracle.h"
+
+/* Fortran entry points (gfortran/flang ABI: lower case, trailing underscore,
+ * all arguments by reference, LOGICAL = 4-byte integer). */
+extern void mc_hms_(double *p_spec, double *th_spec, double *dpp, double *x, double *y, double *z,
+                    double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+                    double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
+                    int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
+                    double *pathlen, int32_t *using_coll);
+extern void mc_shms_(double *p_spec, double *th_spec, double *dpp, double *x, double *y,
+                     double *z, double *dxdz, double *dydz, double *x_fp, double *dx_fp,
+                     double *y_fp, double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
+                     int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
+                     double *pathlen, int32_t *spectr, int32_t *using_coll);
+
+/* Common blocks (hms/struct_hms.inc:66-74, shms/struct_shms.inc:320-338,
+ * simulate.inc:153 /decd/ whose first member is ctau). */
+extern struct { int32_t v[H_NCOUNT]; } spectrometer1_;
+extern struct { int32_t v[S_NCOUNT]; } spectrometer_shms_;
+extern struct { double ctau; } decd_;
+
+static const double *g_tape;
+static int64_t g_stride, g_pos, g_used;
+static int g_overflow;
+
+double grnd_(void)
+{
+    g_used++;
+    if (g_pos >= g_stride) {
+        g_overflow = 1;
+        return 0.5;
+    }
+    return g_tape[g_pos++];
+}
+
+void lfit_(float *X, float *Y, int32_t *L, int32_t *KEY, float *A, float *B, float *E)
+{
+    or_lfit_f32(X, Y, *L, *KEY, A, B, E);
+}
+
+/* Runs n events through mc_hms (arm 1) or mc_shms (arm 5).  Same SoA
+ * conventions as or_run; counters receive the change of the reference's
+ * STOP counters over the call.  mapdir must hold hms/ and shms/ map files
+ * (the reference opens them relative to the working directory). */
+int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int decay_flag,
+            double ctau, int using_coll, int64_t n, const double *dpp_in, const double *x_in,
+            const double *y_in, const double *z_in, const double *dxdz_in,
+            const double *dydz_in, const double *fry_in, double *dpp, double *y, double *dxdz,
+            double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+            double *pathlen, double *resmult, double *m2_out, uint8_t *ok, int32_t *used,
+            const double *tape, int64_t stride, const char *mapdir, int64_t *counters)
+{
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd)) return -1;
+    if (chdir(mapdir) != 0) return -2;
+    int ncount = arm == 1 ? H_NCOUNT : S_NCOUNT;
+    int32_t before[S_NCOUNT];
+    memcpy(before, arm == 1 ? spectrometer1_.v : spectrometer_shms_.v, ncount * sizeof(int32_t));
+    decd_.ctau = ctau;
+    int rc = 0;
+    for (int64_t i = 0; i < n; i++) {
+        g_tape = tape + i * stride;
+        g_stride = stride;
+        g_pos = 0;
+        g_used = 0;
+        g_overflow = 0;
+        double ps = p_spec, th = 0.0, d = dpp_in[i], xx = x_in[i], yy = y_in[i];
+        double zz = z_in ? z_in[i] : 0.0, dx = dxdz_in[i], dy = dydz_in[i];
+        double fry = fry_in ? fry_in[i] : x_in[i];
+        double xf = x_fp[i], dxf = dx_fp[i], yf = y_fp[i], dyf = dy_fp[i];
+        double m2 = m2_in, rm = resmult ? resmult[i] : 0.0, pl = pathlen[i];
+        int32_t msf = ms_flag, wcf = wcs_flag, dcf = decay_flag, okv = 0, uc = using_coll;
+        int32_t spectr = 5;
+        if (arm == 1)
+            mc_hms_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
+                    &dcf, &rm, &fry, &okv, &pl, &uc);
+        else
+            mc_shms_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf,
+                     &wcf, &dcf, &rm, &fry, &okv, &pl, &spectr, &uc);
+        dpp[i] = d;
+        y[i] = yy;
+        dxdz[i] = dx;
+        dydz[i] = dy;
+        x_fp[i] = xf;
+        dx_fp[i] = dxf;
+        y_fp[i] = yf;
+        dy_fp[i] = dyf;
+        pathlen[i] = pl;
+        if (resmult) resmult[i] = rm;
+        if (m2_out) m2_out[i] = m2;
+        ok[i] = (uint8_t)(okv != 0);
+        if (used) used[i] = (int32_t)g_used;
+        if (g_overflow) rc = OR_E_TAPE;
+    }
+    const int32_t *after = arm == 1 ? spectrometer1_.v : spectrometer_shms_.v;
+    for (int c = 0; c < ncount; c++) counters[c] += (int64_t)(after[c] - before[c]);
+    if (chdir(cwd) != 0) return -3;
+    return rc;
+}

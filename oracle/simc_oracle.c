@@ -1,0 +1,1877 @@
+/*
+ * simc_oracle.c -- FP64 CPU restatement of SIMC's spectrometer transport and
+ * reconstruction (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker for the HIP kernels and
+ * the timed CPU baseline; it is never linked into the product library.
+ *
+ * Each function follows one reference routine, cited as path:line into the
+ * reference tree (JeffersonLab/simc_gfortran).  Operation order, literal
+ * constants (all REAL literals are 8-byte under -fdefault-real-8, Makefile:63),
+ * early exits and the order in which uniforms are drawn are kept exactly, so
+ * the results agree with the reference compiled from its own sources on the
+ * same uniform tape.  Build with -ffp-contract=off (no FMA contraction), as
+ * gfortran -O on x86-64 does not contract either.
+ */
+#include "simc_oracle.h"
+
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* constants.inc:91-121 */
+#define PI_C 3.141592653589793
+static const double Mpi_C = 139.57018;
+static const double Mmu_C = 105.6583755;
+static const double Mk_C = 493.677;
+
+/* ========================================================================= */
+/* Map files                                                                  */
+/* ========================================================================= */
+
+/* Read one record into a 132-character buffer padded with blanks, like
+ * READ(chan,'(a)') into CHARACTER*132 (shared/transp.f:75,327).  Returns 0 at
+ * end of file. */
+static int f_read_record(FILE *f, char *buf)
+{
+    char tmp[4096];
+    if (!fgets(tmp, sizeof tmp, f)) return 0;
+    size_t n = strcspn(tmp, "\n");
+    if (n > 132) n = 132;
+    memset(buf, ' ', 132);
+    memcpy(buf, tmp, n);
+    buf[132] = 0;
+    return 1;
+}
+
+/* Fortran G/F/E input editing of a w-character field with d fraction digits
+ * and BLANK='NULL' (blanks ignored; an all-blank field is zero).  Without a
+ * decimal point the last d mantissa digits are the fraction. */
+static int f_real_field(const char *p, int w, int d, double *out)
+{
+    char s[64];
+    int n = 0;
+    for (int i = 0; i < w; i++)
+        if (p[i] != ' ' && p[i] != '\t' && n < 62) s[n++] = p[i];
+    s[n] = 0;
+    if (n == 0) { *out = 0.0; return 0; }
+    int i = 0, neg = 0;
+    if (s[i] == '+' || s[i] == '-') { neg = (s[i] == '-'); i++; }
+    char mant[64];
+    int nm = 0, have_point = 0, frac_digits = 0, ndig = 0;
+    for (; s[i]; i++) {
+        if (isdigit((unsigned char)s[i])) { mant[nm++] = s[i]; ndig++; if (have_point) frac_digits++; }
+        else if (s[i] == '.' && !have_point) { have_point = 1; }
+        else break;
+    }
+    if (ndig == 0) return -1;
+    long ex = 0;
+    if (s[i]) {
+        int j = i;
+        if (s[j] == 'E' || s[j] == 'e' || s[j] == 'D' || s[j] == 'd' || s[j] == 'Q' || s[j] == 'q') j++;
+        int eneg = 0;
+        if (s[j] == '+' || s[j] == '-') { eneg = (s[j] == '-'); j++; }
+        if (!isdigit((unsigned char)s[j])) return -1;
+        for (; s[j]; j++) {
+            if (!isdigit((unsigned char)s[j])) return -1;
+            ex = ex * 10 + (s[j] - '0');
+        }
+        if (eneg) ex = -ex;
+    }
+    mant[nm] = 0;
+    long scale = have_point ? -frac_digits : -d;
+    char buf[128];
+    snprintf(buf, sizeof buf, "%s%se%ld", neg ? "-" : "", mant, ex + scale);
+    *out = strtod(buf, NULL);
+    return 0;
+}
+
+/* I1 input editing: blank is zero. */
+static int f_int1(char c, int *v)
+{
+    if (c == ' ') { *v = 0; return 0; }
+    if (c >= '0' && c <= '9') { *v = c - '0'; return 0; }
+    return -1;
+}
+
+/* List-directed read of the first real on a line (shared/transp.f:330). */
+static double f_list_real(const char *p)
+{
+    while (*p == ' ' || *p == ',') p++;
+    char tok[64];
+    int n = 0;
+    while (*p && *p != ' ' && *p != ',' && *p != '/' && n < 63) tok[n++] = *p++;
+    tok[n] = 0;
+    for (int i = 0; i < n; i++)
+        if (tok[i] == 'd' || tok[i] == 'D') tok[i] = 'e';
+    return strtod(tok, NULL);
+}
+
+/* transp_init / transp_init_file, shared/transp.f:287-478. */
+int or_read_forward(const char *path, or_fwd_map *m)
+{
+    const double coeff_min = 1.0e-14; /* shared/transp.f:66 */
+    memset(m, 0, sizeof *m);
+    FILE *f = fopen(path, "r");
+    if (!f) return OR_E_IO;
+    char line[133];
+    /* strip header (:325-334) */
+    line[0] = '!';
+    while (line[0] == '!') {
+        if (!f_read_record(f, line)) { fclose(f); return OR_E_MAPFMT; }
+        if (strncmp(line, "!LENGTH:", 8) == 0) m->length[0] = 100. * f_list_real(line + 9);
+    }
+    int n_classes = 0;
+    for (int i = 0; i < OR_MAX_CLASS; i++) {
+        m->n_terms[i] = 0;
+        m->adrift[i] = 1;
+        m->driftdist[i] = 0.;
+    }
+    for (;;) {
+        int kk = n_classes; /* 0-based */
+        if (kk >= OR_MAX_CLASS) { fclose(f); return OR_E_TOOMANY; }
+        while (strncmp(line, " ---", 4) != 0) {
+            int it = m->n_terms[kk]++;
+            if (m->n_terms[kk] > OR_MAX_TERMS) { fclose(f); return OR_E_TOOMANY; }
+            double c[5];
+            int e[6];
+            for (int j = 0; j < 5; j++)
+                if (f_real_field(line + 1 + 14 * j, 14, 7, &c[j])) { fclose(f); return OR_E_MAPFMT; }
+            for (int j = 0; j < 6; j++)
+                if (f_int1(line[72 + j], &e[j])) { fclose(f); return OR_E_MAPFMT; }
+            int idummy = e[4];
+            for (int j = 0; j < 5; j++) m->coeff[kk][it][j] = c[j];
+            m->expon[kk][it][0] = (short)e[0];
+            m->expon[kk][it][1] = (short)e[1];
+            m->expon[kk][it][2] = (short)e[2];
+            m->expon[kk][it][3] = (short)e[3];
+            m->expon[kk][it][4] = (short)e[5];
+            if (idummy != 0) { /* :384-391 */
+                if (c[0] != 0 || c[1] != 0 || c[2] != 0 || c[3] != 0) { fclose(f); return OR_E_TOF; }
+                m->n_terms[kk]--;
+            }
+            if (idummy == 0 && m->adrift[kk]) { /* :399-438 */
+                int e1 = e[0], e2 = e[1], e3 = e[2], e4 = e[3];
+                double c1 = c[0], c2 = c[1], c3 = c[2], c4 = c[3];
+                double csum = fabs(c1) + fabs(c2) + fabs(c3) + fabs(c4);
+                int order = e1 + e2 + e3 + e4;
+                if (order == 1) {
+                    if (e1 == 1) {
+                        if (fabs(c1 - 1.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c2 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c3 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c4 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                    } else if (e2 == 1) {
+                        m->driftdist[kk] = 1000.e0 * c1;
+                        if (fabs(c2 - 1.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c3 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c4 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                    } else if (e3 == 1) {
+                        if (fabs(c1 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c2 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c3 - 1.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c4 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                    } else if (e4 == 1) {
+                        if (fabs(c1 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c2 - 0.e0) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(m->driftdist[kk] - 1000.e0 * c3) > coeff_min) m->adrift[kk] = 0;
+                        if (fabs(c4 - 1.e0) > coeff_min) m->adrift[kk] = 0;
+                    }
+                } else {
+                    if (fabs(csum) > coeff_min) m->adrift[kk] = 0;
+                }
+            }
+            if (!f_read_record(f, line)) { fclose(f); return OR_E_MAPFMT; }
+        }
+        n_classes = kk + 1;
+        /* :461-468, read until a data line; EOF ends the file */
+        for (;;) {
+            if (!f_read_record(f, line)) goto done;
+            if (strncmp(line, "!LENGTH:", 8) == 0 && kk + 1 < OR_MAX_CLASS)
+                m->length[kk + 1] = 100. * f_list_real(line + 9);
+            int blank = 1;
+            for (int i = 0; i < 132; i++)
+                if (line[i] != ' ') { blank = 0; break; }
+            if (line[0] == '!' || strncmp(line, " ---", 4) == 0 || blank) continue;
+            break;
+        }
+    }
+done:
+    fclose(f);
+    m->n_classes = n_classes;
+    return OR_OK;
+}
+
+/* Reconstruction file reader, hms/mc_hms_recon.f:70-102 (format :144). */
+int or_read_recon(const char *path, or_recon_map *m)
+{
+    memset(m, 0, sizeof *m);
+    FILE *f = fopen(path, "r");
+    if (!f) return OR_E_IO;
+    char line[133];
+    line[0] = '!';
+    while (line[0] == '!')
+        if (!f_read_record(f, line)) { fclose(f); return OR_E_MAPFMT; }
+    m->n_terms = 0;
+    m->max_order = 0;
+    while (strncmp(line, " ---", 4) != 0) {
+        int it = m->n_terms++;
+        if (m->n_terms > OR_MAX_TERMS) { fclose(f); return OR_E_TOOMANY; }
+        for (int j = 0; j < 4; j++)
+            if (f_real_field(line + 1 + 16 * j, 16, 9, &m->coeff[it][j])) { fclose(f); return OR_E_MAPFMT; }
+        int ord = 0;
+        for (int j = 0; j < 5; j++) {
+            int e;
+            if (f_int1(line[66 + j], &e)) { fclose(f); return OR_E_MAPFMT; }
+            m->expon[it][j] = (short)e;
+            ord += e;
+        }
+        if (!f_read_record(f, line)) { fclose(f); return OR_E_MAPFMT; }
+        if (ord > m->max_order) m->max_order = ord;
+    }
+    fclose(f);
+    return OR_OK;
+}
+
+or_maps *or_maps_load(const char *dir)
+{
+    or_maps *M = (or_maps *)calloc(1, sizeof *M);
+    M->hms_fwd = (or_fwd_map *)calloc(1, sizeof(or_fwd_map));
+    M->shms_fwd = (or_fwd_map *)calloc(1, sizeof(or_fwd_map));
+    M->hms_rec = (or_recon_map *)calloc(1, sizeof(or_recon_map));
+    M->shms_rec = (or_recon_map *)calloc(1, sizeof(or_recon_map));
+    char p[4096];
+    int rc = 0;
+    snprintf(p, sizeof p, "%s/hms/forward_cosy.dat", dir);
+    rc |= or_read_forward(p, M->hms_fwd);
+    snprintf(p, sizeof p, "%s/hms/recon_cosy.dat", dir);
+    rc |= or_read_recon(p, M->hms_rec);
+    snprintf(p, sizeof p, "%s/shms/shms_forward.dat", dir);
+    rc |= or_read_forward(p, M->shms_fwd);
+    snprintf(p, sizeof p, "%s/shms/shms_recon.dat", dir);
+    rc |= or_read_recon(p, M->shms_rec);
+    if (rc || M->hms_fwd->n_classes != 12 || M->shms_fwd->n_classes != 32) {
+        or_maps_free(M);
+        return NULL;
+    }
+    return M;
+}
+
+void or_maps_free(or_maps *M)
+{
+    if (!M) return;
+    free(M->hms_fwd);
+    free(M->shms_fwd);
+    free(M->hms_rec);
+    free(M->shms_rec);
+    free(M);
+}
+
+/* ========================================================================= */
+/* Random numbers                                                             */
+/* ========================================================================= */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:
+ * as easy as 1, 2, 3"; Random123 multipliers and Weyl key increments).  This
+ * is the counter-based generator of the GPU path; it has no counterpart in the
+ * reference, which uses RANLUX. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Uniform number k of event `event`: counter (k/4, event lo, event hi, 0x53494d43),
+ * word k%4, mapped to (w + 1/2) 2^-32 in (0,1). */
+#define OR_PHILOX_TAG 0x53494d43u
+double or_philox_uniform(uint32_t key0, uint32_t key1, uint64_t event, uint64_t k)
+{
+    uint32_t ctr[4] = {(uint32_t)(k >> 2), (uint32_t)event, (uint32_t)(event >> 32), OR_PHILOX_TAG};
+    uint32_t key[2] = {key0, key1};
+    uint32_t o[4];
+    or_philox4x32_10(ctr, key, o);
+    return ((double)o[k & 3] + 0.5) * 2.3283064365386963e-10;
+}
+
+/* RANLUX, cern/ranlux.f:3-311 (F. James 1993), REAL state is 8-byte under
+ * -fdefault-real-8. */
+static const int ndskip[5] = {0, 24, 73, 199, 365};
+
+static void ranlux_go(or_ranlux *r, int lux, int ins)
+{
+    /* RLUXGO (cern/ranlux.f:217-272), K1 = K2 = 0 */
+    if (lux < 0) r->luxlev = 3;
+    else if (lux <= 4) r->luxlev = lux;
+    else r->luxlev = 4;
+    r->nskip = ndskip[r->luxlev];
+    r->in24 = 0;
+    int jseed = ins > 0 ? ins : 314159265;
+    r->twom24 = 1.;
+    int iseeds[25];
+    for (int i = 1; i <= 24; i++) {
+        r->twom24 = r->twom24 * 0.5;
+        int k = jseed / 53668;
+        jseed = 40014 * (jseed - k * 53668) - k * 12211;
+        if (jseed < 0) jseed = jseed + 2147483563;
+        iseeds[i] = jseed % (1 << 24);
+    }
+    r->twom12 = r->twom24 * 4096.;
+    for (int i = 1; i <= 24; i++) {
+        r->seeds[i] = (double)iseeds[i] * r->twom24;
+        r->next[i] = i - 1;
+    }
+    r->next[1] = 24;
+    r->i24 = 24;
+    r->j24 = 10;
+    r->carry = 0.;
+    if (r->seeds[24] == 0.) r->carry = r->twom24;
+    r->kount = 0;
+}
+
+or_ranlux *or_ranlux_new(int lux, int seed)
+{
+    or_ranlux *r = (or_ranlux *)calloc(1, sizeof *r);
+    ranlux_go(r, lux, seed);
+    r->latest = 0;
+    return r;
+}
+
+/* RANLUX(RVEC, LENV), cern/ranlux.f:112-148 */
+void or_ranlux_fill(or_ranlux *r, double *rvec, int len)
+{
+    for (int iv = 0; iv < len; iv++) {
+        double uni = r->seeds[r->j24] - r->seeds[r->i24] - r->carry;
+        if (uni < 0.) {
+            uni = uni + 1.0;
+            r->carry = r->twom24;
+        } else {
+            r->carry = 0.;
+        }
+        r->seeds[r->i24] = uni;
+        r->i24 = r->next[r->i24];
+        r->j24 = r->next[r->j24];
+        rvec[iv] = uni;
+        if (uni < r->twom12) {
+            rvec[iv] = rvec[iv] + r->twom24 * r->seeds[r->j24];
+            if (rvec[iv] == 0.) rvec[iv] = r->twom24 * r->twom24;
+        }
+        r->in24 = r->in24 + 1;
+        if (r->in24 == 24) {
+            r->in24 = 0;
+            r->kount += r->nskip;
+            for (int isk = 1; isk <= r->nskip; isk++) {
+                uni = r->seeds[r->j24] - r->seeds[r->i24] - r->carry;
+                if (uni < 0.) {
+                    uni = uni + 1.0;
+                    r->carry = r->twom24;
+                } else {
+                    r->carry = 0.;
+                }
+                r->seeds[r->i24] = uni;
+                r->i24 = r->next[r->i24];
+                r->j24 = r->next[r->j24];
+            }
+        }
+    }
+    r->kount += len;
+}
+
+/* RLUXUT, cern/ranlux.f:196-203 */
+void or_ranlux_save(const or_ranlux *r, int32_t isdext[25])
+{
+    for (int i = 1; i <= 24; i++) isdext[i - 1] = (int32_t)(r->seeds[i] * 4096. * 4096.);
+    int v = r->i24 + 100 * r->j24 + 10000 * r->in24 + 1000000 * r->luxlev;
+    if (r->carry > 0.) v = -v;
+    isdext[24] = v;
+}
+
+/* RLUXIN, cern/ranlux.f:156-194 */
+void or_ranlux_restore(or_ranlux *r, const int32_t isdext[25])
+{
+    r->twom24 = 1.;
+    for (int i = 1; i <= 24; i++) {
+        r->next[i] = i - 1;
+        r->twom24 = r->twom24 * 0.5;
+    }
+    r->next[1] = 24;
+    r->twom12 = r->twom24 * 4096.;
+    for (int i = 1; i <= 24; i++) r->seeds[i] = (double)isdext[i - 1] * r->twom24;
+    r->carry = 0.;
+    if (isdext[24] < 0) r->carry = r->twom24;
+    int isd = abs(isdext[24]);
+    r->i24 = isd % 100;
+    isd /= 100;
+    r->j24 = isd % 100;
+    isd /= 100;
+    r->in24 = isd % 100;
+    isd /= 100;
+    r->luxlev = isd;
+    if (r->luxlev <= 4) r->nskip = ndskip[r->luxlev];
+    else if (r->luxlev >= 24) r->nskip = r->luxlev - 24;
+    else { r->nskip = ndskip[4]; r->luxlev = 4; }
+}
+
+/* grnd(), call_ranlux.f:58-73 (uses rvec(1..999), refills when latest=1000) */
+static double ranlux_grnd(or_ranlux *r)
+{
+    if (r->latest <= 0 || r->latest >= 1000) {
+        or_ranlux_fill(r, r->rvec + 1, 1000);
+        r->latest = 1;
+    }
+    double v = r->rvec[r->latest];
+    r->latest = r->latest + 1;
+    return v;
+}
+
+double or_grnd(or_rng *g)
+{
+    g->used++;
+    switch (g->kind) {
+    case 0: {
+        if (g->pos >= g->stride) { g->overflow = 1; return 0.5; }
+        return g->tape[g->pos++];
+    }
+    case 1: {
+        uint64_t k = (uint64_t)g->pos++;
+        if ((k & 3) == 0) {
+            uint32_t ctr[4] = {(uint32_t)(k >> 2), (uint32_t)g->event, (uint32_t)(g->event >> 32), OR_PHILOX_TAG};
+            or_philox4x32_10(ctr, g->key, g->buf);
+        }
+        return ((double)g->buf[k & 3] + 0.5) * 2.3283064365386963e-10;
+    }
+    default:
+        return ranlux_grnd(g->rl);
+    }
+}
+
+/* ========================================================================= */
+/* Elementary routines                                                        */
+/* ========================================================================= */
+
+/* Integer power by binary exponentiation, the algorithm of libgcc's
+ * __powidf2 / compiler-rt's __powidf2, which gfortran and flang call for
+ * REAL**INTEGER with a variable exponent (shared/transp.f:210,
+ * hms/mc_hms_recon.f:123-125). */
+double or_powi(double x, int m)
+{
+    unsigned int n = m < 0 ? -(unsigned int)m : (unsigned int)m;
+    double y = (n % 2) ? x : 1;
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2) y = y * x;
+    }
+    return m < 0 ? 1 / y : y;
+}
+
+/* gauss1, gauss1.f:1-30 (Marsaglia polar; second normal discarded) */
+double or_gauss1(or_rng *g, double nsigmax)
+{
+    for (;;) {
+        double u1 = or_grnd(g);
+        double u2 = or_grnd(g);
+        if (g->overflow) return 0.0;
+        double v1 = 2.0 * u1 - 1.0;
+        double v2 = 2.0 * u2 - 1.0;
+        double s = v1 * v1 + v2 * v2;
+        if (s > 1. || s == 0) continue;
+        double r = v1 * sqrt(-(2. * log(s) / s));
+        if (fabs(r) > nsigmax) continue;
+        return r;
+    }
+}
+
+/* LFIT, cern/lfit.f:11-60.  X, Y, A, B, E are REAL*4; the sums are the
+ * implicit REAL type, 8-byte under -fdefault-real-8. */
+void or_lfit_f32(const float *X, const float *Y, int L, int KEY, float *A, float *B, float *E)
+{
+    if (L - 2 < 0) goto l25;
+    {
+        double count = 0.0, sumx = 0.0, sumy = 0.0, sumxy = 0.0, sumxx = 0.0, sumyy = 0.0;
+        for (int j = 0; j < L; j++) {
+            if (Y[j] == 0. && KEY == 0) continue;
+            sumx = sumx + (double)X[j];
+            sumy = sumy + (double)Y[j];
+            count = count + 1.0;
+        }
+        if (count <= 1.) goto l25;
+        double ymed = sumy / count;
+        double xmed = sumx / count;
+        for (int j = 0; j < L; j++) {
+            if (Y[j] == 0. && KEY == 0) continue;
+            double scartx = (double)X[j] - xmed;
+            double scarty = (double)Y[j] - ymed;
+            sumxy = sumxy + scartx * scarty;
+            sumxx = sumxx + scartx * scartx;
+            sumyy = sumyy + scarty * scarty;
+        }
+        if (sumxx == 0.) goto l25;
+        *A = (float)(sumxy / sumxx);
+        *B = (float)(ymed - (double)*A * xmed);
+        if (count < 3.) goto l101;
+        *E = (float)((sumyy - sumxy * (double)*A) / (count - 2.0));
+        return;
+    }
+l25:
+    *A = 0.0f;
+    *B = 0.0f;
+l101:
+    *E = 0.0f;
+}
+
+/* The same fit carried out entirely in FP64 (the `fp64` track-fit mode). */
+void or_lfit_f64(const double *X, const double *Y, int L, int KEY, double *A, double *B, double *E)
+{
+    if (L - 2 < 0) goto l25;
+    {
+        double count = 0.0, sumx = 0.0, sumy = 0.0, sumxy = 0.0, sumxx = 0.0, sumyy = 0.0;
+        for (int j = 0; j < L; j++) {
+            if (Y[j] == 0. && KEY == 0) continue;
+            sumx = sumx + X[j];
+            sumy = sumy + Y[j];
+            count = count + 1.0;
+        }
+        if (count <= 1.) goto l25;
+        double ymed = sumy / count;
+        double xmed = sumx / count;
+        for (int j = 0; j < L; j++) {
+            if (Y[j] == 0. && KEY == 0) continue;
+            double scartx = X[j] - xmed;
+            double scarty = Y[j] - ymed;
+            sumxy = sumxy + scartx * scarty;
+            sumxx = sumxx + scartx * scartx;
+            sumyy = sumyy + scarty * scarty;
+        }
+        if (sumxx == 0.) goto l25;
+        *A = sumxy / sumxx;
+        *B = ymed - *A * xmed;
+        if (count < 3.) goto l101;
+        *E = (sumyy - sumxy * *A) / (count - 2.0);
+        return;
+    }
+l25:
+    *A = 0.0;
+    *B = 0.0;
+l101:
+    *E = 0.0;
+}
+
+/* loren, loren.f:1-26 */
+void or_loren(double gam, double bx, double by, double bz, double e, double x, double y,
+              double z, double *ef1, double *pxf, double *pyf, double *pzf, double *pf1)
+{
+    double gam1 = gam * gam / (1. + gam);
+    *ef1 = gam * (e - bx * x - by * y - bz * z);
+    *pxf = (1 + gam1 * (bx * bx)) * x + gam1 * bx * (by * y + bz * z) - gam * bx * e;
+    *pyf = (1 + gam1 * (by * by)) * y + gam1 * by * (bx * x + bz * z) - gam * by * e;
+    *pzf = (1 + gam1 * (bz * bz)) * z + gam1 * bz * (by * y + bx * x) - gam * bz * e;
+    *pf1 = sqrt((*pxf) * (*pxf) + (*pyf) * (*pyf) + (*pzf) * (*pzf));
+}
+
+/* enerloss_new, enerloss_new.f:1-85 */
+void or_enerloss_new(or_rng *g, double len, double dens, double zeff, double aeff,
+                     double epart, double mpart, int typeflag, double *eloss)
+{
+    const double me = 0.51099906;
+    double thick = len * dens;
+    double gamma = epart / mpart;
+    double beta = sqrt(1. - 1. / (gamma * gamma));
+    double I;
+    if (zeff == 1) I = 21.8e-06;
+    else I = (16. * pow(zeff, 0.9)) * 1.0e-06;
+    double hnup = 28.816e-06 * sqrt(dens * zeff / aeff);
+    double log10bg = log(beta * gamma) / log(10.);
+    double CO = log(hnup) - log(I) + 0.5;
+    double denscorr;
+    if (log10bg < 0.) denscorr = 0.;
+    else if (log10bg < 3.) {
+        double t3 = 3. - log10bg;
+        denscorr = CO + log(10.) * log10bg + fabs(CO / 27.) * (t3 * t3 * t3);
+    } else if (log10bg < 4.7) denscorr = CO + log(10.) * log10bg;
+    else denscorr = CO + log(10.) * 4.7;
+    double E;
+    if (thick <= 0.) {
+        E = 0.;
+    } else {
+        double b2 = beta * beta;
+        double Eloss_mp_new = 0.1536e-03 * zeff / aeff * thick / b2 *
+                              (log(me / (I * I)) + 1.063 + 2. * log(gamma * beta) +
+                               log(0.1536 * zeff / aeff * thick / b2) - b2 - denscorr);
+        double Eloss_mp = Eloss_mp_new * 1000.;
+        double chsi = 0.307075 / 2. * zeff / aeff * thick / b2;
+        double x = 0.;
+        if (typeflag == 1) x = fabs(or_gauss1(g, 10.0e0));
+        else if (typeflag == 2) x = 3;
+        else if (typeflag == 3) x = 0.0067;
+        else if (typeflag == 4) x = 1;
+        double lambda;
+        if (x > 0.0) lambda = -2.0 * log(x);
+        else lambda = 100000.;
+        E = lambda * chsi + Eloss_mp;
+    }
+    if (E > (epart - mpart)) E = (epart - mpart) - 0.0000001;
+    *eloss = E;
+}
+
+/* pion_coll_absorb, hms/pion_coll_absorb.f:1-239.  The d-exponent literal
+ * 1.d-27 is DOUBLE PRECISION, which -fdefault-real-8 promotes to a 16-byte
+ * real, so that product is formed in quad precision (:223,229). */
+double or_pion_coll_absorb(double ppi, double thick)
+{
+    static const double T[14] = {85.0, 125.0, 165.0, 205.0, 245.0, 315.0, 584.02, 711.95, 870.12,
+                                 1227.57, 1446.58, 1865.29, 2858.0, 4159.0};
+    static const double sigreac[14] = {26.03, 84.47, 117.3, 117.4, 101.9, 69.58, 42.5, 44.7, 47.9,
+                                       46.5, 45.2, 39.6, 35.34, 33.15};
+    static const double qreac[14] = {0.948, 0.659, 0.56, 0.5342, 0.5452, 0.60796, 0.699, 0.689,
+                                     0.679, 0.683, 0.688, 0.704, 0.7483, 0.7705};
+    const double Navagadro = 6.0221367e+23;
+    const double mpi = 139.56995;
+    double mate_dens = 17.0, mate_A = 171.57;
+    double Epi = sqrt(ppi * ppi + mpi * mpi);
+    double Tpi = Epi - mpi;
+    double sigA = 0.0;
+    for (int i = 0; i < 13; i++) {
+        if (Tpi > T[i] && Tpi <= T[i + 1]) {
+            double Thi = T[i + 1], Tlo = T[i];
+            double sigAhi = sigreac[i + 1] * pow(mate_A, qreac[i + 1]);
+            double sigAlo = sigreac[i] * pow(mate_A, qreac[i]);
+            sigA = (sigAlo * (Thi - Tpi) + sigAhi * (Tpi - Tlo)) / (Thi - Tlo);
+            sigA = (double)((__float128)sigA * 1.0e-27Q);
+        }
+    }
+    if (Tpi > T[13]) {
+        sigA = sigreac[13] * pow(mate_A, qreac[13]);
+        sigA = (double)((__float128)sigA * 1.0e-27Q);
+    }
+    double sum = 0.0;
+    double lambdai = mate_dens * Navagadro * sigA / mate_A;
+    sum = sum + thick * lambdai;
+    return exp(-sum);
+}
+
+/* rotate_haxis, shared/rotate_haxis.f:1-64 (truncated degree constant :41) */
+void or_rotate_haxis(const or_track *t, double rotang, double *xp0, double *yp0)
+{
+    const double raddeg = 0.017453292;
+    double rotang_rad = rotang * raddeg;
+    double tan_th = tan(rotang_rad), sin_th = sin(rotang_rad), cos_th = cos(rotang_rad);
+    double alpha = t->dxdzs, beta = t->dydzs;
+    double alpha_p = (alpha + tan_th) / (1. - alpha * tan_th);
+    double beta_p = beta / (cos_th - alpha * sin_th);
+    double xi = *xp0;
+    *xp0 = xi * (cos_th + alpha_p * sin_th);
+    *yp0 = *yp0 + xi * beta_p * sin_th;
+}
+
+/* rotate_vaxis, shared/rotate_vaxis.f:1-60 */
+void or_rotate_vaxis(const or_track *t, double rotang, double *xp0, double *yp0)
+{
+    const double raddeg = 0.017453292;
+    double rotang_rad = rotang * raddeg;
+    double tan_th = tan(rotang_rad), sin_th = sin(rotang_rad), cos_th = cos(rotang_rad);
+    double alpha = t->dydzs, beta = t->dxdzs;
+    double alpha_p = (alpha + tan_th) / (1. - alpha * tan_th);
+    double beta_p = beta / (cos_th - alpha * sin_th);
+    double yi = *yp0;
+    *yp0 = yi * (cos_th + alpha_p * sin_th);
+    *xp0 = *xp0 + yi * beta_p * sin_th;
+}
+
+/* musc, shared/musc.f:1-58 */
+static void musc(or_rng *g, double m2, double p, double rad_len, double *dth, double *dph)
+{
+    const double Es = 13.6, epsilon = 0.088, nsig_max = 99.0e0;
+    if (rad_len == 0) return;
+    double beta = p / sqrt(m2 + p * p);
+    double theta_sigma = Es / p / beta * sqrt(rad_len) * (1 + epsilon * log10(rad_len / (beta * beta)));
+    *dth = *dth + theta_sigma * or_gauss1(g, nsig_max);
+    *dph = *dph + theta_sigma * or_gauss1(g, nsig_max);
+}
+
+/* musc_ext, shared/musc_ext.f:1-54 (argument order dph, dth, y, x) */
+static void musc_ext(or_rng *g, double m2, double p, double rad_len, double x_len, double *dph,
+                     double *dth, double *y, double *x)
+{
+    const double Es = 13.6, epsilon = 0.088, nsig_max = 99.0e0;
+    if (rad_len == 0) return;
+    if (x_len <= 0 || rad_len < 0) return; /* the reference stops here (:146-151) */
+    double beta = p / sqrt(m2 + p * p);
+    double theta_sigma = Es / p / beta * sqrt(rad_len) * (1 + epsilon * log10(rad_len / (beta * beta)));
+    double g1 = or_gauss1(g, nsig_max);
+    double g2 = or_gauss1(g, nsig_max);
+    *dth = *dth + theta_sigma * g1;
+    *x = *x + theta_sigma * x_len * g2 / sqrt(12.) + theta_sigma * x_len * g1 / 2.;
+    g1 = or_gauss1(g, nsig_max);
+    g2 = or_gauss1(g, nsig_max);
+    *dph = *dph + theta_sigma * g1;
+    *y = *y + theta_sigma * x_len * g2 / sqrt(12.) + theta_sigma * x_len * g1 / 2.;
+}
+
+/* Shared decay kinematics of project.f:72-109 and transp.f:146-184.
+ * m_kk is the final-state mass assigned on the kaon two-body branch (Mk in the
+ * first half of transp, Mpi elsewhere, transp.f:158 vs :251, project.f:84). */
+static void decay_products(or_rng *g, or_track *t, double beta, double gamma, double p_spec,
+                           double *m2, double *ph, double m_kk)
+{
+    double rph = or_grnd(g) * 2. * PI_C;
+    double rth1 = or_grnd(g) * 2. - 1.;
+    double rth = acos(rth1);
+    double pr = 0.;
+    double m_final = Mmu_C;
+    if (fabs(sqrt(*m2) - Mpi_C) < 2) pr = 29.783;
+    if (fabs(sqrt(*m2) - Mk_C) < 2) {
+        if (or_grnd(g) < 0.7) {
+            pr = 235.5;
+        } else {
+            pr = sqrt(Mk_C * Mk_C / 4. - Mpi_C * Mpi_C);
+            m_final = m_kk;
+        }
+    }
+    /* pr == 0 (a mass that cannot decay) stops the reference program; the
+     * decay then proceeds at rest here. */
+    double er = sqrt(m_final * m_final + pr * pr);
+    double pxr = pr * sin(rth) * cos(rph);
+    double pyr = pr * sin(rth) * sin(rph);
+    double pzr = pr * cos(rth);
+    double s = sqrt(1. + t->dxdzs * t->dxdzs + t->dydzs * t->dydzs);
+    double bx = -(beta * t->dxdzs / s);
+    double by = -(beta * t->dydzs / s);
+    double bz = -(beta * 1. / s);
+    double ef, pf, pxf, pyf, pzf;
+    or_loren(gamma, bx, by, bz, er, pxr, pyr, pzr, &ef, &pxf, &pyf, &pzf, &pf);
+    t->dxdzs = pxf / pzf;
+    t->dydzs = pyf / pzf;
+    t->dpps = 100. * (pf / p_spec - 1.);
+    *ph = pf;
+    *m2 = m_final * m_final;
+    t->Mh2_final = *m2;
+}
+
+/* project, shared/project.f:1-122.  x_new/y_new alias the track or temporaries. */
+static void project(or_rng *g, or_track *t, const or_settings *S, double *x_new, double *y_new,
+                    double z_drift, int decay_flag, int *dflag, double *m2, double *ph,
+                    double *pathlen)
+{
+    if (!decay_flag || *dflag) {
+        *pathlen = *pathlen + z_drift * sqrt(1 + t->dxdzs * t->dxdzs + t->dydzs * t->dydzs);
+        *x_new = *x_new + t->dxdzs * z_drift;
+        *y_new = *y_new + t->dydzs * z_drift;
+        return;
+    }
+    double p_spec = *ph / (1. + t->dpps / 100.);
+    double beta = *ph / sqrt((*ph) * (*ph) + *m2);
+    double gamma = 1. / sqrt(1. - beta * beta);
+    double dlen = S->ctau * beta * gamma;
+    double z_decay = -1. * dlen * log(1 - or_grnd(g));
+    double sq = sqrt(1 + t->dxdzs * t->dxdzs + t->dydzs * t->dydzs);
+    if (z_decay > z_drift * sq) {
+        t->decdist = t->decdist + z_drift * sq;
+        *pathlen = *pathlen + z_drift * sq;
+        *x_new = *x_new + t->dxdzs * z_drift;
+        *y_new = *y_new + t->dydzs * z_drift;
+    } else {
+        *dflag = 1;
+        t->decdist = t->decdist + z_decay;
+        *pathlen = *pathlen + z_decay;
+        *x_new = *x_new + t->dxdzs * z_decay / sq;
+        *y_new = *y_new + t->dydzs * z_decay / sq;
+        decay_products(g, t, beta, gamma, p_spec, m2, ph, Mpi_C);
+        double sq2 = sqrt(1 + t->dxdzs * t->dxdzs + t->dydzs * t->dydzs);
+        double tmpdrift = z_drift - z_decay / sq2;
+        *pathlen = *pathlen + tmpdrift * sq2;
+        *x_new = *x_new + t->dxdzs * tmpdrift;
+        *y_new = *y_new + t->dydzs * tmpdrift;
+    }
+}
+
+/* COSY sum of one forward class, shared/transp.f:199-218 */
+void or_transp_eval(const or_fwd_map *m, int cls, const double ray[5], double sum[5])
+{
+    int k = cls - 1;
+    for (int i = 0; i < 5; i++) sum[i] = 0.;
+    for (int i = 0; i < m->n_terms[k]; i++) {
+        double term = 1.0e0;
+        for (int j = 0; j < 5; j++) {
+            double temp = 1.0e0;
+            if (m->expon[k][i][j] != 0.) temp = or_powi(ray[j], m->expon[k][i][j]);
+            term = term * temp;
+        }
+        sum[0] = sum[0] + term * m->coeff[k][i][0];
+        sum[1] = sum[1] + term * m->coeff[k][i][1];
+        sum[2] = sum[2] + term * m->coeff[k][i][2];
+        sum[3] = sum[3] + term * m->coeff[k][i][3];
+        sum[4] = sum[4] + term * m->coeff[k][i][4];
+    }
+}
+
+/* transp, shared/transp.f:1-281 */
+static void transp(const or_fwd_map *m, or_rng *g, or_track *t, const or_settings *S, int cls,
+                   int decay_flag, int *dflag, double *m2, double *ph, double zd, double *pathlen)
+{
+    double p_spec = 0, beta = 0, gamma = 0, z_decay = 0;
+    if (decay_flag && !*dflag) { /* :134-187 */
+        p_spec = *ph / (1. + t->dpps / 100.);
+        beta = *ph / sqrt((*ph) * (*ph) + *m2);
+        gamma = 1. / sqrt(1. - beta * beta);
+        double dlen = S->ctau * beta * gamma;
+        z_decay = -1. * dlen * log(1 - or_grnd(g));
+        if (z_decay <= zd / 2) {
+            *dflag = 1;
+            t->decdist = t->decdist + z_decay;
+            decay_products(g, t, beta, gamma, p_spec, m2, ph, Mk_C);
+        }
+    }
+    double ray[5], sum[5];
+    ray[0] = t->xs;
+    ray[1] = t->dxdzs * 1000.e0;
+    ray[2] = t->ys;
+    ray[3] = t->dydzs * 1000.e0;
+    ray[4] = t->dpps;
+    or_transp_eval(m, cls, ray, sum);
+    t->xs = sum[0];
+    t->dxdzs = sum[1] / 1000.e0;
+    t->ys = sum[2];
+    t->dydzs = sum[3] / 1000.e0;
+    double delta_z = -sum[4];
+    if (decay_flag && !*dflag) { /* :231-276 */
+        if (z_decay > zd + delta_z) {
+            t->decdist = t->decdist + (zd + delta_z);
+        } else {
+            *dflag = 1;
+            t->decdist = t->decdist + z_decay;
+            decay_products(g, t, beta, gamma, p_spec, m2, ph, Mpi_C);
+        }
+    }
+    *pathlen = *pathlen + (zd + delta_z);
+}
+
+/* COSY reconstruction sum, hms/mc_hms_recon.f:104-137 (all five powers taken) */
+void or_recon_eval(const or_recon_map *m, const double hut[5], double sum[4])
+{
+    for (int i = 0; i < 4; i++) sum[i] = 0.;
+    for (int i = 0; i < m->n_terms; i++) {
+        double term = or_powi(hut[0], m->expon[i][0]) * or_powi(hut[1], m->expon[i][1]) *
+                      or_powi(hut[2], m->expon[i][2]) * or_powi(hut[3], m->expon[i][3]) *
+                      or_powi(hut[4], m->expon[i][4]);
+        sum[0] = sum[0] + term * m->coeff[i][0];
+        sum[1] = sum[1] + term * m->coeff[i][1];
+        sum[2] = sum[2] + term * m->coeff[i][2];
+        sum[3] = sum[3] + term * m->coeff[i][3];
+    }
+}
+
+/* mc_hms_recon / mc_shms_recon (hms/mc_hms_recon.f:110-137,
+ * shms/mc_shms_recon.f:124-160): identical arithmetic. */
+static void recon(const or_recon_map *m, const or_track *t, double fry, double *delta_p,
+                  double *delta_t, double *delta_phi, double *y_tgt)
+{
+    double hut[5], sum[4];
+    hut[0] = t->xs / 100.;
+    hut[1] = t->dxdzs;
+    hut[2] = t->ys / 100.;
+    hut[3] = t->dydzs;
+    hut[4] = fry / 100.;
+    if (fabs(hut[4]) <= 1.e-30) hut[4] = 1.e-30;
+    or_recon_eval(m, hut, sum);
+    *delta_phi = sum[0];
+    *y_tgt = sum[1] * 100.;
+    *delta_t = sum[2];
+    *delta_p = sum[3] * 100.;
+}
+
+/* Straight-line fit of the 12 chamber planes, hms/mc_hms_hut.f:439-455 and
+ * shms/mc_shms_hut.f:410-426.  In fit_f32 mode the hits, plane positions and
+ * fitted parameters are REAL*4 as in the reference. */
+typedef struct {
+    double xdc[12], ydc[12], zdc[12];
+} hut_hits;
+
+static void fit_planes(const or_settings *S, const hut_hits *h, double *x_fp, double *dx_fp,
+                       double *y_fp, double *dy_fp)
+{
+    if (S->fit_f32) {
+        float xdc[12], ydc[12], zdc[12], a, b, e;
+        for (int i = 0; i < 12; i++) {
+            xdc[i] = (float)h->xdc[i];
+            ydc[i] = (float)h->ydc[i];
+            zdc[i] = (float)h->zdc[i];
+        }
+        or_lfit_f32(zdc, xdc, 12, 0, &a, &b, &e);
+        *dx_fp = (double)a;
+        *x_fp = (double)b;
+        or_lfit_f32(zdc, ydc, 12, 0, &a, &b, &e);
+        *dy_fp = (double)a;
+        *y_fp = (double)b;
+    } else {
+        double a, b, e;
+        or_lfit_f64(h->zdc, h->xdc, 12, 0, &a, &b, &e);
+        *dx_fp = a;
+        *x_fp = b;
+        or_lfit_f64(h->zdc, h->ydc, 12, 0, &a, &b, &e);
+        *dy_fp = a;
+        *y_fp = b;
+    }
+}
+
+/* ========================================================================= */
+/* HMS                                                                        */
+/* ========================================================================= */
+
+/* mc_hms_coll, hms/mc_hms_coll.f:1-147 (h/v/x/y offsets, 20 steps of 6.30/20) */
+static int coll_step(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, int arm,
+                     double *m2, double *p, double p_spec, int decay_flag, int *dflag,
+                     double *pathlen)
+{
+    double h_entr, v_entr, h_exit, v_exit, x_off, y_off, coll_radl, step_size;
+    const double coll_dens = 17.0, zcoll = 69.45, acoll = 171.56797;
+    const int nstep = 20;
+    int ih, iv, io;
+    if (arm == 1) {
+        h_entr = 4.575; v_entr = 11.646; h_exit = 4.759; v_exit = 12.114;
+        x_off = 0.000; y_off = +0.028; coll_radl = 0.41753; step_size = 6.30 / nstep;
+        ih = H_SLIT_HOR; iv = H_SLIT_VERT; io = H_SLIT_OCT;
+    } else {
+        h_entr = 8.50; v_entr = 12.50; h_exit = 8.65; v_exit = 12.85;
+        x_off = 0.000; y_off = +0.000; coll_radl = 0.42084; step_size = 6.35 / nstep;
+        ih = S_SLIT_HOR; iv = S_SLIT_VERT; io = S_SLIT_OCT;
+    }
+    double h_step = h_entr, v_step = v_entr;
+    double epart = sqrt((*p) * (*p) + *m2);
+    for (int n = 1; n <= nstep; n++) {
+        int step_flag = 0;
+        if (fabs(t->ys - y_off) > h_step) { step_flag = 1; cnt[ih]++; }
+        if (fabs(t->xs - x_off) > v_step) { step_flag = 1; cnt[iv]++; }
+        if (fabs(t->xs - x_off) > (-(v_step / h_step * fabs(t->ys - y_off)) + 3 * v_step / 2)) {
+            step_flag = 1;
+            cnt[io]++;
+        }
+        double thick = step_size;
+        if (step_flag) {
+            if (*m2 > 12000. && *m2 < 20000.) {
+                double trans = or_pion_coll_absorb(*p, thick);
+                double rantemp = or_grnd(g);
+                if (rantemp > trans) return 0;
+            }
+            double coll_radw = thick / coll_radl;
+            musc(g, *m2, *p, coll_radw, &t->dydzs, &t->dxdzs);
+            epart = sqrt((*p) * (*p) + *m2);
+            double eloss;
+            or_enerloss_new(g, thick, coll_dens, zcoll, acoll, epart, sqrt(*m2), 1, &eloss);
+            epart = epart - eloss;
+            if (epart < sqrt(*m2)) return 0;
+            *p = sqrt(epart * epart - *m2);
+            t->dpps = 100. * (*p / p_spec - 1.);
+        }
+        project(g, t, S, &t->xs, &t->ys, step_size, decay_flag, dflag, m2, p, pathlen);
+        h_step = h_step + (h_exit - h_entr) / nstep;
+        v_step = v_step + (v_exit - v_entr) / nstep;
+    }
+    return 1;
+}
+
+/* hit_dipole, hms/mc_hms.f:445-492 with hms/apertures_hms.inc:20-32 */
+static int hms_hit_dipole(double x, double y)
+{
+    const double x_d1 = 34.29, y_d1 = 12.07, x_d2 = 27.94, y_d2 = 18.42, x_d3 = 13.97,
+                 y_d3 = 18.95, x_d4 = 1.956, y_d4 = 20.32, x_d5 = 27.94, y_d5 = 12.065,
+                 r_d5 = 6.35, a_d6 = -0.114, b_d6 = 20.54;
+    double xl = fabs(x), yl = fabs(y);
+    int c1 = (xl <= x_d1) && (yl <= y_d1);
+    int c2 = (xl <= x_d2) && (yl <= y_d2);
+    int c3 = (xl <= x_d3) && (yl <= y_d3);
+    int c4 = (xl <= x_d4) && (yl <= y_d4);
+    int c5 = ((xl - x_d5) * (xl - x_d5) + (yl - y_d5) * (yl - y_d5)) <= r_d5 * r_d5;
+    int c6 = (xl >= x_d4) && (xl <= x_d3) && ((yl - a_d6 * xl - b_d6) <= 0.0);
+    return !(c1 || c2 || c3 || c4 || c5 || c6);
+}
+
+/* mc_hms_hut, hms/mc_hms_hut.f:1-608.  Returns ok_hut; *fit_done is set once
+ * x_fp..dy_fp have been written (:452-455). */
+static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+                   double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                   int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
+                   double zinit, double *pathlen, int *stop_code, int *fit_done)
+{
+    const double hfoil_exit_radlen = 8.90, hfoil_exit_thick = 0.011 * 2.54;
+    const double hair_radlen = 30420.;
+    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 1.8;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
+    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
+    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
+    const double haer_entr_radlen = 8.90, haer_entr_thick = 0.15;
+    const double haer_radlen = 150.0, haer_thick = 9.0;
+    const double haer_air_radlen = 30420.0, haer_air_thick = 16.0;
+    const double haer_exit_radlen = 8.90, haer_exit_thick = 0.1;
+    const double hscin_radlen = 42.4;
+    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54;
+    const double hcer_radlen = 9620.0;
+    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
+    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_plan = 6;
+    const double hdc_1_zpos = -52.1084, hdc_2_zpos = 29.2608;
+    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
+    const double hdc_1_left = 26.0, hdc_1_right = -26.0, hdc_1y_offset = 1.443, hdc_1_top = -56.5,
+                 hdc_1_bot = 56.5, hdc_1x_offset = 1.670;
+    const double hdc_2_left = 26.0, hdc_2_right = -26.0, hdc_2y_offset = 2.753, hdc_2_top = -56.5,
+                 hdc_2_bot = 56.5, hdc_2x_offset = 2.758;
+    const double haer_zentrance = 35.699, haer_zexit = 60.949;
+    const double hscin_1x_zpos = 77.830, hscin_1y_zpos = 97.520, hscin_2x_zpos = 298.820,
+                 hscin_2y_zpos = 318.510;
+    const double hscin_1x_thick = 1.067, hscin_1y_thick = 1.067, hscin_2x_thick = 1.067,
+                 hscin_2y_thick = 1.067;
+    const double hscin_1x_left = 37.75, hscin_1x_right = -37.75, hscin_1x_offset = -1.3;
+    const double hscin_1y_top = -60.25, hscin_1y_bot = 60.25, hscin_1y_offset = -1.3;
+    const double hcer_zentrance = 110.000, hcer_zmirror = 230.000, hcer_zexit = 265.000;
+    const double hcal_4ta_zpos = 371.69;
+    const double nsig_max = 99.0e0;
+    const int scintrig = 3;
+    double sigma = S->wire_sigma > 0 ? S->wire_sigma : 0.030; /* hdc_sigma, :143-144 */
+
+    double radw, drift, tmpran1, tmpran2;
+    hut_hits h;
+    int scincount;
+
+    double tmpran = or_grnd(g); /* :292-298 */
+    double tmplim = 0.15;
+    if (tmpran < tmplim) *resmult = 2.0;
+    else *resmult = 1.0;
+    for (int i = 0; i < 12; i++) { h.xdc[i] = 0.; h.ydc[i] = 0.; }
+    scincount = 0;
+
+    drift = (hdc_1_zpos - 25.000) - zinit;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    radw = hfoil_exit_thick / hfoil_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos - 25.000);
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+
+    for (int jchamber = 1; jchamber <= 2; jchamber++) {
+        if (jchamber == 2) {
+            drift = (hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) -
+                    (hdc_1_zpos + 0.5 * hdc_nr_plan * hdc_del_plane);
+            radw = drift / hair_radlen;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        }
+        radw = hdc_entr_thick / hdc_entr_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
+            radw = hdc_cath_thick / hdc_cath_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_cath_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            radw = hdc_wire_thick / hdc_wire_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (wcs_flag) {
+                tmpran1 = or_gauss1(g, nsig_max);
+                tmpran2 = or_gauss1(g, nsig_max);
+            } else {
+                tmpran1 = 0.;
+                tmpran2 = 0.;
+            }
+            h.xdc[npl_off + iplane - 1] = t->xs + sigma * tmpran1 * *resmult;
+            h.ydc[npl_off + iplane - 1] = t->ys + sigma * tmpran2 * *resmult;
+            if (iplane == 2 || iplane == 5) h.xdc[npl_off + iplane - 1] = 0.;
+            else h.ydc[npl_off + iplane - 1] = 0.;
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_wire_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        }
+        radw = hdc_exit_thick / hdc_exit_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (jchamber == 1) {
+            if (t->xs > (hdc_1_bot - hdc_1x_offset) || t->xs < (hdc_1_top - hdc_1x_offset) ||
+                t->ys > (hdc_1_left - hdc_1y_offset) || t->ys < (hdc_1_right - hdc_1y_offset)) {
+                cnt[H_DC1]++;
+                *stop_code = STOP_H_DC1;
+                return 0;
+            }
+        } else {
+            if (t->xs > (hdc_2_bot - hdc_2x_offset) || t->xs < (hdc_2_top - hdc_2x_offset) ||
+                t->ys > (hdc_2_left - hdc_2y_offset) || t->ys < (hdc_2_right - hdc_2y_offset)) {
+                cnt[H_DC2]++;
+                *stop_code = STOP_H_DC2;
+                return 0;
+            }
+        }
+        radw = hdc_cath_thick / hdc_cath_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    }
+
+    for (int jchamber = 1; jchamber <= 2; jchamber++) {
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
+            double z0 = jchamber == 1 ? hdc_1_zpos : hdc_2_zpos;
+            h.zdc[npl_off + iplane - 1] = z0 + ((double)iplane - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        }
+    }
+    fit_planes(S, &h, x_fp, dx_fp, y_fp, dy_fp);
+    *fit_done = 1;
+
+    drift = haer_zentrance - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = haer_entr_thick / haer_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = haer_thick;
+    radw = drift / haer_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    drift = haer_air_thick;
+    radw = drift / haer_air_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = haer_exit_thick / haer_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+#define HMS_SCIN_IN()                                                                          \
+    (t->ys < (hscin_1x_left + hscin_1y_offset) && t->ys > (hscin_1x_right + hscin_1y_offset) && \
+     t->xs < (hscin_1y_bot + hscin_1x_offset) && t->xs > (hscin_1y_top + hscin_1x_offset))
+
+    drift = hscin_1x_zpos - haer_zexit;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (HMS_SCIN_IN()) scincount = scincount + 1;
+    radw = hscin_1x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hscin_1y_zpos - hscin_1x_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (HMS_SCIN_IN()) scincount = scincount + 1;
+    radw = hscin_1y_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hcer_zentrance - hscin_1y_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_entr_thick / hcer_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_zmirror - hcer_zentrance;
+    radw = drift / hcer_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_mir_thick / hcer_mir_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_zexit - hcer_zmirror;
+    radw = hcer_exit_thick / hcer_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    radw = hcer_exit_thick / hcer_exit_radlen; /* exit window scattering a second time, :528-529 */
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hscin_2x_zpos - hcer_zexit;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (HMS_SCIN_IN()) scincount = scincount + 1; /* first-plane limits reused, :537-540 */
+    radw = hscin_2x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hscin_2y_zpos - hscin_2x_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (HMS_SCIN_IN()) scincount = scincount + 1;
+    radw = hscin_2y_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+#undef HMS_SCIN_IN
+
+    if (scincount < scintrig) {
+        cnt[H_SCIN]++;
+        *stop_code = STOP_H_SCIN;
+        return 0;
+    }
+    drift = hcal_4ta_zpos - hscin_2y_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    return 1;
+}
+
+/* mc_hms, hms/mc_hms.f:1-441 */
+void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+               double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+               double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+               double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+               int *stop_code, int *fit_done)
+{
+    const or_fwd_map *F = M->hms_fwd;
+    const double h_entr = 4.575, v_entr = 11.646, h_exit = 4.759, v_exit = 12.114;
+    const double x_off = +0.000, y_off = +0.028, z_off = +40.17;
+    const double z_entr = 126.2e0 + z_off, z_exit = z_entr + 6.3e0;
+    const double z_dip1 = 64.77e0, z_dip2 = z_dip1 + 297.18e0, z_dip3 = z_dip2 + 115.57e0;
+    const double r_Q1 = 20.50, r_Q2 = 30.22, r_Q3 = 30.22;
+    const double x_offset_pipes = 2.8, y_offset_pipes = 0.0;
+    const int decay_flag = S->decay_flag, ms_flag = S->ms_flag, wcs_flag = S->wcs_flag;
+
+    double p, xt, yt, zdrift;
+    double dpp_recon, dth_recon, dph_recon, y_recon;
+    int dflag = 0, ok;
+    *ok_spec = 0;
+    *stop_code = STOP_OK;
+    *fit_done = 0;
+    cnt[H_TRIALS]++;
+
+    t->xs = *x;
+    t->ys = *y;
+    t->zs = *z;
+    t->dxdzs = *dxdz;
+    t->dydzs = *dydz;
+    t->dpps = *dpp;
+    p = p_spec * (1. + t->dpps / 100.);
+
+    zdrift = z_entr;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (S->using_coll && (*m2 > 100.0 * 100.0) && (*m2 < 200.0 * 200.0)) {
+        if (!coll_step(g, t, S, cnt, 1, m2, &p, p_spec, decay_flag, &dflag, pathlen)) {
+            cnt[H_COLL]++;
+            *stop_code = STOP_COLL;
+            return;
+        }
+    } else {
+        if (fabs(t->ys - y_off) > h_entr) { cnt[H_SLIT_HOR]++; *stop_code = STOP_SLIT_HOR; return; }
+        if (fabs(t->xs - x_off) > v_entr) { cnt[H_SLIT_VERT]++; *stop_code = STOP_SLIT_VERT; return; }
+        if (fabs(t->xs - x_off) > (-(v_entr / h_entr * fabs(t->ys - y_off)) + 3 * v_entr / 2)) {
+            cnt[H_SLIT_OCT]++; *stop_code = STOP_SLIT_OCT; return;
+        }
+        zdrift = z_exit - z_entr;
+        project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+        if (fabs(t->ys - y_off) > h_exit) { cnt[H_SLIT_HOR]++; *stop_code = STOP_SLIT_HOR; return; }
+        if (fabs(t->xs - x_off) > v_exit) { cnt[H_SLIT_VERT]++; *stop_code = STOP_SLIT_VERT; return; }
+        if (fabs(t->xs - x_off) > (-(v_exit / h_exit * fabs(t->ys - y_off)) + 3 * v_exit / 2)) {
+            cnt[H_SLIT_OCT]++; *stop_code = STOP_SLIT_OCT; return;
+        }
+    }
+
+#define RQ(r) ((t->xs * t->xs + t->ys * t->ys) > (r) * (r))
+    zdrift = F->driftdist[0] - z_exit;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q1)) { cnt[H_Q1_IN]++; *stop_code = STOP_H_Q1_IN; return; }
+    transp(F, g, t, S, 2, decay_flag, &dflag, m2, &p, 125.233e0, pathlen);
+    if (RQ(r_Q1)) { cnt[H_Q1_MID]++; *stop_code = STOP_H_Q1_MID; return; }
+    transp(F, g, t, S, 3, decay_flag, &dflag, m2, &p, 62.617e0, pathlen);
+    if (RQ(r_Q1)) { cnt[H_Q1_OUT]++; *stop_code = STOP_H_Q1_OUT; return; }
+
+    zdrift = F->driftdist[3];
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q2)) { cnt[H_Q2_IN]++; *stop_code = STOP_H_Q2_IN; return; }
+    transp(F, g, t, S, 5, decay_flag, &dflag, m2, &p, 143.90e0, pathlen);
+    if (RQ(r_Q2)) { cnt[H_Q2_MID]++; *stop_code = STOP_H_Q2_MID; return; }
+    transp(F, g, t, S, 6, decay_flag, &dflag, m2, &p, 71.95e0, pathlen);
+    if (RQ(r_Q2)) { cnt[H_Q2_OUT]++; *stop_code = STOP_H_Q2_OUT; return; }
+
+    zdrift = F->driftdist[6];
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q3)) { cnt[H_Q3_IN]++; *stop_code = STOP_H_Q3_IN; return; }
+    transp(F, g, t, S, 8, decay_flag, &dflag, m2, &p, 143.8e0, pathlen);
+    if (RQ(r_Q3)) { cnt[H_Q3_MID]++; *stop_code = STOP_H_Q3_MID; return; }
+    transp(F, g, t, S, 9, decay_flag, &dflag, m2, &p, 71.9e0, pathlen);
+    if (RQ(r_Q3)) { cnt[H_Q3_OUT]++; *stop_code = STOP_H_Q3_OUT; return; }
+#undef RQ
+
+    zdrift = F->driftdist[9];
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, -6.0e0, &xt, &yt);
+    if (hms_hit_dipole(xt, yt)) { cnt[H_D1_IN]++; *stop_code = STOP_H_D1_IN; return; }
+
+    transp(F, g, t, S, 11, decay_flag, &dflag, m2, &p, 526.053e0, pathlen);
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 6.0e0, &xt, &yt);
+    if (hms_hit_dipole(xt, yt)) { cnt[H_D1_OUT]++; *stop_code = STOP_H_D1_OUT; return; }
+
+    if ((((xt - x_offset_pipes) * (xt - x_offset_pipes) + (yt - y_offset_pipes) * (yt - y_offset_pipes)) >
+         30.48 * 30.48) ||
+        (fabs(yt - y_offset_pipes) > 20.5232)) {
+        cnt[H_D1_OUT]++; *stop_code = STOP_H_PIPE0; return;
+    }
+#define PIPE(lim) ((((t->xs - x_offset_pipes) * (t->xs - x_offset_pipes) + \
+                     (t->ys - y_offset_pipes) * (t->ys - y_offset_pipes))) > (lim))
+    zdrift = z_dip1;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (PIPE(1145.518)) { cnt[H_D1_OUT]++; *stop_code = STOP_H_PIPE1; return; }
+    zdrift = z_dip2 - z_dip1;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (PIPE(1512.2299)) { cnt[H_D1_OUT]++; *stop_code = STOP_H_PIPE2; return; }
+    zdrift = z_dip3 - z_dip2;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (PIPE(2162.9383)) { cnt[H_D1_OUT]++; *stop_code = STOP_H_PIPE3; return; }
+#undef PIPE
+
+    cnt[H_HUT]++;
+    zdrift = F->driftdist[11] - z_dip3;
+    ok = hms_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+                 &dflag, resmult, -zdrift, pathlen, stop_code, fit_done);
+    if (!ok) return;
+
+    t->xs = *x_fp;
+    t->ys = *y_fp;
+    t->dxdzs = *dx_fp;
+    t->dydzs = *dy_fp;
+    recon(M->hms_rec, t, fry, &dpp_recon, &dth_recon, &dph_recon, &y_recon);
+    *dpp = dpp_recon;
+    *dxdz = dph_recon;
+    *dydz = dth_recon;
+    *y = y_recon;
+    *ok_spec = 1;
+    cnt[H_SUCCESSES]++;
+}
+
+/* ========================================================================= */
+/* SHMS                                                                       */
+/* ========================================================================= */
+
+/* mc_shms_hut, shms/mc_shms_hut.f:1-458 with shms/hut.inc (cer_flag forced
+ * true, vac_flag false by mc_shms, shms/mc_shms.f:257-258; use_det_cut true) */
+static int shms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+                    double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                    int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
+                    double *pathlen, int *stop_code, int *fit_done)
+{
+    const double hfoil_exit_radlen = 8.89, hfoil_exit_thick = 0.020 * 2.54;
+    const double hair_radlen = 30420.;
+    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 0.125 * 2.54;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000354;
+    const double hdc_cath_radlen = 28.6, hdc_cath_thick = 0.001 * 2.54;
+    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
+    const double hscin_radlen = 42.4;
+    const double hcer_entr_radlen = 19.63, hcer_entr_thick = 0.002 * 2.54;
+    const double hcer_1_radlen = 11700.0;
+    const double hcer_mirglass_radlen = 12.29, hcer_mirglass_thick = 0.3;
+    const double hcer_exit_radlen = 19.63, hcer_exit_thick = 0.002 * 2.54;
+    const double hcer_2_entr_radlen = 8.90, hcer_2_entr_thick = 0.040 * 2.54;
+    const double hcer_2_radlen = 1202.5;
+    const double hcer_mir_radlen = 400., hcer_mir_thick = 2.00;
+    const double hcer_2_exit_radlen = 8.90, hcer_2_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_plan = 6;
+    const double hdc_1_zpos = -40.656, hdc_2_zpos = 39.332;
+    const double hdc_1_left = 40.0, hdc_1_right = -40.0, hdc_1y_offset = 0.0, hdc_1_top = -40.,
+                 hdc_1_bot = 40., hdc_1x_offset = 0.0;
+    const double hdc_2_left = 40.0, hdc_2_right = -40.0, hdc_2y_offset = 0.0, hdc_2_top = -40.,
+                 hdc_2_bot = 40., hdc_2x_offset = 0.;
+    const double hscin_1x_zpos = 52.1, hscin_1y_zpos = 61.7, hscin_2x_zpos = 271.4,
+                 hscin_2y_zpos = 282.4;
+    const double hscin_1x_thick = 1.000 * 1.067, hscin_1y_thick = 1.000 * 1.067,
+                 hscin_2x_thick = 1.000 * 1.067, hscin_2y_thick = 1.000 * 1.067;
+    const double hscin_1x_left = 50., hscin_1x_right = -50., hscin_1x_offset = 0.0;
+    const double hscin_1y_top = -45., hscin_1y_bot = 45., hscin_1y_offset = 0.0;
+    const double hscin_2x_left = 55., hscin_2x_right = -55., hscin_2x_offset = 0.;
+    const double hscin_2y_top = -62.5, hscin_2y_bot = 62.5, hscin_2y_offset = 0;
+    const double hcer_1_zentrance = -291.700, hcer_1_zmirror = -84.900, hcer_1_zexit = -61.700;
+    const double hcer_2_zentrance = 72.600, hcer_2_zmirror = 179.400, hcer_2_zexit = 202.600;
+    const double hcal_4ta_zpos = 341.0;
+    const double hcal_left = 63.00, hcal_right = -63.00, hcal_top = -70.00, hcal_bottom = 70.00;
+    const double nsig_max = 99.0e0;
+    double sigma = S->wire_sigma > 0 ? S->wire_sigma : 0.020; /* hdc_sigma, hut.inc:125-126 */
+
+    double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
+    double radw, drift, tmpran1, tmpran2;
+    hut_hits h;
+    *resmult = 1.0;
+    for (int i = 0; i < 12; i++) { h.xdc[i] = 0.; h.ydc[i] = 0.; }
+
+    radw = hfoil_exit_thick / hfoil_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    radw = hcer_entr_thick / hcer_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_1_zmirror - hcer_1_zentrance - hcer_mirglass_thick / 2;
+    radw = drift / hcer_1_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_mirglass_thick / hcer_mirglass_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_1_zexit - hcer_1_zmirror - hcer_mirglass_thick / 2;
+    radw = drift / hcer_1_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_exit_thick / hcer_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - hcer_1_zexit;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+
+    for (int jchamber = 1; jchamber <= 2; jchamber++) {
+        if (jchamber == 2) {
+            drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            radw = drift / hair_radlen;
+            if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        }
+        radw = hdc_entr_thick / hdc_entr_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
+            radw = hdc_cath_thick / hdc_cath_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_cath_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            radw = hdc_wire_thick / hdc_wire_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (wcs_flag) {
+                tmpran1 = or_gauss1(g, nsig_max);
+                tmpran2 = or_gauss1(g, nsig_max);
+            } else {
+                tmpran1 = 0.;
+                tmpran2 = 0.;
+            }
+            h.xdc[npl_off + iplane - 1] = t->xs + sigma * tmpran1 * *resmult;
+            h.ydc[npl_off + iplane - 1] = t->ys + sigma * tmpran2 * *resmult;
+            if (iplane == 2 || iplane == 5) h.xdc[npl_off + iplane - 1] = 0.;
+            else h.ydc[npl_off + iplane - 1] = 0.;
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_wire_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        }
+        radw = hdc_exit_thick / hdc_exit_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (jchamber == 1) {
+            if (t->xs > (hdc_1_bot - hdc_1x_offset) || t->xs < (hdc_1_top - hdc_1x_offset) ||
+                t->ys > (hdc_1_left - hdc_1y_offset) || t->ys < (hdc_1_right - hdc_1y_offset)) {
+                cnt[S_DC1]++;
+                *stop_code = STOP_S_DC1;
+                return 0;
+            }
+        } else {
+            if (t->xs > (hdc_2_bot - hdc_2x_offset) || t->xs < (hdc_2_top - hdc_2x_offset) ||
+                t->ys > (hdc_2_left - hdc_2y_offset) || t->ys < (hdc_2_right - hdc_2y_offset)) {
+                cnt[S_DC2]++;
+                *stop_code = STOP_S_DC2;
+                return 0;
+            }
+        }
+        radw = hdc_cath_thick / hdc_cath_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    }
+
+    drift = hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (t->ys > (hscin_1x_left + hscin_1y_offset) || t->ys < (hscin_1x_right + hscin_1y_offset)) {
+        cnt[S_S1]++;
+        *stop_code = STOP_S_S1X;
+        return 0;
+    }
+    radw = hscin_1x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hscin_1y_zpos - hscin_1x_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (t->xs > (hscin_1y_bot + hscin_1x_offset) || t->xs < (hscin_1y_top + hscin_1x_offset)) {
+        cnt[S_S1]++;
+        *stop_code = STOP_S_S1Y;
+        return 0;
+    }
+    radw = hscin_1y_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hcer_2_zentrance - hscin_1y_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_2_entr_thick / hcer_2_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_2_zmirror - hcer_2_zentrance;
+    radw = drift / hcer_2_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_mir_thick / hcer_mir_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hcer_2_zexit - hcer_2_zmirror;
+    radw = drift / hcer_2_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    radw = hcer_2_exit_thick / hcer_2_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hscin_2x_zpos - hcer_2_zexit;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (t->ys > (hscin_2x_left + hscin_2y_offset) || t->ys < (hscin_2x_right + hscin_2y_offset)) {
+        cnt[S_S3]++;
+        *stop_code = STOP_S_S2X;
+        return 0;
+    }
+    radw = hscin_2x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    drift = hscin_2y_zpos - hscin_2x_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (t->xs > (hscin_2y_bot + hscin_2x_offset) || t->xs < (hscin_2y_top + hscin_2x_offset)) {
+        cnt[S_S2]++;
+        *stop_code = STOP_S_S2Y;
+        return 0;
+    }
+    radw = hscin_2y_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    drift = hcal_4ta_zpos - hscin_2y_zpos;
+    radw = drift / hair_radlen;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (t->ys > hcal_left || t->ys < hcal_right || t->xs > hcal_bottom || t->xs < hcal_top) {
+        cnt[S_CAL]++;
+        *stop_code = STOP_S_CAL;
+        return 0;
+    }
+
+    for (int jchamber = 1; jchamber <= 2; jchamber++) {
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
+            double z0 = jchamber == 1 ? hdc_1_zpos : hdc_2_zpos;
+            h.zdc[npl_off + iplane - 1] = z0 + ((double)iplane - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        }
+    }
+    fit_planes(S, &h, x_fp, dx_fp, y_fp, dy_fp);
+    *fit_done = 1;
+
+    double xcal = *x_fp + *dx_fp * hcal_4ta_zpos;
+    double ycal = *y_fp + *dy_fp * hcal_4ta_zpos;
+    if (ycal > (hcal_left - 5.0) || ycal < (hcal_right + 5.0) || xcal > (hcal_bottom - 5.0) ||
+        xcal < (hcal_top + 5.0)) {
+        cnt[S_CAL]++;
+        *stop_code = STOP_S_CALFID;
+        return 0;
+    }
+    return 1;
+}
+
+/* mc_shms, shms/mc_shms.f:1-1108 (use_coll true, skip_hb / sieves false) */
+void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+                double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+                double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+                int *stop_code, int *fit_done)
+{
+    const or_fwd_map *F = M->shms_fwd;
+    const double h_entr = 8.5, v_entr = 12.5, h_exit = 8.65, v_exit = 12.85, x_off = +0.00,
+                 y_off = +0.00;
+    const double zd_hbin = 118.39, zd_hbmen = 17.61, zd_hbmex = 80.0, zd_hbout = 17.61;
+    const double z_entr = 25.189, z_thick = 6.35;
+    const double zd_q1in = 58.39, zd_q1men = 28.35, zd_q1mid = 93.65, zd_q1mex = 93.65,
+                 zd_q1out = 28.35;
+    const double zd_q2in = 25.55, zd_q2men = 39.1, zd_q2mid = 79.35, zd_q2mex = 79.35,
+                 zd_q2out = 39.1;
+    const double zd_q3in = 28.10, zd_q3men = 39.1, zd_q3mid = 79.35, zd_q3mex = 79.35,
+                 zd_q3out = 39.1;
+    const double zd_q3d1trans = 18.00, zd_d1flare = 30.10, zd_d1men = 39.47;
+    const double zd_d1mid = 36.406263, zd_d1out = 60.68, zd_fp = 307.95;
+    const double hcer_1_zentrance = -291.700;
+    /* shms/apertures_shms.inc:240-259 */
+    const double r_HBx = 11.2, r_HBfym = -4.13, r_HBfyp = 11.75, r_HBmenym = -5.45,
+                 r_HBmenyp = 11.74, r_HBmexym = -10.25, r_HBmexyp = 11.71, r_HBbym = -11.71,
+                 r_HBbyp = 11.70;
+    const double r_Q1 = 20.00, r_Q2 = 30.00, r_Q3 = 30.00, r_D1 = 30.00;
+    const int decay_flag = S->decay_flag, ms_flag = S->ms_flag, wcs_flag = S->wcs_flag;
+
+    double p, xt, yt, zdrift;
+    double dpp_recon, dth_recon, dph_recon, y_recon;
+    int dflag = 0;
+    *ok_spec = 0;
+    *stop_code = STOP_OK;
+    *fit_done = 0;
+    cnt[S_TRIALS]++;
+
+    t->dpps = *dpp;
+    p = p_spec * (1. + t->dpps / 100.);
+    t->xs = *x;
+    t->ys = *y;
+    t->zs = *z;
+    t->dxdzs = *dxdz;
+    t->dydzs = *dydz;
+
+#define STOPS(c, code) do { cnt[c]++; *stop_code = (code); return; } while (0)
+    /* HB, :421-495 */
+    zdrift = zd_hbin;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_vaxis(t, 1.5, &xt, &yt);
+    yt = yt + 1.51;
+    if ((xt * xt > r_HBx * r_HBx) || (yt > r_HBfyp) || (yt < r_HBfym)) STOPS(S_HB_IN, STOP_S_HB_IN);
+    zdrift = zd_hbmen;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_vaxis(t, 1.5, &xt, &yt);
+    yt = yt + 0.98;
+    if ((xt * xt > r_HBx * r_HBx) || (yt > r_HBmenyp) || (yt < r_HBmenym)) STOPS(S_HB_MEN, STOP_S_HB_MEN);
+    transp(F, g, t, S, 3, decay_flag, &dflag, m2, &p, zd_hbmex, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_vaxis(t, -1.5, &xt, &yt);
+    yt = yt + 0.98;
+    if ((xt * xt > r_HBx * r_HBx) || (yt > r_HBmexyp) || (yt < r_HBmexym)) STOPS(S_HB_MEX, STOP_S_HB_MEX);
+    zdrift = zd_hbout;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_vaxis(t, -1.5, &xt, &yt);
+    yt = yt + 1.51;
+    if ((xt * xt > r_HBx * r_HBx) || (yt > r_HBbyp) || (yt < r_HBbym)) STOPS(S_HB_OUT, STOP_S_HB_OUT);
+
+    /* collimator, :500-568 */
+    if (S->using_coll && (*m2 > 100.0 * 100.0) && (*m2 < 200.0 * 200.0)) {
+        zdrift = z_entr;
+        project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+        if (!coll_step(g, t, S, cnt, 5, m2, &p, p_spec, decay_flag, &dflag, pathlen))
+            STOPS(S_COLL, STOP_COLL);
+    } else {
+        zdrift = z_entr;
+        project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+        xt = t->xs; yt = t->ys;
+        if (fabs(yt - y_off) > h_entr) STOPS(S_SLIT_HOR, STOP_SLIT_HOR);
+        if (fabs(xt - x_off) > v_entr) STOPS(S_SLIT_VERT, STOP_SLIT_VERT);
+        if (fabs(xt - x_off) > (-(v_entr / h_entr * fabs(yt - y_off)) + 3 * v_entr / 2))
+            STOPS(S_SLIT_OCT, STOP_SLIT_OCT);
+        zdrift = z_thick;
+        project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+        xt = t->xs; yt = t->ys;
+        if (fabs(yt - y_off) > (h_exit)) STOPS(S_SLIT_HOR, STOP_SLIT_HOR);
+        if (fabs(xt - x_off) > (v_exit)) STOPS(S_SLIT_VERT, STOP_SLIT_VERT);
+        if (fabs(xt - x_off) > ((-v_exit) / (h_exit)*fabs(yt - y_off) + 3 * (v_exit) / 2))
+            STOPS(S_SLIT_OCT, STOP_SLIT_OCT);
+    }
+
+#define RQ(r) ((t->xs * t->xs + t->ys * t->ys) > (r) * (r))
+    /* Q1, :571-642 */
+    zdrift = zd_q1in - z_entr - z_thick;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q1)) STOPS(S_Q1_IN, STOP_S_Q1_IN);
+    zdrift = zd_q1men;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q1)) STOPS(S_Q1_MEN, STOP_S_Q1_MEN);
+    transp(F, g, t, S, 7, decay_flag, &dflag, m2, &p, zd_q1mid, pathlen);
+    if (RQ(r_Q1)) STOPS(S_Q1_MID, STOP_S_Q1_MID);
+    transp(F, g, t, S, 8, decay_flag, &dflag, m2, &p, zd_q1mex, pathlen);
+    if (RQ(r_Q1)) STOPS(S_Q1_MEX, STOP_S_Q1_MEX);
+    zdrift = zd_q1out;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q1)) STOPS(S_Q1_OUT, STOP_S_Q1_OUT);
+    /* Q2, :644-712 */
+    zdrift = zd_q2in;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q2)) STOPS(S_Q2_IN, STOP_S_Q2_IN);
+    zdrift = zd_q2men;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q2)) STOPS(S_Q2_MEN, STOP_S_Q2_MEN);
+    transp(F, g, t, S, 12, decay_flag, &dflag, m2, &p, zd_q2mid, pathlen);
+    if (RQ(r_Q2)) STOPS(S_Q2_MID, STOP_S_Q2_MID);
+    transp(F, g, t, S, 13, decay_flag, &dflag, m2, &p, zd_q2mex, pathlen);
+    if (RQ(r_Q2)) STOPS(S_Q2_MEX, STOP_S_Q2_MEX);
+    zdrift = zd_q2out;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q2)) STOPS(S_Q2_OUT, STOP_S_Q2_OUT);
+    /* Q3, :714-781 */
+    zdrift = zd_q3in;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q3)) STOPS(S_Q3_IN, STOP_S_Q3_IN);
+    zdrift = zd_q3men;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q3)) STOPS(S_Q3_MEN, STOP_S_Q3_MEN);
+    transp(F, g, t, S, 17, decay_flag, &dflag, m2, &p, zd_q3mid, pathlen);
+    if (RQ(r_Q3)) STOPS(S_Q3_MID, STOP_S_Q3_MID);
+    transp(F, g, t, S, 18, decay_flag, &dflag, m2, &p, zd_q3mex, pathlen);
+    if (RQ(r_Q3)) STOPS(S_Q3_MEX, STOP_S_Q3_MEX);
+    zdrift = zd_q3out;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_Q3)) STOPS(S_Q3_OUT, STOP_S_Q3_OUT);
+    /* D1 mechanical entrance, :815-827 */
+    zdrift = zd_q3d1trans;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (RQ(r_D1)) STOPS(S_D1_IN, STOP_S_D1_IN);
+#undef RQ
+#define RT(r) ((xt * xt + yt * yt) > (r) * (r))
+    /* flare and magnetic entrance, :829-869 */
+    zdrift = zd_d1flare;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_haxis(t, 9.200, &xt, &yt);
+    xt = xt - 3.5;
+    if (RT(r_D1)) STOPS(S_D1_FLR, STOP_S_D1_FLR);
+    zdrift = zd_d1men;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_haxis(t, 9.200, &xt, &yt);
+    xt = xt + 2.82;
+    if (RT(r_D1)) STOPS(S_D1_MEN, STOP_S_D1_MEN);
+    /* dipole interior, :871-990; magnetic exit :995-1011 */
+    {
+        static const double ang[8] = {6.9, 4.6, 2.3, 0.0, -2.3, -4.6, -6.9, -9.2};
+        static const double off[8] = {8.05, 11.75, 13.96, 14.70, 13.96, 11.75, 8.05, 2.82};
+        static const int ctr[8] = {S_D1_MID1, S_D1_MID2, S_D1_MID3, S_D1_MID4,
+                                   S_D1_MID5, S_D1_MID6, S_D1_MID7, S_D1_MEX};
+        for (int k = 0; k < 8; k++) {
+            transp(F, g, t, S, 23 + k, decay_flag, &dflag, m2, &p, zd_d1mid, pathlen);
+            xt = t->xs; yt = t->ys;
+            or_rotate_haxis(t, ang[k], &xt, &yt);
+            xt = xt + off[k];
+            if (RT(r_D1)) STOPS(ctr[k], STOP_S_D1_MID1 + k);
+        }
+    }
+    /* mechanical exit, :1015-1031 */
+    zdrift = zd_d1out;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    xt = t->xs; yt = t->ys;
+    or_rotate_haxis(t, -9.20, &xt, &yt);
+    xt = xt - 6.88;
+    if (RT(r_D1)) STOPS(S_D1_OUT, STOP_S_D1_OUT);
+#undef RT
+#undef STOPS
+
+    /* to the Cherenkov entrance, :1036-1047 */
+    zdrift = zd_fp + hcer_1_zentrance;
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    cnt[S_HUT]++;
+    t->xs_fp = t->xs;
+    t->ys_fp = t->ys;
+    t->dxdzs_fp = t->dxdzs;
+    t->dydzs_fp = t->dydzs;
+
+    int ok = shms_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag,
+                      decay_flag, &dflag, resmult, pathlen, stop_code, fit_done);
+    if (!ok) return;
+    t->xs = *x_fp;
+    t->ys = *y_fp;
+    t->dxdzs = *dx_fp;
+    t->dydzs = *dy_fp;
+    recon(M->shms_rec, t, fry, &dpp_recon, &dth_recon, &dph_recon, &y_recon);
+    *dpp = dpp_recon;
+    *dxdz = dph_recon;
+    *dydz = dth_recon;
+    *y = y_recon;
+    *ok_spec = 1;
+    cnt[S_SUCCESSES]++;
+}
+
+/* ========================================================================= */
+/* Batch driver                                                               */
+/* ========================================================================= */
+
+int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_events_in *in,
+           or_events_out *out, int rng_kind, const double *tape, int64_t stride,
+           uint64_t seed, uint64_t event0, int nthreads, int64_t *counters)
+{
+    if (!M || !S || !in || !out || n < 0) return OR_E_ARG;
+    if (arm != 1 && arm != 5) return OR_E_ARG;
+    if (rng_kind == 0 && (!tape || stride <= 0)) return OR_E_TAPE;
+    int ncount = arm == 1 ? H_NCOUNT : S_NCOUNT;
+    or_ranlux *rl = NULL;
+    if (rng_kind == 2) {
+        rl = or_ranlux_new(3, (int)(seed & 0x7fffffff));
+        nthreads = 1;
+    }
+    if (nthreads <= 0) nthreads = 1;
+    int64_t tape_overflow = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads) reduction(+ : tape_overflow)
+#endif
+    {
+        int64_t cnt[S_NCOUNT];
+        memset(cnt, 0, sizeof cnt);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+        for (int64_t i = 0; i < n; i++) {
+            or_rng g;
+            memset(&g, 0, sizeof g);
+            g.kind = rng_kind;
+            if (rng_kind == 0) {
+                g.tape = tape + i * stride;
+                g.stride = stride;
+            } else if (rng_kind == 1) {
+                g.key[0] = (uint32_t)seed;
+                g.key[1] = (uint32_t)(seed >> 32);
+                g.event = event0 + (uint64_t)i;
+            } else {
+                g.rl = rl;
+            }
+            or_track t;
+            memset(&t, 0, sizeof t);
+            double dpp = in->dpp[i], x = in->x[i], y = in->y[i], z = in->z ? in->z[i] : 0.0;
+            double dxdz = in->dxdz[i], dydz = in->dydz[i];
+            double fry = in->fry ? in->fry[i] : x;
+            double x_fp = out->x_fp[i], dx_fp = out->dx_fp[i], y_fp = out->y_fp[i],
+                   dy_fp = out->dy_fp[i];
+            double m2 = S->m2, resmult = out->resmult ? out->resmult[i] : 0.0;
+            double pathlen = out->pathlen[i];
+            int ok, stop, fit_done;
+            if (arm == 1)
+                or_mc_hms(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
+                          &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
+                          &fit_done);
+            else
+                or_mc_shms(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
+                           &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
+                           &fit_done);
+            if (g.overflow) {
+                stop = STOP_TAPE_OVERFLOW;
+                ok = 0;
+                tape_overflow++;
+            }
+            out->dpp[i] = dpp;
+            out->y[i] = y;
+            out->dxdz[i] = dxdz;
+            out->dydz[i] = dydz;
+            out->x_fp[i] = x_fp;
+            out->dx_fp[i] = dx_fp;
+            out->y_fp[i] = y_fp;
+            out->dy_fp[i] = dy_fp;
+            out->pathlen[i] = pathlen;
+            if (out->resmult) out->resmult[i] = resmult;
+            if (out->m2) out->m2[i] = m2;
+            out->ok[i] = (uint8_t)ok;
+            out->stop[i] = (uint8_t)stop;
+            if (out->used) out->used[i] = (int32_t)g.used;
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (int c = 0; c < ncount; c++) counters[c] += cnt[c];
+        }
+    }
+    if (rl) free(rl);
+    return tape_overflow ? OR_E_TAPE : OR_OK;
+}

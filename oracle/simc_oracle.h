@@ -1,0 +1,216 @@
+/*
+ * simc_oracle.h -- FP64 CPU restatement of SIMC's spectrometer hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in simc_gfortran_amd/ and the timed CPU baseline of
+ * bench.py.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Every routine restates one reference routine (file:line cited at its
+ * definition in simc_oracle.c) with the same operation order, constants and
+ * random-number consumption order, so that the reference compiled from its
+ * own sources (oracle/_ref) and this restatement agree to the last bit on the
+ * same uniform tape, up to libm ulp differences.
+ */
+#ifndef SIMC_ORACLE_H
+#define SIMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_CLASS 41     /* spectrometers.inc:5 */
+#define OR_MAX_TERMS 1000   /* shared/transp.f:81, hms/mc_hms_recon.f:43 */
+
+/* error codes (negative) */
+#define OR_OK 0
+#define OR_E_IO (-1)
+#define OR_E_MAPFMT (-2)
+#define OR_E_NCLASS (-3)
+#define OR_E_TOF (-4)
+#define OR_E_TOOMANY (-5)
+#define OR_E_ARG (-6)
+#define OR_E_TAPE (-7)
+
+typedef struct {
+    int n_classes;
+    int n_terms[OR_MAX_CLASS];
+    double length[OR_MAX_CLASS];     /* cm, from !LENGTH: comments (0 if none) */
+    int adrift[OR_MAX_CLASS];
+    double driftdist[OR_MAX_CLASS];  /* cm */
+    double coeff[OR_MAX_CLASS][OR_MAX_TERMS][5];
+    short expon[OR_MAX_CLASS][OR_MAX_TERMS][5]; /* x, x', y, y', delta */
+} or_fwd_map;
+
+typedef struct {
+    int n_terms;
+    int max_order;
+    double coeff[OR_MAX_TERMS][4];
+    short expon[OR_MAX_TERMS][5];
+} or_recon_map;
+
+/* Uniform source.  kind: 0 = tape, 1 = philox (per event), 2 = ranlux (global). */
+typedef struct {
+    int kind;
+    /* tape */
+    const double *tape;
+    int64_t stride;
+    int64_t pos;
+    /* philox */
+    uint32_t key[2];
+    uint64_t event;
+    uint32_t buf[4];
+    /* ranlux */
+    struct or_ranlux *rl;
+    /* bookkeeping */
+    int64_t used;
+    int overflow;
+} or_rng;
+
+typedef struct or_ranlux {
+    double seeds[25];           /* 1-based as in cern/ranlux.f */
+    int next[25];
+    int i24, j24, in24, nskip, luxlev;
+    double carry, twom24, twom12;
+    int64_t kount;
+    /* grnd() buffer, call_ranlux.f:58-73 */
+    double rvec[1001];
+    int latest;
+} or_ranlux;
+
+/* Per-event transport state: the reference's /track/ common plus the decay
+ * bookkeeping of /ntuple_variables/ and /gnrl/. */
+typedef struct {
+    double xs, ys, zs, dxdzs, dydzs, dpps;
+    double decdist, Mh2_final;
+    double xs_fp, ys_fp, dxdzs_fp, dydzs_fp;
+} or_track;
+
+/* HMS stop counters, same order as common /spectrometer1/ (hms/struct_hms.inc:66-74) */
+enum {
+    H_TRIALS, H_SLIT_HOR, H_SLIT_VERT, H_SLIT_OCT, H_Q1_IN, H_Q1_MID, H_Q1_OUT,
+    H_Q2_IN, H_Q2_MID, H_Q2_OUT, H_Q3_IN, H_Q3_MID, H_Q3_OUT, H_D1_IN, H_D1_OUT,
+    H_HUT, H_DC1, H_DC2, H_SCIN, H_CAL, H_SUCCESSES, H_COLL, H_NCOUNT
+};
+/* SHMS stop counters, same order as common /spectrometer_shms/ (shms/struct_shms.inc:320-338) */
+enum {
+    S_TRIALS, S_SUCCESSES, S_TARG_HOR, S_TARG_VERT, S_TARG_OCT, S_SLIT_HOR, S_SLIT_VERT,
+    S_SLIT_OCT, S_HB_IN, S_HB_MEN, S_HB_MEX, S_HB_OUT, S_Q1_IN, S_Q1_MEN, S_Q1_MID,
+    S_Q1_MEX, S_Q1_OUT, S_Q2_IN, S_Q2_MEN, S_Q2_MID, S_Q2_MEX, S_Q2_OUT, S_Q3_IN,
+    S_Q3_MEN, S_Q3_MID, S_Q3_MEX, S_Q3_OUT, S_D1_IN, S_D1_FLR, S_D1_MEN, S_D1_MID1,
+    S_D1_MID2, S_D1_MID3, S_D1_MID4, S_D1_MID5, S_D1_MID6, S_D1_MID7, S_D1_MEX,
+    S_D1_OUT, S_BP_IN, S_BP_OUT, S_HUT, S_DC1, S_DC2, S_CER1, S_CER2, S_S1, S_S2,
+    S_S3, S_CAL, S_STOP_ID, S_COLL, S_NCOUNT
+};
+
+/* Stop codes: one unambiguous code per exit point of the path.  The batch
+ * drivers write one per event; 0 = accepted. */
+enum {
+    STOP_OK = 0,
+    /* shared front end */
+    STOP_SLIT_HOR = 1, STOP_SLIT_VERT, STOP_SLIT_OCT, STOP_COLL,
+    /* HMS */
+    STOP_H_Q1_IN = 10, STOP_H_Q1_MID, STOP_H_Q1_OUT, STOP_H_Q2_IN, STOP_H_Q2_MID,
+    STOP_H_Q2_OUT, STOP_H_Q3_IN, STOP_H_Q3_MID, STOP_H_Q3_OUT, STOP_H_D1_IN,
+    STOP_H_D1_OUT, STOP_H_PIPE0, STOP_H_PIPE1, STOP_H_PIPE2, STOP_H_PIPE3,
+    STOP_H_DC1, STOP_H_DC2, STOP_H_SCIN,
+    /* SHMS */
+    STOP_S_HB_IN = 40, STOP_S_HB_MEN, STOP_S_HB_MEX, STOP_S_HB_OUT,
+    STOP_S_Q1_IN, STOP_S_Q1_MEN, STOP_S_Q1_MID, STOP_S_Q1_MEX, STOP_S_Q1_OUT,
+    STOP_S_Q2_IN, STOP_S_Q2_MEN, STOP_S_Q2_MID, STOP_S_Q2_MEX, STOP_S_Q2_OUT,
+    STOP_S_Q3_IN, STOP_S_Q3_MEN, STOP_S_Q3_MID, STOP_S_Q3_MEX, STOP_S_Q3_OUT,
+    STOP_S_D1_IN, STOP_S_D1_FLR, STOP_S_D1_MEN, STOP_S_D1_MID1, STOP_S_D1_MID2,
+    STOP_S_D1_MID3, STOP_S_D1_MID4, STOP_S_D1_MID5, STOP_S_D1_MID6, STOP_S_D1_MID7,
+    STOP_S_D1_MEX, STOP_S_D1_OUT, STOP_S_DC1, STOP_S_DC2, STOP_S_S1X, STOP_S_S1Y,
+    STOP_S_S2X, STOP_S_S2Y, STOP_S_CAL, STOP_S_CALFID,
+    /* bookkeeping */
+    STOP_TAPE_OVERFLOW = 250
+};
+
+typedef struct {
+    double p_spec;      /* MeV/c */
+    double m2;          /* MeV^2, initial particle mass squared */
+    double ctau;        /* cm, decay length constant (simulate.inc:145) */
+    int ms_flag, wcs_flag, decay_flag, using_coll;
+    int fit_f32;        /* 1: REAL*4 track fit as the reference; 0: FP64 fit */
+    double wire_sigma;  /* cm; <= 0 keeps the reference table */
+} or_settings;
+
+typedef struct {
+    or_fwd_map *hms_fwd, *shms_fwd;
+    or_recon_map *hms_rec, *shms_rec;
+} or_maps;
+
+/* ---- map files ---- */
+int or_read_forward(const char *path, or_fwd_map *m);
+int or_read_recon(const char *path, or_recon_map *m);
+or_maps *or_maps_load(const char *dir);   /* dir holds hms/ and shms/ */
+void or_maps_free(or_maps *m);
+
+/* ---- random numbers ---- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double or_philox_uniform(uint32_t key0, uint32_t key1, uint64_t event, uint64_t k);
+or_ranlux *or_ranlux_new(int lux, int seed);
+void or_ranlux_fill(or_ranlux *r, double *rvec, int len);
+void or_ranlux_save(const or_ranlux *r, int32_t isdext[25]);
+void or_ranlux_restore(or_ranlux *r, const int32_t isdext[25]);
+double or_grnd(or_rng *g);
+
+/* ---- single routines (exported for unit tests) ---- */
+double or_powi(double x, int m);
+double or_gauss1(or_rng *g, double nsigmax);
+void or_lfit_f32(const float *X, const float *Y, int L, int KEY, float *A, float *B, float *E);
+void or_lfit_f64(const double *X, const double *Y, int L, int KEY, double *A, double *B, double *E);
+void or_loren(double gam, double bx, double by, double bz, double e, double x, double y,
+              double z, double *ef1, double *pxf, double *pyf, double *pzf, double *pf1);
+void or_enerloss_new(or_rng *g, double len, double dens, double zeff, double aeff,
+                     double epart, double mpart, int typeflag, double *eloss);
+double or_pion_coll_absorb(double ppi, double thick);
+void or_rotate_haxis(const or_track *t, double rotang, double *xp0, double *yp0);
+void or_rotate_vaxis(const or_track *t, double rotang, double *xp0, double *yp0);
+void or_transp_eval(const or_fwd_map *m, int cls, const double ray[5], double sum[5]);
+void or_recon_eval(const or_recon_map *m, const double hut[5], double sum[4]);
+
+/* ---- per-event arms (argument lists of hms/mc_hms.f:1-4, shms/mc_shms.f:1-4) ---- */
+void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+               double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+               double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+               double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+               int *stop_code, int *fit_done);
+void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+                double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+                double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+                int *stop_code, int *fit_done);
+
+/* ---- batch driver ----
+ * arm: 1 = HMS, 5 = SHMS.  rng_kind: 0 tape (tape[i*stride + k]), 1 philox
+ * (key = seed, stream = global event id = event0 + i), 2 ranlux (one global
+ * stream in event order; forces a single thread).
+ * Inputs are SoA; outputs follow the reference's INOUT conventions:
+ *   dpp/y/dxdz/dydz: recon values on success, input values otherwise;
+ *   x_fp..dy_fp: the track fit when it was reached, else left untouched;
+ *   pathlen: input + accumulated path.
+ * counters: H_NCOUNT or S_NCOUNT int64, accumulated (not cleared). */
+typedef struct {
+    const double *dpp, *x, *y, *z, *dxdz, *dydz, *fry;
+} or_events_in;
+typedef struct {
+    double *dpp, *y, *dxdz, *dydz;        /* recon (INOUT semantics) */
+    double *x_fp, *dx_fp, *y_fp, *dy_fp;  /* focal plane */
+    double *pathlen;                      /* INOUT */
+    double *resmult, *m2;                 /* optional (may be NULL) */
+    uint8_t *ok, *stop;                   /* stop code per event */
+    int32_t *used;                        /* uniforms consumed (optional) */
+} or_events_out;
+
+int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_events_in *in,
+           or_events_out *out, int rng_kind, const double *tape, int64_t stride,
+           uint64_t seed, uint64_t event0, int nthreads, int64_t *counters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

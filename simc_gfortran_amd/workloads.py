@@ -1,0 +1,85 @@
+"""Synthetic event workloads for the spectrometer path (the inputs the caller
+of mc_hms / mc_shms prepares in simc.f:1403-1443).
+
+Each workload draws target-frame vertices and angles with numpy from a fixed
+seed and converts them to spectrometer TRANSPORT coordinates exactly as
+montecarlo() does before calling the arm (simc.f:1411-1441):
+
+    x_arm = -y_tgt
+    y_arm = -x_tgt cos(th) - z_tgt sin(th) sin(phi)
+    z_arm =  z_tgt cos(th) + x_tgt sin(th) sin(phi)
+    (x_arm, y_arm) -= z_arm (dx/dz, dy/dz)        # drift back to z = 0
+    fry = x_arm                                    # xtar_init, simc.f:1441
+
+with phi = 3 pi/2 for a left-side spectrometer (HMS) and pi/2 for the SHMS
+(dbase.f sets spec%p%phi / spec%e%phi this way).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ME = 0.51099906        # constants.inc:91
+MP = 938.27231         # constants.inc:93
+MPI = 139.57018        # constants.inc:97
+
+
+@dataclass
+class Workload:
+    name: str
+    arm: int                 # 1 = HMS, 5 = SHMS
+    n: int
+    p_spec: float            # MeV/c
+    theta_deg: float
+    mass: float              # MeV
+    dpp: tuple               # uniform range, %
+    dxdz: tuple              # uniform range, rad
+    dydz: tuple              # uniform range, rad
+    z: tuple = (-5.0, 5.0)   # cm, 10 cm LH2 cell
+    sigma_xy: float = 0.01   # cm
+    wire_sigma: float = 0.0  # cm; 0 keeps the spectrometer's own value
+    seed: int = 1001
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def m2(self):
+        return self.mass * self.mass
+
+
+# BASELINE configurations (sizes as quoted there)
+HMS_C0 = Workload("hms_proton_3GeV", 1, 100_000, 3000.0, 20.0, MP, (-12.0, 12.0),
+                  (-0.08, 0.08), (-0.05, 0.05), seed=1001)
+SHMS_C1 = Workload("shms_electron_6GeV", 5, 10_000_000, 6000.0, 12.5, ME, (-15.0, 25.0),
+                   (-0.06, 0.06), (-0.04, 0.04), wire_sigma=0.030, seed=1002)
+
+
+def generate(w: Workload, n: int | None = None, seed: int | None = None) -> dict:
+    """Spectrometer-frame SoA inputs (float64) for workload w."""
+    n = w.n if n is None else n
+    rng = np.random.default_rng(w.seed if seed is None else seed)
+    xt = rng.normal(0.0, w.sigma_xy, n)
+    yt = rng.normal(0.0, w.sigma_xy, n)
+    zt = rng.uniform(w.z[0], w.z[1], n)
+    dpp = rng.uniform(w.dpp[0], w.dpp[1], n)
+    dxdz = rng.uniform(w.dxdz[0], w.dxdz[1], n)
+    dydz = rng.uniform(w.dydz[0], w.dydz[1], n)
+    th = math.radians(w.theta_deg)
+    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    cth, sth, sphi = math.cos(th), math.sin(th), math.sin(phi)
+    x = -yt
+    y = -xt * cth - zt * sth * sphi
+    z = zt * cth + xt * sth * sphi
+    x = x - z * dxdz
+    y = y - z * dydz
+    z = np.zeros(n)
+    return {"dpp": dpp, "x": x, "y": y, "z": z, "dxdz": dxdz, "dydz": dydz, "fry": x.copy()}
+
+
+def uniform_tape(n: int, stride: int, seed: int) -> np.ndarray:
+    """Per-event uniform tapes in (0,1): event i draws tape[i*stride + k]."""
+    rng = np.random.default_rng(seed)
+    t = rng.random(n * stride)
+    t[t == 0.0] = 0.5
+    return t
