@@ -1,0 +1,161 @@
+"""Throughput of the spectrometer transport + reconstruction path on one GPU.
+
+Workload (one step): the SHMS single-arm configuration -- 1e7 electrons at a
+6.0 GeV/c central momentum, delta ~ U(-15, 25) %, dx/dz ~ U(-0.06, 0.06),
+dy/dz ~ U(-0.04, 0.04), z ~ U(-5, 5) cm, multiple scattering on, wire-chamber
+smearing sigma = 0.030 cm, track fit on (FP64), through mc_shms + hut +
+reconstruction.  Inputs are synthetic (numpy, fixed seed) and resident in HBM
+before timing; each step draws fresh per-event Philox streams.
+
+Prints one JSON line (the driver's contract) including the roofline of the
+transport kernel and the CPU baseline (the FP64 oracle on the host cores over
+a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(wl, seconds=12.0):
+    """The FP64 oracle over a bounded sample of the workload on the host cores."""
+    from oracle import oracle as O
+    from simc_gfortran_amd import workloads as W
+    threads = min(os.cpu_count() or 1, 16)
+    chunk = 200_000
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma,
+              fit_f32=False)
+    done, t_used, k = 0, 0.0, 0
+    while t_used < seconds:
+        ev = W.generate(wl, chunk, seed=wl.seed + 1000 + k)
+        t0 = time.perf_counter()
+        O.run(wl.arm, ev, rng="philox", seed=99, event0=done, nthreads=threads, **kw)
+        t_used += time.perf_counter() - t0
+        done += chunk
+        k += 1
+    # one thread, the reference's execution model, on a smaller sample
+    ev = W.generate(wl, 40_000, seed=wl.seed + 999)
+    t0 = time.perf_counter()
+    O.run(wl.arm, ev, rng="philox", seed=99, nthreads=1, **kw)
+    t1 = time.perf_counter() - t0
+    return {"value": done / t_used, "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"{done} events of the same workload ({k} x {chunk}), FP64 oracle "
+                      f"(oracle/simc_oracle.c), OpenMP over {threads} threads, {cpu_model()}",
+            "single_thread_events_per_s": 40_000 / t1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--events", type=int, default=10_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(events shard trivially; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from simc_gfortran_amd import api
+    from simc_gfortran_amd import workloads as W
+
+    wl = W.SHMS_C1
+    n = args.events
+    ev = W.generate(wl, n)
+    dev = torch.device("cuda:0")
+    d_ev = {k: torch.from_numpy(v).to(dev) for k, v in ev.items() if k != "z"}
+    ctx = api.Spectrometers()
+    out = {k: torch.empty(n, dtype=torch.float64, device=dev)
+           for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen")}
+    out["ok"] = torch.empty(n, dtype=torch.uint8, device=dev)
+    out["stop"] = torch.empty(n, dtype=torch.uint8, device=dev)
+    out["resmult"] = out["m2"] = out["used"] = None
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma,
+              fit="fp64", out=out)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(s):
+        out["pathlen"].zero_()
+        ctx.run(wl.arm, d_ev, rng="philox", seed=1234, event0=s * n, **kw)
+
+    for s in range(args.warmup):
+        step(s)
+    torch.cuda.synchronize()
+    ev_start = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_stop = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        out["pathlen"].zero_()
+        ev_start[s].record(stream)
+        ctx.run(wl.arm, d_ev, rng="philox", seed=1234, event0=(args.warmup + s) * n, **kw)
+        ev_stop[s].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_start, ev_stop)]))
+
+    # algorithmic work of the last launch, from its stop codes
+    stop = out["stop"].to(torch.int64)
+    hist = torch.bincount(stop, minlength=256).cpu().numpy()
+    flops_by_stop = np.array(ctx.stop_flops(wl.arm), dtype=np.float64)
+    flops = float((hist * flops_by_stop).sum())
+    accepted = int(hist[0])
+    fit_events = int(out["ok"].sum().item())  # lower bound on events writing the fit
+    bytes_launch = n * (6 * 8 + 8) + n * (4 * 8 + 8 + 2) + fit_events * 4 * 8
+    tflops = flops / (kernel_ms * 1e-3) / 1e12
+    value = args.steps * n / wall
+    line = {
+        "metric": "events/s transported+reconstructed (FP64, 1 GPU); batched & adjoint vs plain",
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "shms_single_arm_electron_6GeV_1e7", "events_per_step": n,
+                   "p_spec_MeV": wl.p_spec, "theta_deg": wl.theta_deg,
+                   "wire_sigma_cm": wl.wire_sigma, "ms": True, "wcs": True, "fit": "fp64",
+                   "rng": "philox4x32-10 per event"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": kernel_ms, "algorithmic_flop_per_launch": flops,
+                     "hbm_GBps_algorithmic": bytes_launch / (kernel_ms * 1e-3) / 1e9},
+        "accepted_fraction": accepted / n,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(wl)
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

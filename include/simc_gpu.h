@@ -1,0 +1,161 @@
+/*
+ * simc_gpu.h -- C ABI of the MI355X spectrometer transport + reconstruction
+ * library (libsimc_gpu.so).
+ *
+ * It replaces, for SIMC's per-event spectrometer hot path, the Fortran
+ * subroutines
+ *   mc_hms(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp,
+ *          dy_fp, m2, ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec,
+ *          pathlen, using_HMScoll)                       hms/mc_hms.f:1-4
+ *   mc_shms(... same ..., pathlen, spectr, using_SHMScoll) shms/mc_shms.f:1-4
+ *   mc_hms_recon(delta_p, delta_t, delta_phi, y_tgt, fry) hms/mc_hms_recon.f:1
+ *   mc_shms_recon(..., fry, spectr)                       shms/mc_shms_recon.f:1-2
+ *   transp_init(spectr, n_classes)                       shared/transp.f:294
+ * called once per event per arm from montecarlo() (simc.f:1464-1488,
+ * 1730-1754).  Here one call transports a whole batch of events held in
+ * device memory as structure-of-arrays.
+ *
+ * Conventions: every entry point returns 0 or a negative SIMC_E_* code and
+ * never exits the process; simc_last_error() describes the last failure of
+ * the calling thread.  Pointers in simc_events_in / simc_events_out / tapes
+ * are DEVICE pointers.  A context owns the device copies of the maps and the
+ * stop counters; use one context per host thread.
+ */
+#ifndef SIMC_GPU_H
+#define SIMC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMC_OK 0
+#define SIMC_E_IO (-1)        /* map file cannot be opened (transp.f:318-321 stops)  */
+#define SIMC_E_MAPFMT (-2)    /* malformed map line (Fortran read error)             */
+#define SIMC_E_NCLASS (-3)    /* wrong class count (hms/mc_hms.f:188, shms/mc_shms.f:344) */
+#define SIMC_E_TOF (-4)       /* non-zero TOF term (transp.f:385-389)                */
+#define SIMC_E_TOOMANY (-5)   /* > 1000 terms or > 41 classes (transp.f:365,371)     */
+#define SIMC_E_ARG (-6)       /* invalid argument                                    */
+#define SIMC_E_TAPE (-7)      /* an event needed more uniforms than its tape slot    */
+#define SIMC_E_MAPSTRUCT (-8) /* map terms differ from the compiled polynomial structure */
+#define SIMC_E_HIP (-9)       /* HIP runtime error                                   */
+#define SIMC_E_NOMEM (-10)
+
+#define SIMC_ARM_HMS 1        /* spectrometer numbers of transp_init, transp.f:300-313 */
+#define SIMC_ARM_SHMS 5
+
+#define SIMC_RNG_PHILOX 1     /* counter-based: uniform k of event e = Philox4x32-10(key=seed, ctr=(k/4, e, tag)) */
+#define SIMC_RNG_TAPE 0       /* uniform k of event i = tape[i*stride + k] (parity runs)               */
+
+#define SIMC_FIT_REF_F32 1    /* REAL*4 hits and fit parameters, as cern/lfit.f:18 */
+#define SIMC_FIT_FP64 0       /* the same fit carried out in FP64                 */
+
+#define SIMC_HMS_NCOUNTERS 22  /* common /spectrometer1/,      hms/struct_hms.inc:66-74   */
+#define SIMC_SHMS_NCOUNTERS 52 /* common /spectrometer_shms/, shms/struct_shms.inc:320-338 */
+
+typedef struct simc_ctx simc_ctx;
+
+/* Per-call (per kinematic setting) scalars: the by-value arguments of mc_hms. */
+typedef struct {
+    double p_spec;        /* central momentum, MeV/c                          */
+    double m2;            /* particle mass squared, MeV^2                     */
+    double ctau;          /* decay length c*tau, cm (simulate.inc:145)        */
+    int32_t ms_flag;      /* multiple scattering                              */
+    int32_t wcs_flag;     /* wire-chamber smearing                            */
+    int32_t decay_flag;   /* in-flight decay                                  */
+    int32_t using_coll;   /* step pions/muons through the collimator          */
+    int32_t fit_mode;     /* SIMC_FIT_REF_F32 or SIMC_FIT_FP64                */
+    int32_t reserved;
+    double wire_sigma;    /* cm; <= 0 keeps the spectrometer's own table      */
+} simc_settings;
+
+/* Event inputs (device SoA, n doubles each).  z is accepted for interface
+ * parity (the arms store it but never use it); fry may be NULL (= x). */
+typedef struct {
+    const double *dpp, *x, *y, *z, *dxdz, *dydz, *fry;
+} simc_events_in;
+
+/* Event outputs (device SoA).  dpp, y, dxdz, dydz, pathlen are INOUT with
+ * mc_hms semantics and may alias the inputs: on success they receive the
+ * reconstructed delta (%), y_tar (cm), x'_tar, y'_tar; on failure the input
+ * values (hms/mc_hms.f:427-440).  pathlen accumulates.  x_fp..dy_fp are
+ * written when the track fit was reached, else left untouched.  resmult, m2,
+ * used may be NULL.  stop receives the exit point (SIMC_STOP_*), ok 1/0. */
+typedef struct {
+    double *dpp, *y, *dxdz, *dydz;
+    double *x_fp, *dx_fp, *y_fp, *dy_fp;
+    double *pathlen;
+    double *resmult, *m2;
+    uint8_t *ok, *stop;
+    int32_t *used;
+} simc_events_out;
+
+typedef struct {
+    int32_t kind;          /* SIMC_RNG_PHILOX or SIMC_RNG_TAPE */
+    int32_t reserved;
+    uint64_t seed;         /* Philox key                        */
+    uint64_t event0;       /* global id of event 0 of this call */
+    const double *tape;    /* device tape (SIMC_RNG_TAPE)       */
+    int64_t stride;        /* uniforms per event slot           */
+} simc_rng;
+
+/* transp_init + the recon readers for both arms.  map_dir holds
+ * hms/forward_cosy.dat, hms/recon_cosy.dat, shms/shms_forward.dat,
+ * shms/shms_recon.dat (the reference's relative paths, transp.f:300-309,
+ * hms/mc_hms_recon.f:73, shms/mc_shms_recon.f:74). */
+int simc_ctx_create(const char *map_dir, simc_ctx **out);
+int simc_ctx_create_files(const char *hms_forward, const char *hms_recon,
+                          const char *shms_forward, const char *shms_recon, simc_ctx **out);
+void simc_ctx_destroy(simc_ctx *ctx);
+
+/* Transport + reconstruct n events through one arm (mc_hms / mc_shms).
+ * stream: a hipStream_t (NULL = default stream).  Asynchronous: returns after
+ * the launch. */
+int simc_hms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
+                 simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+int simc_shms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
+                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
+                 simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+
+/* Reconstruction only (mc_hms_recon / mc_shms_recon) of focal-plane tracks:
+ * x_fp, y_fp in cm, slopes in rad, fry in cm.  Outputs delta (%), x'_tar,
+ * y'_tar (rad), y_tar (cm). */
+int simc_recon_run(simc_ctx *ctx, int arm, int64_t n, const double *x_fp, const double *dx_fp,
+                   const double *y_fp, const double *dy_fp, const double *fry, double *delta,
+                   double *dxdz_tar, double *dydz_tar, double *y_tar, void *stream);
+
+/* STOP counters accumulated by the context since creation / reset, in the
+ * order of the reference's common blocks.  Synchronises the device. */
+int simc_counters_get(simc_ctx *ctx, int arm, int64_t *counters, int ncounters);
+int simc_counters_reset(simc_ctx *ctx, int arm);
+
+/* Central-ray geometry of the loaded forward map: drift length of a class
+ * (cm) and its pure-drift flag (/driftdistances/, spectrometers.inc:130-133). */
+int simc_map_class_info(simc_ctx *ctx, int arm, int cls, double *driftdist, int32_t *adrift,
+                        int32_t *n_terms);
+
+/* Algorithmic FP64 operations of the polynomial part for one event that
+ * reaches every segment of the arm (monomial multiplies + 2 per coefficient). */
+int64_t simc_poly_flops_full(simc_ctx *ctx, int arm);
+
+/* The same count for an event that ends with stop code c, written to
+ * flops[c] for c = 0..255 (nflops >= 256): the segments it passes through. */
+int simc_stop_flops(simc_ctx *ctx, int arm, int64_t *flops, int nflops);
+
+const char *simc_last_error(void);
+const char *simc_version(void);
+
+/* Stop codes written to simc_events_out.stop */
+#define SIMC_STOP_OK 0
+#define SIMC_STOP_SLIT_HOR 1
+#define SIMC_STOP_SLIT_VERT 2
+#define SIMC_STOP_SLIT_OCT 3
+#define SIMC_STOP_COLL 4
+#define SIMC_STOP_TAPE_OVERFLOW 250
+
+#ifdef __cplusplus
+}
+#endif
+#endif

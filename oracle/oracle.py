@@ -89,6 +89,8 @@ def lib():
         L.or_philox4x32_10.restype = None
         L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint32)]
+        L.or_recon_run.restype = C.c_int
+        L.or_recon_run.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 9
         L.or_pion_coll_absorb.restype = C.c_double
         L.or_pion_coll_absorb.argtypes = [C.c_double, C.c_double]
         _lib = L
@@ -194,3 +196,14 @@ def philox_block(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().or_philox4x32_10(a, k, o)
     return list(o)
+
+
+def recon(arm, x_fp, dx_fp, y_fp, dy_fp, fry):
+    """Reconstruction only: returns delta (%), x'_tar, y'_tar, y_tar (cm)."""
+    n = len(x_fp)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_fp, dx_fp, y_fp, dy_fp, fry)]
+    outs = [np.zeros(n) for _ in range(4)]
+    rc = lib().or_recon_run(maps(), arm, n, *[_p(v) for v in a], *[_p(v) for v in outs])
+    if rc:
+        raise RuntimeError(f"or_recon_run: {rc}")
+    return tuple(outs)

@@ -1875,3 +1875,27 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
     if (rl) free(rl);
     return tape_overflow ? OR_E_TAPE : OR_OK;
 }
+
+/* Reconstruction only (mc_hms_recon / mc_shms_recon) for n focal-plane tracks. */
+int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const double *dx_fp,
+                 const double *y_fp, const double *dy_fp, const double *fry, double *delta,
+                 double *dxdz, double *dydz, double *ytar)
+{
+    if (!M || (arm != 1 && arm != 5) || n < 0) return OR_E_ARG;
+    const or_recon_map *R = arm == 1 ? M->hms_rec : M->shms_rec;
+    for (int64_t i = 0; i < n; i++) {
+        or_track t;
+        memset(&t, 0, sizeof t);
+        t.xs = x_fp[i];
+        t.dxdzs = dx_fp[i];
+        t.ys = y_fp[i];
+        t.dydzs = dy_fp[i];
+        double dp, dt, dph, yt;
+        recon(R, &t, fry ? fry[i] : 0.0, &dp, &dt, &dph, &yt);
+        delta[i] = dp;
+        dxdz[i] = dph;
+        dydz[i] = dt;
+        ytar[i] = yt;
+    }
+    return OR_OK;
+}

@@ -1,0 +1,267 @@
+"""Host-side interface to libsimc_gpu.so (the MI355X spectrometer path).
+
+Mirrors the reference's per-arm subroutines, vectorised over events held in
+GPU memory (torch float64 tensors on the current HIP device):
+
+    mc_hms(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, ..., m2, ms_flag,
+           wcs_flag, decay_flag, fry, pathlen, using_HMScoll)   hms/mc_hms.f:1-4
+    mc_shms(..., spectr, using_SHMScoll)                        shms/mc_shms.f:1-4
+    mc_hms_recon / mc_shms_recon                                hms/mc_hms_recon.f:1
+
+The compute runs only in the HIP library; importing this module on a machine
+without the built library raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libsimc_gpu.so")
+MAP_DIR = os.path.join(REPO, "data", "maps")
+
+HMS = 1
+SHMS = 5
+RNG_TAPE = 0
+RNG_PHILOX = 1
+FIT_REF_F32 = 1
+FIT_FP64 = 0
+
+HMS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "slit_oct", "Q1_in", "Q1_mid", "Q1_out", "Q2_in",
+    "Q2_mid", "Q2_out", "Q3_in", "Q3_mid", "Q3_out", "D1_in", "D1_out", "hut", "dc1", "dc2",
+    "scin", "cal", "successes", "coll"]
+SHMS_COUNTERS = [
+    "trials", "successes", "targ_hor", "targ_vert", "targ_oct", "slit_hor", "slit_vert",
+    "slit_oct", "HB_in", "HB_men", "HB_mex", "HB_out", "Q1_in", "Q1_men", "Q1_mid", "Q1_mex",
+    "Q1_out", "Q2_in", "Q2_men", "Q2_mid", "Q2_mex", "Q2_out", "Q3_in", "Q3_men", "Q3_mid",
+    "Q3_mex", "Q3_out", "D1_in", "D1_flr", "D1_men", "D1_mid1", "D1_mid2", "D1_mid3",
+    "D1_mid4", "D1_mid5", "D1_mid6", "D1_mid7", "D1_mex", "D1_out", "BP_in", "BP_out", "hut",
+    "dc1", "dc2", "cer1", "cer2", "s1", "s2", "s3", "cal", "stop_id", "coll"]
+
+# Every exported C entry point (include/simc_gpu.h).
+EXPORTS = [
+    "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_destroy", "simc_hms_run",
+    "simc_shms_run", "simc_arm_run", "simc_recon_run", "simc_counters_get",
+    "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
+    "simc_last_error",
+    "simc_version",
+]
+
+_vp = C.c_void_p
+
+
+class Settings(C.Structure):
+    _fields_ = [("p_spec", C.c_double), ("m2", C.c_double), ("ctau", C.c_double),
+                ("ms_flag", C.c_int32), ("wcs_flag", C.c_int32), ("decay_flag", C.c_int32),
+                ("using_coll", C.c_int32), ("fit_mode", C.c_int32), ("reserved", C.c_int32),
+                ("wire_sigma", C.c_double)]
+
+
+class EventsIn(C.Structure):
+    _fields_ = [(n, _vp) for n in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")]
+
+
+OUT_FIELDS = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
+              "resmult", "m2", "ok", "stop", "used")
+
+
+class EventsOut(C.Structure):
+    _fields_ = [(n, _vp) for n in OUT_FIELDS]
+
+
+class Rng(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64),
+                ("event0", C.c_uint64), ("tape", _vp), ("stride", C.c_int64)]
+
+
+class SimcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"simc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def library(path: str = LIB_PATH):
+    """Load libsimc_gpu.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m simc_gfortran_amd.build`")
+    L = C.CDLL(path)
+    L.simc_ctx_create.argtypes = [C.c_char_p, C.POINTER(_vp)]
+    L.simc_ctx_create_files.argtypes = [C.c_char_p] * 4 + [C.POINTER(_vp)]
+    L.simc_ctx_destroy.argtypes = [_vp]
+    L.simc_ctx_destroy.restype = None
+    L.simc_arm_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
+                               C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp]
+    L.simc_recon_run.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9 + [_vp]
+    L.simc_counters_get.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.simc_counters_reset.argtypes = [_vp, C.c_int]
+    L.simc_map_class_info.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.simc_poly_flops_full.argtypes = [_vp, C.c_int]
+    L.simc_poly_flops_full.restype = C.c_int64
+    L.simc_stop_flops.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.simc_last_error.restype = C.c_char_p
+    L.simc_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise SimcError(rc, library().simc_last_error().decode())
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _dev_f64(t, n, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise TypeError(f"{name}: expected a float64 tensor on a HIP device")
+    if t.numel() != n or not t.is_contiguous():
+        raise ValueError(f"{name}: expected {n} contiguous elements")
+    return t
+
+
+class Spectrometers:
+    """A context: the COSY maps of both arms on the device plus stop counters
+    (transp_init and the recon readers, shared/transp.f:294-478)."""
+
+    def __init__(self, map_dir: str = MAP_DIR, files=None):
+        L = library()
+        h = _vp()
+        if files is not None:
+            _check(L.simc_ctx_create_files(*[f.encode() for f in files], C.byref(h)))
+        else:
+            _check(L.simc_ctx_create(map_dir.encode(), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            library().simc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- transport + reconstruction -------------------------------------------------
+    def run(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
+            coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
+            tape=None, stride=0, out=None, stream=None):
+        """Transport + reconstruct events through one arm.
+
+        events: dict of float64 device tensors dpp, x, y, dxdz, dydz (+ z, fry).
+        Returns a dict of device tensors (see include/simc_gpu.h for semantics):
+        dpp, y, dxdz, dydz (recon or input), x_fp, dx_fp, y_fp, dy_fp, pathlen,
+        resmult, m2, ok (uint8), stop (uint8), used (int32)."""
+        n = events["dpp"].numel()
+        ev = {k: _dev_f64(events[k], n, k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
+        for k in ("z", "fry"):
+            ev[k] = _dev_f64(events[k], n, k) if events.get(k) is not None else None
+        dev = ev["dpp"].device
+        if out is None:
+            out = {k: ev[k].clone() for k in ("dpp", "y", "dxdz", "dydz")}
+            for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2"):
+                out[k] = torch.zeros(n, dtype=torch.float64, device=dev)
+            out["ok"] = torch.zeros(n, dtype=torch.uint8, device=dev)
+            out["stop"] = torch.zeros(n, dtype=torch.uint8, device=dev)
+            out["used"] = torch.zeros(n, dtype=torch.int32, device=dev)
+        S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
+                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
+        ein = EventsIn(*[_ptr(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
+        eout = EventsOut(*[_ptr(out.get(k)) for k in OUT_FIELDS])
+        if rng == "tape":
+            if tape is None or stride <= 0:
+                raise ValueError("tape rng needs tape and stride")
+            R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _check(library().simc_arm_run(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
+                                      C.byref(R), C.c_void_p(st)))
+        return out
+
+    def recon(self, arm, x_fp, dx_fp, y_fp, dy_fp, fry=None, stream=None):
+        """mc_hms_recon / mc_shms_recon over focal-plane tracks."""
+        n = x_fp.numel()
+        ins = [_dev_f64(t, n, k) for t, k in ((x_fp, "x_fp"), (dx_fp, "dx_fp"), (y_fp, "y_fp"),
+                                               (dy_fp, "dy_fp"))]
+        fr = _dev_f64(fry, n, "fry") if fry is not None else None
+        outs = [torch.empty_like(ins[0]) for _ in range(4)]
+        st = stream if stream is not None else torch.cuda.current_stream(ins[0].device).cuda_stream
+        _check(library().simc_recon_run(self._h, arm, n, *[_ptr(t) for t in ins], _ptr(fr),
+                                        *[_ptr(t) for t in outs], C.c_void_p(st)))
+        return tuple(outs)  # delta (%), x'_tar, y'_tar, y_tar (cm)
+
+    # -- bookkeeping -------------------------------------------------------------------
+    def counters(self, arm):
+        names = HMS_COUNTERS if arm == HMS else SHMS_COUNTERS
+        buf = (C.c_int64 * len(names))()
+        _check(library().simc_counters_get(self._h, arm, buf, len(names)))
+        return dict(zip(names, list(buf)))
+
+    def reset_counters(self, arm):
+        _check(library().simc_counters_reset(self._h, arm))
+
+    def class_info(self, arm, cls):
+        d, a, n = C.c_double(), C.c_int32(), C.c_int32()
+        _check(library().simc_map_class_info(self._h, arm, cls, C.byref(d), C.byref(a), C.byref(n)))
+        return d.value, bool(a.value), n.value
+
+    def poly_flops_full(self, arm):
+        return int(library().simc_poly_flops_full(self._h, arm))
+
+    def stop_flops(self, arm):
+        """Algorithmic polynomial FP64 operations of an event, by stop code."""
+        buf = (C.c_int64 * 256)()
+        _check(library().simc_stop_flops(self._h, arm, buf, 256))
+        return list(buf)
+
+
+def _arm_call(ctx, arm, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+              fry, pathlen, using_coll, ctau, **kw):
+    ev = {"dpp": dpp, "x": x, "y": y, "z": z, "dxdz": dxdz, "dydz": dydz, "fry": fry}
+    n = dpp.numel()
+    out = {"dpp": dpp, "y": y, "dxdz": dxdz, "dydz": dydz,
+           "pathlen": pathlen if pathlen is not None else torch.zeros_like(dpp)}
+    for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "resmult", "m2"):
+        out[k] = torch.zeros_like(dpp)
+    out["ok"] = torch.zeros(n, dtype=torch.uint8, device=dpp.device)
+    out["stop"] = torch.zeros(n, dtype=torch.uint8, device=dpp.device)
+    out["used"] = None
+    ctx.run(arm, ev, p_spec=p_spec, m2=m2, ms=ms_flag, wcs=wcs_flag, decay=decay_flag,
+            ctau=ctau, coll=using_coll, out=out, **kw)
+    return out
+
+
+def mc_hms(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+           fry, pathlen=None, using_HMScoll=False, ctau=0.0, **kw):
+    """Vectorised mc_hms (hms/mc_hms.f:1-4).  dpp, y, dxdz, dydz, pathlen are
+    updated in place (INOUT); returns the output dict.  th_spec is unused by
+    the reference too (hms/mc_hms.f:153)."""
+    del th_spec
+    return _arm_call(ctx, HMS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
+                     decay_flag, fry, pathlen, using_HMScoll, ctau, **kw)
+
+
+def mc_shms(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+            fry, pathlen=None, spectr=5, using_SHMScoll=False, ctau=0.0, **kw):
+    """Vectorised mc_shms (shms/mc_shms.f:1-4); spectr must be 5 (:336-341)."""
+    del th_spec
+    if spectr != 5:
+        raise SimcError(-6, "mc_shms: spectr must be 5 (the LSA tune is no longer supported)")
+    return _arm_call(ctx, SHMS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
+                     decay_flag, fry, pathlen, using_SHMScoll, ctau, **kw)

@@ -1,0 +1,224 @@
+"""Build-time generator of the COSY polynomial device code.
+
+Reads the forward and reconstruction map files (the same files mc_hms /
+mc_shms load at run time: hms/forward_cosy.dat, hms/recon_cosy.dat,
+shms/shms_forward.dat, shms/shms_recon.dat) and writes cosy_gen.inc:
+
+* one __device__ function per non-drift forward class and per recon map that
+  evaluates  out_k = sum_i c_{k,i} prod_v r_v^{e_{v,i}}  (shared/transp.f:205-218,
+  hms/mc_hms_recon.f:122-130) with
+    - every monomial formed from a parent monomial by ONE multiply, walking the
+      monomial trie depth first (the live set stays at the depth of the trie),
+    - one FMA per NON-ZERO coefficient (about 2 of the 5 forward outputs of a
+      term are non-zero: mid-plane symmetry),
+    - coefficients read from a packed table with compile-time offsets, so the
+      loads are uniform scalar loads through the constant address space and
+      coefficient VALUES stay a run-time input;
+* the structure tables (exponents, non-zero masks, pack order) the host uses
+  to check a map loaded at run time against the compiled structure and to
+  pack its coefficients.
+
+Only the exponent STRUCTURE is compiled in.  A map file with the same terms
+and different coefficient values (a new tune, an optics fit) needs no rebuild.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+NVAR = 5
+
+
+def _read_records(path):
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            yield (line + " " * 132)[:132]
+
+
+def _fval(field):
+    s = field.replace(" ", "")
+    if not s:
+        return 0.0
+    return float(s.replace("D", "E").replace("d", "e"))
+
+
+def parse_forward(path):
+    """Classes of a forward map (shared/transp.f:318-470): list of lists of
+    (coeff[5], expon[5]) with the TOF line dropped; plus per-class drift flag."""
+    recs = list(_read_records(path))
+    i = 0
+    while recs[i][0] == "!":
+        i += 1
+    classes = []
+    while True:
+        terms = []
+        while recs[i][:4] != " ---":
+            s = recs[i]
+            c = [_fval(s[1 + 14 * k: 1 + 14 * (k + 1)]) for k in range(5)]
+            d = [0 if ch == " " else int(ch) for ch in s[72:78]]
+            if d[4] == 0:
+                terms.append((c, [d[0], d[1], d[2], d[3], d[5]]))
+            i += 1
+        classes.append(terms)
+        i += 1
+        while i < len(recs) and (recs[i][0] == "!" or recs[i][:4] == " ---" or not recs[i].strip()):
+            i += 1
+        if i >= len(recs):
+            break
+    return classes
+
+
+def parse_recon(path):
+    recs = list(_read_records(path))
+    i = 0
+    while recs[i][0] == "!":
+        i += 1
+    terms = []
+    while recs[i][:4] != " ---":
+        s = recs[i]
+        c = [_fval(s[1 + 16 * k: 1 + 16 * (k + 1)]) for k in range(4)]
+        e = [0 if ch == " " else int(ch) for ch in s[66:71]]
+        terms.append((c, e))
+        i += 1
+    return terms
+
+
+# Forward classes evaluated through transp(): hms/mc_hms.f:268,276,294,302,320,328,351
+# and shms/mc_shms.f:462,606,617,669,680,732,743,874,893,910,928,945,962,979,999.
+# The other classes are pure drifts, which the arms step with project().
+TRANSP_CLASSES = {
+    "hms": [2, 3, 5, 6, 8, 9, 11],
+    "shms": [3, 7, 8, 12, 13, 17, 18, 23, 24, 25, 26, 27, 28, 29, 30],
+}
+
+
+def _last_var(e):
+    for v in range(NVAR - 1, -1, -1):
+        if e[v]:
+            return v
+    return 0
+
+
+def _parent(e):
+    v = _last_var(e)
+    p = list(e)
+    p[v] -= 1
+    return tuple(p)
+
+
+def emit_poly(fname, terms, nout, inputs, lines):
+    """Emit one device function; returns the pack list [(term_index, out)]."""
+    byexp = {}
+    for ti, (c, e) in enumerate(terms):
+        byexp.setdefault(tuple(e), []).append(ti)
+    root = (0,) * NVAR
+    needed = {root}
+    for e in byexp:
+        while e not in needed:
+            needed.add(e)
+            e = _parent(e)
+    pack = []
+    body = []
+    counter = [0]
+
+    def fmas(node, mname):
+        for ti in byexp.get(node, []):
+            c = terms[ti][0]
+            for k in range(nout):
+                if c[k] != 0.0:
+                    idx = len(pack)
+                    pack.append((ti, k))
+                    if mname is None:
+                        body.append(f"  o{k} += c[{idx}];")
+                    else:
+                        body.append(f"  o{k} = __builtin_fma({mname}, c[{idx}], o{k});")
+
+    def walk(node, mname):
+        fmas(node, mname)
+        lv = _last_var(node) if node != root else 0
+        for v in range(lv, NVAR):
+            ch = list(node)
+            ch[v] += 1
+            ch = tuple(ch)
+            if ch not in needed:
+                continue
+            if mname is None:
+                cname = inputs[v]
+            else:
+                counter[0] += 1
+                cname = f"m{counter[0]}"
+                body.append(f"  const double {cname} = {mname} * {inputs[v]};")
+            walk(ch, cname)
+
+    walk(root, None)
+    args = ", ".join(f"double {x}" for x in inputs)
+    outs = ", ".join(f"double& o{k}" for k in range(nout))
+    lines.append(f"__device__ __forceinline__ void {fname}(const double __attribute__((address_space(4)))* __restrict__ c, {args}, {outs})")
+    lines.append("{")
+    for k in range(nout):
+        lines.append(f"  o{k} = 0.0;")
+    lines.extend(body)
+    lines.append("}")
+    lines.append("")
+    return pack, counter[0]
+
+
+def generate(map_dir, out_path):
+    specs = [
+        ("hms", os.path.join(map_dir, "hms", "forward_cosy.dat"),
+         os.path.join(map_dir, "hms", "recon_cosy.dat")),
+        ("shms", os.path.join(map_dir, "shms", "shms_forward.dat"),
+         os.path.join(map_dir, "shms", "shms_recon.dat")),
+    ]
+    dev, host = [], []
+    dev.append("// Generated by gen_cosy.py from the COSY map files; do not edit.")
+    dev.append("#pragma once")
+    host.append("// Generated by gen_cosy.py from the COSY map files; do not edit.")
+    host.append("#pragma once")
+    host.append("#include <cstdint>")
+    host.append("namespace simc_gen {")
+    host.append("struct PolyInfo { int cls; int n_terms; int nout; int coef_offset; int n_coef;"
+                " const uint8_t (*expon)[5]; const uint16_t (*pack)[2]; int n_mul; };")
+    summary = []
+    for arm, fwd, rec in specs:
+        classes = parse_forward(fwd)
+        recon = parse_recon(rec)
+        offset = 0
+        infos = []
+        for cls in TRANSP_CLASSES[arm]:
+            terms = classes[cls - 1]
+            fname = f"cosy_{arm}_fwd_c{cls}"
+            pack, nmul = emit_poly(fname, terms, 5, ["r0", "r1", "r2", "r3", "r4"], dev)
+            infos.append((cls, terms, 5, offset, pack, nmul))
+            offset += len(pack)
+        fname = f"cosy_{arm}_recon"
+        pack, nmul = emit_poly(fname, recon, 4, ["h0", "h1", "h2", "h3", "h4"], dev)
+        infos.append((0, recon, 4, offset, pack, nmul))
+        offset += len(pack)
+        for cls, terms, nout, off, pack, nmul in infos:
+            tag = f"{arm}_c{cls}" if cls else f"{arm}_recon"
+            host.append(f"static const uint8_t expon_{tag}[{len(terms)}][5] = {{" +
+                        ",".join("{" + ",".join(str(x) for x in e) + "}" for _, e in terms) + "};")
+            host.append(f"static const uint16_t pack_{tag}[{len(pack)}][2] = {{" +
+                        ",".join(f"{{{t},{k}}}" for t, k in pack) + "};")
+            dev.append(f"#define COSY_{tag.upper()}_OFF {off}")
+        host.append(f"static const PolyInfo polys_{arm}[] = {{")
+        for cls, terms, nout, off, pack, nmul in infos:
+            tag = f"{arm}_c{cls}" if cls else f"{arm}_recon"
+            host.append(f"  {{{cls}, {len(terms)}, {nout}, {off}, {len(pack)}, expon_{tag}, pack_{tag}, {nmul}}},")
+        host.append("};")
+        host.append(f"static const int n_polys_{arm} = {len(infos)};")
+        host.append(f"static const int n_coef_{arm} = {offset};")
+        dev.append(f"#define COSY_{arm.upper()}_NCOEF {offset}")
+        summary.append((arm, len(classes), len(infos), offset))
+    host.append("}  // namespace simc_gen")
+    with open(out_path + ".dev.inc", "w") as f:
+        f.write("\n".join(dev) + "\n")
+    with open(out_path + ".host.inc", "w") as f:
+        f.write("\n".join(host) + "\n")
+    return summary
+
+
+if __name__ == "__main__":
+    print(generate(sys.argv[1], sys.argv[2]))

@@ -1,0 +1,1168 @@
+// Spectrometer transport + reconstruction kernels (HMS, SHMS) and the C ABI
+// of libsimc_gpu.so (include/simc_gpu.h).
+//
+// Layout: events are structure-of-arrays in HBM, one event per lane; a lane
+// carries its event from the target through the magnets, the detector hut
+// and the track fit to the reconstructed target quantities.  The COSY
+// polynomials are generated straight-line code (cosy_gen.dev.inc) whose
+// coefficients come from a packed per-arm table read with uniform (scalar)
+// loads.  Stop codes are histogrammed per workgroup in LDS and folded into
+// the context's device counters with one atomic per non-empty bin.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/simc_gpu.h"
+#include "simc_device.h"
+#include "simc_maps.h"
+#include "simc_ops.h"
+#include "cosy_gen.dev.inc"
+
+using namespace simcdev;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Stop codes (one per exit point; 0 = accepted)
+// ---------------------------------------------------------------------------
+enum : int {
+    ST_OK = 0, ST_SLIT_HOR = 1, ST_SLIT_VERT = 2, ST_SLIT_OCT = 3, ST_COLL = 4,
+    // HMS
+    ST_H_Q1_IN = 10, ST_H_Q1_MID, ST_H_Q1_OUT, ST_H_Q2_IN, ST_H_Q2_MID, ST_H_Q2_OUT,
+    ST_H_Q3_IN, ST_H_Q3_MID, ST_H_Q3_OUT, ST_H_D1_IN, ST_H_D1_OUT, ST_H_PIPE0, ST_H_PIPE1,
+    ST_H_PIPE2, ST_H_PIPE3, ST_H_DC1, ST_H_DC2, ST_H_SCIN,
+    // SHMS
+    ST_S_HB_IN = 40, ST_S_HB_MEN, ST_S_HB_MEX, ST_S_HB_OUT, ST_S_Q1_IN, ST_S_Q1_MEN,
+    ST_S_Q1_MID, ST_S_Q1_MEX, ST_S_Q1_OUT, ST_S_Q2_IN, ST_S_Q2_MEN, ST_S_Q2_MID, ST_S_Q2_MEX,
+    ST_S_Q2_OUT, ST_S_Q3_IN, ST_S_Q3_MEN, ST_S_Q3_MID, ST_S_Q3_MEX, ST_S_Q3_OUT, ST_S_D1_IN,
+    ST_S_D1_FLR, ST_S_D1_MEN, ST_S_D1_MID1, ST_S_D1_MID2, ST_S_D1_MID3, ST_S_D1_MID4,
+    ST_S_D1_MID5, ST_S_D1_MID6, ST_S_D1_MID7, ST_S_D1_MEX, ST_S_D1_OUT, ST_S_DC1, ST_S_DC2,
+    ST_S_S1X, ST_S_S1Y, ST_S_S2X, ST_S_S2Y, ST_S_CAL, ST_S_CALFID,
+    ST_TAPE = 250
+};
+constexpr int kHistBins = 256;
+constexpr int kBlock = 256;  // threads per workgroup (one event each)
+constexpr int kHistSlots = kHistBins + 3;  // + per-step collimator slit hits (hor, vert, oct)
+
+// Per-arm tables owned by the context (device memory, read through the
+// constant address space).
+struct ArmTables {
+    CollGeom coll;     // collimator stepping geometry
+    double zdc[12];    // wire-plane z positions, planes 1..12
+};
+
+// Per-launch arguments.  Array pointers address the first event of the
+// launch; all event arrays are reached through buffer resources with one
+// 32-bit byte offset per lane, so no 64-bit per-lane addresses are kept.
+struct KArgs {
+    double p_spec, m2, ctau, sigma, wire_sigma;
+    int32_t flags;        // KF_* bits
+    uint32_t count;       // events in this launch
+    uint64_t seed, event0;
+    const ArmTables* tab;
+    const double* coef;
+    const Op* ops;
+    const double *dpp, *x, *y, *dxdz, *dydz, *fry;
+    double *o_dpp, *o_y, *o_dxdz, *o_dydz, *x_fp, *dx_fp, *y_fp, *dy_fp, *pathlen, *resmult, *m2o;
+    uint8_t *ok, *stop;
+    int32_t* used;
+    const double* tape;
+    uint32_t stride_bytes;
+    unsigned long long* hist;
+};
+enum : int32_t { KF_MS = 1, KF_WCS = 2, KF_DECAY = 4, KF_COLL = 8, KF_FIT_F32 = 16 };
+
+typedef const double __attribute__((address_space(4)))* cdptr;
+typedef const Op __attribute__((address_space(4)))* coptr;
+typedef const ArmTables __attribute__((address_space(4)))* ctptr;
+
+// Results of one event besides the track.
+struct EvOut {
+    double x_fp, dx_fp, y_fp, dy_fp;
+    double dpp, dxdz, dydz, y;
+    double resmult;
+    bool fit_done;
+};
+
+// COSY forward class `cls` of the arm (generated straight-line code).
+template <int ARM>
+__device__ __forceinline__ void poly_fwd(int cls, cdptr c, double r0, double r1, double r2, double r3,
+                                         double r4, double& o0, double& o1, double& o2, double& o3,
+                                         double& o4)
+{
+    if (ARM == SIMC_ARM_HMS) {
+        switch (cls) {
+            case 2: cosy_hms_fwd_c2(c + COSY_HMS_C2_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            case 3: cosy_hms_fwd_c3(c + COSY_HMS_C3_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            case 5: cosy_hms_fwd_c5(c + COSY_HMS_C5_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            case 6: cosy_hms_fwd_c6(c + COSY_HMS_C6_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            case 8: cosy_hms_fwd_c8(c + COSY_HMS_C8_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            case 9: cosy_hms_fwd_c9(c + COSY_HMS_C9_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+            default: cosy_hms_fwd_c11(c + COSY_HMS_C11_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        }
+    } else {
+#define SFWD(N) case N: cosy_shms_fwd_c##N(c + COSY_SHMS_C##N##_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        switch (cls) {
+            SFWD(3) SFWD(7) SFWD(8) SFWD(12) SFWD(13) SFWD(17) SFWD(18) SFWD(23) SFWD(24) SFWD(25)
+            SFWD(26) SFWD(27) SFWD(28) SFWD(29)
+            default: cosy_shms_fwd_c30(c + COSY_SHMS_C30_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        }
+#undef SFWD
+    }
+}
+
+template <int ARM>
+__device__ __forceinline__ void poly_recon(cdptr c, double h0, double h1, double h2, double h3, double h4,
+                                           double& s0, double& s1, double& s2, double& s3)
+{
+    if (ARM == SIMC_ARM_HMS) cosy_hms_recon(c + COSY_HMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
+    else cosy_shms_recon(c + COSY_SHMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
+}
+
+// Reconstruction (hms/mc_hms_recon.f:113-137, shms/mc_shms_recon.f:131-160)
+template <int ARM>
+__device__ __forceinline__ void recon(cdptr c, const EvOut& o, double fry, EvOut& r)
+{
+    double h4 = fry / 100.0;
+    if (fabs(h4) <= 1.e-30) h4 = 1.e-30;
+    double s0, s1, s2, s3;
+    poly_recon<ARM>(c, o.x_fp / 100.0, o.dx_fp, o.y_fp / 100.0, o.dy_fp, h4, s0, s1, s2, s3);
+    r.dxdz = s0;         // delta_phi
+    r.y = s1 * 100.0;    // y_tgt
+    r.dydz = s2;         // delta_t
+    r.dpp = s3 * 100.0;  // delta_p
+}
+
+// hit_dipole, hms/mc_hms.f:445-492 with hms/apertures_hms.inc:20-32
+__device__ __forceinline__ bool hms_hit_dipole(double x, double y)
+{
+    const double x_d1 = 34.29, y_d1 = 12.07, x_d2 = 27.94, y_d2 = 18.42, x_d3 = 13.97,
+                 y_d3 = 18.95, x_d4 = 1.956, y_d4 = 20.32, x_d5 = 27.94, y_d5 = 12.065,
+                 r_d5 = 6.35, a_d6 = -0.114, b_d6 = 20.54;
+    const double xl = fabs(x), yl = fabs(y);
+    const bool c1 = (xl <= x_d1) && (yl <= y_d1);
+    const bool c2 = (xl <= x_d2) && (yl <= y_d2);
+    const bool c3 = (xl <= x_d3) && (yl <= y_d3);
+    const bool c4 = (xl <= x_d4) && (yl <= y_d4);
+    const bool c5 = ((xl - x_d5) * (xl - x_d5) + (yl - y_d5) * (yl - y_d5)) <= r_d5 * r_d5;
+    const bool c6 = (xl >= x_d4) && (xl <= x_d3) && ((yl - a_d6 * xl - b_d6) <= 0.0);
+    return !(c1 || c2 || c3 || c4 || c5 || c6);
+}
+
+// Track fit of the 12 chamber planes (hms/mc_hms_hut.f:439-455,
+// shms/mc_shms_hut.f:410-426) with LFIT (cern/lfit.f:22-59, KEY = 0).
+// Plane k+1 measures x on k = 0,2,3,5,6,8,9,11 and y on k = 1,4,7,10; the
+// reference stores the other coordinate as zero, which LFIT's zero test
+// skips, so each fit runs over its own planes in increasing order.  Hits are
+// re-read from LDS in the second pass instead of being held in registers.
+// T is float for the reference's REAL*4 hits and results (sums in FP64).
+template <class T, int NP>
+__device__ __forceinline__ void lfit_planes(const int (&pl)[NP], const double* lds, const double (&zdc)[12],
+                                            T& A, T& B)
+{
+    double count = 0.0, sumx = 0.0, sumy = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const T y = (T)lds[pl[j] * kBlock + threadIdx.x];
+        const T x = (T)zdc[pl[j]];
+        if (y != (T)0) {
+            sumx = sumx + (double)x;
+            sumy = sumy + (double)y;
+            count = count + 1.0;
+        }
+    }
+    if (count <= 1.0) { A = (T)0; B = (T)0; return; }
+    const double ymed = sumy / count;
+    const double xmed = sumx / count;
+    double sumxy = 0.0, sumxx = 0.0;
+    __asm__ volatile("" ::: "memory");  // second pass reloads the hits
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const T y = (T)lds[pl[j] * kBlock + threadIdx.x];
+        const T x = (T)zdc[pl[j]];
+        if (y != (T)0) {
+            const double sx = (double)x - xmed;
+            const double sy = (double)y - ymed;
+            sumxy = sumxy + sx * sy;
+            sumxx = sumxx + sx * sx;
+        }
+    }
+    if (sumxx == 0.0) { A = (T)0; B = (T)0; return; }
+    A = (T)(sumxy / sumxx);
+    B = (T)(ymed - (double)A * xmed);
+}
+
+__device__ __forceinline__ void fit_track(const Flags& F, const double* lds, const double (&zdc)[12],
+                                          double& x_fp, double& dx_fp, double& y_fp, double& dy_fp)
+{
+    const int xpl[8] = {0, 2, 3, 5, 6, 8, 9, 11};
+    const int ypl[4] = {1, 4, 7, 10};
+    if (F.fit_f32) {
+        float a, b;
+        lfit_planes<float, 8>(xpl, lds, zdc, a, b);
+        dx_fp = (double)a; x_fp = (double)b;
+        lfit_planes<float, 4>(ypl, lds, zdc, a, b);
+        dy_fp = (double)a; y_fp = (double)b;
+    } else {
+        double a, b;
+        lfit_planes<double, 8>(xpl, lds, zdc, a, b);
+        dx_fp = a; x_fp = b;
+        lfit_planes<double, 4>(ypl, lds, zdc, a, b);
+        dy_fp = a; y_fp = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The step-program interpreter: one event of arm ARM.
+//
+// Only the track, the random stream, the tilted-plane point (xt, yt), resmult
+// and the scintillator count are carried from step to step.  Wire hits go to
+// LDS (hits[plane][lane]); the fitted focal-plane track is written to its
+// output arrays at the fit and read back where it is used again.
+// ---------------------------------------------------------------------------
+template <int ARM, class R, bool RARE>
+__device__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off, double* lds_hits,
+                           unsigned* coll_cnt, double& resmult)
+{
+    // The in-flight decay and the collimator stepping of pions/muons are
+    // compiled only into the RARE variant, so the common variant does not
+    // pay their register cost.
+    Flags F = FA;
+    if (!RARE) {
+        F.decay = 0;
+        F.coll = 0;
+    }
+    const cdptr coef = (cdptr)a.coef;
+    const coptr ops = (coptr)a.ops;
+    const ctptr tab = (ctptr)a.tab;
+    double xt = 0.0, yt = 0.0;
+    int scincount = 0;
+    int st = ST_OK;
+    for (int pc = 0;; ++pc) {
+        const int code = ops[pc].code;
+        if (code == OP_END) break;
+        const double p0 = ops[pc].p[0], p1 = ops[pc].p[1];
+        // Gaussian draws of this step, in call order (one gauss1 instance
+        // serves musc, musc_ext and the wire smearing).
+        int ng = 0;
+        if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
+        else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
+        else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+        for (int q = 0; q < ng; ++q) {
+            const double v = gauss1(g, 99.0);
+            d0 = d1; d1 = d2; d2 = d3; d3 = v;
+        }
+        switch (code) {
+        case OP_PROJECT:
+            project(g, t, F, p0);
+            break;
+        case OP_TRANSP: {
+            const int cls = ops[pc].arg;
+            transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
+                                    double& o1, double& o2, double& o3, double& o4) {
+                poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
+            });
+            break;
+        }
+        case OP_CHECK_R2:
+            if ((t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
+            break;
+        case OP_ROT_H:
+        case OP_ROT_V: {
+            const Rot r{p0, p1, ops[pc].p[2]};
+            xt = t.xs;
+            yt = t.ys;
+            if (code == OP_ROT_H) rotate_haxis(t, r, xt, yt);
+            else rotate_vaxis(t, r, xt, yt);
+            xt = xt + ops[pc].p[3];
+            yt = yt + ops[pc].p[4];
+            break;
+        }
+        case OP_CHECK_T_R2:
+            if ((xt * xt + yt * yt) > p0) st = ops[pc].stop;
+            break;
+        case OP_CHECK_T_HB:
+            if ((xt * xt > p0) || (yt > p1) || (yt < ops[pc].p[2])) st = ops[pc].stop;
+            break;
+        case OP_HIT_DIPOLE_T:
+            if (hms_hit_dipole(xt, yt)) st = ops[pc].stop;
+            break;
+        case OP_PIPE_T: {
+            const double dx = xt - p1, dy = yt - ops[pc].p[2];
+            if (((dx * dx + dy * dy) > p0) || (fabs(dy) > ops[pc].p[3])) st = ops[pc].stop;
+            break;
+        }
+        case OP_PIPE_S: {
+            const double dx = t.xs - p1, dy = t.ys - ops[pc].p[2];
+            if ((dx * dx + dy * dy) > p0) st = ops[pc].stop;
+            break;
+        }
+        case OP_COLLIMATOR:
+            if (F.coll && t.m2 > 100.0 * 100.0 && t.m2 < 200.0 * 200.0) {
+                CollGeom cg;
+                cg.h_entr = tab->coll.h_entr; cg.v_entr = tab->coll.v_entr;
+                cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
+                cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
+                cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
+                if (!coll_step(g, t, F, cg, a.p_spec, coll_cnt)) st = ST_COLL;
+            } else {
+                const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
+                int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
+                if (!s) {
+                    project(g, t, F, ops[pc].p[6]);
+                    s = slit(t.xs, t.ys, ops[pc].p[2], ops[pc].p[3], x_off, y_off);
+                }
+                st = s;
+            }
+            break;
+        case OP_RESMULT_DRAW:
+            resmult = g.next() < p0 ? 2.0 : 1.0;
+            break;
+        case OP_RESMULT_ONE:
+            resmult = 1.0;
+            break;
+        case OP_MUSC:
+            if (ng) {  // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
+                const double ts = theta_sigma(t.m2, t.p, p0);
+                t.dydzs = t.dydzs + ts * d2;
+                t.dxdzs = t.dxdzs + ts * d3;
+            }
+            break;
+        case OP_MUSC_EXT:
+            if (ng) {  // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
+                const double ts = theta_sigma(t.m2, t.p, p0);
+                const double s12 = 3.4641016151377544;  // sqrt(12.)
+                t.dxdzs = t.dxdzs + ts * d0;
+                t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
+                t.dydzs = t.dydzs + ts * d2;
+                t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
+            }
+            break;
+        case OP_WIRE: {  // plane ops[pc].stop (0-based); y plane when arg = 1
+            const double h = ops[pc].arg ? t.ys + a.sigma * d3 * resmult : t.xs + a.sigma * d2 * resmult;
+            lds_hits[ops[pc].stop * kBlock + threadIdx.x] = h;
+            break;
+        }
+        case OP_BOX:
+            if (t.xs > p0 || t.xs < p1 || t.ys > ops[pc].p[2] || t.ys < ops[pc].p[3]) st = ops[pc].stop;
+            break;
+        case OP_SCIN:
+            if (t.ys < p0 && t.ys > p1 && t.xs < ops[pc].p[2] && t.xs > ops[pc].p[3]) ++scincount;
+            break;
+        case OP_TRIGGER:
+            if (scincount < ops[pc].arg) st = ops[pc].stop;
+            break;
+        case OP_FIT: {
+            double zdc[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
+            double x_fp, dx_fp, y_fp, dy_fp;
+            fit_track(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
+            st64(a.x_fp, a.count, off, x_fp);
+            st64(a.dx_fp, a.count, off, dx_fp);
+            st64(a.y_fp, a.count, off, y_fp);
+            st64(a.dy_fp, a.count, off, dy_fp);
+            break;
+        }
+        case OP_FIDUCIAL: {
+            const double xcal = ld64(a.x_fp, a.count, off) + ld64(a.dx_fp, a.count, off) * p0;
+            const double ycal = ld64(a.y_fp, a.count, off) + ld64(a.dy_fp, a.count, off) * p0;
+            if (ycal > p1 || ycal < ops[pc].p[2] || xcal > ops[pc].p[3] || xcal < ops[pc].p[4]) st = ops[pc].stop;
+            break;
+        }
+        default:
+            break;
+        }
+        if (st) return st;
+    }
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel: one event per lane; LDS stop-code histogram per workgroup.
+// ---------------------------------------------------------------------------
+template <int ARM, class R, bool RARE>
+__global__ __launch_bounds__(kBlock) void arm_kernel(KArgs a)
+{
+    __shared__ unsigned hist[kHistSlots];
+    __shared__ double lds_hits[12 * kBlock];
+    for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    unsigned coll_cnt[3] = {0, 0, 0};
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    if (li < a.count) {
+        const uint32_t off = li * 8u;
+        const uint32_t n = a.count;
+        Flags F;
+        F.ctau = a.ctau;
+        F.ms = (a.flags & KF_MS) != 0;
+        F.wcs = (a.flags & KF_WCS) != 0;
+        F.decay = (a.flags & KF_DECAY) != 0;
+        F.coll = (a.flags & KF_COLL) != 0;
+        F.fit_f32 = (a.flags & KF_FIT_F32) != 0;
+        F.wire_sigma = a.wire_sigma;
+        R g;
+        g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
+        Track t;
+        t.xs = ld64(a.x, n, off);
+        t.ys = ld64(a.y, n, off);
+        t.dxdzs = ld64(a.dxdz, n, off);
+        t.dydzs = ld64(a.dydz, n, off);
+        t.dpps = ld64(a.dpp, n, off);
+        t.p = a.p_spec * (1.0 + t.dpps / 100.0);
+        t.m2 = a.m2;
+        t.pathlen = ld64(a.pathlen, n, off);
+        t.decdist = 0.0;
+        t.dflag = 0;
+        double resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
+        int st = run_program<ARM, R, RARE>(g, t, a, F, off, lds_hits, coll_cnt, resmult);
+        if (g.overflow) st = ST_TAPE;
+        if (st == ST_OK) {
+            EvOut o, r;
+            o.x_fp = ld64(a.x_fp, n, off);
+            o.dx_fp = ld64(a.dx_fp, n, off);
+            o.y_fp = ld64(a.y_fp, n, off);
+            o.dy_fp = ld64(a.dy_fp, n, off);
+            recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
+            st64(a.o_dpp, n, off, r.dpp);
+            st64(a.o_y, n, off, r.y);
+            st64(a.o_dxdz, n, off, r.dxdz);
+            st64(a.o_dydz, n, off, r.dydz);
+        } else {
+            // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440)
+            st64(a.o_dpp, n, off, ld64(a.dpp, n, off));
+            st64(a.o_y, n, off, ld64(a.y, n, off));
+            st64(a.o_dxdz, n, off, ld64(a.dxdz, n, off));
+            st64(a.o_dydz, n, off, ld64(a.dydz, n, off));
+        }
+        st64(a.pathlen, n, off, t.pathlen);
+        if (a.resmult) st64(a.resmult, n, off, resmult);
+        if (a.m2o) st64(a.m2o, n, off, t.m2);
+        if (a.used) st32(a.used, n, li * 4u, g.used());
+        st8(a.ok, n, li, st == ST_OK ? 1 : 0);
+        st8(a.stop, n, li, (uint8_t)st);
+        atomicAdd(&hist[st], 1u);
+    }
+    if (coll_cnt[0]) atomicAdd(&hist[kHistBins + 0], coll_cnt[0]);
+    if (coll_cnt[1]) atomicAdd(&hist[kHistBins + 1], coll_cnt[1]);
+    if (coll_cnt[2]) atomicAdd(&hist[kHistBins + 2], coll_cnt[2]);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x)
+        if (hist[k]) atomicAdd(&a.hist[k], (unsigned long long)hist[k]);
+}
+
+template <int ARM>
+__global__ __launch_bounds__(256) void recon_kernel(int64_t n, const double* coef, const double* x_fp,
+                                                     const double* dx_fp, const double* y_fp,
+                                                     const double* dy_fp, const double* fry, double* delta,
+                                                     double* dxdz, double* dydz, double* ytar)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        EvOut o, r;
+        o.x_fp = x_fp[i]; o.dx_fp = dx_fp[i]; o.y_fp = y_fp[i]; o.dy_fp = dy_fp[i];
+        recon<ARM>((cdptr)coef, o, fry ? fry[i] : 0.0, r);
+        delta[i] = r.dpp; dxdz[i] = r.dxdz; dydz[i] = r.dydz; ytar[i] = r.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(SIMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct simc_ctx {
+    simc::ForwardMap fwd[2];
+    simc::ReconMap rec[2];
+    double* d_coef[2] = {nullptr, nullptr};
+    Op* d_ops[2] = {nullptr, nullptr};
+    ArmTables* d_tab[2] = {nullptr, nullptr};
+    std::vector<Op> ops[2];
+    int n_ops[2] = {0, 0};
+    double zdc[2][12];
+    unsigned long long* d_hist[2] = {nullptr, nullptr};
+    int device = 0;
+    int n_cu = 256;
+};
+
+namespace {
+
+int arm_index(int arm) { return arm == SIMC_ARM_HMS ? 0 : arm == SIMC_ARM_SHMS ? 1 : -1; }
+
+// Step-program builder.  All parameters are formed with the reference's own
+// expressions in double precision, so they equal the reference's values.
+struct Program {
+    std::vector<Op> ops;
+    Op& add(int code, std::initializer_list<double> p = {}, int stop = 0, int arg = 0)
+    {
+        Op o;
+        std::memset(&o, 0, sizeof o);
+        o.code = code;
+        o.stop = stop;
+        o.arg = arg;
+        int k = 0;
+        for (double v : p) o.p[k++] = v;
+        ops.push_back(o);
+        return ops.back();
+    }
+    // rotate_haxis / rotate_vaxis of the track into a tilted aperture plane
+    // (shared/rotate_haxis.f:163-173: truncated degree constant, libm trig of
+    // the product), then the aperture-centre offsets.
+    void rot(int code, double rotang, double xoff, double yoff)
+    {
+        const double raddeg = 0.017453292;
+        const double r = rotang * raddeg;
+        add(code, {std::tan(r), std::sin(r), std::cos(r), xoff, yoff});
+    }
+};
+
+const double kInf = HUGE_VAL;
+
+// Wire-chamber planes of one chamber: hms/mc_hms_hut.f:341-370 (the SHMS loop,
+// shms/mc_shms_hut.f:172-205, is the same with its own constants).
+void chamber(Program& P, int jchamber, double cath_radw, double hdc_thick, double hdc_radlen,
+             double cath_thick, double wire_radw, double wire_thick)
+{
+    for (int iplane = 1; iplane <= 6; ++iplane) {
+        P.add(OP_MUSC, {cath_radw});
+        double drift = 0.5 * hdc_thick;
+        P.add(OP_MUSC, {drift / hdc_radlen});
+        drift = drift + cath_thick;
+        P.add(OP_PROJECT, {drift});
+        P.add(OP_MUSC, {wire_radw});
+        P.add(OP_WIRE, {}, (jchamber - 1) * 6 + iplane - 1, (iplane == 2 || iplane == 5) ? 1 : 0);
+        drift = 0.5 * hdc_thick;
+        P.add(OP_MUSC, {drift / hdc_radlen});
+        drift = drift + wire_thick;
+        P.add(OP_PROJECT, {drift});
+    }
+}
+
+// mc_hms (hms/mc_hms.f:150-441) followed by mc_hms_hut (hms/mc_hms_hut.f:292-601)
+void build_hms(const simc::ForwardMap& f, Program& P, double zdc[12])
+{
+    const double z_off = +40.17, z_entr = 126.2e0 + z_off, z_exit = z_entr + 6.3e0;  // :84-89
+    const double z_dip1 = 64.77e0, z_dip2 = z_dip1 + 297.18e0, z_dip3 = z_dip2 + 115.57e0;  // :92-94
+    const double rq1 = 20.50 * 20.50, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;  // apertures_hms.inc:5-7
+    const double* dd = f.driftdist.data();  // driftdist(spectr, k) = dd[k-1]
+    P.add(OP_PROJECT, {z_entr});                                                        // :201-202
+    P.add(OP_COLLIMATOR, {4.575, 11.646, 4.759, 12.114, 0.000, +0.028, z_exit - z_entr});  // :206-254
+    P.add(OP_PROJECT, {dd[0] - z_exit});                                                // :259-260
+    P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_IN);
+    P.add(OP_TRANSP, {125.233e0}, 0, 2);  P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_MID);    // :268-272
+    P.add(OP_TRANSP, {62.617e0}, 0, 3);   P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_OUT);    // :276-280
+    P.add(OP_PROJECT, {dd[3]});           P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_IN);     // :285-290
+    P.add(OP_TRANSP, {143.90e0}, 0, 5);   P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_MID);
+    P.add(OP_TRANSP, {71.95e0}, 0, 6);    P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_OUT);
+    P.add(OP_PROJECT, {dd[6]});           P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_IN);     // :311-316
+    P.add(OP_TRANSP, {143.8e0}, 0, 8);    P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_MID);
+    P.add(OP_TRANSP, {71.9e0}, 0, 9);     P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_OUT);
+    P.add(OP_PROJECT, {dd[9]});                                                         // :338-339
+    P.rot(OP_ROT_H, -6.0e0, 0.0, 0.0);    P.add(OP_HIT_DIPOLE_T, {}, ST_H_D1_IN);    // :340-346
+    P.add(OP_TRANSP, {526.053e0}, 0, 11);                                               // :351
+    P.rot(OP_ROT_H, 6.0e0, 0.0, 0.0);     P.add(OP_HIT_DIPOLE_T, {}, ST_H_D1_OUT);   // :352-358
+    P.add(OP_PIPE_T, {30.48 * 30.48, 2.8, 0.0, 20.5232}, ST_H_PIPE0);                  // :363-367
+    P.add(OP_PROJECT, {z_dip1});           P.add(OP_PIPE_S, {1145.518, 2.8, 0.0}, ST_H_PIPE1);
+    P.add(OP_PROJECT, {z_dip2 - z_dip1});  P.add(OP_PIPE_S, {1512.2299, 2.8, 0.0}, ST_H_PIPE2);
+    P.add(OP_PROJECT, {z_dip3 - z_dip2});  P.add(OP_PIPE_S, {2162.9383, 2.8, 0.0}, ST_H_PIPE3);
+
+    // detector hut, hms/mc_hms_hut.f (constants :57-232)
+    const double zinit = -(dd[11] - z_dip3);  // mc_hms.f:413-415
+    const double hfoil_exit_radlen = 8.90, hfoil_exit_thick = 0.011 * 2.54, hair_radlen = 30420.;
+    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 1.8;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
+    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
+    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
+    const double haer_entr_radlen = 8.90, haer_entr_thick = 0.15, haer_radlen = 150.0, haer_thick = 9.0;
+    const double haer_air_radlen = 30420.0, haer_air_thick = 16.0, haer_exit_radlen = 8.90, haer_exit_thick = 0.1;
+    const double hscin_radlen = 42.4, hscin_thick = 1.067;
+    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54, hcer_radlen = 9620.0;
+    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
+    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_plan = 6;
+    const double hdc_1_zpos = -52.1084, hdc_2_zpos = 29.2608;
+    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
+    const double haer_zentrance = 35.699, haer_zexit = 60.949;
+    const double hscin_1x_zpos = 77.830, hscin_1y_zpos = 97.520, hscin_2x_zpos = 298.820, hscin_2y_zpos = 318.510;
+    const double hcer_zentrance = 110.000, hcer_zmirror = 230.000, hcer_zexit = 265.000;
+    const double hcal_4ta_zpos = 371.69;
+    // scintillator box of :487-490 (reused for the second hodoscope, :537-551)
+    const double scin[4] = {37.75 + -1.3, -37.75 + -1.3, 60.25 + -1.3, -60.25 + -1.3};
+    double drift;
+
+    P.add(OP_RESMULT_DRAW, {0.15});                                               // :292-298
+    P.add(OP_PROJECT, {(hdc_1_zpos - 25.000) - zinit});                           // :322-323
+    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});
+    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos - 25.000);
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 1, :337-381
+    chamber(P, 1, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
+            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
+    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
+    P.add(OP_BOX, {56.5 - 1.670, -56.5 - 1.670, 26.0 - 1.443, -26.0 - 1.443}, ST_H_DC1);
+    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
+    drift = (hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos + 0.5 * hdc_nr_plan * hdc_del_plane);
+    P.add(OP_PROJECT, {drift});                                                   // :385-389
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 2, :391-436
+    chamber(P, 2, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
+            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
+    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
+    P.add(OP_BOX, {56.5 - 2.758, -56.5 - 2.758, 26.0 - 2.753, -26.0 - 2.753}, ST_H_DC2);
+    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
+    for (int i = 0; i < 6; ++i) {                                                 // :439-447
+        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+    }
+    P.add(OP_FIT);                                                                // :449-455
+    drift = haer_zentrance - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;      // :459-462
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {haer_entr_thick / haer_entr_radlen});
+    P.add(OP_PROJECT, {haer_thick});
+    P.add(OP_MUSC_EXT, {haer_thick / haer_radlen, haer_thick});
+    P.add(OP_PROJECT, {haer_air_thick});
+    P.add(OP_MUSC_EXT, {haer_air_thick / haer_air_radlen, haer_air_thick});
+    P.add(OP_MUSC, {haer_exit_thick / haer_exit_radlen});
+    drift = hscin_1x_zpos - haer_zexit;                                           // :482-492
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hscin_1y_zpos - hscin_1x_zpos;                                        // :494-503
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hcer_zentrance - hscin_1y_zpos;                                       // :507-513
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
+    drift = hcer_zmirror - hcer_zentrance;                                        // :515-521
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hcer_radlen, drift});
+    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
+    drift = hcer_zexit - hcer_zmirror;                                            // :523-529
+    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
+    drift = hscin_2x_zpos - hcer_zexit;                                           // :533-542
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hscin_2y_zpos - hscin_2x_zpos;                                        // :544-553
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    P.add(OP_TRIGGER, {}, ST_H_SCIN, 3);                                          // :556-559
+    drift = hcal_4ta_zpos - hscin_2y_zpos;                                        // :563-566
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_END);
+}
+
+// mc_shms (shms/mc_shms.f:255-1106) followed by mc_shms_hut (shms/mc_shms_hut.f:74-448)
+void build_shms(Program& P, double zdc[12])
+{
+    const double zd_hbin = 118.39, zd_hbmen = 17.61, zd_hbmex = 80.0, zd_hbout = 17.61;  // :176-179
+    const double z_entr = 25.189, z_thick = 6.35;                                        // :181-182
+    const double zd_q1in = 58.39, zd_q1men = 28.35, zd_q1mid = 93.65, zd_q1mex = 93.65, zd_q1out = 28.35;
+    const double zd_q2in = 25.55, zd_q2men = 39.1, zd_q2mid = 79.35, zd_q2mex = 79.35, zd_q2out = 39.1;
+    const double zd_q3in = 28.10, zd_q3men = 39.1, zd_q3mid = 79.35, zd_q3mex = 79.35, zd_q3out = 39.1;
+    const double zd_q3d1trans = 18.00, zd_d1flare = 30.10, zd_d1men = 39.47, zd_d1mid = 36.406263;
+    const double zd_d1out = 60.68, zd_fp = 307.95;
+    // shms/apertures_shms.inc:240-259
+    const double rhb = 11.2 * 11.2, rq1 = 20.00 * 20.00, rq2 = 30.00 * 30.00, rq3 = 30.00 * 30.00,
+                 rd1 = 30.00 * 30.00;
+    P.add(OP_PROJECT, {zd_hbin});                                                 // :422-423
+    P.rot(OP_ROT_V, 1.5, 0.0, 1.51);
+    P.add(OP_CHECK_T_HB, {rhb, 11.75, -4.13}, ST_S_HB_IN);
+    P.add(OP_PROJECT, {zd_hbmen});                                                // :441-455
+    P.rot(OP_ROT_V, 1.5, 0.0, 0.98);
+    P.add(OP_CHECK_T_HB, {rhb, 11.74, -5.45}, ST_S_HB_MEN);
+    P.add(OP_TRANSP, {zd_hbmex}, 0, 3);                                           // :462-476
+    P.rot(OP_ROT_V, -1.5, 0.0, 0.98);
+    P.add(OP_CHECK_T_HB, {rhb, 11.71, -10.25}, ST_S_HB_MEX);
+    P.add(OP_PROJECT, {zd_hbout});                                                // :479-495
+    P.rot(OP_ROT_V, -1.5, 0.0, 1.51);
+    P.add(OP_CHECK_T_HB, {rhb, 11.70, -11.71}, ST_S_HB_OUT);
+    P.add(OP_PROJECT, {z_entr});                                                  // :505-506, 527-528
+    P.add(OP_COLLIMATOR, {8.5, 12.5, 8.65, 12.85, 0.00, 0.00, z_thick});          // :500-568
+    P.add(OP_PROJECT, {zd_q1in - z_entr - z_thick});                              // :573-585
+    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_IN);
+    P.add(OP_PROJECT, {zd_q1men});         P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MEN);
+    P.add(OP_TRANSP, {zd_q1mid}, 0, 7);    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MID);
+    P.add(OP_TRANSP, {zd_q1mex}, 0, 8);    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MEX);
+    P.add(OP_PROJECT, {zd_q1out});         P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_OUT);
+    P.add(OP_PROJECT, {zd_q2in});          P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_IN);   // :644-712
+    P.add(OP_PROJECT, {zd_q2men});         P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MEN);
+    P.add(OP_TRANSP, {zd_q2mid}, 0, 12);   P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MID);
+    P.add(OP_TRANSP, {zd_q2mex}, 0, 13);   P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MEX);
+    P.add(OP_PROJECT, {zd_q2out});         P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_OUT);
+    P.add(OP_PROJECT, {zd_q3in});          P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_IN);   // :714-781
+    P.add(OP_PROJECT, {zd_q3men});         P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MEN);
+    P.add(OP_TRANSP, {zd_q3mid}, 0, 17);   P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MID);
+    P.add(OP_TRANSP, {zd_q3mex}, 0, 18);   P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MEX);
+    P.add(OP_PROJECT, {zd_q3out});         P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_OUT);
+    P.add(OP_PROJECT, {zd_q3d1trans});     P.add(OP_CHECK_R2, {rd1}, ST_S_D1_IN);   // :815-827
+    P.add(OP_PROJECT, {zd_d1flare});                                              // :832-846
+    P.rot(OP_ROT_H, 9.200, -3.5, 0.0);     P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_FLR);
+    P.add(OP_PROJECT, {zd_d1men});                                                // :851-865
+    P.rot(OP_ROT_H, 9.200, 2.82, 0.0);     P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_MEN);
+    const double ang[8] = {6.9, 4.6, 2.3, 0.0, -2.3, -4.6, -6.9, -9.2};          // :871-1011
+    const double off[8] = {8.05, 11.75, 13.96, 14.70, 13.96, 11.75, 8.05, 2.82};
+    for (int k = 0; k < 8; ++k) {
+        P.add(OP_TRANSP, {zd_d1mid}, 0, 23 + k);
+        P.rot(OP_ROT_H, ang[k], off[k], 0.0);
+        P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_MID1 + k);
+    }
+    P.add(OP_PROJECT, {zd_d1out});                                                // :1015-1031
+    P.rot(OP_ROT_H, -9.20, -6.88, 0.0);    P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_OUT);
+    const double hcer_1_zentrance = -291.700, hcer_1_zmirror = -84.900, hcer_1_zexit = -61.700;
+    P.add(OP_PROJECT, {zd_fp + hcer_1_zentrance});                                // :1036-1047
+
+    // detector hut, shms/mc_shms_hut.f with shms/hut.inc
+    const double hfoil_exit_radlen = 8.89, hfoil_exit_thick = 0.020 * 2.54, hair_radlen = 30420.;
+    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 0.125 * 2.54;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000354;
+    const double hdc_cath_radlen = 28.6, hdc_cath_thick = 0.001 * 2.54;
+    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
+    const double hscin_radlen = 42.4, hscin_thick = 1.000 * 1.067;
+    const double hcer_entr_radlen = 19.63, hcer_entr_thick = 0.002 * 2.54, hcer_1_radlen = 11700.0;
+    const double hcer_mirglass_radlen = 12.29, hcer_mirglass_thick = 0.3;
+    const double hcer_exit_radlen = 19.63, hcer_exit_thick = 0.002 * 2.54;
+    const double hcer_2_entr_radlen = 8.90, hcer_2_entr_thick = 0.040 * 2.54, hcer_2_radlen = 1202.5;
+    const double hcer_mir_radlen = 400., hcer_mir_thick = 2.00;
+    const double hcer_2_exit_radlen = 8.90, hcer_2_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_plan = 6;
+    const double hdc_1_zpos = -40.656, hdc_2_zpos = 39.332;
+    const double hscin_1x_zpos = 52.1, hscin_1y_zpos = 61.7, hscin_2x_zpos = 271.4, hscin_2y_zpos = 282.4;
+    const double hcer_2_zentrance = 72.600, hcer_2_zmirror = 179.400, hcer_2_zexit = 202.600;
+    const double hcal_4ta_zpos = 341.0;
+    const double hcal_left = 63.00, hcal_right = -63.00, hcal_top = -70.00, hcal_bottom = 70.00;
+    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;     // :74
+    double drift;
+
+    P.add(OP_RESMULT_ONE);                                                        // :80
+    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});                       // :99-120
+    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
+    drift = hcer_1_zmirror - hcer_1_zentrance - hcer_mirglass_thick / 2;
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hcer_1_radlen, drift});
+    P.add(OP_MUSC, {hcer_mirglass_thick / hcer_mirglass_radlen});
+    drift = hcer_1_zexit - hcer_1_zmirror - hcer_mirglass_thick / 2;
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hcer_1_radlen, drift});
+    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
+    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - hcer_1_zexit;      // :160-165
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 1, :168-222
+    chamber(P, 1, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
+            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
+    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
+    P.add(OP_BOX, {40. - 0.0, -40. - 0.0, 40.0 - 0.0, -40.0 - 0.0}, ST_S_DC1);
+    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
+    drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;                // :227-234
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 2, :236-287
+    chamber(P, 2, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
+            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
+    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
+    P.add(OP_BOX, {40. - 0., -40. - 0., 40.0 - 0.0, -40.0 - 0.0}, ST_S_DC2);
+    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
+    drift = hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;       // :291-306
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_BOX, {kInf, -kInf, 50. + 0.0, -50. + 0.0}, ST_S_S1X);
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hscin_1y_zpos - hscin_1x_zpos;                                        // :307-321
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_BOX, {45. + 0.0, -45. + 0.0, kInf, -kInf}, ST_S_S1Y);
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hcer_2_zentrance - hscin_1y_zpos;                                     // :326-332
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_MUSC, {hcer_2_entr_thick / hcer_2_entr_radlen});
+    drift = hcer_2_zmirror - hcer_2_zentrance;                                    // :334-341
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hcer_2_radlen, drift});
+    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
+    drift = hcer_2_zexit - hcer_2_zmirror;                                        // :343-349
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hcer_2_radlen, drift});
+    P.add(OP_MUSC, {hcer_2_exit_thick / hcer_2_exit_radlen});
+    drift = hscin_2x_zpos - hcer_2_zexit;                                         // :353-367
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_BOX, {kInf, -kInf, 55. + 0., -55. + 0.}, ST_S_S2X);
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hscin_2y_zpos - hscin_2x_zpos;                                        // :368-382
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_BOX, {62.5 + 0., -62.5 + 0., kInf, -kInf}, ST_S_S2Y);
+    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
+    drift = hcal_4ta_zpos - hscin_2y_zpos;                                        // :390-401
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
+    P.add(OP_BOX, {hcal_bottom, hcal_top, hcal_left, hcal_right}, ST_S_CAL);
+    for (int i = 0; i < 6; ++i) {                                                 // :410-418
+        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+    }
+    P.add(OP_FIT);                                                                // :419-426
+    P.add(OP_FIDUCIAL, {hcal_4ta_zpos, hcal_left - 5.0, hcal_right + 5.0, hcal_bottom - 5.0,
+                        hcal_top + 5.0}, ST_S_CALFID);                            // :435-444
+    P.add(OP_END);
+}
+
+ArmTables make_tables(const simc_ctx* ctx, int arm)
+{
+    ArmTables T;
+    std::memset(&T, 0, sizeof T);
+    if (arm == SIMC_ARM_HMS)
+        T.coll = CollGeom{4.575, 11.646, 4.759, 12.114, 0.000, +0.028, 0.41753, 6.30 / 20};  // hms/mc_hms_coll.f:22-53
+    else
+        T.coll = CollGeom{8.50, 12.50, 8.65, 12.85, 0.000, +0.000, 0.42084, 6.35 / 20};      // shms/mc_shms_coll.f:23-54
+    std::memcpy(T.zdc, ctx->zdc[arm_index(arm)], sizeof T.zdc);
+    return T;
+}
+
+int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::string& rec_path)
+{
+    const int ai = arm_index(arm);
+    std::string msg;
+    int rc = simc::read_forward_map(fwd_path, ctx->fwd[ai], msg);
+    if (rc) return fail(rc, msg);
+    const int want = arm == SIMC_ARM_HMS ? 12 : 32;  // hms/mc_hms.f:188, shms/mc_shms.f:344
+    if (ctx->fwd[ai].n_classes != want)
+        return fail(SIMC_E_NCLASS, fwd_path + ": " + std::to_string(ctx->fwd[ai].n_classes) +
+                                       " transport classes, expected " + std::to_string(want));
+    rc = simc::read_recon_map(rec_path, ctx->rec[ai], msg);
+    if (rc) return fail(rc, msg);
+    std::vector<double> packed;
+    rc = simc::pack_arm(arm, ctx->fwd[ai], ctx->rec[ai], packed, msg);
+    if (rc) return fail(rc, msg);
+    Program P;
+    if (arm == SIMC_ARM_HMS) build_hms(ctx->fwd[ai], P, ctx->zdc[ai]);
+    else build_shms(P, ctx->zdc[ai]);
+    ctx->n_ops[ai] = (int)P.ops.size();
+    ctx->ops[ai] = P.ops;
+    HIPCHK(hipMalloc(&ctx->d_coef[ai], packed.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_coef[ai], packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->d_ops[ai], P.ops.size() * sizeof(Op)));
+    HIPCHK(hipMemcpy(ctx->d_ops[ai], P.ops.data(), P.ops.size() * sizeof(Op), hipMemcpyHostToDevice));
+    const ArmTables T = make_tables(ctx, arm);
+    HIPCHK(hipMalloc(&ctx->d_tab[ai], sizeof(ArmTables)));
+    HIPCHK(hipMemcpy(ctx->d_tab[ai], &T, sizeof(ArmTables), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->d_hist[ai], kHistSlots * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
+    return SIMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* simc_last_error(void) { return g_err.c_str(); }
+const char* simc_version(void) { return "simc_gpu 0.1 (gfx950)"; }
+
+int simc_ctx_create_files(const char* hms_forward, const char* hms_recon, const char* shms_forward,
+                          const char* shms_recon, simc_ctx** out)
+{
+    if (!out || !hms_forward || !hms_recon || !shms_forward || !shms_recon)
+        return fail(SIMC_E_ARG, "simc_ctx_create_files: NULL argument");
+    *out = nullptr;
+    simc_ctx* ctx = new (std::nothrow) simc_ctx();
+    if (!ctx) return fail(SIMC_E_NOMEM, "out of host memory");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete ctx;
+        return fail(SIMC_E_HIP, "no HIP device available");
+    }
+    hipGetDevice(&ctx->device);
+    hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    int rc = load_arm(ctx, SIMC_ARM_HMS, hms_forward, hms_recon);
+    if (!rc) rc = load_arm(ctx, SIMC_ARM_SHMS, shms_forward, shms_recon);
+    if (rc) {
+        std::string keep = g_err;
+        simc_ctx_destroy(ctx);
+        g_err = keep;
+        return rc;
+    }
+    *out = ctx;
+    return SIMC_OK;
+}
+
+int simc_ctx_create(const char* map_dir, simc_ctx** out)
+{
+    if (!map_dir) return fail(SIMC_E_ARG, "simc_ctx_create: NULL map_dir");
+    const std::string d(map_dir);
+    return simc_ctx_create_files((d + "/hms/forward_cosy.dat").c_str(), (d + "/hms/recon_cosy.dat").c_str(),
+                                 (d + "/shms/shms_forward.dat").c_str(), (d + "/shms/shms_recon.dat").c_str(), out);
+}
+
+void simc_ctx_destroy(simc_ctx* ctx)
+{
+    if (!ctx) return;
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->d_coef[i]) (void)hipFree(ctx->d_coef[i]);
+        if (ctx->d_ops[i]) (void)hipFree(ctx->d_ops[i]);
+        if (ctx->d_tab[i]) (void)hipFree(ctx->d_tab[i]);
+        if (ctx->d_hist[i]) (void)hipFree(ctx->d_hist[i]);
+    }
+    delete ctx;
+}
+
+int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                 simc_events_out* out, int64_t n, const simc_rng* rng, void* stream)
+{
+    if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1) or SIMC_ARM_SHMS (5)");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz)
+        return fail(SIMC_E_ARG, "missing event input array");
+    if (!out->dpp || !out->y || !out->dxdz || !out->dydz || !out->x_fp || !out->dx_fp || !out->y_fp ||
+        !out->dy_fp || !out->pathlen || !out->ok || !out->stop)
+        return fail(SIMC_E_ARG, "missing event output array");
+    if (!(s->p_spec > 0.0) || !(s->m2 >= 0.0)) return fail(SIMC_E_ARG, "p_spec must be > 0 and m2 >= 0");
+    if (s->decay_flag && !(s->ctau > 0.0)) return fail(SIMC_E_ARG, "decay_flag needs ctau > 0");
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
+    KArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.p_spec = s->p_spec;
+    a.m2 = s->m2;
+    a.ctau = s->ctau;
+    a.wire_sigma = s->wire_sigma;
+    // hdc_sigma: hms/mc_hms_hut.f:143-144, shms/hut.inc:125-126
+    a.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_HMS ? 0.030 : 0.020);
+    a.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
+              (s->using_coll ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
+    a.seed = rng->seed;
+    a.tab = ctx->d_tab[ai];
+    a.coef = ctx->d_coef[ai];
+    a.ops = ctx->d_ops[ai];
+    a.hist = ctx->d_hist[ai];
+    const bool tape = rng->kind == SIMC_RNG_TAPE;
+    // Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
+    // shms/mc_shms.f:503); m2 changes only through a decay.
+    const bool rare = s->decay_flag || (s->using_coll && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
+    // Launches are cut so that every byte offset of a launch fits 32 bits.
+    int64_t chunk = (int64_t)1 << 26;
+    if (tape) {
+        const int64_t sb = rng->stride * 8;
+        if (sb > 0x7fffffffLL) return fail(SIMC_E_ARG, "tape stride too large");
+        const int64_t c = 0xffffffffLL / sb;
+        if (c < chunk) chunk = c;
+        a.stride_bytes = (uint32_t)sb;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    for (int64_t b = 0; b < n; b += chunk) {
+        const int64_t cnt = n - b < chunk ? n - b : chunk;
+        a.count = (uint32_t)cnt;
+        a.event0 = rng->event0 + (uint64_t)b;
+        a.dpp = in->dpp + b; a.x = in->x + b; a.y = in->y + b; a.dxdz = in->dxdz + b; a.dydz = in->dydz + b;
+        a.fry = in->fry ? in->fry + b : nullptr;
+        a.o_dpp = out->dpp + b; a.o_y = out->y + b; a.o_dxdz = out->dxdz + b; a.o_dydz = out->dydz + b;
+        a.x_fp = out->x_fp + b; a.dx_fp = out->dx_fp + b; a.y_fp = out->y_fp + b; a.dy_fp = out->dy_fp + b;
+        a.pathlen = out->pathlen + b;
+        a.resmult = out->resmult ? out->resmult + b : nullptr;
+        a.m2o = out->m2 ? out->m2 + b : nullptr;
+        a.ok = out->ok + b;
+        a.stop = out->stop + b;
+        a.used = out->used ? out->used + b : nullptr;
+        a.tape = tape ? rng->tape + b * rng->stride : nullptr;
+        const unsigned blocks = (unsigned)((cnt + kBlock - 1) / kBlock);
+        const dim3 G(blocks), B(kBlock);
+#define LAUNCH(ARMV, RV, RARE) hipLaunchKernelGGL((arm_kernel<ARMV, RV, RARE>), G, B, 0, st, a)
+        if (arm == SIMC_ARM_HMS) {
+            if (tape) { if (rare) LAUNCH(SIMC_ARM_HMS, TapeStream, true); else LAUNCH(SIMC_ARM_HMS, TapeStream, false); }
+            else { if (rare) LAUNCH(SIMC_ARM_HMS, PhiloxStream, true); else LAUNCH(SIMC_ARM_HMS, PhiloxStream, false); }
+        } else {
+            if (tape) { if (rare) LAUNCH(SIMC_ARM_SHMS, TapeStream, true); else LAUNCH(SIMC_ARM_SHMS, TapeStream, false); }
+            else { if (rare) LAUNCH(SIMC_ARM_SHMS, PhiloxStream, true); else LAUNCH(SIMC_ARM_SHMS, PhiloxStream, false); }
+        }
+#undef LAUNCH
+        HIPCHK(hipGetLastError());
+    }
+    return SIMC_OK;
+}
+
+int simc_hms_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+                 int64_t n, const simc_rng* rng, void* stream)
+{
+    return simc_arm_run(ctx, SIMC_ARM_HMS, s, in, out, n, rng, stream);
+}
+
+int simc_shms_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+                  int64_t n, const simc_rng* rng, void* stream)
+{
+    return simc_arm_run(ctx, SIMC_ARM_SHMS, s, in, out, n, rng, stream);
+}
+
+int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const double* dx_fp,
+                   const double* y_fp, const double* dy_fp, const double* fry, double* delta,
+                   double* dxdz_tar, double* dydz_tar, double* y_tar, void* stream)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "simc_recon_run: NULL context");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (!x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !dxdz_tar || !dydz_tar || !y_tar)
+        return fail(SIMC_E_ARG, "simc_recon_run: NULL array");
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(SIMC_E_ARG, "too many events for one launch");
+    hipStream_t st = (hipStream_t)stream;
+    if (arm == SIMC_ARM_HMS)
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+                           dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
+    else
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+                           dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
+    HIPCHK(hipGetLastError());
+    return SIMC_OK;
+}
+
+int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
+{
+    if (!ctx || !c) return fail(SIMC_E_ARG, "simc_counters_get: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    const int need = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    if (ncounters < need) return fail(SIMC_E_ARG, "counter array too short");
+    unsigned long long h[kHistSlots];
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(h, ctx->d_hist[ai], sizeof h, hipMemcpyDeviceToHost));
+    for (int i = 0; i < ncounters; ++i) c[i] = 0;
+    int64_t total = 0;
+    for (int i = 0; i < kHistBins; ++i) total += (int64_t)h[i];
+    auto H = [&](int code) { return (int64_t)h[code]; };
+    if (arm == SIMC_ARM_HMS) {
+        // order of common /spectrometer1/, hms/struct_hms.inc:66-74
+        c[0] = total;
+        c[1] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
+        c[2] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
+        c[3] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
+        for (int k = 0; k < 9; ++k) c[4 + k] = H(ST_H_Q1_IN + k);
+        c[13] = H(ST_H_D1_IN);
+        c[14] = H(ST_H_D1_OUT) + H(ST_H_PIPE0) + H(ST_H_PIPE1) + H(ST_H_PIPE2) + H(ST_H_PIPE3);
+        c[16] = H(ST_H_DC1);
+        c[17] = H(ST_H_DC2);
+        c[18] = H(ST_H_SCIN);
+        c[19] = 0;  // hSTOP_cal: the calorimeter cut is commented out, hms/mc_hms_hut.f:584-588
+        c[20] = H(ST_OK);
+        c[15] = c[16] + c[17] + c[18] + c[20];
+        c[21] = H(ST_COLL);
+    } else {
+        // order of common /spectrometer_shms/, shms/struct_shms.inc:320-338
+        c[0] = total;
+        c[1] = H(ST_OK);
+        c[5] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
+        c[6] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
+        c[7] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
+        for (int k = 0; k < 4; ++k) c[8 + k] = H(ST_S_HB_IN + k);
+        for (int k = 0; k < 15; ++k) c[12 + k] = H(ST_S_Q1_IN + k);
+        for (int k = 0; k < 12; ++k) c[27 + k] = H(ST_S_D1_IN + k);
+        c[42] = H(ST_S_DC1);
+        c[43] = H(ST_S_DC2);
+        c[46] = H(ST_S_S1X) + H(ST_S_S1Y);
+        c[47] = H(ST_S_S2Y);
+        c[48] = H(ST_S_S2X);
+        c[49] = H(ST_S_CAL) + H(ST_S_CALFID);
+        c[41] = c[42] + c[43] + c[46] + c[47] + c[48] + c[49] + c[1];
+        c[51] = H(ST_COLL);
+    }
+    return SIMC_OK;
+}
+
+int simc_counters_reset(simc_ctx* ctx, int arm)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "simc_counters_reset: NULL context");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
+    HIPCHK(hipDeviceSynchronize());
+    return SIMC_OK;
+}
+
+int simc_map_class_info(simc_ctx* ctx, int arm, int cls, double* driftdist, int32_t* adrift, int32_t* n_terms)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "NULL context");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    const simc::ForwardMap& f = ctx->fwd[ai];
+    if (cls < 1 || cls > f.n_classes) return fail(SIMC_E_ARG, "class out of range");
+    if (driftdist) *driftdist = f.driftdist[cls - 1];
+    if (adrift) *adrift = f.adrift[cls - 1];
+    if (n_terms) *n_terms = (int32_t)f.classes[cls - 1].size();
+    return SIMC_OK;
+}
+
+int simc_stop_flops(simc_ctx* ctx, int arm, int64_t* flops, int nflops)
+{
+    if (!ctx || !flops) return fail(SIMC_E_ARG, "simc_stop_flops: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
+    for (int i = 0; i < nflops; ++i) flops[i] = 0;
+    int64_t acc = 0;
+    for (const Op& o : ctx->ops[ai]) {
+        if (o.code == OP_TRANSP) acc += simc::poly_flops(arm, o.arg);
+        if (o.code == OP_COLLIMATOR) {
+            flops[ST_SLIT_HOR] = flops[ST_SLIT_VERT] = flops[ST_SLIT_OCT] = flops[ST_COLL] = acc;
+        } else if (o.stop > 0 && o.code != OP_WIRE && o.stop < kHistBins) {
+            flops[o.stop] = acc;
+        }
+        if (o.code == OP_END) flops[ST_OK] = acc + simc::poly_flops(arm, 0);
+    }
+    return SIMC_OK;
+}
+
+int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)
+{
+    if (!ctx || arm_index(arm) < 0) return 0;
+    static const int hms_cls[] = {2, 3, 5, 6, 8, 9, 11, 0};
+    static const int shms_cls[] = {3, 7, 8, 12, 13, 17, 18, 23, 24, 25, 26, 27, 28, 29, 30, 0};
+    int64_t f = 0;
+    if (arm == SIMC_ARM_HMS)
+        for (int c : hms_cls) f += simc::poly_flops(arm, c);
+    else
+        for (int c : shms_cls) f += simc::poly_flops(arm, c);
+    return f;
+}
+
+}  // extern "C"

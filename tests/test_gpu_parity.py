@@ -1,0 +1,160 @@
+"""GPU kernels vs the FP64 CPU oracle on identical inputs and uniforms.
+
+Two uniform sources are compared:
+  * tape  -- both sides read the same per-event uniform tape;
+  * philox -- both sides generate the same counter-based Philox4x32-10 stream.
+Criteria: stop codes identical except events whose aperture
+margin is below 1e-9 (none expected at these sizes); focal-plane and
+reconstructed coordinates within 1e-12 relative in the FP64 track-fit mode.
+In the reference REAL*4 fit mode the values are float-quantised; events whose
+float rounding flips are counted and must be rare.
+"""
+import numpy as np
+import pytest
+
+from simc_gfortran_amd import workloads as W
+from tests._compare import parity_report
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from simc_gfortran_amd import api
+    c = api.Spectrometers()
+    yield c
+    c.close()
+
+
+def to_dev(ev):
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in ev.items()}
+
+
+def to_host(out):
+    return {k: v.cpu().numpy() for k, v in out.items() if v is not None}
+
+
+CASES = [
+    ("hms_c0", W.HMS_C0),
+    ("shms_c1", W.SHMS_C1),
+]
+
+
+@pytest.mark.parametrize("name,wl", CASES)
+@pytest.mark.parametrize("fit", ["fp64", "ref_f32"])
+def test_tape_parity(ctx, oracle, name, wl, fit):
+    n, stride = 20000, 1024
+    ev = W.generate(wl, n)
+    tape = W.uniform_tape(n, stride, 4242)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma)
+    ref = oracle.run(wl.arm, ev, rng="tape", tape=tape, stride=stride, fit_f32=(fit == "ref_f32"), **kw)
+    assert ref["rc"] == 0
+    out = ctx.run(wl.arm, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+                  fit=fit, **kw)
+    torch.cuda.synchronize()
+    g = to_host(out)
+    rep = parity_report(g, ref, 1e-12)
+    print(name, fit, rep)
+    assert rep["flag_mismatch"] == 0
+    assert (g["used"] == ref["used"]).all()
+    assert rep["n_both_ok"] > 0.2 * n
+    if fit == "fp64":
+        assert all(v == 0 for v in rep["bad"].values()), rep
+    else:
+        # float-rounding flips of the REAL*4 fit: rare
+        flips = max(rep["bad"].values())
+        assert flips <= max(2, n // 10000), rep
+
+
+@pytest.mark.parametrize("name,wl", CASES)
+def test_philox_parity_and_counters(ctx, oracle, name, wl):
+    n = 50000
+    ev = W.generate(wl, n, seed=77)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma)
+    ref = oracle.run(wl.arm, ev, rng="philox", seed=2024, event0=1000, fit_f32=False, **kw)
+    ctx.reset_counters(wl.arm)
+    out = ctx.run(wl.arm, to_dev(ev), rng="philox", seed=2024, event0=1000, fit="fp64", **kw)
+    torch.cuda.synchronize()
+    g = to_host(out)
+    rep = parity_report(g, ref, 1e-12)
+    print(name, rep)
+    assert rep["flag_mismatch"] == 0
+    assert all(v == 0 for v in rep["bad"].values()), rep
+    assert (g["used"] == ref["used"]).all()
+    cnt = ctx.counters(wl.arm)
+    names = list(cnt)
+    for i, k in enumerate(names):
+        if k == "stop_id":
+            continue
+        assert cnt[k] == ref["counters"][i], (k, cnt[k], ref["counters"][i])
+
+
+MPI = 139.57018
+
+
+@pytest.mark.parametrize("arm", [1, 5])
+@pytest.mark.parametrize("coll,decay", [(True, False), (False, True), (True, True)])
+def test_rare_paths_parity(ctx, oracle, arm, coll, decay):
+    """Pions: collimator punch-through (absorption, scattering, energy loss)
+    and in-flight decay, both of which the reference only runs for pions."""
+    n, stride = 20000, 2048
+    wl = W.HMS_C0 if arm == 1 else W.SHMS_C1
+    ev = W.generate(wl, n, seed=5)
+    tape = W.uniform_tape(n, stride, 99)
+    kw = dict(p_spec=2000.0, m2=MPI * MPI, ms=True, wcs=True, coll=coll, decay=decay, ctau=780.45)
+    ref = oracle.run(arm, ev, rng="tape", tape=tape, stride=stride, fit_f32=False, **kw)
+    out = ctx.run(arm, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+                  fit="fp64", **kw)
+    torch.cuda.synchronize()
+    g = to_host(out)
+    rep = parity_report(g, ref, 1e-10)
+    print(arm, coll, decay, rep)
+    assert rep["flag_mismatch"] <= 2
+    assert sum(rep["bad"].values()) <= 2, rep
+    np.testing.assert_array_equal(g["m2"] != MPI * MPI, ref["m2"] != MPI * MPI)
+
+
+@pytest.mark.parametrize("arm", [1, 5])
+def test_recon_only(ctx, oracle, arm):
+    rng = np.random.default_rng(3)
+    n = 10000
+    x = rng.uniform(-30, 30, n)
+    dx = rng.uniform(-0.05, 0.05, n)
+    y = rng.uniform(-20, 20, n)
+    dy = rng.uniform(-0.04, 0.04, n)
+    fry = rng.normal(0, 0.2, n)
+    fry[:10] = 0.0
+    ref = oracle.recon(arm, x, dx, y, dy, fry)
+    d = [torch.from_numpy(v).cuda() for v in (x, dx, y, dy, fry)]
+    got = [t.cpu().numpy() for t in ctx.recon(arm, *d)]
+    for a, b, s in zip(got, ref, (1.0, 1e-2, 1e-2, 1.0)):
+        assert np.all(np.abs(a - b) <= 1e-12 * (np.abs(b) + s))
+
+
+def test_errors(ctx):
+    from simc_gfortran_amd import api
+    ev = to_dev(W.generate(W.HMS_C0, 16))
+    with pytest.raises(api.SimcError) as e:
+        ctx.run(3, ev, p_spec=3000.0, m2=1.0)
+    assert e.value.code == -6
+    with pytest.raises(api.SimcError):
+        ctx.run(1, ev, p_spec=-1.0, m2=1.0)
+    with pytest.raises(api.SimcError):
+        ctx.run(1, ev, p_spec=3000.0, m2=1.0, decay=True, ctau=0.0)
+
+
+def test_tape_overflow_is_reported(ctx, oracle):
+    n, stride = 64, 8
+    ev = W.generate(W.HMS_C0, n)
+    tape = W.uniform_tape(n, stride, 1)
+    out = ctx.run(1, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+                  p_spec=3000.0, m2=W.MP ** 2)
+    g = to_host(out)
+    ref = oracle.run(1, ev, rng="tape", tape=tape, stride=stride, p_spec=3000.0, m2=W.MP ** 2)
+    # events reaching the hut need far more than 8 uniforms
+    assert ((g["stop"] == 250) == (ref["stop"] == 250)).all()
+    assert (g["stop"] == 250).any()
