@@ -1899,3 +1899,26 @@ int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const
     }
     return OR_OK;
 }
+
+/* gauss1 on a uniform tape (unit tests): returns the draw, *used = uniforms read. */
+double or_gauss1_tape(const double *tape, int64_t len, double nsigmax, int64_t *used)
+{
+    or_rng g;
+    memset(&g, 0, sizeof g);
+    g.kind = 0;
+    g.tape = tape;
+    g.stride = len;
+    double v = or_gauss1(&g, nsigmax);
+    if (used) *used = g.overflow ? -1 : g.pos;
+    return v;
+}
+
+/* RANLUX state after RLUXGO(lux, seed, 0, 0) in RLUXUT form, then n values of
+ * RANLUX(RVEC, n) (unit tests). */
+void or_ranlux_sequence(int lux, int seed, int32_t isdext[25], double *rvec, int n)
+{
+    or_ranlux *r = or_ranlux_new(lux, seed);
+    or_ranlux_save(r, isdext);
+    or_ranlux_fill(r, rvec, n);
+    free(r);
+}

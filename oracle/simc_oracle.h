@@ -210,6 +210,8 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
            or_events_out *out, int rng_kind, const double *tape, int64_t stride,
            uint64_t seed, uint64_t event0, int nthreads, int64_t *counters);
 
+double or_gauss1_tape(const double *tape, int64_t len, double nsigmax, int64_t *used);
+void or_ranlux_sequence(int lux, int seed, int32_t isdext[25], double *rvec, int n);
 int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const double *dx_fp,
                  const double *y_fp, const double *dy_fp, const double *fry, double *delta,
                  double *dxdz, double *dydz, double *ytar);

@@ -1,0 +1,286 @@
+"""Pins of the FP64 oracle (CPU only).
+
+* golden vectors written by the reference's own Fortran (tests/golden/):
+  the oracle must reproduce every output bit for bit on the same tapes;
+* known answers and closed forms for the single routines (Philox KAT,
+  RANLUX vs the C++ standard engine, LFIT on exact lines, the Marsaglia polar
+  step, drifts as COSY classes, the absorption law, rotations);
+* the map files: checksums and term counts against an independent count.
+"""
+import ctypes as C
+import glob
+import hashlib
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from simc_gfortran_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+# ---------------------------------------------------------------------------
+# golden vectors of the reference
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_matches_reference_golden(oracle, path):
+    g = np.load(path)
+    stride = int(g["stride"])
+    n = len(g["in_dpp"])
+    tape = W.uniform_tape(n, stride, int(g["tape_seed"]))
+    assert hashlib.sha256(tape.tobytes()).hexdigest() == str(g["tape_sha256"])
+    p_spec, m2, ms, wcs, decay, coll, ctau = g["settings"]
+    ev = {k: g["in_" + k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
+    out = oracle.run(int(g["arm"]), ev, p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs),
+                     decay=bool(decay), coll=bool(coll), ctau=ctau, rng="tape", tape=tape,
+                     stride=stride, fit_f32=True)
+    assert out["rc"] == 0
+    for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
+              "resmult", "m2", "ok", "used"):
+        np.testing.assert_array_equal(out[k], g["out_" + k], err_msg=k)
+    names = oracle.HMS_COUNTERS if int(g["arm"]) == 1 else oracle.SHMS_COUNTERS
+    for i, name in enumerate(names):
+        if name != "stop_id":  # a last-value register, not a counter
+            assert out["counters"][i] == g["counters"][i], name
+
+
+# ---------------------------------------------------------------------------
+# random numbers
+# ---------------------------------------------------------------------------
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors, philox4x32_10
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kat:
+        assert tuple(oracle.philox_block(ctr, key)) == want
+
+
+def test_philox_uniform_mapping(oracle):
+    L = oracle.lib()
+    blk = oracle.philox_block((0, 5, 0, 0x53494D43), (7, 0))
+    for k in range(4):
+        assert L.or_philox_uniform(7, 0, 5, k) == (blk[k] + 0.5) * 2.0 ** -32
+    u = np.array([L.or_philox_uniform(1, 2, 3, k) for k in range(4000)])
+    assert 0.0 < u.min() and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.02
+
+
+@pytest.fixture(scope="module")
+def ranlux_std(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ranlux")
+    exe = str(d / "ranlux_std")
+    subprocess.run(["g++", "-O1", "-std=c++17", os.path.join(HERE, "ranlux_std.cpp"), "-o", exe],
+                   check=True)
+    return exe
+
+
+def test_std_ranlux_engine_known_answers(ranlux_std):
+    # [rand.predef] fixes the 10000th value of ranlux24_base and ranlux24
+    out = subprocess.run([ranlux_std, "1", "kat"], capture_output=True, text=True, check=True)
+    assert out.stdout.split() == ["7937952", "9901578"]
+
+
+@pytest.mark.parametrize("seed", [1, 314159265, 123456789])
+def test_ranlux_matches_std_discard_block(oracle, ranlux_std, seed):
+    """James's RANLUX (cern/ranlux.f), luxury 3, equals
+    discard_block_engine<ranlux24_base, 223, 24> started from the same state."""
+    L = oracle.lib()
+    n = 3000
+    isd = (C.c_int32 * 25)()
+    rvec = (C.c_double * n)()
+    L.or_ranlux_sequence(3, seed, isd, rvec, n)
+    seeds = list(isd)[:24]
+    i24 = abs(isd[24]) % 100
+    assert i24 == 24 and isd[24] > 0  # fresh RLUXGO state: next write at 24, no carry
+    # seeds(24) is the oldest value, seeds(1) the newest (transp of NEXT(), ranlux.f:64-69)
+    state = " ".join(str(s) for s in seeds[::-1]) + " 0"
+    out = subprocess.run([ranlux_std, str(n)], input=state + "\n", capture_output=True, text=True,
+                         check=True)
+    want = np.array([int(v) for v in out.stdout.split()], dtype=np.float64)
+    got = np.array(list(rvec))
+    # RANLUX pads values below 2^-12 with lower bits (ranlux.f:125-129)
+    assert np.array_equal(np.floor(got * 2.0 ** 24), want)
+    big = want >= 4096
+    assert np.array_equal(got[big], want[big] * 2.0 ** -24)
+
+
+# ---------------------------------------------------------------------------
+# single routines
+# ---------------------------------------------------------------------------
+def gauss_on_tape(oracle, tape, nsig=99.0):
+    L = oracle.lib()
+    L.or_gauss1_tape.restype = C.c_double
+    L.or_gauss1_tape.argtypes = [C.POINTER(C.c_double), C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+    t = (C.c_double * len(tape))(*tape)
+    used = C.c_int64()
+    v = L.or_gauss1_tape(t, len(tape), nsig, C.byref(used))
+    return v, used.value
+
+
+def test_gauss1_polar_step(oracle):
+    # u = 0.75, 0.75 -> v = (0.5, 0.5), s = 1/2: g = 0.5 sqrt(4 ln 2) = sqrt(ln 2)
+    v, used = gauss_on_tape(oracle, [0.75, 0.75])
+    assert used == 2 and abs(v - math.sqrt(math.log(2.0))) < 1e-15
+    # s > 1 is rejected and the next pair is taken; s == 0 is rejected too
+    v, used = gauss_on_tape(oracle, [0.95, 0.95, 0.5, 0.5, 0.25, 0.5])
+    assert used == 6 and abs(v - (-0.5) * math.sqrt(-2.0 * math.log(0.25) / 0.25)) < 1e-15
+    # truncation at nsig rejects the pair
+    v, used = gauss_on_tape(oracle, [0.5000001, 0.5, 0.75, 0.75], nsig=3.0)
+    assert used == 4
+
+
+def lfit(oracle, z, y, f32):
+    L = oracle.lib()
+    n = len(z)
+    if f32:
+        T = C.c_float
+        fn = L.or_lfit_f32
+    else:
+        T = C.c_double
+        fn = L.or_lfit_f64
+    fn.argtypes = [C.POINTER(T), C.POINTER(T), C.c_int, C.c_int, C.POINTER(T), C.POINTER(T), C.POINTER(T)]
+    a, b, e = T(), T(), T()
+    fn((T * n)(*z), (T * n)(*y), n, 0, C.byref(a), C.byref(b), C.byref(e))
+    return a.value, b.value, e.value
+
+
+@pytest.mark.parametrize("f32", [True, False])
+def test_lfit_exact_line_and_zero_skip(oracle, f32):
+    z = [-52.0, -50.0, -48.0, -46.0, 30.0, 32.0, 34.0, 36.0]
+    y = [0.25 * zz + 3.0 for zz in z]   # exactly representable points on a line
+    a, b, e = lfit(oracle, z, y, f32)
+    assert a == 0.25 and b == 3.0 and e == 0.0
+    # a zero hit is ignored (cern/lfit.f:30): same answer with an extra (z, 0) point
+    a2, b2, _ = lfit(oracle, z + [10.0], y + [0.0], f32)
+    assert (a2, b2) == (a, b)
+    # fewer than two points: A = B = 0
+    assert lfit(oracle, [1.0, 2.0], [0.0, 5.0], f32)[:2] == (0.0, 0.0)
+
+
+def test_pion_absorption_law(oracle):
+    L = oracle.lib()
+    p = 1500.0
+    assert L.or_pion_coll_absorb(p, 0.0) == 1.0
+    t1 = L.or_pion_coll_absorb(p, 0.3)
+    t2 = L.or_pion_coll_absorb(p, 0.6)
+    assert 0.0 < t1 < 1.0 and abs(t2 - t1 * t1) < 1e-15   # exp(-2 x) = exp(-x)^2
+    assert L.or_pion_coll_absorb(20000.0, 0.3) < 1.0      # above the table: last entry
+
+
+def test_powi_is_binary_exponentiation(oracle):
+    L = oracle.lib()
+    for x in (1.7, -0.3, 12.5, 1e-3):
+        assert L.or_powi(x, 0) == 1.0
+        assert L.or_powi(x, 1) == x
+        assert L.or_powi(x, 2) == x * x
+        assert L.or_powi(x, 4) == (x * x) * (x * x)
+        for n in range(7):
+            assert abs(L.or_powi(x, n) - x ** n) <= 4e-16 * abs(x ** n)
+    assert L.or_powi(2.0, 6) == 64.0 and L.or_powi(2.0, -2) == 0.25
+
+
+# ---------------------------------------------------------------------------
+# the COSY maps
+# ---------------------------------------------------------------------------
+MAP_SHA = {
+    "hms/forward_cosy.dat": "7e9a5bc770f46fda04f3fba1ff54b9b00f6e1e2e61aee027b6b5b700e3a729c7",
+    "hms/recon_cosy.dat": "61cf89af0c3a5f7370ca69b84f2a2cd47d8b201c34eb4bbd761afd83e51d80ca",
+    "shms/shms_forward.dat": "398595898491378e3d4ade794e1af6b324afd1fcccc20ec9bb5d4456ff75cffc",
+    "shms/shms_recon.dat": "7477b37ed142de7743b50080f38d7e45074196e7387ee899f79d04c86bb655b0",
+}
+
+
+def test_map_files_checksums():
+    for rel, sha in MAP_SHA.items():
+        with open(os.path.join(REPO, "data", "maps", rel), "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == sha, rel
+
+
+class FwdMap(C.Structure):
+    _fields_ = [("n_classes", C.c_int), ("n_terms", C.c_int * 41), ("length", C.c_double * 41),
+                ("adrift", C.c_int * 41), ("driftdist", C.c_double * 41),
+                ("coeff", C.c_double * (41 * 1000 * 5)), ("expon", C.c_short * (41 * 1000 * 5))]
+
+
+class ReconMap(C.Structure):
+    _fields_ = [("n_terms", C.c_int), ("max_order", C.c_int), ("coeff", C.c_double * 4000),
+                ("expon", C.c_short * 5000)]
+
+
+class Maps(C.Structure):
+    _fields_ = [("hms_fwd", C.POINTER(FwdMap)), ("shms_fwd", C.POINTER(FwdMap)),
+                ("hms_rec", C.POINTER(ReconMap)), ("shms_rec", C.POINTER(ReconMap))]
+
+
+@pytest.fixture(scope="module")
+def maps(oracle):
+    return C.cast(C.c_void_p(oracle.maps()), C.POINTER(Maps)).contents
+
+
+# Term counts per class after dropping the time-of-flight line, counted
+# independently of both parsers (one TOF line per class).
+HMS_TERMS = {1: 9, 2: 291, 3: 278, 4: 9, 5: 339, 6: 318, 7: 9, 8: 269, 9: 253, 10: 9, 11: 361, 12: 9}
+SHMS_TERMS = {3: 212, 7: 279, 8: 284, 12: 287, 13: 288, 17: 271, 18: 280, 23: 220, 30: 218,
+              **{k: 138 for k in range(24, 30)},
+              **{k: 9 for k in (1, 2, 4, 5, 6, 9, 10, 11, 14, 15, 16, 19, 20, 21, 22, 31, 32)}}
+
+
+def test_forward_map_structure(maps):
+    h, s = maps.hms_fwd.contents, maps.shms_fwd.contents
+    assert h.n_classes == 12 and s.n_classes == 32
+    for cls, n in HMS_TERMS.items():
+        assert h.n_terms[cls - 1] == n, cls
+    for cls, n in SHMS_TERMS.items():
+        assert s.n_terms[cls - 1] == n, cls
+    # pure drifts are exactly the 9-term classes; their length matches the comment
+    for m, terms in ((h, HMS_TERMS), (s, SHMS_TERMS)):
+        for cls, n in terms.items():
+            assert bool(m.adrift[cls - 1]) == (n == 9), cls
+            if n == 9 and m.length[cls - 1] > 0:
+                assert abs(m.driftdist[cls - 1] - m.length[cls - 1]) < 0.01
+    assert abs(h.driftdist[0] - 216.075) < 1e-9
+
+
+def test_recon_map_structure(maps):
+    for r in (maps.hms_rec.contents, maps.shms_rec.contents):
+        assert r.n_terms == 461 and r.max_order <= 6
+        e = np.array(r.expon[:5 * r.n_terms]).reshape(-1, 5)
+        assert not (e.sum(axis=1) == 0).any()          # no constant term
+        assert len({tuple(x) for x in e}) == r.n_terms  # no duplicate monomial
+
+
+def test_drift_class_is_a_field_free_drift(oracle, maps):
+    """A pure-drift class evaluated as a COSY polynomial equals `project`:
+    x += L x', y += L y' and the path-length term reproduces L(sqrt(1+x'^2+y'^2)-1)."""
+    L = oracle.lib()
+    L.or_transp_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fwd = C.cast(maps.hms_fwd, C.c_void_p)
+    h = maps.hms_fwd.contents
+    for cls in (1, 4, 7, 10, 12):
+        Ld = h.driftdist[cls - 1]
+        for xp, yp in ((0.01, -0.02), (-0.03, 0.005)):
+            ray = (C.c_double * 5)(1.5, xp * 1000.0, -2.0, yp * 1000.0, 3.0)
+            out = (C.c_double * 5)()
+            L.or_transp_eval(fwd, cls, ray, out)
+            assert abs(out[0] - (1.5 + Ld * xp)) < 1e-11
+            assert abs(out[2] - (-2.0 + Ld * yp)) < 1e-11
+            assert abs(out[1] - xp * 1000.0) < 1e-12 and abs(out[3] - yp * 1000.0) < 1e-12
+            want_dl = Ld * (math.sqrt(1 + xp * xp + yp * yp) - 1.0)
+            assert abs(-out[4] - want_dl) < 1e-6 * Ld * (xp * xp + yp * yp) + 1e-9
+
+
+def test_central_ray_stays_on_axis(oracle, maps):
+    L = oracle.lib()
+    L.or_transp_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    for ptr, n in ((maps.hms_fwd, 12), (maps.shms_fwd, 32)):
+        for cls in range(1, n + 1):
+            out = (C.c_double * 5)()
+            L.or_transp_eval(C.cast(ptr, C.c_void_p), cls, (C.c_double * 5)(0, 0, 0, 0, 0), out)
+            assert max(abs(v) for v in out) < 1e-12, cls
