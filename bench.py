@@ -66,6 +66,55 @@ def cpu_baseline(wl, seconds=12.0):
             "single_thread_events_per_s": 40_000 / t1}
 
 
+STAGE_NAMES = ["stage1_target_to_collimator", "stage2_magnets", "stage3_hut_fit_recon"]
+
+
+def pmc_summary(fetch_csv, write_csv, events):
+    """HBM bytes per launch of each kernel stage from FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch), each scaled by what the same counter reads for the
+    calibration copy (the largest copy dispatch: M MiB read and M MiB written)."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            key = None
+            if "stage_kernel" in name:
+                key = STAGE_NAMES[0] if "false, 1, false, true" in name else \
+                    STAGE_NAMES[1] if "false, 1, true, true" in name else \
+                    STAGE_NAMES[2] if "false, 2, true, false" in name else None
+            elif "at::native" in name and "copy" in name:
+                key = "copy"
+            if key is None:
+                continue
+            acc.setdefault(key, {}).setdefault(r["Dispatch_Id"], 0.0)
+            acc[key][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        return acc
+
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    copy_f = max(fetch["copy"].values())  # KiB
+    copy_w = max(write["copy"].values())
+    # the probe copies M MiB: infer M from the write side (nearest MiB)
+    mib = round(copy_w / 1024.0)
+    f_fetch = copy_f / (mib * 1024.0)
+    f_write = copy_w / (mib * 1024.0)
+    res = {"events_per_step": events, "calibration_copy_MiB": mib,
+           "fetch_factor": f_fetch, "write_factor": f_write, "kernels": {}}
+    for k in STAGE_NAMES:
+        fk = float(np.mean(list(fetch[k].values()))) * 1024.0 / f_fetch
+        wk = float(np.mean(list(write[k].values()))) * 1024.0 / f_write
+        res["kernels"][k] = {"fetch_bytes": fk, "write_bytes": wk, "hbm_bytes_per_launch": fk + wk,
+                             "dispatches": len(fetch[k])}
+    out = os.path.join(REPO, "profiles", "hbm_traffic_shms_1e7.json")
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,10 +122,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "`bench.py --copy-probe-mib M` into profiles/hbm_traffic_shms_1e7.json")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.events)
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(events shard trivially; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: this library has no multi-GPU path "
+              "(events are independent and shard by event range)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -97,6 +154,14 @@ def main():
     kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma,
               fit="fp64", out=out)
     stream = torch.cuda.current_stream(dev)
+    if args.copy_probe_mib:
+        m = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(m, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step(s):
         out["pathlen"].zero_()
@@ -107,25 +172,42 @@ def main():
     torch.cuda.synchronize()
     ev_start = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ev_stop = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    # stage boundaries of every timed step (recorded once here so that the
+    # underlying HIP events exist before their handles are passed down)
+    ev_stage = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    for evs in ev_stage:
+        for e in evs:
+            e.record(stream)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         out["pathlen"].zero_()
         ev_start[s].record(stream)
-        ctx.run(wl.arm, d_ev, rng="philox", seed=1234, event0=(args.warmup + s) * n, **kw)
+        ctx.run(wl.arm, d_ev, rng="philox", seed=1234, event0=(args.warmup + s) * n,
+                stage_events=ev_stage[s], **kw)
         ev_stop[s].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_start, ev_stop)]))
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_start, ev_stop)]))
+    stage_ms = [float(np.mean([evs[k].elapsed_time(evs[k + 1]) for evs in ev_stage])) for k in range(3)]
 
-    # algorithmic work of the last launch, from its stop codes
+    # algorithmic work of the last launch per kernel stage, from its stop codes
     stop = out["stop"].to(torch.int64)
     hist = torch.bincount(stop, minlength=256).cpu().numpy()
-    flops_by_stop = np.array(ctx.stop_flops(wl.arm), dtype=np.float64)
-    flops = float((hist * flops_by_stop).sum())
+    stage_flops = [float((hist * np.array(ctx.stop_flops(wl.arm, k + 1), dtype=np.float64)).sum())
+                   for k in range(3)]
+    names = STAGE_NAMES
+    stages = {names[k]: {"ms": stage_ms[k], "algorithmic_flop": stage_flops[k],
+                         "TFLOPs": stage_flops[k] / (stage_ms[k] * 1e-3) / 1e12} for k in range(3)}
+    dom = int(np.argmax(stage_ms))
     accepted = int(hist[0])
-    fit_events = int(out["ok"].sum().item())  # lower bound on events writing the fit
-    bytes_launch = n * (6 * 8 + 8) + n * (4 * 8 + 8 + 2) + fit_events * 4 * 8
-    tflops = flops / (kernel_ms * 1e-3) / 1e12
+    tflops = stage_flops[dom] / (stage_ms[dom] * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "hbm_traffic_shms_1e7.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        if tr.get("events_per_step") == n:
+            traffic = tr["kernels"].get(names[dom], {}).get("hbm_bytes_per_launch")
     value = args.steps * n / wall
     line = {
         "metric": "events/s transported+reconstructed (FP64, 1 GPU); batched & adjoint vs plain",
@@ -145,9 +227,11 @@ def main():
                    "wire_sigma_cm": wl.wire_sigma, "ms": True, "wcs": True, "fit": "fp64",
                    "rng": "philox4x32-10 per event"},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": kernel_ms, "algorithmic_flop_per_launch": flops,
-                     "hbm_GBps_algorithmic": bytes_launch / (kernel_ms * 1e-3) / 1e9},
+                     "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": names[dom], "kernel_ms": stage_ms[dom],
+                     "algorithmic_flop_per_launch": stage_flops[dom]},
+        "stages": stages,
+        "step_ms_hip_events": step_ms,
         "accepted_fraction": accepted / n,
     }
     if not args.no_cpu:

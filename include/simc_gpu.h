@@ -41,6 +41,7 @@ extern "C" {
 #define SIMC_E_MAPSTRUCT (-8) /* map terms differ from the compiled polynomial structure */
 #define SIMC_E_HIP (-9)       /* HIP runtime error                                   */
 #define SIMC_E_NOMEM (-10)
+#define SIMC_E_INTERNAL (-11) /* inconsistent step program (library defect)          */
 
 #define SIMC_ARM_HMS 1        /* spectrometer numbers of transp_init, transp.f:300-313 */
 #define SIMC_ARM_SHMS 5
@@ -111,13 +112,26 @@ void simc_ctx_destroy(simc_ctx *ctx);
 
 /* Transport + reconstruct n events through one arm (mc_hms / mc_shms).
  * stream: a hipStream_t (NULL = default stream).  Asynchronous: returns after
- * the launch. */
+ * the launch.  The context keeps one survivor queue per arm (about 100 bytes
+ * per event of a launch, at most 2^24 events, allocated on first use): runs of
+ * the same arm on one context must be ordered (same stream, or synchronised
+ * streams). */
 int simc_hms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 int simc_shms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
                   simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+
+/* simc_arm_run with stage timing (no counterpart in the reference: a
+ * measurement hook).  stage_events[0..3] are hipEvent_t handles created by
+ * the caller; they are recorded on `stream` before stage 1 (target to
+ * collimator), after stage 1, after stage 2 (the magnets) and after stage 3
+ * (detector hut, fit, reconstruction) of the first launch chunk (2^24
+ * events, fewer for long tapes). */
+int simc_arm_run_timed(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
+                       simc_events_out *out, int64_t n, const simc_rng *rng, void *stream,
+                       void *const *stage_events);
 
 /* Reconstruction only (mc_hms_recon / mc_shms_recon) of focal-plane tracks:
  * x_fp, y_fp in cm, slopes in rad, fry in cm.  Outputs delta (%), x'_tar,
@@ -143,6 +157,9 @@ int64_t simc_poly_flops_full(simc_ctx *ctx, int arm);
 /* The same count for an event that ends with stop code c, written to
  * flops[c] for c = 0..255 (nflops >= 256): the segments it passes through. */
 int simc_stop_flops(simc_ctx *ctx, int arm, int64_t *flops, int nflops);
+/* The same restricted to the segments of one kernel stage (1: target to
+ * collimator, 2: the magnets, 3: detector hut and reconstruction; 0: all). */
+int simc_stop_flops_stage(simc_ctx *ctx, int arm, int stage, int64_t *flops, int nflops);
 
 const char *simc_last_error(void);
 const char *simc_version(void);
@@ -154,6 +171,7 @@ const char *simc_version(void);
 #define SIMC_STOP_SLIT_OCT 3
 #define SIMC_STOP_COLL 4
 #define SIMC_STOP_TAPE_OVERFLOW 250
+#define SIMC_STOP_INTERNAL 251    /* a step outside its stage (cannot occur: checked at load) */
 
 #ifdef __cplusplus
 }

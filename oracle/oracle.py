@@ -93,6 +93,12 @@ def lib():
         L.or_recon_run.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 9
         L.or_pion_coll_absorb.restype = C.c_double
         L.or_pion_coll_absorb.argtypes = [C.c_double, C.c_double]
+        L.or_bm_log.restype = C.c_double
+        L.or_bm_log.argtypes = [C.c_double]
+        L.or_bm_sincos2pi.restype = None
+        L.or_bm_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_philox_normals.restype = C.c_int64
+        L.or_philox_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -207,3 +213,10 @@ def recon(arm, x_fp, dx_fp, y_fp, dy_fp, fry):
     if rc:
         raise RuntimeError(f"or_recon_run: {rc}")
     return tuple(outs)
+
+
+def philox_normals(seed, event, n, nsigmax=99.0):
+    """n normal deviates of one event's philox stream; returns (values, uniforms used)."""
+    out = np.zeros(n)
+    used = lib().or_philox_normals(seed, event, n, nsigmax, _p(out))
+    return out, used

@@ -481,9 +481,107 @@ double or_powi(double x, int m)
     return m < 0 ? 1 / y : y;
 }
 
-/* gauss1, gauss1.f:1-30 (Marsaglia polar; second normal discarded) */
+/* Elementary functions of the Box-Muller generator of the philox stream,
+ * the same sequence of IEEE operations as the GPU library (explicit fma,
+ * exact reductions), so both produce identical bits.
+ *
+ * or_bm_log: log u for u in (0, 1): u = m 2^e, m in [sqrt(1/2), sqrt(2)),
+ * log m = 2 atanh s, s = (m-1)/(m+1), as 2s + 2s (z/3 + ... + z^11/23),
+ * z = s^2; then e ln2 in two parts (ln2_hi has 21 significant bits). */
+double or_bm_log(double u)
+{
+    uint64_t b;
+    memcpy(&b, &u, 8);
+    int e = (int)(b >> 52) - 1023;
+    uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    memcpy(&m, &mb, 8);
+    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double q = 0x1.642c8590b2164p-4;
+    q = fma(z, q, 0x1.8618618618618p-4);
+    q = fma(z, q, 0x1.af286bca1af28p-4);
+    q = fma(z, q, 0x1.e1e1e1e1e1e1ep-4);
+    q = fma(z, q, 0x1.1111111111111p-3);
+    q = fma(z, q, 0x1.3b13b13b13b14p-3);
+    q = fma(z, q, 0x1.745d1745d1746p-3);
+    q = fma(z, q, 0x1.c71c71c71c71cp-3);
+    q = fma(z, q, 0x1.2492492492492p-2);
+    q = fma(z, q, 0x1.999999999999ap-2);
+    q = fma(z, q, 0x1.5555555555555p-1);
+    double t = z * q;
+    double logm = fma(s, t, 2.0 * s);
+    double de = (double)e;
+    return fma(de, 0x1.62e42p-1, fma(de, 0x1.fdf473de6af28p-22, logm));
+}
+
+/* sin and cos of 2 pi u, u in [0, 1): x = 4u, q = rint(x), r = x - q in
+ * [-1/2, 1/2]; Taylor series of sin(pi r/2), cos(pi r/2) in r^2, then the
+ * quarter-turn rotation by q. */
+void or_bm_sincos2pi(double u, double *sn, double *cs)
+{
+    double x = 4.0 * u;
+    double qd = nearbyint(x);
+    double r = x - qd;
+    double z = r * r;
+    double ps = 0x1.aaec32af93359p-38;
+    ps = fma(z, ps, -0x1.6fadb9f155744p-31);
+    ps = fma(z, ps, 0x1.e8f434d018d63p-25);
+    ps = fma(z, ps, -0x1.e3074fde8871fp-19);
+    ps = fma(z, ps, 0x1.50783487ee782p-13);
+    ps = fma(z, ps, -0x1.32d2cce62bd86p-8);
+    ps = fma(z, ps, 0x1.466bc6775aae2p-4);
+    ps = fma(z, ps, -0x1.4abbce625be53p-1);
+    ps = fma(z, ps, 0x1.921fb54442d18p+0);
+    double sr = r * ps;
+    double pc = 0x1.20c62c2f2d7f5p-34;
+    pc = fma(z, pc, -0x1.b6e24f44b128fp-28);
+    pc = fma(z, pc, 0x1.f9d38a3763cc3p-22);
+    pc = fma(z, pc, -0x1.a6d1f2a204a8cp-16);
+    pc = fma(z, pc, 0x1.e1f506891babbp-11);
+    pc = fma(z, pc, -0x1.55d3c7e3cbffap-6);
+    pc = fma(z, pc, 0x1.03c1f081b5ac4p-2);
+    pc = fma(z, pc, -0x1.3bd3cc9be45dep+0);
+    double cr = fma(z, pc, 1.0);
+    int q = (int)qd & 3;
+    double s0 = (q & 1) ? cr : sr;
+    double c0 = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -s0 : s0;
+    *cs = ((q + 1) & 2) ? -c0 : c0;
+}
+
+/* Normal deviates of the counter-based (philox) stream: Box-Muller pairs,
+ * both values used.  (u1, u2) -> sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2);
+ * the cosine value is returned first, the sine value on the next call.
+ * Values beyond nsigmax are skipped like gauss1 rejects them. */
+static double or_gauss_bm(or_rng *g, double nsigmax)
+{
+    for (;;) {
+        if (g->have) {
+            g->have = 0;
+            if (fabs(g->pending) <= nsigmax) return g->pending;
+            continue;
+        }
+        double u1 = or_grnd(g);
+        double u2 = or_grnd(g);
+        double r = sqrt(-2.0 * or_bm_log(u1));
+        double sn, cs;
+        or_bm_sincos2pi(u2, &sn, &cs);
+        g->pending = r * sn;
+        g->have = 1;
+        double g1 = r * cs;
+        if (fabs(g1) <= nsigmax) return g1;
+    }
+}
+
+/* gauss1, gauss1.f:1-30 (Marsaglia polar; second normal discarded).  The
+ * tape and RANLUX sources use it as the reference does; the philox source
+ * uses Box-Muller pairs (no rejection, so SIMD lanes stay in step). */
 double or_gauss1(or_rng *g, double nsigmax)
 {
+    if (g->kind == 1) return or_gauss_bm(g, nsigmax);
     for (;;) {
         double u1 = or_grnd(g);
         double u2 = or_grnd(g);
@@ -1898,6 +1996,18 @@ int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const
         ytar[i] = yt;
     }
     return OR_OK;
+}
+
+int64_t or_philox_normals(uint64_t seed, uint64_t event, int n, double nsigmax, double *out)
+{
+    or_rng g;
+    memset(&g, 0, sizeof g);
+    g.kind = 1;
+    g.key[0] = (uint32_t)seed;
+    g.key[1] = (uint32_t)(seed >> 32);
+    g.event = event;
+    for (int i = 0; i < n; ++i) out[i] = or_gauss1(&g, nsigmax);
+    return g.pos;
 }
 
 /* gauss1 on a uniform tape (unit tests): returns the draw, *used = uniforms read. */

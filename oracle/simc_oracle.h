@@ -67,6 +67,9 @@ typedef struct {
     /* bookkeeping */
     int64_t used;
     int overflow;
+    /* philox normal deviates: Box-Muller second value pending */
+    int have;
+    double pending;
 } or_rng;
 
 typedef struct or_ranlux {
@@ -161,6 +164,11 @@ double or_grnd(or_rng *g);
 /* ---- single routines (exported for unit tests) ---- */
 double or_powi(double x, int m);
 double or_gauss1(or_rng *g, double nsigmax);
+double or_bm_log(double u);
+void or_bm_sincos2pi(double u, double *sn, double *cs);
+/* n normal deviates of the philox stream of one event (unit tests);
+ * returns the uniforms consumed. */
+int64_t or_philox_normals(uint64_t seed, uint64_t event, int n, double nsigmax, double *out);
 void or_lfit_f32(const float *X, const float *Y, int L, int KEY, float *A, float *B, float *E);
 void or_lfit_f64(const double *X, const double *Y, int L, int KEY, double *A, double *B, double *E);
 void or_loren(double gam, double bx, double by, double bz, double e, double x, double y,

@@ -45,8 +45,9 @@ SHMS_COUNTERS = [
 # Every exported C entry point (include/simc_gpu.h).
 EXPORTS = [
     "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_destroy", "simc_hms_run",
-    "simc_shms_run", "simc_arm_run", "simc_recon_run", "simc_counters_get",
+    "simc_shms_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
+    "simc_stop_flops_stage",
     "simc_last_error",
     "simc_version",
 ]
@@ -102,6 +103,9 @@ def library(path: str = LIB_PATH):
     L.simc_ctx_destroy.restype = None
     L.simc_arm_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
                                C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp]
+    L.simc_arm_run_timed.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
+                                     C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp,
+                                     C.POINTER(_vp)]
     L.simc_recon_run.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9 + [_vp]
     L.simc_counters_get.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
     L.simc_counters_reset.argtypes = [_vp, C.c_int]
@@ -110,6 +114,7 @@ def library(path: str = LIB_PATH):
     L.simc_poly_flops_full.argtypes = [_vp, C.c_int]
     L.simc_poly_flops_full.restype = C.c_int64
     L.simc_stop_flops.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.simc_stop_flops_stage.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int]
     L.simc_last_error.restype = C.c_char_p
     L.simc_version.restype = C.c_char_p
     _lib = L
@@ -160,13 +165,15 @@ class Spectrometers:
     # -- transport + reconstruction -------------------------------------------------
     def run(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
             coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
-            tape=None, stride=0, out=None, stream=None):
+            tape=None, stride=0, out=None, stream=None, stage_events=None):
         """Transport + reconstruct events through one arm.
 
         events: dict of float64 device tensors dpp, x, y, dxdz, dydz (+ z, fry).
         Returns a dict of device tensors (see include/simc_gpu.h for semantics):
         dpp, y, dxdz, dydz (recon or input), x_fp, dx_fp, y_fp, dy_fp, pathlen,
-        resmult, m2, ok (uint8), stop (uint8), used (int32)."""
+        resmult, m2, ok (uint8), stop (uint8), used (int32).
+        stage_events: optional four recorded torch.cuda.Event(enable_timing=True)
+        marking the boundaries of the three kernel stages (first launch chunk)."""
         n = events["dpp"].numel()
         ev = {k: _dev_f64(events[k], n, k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
         for k in ("z", "fry"):
@@ -190,8 +197,15 @@ class Spectrometers:
         else:
             R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
         st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
-        _check(library().simc_arm_run(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
-                                      C.byref(R), C.c_void_p(st)))
+        if stage_events is None:
+            _check(library().simc_arm_run(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
+                                          C.byref(R), C.c_void_p(st)))
+        else:
+            if len(stage_events) != 4 or any(e.cuda_event == 0 for e in stage_events):
+                raise ValueError("stage_events: four events, each recorded once beforehand")
+            evs = (_vp * 4)(*[e.cuda_event for e in stage_events])
+            _check(library().simc_arm_run_timed(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout),
+                                                n, C.byref(R), C.c_void_p(st), evs))
         return out
 
     def recon(self, arm, x_fp, dx_fp, y_fp, dy_fp, fry=None, stream=None):
@@ -224,10 +238,11 @@ class Spectrometers:
     def poly_flops_full(self, arm):
         return int(library().simc_poly_flops_full(self._h, arm))
 
-    def stop_flops(self, arm):
-        """Algorithmic polynomial FP64 operations of an event, by stop code."""
+    def stop_flops(self, arm, stage=0):
+        """Algorithmic polynomial FP64 operations of an event, by stop code
+        (stage 0: the whole path; 1, 2, 3: one kernel stage)."""
         buf = (C.c_int64 * 256)()
-        _check(library().simc_stop_flops(self._h, arm, buf, 256))
+        _check(library().simc_stop_flops_stage(self._h, arm, stage, buf, 256))
         return list(buf)
 
 

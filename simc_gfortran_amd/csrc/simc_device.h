@@ -28,13 +28,17 @@ __device__ __forceinline__ double ld64(const double* p, uint32_t n, uint32_t off
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(p, n * 8u), off, 0, 0));
 }
+__device__ __forceinline__ uint32_t ld32(const uint32_t* p, uint32_t n, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(rsrc(p, n * 4u), off, 0, 0);
+}
 __device__ __forceinline__ void st64(double* p, uint32_t n, uint32_t off, double v)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rsrc(p, n * 8u), off, 0, 0);
 }
-__device__ __forceinline__ void st32(int32_t* p, uint32_t n, uint32_t off, int32_t v)
+__device__ __forceinline__ void st32(void* p, uint32_t n, uint32_t off, uint32_t v)
 {
-    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, rsrc(p, n * 4u), off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(v, rsrc(p, n * 4u), off, 0, 0);
 }
 __device__ __forceinline__ void st8(uint8_t* p, uint32_t n, uint32_t off, uint8_t v)
 {
@@ -75,16 +79,103 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// ---------------------------------------------------------------------------
+// Elementary functions of the Box-Muller normal generator.  Both are written
+// as fixed sequences of IEEE operations (explicit fma, exact reductions) so
+// that the host restatement produces bit-identical values.
+// ---------------------------------------------------------------------------
+
+// Natural logarithm of u in (0, 1) (normal numbers): u = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), summed as
+// 2s + 2s (z/3 + z^2/5 + ... + z^11/23), z = s^2 <= 0.0295 (truncation
+// below 2^-58 relative), then e ln2 in two parts (ln2_hi has 21 significant
+// bits, so e ln2_hi is exact).
+__device__ __forceinline__ double bm_log(double u)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, u);
+    int e = (int)(b >> 52) - 1023;
+    double m = __builtin_bit_cast(double, (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double q = 0x1.642c8590b2164p-4;             // 2/23
+    q = __builtin_fma(z, q, 0x1.8618618618618p-4);  // 2/21
+    q = __builtin_fma(z, q, 0x1.af286bca1af28p-4);  // 2/19
+    q = __builtin_fma(z, q, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+    q = __builtin_fma(z, q, 0x1.1111111111111p-3);  // 2/15
+    q = __builtin_fma(z, q, 0x1.3b13b13b13b14p-3);  // 2/13
+    q = __builtin_fma(z, q, 0x1.745d1745d1746p-3);  // 2/11
+    q = __builtin_fma(z, q, 0x1.c71c71c71c71cp-3);  // 2/9
+    q = __builtin_fma(z, q, 0x1.2492492492492p-2);  // 2/7
+    q = __builtin_fma(z, q, 0x1.999999999999ap-2);  // 2/5
+    q = __builtin_fma(z, q, 0x1.5555555555555p-1);  // 2/3
+    const double t = z * q;
+    const double logm = __builtin_fma(s, t, 2.0 * s);
+    const double de = (double)e;
+    return __builtin_fma(de, 0x1.62e42p-1, __builtin_fma(de, 0x1.fdf473de6af28p-22, logm));
+}
+
+// sin and cos of 2 pi u for u in [0, 1): x = 4u (exact), quadrant q =
+// rint(x), r = x - q in [-1/2, 1/2] (exact); sin(pi r/2) and cos(pi r/2) by
+// their Taylor series in r^2 (coefficients (pi/2)^k/k! rounded to double;
+// truncation below 2^-56), then the quarter-turn rotation by q.
+__device__ __forceinline__ void bm_sincos2pi(double u, double& sn, double& cs)
+{
+    const double x = 4.0 * u;
+    const double qd = rint(x);
+    const double r = x - qd;
+    const double z = r * r;
+    double ps = 0x1.aaec32af93359p-38;
+    ps = __builtin_fma(z, ps, -0x1.6fadb9f155744p-31);
+    ps = __builtin_fma(z, ps, 0x1.e8f434d018d63p-25);
+    ps = __builtin_fma(z, ps, -0x1.e3074fde8871fp-19);
+    ps = __builtin_fma(z, ps, 0x1.50783487ee782p-13);
+    ps = __builtin_fma(z, ps, -0x1.32d2cce62bd86p-8);
+    ps = __builtin_fma(z, ps, 0x1.466bc6775aae2p-4);
+    ps = __builtin_fma(z, ps, -0x1.4abbce625be53p-1);
+    ps = __builtin_fma(z, ps, 0x1.921fb54442d18p+0);
+    const double sr = r * ps;
+    double pc = 0x1.20c62c2f2d7f5p-34;
+    pc = __builtin_fma(z, pc, -0x1.b6e24f44b128fp-28);
+    pc = __builtin_fma(z, pc, 0x1.f9d38a3763cc3p-22);
+    pc = __builtin_fma(z, pc, -0x1.a6d1f2a204a8cp-16);
+    pc = __builtin_fma(z, pc, 0x1.e1f506891babbp-11);
+    pc = __builtin_fma(z, pc, -0x1.55d3c7e3cbffap-6);
+    pc = __builtin_fma(z, pc, 0x1.03c1f081b5ac4p-2);
+    pc = __builtin_fma(z, pc, -0x1.3bd3cc9be45dep+0);
+    const double cr = __builtin_fma(z, pc, 1.0);
+    const int q = (int)qd & 3;
+    const double s0 = (q & 1) ? cr : sr;
+    const double c0 = (q & 1) ? sr : cr;
+    sn = (q & 2) ? -s0 : s0;
+    cs = ((q + 1) & 2) ? -c0 : c0;
+}
+
 struct PhiloxStream {
     uint32_t k0, k1, e0, e1;
     uint32_t pos;
     uint32_t w[4];
     int overflow;
+    int have;        // a second Box-Muller normal is pending
+    double pending;
     __device__ void init(uint64_t seed, uint64_t event, const double*, uint32_t, uint32_t)
     {
         k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
         e0 = (uint32_t)event; e1 = (uint32_t)(event >> 32);
-        pos = 0; overflow = 0;
+        pos = 0; overflow = 0; have = 0; pending = 0.0;
+    }
+    // Stream position for a later resume: uniforms drawn, and the pending
+    // Box-Muller value.
+    __device__ __forceinline__ uint32_t state_word() const { return pos | ((uint32_t)have << 31); }
+    // Resume at a saved position (re-forms the current Philox block when the
+    // position is inside one).
+    __device__ __forceinline__ void resume(uint32_t word, double pend)
+    {
+        pos = word & 0x7fffffffu;
+        have = (int)(word >> 31);
+        pending = pend;
+        if (pos & 3u) philox4x32_10(pos >> 2, e0, e1, kPhiloxTag, k0, k1, w);
     }
     __device__ __forceinline__ double next()
     {
@@ -94,18 +185,51 @@ struct PhiloxStream {
         const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
         return ((double)v + 0.5) * 2.3283064365386963e-10;
     }
+    // Normal deviates for the counter-based stream: Box-Muller pairs
+    // (u1, u2) -> sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2), both used, the
+    // cosine value first.  There is no rejection step, so all lanes of a
+    // wavefront consume uniforms in step and every Philox block is formed once
+    // for the whole wavefront.  |g| <= sqrt(-2 ln 2^-33) = 6.8 always, below
+    // every truncation used; a value beyond nsigmax would be skipped.
+    __device__ __forceinline__ double gauss(double nsigmax)
+    {
+        for (;;) {
+            if (have) {
+                have = 0;
+                if (fabs(pending) <= nsigmax) return pending;
+                continue;
+            }
+            const double u1 = next();
+            const double u2 = next();
+            const double r = sqrt(-2.0 * bm_log(u1));
+            double sn, cs;
+            bm_sincos2pi(u2, sn, cs);
+            pending = r * sn;
+            have = 1;
+            const double g1 = r * cs;
+            if (fabs(g1) <= nsigmax) return g1;
+        }
+    }
     __device__ __forceinline__ int32_t used() const { return (int32_t)pos; }
 };
 
+template <class R>
+__device__ __forceinline__ double polar_gauss(R& g, double nsigmax);
+
 // Tape of the launch: event slot at byte offset `base`, `stride` bytes long.
+// Normal deviates use the reference's Marsaglia polar method (gauss1.f).
 struct TapeStream {
     const double* tape;
     uint32_t base, stride, pos;
     int overflow;
+    int have;          // unused (kept for a common state layout)
+    double pending;
     __device__ void init(uint64_t, uint64_t, const double* t, uint32_t slot_off, uint32_t s)
     {
-        tape = t; base = slot_off; stride = s; pos = 0; overflow = 0;
+        tape = t; base = slot_off; stride = s; pos = 0; overflow = 0; have = 0; pending = 0.0;
     }
+    __device__ __forceinline__ uint32_t state_word() const { return pos; }
+    __device__ __forceinline__ void resume(uint32_t word, double) { pos = word; }
     __device__ __forceinline__ double next()
     {
         if (pos >= stride) { overflow = 1; pos += 8u; return 0.5; }
@@ -114,6 +238,7 @@ struct TapeStream {
         pos += 8u;
         return v;
     }
+    __device__ __forceinline__ double gauss(double nsigmax) { return polar_gauss(*this, nsigmax); }
     __device__ __forceinline__ int32_t used() const { return (int32_t)(pos / 8u); }
 };
 
@@ -140,10 +265,10 @@ struct Rot {
 };
 
 // ---------------------------------------------------------------------------
-// gauss1, gauss1.f:17-27
+// gauss1, gauss1.f:17-27: Marsaglia polar method, second normal discarded.
 // ---------------------------------------------------------------------------
 template <class R>
-__device__ __forceinline__ double gauss1(R& g, double nsigmax)
+__device__ __forceinline__ double polar_gauss(R& g, double nsigmax)
 {
     for (;;) {
         const double u1 = g.next();
@@ -159,6 +284,14 @@ __device__ __forceinline__ double gauss1(R& g, double nsigmax)
     }
 }
 
+// A normal deviate from the event's stream (polar method on a tape,
+// Box-Muller pairs on the counter-based stream).
+template <class R>
+__device__ __forceinline__ double gauss1(R& g, double nsigmax)
+{
+    return g.gauss(nsigmax);
+}
+
 // theta_sigma of musc / musc_ext (shared/musc.f:109-112, shared/musc_ext.f:155-158)
 __device__ __forceinline__ double theta_sigma(double m2, double p, double rad_len)
 {
@@ -166,6 +299,27 @@ __device__ __forceinline__ double theta_sigma(double m2, double p, double rad_le
     const double beta = p / sqrt(m2 + p * p);
     return Es / p / beta * sqrt(rad_len) * (1.0 + epsilon * log10(rad_len / (beta * beta)));
 }
+
+// theta_sigma for the detector-hut steps, whose radiation lengths are
+// constants: sqrt(rad_len) and log10(rad_len) come with the step, the track
+// factors Es/p/beta and log10(beta^2) are formed once per momentum, and
+// log10(rad_len/beta^2) is taken as log10(rad_len) - log10(beta^2) (the same
+// value to a few units in the last place).
+struct ThetaCache {
+    double p = -1.0, m2 = -1.0, a = 0.0, lb = 0.0;
+    __device__ __forceinline__ double theta(const Track& t, double sqrt_rl, double log10_rl)
+    {
+        const double Es = 13.6, epsilon = 0.088;
+        if (t.p != p || t.m2 != m2) {
+            p = t.p;
+            m2 = t.m2;
+            const double beta = p / sqrt(m2 + p * p);
+            a = Es / p / beta;
+            lb = log10(beta * beta);
+        }
+        return a * sqrt_rl * (1.0 + epsilon * (log10_rl - lb));
+    }
+};
 
 // musc(m2, p, rad_len, dth, dph), shared/musc.f:93-118
 template <class R>

@@ -43,7 +43,7 @@ enum : int {
     ST_S_D1_FLR, ST_S_D1_MEN, ST_S_D1_MID1, ST_S_D1_MID2, ST_S_D1_MID3, ST_S_D1_MID4,
     ST_S_D1_MID5, ST_S_D1_MID6, ST_S_D1_MID7, ST_S_D1_MEX, ST_S_D1_OUT, ST_S_DC1, ST_S_DC2,
     ST_S_S1X, ST_S_S1Y, ST_S_S2X, ST_S_S2Y, ST_S_CAL, ST_S_CALFID,
-    ST_TAPE = 250
+    ST_TAPE = 250, ST_BADOP = 251
 };
 constexpr int kHistBins = 256;
 constexpr int kBlock = 256;  // threads per workgroup (one event each)
@@ -218,16 +218,19 @@ __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, con
 }
 
 // ---------------------------------------------------------------------------
-// The step-program interpreter: one event of arm ARM.
+// The step-program interpreter: one event of arm ARM, steps [pc_begin,
+// pc_end) of one stage.  The magnet stage's switch holds the drifts, COSY
+// segments, apertures and the collimator; the hut stage's switch the
+// materials, wire planes, fit and trigger.
 //
 // Only the track, the random stream, the tilted-plane point (xt, yt), resmult
 // and the scintillator count are carried from step to step.  Wire hits go to
 // LDS (hits[plane][lane]); the fitted focal-plane track is written to its
 // output arrays at the fit and read back where it is used again.
 // ---------------------------------------------------------------------------
-template <int ARM, class R, bool RARE>
-__device__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off, double* lds_hits,
-                           unsigned* coll_cnt, double& resmult)
+template <int ARM, class R, bool RARE, int STAGE>
+__device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off, double* lds_hits,
+                           unsigned* coll_cnt, double& resmult, int pc_begin, int pc_end)
 {
     // The in-flight decay and the collimator stepping of pions/muons are
     // compiled only into the RARE variant, so the common variant does not
@@ -243,141 +246,155 @@ __device__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint
     double xt = 0.0, yt = 0.0;
     int scincount = 0;
     int st = ST_OK;
-    for (int pc = 0;; ++pc) {
+    ThetaCache ms;
+    for (int pc = pc_begin; pc < pc_end; ++pc) {
         const int code = ops[pc].code;
         if (code == OP_END) break;
         const double p0 = ops[pc].p[0], p1 = ops[pc].p[1];
-        // Gaussian draws of this step, in call order (one gauss1 instance
-        // serves musc, musc_ext and the wire smearing).
-        int ng = 0;
-        if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
-        else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
-        else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-        for (int q = 0; q < ng; ++q) {
-            const double v = gauss1(g, 99.0);
-            d0 = d1; d1 = d2; d2 = d3; d3 = v;
-        }
-        switch (code) {
-        case OP_PROJECT:
-            project(g, t, F, p0);
-            break;
-        case OP_TRANSP: {
-            const int cls = ops[pc].arg;
-            transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
-                                    double& o1, double& o2, double& o3, double& o4) {
-                poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
-            });
-            break;
-        }
-        case OP_CHECK_R2:
-            if ((t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
-            break;
-        case OP_ROT_H:
-        case OP_ROT_V: {
-            const Rot r{p0, p1, ops[pc].p[2]};
-            xt = t.xs;
-            yt = t.ys;
-            if (code == OP_ROT_H) rotate_haxis(t, r, xt, yt);
-            else rotate_vaxis(t, r, xt, yt);
-            xt = xt + ops[pc].p[3];
-            yt = yt + ops[pc].p[4];
-            break;
-        }
-        case OP_CHECK_T_R2:
-            if ((xt * xt + yt * yt) > p0) st = ops[pc].stop;
-            break;
-        case OP_CHECK_T_HB:
-            if ((xt * xt > p0) || (yt > p1) || (yt < ops[pc].p[2])) st = ops[pc].stop;
-            break;
-        case OP_HIT_DIPOLE_T:
-            if (hms_hit_dipole(xt, yt)) st = ops[pc].stop;
-            break;
-        case OP_PIPE_T: {
-            const double dx = xt - p1, dy = yt - ops[pc].p[2];
-            if (((dx * dx + dy * dy) > p0) || (fabs(dy) > ops[pc].p[3])) st = ops[pc].stop;
-            break;
-        }
-        case OP_PIPE_S: {
-            const double dx = t.xs - p1, dy = t.ys - ops[pc].p[2];
-            if ((dx * dx + dy * dy) > p0) st = ops[pc].stop;
-            break;
-        }
-        case OP_COLLIMATOR:
-            if (F.coll && t.m2 > 100.0 * 100.0 && t.m2 < 200.0 * 200.0) {
-                CollGeom cg;
-                cg.h_entr = tab->coll.h_entr; cg.v_entr = tab->coll.v_entr;
-                cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
-                cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
-                cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
-                if (!coll_step(g, t, F, cg, a.p_spec, coll_cnt)) st = ST_COLL;
-            } else {
-                const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
-                int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
-                if (!s) {
-                    project(g, t, F, ops[pc].p[6]);
-                    s = slit(t.xs, t.ys, ops[pc].p[2], ops[pc].p[3], x_off, y_off);
+        if constexpr (STAGE == STAGE_MAGNETS) {
+            switch (code) {
+            case OP_PROJECT:
+                project(g, t, F, p0);
+                break;
+            case OP_TRANSP: {
+                const int cls = ops[pc].arg;
+                transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
+                                        double& o1, double& o2, double& o3, double& o4) {
+                    poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
+                });
+                break;
+            }
+            case OP_CHECK_R2:
+                if ((t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
+                break;
+            case OP_ROT_H:
+            case OP_ROT_V: {
+                const Rot r{p0, p1, ops[pc].p[2]};
+                xt = t.xs;
+                yt = t.ys;
+                if (code == OP_ROT_H) rotate_haxis(t, r, xt, yt);
+                else rotate_vaxis(t, r, xt, yt);
+                xt = xt + ops[pc].p[3];
+                yt = yt + ops[pc].p[4];
+                break;
+            }
+            case OP_CHECK_T_R2:
+                if ((xt * xt + yt * yt) > p0) st = ops[pc].stop;
+                break;
+            case OP_CHECK_T_HB:
+                if ((xt * xt > p0) || (yt > p1) || (yt < ops[pc].p[2])) st = ops[pc].stop;
+                break;
+            case OP_HIT_DIPOLE_T:
+                if (hms_hit_dipole(xt, yt)) st = ops[pc].stop;
+                break;
+            case OP_PIPE_T: {
+                const double dx = xt - p1, dy = yt - ops[pc].p[2];
+                if (((dx * dx + dy * dy) > p0) || (fabs(dy) > ops[pc].p[3])) st = ops[pc].stop;
+                break;
+            }
+            case OP_PIPE_S: {
+                const double dx = t.xs - p1, dy = t.ys - ops[pc].p[2];
+                if ((dx * dx + dy * dy) > p0) st = ops[pc].stop;
+                break;
+            }
+            case OP_COLLIMATOR:
+                if (F.coll && t.m2 > 100.0 * 100.0 && t.m2 < 200.0 * 200.0) {
+                    CollGeom cg;
+                    cg.h_entr = tab->coll.h_entr; cg.v_entr = tab->coll.v_entr;
+                    cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
+                    cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
+                    cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
+                    if (!coll_step(g, t, F, cg, a.p_spec, coll_cnt)) st = ST_COLL;
+                } else {
+                    const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
+                    int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
+                    if (!s) {
+                        project(g, t, F, ops[pc].p[6]);
+                        s = slit(t.xs, t.ys, ops[pc].p[2], ops[pc].p[3], x_off, y_off);
+                    }
+                    st = s;
                 }
-                st = s;
+                break;
+            default:
+                st = ST_BADOP;  // not a magnet step (excluded by the host's check)
+                break;
             }
-            break;
-        case OP_RESMULT_DRAW:
-            resmult = g.next() < p0 ? 2.0 : 1.0;
-            break;
-        case OP_RESMULT_ONE:
-            resmult = 1.0;
-            break;
-        case OP_MUSC:
-            if (ng) {  // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
-                const double ts = theta_sigma(t.m2, t.p, p0);
-                t.dydzs = t.dydzs + ts * d2;
-                t.dxdzs = t.dxdzs + ts * d3;
+        } else {
+            // Gaussian draws of this step, in call order (one gauss1 instance
+            // serves musc, musc_ext and the wire smearing).
+            int ng = 0;
+            if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
+            else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
+            else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
+            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+            for (int q = 0; q < ng; ++q) {
+                const double v = gauss1(g, 99.0);
+                d0 = d1; d1 = d2; d2 = d3; d3 = v;
             }
-            break;
-        case OP_MUSC_EXT:
-            if (ng) {  // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
-                const double ts = theta_sigma(t.m2, t.p, p0);
-                const double s12 = 3.4641016151377544;  // sqrt(12.)
-                t.dxdzs = t.dxdzs + ts * d0;
-                t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
-                t.dydzs = t.dydzs + ts * d2;
-                t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
+            switch (code) {
+            case OP_PROJECT:
+                project(g, t, F, p0);
+                break;
+            case OP_RESMULT_DRAW:
+                resmult = g.next() < p0 ? 2.0 : 1.0;
+                break;
+            case OP_RESMULT_ONE:
+                resmult = 1.0;
+                break;
+            case OP_MUSC:
+                if (ng) {  // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
+                    const double ts = ms.theta(t, p1, ops[pc].p[2]);
+                    t.dydzs = t.dydzs + ts * d2;
+                    t.dxdzs = t.dxdzs + ts * d3;
+                }
+                break;
+            case OP_MUSC_EXT:
+                if (ng) {  // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
+                    const double ts = ms.theta(t, ops[pc].p[2], ops[pc].p[3]);
+                    const double s12 = 3.4641016151377544;  // sqrt(12.)
+                    t.dxdzs = t.dxdzs + ts * d0;
+                    t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
+                    t.dydzs = t.dydzs + ts * d2;
+                    t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
+                }
+                break;
+            case OP_WIRE: {  // plane ops[pc].stop (0-based); y plane when arg = 1
+                const double h = ops[pc].arg ? t.ys + a.sigma * d3 * resmult : t.xs + a.sigma * d2 * resmult;
+                lds_hits[ops[pc].stop * kBlock + threadIdx.x] = h;
+                break;
             }
-            break;
-        case OP_WIRE: {  // plane ops[pc].stop (0-based); y plane when arg = 1
-            const double h = ops[pc].arg ? t.ys + a.sigma * d3 * resmult : t.xs + a.sigma * d2 * resmult;
-            lds_hits[ops[pc].stop * kBlock + threadIdx.x] = h;
-            break;
-        }
-        case OP_BOX:
-            if (t.xs > p0 || t.xs < p1 || t.ys > ops[pc].p[2] || t.ys < ops[pc].p[3]) st = ops[pc].stop;
-            break;
-        case OP_SCIN:
-            if (t.ys < p0 && t.ys > p1 && t.xs < ops[pc].p[2] && t.xs > ops[pc].p[3]) ++scincount;
-            break;
-        case OP_TRIGGER:
-            if (scincount < ops[pc].arg) st = ops[pc].stop;
-            break;
-        case OP_FIT: {
-            double zdc[12];
+            case OP_BOX:
+                if (t.xs > p0 || t.xs < p1 || t.ys > ops[pc].p[2] || t.ys < ops[pc].p[3]) st = ops[pc].stop;
+                break;
+            case OP_SCIN:
+                if (t.ys < p0 && t.ys > p1 && t.xs < ops[pc].p[2] && t.xs > ops[pc].p[3]) ++scincount;
+                break;
+            case OP_TRIGGER:
+                if (scincount < ops[pc].arg) st = ops[pc].stop;
+                break;
+            case OP_FIT: {
+                double zdc[12];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
-            double x_fp, dx_fp, y_fp, dy_fp;
-            fit_track(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
-            st64(a.x_fp, a.count, off, x_fp);
-            st64(a.dx_fp, a.count, off, dx_fp);
-            st64(a.y_fp, a.count, off, y_fp);
-            st64(a.dy_fp, a.count, off, dy_fp);
-            break;
-        }
-        case OP_FIDUCIAL: {
-            const double xcal = ld64(a.x_fp, a.count, off) + ld64(a.dx_fp, a.count, off) * p0;
-            const double ycal = ld64(a.y_fp, a.count, off) + ld64(a.dy_fp, a.count, off) * p0;
-            if (ycal > p1 || ycal < ops[pc].p[2] || xcal > ops[pc].p[3] || xcal < ops[pc].p[4]) st = ops[pc].stop;
-            break;
-        }
-        default:
-            break;
+                for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
+                double x_fp, dx_fp, y_fp, dy_fp;
+                fit_track(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
+                st64(a.x_fp, a.count, off, x_fp);
+                st64(a.dx_fp, a.count, off, dx_fp);
+                st64(a.y_fp, a.count, off, y_fp);
+                st64(a.dy_fp, a.count, off, dy_fp);
+                break;
+            }
+            case OP_FIDUCIAL: {
+                const double xcal = ld64(a.x_fp, a.count, off) + ld64(a.dx_fp, a.count, off) * p0;
+                const double ycal = ld64(a.y_fp, a.count, off) + ld64(a.dy_fp, a.count, off) * p0;
+                if (ycal > p1 || ycal < ops[pc].p[2] || xcal > ops[pc].p[3] || xcal < ops[pc].p[4])
+                    st = ops[pc].stop;
+                break;
+            }
+            default:
+                st = ST_BADOP;  // not a hut step (excluded by the host's check)
+                break;
+            }
         }
         if (st) return st;
     }
@@ -385,76 +402,190 @@ __device__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint
 }
 
 // ---------------------------------------------------------------------------
-// Kernel: one event per lane; LDS stop-code histogram per workgroup.
+// Kernels.  A launch runs in three stages, each over full wavefronts of the
+// events still in flight:
+//   1  target -> collimator (program steps [0, coll_pc)), from the inputs;
+//   2  the magnets (steps [coll_pc, hut_pc)), from queue 1;
+//   3  the detector hut, track fit and reconstruction (steps [hut_pc, end)),
+//      from queue 2.
+// Events stopped in stage 1 or 2 are finished there; survivors are appended
+// to the next queue (track, random-stream position) in wavefront order.
+// Every stage writes outputs at the event's own index, so results do not
+// depend on queue order.  Each workgroup histograms stop codes in LDS and
+// adds its non-empty bins to the context's device counters.
 // ---------------------------------------------------------------------------
-template <int ARM, class R, bool RARE>
-__global__ __launch_bounds__(kBlock) void arm_kernel(KArgs a)
+
+// Survivor queue (structure of arrays, capacity `cap` entries).
+struct Queue {
+    uint32_t* count;     // entries appended (device counter, zeroed per launch)
+    uint32_t *idx, *rng, *bits;   // event index in the launch, stream word, dflag
+    double *pend, *xs, *ys, *dxdzs, *dydzs, *dpps, *p, *m2, *pathlen, *decdist, *resmult;
+    uint32_t cap;
+};
+
+__device__ __forceinline__ Flags flags_of(const KArgs& a)
+{
+    Flags F;
+    F.ctau = a.ctau;
+    F.ms = (a.flags & KF_MS) != 0;
+    F.wcs = (a.flags & KF_WCS) != 0;
+    F.decay = (a.flags & KF_DECAY) != 0;
+    F.coll = (a.flags & KF_COLL) != 0;
+    F.fit_f32 = (a.flags & KF_FIT_F32) != 0;
+    F.wire_sigma = a.wire_sigma;
+    return F;
+}
+
+// Outputs of a finished event li (byte offset off = 8 li).
+template <int ARM, class R>
+__device__ __forceinline__ void finish_event(const KArgs& a, uint32_t li, int st, const R& g, const Track& t,
+                                             double resmult)
+{
+    const uint32_t n = a.count, off = li * 8u;
+    if (st == ST_OK) {
+        EvOut o, r;
+        o.x_fp = ld64(a.x_fp, n, off);
+        o.dx_fp = ld64(a.dx_fp, n, off);
+        o.y_fp = ld64(a.y_fp, n, off);
+        o.dy_fp = ld64(a.dy_fp, n, off);
+        recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
+        st64(a.o_dpp, n, off, r.dpp);
+        st64(a.o_y, n, off, r.y);
+        st64(a.o_dxdz, n, off, r.dxdz);
+        st64(a.o_dydz, n, off, r.dydz);
+    } else {
+        // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440)
+        st64(a.o_dpp, n, off, ld64(a.dpp, n, off));
+        st64(a.o_y, n, off, ld64(a.y, n, off));
+        st64(a.o_dxdz, n, off, ld64(a.dxdz, n, off));
+        st64(a.o_dydz, n, off, ld64(a.dydz, n, off));
+    }
+    st64(a.pathlen, n, off, t.pathlen);
+    if (a.resmult) st64(a.resmult, n, off, resmult);
+    if (a.m2o) st64(a.m2o, n, off, t.m2);
+    if (a.used) st32(a.used, n, li * 4u, (uint32_t)g.used());
+    st8(a.ok, n, li, st == ST_OK ? 1 : 0);
+    st8(a.stop, n, li, (uint8_t)st);
+}
+
+// Stage kernel.  FROM_Q: events come from queue qin (else from the inputs);
+// TO_Q: survivors go to queue qout (else every event is finished here).
+// The common variants are held to 168 VGPRs (three wavefronts per SIMD),
+// which the hut stage would exceed slightly on its own.
+template <int ARM, class R, bool RARE, int STAGE, bool FROM_Q, bool TO_Q>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : 3))) void stage_kernel(KArgs a, Queue qin, Queue qout, int32_t pc_begin,
+                                                       int32_t pc_end)
 {
     __shared__ unsigned hist[kHistSlots];
-    __shared__ double lds_hits[12 * kBlock];
+    __shared__ uint32_t wave_n[kBlock / 64];
+    __shared__ uint32_t block_base;
+    __shared__ double lds_hits[STAGE == STAGE_HUT ? 12 * kBlock : 1];
+    uint32_t n_in = a.count;
+    if (FROM_Q) {
+        n_in = *qin.count;  // complete: written by the previous launch
+        if (blockIdx.x * kBlock >= n_in) return;  // whole workgroup past the queue
+    }
     for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x) hist[k] = 0;
     __syncthreads();
     unsigned coll_cnt[3] = {0, 0, 0};
-    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
-    if (li < a.count) {
-        const uint32_t off = li * 8u;
-        const uint32_t n = a.count;
-        Flags F;
-        F.ctau = a.ctau;
-        F.ms = (a.flags & KF_MS) != 0;
-        F.wcs = (a.flags & KF_WCS) != 0;
-        F.decay = (a.flags & KF_DECAY) != 0;
-        F.coll = (a.flags & KF_COLL) != 0;
-        F.fit_f32 = (a.flags & KF_FIT_F32) != 0;
-        F.wire_sigma = a.wire_sigma;
-        R g;
-        g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
-        Track t;
-        t.xs = ld64(a.x, n, off);
-        t.ys = ld64(a.y, n, off);
-        t.dxdzs = ld64(a.dxdz, n, off);
-        t.dydzs = ld64(a.dydz, n, off);
-        t.dpps = ld64(a.dpp, n, off);
-        t.p = a.p_spec * (1.0 + t.dpps / 100.0);
-        t.m2 = a.m2;
-        t.pathlen = ld64(a.pathlen, n, off);
-        t.decdist = 0.0;
-        t.dflag = 0;
-        double resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
-        int st = run_program<ARM, R, RARE>(g, t, a, F, off, lds_hits, coll_cnt, resmult);
-        if (g.overflow) st = ST_TAPE;
-        if (st == ST_OK) {
-            EvOut o, r;
-            o.x_fp = ld64(a.x_fp, n, off);
-            o.dx_fp = ld64(a.dx_fp, n, off);
-            o.y_fp = ld64(a.y_fp, n, off);
-            o.dy_fp = ld64(a.dy_fp, n, off);
-            recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
-            st64(a.o_dpp, n, off, r.dpp);
-            st64(a.o_y, n, off, r.y);
-            st64(a.o_dxdz, n, off, r.dxdz);
-            st64(a.o_dydz, n, off, r.dydz);
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    bool survivor = false;
+    uint32_t li = 0;
+    R g;
+    Track t;
+    double resmult = 0.0;
+    if (i < n_in) {
+        const Flags F = flags_of(a);
+        if (FROM_Q) {
+            const uint32_t c = qin.cap, o8 = i * 8u, o4 = i * 4u;
+            li = ld32(qin.idx, c, o4);
+            g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
+            g.resume(ld32(qin.rng, c, o4), ld64(qin.pend, c, o8));
+            t.xs = ld64(qin.xs, c, o8);
+            t.ys = ld64(qin.ys, c, o8);
+            t.dxdzs = ld64(qin.dxdzs, c, o8);
+            t.dydzs = ld64(qin.dydzs, c, o8);
+            t.dpps = ld64(qin.dpps, c, o8);
+            t.p = ld64(qin.p, c, o8);
+            t.m2 = ld64(qin.m2, c, o8);
+            t.pathlen = ld64(qin.pathlen, c, o8);
+            t.decdist = ld64(qin.decdist, c, o8);
+            t.dflag = (int)ld32(qin.bits, c, o4);
+            resmult = ld64(qin.resmult, c, o8);
         } else {
-            // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440)
-            st64(a.o_dpp, n, off, ld64(a.dpp, n, off));
-            st64(a.o_y, n, off, ld64(a.y, n, off));
-            st64(a.o_dxdz, n, off, ld64(a.dxdz, n, off));
-            st64(a.o_dydz, n, off, ld64(a.dydz, n, off));
+            li = i;
+            const uint32_t n = a.count, off = li * 8u;
+            g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
+            t.xs = ld64(a.x, n, off);
+            t.ys = ld64(a.y, n, off);
+            t.dxdzs = ld64(a.dxdz, n, off);
+            t.dydzs = ld64(a.dydz, n, off);
+            t.dpps = ld64(a.dpp, n, off);
+            t.p = a.p_spec * (1.0 + t.dpps / 100.0);
+            t.m2 = a.m2;
+            t.pathlen = ld64(a.pathlen, n, off);
+            t.decdist = 0.0;
+            t.dflag = 0;
+            resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
         }
-        st64(a.pathlen, n, off, t.pathlen);
-        if (a.resmult) st64(a.resmult, n, off, resmult);
-        if (a.m2o) st64(a.m2o, n, off, t.m2);
-        if (a.used) st32(a.used, n, li * 4u, g.used());
-        st8(a.ok, n, li, st == ST_OK ? 1 : 0);
-        st8(a.stop, n, li, (uint8_t)st);
-        atomicAdd(&hist[st], 1u);
+        int st = run_program<ARM, R, RARE, STAGE>(g, t, a, F, li * 8u, lds_hits, coll_cnt, resmult, pc_begin,
+                                                   pc_end);
+        if (g.overflow) st = ST_TAPE;
+        if (TO_Q && st == ST_OK) {
+            survivor = true;
+        } else {
+            finish_event<ARM>(a, li, st, g, t, resmult);
+            atomicAdd(&hist[st], 1u);
+        }
     }
     if (coll_cnt[0]) atomicAdd(&hist[kHistBins + 0], coll_cnt[0]);
     if (coll_cnt[1]) atomicAdd(&hist[kHistBins + 1], coll_cnt[1]);
     if (coll_cnt[2]) atomicAdd(&hist[kHistBins + 2], coll_cnt[2]);
+
+    if (TO_Q) {
+        // Queue slots: rank within the wavefront, wavefront offset within
+        // the workgroup, one counter update per workgroup.
+        const uint64_t mask = __ballot(survivor);
+        const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_n[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kBlock / 64; ++w) tot += wave_n[w];
+            block_base = tot ? atomicAdd(qout.count, tot) : 0u;
+        }
+        __syncthreads();
+        if (survivor) {
+            uint32_t slot = block_base + rank;
+            for (int w = 0; w < wave; ++w) slot += wave_n[w];
+            const uint32_t c = qout.cap, o8 = slot * 8u, o4 = slot * 4u;
+            st32(qout.idx, c, o4, li);
+            st32(qout.rng, c, o4, g.state_word());
+            st32(qout.bits, c, o4, (uint32_t)t.dflag);
+            st64(qout.pend, c, o8, g.pending);
+            st64(qout.xs, c, o8, t.xs);
+            st64(qout.ys, c, o8, t.ys);
+            st64(qout.dxdzs, c, o8, t.dxdzs);
+            st64(qout.dydzs, c, o8, t.dydzs);
+            st64(qout.dpps, c, o8, t.dpps);
+            st64(qout.p, c, o8, t.p);
+            st64(qout.m2, c, o8, t.m2);
+            st64(qout.pathlen, c, o8, t.pathlen);
+            st64(qout.decdist, c, o8, t.decdist);
+            st64(qout.resmult, c, o8, resmult);
+        }
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x)
         if (hist[k]) atomicAdd(&a.hist[k], (unsigned long long)hist[k]);
+}
+
+// Queue counter reset (a kernel rather than a memset, so that a captured
+// launch sequence replays correctly).
+__global__ void zero_u32x2(uint32_t* p)
+{
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
 }
 
 template <int ARM>
@@ -502,6 +633,13 @@ struct simc_ctx {
     int n_ops[2] = {0, 0};
     double zdc[2][12];
     unsigned long long* d_hist[2] = {nullptr, nullptr};
+    int coll_pc[2] = {0, 0};         // first step after the collimator
+    int hut_pc[2] = {0, 0};          // first step of the detector hut
+    // survivor queues between the stages (per arm; grown on demand)
+    void* d_queue[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t* d_qcount[2] = {nullptr, nullptr};   // the two queue counters
+    uint32_t q_cap[2] = {0, 0};
+    std::mutex mu;                   // guards queue growth
     int device = 0;
     int n_cu = 256;
 };
@@ -523,6 +661,13 @@ struct Program {
         o.arg = arg;
         int k = 0;
         for (double v : p) o.p[k++] = v;
+        // multiple scattering steps also carry sqrt(rad_len) and log10(rad_len)
+        if (code == OP_MUSC || code == OP_MUSC_EXT) {
+            const double rl = o.p[0];
+            const int at = code == OP_MUSC ? 1 : 2;
+            o.p[at] = rl > 0.0 ? std::sqrt(rl) : 0.0;
+            o.p[at + 1] = rl > 0.0 ? std::log10(rl) : 0.0;
+        }
         ops.push_back(o);
         return ops.back();
     }
@@ -876,6 +1021,19 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     else build_shms(P, ctx->zdc[ai]);
     ctx->n_ops[ai] = (int)P.ops.size();
     ctx->ops[ai] = P.ops;
+    // Split point of the two stages; every step must belong to its stage.
+    int hut = 0;
+    while (hut < (int)P.ops.size() && op_stages(P.ops[hut].code) != STAGE_HUT) ++hut;
+    for (int k = 0; k < (int)P.ops.size(); ++k) {
+        const int want = k < hut ? STAGE_MAGNETS : STAGE_HUT;
+        if (!(op_stages(P.ops[k].code) & want))
+            return fail(SIMC_E_INTERNAL, "step " + std::to_string(k) + " outside its stage");
+    }
+    ctx->hut_pc[ai] = hut;
+    int coll = 0;
+    while (coll < hut && P.ops[coll].code != OP_COLLIMATOR) ++coll;
+    if (coll == hut) return fail(SIMC_E_INTERNAL, "no collimator step before the hut");
+    ctx->coll_pc[ai] = coll + 1;
     HIPCHK(hipMalloc(&ctx->d_coef[ai], packed.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_coef[ai], packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&ctx->d_ops[ai], P.ops.size() * sizeof(Op)));
@@ -885,6 +1043,44 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     HIPCHK(hipMemcpy(ctx->d_tab[ai], &T, sizeof(ArmTables), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&ctx->d_hist[ai], kHistSlots * sizeof(unsigned long long)));
     HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
+    return SIMC_OK;
+}
+
+// The two survivor queues of arm index ai, room for `cap` entries each.
+constexpr size_t kQueueEntryBytes = 11 * 8 + 3 * 4;
+
+void queue_view(void* base, uint32_t c, uint32_t* count, Queue& q)
+{
+    double* d = (double*)base;
+    double** dp[11] = {&q.pend, &q.xs, &q.ys, &q.dxdzs, &q.dydzs, &q.dpps, &q.p, &q.m2, &q.pathlen,
+                       &q.decdist, &q.resmult};
+    for (int k = 0; k < 11; ++k) *dp[k] = d + (size_t)k * c;
+    uint32_t* u = (uint32_t*)(d + (size_t)11 * c);
+    q.idx = u;
+    q.rng = u + c;
+    q.bits = u + 2 * (size_t)c;
+    q.count = count;
+    q.cap = c;
+}
+
+int ensure_queues(simc_ctx* ctx, int ai, uint32_t cap, Queue& q1, Queue& q2)
+{
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->d_qcount[ai]) HIPCHK(hipMalloc(&ctx->d_qcount[ai], 256));
+    if (ctx->q_cap[ai] < cap) {
+        if (ctx->d_queue[ai][0] || ctx->d_queue[ai][1]) {
+            HIPCHK(hipDeviceSynchronize());  // earlier launches may still use them
+            for (int j = 0; j < 2; ++j) {
+                if (ctx->d_queue[ai][j]) HIPCHK(hipFree(ctx->d_queue[ai][j]));
+                ctx->d_queue[ai][j] = nullptr;
+            }
+            ctx->q_cap[ai] = 0;
+        }
+        for (int j = 0; j < 2; ++j) HIPCHK(hipMalloc(&ctx->d_queue[ai][j], (size_t)cap * kQueueEntryBytes));
+        ctx->q_cap[ai] = cap;
+    }
+    queue_view(ctx->d_queue[ai][0], ctx->q_cap[ai], ctx->d_qcount[ai], q1);
+    queue_view(ctx->d_queue[ai][1], ctx->q_cap[ai], ctx->d_qcount[ai] + 1, q2);
     return SIMC_OK;
 }
 
@@ -908,8 +1104,11 @@ int simc_ctx_create_files(const char* hms_forward, const char* hms_recon, const 
         delete ctx;
         return fail(SIMC_E_HIP, "no HIP device available");
     }
-    hipGetDevice(&ctx->device);
-    hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (hipGetDevice(&ctx->device) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
+        delete ctx;
+        return fail(SIMC_E_HIP, "cannot query the current HIP device");
+    }
     int rc = load_arm(ctx, SIMC_ARM_HMS, hms_forward, hms_recon);
     if (!rc) rc = load_arm(ctx, SIMC_ARM_SHMS, shms_forward, shms_recon);
     if (rc) {
@@ -938,12 +1137,19 @@ void simc_ctx_destroy(simc_ctx* ctx)
         if (ctx->d_ops[i]) (void)hipFree(ctx->d_ops[i]);
         if (ctx->d_tab[i]) (void)hipFree(ctx->d_tab[i]);
         if (ctx->d_hist[i]) (void)hipFree(ctx->d_hist[i]);
+        for (int j = 0; j < 2; ++j)
+            if (ctx->d_queue[i][j]) (void)hipFree(ctx->d_queue[i][j]);
+        if (ctx->d_qcount[i]) (void)hipFree(ctx->d_qcount[i]);
     }
     delete ctx;
 }
 
-int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
-                 simc_events_out* out, int64_t n, const simc_rng* rng, void* stream)
+}  // extern "C"
+
+namespace {
+
+int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+            int64_t n, const simc_rng* rng, void* stream, void* const* stage_events)
 {
     if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run: NULL argument");
     const int ai = arm_index(arm);
@@ -979,8 +1185,9 @@ int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_even
     // Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
     // shms/mc_shms.f:503); m2 changes only through a decay.
     const bool rare = s->decay_flag || (s->using_coll && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
-    // Launches are cut so that every byte offset of a launch fits 32 bits.
-    int64_t chunk = (int64_t)1 << 26;
+    // Launches are cut so that every byte offset of a launch fits 32 bits
+    // and each survivor queue stays at most 2^24 entries (1.7 GB).
+    int64_t chunk = (int64_t)1 << 24;
     if (tape) {
         const int64_t sb = rng->stride * 8;
         if (sb > 0x7fffffffLL) return fail(SIMC_E_ARG, "tape stride too large");
@@ -988,6 +1195,10 @@ int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_even
         if (c < chunk) chunk = c;
         a.stride_bytes = (uint32_t)sb;
     }
+    Queue q1, q2;
+    int rc = ensure_queues(ctx, ai, (uint32_t)(n < chunk ? n : chunk), q1, q2);
+    if (rc) return rc;
+    const int32_t coll = ctx->coll_pc[ai], hut = ctx->hut_pc[ai];
     hipStream_t st = (hipStream_t)stream;
     for (int64_t b = 0; b < n; b += chunk) {
         const int64_t cnt = n - b < chunk ? n - b : chunk;
@@ -1006,18 +1217,52 @@ int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_even
         a.tape = tape ? rng->tape + b * rng->stride : nullptr;
         const unsigned blocks = (unsigned)((cnt + kBlock - 1) / kBlock);
         const dim3 G(blocks), B(kBlock);
-#define LAUNCH(ARMV, RV, RARE) hipLaunchKernelGGL((arm_kernel<ARMV, RV, RARE>), G, B, 0, st, a)
+        hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, q1.count);
+        // stage_events: boundaries of the first chunk's three stages
+        hipEvent_t* ev = (stage_events && b == 0) ? (hipEvent_t*)stage_events : nullptr;
+        if (ev) HIPCHK(hipEventRecord(ev[0], st));
+#define LAUNCH3(ARMV, RV, RARE)                                                                             \
+    do {                                                                                                    \
+        hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, false, true>), G, B, 0, st, a, q2, q1, \
+                           0, coll);                                                                        \
+        if (ev) HIPCHK(hipEventRecord(ev[1], st));                                                          \
+        hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, true, true>), G, B, 0, st, a, q1, q2,  \
+                           coll, hut);                                                                      \
+        if (ev) HIPCHK(hipEventRecord(ev[2], st));                                                          \
+        hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_HUT, true, false>), G, B, 0, st, a, q2, q1,     \
+                           hut, 0x7fffffff);                                                                \
+        if (ev) HIPCHK(hipEventRecord(ev[3], st));                                                          \
+    } while (0)
+#define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
         if (arm == SIMC_ARM_HMS) {
-            if (tape) { if (rare) LAUNCH(SIMC_ARM_HMS, TapeStream, true); else LAUNCH(SIMC_ARM_HMS, TapeStream, false); }
-            else { if (rare) LAUNCH(SIMC_ARM_HMS, PhiloxStream, true); else LAUNCH(SIMC_ARM_HMS, PhiloxStream, false); }
+            if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
         } else {
-            if (tape) { if (rare) LAUNCH(SIMC_ARM_SHMS, TapeStream, true); else LAUNCH(SIMC_ARM_SHMS, TapeStream, false); }
-            else { if (rare) LAUNCH(SIMC_ARM_SHMS, PhiloxStream, true); else LAUNCH(SIMC_ARM_SHMS, PhiloxStream, false); }
+            if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
         }
-#undef LAUNCH
+#undef LAUNCH1
+#undef LAUNCH3
         HIPCHK(hipGetLastError());
     }
     return SIMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                 simc_events_out* out, int64_t n, const simc_rng* rng, void* stream)
+{
+    return arm_run(ctx, arm, s, in, out, n, rng, stream, nullptr);
+}
+
+int simc_arm_run_timed(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                       simc_events_out* out, int64_t n, const simc_rng* rng, void* stream,
+                       void* const* stage_events)
+{
+    if (!stage_events || !stage_events[0] || !stage_events[1] || !stage_events[2] || !stage_events[3])
+        return fail(SIMC_E_ARG, "simc_arm_run_timed: four events required");
+    return arm_run(ctx, arm, s, in, out, n, rng, stream, stage_events);
 }
 
 int simc_hms_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
@@ -1132,24 +1377,33 @@ int simc_map_class_info(simc_ctx* ctx, int arm, int cls, double* driftdist, int3
     return SIMC_OK;
 }
 
-int simc_stop_flops(simc_ctx* ctx, int arm, int64_t* flops, int nflops)
+int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int nflops)
 {
     if (!ctx || !flops) return fail(SIMC_E_ARG, "simc_stop_flops: NULL argument");
     const int ai = arm_index(arm);
     if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (stage < 0 || stage > 3) return fail(SIMC_E_ARG, "stage must be 0 (all), 1, 2 or 3");
     if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
     for (int i = 0; i < nflops; ++i) flops[i] = 0;
     int64_t acc = 0;
-    for (const Op& o : ctx->ops[ai]) {
-        if (o.code == OP_TRANSP) acc += simc::poly_flops(arm, o.arg);
+    const std::vector<Op>& ops = ctx->ops[ai];
+    for (int pc = 0; pc < (int)ops.size(); ++pc) {
+        const Op& o = ops[pc];
+        const int in_stage = pc < ctx->coll_pc[ai] ? 1 : pc < ctx->hut_pc[ai] ? 2 : 3;
+        if (o.code == OP_TRANSP && (stage == 0 || stage == in_stage)) acc += simc::poly_flops(arm, o.arg);
         if (o.code == OP_COLLIMATOR) {
             flops[ST_SLIT_HOR] = flops[ST_SLIT_VERT] = flops[ST_SLIT_OCT] = flops[ST_COLL] = acc;
         } else if (o.stop > 0 && o.code != OP_WIRE && o.stop < kHistBins) {
             flops[o.stop] = acc;
         }
-        if (o.code == OP_END) flops[ST_OK] = acc + simc::poly_flops(arm, 0);
+        if (o.code == OP_END) flops[ST_OK] = acc + (stage == 0 || stage == 3 ? simc::poly_flops(arm, 0) : 0);
     }
     return SIMC_OK;
+}
+
+int simc_stop_flops(simc_ctx* ctx, int arm, int64_t* flops, int nflops)
+{
+    return simc_stop_flops_stage(ctx, arm, 0, flops, nflops);
 }
 
 int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)

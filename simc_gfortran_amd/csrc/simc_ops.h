@@ -29,8 +29,8 @@ enum OpCode : int32_t {
                       // (h p0, v p1) at entry, drift p6, (h p2, v p3) at exit; offsets p4, p5
     OP_RESMULT_DRAW,  // resmult = 2 if uniform < p0 else 1   (hms/mc_hms_hut.f:292-298)
     OP_RESMULT_ONE,   // resmult = 1                          (shms/mc_shms_hut.f:80)
-    OP_MUSC,          // musc with rad_len p0
-    OP_MUSC_EXT,      // musc_ext with rad_len p0, length p1
+    OP_MUSC,          // musc with rad_len p0 (p1 = sqrt p0, p2 = log10 p0)
+    OP_MUSC_EXT,      // musc_ext with rad_len p0, length p1 (p2 = sqrt p0, p3 = log10 p0)
     OP_WIRE,          // wire-plane hit of plane `stop` (0-11); arg = 1: y plane
     OP_BOX,           // stop if xs > p0 or xs < p1 or ys > p2 or ys < p3
     OP_SCIN,          // count++ if ys < p0 and ys > p1 and xs < p2 and xs > p3
@@ -39,6 +39,18 @@ enum OpCode : int32_t {
     OP_FIDUCIAL,      // track at z = p0: stop if yc > p1 or yc < p2 or xc > p3 or xc < p4
     OP_NCODES
 };
+
+// The two stages a program is run in: the magnets (target to the end of the
+// dipole: COSY segments and apertures) and the detector hut (materials, wire
+// planes, fit, trigger).  Each stage's interpreter is compiled with its own
+// steps only; the host checks every program against this table.
+enum : int { STAGE_MAGNETS = 1, STAGE_HUT = 2 };
+constexpr int op_stages(int code)
+{
+    return code == OP_END || code == OP_PROJECT ? (STAGE_MAGNETS | STAGE_HUT)
+         : code <= OP_COLLIMATOR               ? STAGE_MAGNETS
+                                               : STAGE_HUT;
+}
 
 struct Op {
     int32_t code;

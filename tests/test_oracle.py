@@ -136,6 +136,74 @@ def test_gauss1_polar_step(oracle):
     assert used == 4
 
 
+# ---------------------------------------------------------------------------
+# Box-Muller normals of the counter-based stream
+# ---------------------------------------------------------------------------
+def ulp(x):
+    return math.ulp(x)
+
+
+def test_bm_log_against_libm(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(11)
+    us = list(rng.uniform(0.0, 1.0, 20000)) + [2.0 ** -33, 0.5, 0.25, 1.0 - 2.0 ** -33, 1.0 - 2.0 ** -20,
+                                              math.sqrt(0.5), math.nextafter(math.sqrt(0.5), 1.0),
+                                              0.7071067811865476, 0.9999999, 1e-9]
+    worst = 0.0
+    for u in us:
+        if u <= 0.0:
+            continue
+        got, want = L.or_bm_log(u), math.log(u)
+        worst = max(worst, abs(got - want) / ulp(want))
+    assert worst <= 2.0, worst
+    # powers of two are exact multiples of ln2 (two-part constant)
+    for e in range(1, 34):
+        assert abs(L.or_bm_log(2.0 ** -e) - (-e * math.log(2.0))) <= ulp(e * math.log(2.0))
+
+
+def test_bm_sincos2pi_against_libm(oracle):
+    L = oracle.lib()
+    s, c = C.c_double(), C.c_double()
+    exact = {0.0: (0.0, 1.0), 0.25: (1.0, 0.0), 0.5: (0.0, -1.0), 0.75: (-1.0, 0.0),
+             0.125: (math.sqrt(0.5), math.sqrt(0.5)), 1.0 / 12: (0.5, math.sqrt(3) / 2)}
+    for u, (ws, wc) in exact.items():
+        L.or_bm_sincos2pi(u, C.byref(s), C.byref(c))
+        assert abs(s.value - ws) <= 2e-16 and abs(c.value - wc) <= 2e-16, (u, s.value, c.value)
+    rng = np.random.default_rng(12)
+    worst = 0.0
+    for u in rng.uniform(0.0, 1.0, 20000):
+        L.or_bm_sincos2pi(u, C.byref(s), C.byref(c))
+        # libm's argument 2 pi u carries up to ~1e-15 of rounding itself
+        worst = max(worst, abs(s.value - math.sin(2 * math.pi * u)), abs(c.value - math.cos(2 * math.pi * u)))
+        assert abs(s.value * s.value + c.value * c.value - 1.0) <= 5e-16
+    assert worst <= 2e-15, worst
+
+
+def test_philox_normals_pairing(oracle):
+    """First pair of event 9, seed 5: u1, u2 = words 0, 1 of Philox block 0;
+    the cosine value comes first, the sine value second."""
+    blk = oracle.philox_block((0, 9, 0, 0x53494D43), (5, 0))
+    u1, u2 = [(w + 0.5) * 2.0 ** -32 for w in blk[:2]]
+    r = math.sqrt(-2.0 * math.log(u1))
+    g, used = oracle.philox_normals(5, 9, 2)
+    assert used == 2
+    assert abs(g[0] - r * math.cos(2 * math.pi * u2)) <= 1e-14
+    assert abs(g[1] - r * math.sin(2 * math.pi * u2)) <= 1e-14
+    g3, used3 = oracle.philox_normals(5, 9, 3)
+    assert used3 == 4 and (g3[:2] == g).all()
+
+
+def test_philox_normals_are_standard_normal(oracle):
+    from scipy import stats
+    vals = np.concatenate([oracle.philox_normals(2024, e, 2000)[0] for e in range(100)])
+    assert stats.kstest(vals, "norm").pvalue > 1e-3
+    assert abs(vals.mean()) < 5 / math.sqrt(len(vals))
+    assert abs(vals.var() - 1.0) < 5 * math.sqrt(2.0 / len(vals))
+    # the two members of a pair are uncorrelated
+    pairs = vals.reshape(-1, 2)
+    assert abs(np.corrcoef(pairs[:, 0], pairs[:, 1])[0, 1]) < 5 / math.sqrt(len(pairs))
+
+
 def lfit(oracle, z, y, f32):
     L = oracle.lib()
     n = len(z)
