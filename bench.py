@@ -72,7 +72,7 @@ STAGE_NAMES = ["stage1_target_to_collimator", "stage2_magnets", "stage3_hut_fit_
 def pmc_summary(fetch_csv, write_csv, events):
     """HBM bytes per launch of each kernel stage from FETCH_SIZE / WRITE_SIZE
     (KiB per dispatch), each scaled by what the same counter reads for the
-    calibration copy (the largest copy dispatch: M MiB read and M MiB written)."""
+    calibration kernel (an elementwise multiply reading and writing M MiB)."""
     import csv
 
     def per_kernel(path, counter):
@@ -86,7 +86,7 @@ def pmc_summary(fetch_csv, write_csv, events):
                 key = STAGE_NAMES[0] if "false, 1, false, true" in name else \
                     STAGE_NAMES[1] if "false, 1, true, true" in name else \
                     STAGE_NAMES[2] if "false, 2, true, false" in name else None
-            elif "at::native" in name and "copy" in name:
+            elif "at::native" in name and "Mul" in name:
                 key = "copy"
             if key is None:
                 continue
@@ -123,7 +123,8 @@ def main():
     ap.add_argument("--events", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first run one device copy of this many MiB (HBM counter calibration)")
+                    help="first run one elementwise kernel reading and writing this many MiB "
+                         "(HBM counter calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "`bench.py --copy-probe-mib M` into profiles/hbm_traffic_shms_1e7.json")
@@ -155,11 +156,12 @@ def main():
               fit="fp64", out=out)
     stream = torch.cuda.current_stream(dev)
     if args.copy_probe_mib:
+        # one elementwise kernel that reads M MiB and writes M MiB
         m = args.copy_probe_mib * (1 << 20) // 8
         src = torch.ones(m, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 

@@ -52,8 +52,8 @@ extern "C" {
 #define SIMC_FIT_REF_F32 1    /* REAL*4 hits and fit parameters, as cern/lfit.f:18 */
 #define SIMC_FIT_FP64 0       /* the same fit carried out in FP64                 */
 
-#define SIMC_HMS_NCOUNTERS 22  /* common /spectrometer1/,      hms/struct_hms.inc:66-74   */
-#define SIMC_SHMS_NCOUNTERS 52 /* common /spectrometer_shms/, shms/struct_shms.inc:320-338 */
+#define SIMC_HMS_NCOUNTERS 22  /* common /spectrometer1/,      hms/struct_hms.inc:34-42   */
+#define SIMC_SHMS_NCOUNTERS 52 /* common /spectrometer_shms/, shms/struct_shms.inc:26-44 */
 
 typedef struct simc_ctx simc_ctx;
 
@@ -146,7 +146,7 @@ int simc_counters_get(simc_ctx *ctx, int arm, int64_t *counters, int ncounters);
 int simc_counters_reset(simc_ctx *ctx, int arm);
 
 /* Central-ray geometry of the loaded forward map: drift length of a class
- * (cm) and its pure-drift flag (/driftdistances/, spectrometers.inc:130-133). */
+ * (cm) and its pure-drift flag (/driftdistances/, spectrometers.inc:9-12). */
 int simc_map_class_info(simc_ctx *ctx, int arm, int cls, double *driftdist, int32_t *adrift,
                         int32_t *n_terms);
 

@@ -35,7 +35,7 @@ extern void mc_shms_(double *p_spec, double *th_spec, double *dpp, double *x, do
                      int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
                      double *pathlen, int32_t *spectr, int32_t *using_coll);
 
-/* Common blocks (hms/struct_hms.inc:66-74, shms/struct_shms.inc:320-338,
+/* Common blocks (hms/struct_hms.inc:34-42, shms/struct_shms.inc:26-44,
  * simulate.inc:153 /decd/ whose first member is ctau). */
 extern struct { int32_t v[H_NCOUNT]; } spectrometer1_;
 extern struct { int32_t v[S_NCOUNT]; } spectrometer_shms_;

@@ -25,7 +25,7 @@
 #endif
 
 /* ------------------------------------------------------------------------- */
-/* constants.inc:91-121 */
+/* constants.inc:17-42 */
 #define PI_C 3.141592653589793
 static const double Mpi_C = 139.57018;
 static const double Mmu_C = 105.6583755;
@@ -728,9 +728,9 @@ void or_enerloss_new(or_rng *g, double len, double dens, double zeff, double aef
     *eloss = E;
 }
 
-/* pion_coll_absorb, hms/pion_coll_absorb.f:1-239.  The d-exponent literal
+/* pion_coll_absorb, hms/pion_coll_absorb.f:1-92.  The d-exponent literal
  * 1.d-27 is DOUBLE PRECISION, which -fdefault-real-8 promotes to a 16-byte
- * real, so that product is formed in quad precision (:223,229). */
+ * real, so that product is formed in quad precision (:76,82). */
 double or_pion_coll_absorb(double ppi, double thick)
 {
     static const double T[14] = {85.0, 125.0, 165.0, 205.0, 245.0, 315.0, 584.02, 711.95, 870.12,
@@ -1700,7 +1700,7 @@ void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, 
     const double zd_q3d1trans = 18.00, zd_d1flare = 30.10, zd_d1men = 39.47;
     const double zd_d1mid = 36.406263, zd_d1out = 60.68, zd_fp = 307.95;
     const double hcer_1_zentrance = -291.700;
-    /* shms/apertures_shms.inc:240-259 */
+    /* shms/apertures_shms.inc:9-28 */
     const double r_HBx = 11.2, r_HBfym = -4.13, r_HBfyp = 11.75, r_HBmenym = -5.45,
                  r_HBmenyp = 11.74, r_HBmexym = -10.25, r_HBmexyp = 11.71, r_HBbym = -11.71,
                  r_HBbyp = 11.70;

@@ -91,13 +91,13 @@ typedef struct {
     double xs_fp, ys_fp, dxdzs_fp, dydzs_fp;
 } or_track;
 
-/* HMS stop counters, same order as common /spectrometer1/ (hms/struct_hms.inc:66-74) */
+/* HMS stop counters, same order as common /spectrometer1/ (hms/struct_hms.inc:34-42) */
 enum {
     H_TRIALS, H_SLIT_HOR, H_SLIT_VERT, H_SLIT_OCT, H_Q1_IN, H_Q1_MID, H_Q1_OUT,
     H_Q2_IN, H_Q2_MID, H_Q2_OUT, H_Q3_IN, H_Q3_MID, H_Q3_OUT, H_D1_IN, H_D1_OUT,
     H_HUT, H_DC1, H_DC2, H_SCIN, H_CAL, H_SUCCESSES, H_COLL, H_NCOUNT
 };
-/* SHMS stop counters, same order as common /spectrometer_shms/ (shms/struct_shms.inc:320-338) */
+/* SHMS stop counters, same order as common /spectrometer_shms/ (shms/struct_shms.inc:26-44) */
 enum {
     S_TRIALS, S_SUCCESSES, S_TARG_HOR, S_TARG_VERT, S_TARG_OCT, S_SLIT_HOR, S_SLIT_VERT,
     S_SLIT_OCT, S_HB_IN, S_HB_MEN, S_HB_MEX, S_HB_OUT, S_Q1_IN, S_Q1_MEN, S_Q1_MID,

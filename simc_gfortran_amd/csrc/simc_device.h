@@ -45,7 +45,7 @@ __device__ __forceinline__ void st8(uint8_t* p, uint32_t n, uint32_t off, uint8_
     __builtin_amdgcn_raw_buffer_store_b8(v, rsrc(p, n), off, 0, 0);
 }
 
-// constants.inc:91-121
+// constants.inc:23-42
 constexpr double kPi = 3.141592653589793;
 constexpr double kMpi = 139.57018;
 constexpr double kMmu = 105.6583755;
@@ -292,7 +292,7 @@ __device__ __forceinline__ double gauss1(R& g, double nsigmax)
     return g.gauss(nsigmax);
 }
 
-// theta_sigma of musc / musc_ext (shared/musc.f:109-112, shared/musc_ext.f:155-158)
+// theta_sigma of musc / musc_ext (shared/musc.f:48-51, shared/musc_ext.f:36-39)
 __device__ __forceinline__ double theta_sigma(double m2, double p, double rad_len)
 {
     const double Es = 13.6, epsilon = 0.088;
@@ -321,7 +321,7 @@ struct ThetaCache {
     }
 };
 
-// musc(m2, p, rad_len, dth, dph), shared/musc.f:93-118
+// musc(m2, p, rad_len, dth, dph), shared/musc.f:43-57
 template <class R>
 __device__ __forceinline__ void musc(R& g, const Track& t, double rad_len, double& dth, double& dph)
 {
@@ -331,7 +331,7 @@ __device__ __forceinline__ void musc(R& g, const Track& t, double rad_len, doubl
     dph = dph + ts * gauss1(g, 99.0);
 }
 
-// musc_ext(m2, p, rad_len, x_len, dph, dth, y, x), shared/musc_ext.f:136-172.
+// musc_ext(m2, p, rad_len, x_len, dph, dth, y, x), shared/musc_ext.f:25-51.
 // Every call site passes (dydzs, dxdzs, ys, xs).
 template <class R>
 __device__ __forceinline__ void musc_ext(R& g, Track& t, double rad_len, double x_len)
@@ -350,7 +350,7 @@ __device__ __forceinline__ void musc_ext(R& g, Track& t, double rad_len, double 
     t.ys = t.ys + ts * x_len * g2 / s12 + ts * x_len * g1 / 2.0;
 }
 
-// loren, loren.f:261-271
+// loren, loren.f:18-23
 __device__ __forceinline__ void loren(double gam, double bx, double by, double bz, double e,
                                       double x, double y, double z, double& pxf, double& pyf,
                                       double& pzf, double& pf1)
@@ -471,7 +471,7 @@ __device__ __forceinline__ void transp(R& g, Track& t, const Flags& F, double zd
     t.pathlen = t.pathlen + (zd + delta_z);
 }
 
-// rotate_haxis, shared/rotate_haxis.f:165-183: new (xp0, yp0) from (xs, ys)
+// rotate_haxis, shared/rotate_haxis.f:43-61: new (xp0, yp0) from (xs, ys)
 __device__ __forceinline__ void rotate_haxis(const Track& t, const Rot& r, double& xp0, double& yp0)
 {
     const double alpha = t.dxdzs, beta = t.dydzs;
@@ -482,7 +482,7 @@ __device__ __forceinline__ void rotate_haxis(const Track& t, const Rot& r, doubl
     yp0 = yp0 + xi * beta_p * r.sin_th;
 }
 
-// rotate_vaxis, shared/rotate_vaxis.f:228-243
+// rotate_vaxis, shared/rotate_vaxis.f:42-57
 __device__ __forceinline__ void rotate_vaxis(const Track& t, const Rot& r, double& xp0, double& yp0)
 {
     const double alpha = t.dydzs, beta = t.dxdzs;
@@ -548,7 +548,7 @@ __device__ __forceinline__ double times_1em27_quad(double a)
     return p + (e + a * lo);
 }
 
-// pion_coll_absorb, hms/pion_coll_absorb.f:158-237: transmission probability
+// pion_coll_absorb, hms/pion_coll_absorb.f:54-90: transmission probability
 __device__ double pion_coll_absorb(double ppi, double thick)
 {
     const double T[14] = {85.0, 125.0, 165.0, 205.0, 245.0, 315.0, 584.02, 711.95, 870.12,

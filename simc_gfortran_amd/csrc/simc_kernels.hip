@@ -672,7 +672,7 @@ struct Program {
         return ops.back();
     }
     // rotate_haxis / rotate_vaxis of the track into a tilted aperture plane
-    // (shared/rotate_haxis.f:163-173: truncated degree constant, libm trig of
+    // (shared/rotate_haxis.f:41-51: truncated degree constant, libm trig of
     // the product), then the aperture-centre offsets.
     void rot(int code, double rotang, double xoff, double yoff)
     {
@@ -841,7 +841,7 @@ void build_shms(Program& P, double zdc[12])
     const double zd_q3in = 28.10, zd_q3men = 39.1, zd_q3mid = 79.35, zd_q3mex = 79.35, zd_q3out = 39.1;
     const double zd_q3d1trans = 18.00, zd_d1flare = 30.10, zd_d1men = 39.47, zd_d1mid = 36.406263;
     const double zd_d1out = 60.68, zd_fp = 307.95;
-    // shms/apertures_shms.inc:240-259
+    // shms/apertures_shms.inc:9-28
     const double rhb = 11.2 * 11.2, rq1 = 20.00 * 20.00, rq2 = 30.00 * 30.00, rq3 = 30.00 * 30.00,
                  rd1 = 30.00 * 30.00;
     P.add(OP_PROJECT, {zd_hbin});                                                 // :422-423
@@ -1316,7 +1316,7 @@ int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
     for (int i = 0; i < kHistBins; ++i) total += (int64_t)h[i];
     auto H = [&](int code) { return (int64_t)h[code]; };
     if (arm == SIMC_ARM_HMS) {
-        // order of common /spectrometer1/, hms/struct_hms.inc:66-74
+        // order of common /spectrometer1/, hms/struct_hms.inc:34-42
         c[0] = total;
         c[1] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
         c[2] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
@@ -1332,7 +1332,7 @@ int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
         c[15] = c[16] + c[17] + c[18] + c[20];
         c[21] = H(ST_COLL);
     } else {
-        // order of common /spectrometer_shms/, shms/struct_shms.inc:320-338
+        // order of common /spectrometer_shms/, shms/struct_shms.inc:26-44
         c[0] = total;
         c[1] = H(ST_OK);
         c[5] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];

@@ -14,7 +14,7 @@
 namespace simc {
 namespace {
 
-constexpr int kMaxClass = 41;    // spectrometers.inc:126
+constexpr int kMaxClass = 41;    // spectrometers.inc:5
 constexpr int kMaxTerms = 1000;  // shared/transp.f:81
 
 // One formatted record into a fixed 132-column image (CHARACTER*132 read with

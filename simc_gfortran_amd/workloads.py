@@ -21,9 +21,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-ME = 0.51099906        # constants.inc:91
-MP = 938.27231         # constants.inc:93
-MPI = 139.57018        # constants.inc:97
+ME = 0.51099906        # constants.inc:17
+MP = 938.27231         # constants.inc:19
+MPI = 139.57018        # constants.inc:23
 
 
 @dataclass
