@@ -123,6 +123,20 @@ int simc_shms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *i
 int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 
+/* simc_arm_run on HOST arrays (the same structs, host pointers; the tape too):
+ * the library copies the passed arrays to device staging memory it owns,
+ * runs on the default stream and copies every passed output back before
+ * returning (synchronous); ok and stop may be NULL here.  For callers without
+ * device memory management, e.g. a Fortran batch driver of montecarlo()
+ * (simc.f:1464-1488).  Calls on one context are serialised. */
+int simc_arm_run_host(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
+                      simc_events_out *out, int64_t n, const simc_rng *rng);
+
+/* simc_recon_run on host arrays (synchronous, as simc_arm_run_host). */
+int simc_recon_run_host(simc_ctx *ctx, int arm, int64_t n, const double *x_fp, const double *dx_fp,
+                        const double *y_fp, const double *dy_fp, const double *fry, double *delta,
+                        double *dxdz_tar, double *dydz_tar, double *y_tar);
+
 /* simc_arm_run with stage timing (no counterpart in the reference: a
  * measurement hook).  stage_events[0..3] are hipEvent_t handles created by
  * the caller; they are recorded on `stream` before stage 1 (target to

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
 import torch
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -47,7 +48,7 @@ EXPORTS = [
     "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_destroy", "simc_hms_run",
     "simc_shms_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
-    "simc_stop_flops_stage",
+    "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_last_error",
     "simc_version",
 ]
@@ -107,6 +108,9 @@ def library(path: str = LIB_PATH):
                                      C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp,
                                      C.POINTER(_vp)]
     L.simc_recon_run.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9 + [_vp]
+    L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
+                                    C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng)]
+    L.simc_recon_run_host.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9
     L.simc_counters_get.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
     L.simc_counters_reset.argtypes = [_vp, C.c_int]
     L.simc_map_class_info.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -206,6 +210,34 @@ class Spectrometers:
             evs = (_vp * 4)(*[e.cuda_event for e in stage_events])
             _check(library().simc_arm_run_timed(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout),
                                                 n, C.byref(R), C.c_void_p(st), evs))
+        return out
+
+    def run_host(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
+                 coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
+                 tape=None, stride=0):
+        """run() on numpy arrays through the synchronous host-pointer entry point.
+        Returns a dict of numpy arrays with the same fields as run()."""
+        n = len(events["dpp"])
+        ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
+              ("dpp", "x", "y", "z", "dxdz", "dydz", "fry") if events.get(k) is not None}
+        out = {k: ev[k].copy() for k in ("dpp", "y", "dxdz", "dydz")}
+        for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2"):
+            out[k] = np.zeros(n)
+        out["ok"] = np.zeros(n, np.uint8)
+        out["stop"] = np.zeros(n, np.uint8)
+        out["used"] = np.zeros(n, np.int32)
+        S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
+                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
+        hp = lambda a: None if a is None else a.ctypes.data_as(_vp)
+        ein = EventsIn(*[hp(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
+        eout = EventsOut(*[hp(out.get(k)) for k in OUT_FIELDS])
+        if rng == "tape":
+            tape = np.ascontiguousarray(tape, dtype=np.float64)
+            R = Rng(RNG_TAPE, 0, 0, event0, hp(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        _check(library().simc_arm_run_host(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
+                                           C.byref(R)))
         return out
 
     def recon(self, arm, x_fp, dx_fp, y_fp, dy_fp, fry=None, stream=None):

@@ -3,7 +3,10 @@
     python -m simc_gfortran_amd.build
 
 Steps: generate the COSY polynomial code from data/maps (gen_cosy.py), then
-compile the HIP kernels and the host map reader into one shared library.
+compile the HIP kernels and the host map reader into one shared library;
+build the per-event Fortran-ABI drop-in libsimc_dropin.so on top of it; and,
+when a Fortran compiler is present, compile the Fortran module simc_gpu and
+the Fortran test programs of tests/fortran against both libraries.
 """
 from __future__ import annotations
 
@@ -16,6 +19,10 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 GEN = os.path.join(CSRC, "_gen")
 LIB = os.path.join(PKG, "libsimc_gpu.so")
+DROPIN = os.path.join(PKG, "libsimc_dropin.so")
+FORTRAN = os.environ.get("SIMC_FC", "/opt/rocm/lib/llvm/bin/flang")
+FBUILD = os.path.join(PKG, "fortran", "_build")
+FBIN = os.path.join(REPO, "tests", "fortran", "_bin")
 MAP_DIR = os.path.join(REPO, "data", "maps")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SIMC_OFFLOAD_ARCH", "gfx950")
@@ -42,7 +49,46 @@ def build(verbose: bool = True) -> str:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_dropin(verbose)
+    build_fortran(verbose)
     return LIB
+
+
+def build_dropin(verbose: bool = True) -> str:
+    """libsimc_dropin.so: mc_hms_/mc_shms_/mc_*_recon_ on top of libsimc_gpu.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(CSRC, "simc_dropin.cpp"),
+           "-I", os.path.join(REPO, "include"), "-L", PKG, "-lsimc_gpu", "-Wl,-rpath,$ORIGIN",
+           "-o", DROPIN + ".tmp"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(DROPIN + ".tmp", DROPIN)
+    return DROPIN
+
+
+def build_fortran(verbose: bool = True) -> bool:
+    """The module simc_gpu and the Fortran test programs (skipped without a
+    Fortran compiler)."""
+    if not os.path.exists(FORTRAN):
+        if verbose:
+            print("no Fortran compiler at", FORTRAN, "- Fortran module and programs not built")
+        return False
+    os.makedirs(FBUILD, exist_ok=True)
+    os.makedirs(FBIN, exist_ok=True)
+    mod_src = os.path.join(PKG, "fortran", "simc_gpu.f90")
+    mod_obj = os.path.join(FBUILD, "simc_gpu.o")
+    cmds = [[FORTRAN, "-O2", "-c", mod_src, "-module-dir", FBUILD, "-o", mod_obj]]
+    tdir = os.path.join(REPO, "tests", "fortran")
+    rpath = "-Wl,-rpath,$ORIGIN/../../../simc_gfortran_amd"
+    cmds.append([FORTRAN, "-O2", "-I", FBUILD, os.path.join(tdir, "batch_host.f90"), mod_obj,
+                 "-L", PKG, "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "batch_host")])
+    cmds.append([FORTRAN, "-O2", os.path.join(tdir, "dropin_events.f90"),
+                 "-L", PKG, "-lsimc_dropin", "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "dropin_events")])
+    for c in cmds:
+        if verbose:
+            print(" ".join(c))
+        subprocess.run(c, check=True)
+    return True
 
 
 if __name__ == "__main__":

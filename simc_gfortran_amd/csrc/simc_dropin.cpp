@@ -1,0 +1,214 @@
+// Per-event drop-in replacements of the reference's Fortran subroutines
+// mc_hms, mc_shms, mc_hms_recon and mc_shms_recon (gfortran/flang ABI:
+// lower-case names with a trailing underscore, every argument by reference,
+// LOGICAL as 4-byte integers), built as libsimc_dropin.so on top of
+// libsimc_gpu.so.  Linking a SIMC executable against this library instead of
+// the objects of hms/mc_hms*.f and shms/mc_shms*.f keeps every call site of
+// simc.f unchanged; each call transports ONE event on the GPU (synchronous),
+// so this path is for compatibility -- throughput needs the batched entry
+// points of simc_gpu.h.
+//
+// State shared with the caller's program through its common blocks, when
+// they exist (weak references; a program without them still links):
+//   /spectrometer1/, /spectrometer_shms/  STOP counters, incremented per event
+//                                         (hms/struct_hms.inc:34-42,
+//                                          shms/struct_shms.inc:26-44);
+//   /track/  xs, ys, zs, dpps, dxdzs, dydzs (spectrometers.inc:47-60): set to
+//            the fitted focal-plane track of an accepted event, as
+//            hms/mc_hms.f:419-423 does, and read by the recon entry points
+//            (hms/mc_hms_recon.f:113-118);
+//   /decd/   ctau, its first member (simulate.inc:153).
+//
+// Random numbers: a counter-based Philox4x32-10 stream per event, keyed by
+// (seed, call number); seed from SIMC_SEED (default 0) or simc_dropin_seed.
+// Maps are read from SIMC_MAP_DIR, default the current directory, where the
+// reference opens hms/forward_cosy.dat etc. (shared/transp.f:300-309).
+// Errors are printed to stderr and leave ok_spec = .false.; they never end
+// the process.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+
+#include "../../include/simc_gpu.h"
+
+extern "C" {
+extern int32_t spectrometer1_[SIMC_HMS_NCOUNTERS] __attribute__((weak));
+extern int32_t spectrometer_shms_[SIMC_SHMS_NCOUNTERS] __attribute__((weak));
+extern double track_[6] __attribute__((weak));
+extern double decd_[1] __attribute__((weak));
+}
+
+namespace {
+
+std::mutex g_mu;
+simc_ctx* g_ctx = nullptr;
+bool g_failed = false;
+uint64_t g_seed = 0;
+uint64_t g_event = 0;
+bool g_seed_set = false;
+
+simc_ctx* context()
+{
+    if (g_ctx || g_failed) return g_ctx;
+    const char* dir = std::getenv("SIMC_MAP_DIR");
+    if (!g_seed_set) {
+        const char* sd = std::getenv("SIMC_SEED");
+        g_seed = sd ? std::strtoull(sd, nullptr, 10) : 0;
+    }
+    if (simc_ctx_create(dir ? dir : ".", &g_ctx) != SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        g_failed = true;
+        g_ctx = nullptr;
+    }
+    return g_ctx;
+}
+
+void add_counters(int arm)
+{
+    const int nc = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    int32_t* common = arm == SIMC_ARM_HMS ? spectrometer1_ : spectrometer_shms_;
+    if (!common) return;
+    int64_t c[SIMC_SHMS_NCOUNTERS];
+    if (simc_counters_get(g_ctx, arm, c, nc) != SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        return;
+    }
+    for (int i = 0; i < nc; ++i) {
+        // stop_id (SHMS slot 50) is a last-value register in the reference
+        if (arm == SIMC_ARM_SHMS && i == 50) continue;
+        common[i] += (int32_t)c[i];
+    }
+}
+
+void arm_event(int arm, double* p_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+               double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2,
+               int32_t* ms_flag, int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry,
+               int32_t* ok_spec, double* pathlen, int32_t* using_coll)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    *ok_spec = 0;
+    if (!context()) return;
+    simc_settings s = {};
+    s.p_spec = *p_spec;
+    s.m2 = *m2;
+    s.ctau = decd_ ? decd_[0] : 0.0;
+    s.ms_flag = *ms_flag != 0;
+    s.wcs_flag = *wcs_flag != 0;
+    s.decay_flag = *decay_flag != 0;
+    s.using_coll = *using_coll != 0;
+    s.fit_mode = SIMC_FIT_REF_F32;
+    s.wire_sigma = 0.0;
+    simc_events_in in = {dpp, x, y, z, dxdz, dydz, fry};
+    uint8_t ok = 0, stop = 0;
+    simc_events_out out = {dpp, y, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, pathlen, resmult, m2, &ok, &stop,
+                           nullptr};
+    simc_rng rng = {};
+    rng.kind = SIMC_RNG_PHILOX;
+    rng.seed = g_seed;
+    rng.event0 = g_event++;
+    if (simc_counters_reset(g_ctx, arm) != SIMC_OK ||
+        simc_arm_run_host(g_ctx, arm, &s, &in, &out, 1, &rng) != SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        return;
+    }
+    add_counters(arm);
+    *ok_spec = ok ? 1 : 0;
+    if (ok && track_) {  // hms/mc_hms.f:419-423
+        track_[0] = *x_fp;
+        track_[1] = *y_fp;
+        track_[4] = *dx_fp;
+        track_[5] = *dy_fp;
+    }
+}
+
+void recon_event(int arm, double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!track_) {
+        std::fprintf(stderr, "simc_dropin: common /track/ is not present in this program\n");
+        return;
+    }
+    if (!context()) return;
+    double xs = track_[0], ys = track_[1], dxdzs = track_[4], dydzs = track_[5];
+    double delta, dxdz, dydz, yt;
+    if (simc_recon_run_host(g_ctx, arm, 1, &xs, &dxdzs, &ys, &dydzs, fry, &delta, &dxdz, &dydz, &yt) !=
+        SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        return;
+    }
+    // hms/mc_hms_recon.f:134-137: delta_phi = x'_tar, y_tgt, delta_t = y'_tar, delta_p
+    *delta_p = delta;
+    *delta_t = dydz;
+    *delta_phi = dxdz;
+    *y_tgt = yt;
+}
+
+}  // namespace
+
+extern "C" {
+
+// mc_hms(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp,
+//        m2, ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen,
+//        using_HMScoll), hms/mc_hms.f:1-4.  th_spec is unused (hms/mc_hms.f:153).
+void mc_hms_(double* p_spec, double* th_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+             double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2, int32_t* ms_flag,
+             int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry, int32_t* ok_spec,
+             double* pathlen, int32_t* using_coll)
+{
+    (void)th_spec;
+    arm_event(SIMC_ARM_HMS, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag, wcs_flag,
+              decay_flag, resmult, fry, ok_spec, pathlen, using_coll);
+}
+
+// mc_shms(... same ..., pathlen, spectr, using_SHMScoll), shms/mc_shms.f:1-4;
+// spectr must be 5 (shms/mc_shms.f:336-341).
+void mc_shms_(double* p_spec, double* th_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+              double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2,
+              int32_t* ms_flag, int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry,
+              int32_t* ok_spec, double* pathlen, int32_t* spectr, int32_t* using_coll)
+{
+    (void)th_spec;
+    if (*spectr != SIMC_ARM_SHMS) {
+        std::fprintf(stderr, "simc_dropin: mc_shms called with spectr = %d (must be 5)\n", (int)*spectr);
+        *ok_spec = 0;
+        return;
+    }
+    arm_event(SIMC_ARM_SHMS, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag,
+              wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen, using_coll);
+}
+
+// mc_hms_recon(delta_p, delta_t, delta_phi, y_tgt, fry), hms/mc_hms_recon.f:1
+void mc_hms_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
+{
+    recon_event(SIMC_ARM_HMS, delta_p, delta_t, delta_phi, y_tgt, fry);
+}
+
+// mc_shms_recon(delta_p, delta_t, delta_phi, y_tgt, fry, spectr), shms/mc_shms_recon.f:1-2
+void mc_shms_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry,
+                    int32_t* spectr)
+{
+    (void)spectr;
+    recon_event(SIMC_ARM_SHMS, delta_p, delta_t, delta_phi, y_tgt, fry);
+}
+
+// Seed of the per-event streams (Fortran: call simc_dropin_seed(seed) with an
+// integer*8 seed); restarts the event numbering.
+void simc_dropin_seed_(int64_t* seed)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_seed = (uint64_t)*seed;
+    g_seed_set = true;
+    g_event = 0;
+}
+
+// Release the GPU context (Fortran: call simc_dropin_close()).
+void simc_dropin_close_()
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (g_ctx) simc_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+    g_failed = false;
+}
+
+}  // extern "C"

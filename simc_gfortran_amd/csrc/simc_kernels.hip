@@ -640,6 +640,10 @@ struct simc_ctx {
     uint32_t* d_qcount[2] = {nullptr, nullptr};   // the two queue counters
     uint32_t q_cap[2] = {0, 0};
     std::mutex mu;                   // guards queue growth
+    // staging buffers of the host-pointer entry point (grown on demand)
+    std::mutex host_mu;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
     int device = 0;
     int n_cu = 256;
 };
@@ -1141,6 +1145,7 @@ void simc_ctx_destroy(simc_ctx* ctx)
             if (ctx->d_queue[i][j]) (void)hipFree(ctx->d_queue[i][j]);
         if (ctx->d_qcount[i]) (void)hipFree(ctx->d_qcount[i]);
     }
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     delete ctx;
 }
 
@@ -1263,6 +1268,118 @@ int simc_arm_run_timed(simc_ctx* ctx, int arm, const simc_settings* s, const sim
     if (!stage_events || !stage_events[0] || !stage_events[1] || !stage_events[2] || !stage_events[3])
         return fail(SIMC_E_ARG, "simc_arm_run_timed: four events required");
     return arm_run(ctx, arm, s, in, out, n, rng, stream, stage_events);
+}
+
+int simc_arm_run_host(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                      simc_events_out* out, int64_t n, const simc_rng* rng)
+{
+    if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run_host: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_arm_run_host: at most 2^31 events per call");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    // device images of every array the caller passed (inputs, then outputs)
+    const double* hin[7] = {in->dpp, in->x, in->y, in->z, in->dxdz, in->dydz, in->fry};
+    double* hout_d[11] = {out->dpp, out->y, out->dxdz, out->dydz, out->x_fp, out->dx_fp, out->y_fp,
+                          out->dy_fp, out->pathlen, out->resmult, out->m2};
+    const size_t nd = (size_t)n * 8;
+    const size_t tape_bytes = rng->kind == SIMC_RNG_TAPE ? (size_t)n * (size_t)rng->stride * 8 : 0;
+    auto rounded = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    // every carve() below, each rounded up to 256 bytes
+    const size_t need = 18 * rounded(nd) + 2 * rounded((size_t)n) + rounded(4 * (size_t)n) + rounded(tape_bytes);
+    if (ctx->stage_bytes < need) {
+        if (ctx->d_stage) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(ctx->d_stage));
+            ctx->d_stage = nullptr;
+            ctx->stage_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&ctx->d_stage, need));
+        ctx->stage_bytes = need;
+    }
+    char* base = (char*)ctx->d_stage;
+    size_t at = 0;
+    auto carve = [&](size_t bytes) {
+        char* p = base + at;
+        at += rounded(bytes);
+        return p;
+    };
+    const double* din[7];
+    for (int k = 0; k < 7; ++k) {
+        din[k] = nullptr;
+        if (hin[k]) {
+            double* d = (double*)carve(nd);
+            HIPCHK(hipMemcpy(d, hin[k], nd, hipMemcpyHostToDevice));
+            din[k] = d;
+        }
+    }
+    double* dout_d[11];
+    for (int k = 0; k < 11; ++k) {
+        dout_d[k] = nullptr;
+        if (hout_d[k]) {  // INOUT semantics: the device copy starts from the caller's values
+            dout_d[k] = (double*)carve(nd);
+            HIPCHK(hipMemcpy(dout_d[k], hout_d[k], nd, hipMemcpyHostToDevice));
+        }
+    }
+    uint8_t* d_ok = (uint8_t*)carve(n);    // always staged; copied back when asked for
+    uint8_t* d_stop = (uint8_t*)carve(n);
+    int32_t* d_used = out->used ? (int32_t*)carve(4 * (size_t)n) : nullptr;
+    simc_rng drng = *rng;
+    if (tape_bytes) {
+        double* d = (double*)carve(tape_bytes);
+        HIPCHK(hipMemcpy(d, rng->tape, tape_bytes, hipMemcpyHostToDevice));
+        drng.tape = d;
+    }
+    simc_events_in ein = {din[0], din[1], din[2], din[3], din[4], din[5], din[6]};
+    simc_events_out eout = {dout_d[0], dout_d[1], dout_d[2], dout_d[3], dout_d[4], dout_d[5],
+                            dout_d[6], dout_d[7], dout_d[8], dout_d[9], dout_d[10], d_ok, d_stop, d_used};
+    if (at > need) return fail(SIMC_E_INTERNAL, "simc_arm_run_host: staging layout exceeds its buffer");
+    int rc = arm_run(ctx, arm, s, &ein, &eout, n, &drng, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    for (int k = 0; k < 11; ++k)
+        if (hout_d[k]) HIPCHK(hipMemcpy(hout_d[k], dout_d[k], nd, hipMemcpyDeviceToHost));
+    if (out->ok) HIPCHK(hipMemcpy(out->ok, d_ok, n, hipMemcpyDeviceToHost));
+    if (out->stop) HIPCHK(hipMemcpy(out->stop, d_stop, n, hipMemcpyDeviceToHost));
+    if (d_used) HIPCHK(hipMemcpy(out->used, d_used, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return SIMC_OK;
+}
+
+int simc_recon_run_host(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const double* dx_fp,
+                        const double* y_fp, const double* dy_fp, const double* fry, double* delta,
+                        double* dxdz_tar, double* dydz_tar, double* y_tar)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "simc_recon_run_host: NULL context");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (!x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !dxdz_tar || !dydz_tar || !y_tar)
+        return fail(SIMC_E_ARG, "simc_recon_run_host: NULL array");
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    const size_t nd = (size_t)n * 8, need = 9 * ((nd + 255) & ~(size_t)255);
+    if (ctx->stage_bytes < need) {
+        if (ctx->d_stage) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(ctx->d_stage));
+            ctx->d_stage = nullptr;
+            ctx->stage_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&ctx->d_stage, need));
+        ctx->stage_bytes = need;
+    }
+    double* d[9];
+    for (int k = 0; k < 9; ++k) d[k] = (double*)((char*)ctx->d_stage + k * ((nd + 255) & ~(size_t)255));
+    const double* hin[5] = {x_fp, dx_fp, y_fp, dy_fp, fry};
+    for (int k = 0; k < 5; ++k)
+        if (hin[k]) HIPCHK(hipMemcpy(d[k], hin[k], nd, hipMemcpyHostToDevice));
+    int rc = simc_recon_run(ctx, arm, n, d[0], d[1], d[2], d[3], fry ? d[4] : nullptr, d[5], d[6], d[7], d[8],
+                            nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    double* hout[4] = {delta, dxdz_tar, dydz_tar, y_tar};
+    for (int k = 0; k < 4; ++k) HIPCHK(hipMemcpy(hout[k], d[5 + k], nd, hipMemcpyDeviceToHost));
+    return SIMC_OK;
 }
 
 int simc_hms_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,

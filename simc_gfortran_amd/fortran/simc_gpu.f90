@@ -1,0 +1,213 @@
+! Fortran interface (ISO_C_BINDING) of libsimc_gpu.so, the GPU spectrometer
+! transport + reconstruction library declared in include/simc_gpu.h.
+!
+! A SIMC program that batches its events calls simc_arm_run_host (host
+! arrays) or simc_arm_run (device arrays, e.g. from hipfort) in place of one
+! mc_hms / mc_shms call per event (hms/mc_hms.f:1-4, shms/mc_shms.f:1-4);
+! simc_counters_get returns the STOP counters in the order of the common
+! blocks /spectrometer1/ and /spectrometer_shms/.  Every function returns 0
+! or a negative SIMC_E_* code; simc_error_message() describes the last error.
+module simc_gpu
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: SIMC_OK = 0
+  integer(c_int), parameter :: SIMC_ARM_HMS = 1, SIMC_ARM_SHMS = 5
+  integer(c_int32_t), parameter :: SIMC_RNG_TAPE = 0, SIMC_RNG_PHILOX = 1
+  integer(c_int32_t), parameter :: SIMC_FIT_FP64 = 0, SIMC_FIT_REF_F32 = 1
+  integer(c_int), parameter :: SIMC_HMS_NCOUNTERS = 22, SIMC_SHMS_NCOUNTERS = 52
+
+  type, bind(C) :: simc_settings
+    real(c_double)     :: p_spec = 0, m2 = 0, ctau = 0
+    integer(c_int32_t) :: ms_flag = 1, wcs_flag = 1, decay_flag = 0, using_coll = 0
+    integer(c_int32_t) :: fit_mode = SIMC_FIT_REF_F32, reserved = 0
+    real(c_double)     :: wire_sigma = 0
+  end type
+
+  type, bind(C) :: simc_events_in
+    type(c_ptr) :: dpp = c_null_ptr, x = c_null_ptr, y = c_null_ptr, z = c_null_ptr
+    type(c_ptr) :: dxdz = c_null_ptr, dydz = c_null_ptr, fry = c_null_ptr
+  end type
+
+  type, bind(C) :: simc_events_out
+    type(c_ptr) :: dpp = c_null_ptr, y = c_null_ptr, dxdz = c_null_ptr, dydz = c_null_ptr
+    type(c_ptr) :: x_fp = c_null_ptr, dx_fp = c_null_ptr, y_fp = c_null_ptr, dy_fp = c_null_ptr
+    type(c_ptr) :: pathlen = c_null_ptr, resmult = c_null_ptr, m2 = c_null_ptr
+    type(c_ptr) :: ok = c_null_ptr, stop = c_null_ptr, used = c_null_ptr
+  end type
+
+  type, bind(C) :: simc_rng
+    integer(c_int32_t) :: kind = SIMC_RNG_PHILOX, reserved = 0
+    integer(c_int64_t) :: seed = 0, event0 = 0
+    type(c_ptr)        :: tape = c_null_ptr
+    integer(c_int64_t) :: stride = 0
+  end type
+
+  interface
+    integer(c_int) function simc_ctx_create(map_dir, ctx) bind(C, name="simc_ctx_create")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: map_dir
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function simc_ctx_create_files(hms_forward, hms_recon, shms_forward, shms_recon, ctx) &
+        bind(C, name="simc_ctx_create_files")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: hms_forward, hms_recon, shms_forward, shms_recon
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    subroutine simc_ctx_destroy(ctx) bind(C, name="simc_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+
+    integer(c_int) function simc_arm_run(ctx, arm, s, evin, evout, n, rng, stream) bind(C, name="simc_arm_run")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: arm
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_hms_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_hms_run")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_shms_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_shms_run")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_arm_run_host(ctx, arm, s, evin, evout, n, rng) bind(C, name="simc_arm_run_host")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_recon_run(ctx, arm, n, x_fp, dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, &
+        dydz_tar, y_tar, stream) bind(C, name="simc_recon_run")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, x_fp, dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar, stream
+      integer(c_int), value :: arm
+      integer(c_int64_t), value :: n
+    end function
+
+    integer(c_int) function simc_recon_run_host(ctx, arm, n, x_fp, dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, &
+        dydz_tar, y_tar) bind(C, name="simc_recon_run_host")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: x_fp(*), dx_fp(*), y_fp(*), dy_fp(*), fry(*)
+      real(c_double), intent(out) :: delta(*), dxdz_tar(*), dydz_tar(*), y_tar(*)
+    end function
+
+    integer(c_int) function simc_arm_run_timed(ctx, arm, s, evin, evout, n, rng, stream, stage_events) &
+        bind(C, name="simc_arm_run_timed")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: arm
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+      type(c_ptr), intent(in) :: stage_events(4)
+    end function
+
+    integer(c_int) function simc_map_class_info(ctx, arm, cls, driftdist, adrift, n_terms) &
+        bind(C, name="simc_map_class_info")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm, cls
+      real(c_double), intent(out) :: driftdist
+      integer(c_int32_t), intent(out) :: adrift, n_terms
+    end function
+
+    integer(c_int64_t) function simc_poly_flops_full(ctx, arm) bind(C, name="simc_poly_flops_full")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm
+    end function
+
+    integer(c_int) function simc_stop_flops(ctx, arm, flops, nflops) bind(C, name="simc_stop_flops")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm, nflops
+      integer(c_int64_t), intent(out) :: flops(*)
+    end function
+
+    integer(c_int) function simc_stop_flops_stage(ctx, arm, stage, flops, nflops) &
+        bind(C, name="simc_stop_flops_stage")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm, stage, nflops
+      integer(c_int64_t), intent(out) :: flops(*)
+    end function
+
+    integer(c_int) function simc_counters_get(ctx, arm, c, nc) bind(C, name="simc_counters_get")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm, nc
+      integer(c_int64_t), intent(out) :: c(*)
+    end function
+
+    integer(c_int) function simc_counters_reset(ctx, arm) bind(C, name="simc_counters_reset")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm
+    end function
+
+    type(c_ptr) function simc_last_error() bind(C, name="simc_last_error")
+      import :: c_ptr
+    end function
+
+    type(c_ptr) function simc_version() bind(C, name="simc_version")
+      import :: c_ptr
+    end function
+
+    integer(c_size_t) function c_strlen(p) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: p
+    end function
+  end interface
+
+contains
+
+  ! The message of the last failed call on this thread.
+  function simc_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: i, n
+    p = simc_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = chars(i)
+    end do
+  end function
+
+end module simc_gpu
