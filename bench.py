@@ -115,6 +115,108 @@ def pmc_summary(fetch_csv, write_csv, events):
     print(json.dumps(res))
 
 
+def plain_rate(ctx, wl, n, steps=3):
+    """Events/s of one large single-setting run of workload wl (n events)."""
+    import torch
+    from simc_gfortran_amd import workloads as W
+    dev = torch.device("cuda:0")
+    ev = W.generate_device(wl, n, seed=wl.seed, device=dev)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma, fit="fp64")
+    out = None
+    ts = []
+    for s in range(steps + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        out = ctx.run(wl.arm, ev, rng="philox", seed=1234, event0=s * n, out=out, **kw)
+        b.record()
+        torch.cuda.synchronize()
+        if s:
+            ts.append(a.elapsed_time(b))
+    return n / (min(ts) * 1e-3)
+
+
+def bench_batched(ctx, line, args):
+    """The batched form: 64 mixed HMS/SHMS settings, 1.5e8 events in one run,
+    launch tile from the autotuner, the launch sequence captured in a HIP
+    graph and replayed.  Compared with the same settings run one plain call
+    each, and with the single-setting rates of large HMS and SHMS runs
+    weighted by the batch's event mix."""
+    import torch
+    from simc_gfortran_amd import api
+    from simc_gfortran_amd import workloads as W
+    dev = torch.device("cuda:0")
+    sets = W.mixed_settings(64)
+    gap = 0
+    total = sum(w.n for w in sets)
+    f64 = dict(dtype=torch.float64, device=dev)
+    ev = {k: torch.empty(total, **f64) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
+    settings, off = [], 0
+    for w in sets:
+        W.generate_device(w, w.n, seed=w.seed, device=dev, out=ev, at=off)
+        settings.append(dict(arm=w.arm, p_spec=w.p_spec, m2=w.m2, wire_sigma=w.wire_sigma, fit="fp64",
+                             seed=1234, event0=off, offset=off, count=w.n))
+        off += w.n + gap
+    b = api.Batch(ctx, settings)
+    out = b.new_outputs(ev)
+    chunk, tune_ms = b.tune(ev, out)
+    # capture the launch sequence and time graph replays
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        b.run(ev, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.run(ev, out=out)
+    ts = []
+    for r in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        if r:
+            ts.append(e0.elapsed_time(e1))
+    batch_ms = float(np.mean(ts))
+    batch_rate = total / (batch_ms * 1e-3)
+    accepted = int(out["ok"].sum().item())
+    del g
+    # the same settings, one plain call each
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    outs = b.new_outputs(ev)
+    for rep in range(2):
+        t0.record()
+        for d in settings:
+            sl = slice(d["offset"], d["offset"] + d["count"])
+            ctx.run(d["arm"], {k: v[sl] for k, v in ev.items()}, rng="philox", seed=1234,
+                    event0=d["event0"], out={k: v[sl] for k, v in outs.items()},
+                    **{k: d[k] for k in ("p_spec", "m2", "wire_sigma", "fit")})
+        t1.record()
+        torch.cuda.synchronize()
+    per_setting_ms = t0.elapsed_time(t1)
+    b.close()
+    del ev, out, outs
+    torch.cuda.empty_cache()
+    # single-setting rates of large runs, weighted by the batch's event mix
+    n_hms = sum(w.n for w in sets if w.arm == 1)
+    n_shms = total - n_hms
+    r_hms = plain_rate(ctx, W.HMS_C0, 10_000_000)
+    r_shms = line["value"]
+    single_mix = total / (n_hms / r_hms + n_shms / r_shms)
+    return {"settings": len(sets), "events": total, "hms_events": n_hms, "shms_events": n_shms,
+            "events_per_s": batch_rate, "ms": batch_ms, "hip_graph": True, "tuned_chunk": chunk,
+            "tune_best_ms": tune_ms, "accepted_fraction": accepted / total,
+            "per_setting_plain_events_per_s": total / (per_setting_ms * 1e-3),
+            "vs_per_setting_plain": batch_rate / (total / (per_setting_ms * 1e-3)),
+            "single_setting_rates": {"hms_1e7": r_hms, "shms_1e7": r_shms},
+            "single_setting_mix_events_per_s": single_mix,
+            "vs_single_setting": batch_rate / single_mix,
+            "data": "synthetic: 64 settings alternating HMS protons / SHMS electrons, p0 U(1,9) GeV/c, "
+                    "counts log-uniform in [1e4,1e7] scaled to 1.5e8 total"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +224,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--events", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched multi-setting and adjoint legs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one elementwise kernel reading and writing this many MiB "
                          "(HBM counter calibration)")
@@ -236,6 +340,10 @@ def main():
         "step_ms_hip_events": step_ms,
         "accepted_fraction": accepted / n,
     }
+    if not args.no_forms:
+        del d_ev, out
+        torch.cuda.empty_cache()
+        line["batched"] = bench_batched(ctx, line, args)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb

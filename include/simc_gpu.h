@@ -147,6 +147,46 @@ int simc_arm_run_timed(simc_ctx *ctx, int arm, const simc_settings *s, const sim
                        simc_events_out *out, int64_t n, const simc_rng *rng, void *stream,
                        void *const *stage_events);
 
+/* ---- Batched form: many kinematic settings in one run -------------------
+ *
+ * Each setting is one SIMC run's worth of events for one arm (the by-value
+ * arguments of mc_hms / mc_shms, its own Philox key and event numbering) over
+ * a range [offset, offset + count) of the event arrays.  Settings may mix
+ * arms and particle types; their ranges must not overlap and must lie in
+ * [0, 2^29).  simc_batch_create groups the settings by arm and kernel
+ * variant, uploads their table and reserves all device memory, so that
+ * simc_batch_run only launches kernels on `stream` (it may be captured into
+ * a HIP graph).  Counters are kept per setting.  Philox streams only:
+ * event j of a setting draws from stream (seed, event0 + j), so settings
+ * sharing a seed need disjoint event-id ranges; settings are launched
+ * together when they share arm, flags and seed. */
+typedef struct {
+    int32_t arm;            /* SIMC_ARM_HMS or SIMC_ARM_SHMS                    */
+    int32_t reserved;
+    simc_settings s;        /* this setting's scalars                           */
+    uint64_t seed;          /* Philox key of this setting's streams             */
+    uint64_t event0;        /* global id of its first event                     */
+    int64_t offset;         /* index of its first event in the event arrays     */
+    int64_t count;          /* its number of events                             */
+} simc_setting;
+
+typedef struct simc_batch simc_batch;
+
+int simc_batch_create(simc_ctx *ctx, const simc_setting *settings, int n_settings, simc_batch **out);
+/* in / out: device arrays covering every setting's range (the plain run's
+ * semantics per event). */
+int simc_batch_run(simc_batch *b, const simc_events_in *in, simc_events_out *out, void *stream);
+/* STOP counters of setting i (input order) from the last run, common-block order. */
+int simc_batch_counters(simc_batch *b, int setting, int64_t *c, int ncounters);
+/* Launch tile: events per launch sequence (256 .. 2^24; reserves queues). */
+int simc_batch_set_chunk(simc_batch *b, int64_t chunk);
+/* Autotuner: times the run (warm-up + 3 repetitions) for launch tiles of
+ * 2^18 .. 2^24 events, keeps the fastest and returns it and its time.  The
+ * runs write `out` (use outputs that do not alias the inputs). */
+int simc_batch_tune(simc_batch *b, const simc_events_in *in, simc_events_out *out, void *stream,
+                    int64_t *best_chunk, double *best_ms);
+void simc_batch_destroy(simc_batch *b);
+
 /* Reconstruction only (mc_hms_recon / mc_shms_recon) of focal-plane tracks:
  * x_fp, y_fp in cm, slopes in rad, fry in cm.  Outputs delta (%), x'_tar,
  * y'_tar (rad), y_tar (cm). */

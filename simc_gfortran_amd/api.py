@@ -49,6 +49,8 @@ EXPORTS = [
     "simc_shms_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
+    "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
+    "simc_batch_tune", "simc_batch_destroy",
     "simc_last_error",
     "simc_version",
 ]
@@ -80,6 +82,11 @@ class Rng(C.Structure):
                 ("event0", C.c_uint64), ("tape", _vp), ("stride", C.c_int64)]
 
 
+class Setting(C.Structure):
+    _fields_ = [("arm", C.c_int32), ("reserved", C.c_int32), ("s", Settings), ("seed", C.c_uint64),
+                ("event0", C.c_uint64), ("offset", C.c_int64), ("count", C.c_int64)]
+
+
 class SimcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"simc error {code}: {msg}")
@@ -108,6 +115,14 @@ def library(path: str = LIB_PATH):
                                      C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp,
                                      C.POINTER(_vp)]
     L.simc_recon_run.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9 + [_vp]
+    L.simc_batch_create.argtypes = [_vp, C.POINTER(Setting), C.c_int, C.POINTER(_vp)]
+    L.simc_batch_run.argtypes = [_vp, C.POINTER(EventsIn), C.POINTER(EventsOut), _vp]
+    L.simc_batch_counters.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.simc_batch_set_chunk.argtypes = [_vp, C.c_int64]
+    L.simc_batch_tune.argtypes = [_vp, C.POINTER(EventsIn), C.POINTER(EventsOut), _vp,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.simc_batch_destroy.argtypes = [_vp]
+    L.simc_batch_destroy.restype = None
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
                                     C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng)]
     L.simc_recon_run_host.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9
@@ -312,3 +327,86 @@ def mc_shms(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_fla
         raise SimcError(-6, "mc_shms: spectr must be 5 (the LSA tune is no longer supported)")
     return _arm_call(ctx, SHMS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
                      decay_flag, fry, pathlen, using_SHMScoll, ctau, **kw)
+
+
+class Batch:
+    """Many kinematic settings transported in one run (simc_batch_*).
+
+    settings: list of dicts with keys arm, p_spec, m2, offset, count and
+    optionally ms, wcs, decay, ctau, coll, fit, wire_sigma, seed, event0 (the
+    run() keywords).  Event j of a setting is element offset + j of the event
+    arrays and draws Philox stream (seed, event0 + j)."""
+
+    def __init__(self, ctx: Spectrometers, settings):
+        arr = (Setting * len(settings))()
+        for i, d in enumerate(settings):
+            S = Settings(d["p_spec"], d["m2"], d.get("ctau", 0.0), int(d.get("ms", True)),
+                         int(d.get("wcs", True)), int(d.get("decay", False)), int(d.get("coll", False)),
+                         FIT_REF_F32 if d.get("fit", "ref_f32") == "ref_f32" else FIT_FP64, 0,
+                         d.get("wire_sigma", 0.0))
+            arr[i] = Setting(d["arm"], 0, S, int(d.get("seed", 0)), int(d.get("event0", 0)),
+                             int(d["offset"]), int(d["count"]))
+        self.ctx = ctx
+        self.settings = list(settings)
+        self.n_arr = max(int(d["offset"]) + int(d["count"]) for d in settings)
+        h = _vp()
+        _check(library().simc_batch_create(ctx._h, arr, len(settings), C.byref(h)))
+        self._h = h
+
+    def _structs(self, events, out):
+        n = self.n_arr
+        ev = {k: _dev_f64(events[k], events[k].numel(), k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
+        if any(t.numel() < n for t in ev.values()):
+            raise ValueError(f"event arrays must hold at least {n} elements")
+        fry = events.get("fry")
+        ein = EventsIn(*[_ptr(ev.get(k) if k not in ("z", "fry") else None) for k in
+                         ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
+        ein.fry = _ptr(fry)
+        eout = EventsOut(*[_ptr(out.get(k)) for k in OUT_FIELDS])
+        return ein, eout
+
+    def new_outputs(self, events):
+        dev = events["dpp"].device
+        n = events["dpp"].numel()
+        out = {k: events[k].clone() for k in ("dpp", "y", "dxdz", "dydz")}
+        for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen"):
+            out[k] = torch.zeros(n, dtype=torch.float64, device=dev)
+        out["ok"] = torch.zeros(n, dtype=torch.uint8, device=dev)
+        out["stop"] = torch.zeros(n, dtype=torch.uint8, device=dev)
+        return out
+
+    def run(self, events, out=None, stream=None):
+        out = self.new_outputs(events) if out is None else out
+        ein, eout = self._structs(events, out)
+        st = stream if stream is not None else torch.cuda.current_stream(events["dpp"].device).cuda_stream
+        _check(library().simc_batch_run(self._h, C.byref(ein), C.byref(eout), C.c_void_p(st)))
+        return out
+
+    def counters(self, i):
+        names = HMS_COUNTERS if self.settings[i]["arm"] == HMS else SHMS_COUNTERS
+        buf = (C.c_int64 * len(names))()
+        _check(library().simc_batch_counters(self._h, i, buf, len(names)))
+        return dict(zip(names, list(buf)))
+
+    def set_chunk(self, chunk):
+        _check(library().simc_batch_set_chunk(self._h, int(chunk)))
+
+    def tune(self, events, out, stream=None):
+        """Autotune the launch tile; returns (chunk, ms)."""
+        ein, eout = self._structs(events, out)
+        st = stream if stream is not None else torch.cuda.current_stream(events["dpp"].device).cuda_stream
+        c, ms = C.c_int64(), C.c_double()
+        _check(library().simc_batch_tune(self._h, C.byref(ein), C.byref(eout), C.c_void_p(st),
+                                         C.byref(c), C.byref(ms)))
+        return c.value, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().simc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -251,10 +251,12 @@ struct Track {
     int dflag;
 };
 
+// Per-event settings of the run (one kinematic setting).
 struct Flags {
-    double ctau;
+    double ctau;     // decay length c tau (cm)
+    double p_spec;   // central momentum (MeV/c)
+    double sigma;    // wire-chamber resolution (cm)
     int ms, wcs, decay, coll, fit_f32;
-    double wire_sigma;
 };
 
 // Rotation constants tan/sin/cos of rotang * 0.017453292 (the truncated

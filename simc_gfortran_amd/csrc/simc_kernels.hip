@@ -10,6 +10,7 @@
 // the context's device counters with one atomic per non-empty bin.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,14 +57,32 @@ struct ArmTables {
     double zdc[12];    // wire-plane z positions, planes 1..12
 };
 
-// Per-launch arguments.  Array pointers address the first event of the
-// launch; all event arrays are reached through buffer resources with one
-// 32-bit byte offset per lane, so no 64-bit per-lane addresses are kept.
+// One kinematic setting: the by-value arguments of mc_hms / mc_shms and the
+// setting's place in the launch.  Events of a launch are numbered by a pass
+// index P; the setting s covers P in [start, start + count) and its event j
+// = P - start is element offset + j of the event arrays and event id
+// event0 + j of the random streams.
+struct SetDev {
+    double p_spec, m2, ctau, sigma;
+    uint64_t start, seed, event0, offset;
+    int32_t flags;   // KF_* bits (the same for every setting of a launch)
+    uint32_t hist;   // index of the setting's stop-code histogram
+};
+
+// Per-launch arguments.  The event arrays are reached through buffer
+// resources (n_arr elements from the base pointers, one 32-bit byte offset
+// per lane), so no 64-bit per-lane addresses are kept.  A plain run has one
+// setting, passed by value (set == nullptr); a batched run reads its table.
 struct KArgs {
-    double p_spec, m2, ctau, sigma, wire_sigma;
-    int32_t flags;        // KF_* bits
-    uint32_t count;       // events in this launch
-    uint64_t seed, event0;
+    SetDev set0;
+    const SetDev* set;    // n_set settings sorted by start, or nullptr
+    uint32_t n_set;
+    int32_t flags;        // KF_* bits of every setting of the launch (wave-uniform control flow)
+    uint64_t seed;        // Philox key of every setting of the launch (uniform round keys)
+    uint32_t count;       // events in this launch: pass indices pass_base + [0, count)
+    uint64_t pass_base;
+    uint64_t ix_base;     // element of the event arrays that the base pointers address
+    uint32_t n_arr;       // elements addressable from the base pointers
     const ArmTables* tab;
     const double* coef;
     const Op* ops;
@@ -71,9 +90,9 @@ struct KArgs {
     double *o_dpp, *o_y, *o_dxdz, *o_dydz, *x_fp, *dx_fp, *y_fp, *dy_fp, *pathlen, *resmult, *m2o;
     uint8_t *ok, *stop;
     int32_t* used;
-    const double* tape;
+    const double* tape;   // tape slot of launch event li at li * stride_bytes (plain runs)
     uint32_t stride_bytes;
-    unsigned long long* hist;
+    unsigned long long* hist;   // per-setting histograms of kHistSlots bins
 };
 enum : int32_t { KF_MS = 1, KF_WCS = 2, KF_DECAY = 4, KF_COLL = 8, KF_FIT_F32 = 16 };
 
@@ -304,7 +323,7 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
                     cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
                     cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
                     cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
-                    if (!coll_step(g, t, F, cg, a.p_spec, coll_cnt)) st = ST_COLL;
+                    if (!coll_step(g, t, F, cg, F.p_spec, coll_cnt)) st = ST_COLL;
                 } else {
                     const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
                     int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
@@ -359,7 +378,7 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
                 }
                 break;
             case OP_WIRE: {  // plane ops[pc].stop (0-based); y plane when arg = 1
-                const double h = ops[pc].arg ? t.ys + a.sigma * d3 * resmult : t.xs + a.sigma * d2 * resmult;
+                const double h = ops[pc].arg ? t.ys + F.sigma * d3 * resmult : t.xs + F.sigma * d2 * resmult;
                 lds_hits[ops[pc].stop * kBlock + threadIdx.x] = h;
                 break;
             }
@@ -378,15 +397,15 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
                 for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
                 double x_fp, dx_fp, y_fp, dy_fp;
                 fit_track(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
-                st64(a.x_fp, a.count, off, x_fp);
-                st64(a.dx_fp, a.count, off, dx_fp);
-                st64(a.y_fp, a.count, off, y_fp);
-                st64(a.dy_fp, a.count, off, dy_fp);
+                st64(a.x_fp, a.n_arr, off, x_fp);
+                st64(a.dx_fp, a.n_arr, off, dx_fp);
+                st64(a.y_fp, a.n_arr, off, y_fp);
+                st64(a.dy_fp, a.n_arr, off, dy_fp);
                 break;
             }
             case OP_FIDUCIAL: {
-                const double xcal = ld64(a.x_fp, a.count, off) + ld64(a.dx_fp, a.count, off) * p0;
-                const double ycal = ld64(a.y_fp, a.count, off) + ld64(a.dy_fp, a.count, off) * p0;
+                const double xcal = ld64(a.x_fp, a.n_arr, off) + ld64(a.dx_fp, a.n_arr, off) * p0;
+                const double ycal = ld64(a.y_fp, a.n_arr, off) + ld64(a.dy_fp, a.n_arr, off) * p0;
                 if (ycal > p1 || ycal < ops[pc].p[2] || xcal > ops[pc].p[3] || xcal < ops[pc].p[4])
                     st = ops[pc].stop;
                 break;
@@ -418,30 +437,65 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
 // Survivor queue (structure of arrays, capacity `cap` entries).
 struct Queue {
     uint32_t* count;     // entries appended (device counter, zeroed per launch)
-    uint32_t *idx, *rng, *bits;   // event index in the launch, stream word, dflag
+    uint32_t *idx, *rng, *bits;   // launch event index, stream word, dflag | setting << 1
     double *pend, *xs, *ys, *dxdzs, *dydzs, *dpps, *p, *m2, *pathlen, *decdist, *resmult;
     uint32_t cap;
 };
 
-__device__ __forceinline__ Flags flags_of(const KArgs& a)
+__device__ __forceinline__ SetDev setting_of(const KArgs& a, uint32_t s)
+{
+    return a.set ? a.set[s] : a.set0;
+}
+
+// The setting of pass index P: the last s with start <= P (binary search).
+__device__ __forceinline__ uint32_t find_setting(const KArgs& a, uint64_t P)
+{
+    if (!a.set) return 0;
+    uint32_t lo = 0, hi = a.n_set;   // invariant: start[lo] <= P < start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.set[mid].start <= P) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ Flags flags_of(const KArgs& a, const SetDev& S)
 {
     Flags F;
-    F.ctau = a.ctau;
+    F.ctau = S.ctau;
+    F.p_spec = S.p_spec;
+    F.sigma = S.sigma;
     F.ms = (a.flags & KF_MS) != 0;
     F.wcs = (a.flags & KF_WCS) != 0;
     F.decay = (a.flags & KF_DECAY) != 0;
     F.coll = (a.flags & KF_COLL) != 0;
     F.fit_f32 = (a.flags & KF_FIT_F32) != 0;
-    F.wire_sigma = a.wire_sigma;
     return F;
 }
 
-// Outputs of a finished event li (byte offset off = 8 li).
+// Identity of launch event li: its setting, the byte offset of its element
+// in the event arrays and its random-stream event id.
+struct EvId {
+    uint32_t s, off;
+    uint64_t event;
+};
+__device__ __forceinline__ EvId event_id(const KArgs& a, const SetDev& S, uint32_t s, uint32_t li)
+{
+    const uint64_t j = a.pass_base + li - S.start;
+    EvId e;
+    e.s = s;
+    e.off = (uint32_t)(S.offset + j - a.ix_base) * 8u;
+    e.event = S.event0 + j;
+    return e;
+}
+
+// Outputs of a finished event (byte offset off in the event arrays).
 template <int ARM, class R>
-__device__ __forceinline__ void finish_event(const KArgs& a, uint32_t li, int st, const R& g, const Track& t,
+__device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int st, const R& g, const Track& t,
                                              double resmult)
 {
-    const uint32_t n = a.count, off = li * 8u;
+    const uint32_t n = a.n_arr;
     if (st == ST_OK) {
         EvOut o, r;
         o.x_fp = ld64(a.x_fp, n, off);
@@ -463,9 +517,19 @@ __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t li, int st
     st64(a.pathlen, n, off, t.pathlen);
     if (a.resmult) st64(a.resmult, n, off, resmult);
     if (a.m2o) st64(a.m2o, n, off, t.m2);
-    if (a.used) st32(a.used, n, li * 4u, (uint32_t)g.used());
-    st8(a.ok, n, li, st == ST_OK ? 1 : 0);
-    st8(a.stop, n, li, (uint8_t)st);
+    if (a.used) st32(a.used, n, off >> 1, (uint32_t)g.used());
+    st8(a.ok, n, off >> 3, st == ST_OK ? 1 : 0);
+    st8(a.stop, n, off >> 3, (uint8_t)st);
+}
+
+// Stop-code counts: the workgroup's LDS histogram serves the setting of its
+// first event; events of other settings (at setting boundaries) count
+// directly in device memory.
+__device__ __forceinline__ void count_stop(const KArgs& a, unsigned* hist, uint32_t s_block, uint32_t s, int bin,
+                                           unsigned n)
+{
+    if (s == s_block) atomicAdd(&hist[bin], n);
+    else atomicAdd(&a.hist[(size_t)setting_of(a, s).hist * kHistSlots + bin], (unsigned long long)n);
 }
 
 // Stage kernel.  FROM_Q: events come from queue qin (else from the inputs);
@@ -473,33 +537,42 @@ __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t li, int st
 // The common variants are held to 168 VGPRs (three wavefronts per SIMD),
 // which the hut stage would exceed slightly on its own.
 template <int ARM, class R, bool RARE, int STAGE, bool FROM_Q, bool TO_Q>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : 3))) void stage_kernel(KArgs a, Queue qin, Queue qout, int32_t pc_begin,
-                                                       int32_t pc_end)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : 3))) void stage_kernel(
+    KArgs a, Queue qin, Queue qout, int32_t pc_begin, int32_t pc_end)
 {
     __shared__ unsigned hist[kHistSlots];
     __shared__ uint32_t wave_n[kBlock / 64];
     __shared__ uint32_t block_base;
+    __shared__ uint32_t block_set;
     __shared__ double lds_hits[STAGE == STAGE_HUT ? 12 * kBlock : 1];
     uint32_t n_in = a.count;
     if (FROM_Q) {
         n_in = *qin.count;  // complete: written by the previous launch
         if (blockIdx.x * kBlock >= n_in) return;  // whole workgroup past the queue
     }
-    for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x) hist[k] = 0;
-    __syncthreads();
-    unsigned coll_cnt[3] = {0, 0, 0};
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x) hist[k] = 0;
+    if (threadIdx.x == 0) {
+        const uint32_t i0 = blockIdx.x * kBlock;
+        block_set = FROM_Q ? (ld32(qin.bits, qin.cap, i0 * 4u) >> 1) : find_setting(a, a.pass_base + i0);
+    }
+    __syncthreads();
+    const uint32_t s_block = block_set;
+    unsigned coll_cnt[3] = {0, 0, 0};
     bool survivor = false;
-    uint32_t li = 0;
+    uint32_t li = 0, s = s_block;
     R g;
     Track t;
     double resmult = 0.0;
     if (i < n_in) {
-        const Flags F = flags_of(a);
         if (FROM_Q) {
             const uint32_t c = qin.cap, o8 = i * 8u, o4 = i * 4u;
             li = ld32(qin.idx, c, o4);
-            g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
+            const uint32_t bits = ld32(qin.bits, c, o4);
+            s = bits >> 1;
+            const SetDev S = setting_of(a, s);
+            const EvId e = event_id(a, S, s, li);
+            g.init(a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
             g.resume(ld32(qin.rng, c, o4), ld64(qin.pend, c, o8));
             t.xs = ld64(qin.xs, c, o8);
             t.ys = ld64(qin.ys, c, o8);
@@ -510,37 +583,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             t.m2 = ld64(qin.m2, c, o8);
             t.pathlen = ld64(qin.pathlen, c, o8);
             t.decdist = ld64(qin.decdist, c, o8);
-            t.dflag = (int)ld32(qin.bits, c, o4);
+            t.dflag = (int)(bits & 1u);
             resmult = ld64(qin.resmult, c, o8);
         } else {
             li = i;
-            const uint32_t n = a.count, off = li * 8u;
-            g.init(a.seed, a.event0 + li, a.tape, li * a.stride_bytes, a.stride_bytes);
+            s = find_setting(a, a.pass_base + li);
+        }
+        const SetDev S = setting_of(a, s);
+        const EvId e = event_id(a, S, s, li);
+        const Flags F = flags_of(a, S);
+        if (!FROM_Q) {
+            const uint32_t n = a.n_arr, off = e.off;
+            g.init(a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
             t.xs = ld64(a.x, n, off);
             t.ys = ld64(a.y, n, off);
             t.dxdzs = ld64(a.dxdz, n, off);
             t.dydzs = ld64(a.dydz, n, off);
             t.dpps = ld64(a.dpp, n, off);
-            t.p = a.p_spec * (1.0 + t.dpps / 100.0);
-            t.m2 = a.m2;
+            t.p = S.p_spec * (1.0 + t.dpps / 100.0);
+            t.m2 = S.m2;
             t.pathlen = ld64(a.pathlen, n, off);
             t.decdist = 0.0;
             t.dflag = 0;
             resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
         }
-        int st = run_program<ARM, R, RARE, STAGE>(g, t, a, F, li * 8u, lds_hits, coll_cnt, resmult, pc_begin,
+        int st = run_program<ARM, R, RARE, STAGE>(g, t, a, F, e.off, lds_hits, coll_cnt, resmult, pc_begin,
                                                    pc_end);
         if (g.overflow) st = ST_TAPE;
         if (TO_Q && st == ST_OK) {
             survivor = true;
         } else {
-            finish_event<ARM>(a, li, st, g, t, resmult);
-            atomicAdd(&hist[st], 1u);
+            finish_event<ARM>(a, e.off, st, g, t, resmult);
+            count_stop(a, hist, s_block, s, st, 1u);
+        }
+        if (coll_cnt[0] | coll_cnt[1] | coll_cnt[2]) {
+            for (int k = 0; k < 3; ++k)
+                if (coll_cnt[k]) count_stop(a, hist, s_block, s, kHistBins + k, coll_cnt[k]);
         }
     }
-    if (coll_cnt[0]) atomicAdd(&hist[kHistBins + 0], coll_cnt[0]);
-    if (coll_cnt[1]) atomicAdd(&hist[kHistBins + 1], coll_cnt[1]);
-    if (coll_cnt[2]) atomicAdd(&hist[kHistBins + 2], coll_cnt[2]);
 
     if (TO_Q) {
         // Queue slots: rank within the wavefront, wavefront offset within
@@ -562,7 +642,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             const uint32_t c = qout.cap, o8 = slot * 8u, o4 = slot * 4u;
             st32(qout.idx, c, o4, li);
             st32(qout.rng, c, o4, g.state_word());
-            st32(qout.bits, c, o4, (uint32_t)t.dflag);
+            st32(qout.bits, c, o4, (uint32_t)t.dflag | (s << 1));
             st64(qout.pend, c, o8, g.pending);
             st64(qout.xs, c, o8, t.xs);
             st64(qout.ys, c, o8, t.ys);
@@ -577,8 +657,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
         }
     }
     __syncthreads();
+    const uint32_t hist_index = setting_of(a, s_block).hist;
     for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x)
-        if (hist[k]) atomicAdd(&a.hist[k], (unsigned long long)hist[k]);
+        if (hist[k]) atomicAdd(&a.hist[(size_t)hist_index * kHistSlots + k], (unsigned long long)hist[k]);
 }
 
 // Queue counter reset (a kernel rather than a memset, so that a captured
@@ -586,6 +667,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
 __global__ void zero_u32x2(uint32_t* p)
 {
     if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+}
+
+__global__ void zero_u64(unsigned long long* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 template <int ARM>
@@ -646,6 +733,27 @@ struct simc_ctx {
     size_t stage_bytes = 0;
     int device = 0;
     int n_cu = 256;
+};
+
+// A prepared multi-setting run: settings grouped into passes of one arm and
+// one kernel variant; each pass is one event range split into launch chunks.
+struct simc_batch {
+    struct Pass {
+        int arm;
+        bool rare;
+        int32_t flags;      // KF_* bits shared by the pass's settings
+        uint64_t seed;      // Philox key shared by the pass's settings
+        uint64_t total;     // events of the pass
+        uint32_t first, n;  // its settings in d_set
+    };
+    simc_ctx* ctx = nullptr;
+    int n_settings = 0;
+    std::vector<int> arm_of;               // arm of each setting (input order)
+    std::vector<Pass> passes;
+    void* d_set = nullptr;                 // SetDev table, pass by pass
+    unsigned long long* d_hist = nullptr;  // one histogram per setting (its input order)
+    uint32_t n_arr = 0;                    // event-array length the settings address
+    int64_t chunk = (int64_t)1 << 24;
 };
 
 namespace {
@@ -1090,6 +1198,55 @@ int ensure_queues(simc_ctx* ctx, int ai, uint32_t cap, Queue& q1, Queue& q2)
 
 }  // namespace
 
+namespace {
+
+// Stop-code histogram (kHistSlots bins) -> the STOP counters of the arm's
+// common block, in its order.
+void hist_to_counters(int arm, const unsigned long long* h, int64_t* c, int ncounters)
+{
+    for (int i = 0; i < ncounters; ++i) c[i] = 0;
+    int64_t total = 0;
+    for (int i = 0; i < kHistBins; ++i) total += (int64_t)h[i];
+    auto H = [&](int code) { return (int64_t)h[code]; };
+    if (arm == SIMC_ARM_HMS) {
+        // order of common /spectrometer1/, hms/struct_hms.inc:34-42
+        c[0] = total;
+        c[1] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
+        c[2] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
+        c[3] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
+        for (int k = 0; k < 9; ++k) c[4 + k] = H(ST_H_Q1_IN + k);
+        c[13] = H(ST_H_D1_IN);
+        c[14] = H(ST_H_D1_OUT) + H(ST_H_PIPE0) + H(ST_H_PIPE1) + H(ST_H_PIPE2) + H(ST_H_PIPE3);
+        c[16] = H(ST_H_DC1);
+        c[17] = H(ST_H_DC2);
+        c[18] = H(ST_H_SCIN);
+        c[19] = 0;  // hSTOP_cal: the calorimeter cut is commented out, hms/mc_hms_hut.f:584-588
+        c[20] = H(ST_OK);
+        c[15] = c[16] + c[17] + c[18] + c[20];
+        c[21] = H(ST_COLL);
+    } else {
+        // order of common /spectrometer_shms/, shms/struct_shms.inc:26-44
+        c[0] = total;
+        c[1] = H(ST_OK);
+        c[5] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
+        c[6] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
+        c[7] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
+        for (int k = 0; k < 4; ++k) c[8 + k] = H(ST_S_HB_IN + k);
+        for (int k = 0; k < 15; ++k) c[12 + k] = H(ST_S_Q1_IN + k);
+        for (int k = 0; k < 12; ++k) c[27 + k] = H(ST_S_D1_IN + k);
+        c[42] = H(ST_S_DC1);
+        c[43] = H(ST_S_DC2);
+        c[46] = H(ST_S_S1X) + H(ST_S_S1Y);
+        c[47] = H(ST_S_S2Y);
+        c[48] = H(ST_S_S2X);
+        c[49] = H(ST_S_CAL) + H(ST_S_CALFID);
+        c[41] = c[42] + c[43] + c[46] + c[47] + c[48] + c[49] + c[1];
+        c[51] = H(ST_COLL);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* simc_last_error(void) { return g_err.c_str(); }
@@ -1153,79 +1310,58 @@ void simc_ctx_destroy(simc_ctx* ctx)
 
 namespace {
 
-int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
-            int64_t n, const simc_rng* rng, void* stream, void* const* stage_events)
+int check_settings(int arm, const simc_settings* s)
 {
-    if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run: NULL argument");
-    const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1) or SIMC_ARM_SHMS (5)");
-    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
-    if (n == 0) return SIMC_OK;
-    if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz)
-        return fail(SIMC_E_ARG, "missing event input array");
-    if (!out->dpp || !out->y || !out->dxdz || !out->dydz || !out->x_fp || !out->dx_fp || !out->y_fp ||
-        !out->dy_fp || !out->pathlen || !out->ok || !out->stop)
-        return fail(SIMC_E_ARG, "missing event output array");
+    if (arm_index(arm) < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1) or SIMC_ARM_SHMS (5)");
     if (!(s->p_spec > 0.0) || !(s->m2 >= 0.0)) return fail(SIMC_E_ARG, "p_spec must be > 0 and m2 >= 0");
     if (s->decay_flag && !(s->ctau > 0.0)) return fail(SIMC_E_ARG, "decay_flag needs ctau > 0");
-    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
-    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0))
-        return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
-    KArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.p_spec = s->p_spec;
-    a.m2 = s->m2;
-    a.ctau = s->ctau;
-    a.wire_sigma = s->wire_sigma;
+    return SIMC_OK;
+}
+
+// Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
+// shms/mc_shms.f:503); m2 changes only through a decay.
+bool needs_rare(const simc_settings* s)
+{
+    return s->decay_flag || (s->using_coll && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
+}
+
+SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t event0, uint64_t start,
+                   uint64_t offset, uint32_t hist)
+{
+    SetDev d;
+    std::memset(&d, 0, sizeof d);
+    d.p_spec = s->p_spec;
+    d.m2 = s->m2;
+    d.ctau = s->ctau;
     // hdc_sigma: hms/mc_hms_hut.f:143-144, shms/hut.inc:125-126
-    a.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_HMS ? 0.030 : 0.020);
-    a.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
+    d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_HMS ? 0.030 : 0.020);
+    d.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
               (s->using_coll ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
-    a.seed = rng->seed;
-    a.tab = ctx->d_tab[ai];
-    a.coef = ctx->d_coef[ai];
-    a.ops = ctx->d_ops[ai];
-    a.hist = ctx->d_hist[ai];
-    const bool tape = rng->kind == SIMC_RNG_TAPE;
-    // Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
-    // shms/mc_shms.f:503); m2 changes only through a decay.
-    const bool rare = s->decay_flag || (s->using_coll && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
-    // Launches are cut so that every byte offset of a launch fits 32 bits
-    // and each survivor queue stays at most 2^24 entries (1.7 GB).
-    int64_t chunk = (int64_t)1 << 24;
-    if (tape) {
-        const int64_t sb = rng->stride * 8;
-        if (sb > 0x7fffffffLL) return fail(SIMC_E_ARG, "tape stride too large");
-        const int64_t c = 0xffffffffLL / sb;
-        if (c < chunk) chunk = c;
-        a.stride_bytes = (uint32_t)sb;
-    }
-    Queue q1, q2;
-    int rc = ensure_queues(ctx, ai, (uint32_t)(n < chunk ? n : chunk), q1, q2);
-    if (rc) return rc;
+    d.start = start;
+    d.seed = seed;
+    d.event0 = event0;
+    d.offset = offset;
+    d.hist = hist;
+    return d;
+}
+
+bool outputs_complete(const simc_events_in* in, const simc_events_out* out)
+{
+    return in->dpp && in->x && in->y && in->dxdz && in->dydz && out->dpp && out->y && out->dxdz &&
+           out->dydz && out->x_fp && out->dx_fp && out->y_fp && out->dy_fp && out->pathlen && out->ok &&
+           out->stop;
+}
+
+// One launch chunk: counter reset and the three stages on `st`; `ev` (four
+// events or nullptr) marks the stage boundaries.
+int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, const Queue& q1, const Queue& q2,
+                  hipStream_t st, hipEvent_t* ev)
+{
+    const int ai = arm_index(arm);
     const int32_t coll = ctx->coll_pc[ai], hut = ctx->hut_pc[ai];
-    hipStream_t st = (hipStream_t)stream;
-    for (int64_t b = 0; b < n; b += chunk) {
-        const int64_t cnt = n - b < chunk ? n - b : chunk;
-        a.count = (uint32_t)cnt;
-        a.event0 = rng->event0 + (uint64_t)b;
-        a.dpp = in->dpp + b; a.x = in->x + b; a.y = in->y + b; a.dxdz = in->dxdz + b; a.dydz = in->dydz + b;
-        a.fry = in->fry ? in->fry + b : nullptr;
-        a.o_dpp = out->dpp + b; a.o_y = out->y + b; a.o_dxdz = out->dxdz + b; a.o_dydz = out->dydz + b;
-        a.x_fp = out->x_fp + b; a.dx_fp = out->dx_fp + b; a.y_fp = out->y_fp + b; a.dy_fp = out->dy_fp + b;
-        a.pathlen = out->pathlen + b;
-        a.resmult = out->resmult ? out->resmult + b : nullptr;
-        a.m2o = out->m2 ? out->m2 + b : nullptr;
-        a.ok = out->ok + b;
-        a.stop = out->stop + b;
-        a.used = out->used ? out->used + b : nullptr;
-        a.tape = tape ? rng->tape + b * rng->stride : nullptr;
-        const unsigned blocks = (unsigned)((cnt + kBlock - 1) / kBlock);
-        const dim3 G(blocks), B(kBlock);
-        hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, q1.count);
-        // stage_events: boundaries of the first chunk's three stages
-        hipEvent_t* ev = (stage_events && b == 0) ? (hipEvent_t*)stage_events : nullptr;
-        if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    const dim3 G((a.count + kBlock - 1) / kBlock), B(kBlock);
+    hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, q1.count);
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
 #define LAUNCH3(ARMV, RV, RARE)                                                                             \
     do {                                                                                                    \
         hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, false, true>), G, B, 0, st, a, q2, q1, \
@@ -1239,14 +1375,78 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
         if (ev) HIPCHK(hipEventRecord(ev[3], st));                                                          \
     } while (0)
 #define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
-        if (arm == SIMC_ARM_HMS) {
-            if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
-        } else {
-            if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
-        }
+    if (arm == SIMC_ARM_HMS) {
+        if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
+    } else {
+        if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
+    }
 #undef LAUNCH1
 #undef LAUNCH3
-        HIPCHK(hipGetLastError());
+    HIPCHK(hipGetLastError());
+    return SIMC_OK;
+}
+
+int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+            int64_t n, const simc_rng* rng, void* stream, void* const* stage_events)
+{
+    if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run: NULL argument");
+    const int ai = arm_index(arm);
+    int rc = check_settings(arm, s);
+    if (rc) return rc;
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (!outputs_complete(in, out)) return fail(SIMC_E_ARG, "missing event input or output array");
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
+    KArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.set0 = make_setdev(arm, s, rng->seed, rng->event0, 0, 0, 0);
+    a.set = nullptr;
+    a.n_set = 1;
+    a.flags = a.set0.flags;
+    a.seed = rng->seed;
+    a.tab = ctx->d_tab[ai];
+    a.coef = ctx->d_coef[ai];
+    a.ops = ctx->d_ops[ai];
+    a.hist = ctx->d_hist[ai];
+    const bool tape = rng->kind == SIMC_RNG_TAPE;
+    const bool rare = needs_rare(s);
+    // Launches are cut so that every byte offset of a launch fits 32 bits
+    // and each survivor queue stays at most 2^24 entries (1.7 GB).
+    int64_t chunk = (int64_t)1 << 24;
+    if (tape) {
+        const int64_t sb = rng->stride * 8;
+        if (sb > 0x7fffffffLL) return fail(SIMC_E_ARG, "tape stride too large");
+        const int64_t c = 0xffffffffLL / sb;
+        if (c < chunk) chunk = c;
+        a.stride_bytes = (uint32_t)sb;
+    }
+    Queue q1, q2;
+    rc = ensure_queues(ctx, ai, (uint32_t)(n < chunk ? n : chunk), q1, q2);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    for (int64_t b = 0; b < n; b += chunk) {
+        const int64_t cnt = n - b < chunk ? n - b : chunk;
+        a.count = (uint32_t)cnt;
+        a.pass_base = (uint64_t)b;
+        a.ix_base = (uint64_t)b;
+        a.n_arr = (uint32_t)cnt;
+        a.dpp = in->dpp + b; a.x = in->x + b; a.y = in->y + b; a.dxdz = in->dxdz + b; a.dydz = in->dydz + b;
+        a.fry = in->fry ? in->fry + b : nullptr;
+        a.o_dpp = out->dpp + b; a.o_y = out->y + b; a.o_dxdz = out->dxdz + b; a.o_dydz = out->dydz + b;
+        a.x_fp = out->x_fp + b; a.dx_fp = out->dx_fp + b; a.y_fp = out->y_fp + b; a.dy_fp = out->dy_fp + b;
+        a.pathlen = out->pathlen + b;
+        a.resmult = out->resmult ? out->resmult + b : nullptr;
+        a.m2o = out->m2 ? out->m2 + b : nullptr;
+        a.ok = out->ok + b;
+        a.stop = out->stop + b;
+        a.used = out->used ? out->used + b : nullptr;
+        a.tape = tape ? rng->tape + b * rng->stride : nullptr;
+        // stage_events: boundaries of the first chunk's three stages
+        hipEvent_t* ev = (stage_events && b == 0) ? (hipEvent_t*)stage_events : nullptr;
+        rc = launch_stages(ctx, arm, rare, tape, a, q1, q2, st, ev);
+        if (rc) return rc;
     }
     return SIMC_OK;
 }
@@ -1428,45 +1628,7 @@ int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
     unsigned long long h[kHistSlots];
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(h, ctx->d_hist[ai], sizeof h, hipMemcpyDeviceToHost));
-    for (int i = 0; i < ncounters; ++i) c[i] = 0;
-    int64_t total = 0;
-    for (int i = 0; i < kHistBins; ++i) total += (int64_t)h[i];
-    auto H = [&](int code) { return (int64_t)h[code]; };
-    if (arm == SIMC_ARM_HMS) {
-        // order of common /spectrometer1/, hms/struct_hms.inc:34-42
-        c[0] = total;
-        c[1] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
-        c[2] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
-        c[3] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
-        for (int k = 0; k < 9; ++k) c[4 + k] = H(ST_H_Q1_IN + k);
-        c[13] = H(ST_H_D1_IN);
-        c[14] = H(ST_H_D1_OUT) + H(ST_H_PIPE0) + H(ST_H_PIPE1) + H(ST_H_PIPE2) + H(ST_H_PIPE3);
-        c[16] = H(ST_H_DC1);
-        c[17] = H(ST_H_DC2);
-        c[18] = H(ST_H_SCIN);
-        c[19] = 0;  // hSTOP_cal: the calorimeter cut is commented out, hms/mc_hms_hut.f:584-588
-        c[20] = H(ST_OK);
-        c[15] = c[16] + c[17] + c[18] + c[20];
-        c[21] = H(ST_COLL);
-    } else {
-        // order of common /spectrometer_shms/, shms/struct_shms.inc:26-44
-        c[0] = total;
-        c[1] = H(ST_OK);
-        c[5] = H(ST_SLIT_HOR) + (int64_t)h[kHistBins + 0];
-        c[6] = H(ST_SLIT_VERT) + (int64_t)h[kHistBins + 1];
-        c[7] = H(ST_SLIT_OCT) + (int64_t)h[kHistBins + 2];
-        for (int k = 0; k < 4; ++k) c[8 + k] = H(ST_S_HB_IN + k);
-        for (int k = 0; k < 15; ++k) c[12 + k] = H(ST_S_Q1_IN + k);
-        for (int k = 0; k < 12; ++k) c[27 + k] = H(ST_S_D1_IN + k);
-        c[42] = H(ST_S_DC1);
-        c[43] = H(ST_S_DC2);
-        c[46] = H(ST_S_S1X) + H(ST_S_S1Y);
-        c[47] = H(ST_S_S2Y);
-        c[48] = H(ST_S_S2X);
-        c[49] = H(ST_S_CAL) + H(ST_S_CALFID);
-        c[41] = c[42] + c[43] + c[46] + c[47] + c[48] + c[49] + c[1];
-        c[51] = H(ST_COLL);
-    }
+    hist_to_counters(arm, h, c, ncounters);
     return SIMC_OK;
 }
 
@@ -1521,6 +1683,197 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
 int simc_stop_flops(simc_ctx* ctx, int arm, int64_t* flops, int nflops)
 {
     return simc_stop_flops_stage(ctx, arm, 0, flops, nflops);
+}
+
+int simc_batch_create(simc_ctx* ctx, const simc_setting* settings, int n_settings, simc_batch** out)
+{
+    if (!ctx || !settings || !out || n_settings <= 0)
+        return fail(SIMC_E_ARG, "simc_batch_create: NULL argument or no settings");
+    *out = nullptr;
+    std::vector<std::pair<int64_t, int64_t>> ranges;
+    for (int i = 0; i < n_settings; ++i) {
+        const simc_setting& q = settings[i];
+        int rc = check_settings(q.arm, &q.s);
+        if (rc) return fail(rc, "setting " + std::to_string(i) + ": " + g_err);
+        if (q.offset < 0 || q.count < 0 || q.offset + q.count > ((int64_t)1 << 29))
+            return fail(SIMC_E_ARG, "setting " + std::to_string(i) + ": event range outside [0, 2^29)");
+        if (q.count) ranges.push_back({q.offset, q.offset + q.count});
+    }
+    std::sort(ranges.begin(), ranges.end());
+    for (size_t k = 1; k < ranges.size(); ++k)
+        if (ranges[k].first < ranges[k - 1].second)
+            return fail(SIMC_E_ARG, "simc_batch_create: settings' event ranges overlap");
+    simc_batch* b = new (std::nothrow) simc_batch();
+    if (!b) return fail(SIMC_E_NOMEM, "out of host memory");
+    b->ctx = ctx;
+    b->n_settings = n_settings;
+    for (int i = 0; i < n_settings; ++i) b->arm_of.push_back(settings[i].arm);
+    std::vector<SetDev> table;
+    uint64_t max_total = 0;
+    // passes: one per (arm, kernel variant, flag set, seed), settings in input order
+    std::vector<bool> placed(n_settings, false);
+    for (int i0 = 0; i0 < n_settings; ++i0) {
+        const simc_setting& q0 = settings[i0];
+        if (placed[i0] || q0.count == 0) continue;
+        const int32_t f0 = make_setdev(q0.arm, &q0.s, 0, 0, 0, 0, 0).flags;
+        const bool r0 = needs_rare(&q0.s);
+        simc_batch::Pass pass{q0.arm, r0, f0, q0.seed, 0, (uint32_t)table.size(), 0};
+        for (int i = i0; i < n_settings; ++i) {
+            const simc_setting& q = settings[i];
+            if (placed[i] || q.count == 0 || q.arm != q0.arm || needs_rare(&q.s) != r0 ||
+                make_setdev(q.arm, &q.s, 0, 0, 0, 0, 0).flags != f0 || q.seed != q0.seed)
+                continue;
+            placed[i] = true;
+            table.push_back(make_setdev(q.arm, &q.s, q.seed, q.event0, pass.total, (uint64_t)q.offset, (uint32_t)i));
+            pass.total += (uint64_t)q.count;
+            ++pass.n;
+            b->n_arr = std::max(b->n_arr, (uint32_t)(q.offset + q.count));
+        }
+        b->passes.push_back(pass);
+        max_total = std::max(max_total, pass.total);
+    }
+    auto cleanup = [&](int rc) {
+        std::string keep = g_err;
+        simc_batch_destroy(b);
+        g_err = keep;
+        return rc;
+    };
+    if (!table.empty()) {
+        if (hipMalloc(&b->d_set, table.size() * sizeof(SetDev)) != hipSuccess ||
+            hipMemcpy(b->d_set, table.data(), table.size() * sizeof(SetDev), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(SIMC_E_HIP, "simc_batch_create: settings table"));
+    }
+    if (hipMalloc(&b->d_hist, (size_t)n_settings * kHistSlots * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(b->d_hist, 0, (size_t)n_settings * kHistSlots * sizeof(unsigned long long)) != hipSuccess)
+        return cleanup(fail(SIMC_E_HIP, "simc_batch_create: histograms"));
+    // queues for the default chunk, so that runs allocate nothing (graph capture)
+    b->chunk = std::min<int64_t>(b->chunk, std::max<int64_t>((int64_t)max_total, 1));
+    int rc = simc_batch_set_chunk(b, b->chunk);
+    if (rc) return cleanup(rc);
+    *out = b;
+    return SIMC_OK;
+}
+
+int simc_batch_set_chunk(simc_batch* b, int64_t chunk)
+{
+    if (!b) return fail(SIMC_E_ARG, "simc_batch_set_chunk: NULL batch");
+    if (chunk < kBlock || chunk > ((int64_t)1 << 24))
+        return fail(SIMC_E_ARG, "chunk must be in [256, 2^24] events");
+    Queue q1, q2;
+    for (const auto& pass : b->passes) {
+        const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)chunk, pass.total);
+        int rc = ensure_queues(b->ctx, arm_index(pass.arm), cap, q1, q2);
+        if (rc) return rc;
+    }
+    b->chunk = chunk;
+    return SIMC_OK;
+}
+
+int simc_batch_run(simc_batch* b, const simc_events_in* in, simc_events_out* out, void* stream)
+{
+    if (!b || !in || !out) return fail(SIMC_E_ARG, "simc_batch_run: NULL argument");
+    if (!outputs_complete(in, out)) return fail(SIMC_E_ARG, "missing event input or output array");
+    simc_ctx* ctx = b->ctx;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nh = (uint32_t)b->n_settings * kHistSlots;
+    hipLaunchKernelGGL(zero_u64, dim3((nh + 255) / 256), dim3(256), 0, st, b->d_hist, nh);
+    for (const auto& pass : b->passes) {
+        const int ai = arm_index(pass.arm);
+        if (ctx->q_cap[ai] < std::min<uint64_t>((uint64_t)b->chunk, pass.total))
+            return fail(SIMC_E_INTERNAL, "simc_batch_run: queues smaller than the chunk");
+        Queue q1, q2;
+        queue_view(ctx->d_queue[ai][0], ctx->q_cap[ai], ctx->d_qcount[ai], q1);
+        queue_view(ctx->d_queue[ai][1], ctx->q_cap[ai], ctx->d_qcount[ai] + 1, q2);
+        KArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.set = (const SetDev*)b->d_set + pass.first;
+        a.n_set = pass.n;
+        a.flags = pass.flags;
+        a.seed = pass.seed;
+        a.ix_base = 0;
+        a.n_arr = b->n_arr;
+        a.tab = ctx->d_tab[ai];
+        a.coef = ctx->d_coef[ai];
+        a.ops = ctx->d_ops[ai];
+        a.hist = b->d_hist;
+        a.dpp = in->dpp; a.x = in->x; a.y = in->y; a.dxdz = in->dxdz; a.dydz = in->dydz; a.fry = in->fry;
+        a.o_dpp = out->dpp; a.o_y = out->y; a.o_dxdz = out->dxdz; a.o_dydz = out->dydz;
+        a.x_fp = out->x_fp; a.dx_fp = out->dx_fp; a.y_fp = out->y_fp; a.dy_fp = out->dy_fp;
+        a.pathlen = out->pathlen; a.resmult = out->resmult; a.m2o = out->m2;
+        a.ok = out->ok; a.stop = out->stop; a.used = out->used;
+        for (uint64_t c0 = 0; c0 < pass.total; c0 += (uint64_t)b->chunk) {
+            a.pass_base = c0;
+            a.count = (uint32_t)std::min<uint64_t>((uint64_t)b->chunk, pass.total - c0);
+            int rc = launch_stages(ctx, pass.arm, pass.rare, false, a, q1, q2, st, nullptr);
+            if (rc) return rc;
+        }
+    }
+    return SIMC_OK;
+}
+
+int simc_batch_counters(simc_batch* b, int setting, int64_t* c, int ncounters)
+{
+    if (!b || !c) return fail(SIMC_E_ARG, "simc_batch_counters: NULL argument");
+    if (setting < 0 || setting >= b->n_settings) return fail(SIMC_E_ARG, "setting index out of range");
+    const int arm = b->arm_of[setting];
+    const int need = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    if (ncounters < need) return fail(SIMC_E_ARG, "counter array too short");
+    unsigned long long h[kHistSlots];
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(h, b->d_hist + (size_t)setting * kHistSlots, sizeof h, hipMemcpyDeviceToHost));
+    hist_to_counters(arm, h, c, ncounters);
+    return SIMC_OK;
+}
+
+int simc_batch_tune(simc_batch* b, const simc_events_in* in, simc_events_out* out, void* stream,
+                    int64_t* best_chunk, double* best_ms)
+{
+    if (!b || !in || !out) return fail(SIMC_E_ARG, "simc_batch_tune: NULL argument");
+    uint64_t max_total = 0;
+    for (const auto& pass : b->passes) max_total = std::max(max_total, pass.total);
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int64_t best = b->chunk;
+    float best_t = 1e30f;
+    int rc = SIMC_OK;
+    for (int lg = 18; lg <= 24 && rc == SIMC_OK; ++lg) {
+        const int64_t chunk = (int64_t)1 << lg;
+        if (lg > 18 && (uint64_t)(chunk >> 1) >= max_total) break;  // larger chunks launch the same
+        rc = simc_batch_set_chunk(b, chunk);
+        if (rc) break;
+        rc = simc_batch_run(b, in, out, stream);  // warm-up
+        float tmin = 1e30f;
+        for (int r = 0; r < 3 && rc == SIMC_OK; ++r) {
+            if (hipEventRecord(e0, st) != hipSuccess) rc = fail(SIMC_E_HIP, "hipEventRecord");
+            if (!rc) rc = simc_batch_run(b, in, out, stream);
+            if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                rc = fail(SIMC_E_HIP, "hipEventSynchronize");
+            float ms = 0;
+            if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) tmin = std::min(tmin, ms);
+        }
+        if (!rc && tmin < best_t) {
+            best_t = tmin;
+            best = chunk;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = simc_batch_set_chunk(b, best);
+    if (rc) return rc;
+    if (best_chunk) *best_chunk = best;
+    if (best_ms) *best_ms = best_t;
+    return SIMC_OK;
+}
+
+void simc_batch_destroy(simc_batch* b)
+{
+    if (!b) return;
+    if (b->d_set) (void)hipFree(b->d_set);
+    if (b->d_hist) (void)hipFree(b->d_hist);
+    delete b;
 }
 
 int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)

@@ -44,6 +44,13 @@ module simc_gpu
     integer(c_int64_t) :: stride = 0
   end type
 
+  ! one kinematic setting of a batched run (simc_batch_create)
+  type, bind(C) :: simc_setting
+    integer(c_int32_t) :: arm = SIMC_ARM_HMS, reserved = 0
+    type(simc_settings) :: s
+    integer(c_int64_t) :: seed = 0, event0 = 0, offset = 0, count = 0
+  end type
+
   interface
     integer(c_int) function simc_ctx_create(map_dir, ctx) bind(C, name="simc_ctx_create")
       import :: c_int, c_char, c_ptr
@@ -165,6 +172,49 @@ module simc_gpu
       integer(c_int), value :: arm, stage, nflops
       integer(c_int64_t), intent(out) :: flops(*)
     end function
+
+    integer(c_int) function simc_batch_create(ctx, settings, n_settings, b) bind(C, name="simc_batch_create")
+      import :: c_int, c_ptr, simc_setting
+      type(c_ptr), value :: ctx
+      type(simc_setting), intent(in) :: settings(*)
+      integer(c_int), value :: n_settings
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function simc_batch_run(b, evin, evout, stream) bind(C, name="simc_batch_run")
+      import :: c_int, c_ptr, simc_events_in, simc_events_out
+      type(c_ptr), value :: b, stream
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+    end function
+
+    integer(c_int) function simc_batch_counters(b, setting, c, nc) bind(C, name="simc_batch_counters")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: setting, nc
+      integer(c_int64_t), intent(out) :: c(*)
+    end function
+
+    integer(c_int) function simc_batch_set_chunk(b, chunk) bind(C, name="simc_batch_set_chunk")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int64_t), value :: chunk
+    end function
+
+    integer(c_int) function simc_batch_tune(b, evin, evout, stream, best_chunk, best_ms) &
+        bind(C, name="simc_batch_tune")
+      import :: c_int, c_int64_t, c_double, c_ptr, simc_events_in, simc_events_out
+      type(c_ptr), value :: b, stream
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), intent(out) :: best_chunk
+      real(c_double), intent(out) :: best_ms
+    end function
+
+    subroutine simc_batch_destroy(b) bind(C, name="simc_batch_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end subroutine
 
     integer(c_int) function simc_counters_get(ctx, arm, c, nc) bind(C, name="simc_counters_get")
       import :: c_int, c_int64_t, c_ptr

@@ -77,6 +77,67 @@ def generate(w: Workload, n: int | None = None, seed: int | None = None) -> dict
     return {"dpp": dpp, "x": x, "y": y, "z": z, "dxdz": dxdz, "dydz": dydz, "fry": x.copy()}
 
 
+def generate_device(w: Workload, n: int, seed: int, device, out: dict | None = None, at: int = 0) -> dict:
+    """The same distributions and transform as generate(), drawn with torch on
+    `device` (for large batches); writes into out[k][at:at+n] when given."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device=device)
+
+    def uni(lo, hi):
+        return torch.rand(n, generator=g, **f64) * (hi - lo) + lo
+
+    xt = torch.randn(n, generator=g, **f64) * w.sigma_xy
+    yt = torch.randn(n, generator=g, **f64) * w.sigma_xy
+    zt = uni(*w.z)
+    dpp = uni(*w.dpp)
+    dxdz = uni(*w.dxdz)
+    dydz = uni(*w.dydz)
+    th = math.radians(w.theta_deg)
+    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    cth, sth, sphi = math.cos(th), math.sin(th), math.sin(phi)
+    x = -yt
+    y = -xt * cth - zt * sth * sphi
+    z = zt * cth + xt * sth * sphi
+    x = x - z * dxdz
+    y = y - z * dydz
+    ev = {"dpp": dpp, "x": x, "y": y, "z": torch.zeros_like(x), "dxdz": dxdz, "dydz": dydz, "fry": x.clone()}
+    if out is None:
+        return ev
+    for k, v in ev.items():
+        out[k][at:at + n] = v
+    return out
+
+
+def mixed_settings(n_settings: int = 64, seed: int = 2026, lo: float = 1e4, hi: float = 1e7,
+                   total: float | None = 1.5e8):
+    """The batched configuration: n_settings kinematic settings alternating
+    HMS (protons) and SHMS (electrons), central momenta U(1, 9) GeV/c, angles
+    U(10, 40) degrees, beam energy from {6.4, 8.5, 10.6} GeV (carried along;
+    the spectrometer model does not use it), event counts log-uniform in
+    [lo, hi], then scaled by one common factor so that they sum to `total`
+    (64 log-uniform draws in [1e4, 1e7] average 9.3e7 events).  Returns a
+    list of Workload objects with n = the setting's count."""
+    rng = np.random.default_rng(seed)
+    draws = []
+    for i in range(n_settings):
+        p = float(rng.uniform(1000.0, 9000.0))
+        th = float(rng.uniform(10.0, 40.0))
+        beam = float(rng.choice([6400.0, 8500.0, 10600.0]))
+        draws.append((p, th, beam, math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+    scale = 1.0 if total is None else total / sum(d[3] for d in draws)
+    out = []
+    for i, (p, th, beam, c) in enumerate(draws):
+        hms = i % 2 == 0
+        cnt = int(round(c * scale))
+        base = HMS_C0 if hms else SHMS_C1
+        out.append(Workload(f"setting{i:02d}_{'hms' if hms else 'shms'}", base.arm, cnt, p, th, base.mass,
+                            base.dpp, base.dxdz, base.dydz, wire_sigma=base.wire_sigma,
+                            seed=seed * 1000 + i, extra={"beam_MeV": beam}))
+    return out
+
+
 def uniform_tape(n: int, stride: int, seed: int) -> np.ndarray:
     """Per-event uniform tapes in (0,1): event i draws tape[i*stride + k]."""
     rng = np.random.default_rng(seed)

@@ -1,0 +1,143 @@
+"""The batched form: many kinematic settings in one run.  Every setting's
+events must equal a plain run of that setting alone (same Philox streams:
+bit-identical), its counters must equal the plain run's, the launch sequence
+must replay correctly from a captured HIP graph, and the autotuner must leave
+results unchanged."""
+import numpy as np
+import pytest
+
+from simc_gfortran_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+OUT = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "ok", "stop")
+MPI = 139.57018
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from simc_gfortran_amd import api
+    c = api.Spectrometers()
+    yield c
+    c.close()
+
+
+def mixed(counts, gap=17):
+    """Settings alternating arms with assorted momenta, particles and flags, laid
+    out with gaps between their event ranges; returns (settings, events)."""
+    specs = [
+        dict(arm=1, p_spec=3000.0, m2=W.MP ** 2),
+        dict(arm=5, p_spec=6000.0, m2=W.ME ** 2, wire_sigma=0.030, fit="fp64"),
+        dict(arm=1, p_spec=1500.0, m2=W.MP ** 2, ms=False),
+        dict(arm=5, p_spec=2500.0, m2=MPI ** 2, coll=True),
+        dict(arm=1, p_spec=8000.0, m2=W.ME ** 2, fit="fp64"),
+        dict(arm=5, p_spec=4200.0, m2=W.MP ** 2),
+        dict(arm=1, p_spec=2000.0, m2=MPI ** 2, decay=True, ctau=780.45),
+    ]
+    settings, parts, off = [], [], 0
+    for i, cnt in enumerate(counts):
+        d = dict(specs[i % len(specs)])
+        wl = W.HMS_C0 if d["arm"] == 1 else W.SHMS_C1
+        d.update(offset=off, count=cnt, seed=11, event0=(i + 1) << 32)
+        settings.append(d)
+        ev = W.generate(wl, cnt, seed=100 + i)
+        parts.append((off, ev))
+        off += cnt + gap
+    n = off
+    events = {k: np.zeros(n) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
+    for o, ev in parts:
+        for k in events:
+            events[k][o:o + len(ev["dpp"])] = ev[k]
+    return settings, {k: torch.from_numpy(v).cuda() for k, v in events.items()}
+
+
+def plain(ctx, d, events):
+    sl = slice(d["offset"], d["offset"] + d["count"])
+    ev = {k: v[sl].contiguous() for k, v in events.items()}
+    kw = {k: d[k] for k in ("p_spec", "m2", "ms", "wcs", "decay", "ctau", "coll", "fit", "wire_sigma")
+          if k in d}
+    ctx.reset_counters(d["arm"])
+    out = ctx.run(d["arm"], ev, rng="philox", seed=d["seed"], event0=d["event0"], **kw)
+    torch.cuda.synchronize()
+    return out, ctx.counters(d["arm"])
+
+
+def test_batch_equals_plain_runs(ctx):
+    from simc_gfortran_amd import api
+    counts = [20000, 3000, 5, 0, 45000, 1, 12000, 700]
+    settings, events = mixed(counts)
+    b = api.Batch(ctx, settings)
+    out = b.run(events)
+    torch.cuda.synchronize()
+    for i, d in enumerate(settings):
+        if d["count"] == 0:
+            continue
+        ref, cnt = plain(ctx, d, events)
+        sl = slice(d["offset"], d["offset"] + d["count"])
+        for k in OUT:
+            np.testing.assert_array_equal(out[k][sl].cpu().numpy(), ref[k].cpu().numpy(), err_msg=f"{i} {k}")
+        got = b.counters(i)
+        for k, v in cnt.items():
+            if k != "stop_id":
+                assert got[k] == v, (i, k, got[k], v)
+    b.close()
+
+
+def test_batch_graph_replays(ctx):
+    from simc_gfortran_amd import api
+    settings, events = mixed([30000, 20000, 10000, 4000])
+    b = api.Batch(ctx, settings)
+    ref = b.run(events)
+    torch.cuda.synchronize()
+    ref = {k: v.clone() for k, v in ref.items()}
+    out = b.new_outputs(events)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        b.run(events, out=out)  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.run(events, out=out)
+    for rep in range(3):
+        for k in ("dpp", "y", "dxdz", "dydz"):
+            out[k].copy_(events[k])  # INOUT fields back to their inputs (eager work between replays)
+        for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen"):
+            out[k].zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in OUT:
+            assert torch.equal(out[k], ref[k]), (rep, k)
+    b.close()
+
+
+def test_batch_tuner_keeps_results(ctx):
+    from simc_gfortran_amd import api
+    settings, events = mixed([200000, 50000, 90000])
+    b = api.Batch(ctx, settings)
+    ref = b.run(events)
+    torch.cuda.synchronize()
+    ref = {k: v.clone() for k, v in ref.items()}
+    chunk, ms = b.tune(events, b.new_outputs(events))
+    assert chunk in [1 << k for k in range(18, 25)] and ms > 0
+    out = b.run(events)
+    torch.cuda.synchronize()
+    for k in OUT:
+        assert torch.equal(out[k], ref[k]), k
+    b.close()
+
+
+def test_batch_rejects_bad_layouts(ctx):
+    from simc_gfortran_amd import api
+    base = dict(arm=1, p_spec=3000.0, m2=W.MP ** 2)
+    with pytest.raises(api.SimcError):
+        api.Batch(ctx, [dict(base, offset=0, count=10), dict(base, offset=5, count=10)])
+    with pytest.raises(api.SimcError):
+        api.Batch(ctx, [dict(base, offset=(1 << 29) - 5, count=10)])
+    with pytest.raises(api.SimcError):
+        api.Batch(ctx, [dict(base, offset=0, count=10, decay=True)])
