@@ -217,6 +217,48 @@ def bench_batched(ctx, line, args):
                     "counts log-uniform in [1e4,1e7] scaled to 1.5e8 total"}
 
 
+def bench_adjoint(ctx, n=10_000_000, reps=3):
+    """The adjoint form on HMS, 1e7 events of the HMS workload: one pass
+    (loss and every gradient, draws replayed, mask from a plain run) against
+    the plain forward run of the same events."""
+    import torch
+    from simc_gfortran_amd import workloads as W
+    dev = torch.device("cuda:0")
+    wl = W.HMS_C0
+    ev = W.generate_device(wl, n, seed=wl.seed, device=dev)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True)
+    out = None
+    tf = []
+    for r in range(reps + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        out = ctx.run(wl.arm, ev, rng="philox", seed=77, fit="fp64", out=out, **kw)
+        b.record()
+        torch.cuda.synchronize()
+        if r:
+            tf.append(a.elapsed_time(b))
+    mask = out["ok"].clone()
+    coef = torch.from_numpy(ctx.recon_coefficients(wl.arm)).to(dev)
+    g = torch.empty_like(coef)
+    o9 = torch.empty(9, dtype=torch.float64, device=dev)
+    ta = []
+    for r in range(reps + 1):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ctx.adjoint(wl.arm, ev, mask, coef, [0.0] * 6, seed=77, grad_coef=g, out9=o9, **kw)
+        b.record()
+        torch.cuda.synchronize()
+        if r:
+            ta.append(a.elapsed_time(b))
+    fwd, adj = float(np.mean(tf)), float(np.mean(ta))
+    res = o9.cpu().numpy()
+    return {"workload": "hms_proton_3GeV_1e7", "events": n, "events_used": int(res[8]),
+            "forward_ms": fwd, "adjoint_ms": adj, "adjoint_over_forward": adj / fwd,
+            "adjoint_events_per_s": n / (adj * 1e-3), "loss": float(res[0]),
+            "gradients": int(coef.numel()) + 6,
+            "data": "synthetic HMS workload, Philox streams replayed, mask = ok of the forward run"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +386,7 @@ def main():
         del d_ev, out
         torch.cuda.empty_cache()
         line["batched"] = bench_batched(ctx, line, args)
+        line["adjoint"] = bench_adjoint(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb

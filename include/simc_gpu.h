@@ -187,6 +187,27 @@ int simc_batch_tune(simc_batch *b, const simc_events_in *in, simc_events_out *ou
                     int64_t *best_chunk, double *best_ms);
 void simc_batch_destroy(simc_batch *b);
 
+/* ---- Adjoint form: gradients for optics fits ---------------------------
+ *
+ * Loss and gradients over the events with mask[i] != 0 (typically `ok` of a
+ * plain run), their random draws replayed from the same stream (Philox: the
+ * same seed and event0; tape: the same tape) and the mask held fixed:
+ *   delta' = 100 ((1 + delta/100) / (1 + s_p) - 1), dy/dz' = dy/dz - dth,
+ *   transport (FP64 track fit), X = x_fp + ox, Y = y_fp + oy, X' = dx_fp +
+ *   ox', Y' = dy_fp + oy', reconstruction with recon_coef,
+ *   L = 1/N sum [(delta_r - delta)^2 + (x'_r - dx/dz)^2 + (y'_r - dy/dz)^2 + (y_r - y)^2]
+ * offsets = {ox, oy (cm), ox', oy' (rad), s_p, dth (rad)} (host array).
+ * recon_coef, grad_coef: device [n_terms][4] (map file order: x'_tar, y_tar,
+ * y'_tar, delta columns); out9: device, receives L, dL/d offsets (6), the
+ * events excluded (a wire hit exactly 0, which the track fit would skip)
+ * and N.  Asynchronous on `stream`.  No decay or collimator stepping. */
+int simc_adjoint_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
+                     const uint8_t *mask, int64_t n, const simc_rng *rng, const double *recon_coef,
+                     const double *offsets, double *grad_coef, double *out9, void *stream);
+/* The loaded reconstruction map: n_terms, and its coefficients [n_terms][4]
+ * when coef is not NULL (host array). */
+int simc_recon_coefficients(simc_ctx *ctx, int arm, double *coef, int32_t *n_terms);
+
 /* Reconstruction only (mc_hms_recon / mc_shms_recon) of focal-plane tracks:
  * x_fp, y_fp in cm, slopes in rad, fry in cm.  Outputs delta (%), x'_tar,
  * y'_tar (rad), y_tar (cm). */

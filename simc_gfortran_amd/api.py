@@ -50,7 +50,7 @@ EXPORTS = [
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
-    "simc_batch_tune", "simc_batch_destroy",
+    "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
     "simc_last_error",
     "simc_version",
 ]
@@ -121,6 +121,9 @@ def library(path: str = LIB_PATH):
     L.simc_batch_set_chunk.argtypes = [_vp, C.c_int64]
     L.simc_batch_tune.argtypes = [_vp, C.POINTER(EventsIn), C.POINTER(EventsOut), _vp,
                                   C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.simc_adjoint_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn), _vp, C.c_int64,
+                                   C.POINTER(Rng), _vp, C.POINTER(C.c_double), _vp, _vp, _vp]
+    L.simc_recon_coefficients.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.simc_batch_destroy.argtypes = [_vp]
     L.simc_batch_destroy.restype = None
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
@@ -254,6 +257,46 @@ class Spectrometers:
         _check(library().simc_arm_run_host(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
                                            C.byref(R)))
         return out
+
+    def recon_coefficients(self, arm):
+        """The loaded reconstruction map's coefficients, numpy [n_terms, 4]."""
+        n = C.c_int32()
+        _check(library().simc_recon_coefficients(self._h, arm, None, C.byref(n)))
+        c = np.zeros((n.value, 4))
+        _check(library().simc_recon_coefficients(self._h, arm, c.ctypes.data_as(C.POINTER(C.c_double)), None))
+        return c
+
+    def adjoint(self, arm, events, mask, coef, offsets, *, p_spec, m2, ms=True, wcs=True, wire_sigma=0.0,
+                rng="philox", seed=0, event0=0, tape=None, stride=0, grad_coef=None, out9=None, stream=None):
+        """Adjoint form (simc_adjoint_run): loss and gradients over the events
+        with mask != 0, draws replayed, mask fixed.  coef: float64 device
+        tensor [n_terms, 4]; offsets: 6 floats (ox, oy, ox', oy', s_p, dth).
+        Returns device tensors (grad_coef [n_terms, 4], out9) where out9 =
+        [loss, dL/d offsets (6), excluded, used]."""
+        n = events["dpp"].numel()
+        ev = {k: _dev_f64(events[k], n, k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
+        fry = events.get("fry")
+        if fry is not None:
+            ev["fry"] = _dev_f64(fry, n, "fry")
+        if not (mask.is_cuda and mask.dtype == torch.uint8 and mask.numel() == n and mask.is_contiguous()):
+            raise TypeError("mask: expected n contiguous uint8 on the device")
+        if not (coef.is_cuda and coef.dtype == torch.float64 and coef.is_contiguous()):
+            raise TypeError("coef: expected a contiguous float64 device tensor")
+        if grad_coef is None:
+            grad_coef = torch.empty_like(coef)
+        if out9 is None:
+            out9 = torch.empty(9, dtype=torch.float64, device=coef.device)
+        S = Settings(p_spec, m2, 0.0, int(ms), int(wcs), 0, 0, FIT_FP64, 0, wire_sigma)
+        ein = EventsIn(*[_ptr(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
+        if rng == "tape":
+            R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        off = (C.c_double * 6)(*[float(v) for v in offsets])
+        st = stream if stream is not None else torch.cuda.current_stream(coef.device).cuda_stream
+        _check(library().simc_adjoint_run(self._h, arm, C.byref(S), C.byref(ein), _ptr(mask), n, C.byref(R),
+                                          _ptr(coef), off, _ptr(grad_coef), _ptr(out9), C.c_void_p(st)))
+        return grad_coef, out9
 
     def recon(self, arm, x_fp, dx_fp, y_fp, dy_fp, fry=None, stream=None):
         """mc_hms_recon / mc_shms_recon over focal-plane tracks."""

@@ -14,6 +14,10 @@ shms/shms_forward.dat, shms/shms_recon.dat) and writes cosy_gen.inc:
     - coefficients read from a packed table with compile-time offsets, so the
       loads are uniform scalar loads through the constant address space and
       coefficient VALUES stay a run-time input;
+* for the adjoint form: each forward class with two tangent directions
+  carried along (same operations for the values), and the recon polynomial
+  with all coefficients as run-time values plus its gradient with respect to
+  the focal-plane inputs;
 * the structure tables (exponents, non-zero masks, pack order) the host uses
   to check a map loaded at run time against the compiled structure and to
   pack its coefficients.
@@ -164,6 +168,150 @@ def emit_poly(fname, terms, nout, inputs, lines):
     return pack, counter[0]
 
 
+def _trie(terms):
+    byexp = {}
+    for ti, (c, e) in enumerate(terms):
+        byexp.setdefault(tuple(e), []).append(ti)
+    root = (0,) * NVAR
+    needed = {root}
+    for e in byexp:
+        while e not in needed:
+            needed.add(e)
+            e = _parent(e)
+    return byexp, root, needed
+
+
+def emit_poly_tangent(fname, terms, pack, inputs, lines):
+    """The forward polynomial with two tangent directions (A, B) carried along:
+    o_k and d o_k = J d r for dr = rA, rB.  The value part performs the same
+    operations in the same order as emit_poly's function, and the packed
+    coefficient table (pack order) is the same one."""
+    byexp, root, needed = _trie(terms)
+    pos = {tk: i for i, tk in enumerate(pack)}
+    body = []
+    counter = [0]
+
+    def fmas(node, m):
+        for ti in byexp.get(node, []):
+            c = terms[ti][0]
+            for k in range(5):
+                if c[k] != 0.0:
+                    idx = pos[(ti, k)]
+                    if m is None:
+                        body.append(f"  o{k} += c[{idx}];")
+                    else:
+                        body.append(f"  o{k} = __builtin_fma({m[0]}, c[{idx}], o{k});")
+                        body.append(f"  a{k} = __builtin_fma({m[1]}, c[{idx}], a{k});")
+                        body.append(f"  b{k} = __builtin_fma({m[2]}, c[{idx}], b{k});")
+
+    def walk(node, m):
+        fmas(node, m)
+        lv = _last_var(node) if node != root else 0
+        for v in range(lv, NVAR):
+            ch = list(node)
+            ch[v] += 1
+            ch = tuple(ch)
+            if ch not in needed:
+                continue
+            x, xa, xb = inputs[v], "a" + inputs[v], "b" + inputs[v]
+            if m is None:
+                cm = (x, xa, xb)
+            else:
+                counter[0] += 1
+                n = counter[0]
+                cm = (f"m{n}", f"ma{n}", f"mb{n}")
+                body.append(f"  const double m{n} = {m[0]} * {x};")
+                body.append(f"  const double ma{n} = __builtin_fma({m[1]}, {x}, {m[0]} * {xa});")
+                body.append(f"  const double mb{n} = __builtin_fma({m[2]}, {x}, {m[0]} * {xb});")
+            walk(ch, cm)
+
+    walk(root, None)
+    args = ", ".join(f"double {x}" for x in inputs)
+    argsa = ", ".join(f"double a{x}" for x in inputs)
+    argsb = ", ".join(f"double b{x}" for x in inputs)
+    outs = ", ".join(f"double& {p}{k}" for p in "oab" for k in range(5))
+    lines.append(f"__device__ __forceinline__ void {fname}(const double __attribute__((address_space(4)))* "
+                 f"__restrict__ c, {args}, {argsa}, {argsb}, {outs})")
+    lines.append("{")
+    for p in "oab":
+        for k in range(5):
+            lines.append(f"  {p}{k} = 0.0;")
+    lines.extend(body)
+    lines.append("}")
+    lines.append("")
+
+
+def emit_recon_grad(fname, terms, inputs, lines):
+    """The reconstruction polynomial with EVERY coefficient a run-time value,
+    c[4 t + k] for term t of the map file (zero coefficients included: they
+    are parameters of an optics fit), and the derivatives of the four sums
+    with respect to the first four inputs (the fifth, x_tar, is data).  Terms
+    are visited in the same trie order as emit_poly's function."""
+    byexp, root, needed = _trie(terms)
+    body = []
+    counter = [0]
+    ND = 4
+
+    def fmas(node, m):
+        for ti in byexp.get(node, []):
+            for k in range(4):
+                ci = f"c[{4 * ti + k}]"
+                if m is None:
+                    body.append(f"  s{k} += {ci};")
+                else:
+                    body.append(f"  s{k} = __builtin_fma({m[0]}, {ci}, s{k});")
+                    for i in range(ND):
+                        if m[1 + i] is not None:
+                            body.append(f"  g{k}{i} = __builtin_fma({m[1 + i]}, {ci}, g{k}{i});")
+
+    def walk(node, m):
+        fmas(node, m)
+        lv = _last_var(node) if node != root else 0
+        for v in range(lv, NVAR):
+            ch = list(node)
+            ch[v] += 1
+            ch = tuple(ch)
+            if ch not in needed:
+                continue
+            x = inputs[v]
+            if m is None:
+                cm = (x,) + tuple("1.0" if i == v else None for i in range(ND))
+            else:
+                counter[0] += 1
+                n = counter[0]
+                names = [f"m{n}"]
+                body.append(f"  const double m{n} = {m[0]} * {x};")
+                for i in range(ND):
+                    d = m[1 + i]
+                    terms_ = []
+                    if d is not None:
+                        terms_.append(f"{d} * {x}")
+                    if i == v:
+                        terms_.append(m[0])
+                    if not terms_:
+                        names.append(None)
+                        continue
+                    body.append(f"  const double m{n}d{i} = {' + '.join(terms_)};")
+                    names.append(f"m{n}d{i}")
+                cm = tuple(names)
+            walk(ch, cm)
+
+    walk(root, None)
+    args = ", ".join(f"double {x}" for x in inputs)
+    outs = ", ".join([f"double& s{k}" for k in range(4)] +
+                     [f"double& g{k}{i}" for k in range(4) for i in range(ND)])
+    lines.append(f"__device__ __forceinline__ void {fname}(const double __attribute__((address_space(4)))* "
+                 f"__restrict__ c, {args}, {outs})")
+    lines.append("{")
+    for k in range(4):
+        lines.append(f"  s{k} = 0.0;")
+        for i in range(ND):
+            lines.append(f"  g{k}{i} = 0.0;")
+    lines.extend(body)
+    lines.append("}")
+    lines.append("")
+
+
 def generate(map_dir, out_path):
     specs = [
         ("hms", os.path.join(map_dir, "hms", "forward_cosy.dat"),
@@ -190,9 +338,12 @@ def generate(map_dir, out_path):
             terms = classes[cls - 1]
             fname = f"cosy_{arm}_fwd_c{cls}"
             pack, nmul = emit_poly(fname, terms, 5, ["r0", "r1", "r2", "r3", "r4"], dev)
+            emit_poly_tangent(fname + "_tan", terms, pack, ["r0", "r1", "r2", "r3", "r4"], dev)
             infos.append((cls, terms, 5, offset, pack, nmul))
             offset += len(pack)
         fname = f"cosy_{arm}_recon"
+        emit_recon_grad(fname + "_grad", recon, ["h0", "h1", "h2", "h3", "h4"], dev)
+        dev.append(f"#define COSY_{arm.upper()}_RECON_TERMS {len(recon)}")
         pack, nmul = emit_poly(fname, recon, 4, ["h0", "h1", "h2", "h3", "h4"], dev)
         infos.append((0, recon, 4, offset, pack, nmul))
         offset += len(pack)

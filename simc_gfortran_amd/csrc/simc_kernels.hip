@@ -55,6 +55,9 @@ constexpr int kHistSlots = kHistBins + 3;  // + per-step collimator slit hits (h
 struct ArmTables {
     CollGeom coll;     // collimator stepping geometry
     double zdc[12];    // wire-plane z positions, planes 1..12
+    // straight-line fit as a linear map of the hits (all planes of the
+    // plane's coordinate present): slope = sum fw_a h, intercept = sum fw_b h
+    double fw_a[12], fw_b[12];
 };
 
 // One kinematic setting: the by-value arguments of mc_hms / mc_shms and the
@@ -141,6 +144,32 @@ __device__ __forceinline__ void poly_recon(cdptr c, double h0, double h1, double
 {
     if (ARM == SIMC_ARM_HMS) cosy_hms_recon(c + COSY_HMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
     else cosy_shms_recon(c + COSY_SHMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
+}
+
+// COSY forward class `cls` with two tangent directions (adjoint form).
+template <int ARM>
+__device__ __forceinline__ void poly_fwd_tan(int cls, cdptr c, const double (&r)[5], const double (&ra)[5],
+                                             const double (&rb)[5], double (&o)[5], double (&oa)[5],
+                                             double (&ob)[5])
+{
+#define TAN_ARGS r[0], r[1], r[2], r[3], r[4], ra[0], ra[1], ra[2], ra[3], ra[4], rb[0], rb[1], rb[2], rb[3], rb[4], \
+                 o[0], o[1], o[2], o[3], o[4], oa[0], oa[1], oa[2], oa[3], oa[4], ob[0], ob[1], ob[2], ob[3], ob[4]
+    if (ARM == SIMC_ARM_HMS) {
+#define HT(N) case N: cosy_hms_fwd_c##N##_tan(c + COSY_HMS_C##N##_OFF, TAN_ARGS); break;
+        switch (cls) {
+            HT(2) HT(3) HT(5) HT(6) HT(8) HT(9)
+            default: cosy_hms_fwd_c11_tan(c + COSY_HMS_C11_OFF, TAN_ARGS); break;
+        }
+#undef HT
+    } else {
+#define ST(N) case N: cosy_shms_fwd_c##N##_tan(c + COSY_SHMS_C##N##_OFF, TAN_ARGS); break;
+        switch (cls) {
+            ST(3) ST(7) ST(8) ST(12) ST(13) ST(17) ST(18) ST(23) ST(24) ST(25) ST(26) ST(27) ST(28) ST(29)
+            default: cosy_shms_fwd_c30_tan(c + COSY_SHMS_C30_OFF, TAN_ARGS); break;
+        }
+#undef ST
+    }
+#undef TAN_ARGS
 }
 
 // Reconstruction (hms/mc_hms_recon.f:113-137, shms/mc_shms_recon.f:131-160)
@@ -662,6 +691,329 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
         if (hist[k]) atomicAdd(&a.hist[(size_t)hist_index * kHistSlots + k], (unsigned long long)hist[k]);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint form: loss and gradients of an optics fit over a fixed set of
+// events, with the events' random draws replayed (their Philox streams or
+// tape slots) and the survival mask held fixed.
+//
+//   model:  delta' = 100 ((1 + delta/100) / (1 + s_p) - 1)   (momentum scale s_p)
+//           dy/dz' = dy/dz - dth                                (central-angle offset)
+//           transport through magnets and hut, FP64 track fit
+//           X = x_fp + ox, Y = y_fp + oy, X' = dx_fp + ox', Y' = dy_fp + oy'
+//           reconstruction with coefficients C (all 4 x n_terms)
+//   loss:   L = 1/N sum_events [ (delta_r - delta)^2 + (x'_r - x')^2 +
+//                                (y'_r - y')^2 + (y_r - y)^2 ]
+//   grads:  dL/dC, dL/d(ox, oy, ox', oy', s_p, dth)
+//
+// One event per lane runs the step program with the value and two tangent
+// directions (d/ds_p, d/dth) of the track; apertures are not tested (mask
+// fixed); the hut's scattering kicks do not depend on the track; the fit is
+// a fixed linear map of the hits.  The reconstruction gives its sums and
+// their gradient in the focal-plane inputs; dL/dC is accumulated with each
+// lane owning reconstruction terms and looping over the events of its
+// workgroup's tile.  Sums are normalised by N on the device at the end.
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+    SetDev set0;
+    int32_t flags;
+    uint64_t seed;
+    uint32_t n;                    // events in the arrays
+    const ArmTables* tab;
+    const double* coef;
+    const Op* ops;
+    const double *dpp, *x, *y, *dxdz, *dydz, *fry;
+    const uint32_t* sel;           // selected events, *n_sel of them
+    const uint32_t* n_sel;
+    const double* tape;
+    uint32_t stride_bytes;
+    const double* rcoef;           // recon coefficients [n_terms][4]
+    const uint8_t* rexp;           // recon exponents [n_terms][5]
+    uint32_t n_terms;
+    double off[6];                 // ox, oy, ox', oy' (cm, rad), s_p, dth (rad)
+    double* g_coef;                // sums over events of dL_e/dC, [n_terms][4]
+    double* g_out;                 // [0] sum L_e, [1..6] sums of dL_e/d offsets, [7] events excluded,
+                                   // [8] events used (set at the end)
+};
+constexpr int kAdjTermsPerLane = 2;   // n_terms <= 512
+
+__global__ void select_events(const uint8_t* mask, uint32_t n, uint32_t* sel, uint32_t* n_sel)
+{
+    __shared__ uint32_t wave_n[kBlock / 64];
+    __shared__ uint32_t base;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool keep = i < n && mask[i] != 0;
+    const uint64_t m = __ballot(keep);
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wave_n[w];
+        base = tot ? atomicAdd(n_sel, tot) : 0u;
+    }
+    __syncthreads();
+    if (keep) {
+        uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; ++w) slot += wave_n[w];
+        sel[slot] = i;
+    }
+}
+
+// The step program of one event with tangents, up to the track fit.
+// Returns false when a hit of the fit is exactly zero (LFIT would skip it and
+// the fit would not be the fixed linear map), which excludes the event.
+template <int ARM, class R>
+__device__ bool adjoint_track(R& g, Track& t, double (&ta)[5], double (&tb)[5], const AdjArgs& a, const Flags& F,
+                              double* lds_hits, double (&fp)[4], double (&fa)[4], double (&fb)[4])
+{
+    const cdptr coef = (cdptr)a.coef;
+    const coptr ops = (coptr)a.ops;
+    const ctptr tab = (ctptr)a.tab;
+    double resmult = 0.0;
+    ThetaCache ms;
+    // fit tangents: slope / intercept of the x and y plane sets, two directions
+    double sax = 0, sbx = 0, say = 0, sby = 0, iax = 0, ibx = 0, iay = 0, iby = 0;
+    for (int pc = 0;; ++pc) {
+        const int code = ops[pc].code;
+        if (code == OP_END) break;
+        const double p0 = ops[pc].p[0], p1 = ops[pc].p[1];
+        int ng = 0;
+        if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
+        else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
+        else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+        for (int q = 0; q < ng; ++q) {
+            const double v = gauss1(g, 99.0);
+            d0 = d1; d1 = d2; d2 = d3; d3 = v;
+        }
+        switch (code) {
+        case OP_PROJECT:
+        case OP_COLLIMATOR: {  // the collimator's thickness is a drift for an event that passes
+            const double L = code == OP_PROJECT ? p0 : ops[pc].p[6];
+            project(g, t, F, L);
+            ta[0] = ta[0] + ta[1] * L; ta[2] = ta[2] + ta[3] * L;
+            tb[0] = tb[0] + tb[1] * L; tb[2] = tb[2] + tb[3] * L;
+            break;
+        }
+        case OP_TRANSP: {
+            const double r[5] = {t.xs, t.dxdzs * 1000.0, t.ys, t.dydzs * 1000.0, t.dpps};
+            const double ra[5] = {ta[0], ta[1] * 1000.0, ta[2], ta[3] * 1000.0, ta[4]};
+            const double rb[5] = {tb[0], tb[1] * 1000.0, tb[2], tb[3] * 1000.0, tb[4]};
+            double o[5], oa[5], ob[5];
+            poly_fwd_tan<ARM>(ops[pc].arg, coef, r, ra, rb, o, oa, ob);
+            t.xs = o[0]; t.dxdzs = o[1] / 1000.0; t.ys = o[2]; t.dydzs = o[3] / 1000.0;
+            t.pathlen = t.pathlen + (p0 + -o[4]);
+            ta[0] = oa[0]; ta[1] = oa[1] / 1000.0; ta[2] = oa[2]; ta[3] = oa[3] / 1000.0;
+            tb[0] = ob[0]; tb[1] = ob[1] / 1000.0; tb[2] = ob[2]; tb[3] = ob[3] / 1000.0;
+            break;
+        }
+        case OP_RESMULT_DRAW:
+            resmult = g.next() < p0 ? 2.0 : 1.0;
+            break;
+        case OP_RESMULT_ONE:
+            resmult = 1.0;
+            break;
+        case OP_MUSC:
+            if (ng) {
+                const double ts = ms.theta(t, p1, ops[pc].p[2]);
+                t.dydzs = t.dydzs + ts * d2;
+                t.dxdzs = t.dxdzs + ts * d3;
+            }
+            break;
+        case OP_MUSC_EXT:
+            if (ng) {
+                const double ts = ms.theta(t, ops[pc].p[2], ops[pc].p[3]);
+                const double s12 = 3.4641016151377544;  // sqrt(12.)
+                t.dxdzs = t.dxdzs + ts * d0;
+                t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
+                t.dydzs = t.dydzs + ts * d2;
+                t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
+            }
+            break;
+        case OP_WIRE: {
+            const int plane = ops[pc].stop;
+            const bool yplane = ops[pc].arg != 0;
+            const double h = yplane ? t.ys + F.sigma * d3 * resmult : t.xs + F.sigma * d2 * resmult;
+            lds_hits[plane * kBlock + threadIdx.x] = h;
+            const double wa = tab->fw_a[plane], wb = tab->fw_b[plane];
+            if (yplane) {
+                say += wa * ta[2]; sby += wa * tb[2]; iay += wb * ta[2]; iby += wb * tb[2];
+            } else {
+                sax += wa * ta[0]; sbx += wa * tb[0]; iax += wb * ta[0]; ibx += wb * tb[0];
+            }
+            break;
+        }
+        case OP_FIT: {
+            double zdc[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
+            Flags F64 = F;
+            F64.fit_f32 = 0;
+            fit_track(F64, lds_hits, zdc, fp[0], fp[1], fp[2], fp[3]);
+            fa[0] = iax; fa[1] = sax; fa[2] = iay; fa[3] = say;
+            fb[0] = ibx; fb[1] = sbx; fb[2] = iby; fb[3] = sby;
+            bool all = true;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) all = all && lds_hits[k * kBlock + threadIdx.x] != 0.0;
+            return all;
+        }
+        default:  // aperture tests, rotations, scintillators: no change of the track
+            break;
+        }
+    }
+    return false;  // a program without a fit
+}
+
+template <int ARM, class R>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a)
+{
+    __shared__ double lds_hits[12 * kBlock];
+    __shared__ double tile_h[5 * kBlock];
+    __shared__ double tile_w[4 * kBlock];
+    const uint32_t n_sel = *a.n_sel;
+    const uint32_t n_tiles = (n_sel + kBlock - 1) / kBlock;
+    // this lane's reconstruction terms and their exponents
+    uint32_t own_t[kAdjTermsPerLane];
+    uint8_t own_e[kAdjTermsPerLane][5];
+    double gc[kAdjTermsPerLane][4];
+    for (int j = 0; j < kAdjTermsPerLane; ++j) {
+        own_t[j] = threadIdx.x + j * kBlock;
+        for (int v = 0; v < 5; ++v)
+            own_e[j][v] = own_t[j] < a.n_terms ? a.rexp[own_t[j] * 5 + v] : (uint8_t)0;
+        for (int k = 0; k < 4; ++k) gc[j][k] = 0.0;
+    }
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // loss, 6 offsets, excluded
+    Flags F;
+    F.ctau = 0.0;
+    F.p_spec = a.set0.p_spec;
+    F.sigma = a.set0.sigma;
+    F.ms = (a.flags & KF_MS) != 0;
+    F.wcs = (a.flags & KF_WCS) != 0;
+    F.decay = 0;
+    F.coll = 0;
+    F.fit_f32 = 0;
+    const double sp = a.off[4], dth = a.off[5];
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const uint32_t i = tile * kBlock + threadIdx.x;
+        double wk[4] = {0, 0, 0, 0}, hv[5] = {0, 0, 0, 0, 0};
+        if (i < n_sel) {
+            const uint32_t ev = a.sel[i];
+            const uint32_t n = a.n, off = ev * 8u;
+            R g;
+            g.init(a.seed, a.set0.event0 + ev, a.tape, ev * a.stride_bytes, a.stride_bytes);
+            const double dpp = ld64(a.dpp, n, off), dxdz = ld64(a.dxdz, n, off), dydz = ld64(a.dydz, n, off);
+            const double yt = ld64(a.y, n, off);
+            Track t;
+            t.xs = ld64(a.x, n, off);
+            t.ys = yt;
+            t.dxdzs = dxdz;
+            t.dydzs = dydz - dth;
+            t.dpps = sp == 0.0 ? dpp : 100.0 * ((1.0 + dpp / 100.0) / (1.0 + sp) - 1.0);
+            t.p = a.set0.p_spec * (1.0 + dpp / 100.0);
+            t.m2 = a.set0.m2;
+            t.pathlen = 0.0;
+            t.decdist = 0.0;
+            t.dflag = 0;
+            double ta[5] = {0, 0, 0, 0, -100.0 * (1.0 + dpp / 100.0) / ((1.0 + sp) * (1.0 + sp))};
+            double tb[5] = {0, 0, 0, -1.0, 0};
+            double fp[4], fa[4], fb[4];
+            const bool ok = adjoint_track<ARM>(g, t, ta, tb, a, F, lds_hits, fp, fa, fb) && !g.overflow;
+            if (!ok) {
+                acc[7] += 1.0;
+            } else {
+                double h4 = (a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off)) / 100.0;
+                if (fabs(h4) <= 1.e-30) h4 = 1.e-30;
+                hv[0] = (fp[0] + a.off[0]) / 100.0;
+                hv[1] = fp[1] + a.off[2];
+                hv[2] = (fp[2] + a.off[1]) / 100.0;
+                hv[3] = fp[3] + a.off[3];
+                hv[4] = h4;
+                double s0, s1, s2, s3, g00, g01, g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33;
+                if (ARM == SIMC_ARM_HMS)
+                    cosy_hms_recon_grad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3, g00, g01,
+                                        g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33);
+                else
+                    cosy_shms_recon_grad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3, g00, g01,
+                                         g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33);
+                // residuals: x'_r = s0, y_r = 100 s1, y'_r = s2, delta_r = 100 s3
+                const double r0 = s0 - dxdz, r1 = s1 * 100.0 - yt, r2 = s2 - dydz, r3 = s3 * 100.0 - dpp;
+                acc[0] += r0 * r0 + r1 * r1 + r2 * r2 + r3 * r3;
+                wk[0] = 2.0 * r0;
+                wk[1] = 200.0 * r1;
+                wk[2] = 2.0 * r2;
+                wk[3] = 200.0 * r3;
+                // dL_e/dh_i
+                const double dh0 = wk[0] * g00 + wk[1] * g10 + wk[2] * g20 + wk[3] * g30;
+                const double dh1 = wk[0] * g01 + wk[1] * g11 + wk[2] * g21 + wk[3] * g31;
+                const double dh2 = wk[0] * g02 + wk[1] * g12 + wk[2] * g22 + wk[3] * g32;
+                const double dh3 = wk[0] * g03 + wk[1] * g13 + wk[2] * g23 + wk[3] * g33;
+                acc[1] += dh0 / 100.0;   // ox
+                acc[2] += dh2 / 100.0;   // oy
+                acc[3] += dh1;           // ox'
+                acc[4] += dh3;           // oy'
+                acc[5] += dh0 * fa[0] / 100.0 + dh1 * fa[1] + dh2 * fa[2] / 100.0 + dh3 * fa[3];   // s_p
+                acc[6] += dh0 * fb[0] / 100.0 + dh1 * fb[1] + dh2 * fb[2] / 100.0 + dh3 * fb[3];   // dth
+            }
+        }
+        for (int v = 0; v < 5; ++v) tile_h[v * kBlock + threadIdx.x] = hv[v];
+        for (int k = 0; k < 4; ++k) tile_w[k * kBlock + threadIdx.x] = wk[k];
+        __syncthreads();
+        // dL/dC[t][k] += w_k(e) m_t(h_e) over the tile's events (w = 0 for lanes without an event)
+        const uint32_t ne = n_sel - tile * kBlock < (uint32_t)kBlock ? n_sel - tile * kBlock : (uint32_t)kBlock;
+        for (uint32_t e = 0; e < ne; ++e) {
+            double he[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) he[v] = tile_h[v * kBlock + e];
+            const double w0 = tile_w[e], w1 = tile_w[kBlock + e], w2 = tile_w[2 * kBlock + e],
+                         w3 = tile_w[3 * kBlock + e];
+#pragma unroll
+            for (int j = 0; j < kAdjTermsPerLane; ++j) {
+                double m = 1.0;
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    for (int q = 0; q < own_e[j][v]; ++q) m *= he[v];
+                gc[j][0] = __builtin_fma(w0, m, gc[j][0]);
+                gc[j][1] = __builtin_fma(w1, m, gc[j][1]);
+                gc[j][2] = __builtin_fma(w2, m, gc[j][2]);
+                gc[j][3] = __builtin_fma(w3, m, gc[j][3]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < kAdjTermsPerLane; ++j)
+        if (own_t[j] < a.n_terms)
+            for (int k = 0; k < 4; ++k)
+                if (gc[j][k] != 0.0) atomicAdd(&a.g_coef[own_t[j] * 4 + k], gc[j][k]);
+    // workgroup sums of the loss, offset gradients and exclusions
+    for (int q = 0; q < 8; ++q) {
+        lds_hits[q * kBlock + threadIdx.x] = acc[q];
+    }
+    __syncthreads();
+    for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride)
+            for (int q = 0; q < 8; ++q)
+                lds_hits[q * kBlock + threadIdx.x] += lds_hits[q * kBlock + threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8 && lds_hits[threadIdx.x * kBlock] != 0.0)
+        atomicAdd(&a.g_out[threadIdx.x], lds_hits[threadIdx.x * kBlock]);
+}
+
+// Divide the sums by the number of events used, N = selected - excluded;
+// g_out[8] receives N.  (g_out[7], the exclusions, is read by every
+// workgroup and left as it is.)
+__global__ void adjoint_finish(double* g_coef, uint32_t n_coef, double* g_out, const uint32_t* n_sel)
+{
+    const double n_used = (double)*n_sel - g_out[7];
+    const double inv = n_used > 0.0 ? 1.0 / n_used : 0.0;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_coef) g_coef[i] *= inv;
+    if (i == 0) {
+        for (int q = 0; q < 7; ++q) g_out[q] *= inv;
+        g_out[8] = n_used;
+    }
+}
+
 // Queue counter reset (a kernel rather than a memset, so that a captured
 // launch sequence replays correctly).
 __global__ void zero_u32x2(uint32_t* p)
@@ -727,6 +1079,12 @@ struct simc_ctx {
     uint32_t* d_qcount[2] = {nullptr, nullptr};   // the two queue counters
     uint32_t q_cap[2] = {0, 0};
     std::mutex mu;                   // guards queue growth
+    // reconstruction exponents per arm (adjoint form), [n_terms][5]
+    uint8_t* d_rexp[2] = {nullptr, nullptr};
+    // adjoint scratch: selected-event list and its counter (grown on demand)
+    uint32_t* d_sel = nullptr;
+    uint32_t* d_nsel = nullptr;
+    int64_t sel_cap = 0;
     // staging buffers of the host-pointer entry point (grown on demand)
     std::mutex host_mu;
     void* d_stage = nullptr;
@@ -1110,6 +1468,21 @@ ArmTables make_tables(const simc_ctx* ctx, int arm)
     else
         T.coll = CollGeom{8.50, 12.50, 8.65, 12.85, 0.000, +0.000, 0.42084, 6.35 / 20};      // shms/mc_shms_coll.f:23-54
     std::memcpy(T.zdc, ctx->zdc[arm_index(arm)], sizeof T.zdc);
+    // LFIT (cern/lfit.f:22-59) over x planes 1,3,4,6,7,9,10,12 and y planes
+    // 2,5,8,11: A = sum (z - zm)(h - hm) / sum (z - zm)^2, B = hm - A zm.
+    const int xpl[8] = {0, 2, 3, 5, 6, 8, 9, 11}, ypl[4] = {1, 4, 7, 10};
+    auto weights = [&](const int* pl, int np) {
+        double zm = 0.0, szz = 0.0;
+        for (int j = 0; j < np; ++j) zm += T.zdc[pl[j]];
+        zm /= np;
+        for (int j = 0; j < np; ++j) szz += (T.zdc[pl[j]] - zm) * (T.zdc[pl[j]] - zm);
+        for (int j = 0; j < np; ++j) {
+            T.fw_a[pl[j]] = (T.zdc[pl[j]] - zm) / szz;
+            T.fw_b[pl[j]] = 1.0 / np - zm * T.fw_a[pl[j]];
+        }
+    };
+    weights(xpl, 8);
+    weights(ypl, 4);
     return T;
 }
 
@@ -1153,6 +1526,14 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     const ArmTables T = make_tables(ctx, arm);
     HIPCHK(hipMalloc(&ctx->d_tab[ai], sizeof(ArmTables)));
     HIPCHK(hipMemcpy(ctx->d_tab[ai], &T, sizeof(ArmTables), hipMemcpyHostToDevice));
+    {
+        const auto& terms = ctx->rec[ai].terms;
+        std::vector<uint8_t> ex(terms.size() * 5);
+        for (size_t t = 0; t < terms.size(); ++t)
+            for (int v = 0; v < 5; ++v) ex[t * 5 + v] = terms[t].e[v];
+        HIPCHK(hipMalloc(&ctx->d_rexp[ai], ex.size()));
+        HIPCHK(hipMemcpy(ctx->d_rexp[ai], ex.data(), ex.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMalloc(&ctx->d_hist[ai], kHistSlots * sizeof(unsigned long long)));
     HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
     return SIMC_OK;
@@ -1303,6 +1684,10 @@ void simc_ctx_destroy(simc_ctx* ctx)
         if (ctx->d_qcount[i]) (void)hipFree(ctx->d_qcount[i]);
     }
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->d_rexp[i]) (void)hipFree(ctx->d_rexp[i]);
+    if (ctx->d_sel) (void)hipFree(ctx->d_sel);
+    if (ctx->d_nsel) (void)hipFree(ctx->d_nsel);
     delete ctx;
 }
 
@@ -1683,6 +2068,94 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
 int simc_stop_flops(simc_ctx* ctx, int arm, int64_t* flops, int nflops)
 {
     return simc_stop_flops_stage(ctx, arm, 0, flops, nflops);
+}
+
+int simc_recon_coefficients(simc_ctx* ctx, int arm, double* coef, int32_t* n_terms)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "simc_recon_coefficients: NULL context");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    const auto& terms = ctx->rec[ai].terms;
+    if (n_terms) *n_terms = (int32_t)terms.size();
+    if (coef)
+        for (size_t t = 0; t < terms.size(); ++t)
+            for (int k = 0; k < 4; ++k) coef[t * 4 + k] = terms[t].c[k];
+    return SIMC_OK;
+}
+
+int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                     const uint8_t* mask, int64_t n, const simc_rng* rng, const double* recon_coef,
+                     const double* offsets, double* grad_coef, double* out9, void* stream)
+{
+    if (!ctx || !s || !in || !mask || !rng || !recon_coef || !offsets || !grad_coef || !out9)
+        return fail(SIMC_E_ARG, "simc_adjoint_run: NULL argument");
+    int rc = check_settings(arm, s);
+    if (rc) return rc;
+    const int ai = arm_index(arm);
+    if (needs_rare(s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
+    if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");
+    if (n <= 0 || n > ((int64_t)1 << 29)) return fail(SIMC_E_ARG, "simc_adjoint_run: n must be in [1, 2^29]");
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
+    const bool tape = rng->kind == SIMC_RNG_TAPE;
+    AdjArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (tape) {
+        if (!rng->tape || rng->stride <= 0) return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
+        if ((double)n * (double)rng->stride * 8.0 >= 4294967296.0)
+            return fail(SIMC_E_ARG, "simc_adjoint_run: tape larger than 4 GB");
+        a.stride_bytes = (uint32_t)(rng->stride * 8);
+        a.tape = rng->tape;
+    }
+    const int n_terms = (int)ctx->rec[ai].terms.size();
+    if (n_terms > kAdjTermsPerLane * kBlock) return fail(SIMC_E_MAPSTRUCT, "too many reconstruction terms");
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        if (ctx->sel_cap < n) {
+            if (ctx->d_sel) {
+                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipFree(ctx->d_sel));
+                ctx->d_sel = nullptr;
+                ctx->sel_cap = 0;
+            }
+            HIPCHK(hipMalloc(&ctx->d_sel, (size_t)n * 4));
+            ctx->sel_cap = n;
+        }
+        if (!ctx->d_nsel) HIPCHK(hipMalloc(&ctx->d_nsel, 256));
+    }
+    a.set0 = make_setdev(arm, s, rng->seed, rng->event0, 0, 0, 0);
+    a.flags = a.set0.flags;
+    a.seed = rng->seed;
+    a.n = (uint32_t)n;
+    a.tab = ctx->d_tab[ai];
+    a.coef = ctx->d_coef[ai];
+    a.ops = ctx->d_ops[ai];
+    a.dpp = in->dpp; a.x = in->x; a.y = in->y; a.dxdz = in->dxdz; a.dydz = in->dydz; a.fry = in->fry;
+    a.sel = ctx->d_sel;
+    a.n_sel = ctx->d_nsel;
+    a.rcoef = recon_coef;
+    a.rexp = ctx->d_rexp[ai];
+    a.n_terms = (uint32_t)n_terms;
+    for (int q = 0; q < 6; ++q) a.off[q] = offsets[q];
+    a.g_coef = grad_coef;
+    a.g_out = out9;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nc = (uint32_t)n_terms * 4;
+    hipLaunchKernelGGL(zero_u64, dim3((nc + 255) / 256), dim3(256), 0, st, (unsigned long long*)grad_coef, nc);
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, (unsigned long long*)out9, 9u);
+    hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, ctx->d_nsel);
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(select_events, dim3(blocks), dim3(kBlock), 0, st, mask, (uint32_t)n, ctx->d_sel, ctx->d_nsel);
+    const unsigned grid = std::min<unsigned>(blocks, (unsigned)ctx->n_cu * 8u);
+#define ADJ(ARMV, RV) hipLaunchKernelGGL((adjoint_kernel<ARMV, RV>), dim3(grid), dim3(kBlock), 0, st, a)
+    if (arm == SIMC_ARM_HMS) {
+        if (tape) ADJ(SIMC_ARM_HMS, TapeStream); else ADJ(SIMC_ARM_HMS, PhiloxStream);
+    } else {
+        if (tape) ADJ(SIMC_ARM_SHMS, TapeStream); else ADJ(SIMC_ARM_SHMS, PhiloxStream);
+    }
+#undef ADJ
+    hipLaunchKernelGGL(adjoint_finish, dim3((nc + 255) / 256), dim3(256), 0, st, grad_coef, nc, out9, ctx->d_nsel);
+    HIPCHK(hipGetLastError());
+    return SIMC_OK;
 }
 
 int simc_batch_create(simc_ctx* ctx, const simc_setting* settings, int n_settings, simc_batch** out)

@@ -173,6 +173,25 @@ module simc_gpu
       integer(c_int64_t), intent(out) :: flops(*)
     end function
 
+    integer(c_int) function simc_adjoint_run(ctx, arm, s, evin, mask, n, rng, recon_coef, offsets, grad_coef, &
+        out9, stream) bind(C, name="simc_adjoint_run")
+      import :: c_int, c_int64_t, c_ptr, c_double, simc_settings, simc_events_in, simc_rng
+      type(c_ptr), value :: ctx, mask, recon_coef, grad_coef, out9, stream
+      integer(c_int), value :: arm
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+      real(c_double), intent(in) :: offsets(6)
+    end function
+
+    integer(c_int) function simc_recon_coefficients(ctx, arm, coef, n_terms) bind(C, name="simc_recon_coefficients")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, coef
+      integer(c_int), value :: arm
+      integer(c_int32_t), intent(out) :: n_terms
+    end function
+
     integer(c_int) function simc_batch_create(ctx, settings, n_settings, b) bind(C, name="simc_batch_create")
       import :: c_int, c_ptr, simc_setting
       type(c_ptr), value :: ctx
