@@ -16,8 +16,8 @@ shms/shms_forward.dat, shms/shms_recon.dat) and writes cosy_gen.inc:
       coefficient VALUES stay a run-time input;
 * for the adjoint form: each forward class with two tangent directions
   carried along (same operations for the values), and the recon polynomial
-  with all coefficients as run-time values plus its gradient with respect to
-  the focal-plane inputs;
+  with all coefficients as run-time values, plus the gradient of its sums
+  contracted with per-event weights with respect to the focal-plane inputs;
 * the structure tables (exponents, non-zero masks, pack order) the host uses
   to check a map loaded at run time against the compiled structure and to
   pack its coefficients.
@@ -241,28 +241,72 @@ def emit_poly_tangent(fname, terms, pack, inputs, lines):
     lines.append("")
 
 
-def emit_recon_grad(fname, terms, inputs, lines):
+def emit_recon_full(fname, terms, inputs, lines):
     """The reconstruction polynomial with EVERY coefficient a run-time value,
     c[4 t + k] for term t of the map file (zero coefficients included: they
-    are parameters of an optics fit), and the derivatives of the four sums
-    with respect to the first four inputs (the fifth, x_tar, is data).  Terms
-    are visited in the same trie order as emit_poly's function."""
+    are parameters of an optics fit).  Same trie order and operations as
+    emit_poly's function, so with the map's own coefficients the sums are
+    bit-identical."""
+    byexp, root, needed = _trie(terms)
+    body = []
+    counter = [0]
+
+    def fmas(node, m):
+        for ti in byexp.get(node, []):
+            for k in range(4):
+                ci = f"c[{4 * ti + k}]"
+                body.append(f"  s{k} += {ci};" if m is None else f"  s{k} = __builtin_fma({m}, {ci}, s{k});")
+
+    def walk(node, m):
+        fmas(node, m)
+        lv = _last_var(node) if node != root else 0
+        for v in range(lv, NVAR):
+            ch = list(node)
+            ch[v] += 1
+            ch = tuple(ch)
+            if ch not in needed:
+                continue
+            if m is None:
+                cm = inputs[v]
+            else:
+                counter[0] += 1
+                cm = f"m{counter[0]}"
+                body.append(f"  const double {cm} = {m} * {inputs[v]};")
+            walk(ch, cm)
+
+    walk(root, None)
+    args = ", ".join(f"double {x}" for x in inputs)
+    outs = ", ".join(f"double& s{k}" for k in range(4))
+    lines.append(f"__device__ __forceinline__ void {fname}(const double __attribute__((address_space(4)))* "
+                 f"__restrict__ c, {args}, {outs})")
+    lines.append("{")
+    for k in range(4):
+        lines.append(f"  s{k} = 0.0;")
+    lines.extend(body)
+    lines.append("}")
+    lines.append("")
+
+
+def emit_recon_wgrad(fname, terms, inputs, lines):
+    """Gradient of Q = sum_k w_k s_k (the reconstruction sums contracted with
+    per-event weights) with respect to the first four inputs (the fifth,
+    x_tar, is data): every monomial carries its value and four partial
+    derivatives along the trie, every term its weight q_t = sum_k w_k c[4t+k]."""
     byexp, root, needed = _trie(terms)
     body = []
     counter = [0]
     ND = 4
 
     def fmas(node, m):
+        if m is None:
+            return  # the constant term has no gradient
         for ti in byexp.get(node, []):
-            for k in range(4):
-                ci = f"c[{4 * ti + k}]"
-                if m is None:
-                    body.append(f"  s{k} += {ci};")
-                else:
-                    body.append(f"  s{k} = __builtin_fma({m[0]}, {ci}, s{k});")
-                    for i in range(ND):
-                        if m[1 + i] is not None:
-                            body.append(f"  g{k}{i} = __builtin_fma({m[1 + i]}, {ci}, g{k}{i});")
+            q = f"q{ti}"
+            body.append(f"  const double {q} = __builtin_fma(w3, c[{4 * ti + 3}], __builtin_fma(w2, c[{4 * ti + 2}], "
+                        f"__builtin_fma(w1, c[{4 * ti + 1}], w0 * c[{4 * ti}])));")
+            for i in range(ND):
+                if m[1 + i] is not None:
+                    body.append(f"  d{i} = __builtin_fma({m[1 + i]}, {q}, d{i});")
 
     def walk(node, m):
         fmas(node, m)
@@ -274,39 +318,36 @@ def emit_recon_grad(fname, terms, inputs, lines):
             if ch not in needed:
                 continue
             x = inputs[v]
+            counter[0] += 1
+            n = counter[0]
             if m is None:
                 cm = (x,) + tuple("1.0" if i == v else None for i in range(ND))
             else:
-                counter[0] += 1
-                n = counter[0]
                 names = [f"m{n}"]
                 body.append(f"  const double m{n} = {m[0]} * {x};")
                 for i in range(ND):
                     d = m[1 + i]
-                    terms_ = []
+                    parts = []
                     if d is not None:
-                        terms_.append(f"{d} * {x}")
+                        parts.append(f"{d} * {x}")
                     if i == v:
-                        terms_.append(m[0])
-                    if not terms_:
+                        parts.append(m[0])
+                    if not parts:
                         names.append(None)
                         continue
-                    body.append(f"  const double m{n}d{i} = {' + '.join(terms_)};")
+                    body.append(f"  const double m{n}d{i} = {' + '.join(parts)};")
                     names.append(f"m{n}d{i}")
                 cm = tuple(names)
             walk(ch, cm)
 
     walk(root, None)
     args = ", ".join(f"double {x}" for x in inputs)
-    outs = ", ".join([f"double& s{k}" for k in range(4)] +
-                     [f"double& g{k}{i}" for k in range(4) for i in range(ND)])
     lines.append(f"__device__ __forceinline__ void {fname}(const double __attribute__((address_space(4)))* "
-                 f"__restrict__ c, {args}, {outs})")
+                 f"__restrict__ c, {args}, double w0, double w1, double w2, double w3, "
+                 f"double& d0, double& d1, double& d2, double& d3)")
     lines.append("{")
-    for k in range(4):
-        lines.append(f"  s{k} = 0.0;")
-        for i in range(ND):
-            lines.append(f"  g{k}{i} = 0.0;")
+    for i in range(ND):
+        lines.append(f"  d{i} = 0.0;")
     lines.extend(body)
     lines.append("}")
     lines.append("")
@@ -342,7 +383,8 @@ def generate(map_dir, out_path):
             infos.append((cls, terms, 5, offset, pack, nmul))
             offset += len(pack)
         fname = f"cosy_{arm}_recon"
-        emit_recon_grad(fname + "_grad", recon, ["h0", "h1", "h2", "h3", "h4"], dev)
+        emit_recon_full(fname + "_full", recon, ["h0", "h1", "h2", "h3", "h4"], dev)
+        emit_recon_wgrad(fname + "_wgrad", recon, ["h0", "h1", "h2", "h3", "h4"], dev)
         dev.append(f"#define COSY_{arm.upper()}_RECON_TERMS {len(recon)}")
         pack, nmul = emit_poly(fname, recon, 4, ["h0", "h1", "h2", "h3", "h4"], dev)
         infos.append((0, recon, 4, offset, pack, nmul))

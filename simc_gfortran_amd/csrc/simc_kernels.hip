@@ -731,6 +731,8 @@ struct AdjArgs {
     uint32_t n_terms;
     double off[6];                 // ox, oy, ox', oy' (cm, rad), s_p, dth (rad)
     double* g_coef;                // sums over events of dL_e/dC, [n_terms][4]
+    double* hw;                    // per selected event: h0..h4, w0..w3 (SoA, n entries each)
+    double* ft;                    // per selected event: fit (4), its tangents (4 + 4), validity
     double* g_out;                 // [0] sum L_e, [1..6] sums of dL_e/d offsets, [7] events excluded,
                                    // [8] events used (set at the end)
 };
@@ -864,25 +866,16 @@ __device__ bool adjoint_track(R& g, Track& t, double (&ta)[5], double (&tb)[5], 
     return false;  // a program without a fit
 }
 
+// Pass 1: one selected event per lane -- transport with tangents up to the
+// track fit.  Writes the fitted track, its two tangents and a validity flag
+// (SoA [13][n_sel] in ft).
 template <int ARM, class R>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a)
 {
     __shared__ double lds_hits[12 * kBlock];
-    __shared__ double tile_h[5 * kBlock];
-    __shared__ double tile_w[4 * kBlock];
     const uint32_t n_sel = *a.n_sel;
-    const uint32_t n_tiles = (n_sel + kBlock - 1) / kBlock;
-    // this lane's reconstruction terms and their exponents
-    uint32_t own_t[kAdjTermsPerLane];
-    uint8_t own_e[kAdjTermsPerLane][5];
-    double gc[kAdjTermsPerLane][4];
-    for (int j = 0; j < kAdjTermsPerLane; ++j) {
-        own_t[j] = threadIdx.x + j * kBlock;
-        for (int v = 0; v < 5; ++v)
-            own_e[j][v] = own_t[j] < a.n_terms ? a.rexp[own_t[j] * 5 + v] : (uint8_t)0;
-        for (int k = 0; k < 4; ++k) gc[j][k] = 0.0;
-    }
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // loss, 6 offsets, excluded
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_sel) return;
     Flags F;
     F.ctau = 0.0;
     F.p_spec = a.set0.p_spec;
@@ -893,85 +886,156 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a)
     F.coll = 0;
     F.fit_f32 = 0;
     const double sp = a.off[4], dth = a.off[5];
-    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const uint32_t i = tile * kBlock + threadIdx.x;
+    const uint32_t ev = a.sel[i];
+    const uint32_t n = a.n, off = ev * 8u;
+    R g;
+    g.init(a.seed, a.set0.event0 + ev, a.tape, ev * a.stride_bytes, a.stride_bytes);
+    const double dpp = ld64(a.dpp, n, off);
+    Track t;
+    t.xs = ld64(a.x, n, off);
+    t.ys = ld64(a.y, n, off);
+    t.dxdzs = ld64(a.dxdz, n, off);
+    t.dydzs = ld64(a.dydz, n, off) - dth;
+    t.dpps = sp == 0.0 ? dpp : 100.0 * ((1.0 + dpp / 100.0) / (1.0 + sp) - 1.0);
+    t.p = a.set0.p_spec * (1.0 + dpp / 100.0);
+    t.m2 = a.set0.m2;
+    t.pathlen = 0.0;
+    t.decdist = 0.0;
+    t.dflag = 0;
+    double ta[5] = {0, 0, 0, 0, -100.0 * (1.0 + dpp / 100.0) / ((1.0 + sp) * (1.0 + sp))};
+    double tb[5] = {0, 0, 0, -1.0, 0};
+    double fp[4], fa[4], fb[4];
+    const bool ok = adjoint_track<ARM>(g, t, ta, tb, a, F, lds_hits, fp, fa, fb) && !g.overflow;
+    const uint32_t c = a.n, o8 = i * 8u;
+    for (int k = 0; k < 4; ++k) {
+        st64(a.ft + (size_t)k * a.n, c, o8, fp[k]);
+        st64(a.ft + (size_t)(4 + k) * a.n, c, o8, fa[k]);
+        st64(a.ft + (size_t)(8 + k) * a.n, c, o8, fb[k]);
+    }
+    st64(a.ft + (size_t)12 * a.n, c, o8, ok ? 1.0 : 0.0);
+}
+
+// Pass 2: per selected event, the reconstruction with the offsets and its
+// gradient, the event's loss and offset gradients (summed per workgroup),
+// and the reconstruction inputs h and weights w = dL_e/d(sums) for pass 3
+// (both zero for an excluded event).
+template <int ARM>
+__global__ __launch_bounds__(kBlock) void adjoint_recon_kernel(AdjArgs a)
+{
+    __shared__ double red[8 * kBlock];
+    const uint32_t n_sel = *a.n_sel;
+    if (blockIdx.x * kBlock >= n_sel) return;  // whole workgroup past the selection
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // loss, 6 offsets, excluded
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n_sel) {
+        const uint32_t c = a.n, o8 = i * 8u;
         double wk[4] = {0, 0, 0, 0}, hv[5] = {0, 0, 0, 0, 0};
-        if (i < n_sel) {
+        if (ld64(a.ft + (size_t)12 * a.n, c, o8) == 0.0) {
+            acc[7] = 1.0;
+        } else {
             const uint32_t ev = a.sel[i];
             const uint32_t n = a.n, off = ev * 8u;
-            R g;
-            g.init(a.seed, a.set0.event0 + ev, a.tape, ev * a.stride_bytes, a.stride_bytes);
             const double dpp = ld64(a.dpp, n, off), dxdz = ld64(a.dxdz, n, off), dydz = ld64(a.dydz, n, off);
             const double yt = ld64(a.y, n, off);
-            Track t;
-            t.xs = ld64(a.x, n, off);
-            t.ys = yt;
-            t.dxdzs = dxdz;
-            t.dydzs = dydz - dth;
-            t.dpps = sp == 0.0 ? dpp : 100.0 * ((1.0 + dpp / 100.0) / (1.0 + sp) - 1.0);
-            t.p = a.set0.p_spec * (1.0 + dpp / 100.0);
-            t.m2 = a.set0.m2;
-            t.pathlen = 0.0;
-            t.decdist = 0.0;
-            t.dflag = 0;
-            double ta[5] = {0, 0, 0, 0, -100.0 * (1.0 + dpp / 100.0) / ((1.0 + sp) * (1.0 + sp))};
-            double tb[5] = {0, 0, 0, -1.0, 0};
             double fp[4], fa[4], fb[4];
-            const bool ok = adjoint_track<ARM>(g, t, ta, tb, a, F, lds_hits, fp, fa, fb) && !g.overflow;
-            if (!ok) {
-                acc[7] += 1.0;
-            } else {
-                double h4 = (a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off)) / 100.0;
-                if (fabs(h4) <= 1.e-30) h4 = 1.e-30;
-                hv[0] = (fp[0] + a.off[0]) / 100.0;
-                hv[1] = fp[1] + a.off[2];
-                hv[2] = (fp[2] + a.off[1]) / 100.0;
-                hv[3] = fp[3] + a.off[3];
-                hv[4] = h4;
-                double s0, s1, s2, s3, g00, g01, g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33;
-                if (ARM == SIMC_ARM_HMS)
-                    cosy_hms_recon_grad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3, g00, g01,
-                                        g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33);
-                else
-                    cosy_shms_recon_grad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3, g00, g01,
-                                         g02, g03, g10, g11, g12, g13, g20, g21, g22, g23, g30, g31, g32, g33);
-                // residuals: x'_r = s0, y_r = 100 s1, y'_r = s2, delta_r = 100 s3
-                const double r0 = s0 - dxdz, r1 = s1 * 100.0 - yt, r2 = s2 - dydz, r3 = s3 * 100.0 - dpp;
-                acc[0] += r0 * r0 + r1 * r1 + r2 * r2 + r3 * r3;
-                wk[0] = 2.0 * r0;
-                wk[1] = 200.0 * r1;
-                wk[2] = 2.0 * r2;
-                wk[3] = 200.0 * r3;
-                // dL_e/dh_i
-                const double dh0 = wk[0] * g00 + wk[1] * g10 + wk[2] * g20 + wk[3] * g30;
-                const double dh1 = wk[0] * g01 + wk[1] * g11 + wk[2] * g21 + wk[3] * g31;
-                const double dh2 = wk[0] * g02 + wk[1] * g12 + wk[2] * g22 + wk[3] * g32;
-                const double dh3 = wk[0] * g03 + wk[1] * g13 + wk[2] * g23 + wk[3] * g33;
-                acc[1] += dh0 / 100.0;   // ox
-                acc[2] += dh2 / 100.0;   // oy
-                acc[3] += dh1;           // ox'
-                acc[4] += dh3;           // oy'
-                acc[5] += dh0 * fa[0] / 100.0 + dh1 * fa[1] + dh2 * fa[2] / 100.0 + dh3 * fa[3];   // s_p
-                acc[6] += dh0 * fb[0] / 100.0 + dh1 * fb[1] + dh2 * fb[2] / 100.0 + dh3 * fb[3];   // dth
+            for (int k = 0; k < 4; ++k) {
+                fp[k] = ld64(a.ft + (size_t)k * a.n, c, o8);
+                fa[k] = ld64(a.ft + (size_t)(4 + k) * a.n, c, o8);
+                fb[k] = ld64(a.ft + (size_t)(8 + k) * a.n, c, o8);
             }
+            double h4 = (a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off)) / 100.0;
+            if (fabs(h4) <= 1.e-30) h4 = 1.e-30;
+            hv[0] = (fp[0] + a.off[0]) / 100.0;
+            hv[1] = fp[1] + a.off[2];
+            hv[2] = (fp[2] + a.off[1]) / 100.0;
+            hv[3] = fp[3] + a.off[3];
+            hv[4] = h4;
+            double s0, s1, s2, s3;
+            if (ARM == SIMC_ARM_HMS)
+                cosy_hms_recon_full((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3);
+            else
+                cosy_shms_recon_full((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], s0, s1, s2, s3);
+            // residuals: x'_r = s0, y_r = 100 s1, y'_r = s2, delta_r = 100 s3
+            const double r0 = s0 - dxdz, r1 = s1 * 100.0 - yt, r2 = s2 - dydz, r3 = s3 * 100.0 - dpp;
+            acc[0] = r0 * r0 + r1 * r1 + r2 * r2 + r3 * r3;
+            wk[0] = 2.0 * r0;
+            wk[1] = 200.0 * r1;
+            wk[2] = 2.0 * r2;
+            wk[3] = 200.0 * r3;
+            // dL_e/dh_i = d/dh_i sum_k w_k s_k
+            double dh0, dh1, dh2, dh3;
+            if (ARM == SIMC_ARM_HMS)
+                cosy_hms_recon_wgrad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], wk[0], wk[1], wk[2], wk[3],
+                                     dh0, dh1, dh2, dh3);
+            else
+                cosy_shms_recon_wgrad((cdptr)a.rcoef, hv[0], hv[1], hv[2], hv[3], hv[4], wk[0], wk[1], wk[2], wk[3],
+                                      dh0, dh1, dh2, dh3);
+            acc[1] = dh0 / 100.0;   // ox
+            acc[2] = dh2 / 100.0;   // oy
+            acc[3] = dh1;           // ox'
+            acc[4] = dh3;           // oy'
+            acc[5] = dh0 * fa[0] / 100.0 + dh1 * fa[1] + dh2 * fa[2] / 100.0 + dh3 * fa[3];   // s_p
+            acc[6] = dh0 * fb[0] / 100.0 + dh1 * fb[1] + dh2 * fb[2] / 100.0 + dh3 * fb[3];   // dth
         }
-        for (int v = 0; v < 5; ++v) tile_h[v * kBlock + threadIdx.x] = hv[v];
-        for (int k = 0; k < 4; ++k) tile_w[k * kBlock + threadIdx.x] = wk[k];
+        for (int v = 0; v < 5; ++v) st64(a.hw + (size_t)v * a.n, c, o8, hv[v]);
+        for (int k = 0; k < 4; ++k) st64(a.hw + (size_t)(5 + k) * a.n, c, o8, wk[k]);
+    }
+    for (int q = 0; q < 8; ++q) red[q * kBlock + threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride)
+            for (int q = 0; q < 8; ++q) red[q * kBlock + threadIdx.x] += red[q * kBlock + threadIdx.x + stride];
         __syncthreads();
-        // dL/dC[t][k] += w_k(e) m_t(h_e) over the tile's events (w = 0 for lanes without an event)
-        const uint32_t ne = n_sel - tile * kBlock < (uint32_t)kBlock ? n_sel - tile * kBlock : (uint32_t)kBlock;
+    }
+    if (threadIdx.x < 8 && red[threadIdx.x * kBlock] != 0.0) atomicAdd(&a.g_out[threadIdx.x], red[threadIdx.x * kBlock]);
+}
+
+// Pass 3: dL/dC[t][k] = sum_e w_k(e) m_t(h_e).  Each lane owns up to
+// kAdjTermsPerLane terms; a workgroup walks tiles of kAdjTile events: each
+// lane tabulates h_v^p (p <= 6) of one event in LDS, then every lane forms
+// its terms' monomials from the table.  Sums go to the device per workgroup.
+constexpr int kAdjTile = 128;
+constexpr int kAdjMaxExp = 6;
+__global__ __launch_bounds__(kBlock) void adjoint_coef_kernel(AdjArgs a)
+{
+    __shared__ double pw[kAdjTile][5][kAdjMaxExp + 1];
+    __shared__ double wt[kAdjTile][4];
+    const uint32_t n_sel = *a.n_sel;
+    const uint32_t n_tiles = (n_sel + kAdjTile - 1) / kAdjTile;
+    uint32_t own_t[kAdjTermsPerLane];
+    uint8_t own_e[kAdjTermsPerLane][5];
+    double gc[kAdjTermsPerLane][4];
+    for (int j = 0; j < kAdjTermsPerLane; ++j) {
+        own_t[j] = threadIdx.x + j * kBlock;
+        for (int v = 0; v < 5; ++v)
+            own_e[j][v] = own_t[j] < a.n_terms ? a.rexp[own_t[j] * 5 + v] : (uint8_t)0;
+        for (int k = 0; k < 4; ++k) gc[j][k] = 0.0;
+    }
+    const uint32_t c = a.n;
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const uint32_t e0 = tile * kAdjTile;
+        const uint32_t ne = n_sel - e0 < (uint32_t)kAdjTile ? n_sel - e0 : (uint32_t)kAdjTile;
+        if (threadIdx.x < kAdjTile) {
+            const uint32_t e = threadIdx.x;
+            const bool have = e < ne;
+            const uint32_t o8 = (e0 + e) * 8u;
+            for (int v = 0; v < 5; ++v) {
+                const double h = have ? ld64(a.hw + (size_t)v * a.n, c, o8) : 0.0;
+                double q = 1.0;
+                for (int p = 0; p <= kAdjMaxExp; ++p) {
+                    pw[e][v][p] = q;
+                    q *= h;
+                }
+            }
+            for (int k = 0; k < 4; ++k) wt[e][k] = have ? ld64(a.hw + (size_t)(5 + k) * a.n, c, o8) : 0.0;
+        }
+        __syncthreads();
         for (uint32_t e = 0; e < ne; ++e) {
-            double he[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) he[v] = tile_h[v * kBlock + e];
-            const double w0 = tile_w[e], w1 = tile_w[kBlock + e], w2 = tile_w[2 * kBlock + e],
-                         w3 = tile_w[3 * kBlock + e];
+            const double w0 = wt[e][0], w1 = wt[e][1], w2 = wt[e][2], w3 = wt[e][3];
 #pragma unroll
             for (int j = 0; j < kAdjTermsPerLane; ++j) {
-                double m = 1.0;
-#pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    for (int q = 0; q < own_e[j][v]; ++q) m *= he[v];
+                const double m = pw[e][0][own_e[j][0]] * pw[e][1][own_e[j][1]] * pw[e][2][own_e[j][2]] *
+                                 pw[e][3][own_e[j][3]] * pw[e][4][own_e[j][4]];
                 gc[j][0] = __builtin_fma(w0, m, gc[j][0]);
                 gc[j][1] = __builtin_fma(w1, m, gc[j][1]);
                 gc[j][2] = __builtin_fma(w2, m, gc[j][2]);
@@ -984,19 +1048,6 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a)
         if (own_t[j] < a.n_terms)
             for (int k = 0; k < 4; ++k)
                 if (gc[j][k] != 0.0) atomicAdd(&a.g_coef[own_t[j] * 4 + k], gc[j][k]);
-    // workgroup sums of the loss, offset gradients and exclusions
-    for (int q = 0; q < 8; ++q) {
-        lds_hits[q * kBlock + threadIdx.x] = acc[q];
-    }
-    __syncthreads();
-    for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
-        if ((int)threadIdx.x < stride)
-            for (int q = 0; q < 8; ++q)
-                lds_hits[q * kBlock + threadIdx.x] += lds_hits[q * kBlock + threadIdx.x + stride];
-        __syncthreads();
-    }
-    if (threadIdx.x < 8 && lds_hits[threadIdx.x * kBlock] != 0.0)
-        atomicAdd(&a.g_out[threadIdx.x], lds_hits[threadIdx.x * kBlock]);
 }
 
 // Divide the sums by the number of events used, N = selected - excluded;
@@ -1084,6 +1135,7 @@ struct simc_ctx {
     // adjoint scratch: selected-event list and its counter (grown on demand)
     uint32_t* d_sel = nullptr;
     uint32_t* d_nsel = nullptr;
+    double* d_hw = nullptr;          // 22 doubles per selected event (pass-1 and pass-2 outputs)
     int64_t sel_cap = 0;
     // staging buffers of the host-pointer entry point (grown on demand)
     std::mutex host_mu;
@@ -1688,6 +1740,7 @@ void simc_ctx_destroy(simc_ctx* ctx)
         if (ctx->d_rexp[i]) (void)hipFree(ctx->d_rexp[i]);
     if (ctx->d_sel) (void)hipFree(ctx->d_sel);
     if (ctx->d_nsel) (void)hipFree(ctx->d_nsel);
+    if (ctx->d_hw) (void)hipFree(ctx->d_hw);
     delete ctx;
 }
 
@@ -2108,16 +2161,22 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     }
     const int n_terms = (int)ctx->rec[ai].terms.size();
     if (n_terms > kAdjTermsPerLane * kBlock) return fail(SIMC_E_MAPSTRUCT, "too many reconstruction terms");
+    for (const auto& term : ctx->rec[ai].terms)
+        for (int v = 0; v < 5; ++v)
+            if (term.e[v] > kAdjMaxExp) return fail(SIMC_E_MAPSTRUCT, "reconstruction exponent above 6");
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
         if (ctx->sel_cap < n) {
-            if (ctx->d_sel) {
+            if (ctx->d_sel || ctx->d_hw) {
                 HIPCHK(hipDeviceSynchronize());
-                HIPCHK(hipFree(ctx->d_sel));
+                if (ctx->d_sel) HIPCHK(hipFree(ctx->d_sel));
+                if (ctx->d_hw) HIPCHK(hipFree(ctx->d_hw));
                 ctx->d_sel = nullptr;
+                ctx->d_hw = nullptr;
                 ctx->sel_cap = 0;
             }
             HIPCHK(hipMalloc(&ctx->d_sel, (size_t)n * 4));
+            HIPCHK(hipMalloc(&ctx->d_hw, (size_t)n * 22 * 8));
             ctx->sel_cap = n;
         }
         if (!ctx->d_nsel) HIPCHK(hipMalloc(&ctx->d_nsel, 256));
@@ -2138,6 +2197,8 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     for (int q = 0; q < 6; ++q) a.off[q] = offsets[q];
     a.g_coef = grad_coef;
     a.g_out = out9;
+    a.hw = ctx->d_hw;
+    a.ft = ctx->d_hw + (size_t)9 * n;
     hipStream_t st = (hipStream_t)stream;
     const uint32_t nc = (uint32_t)n_terms * 4;
     hipLaunchKernelGGL(zero_u64, dim3((nc + 255) / 256), dim3(256), 0, st, (unsigned long long*)grad_coef, nc);
@@ -2145,14 +2206,19 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, ctx->d_nsel);
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(select_events, dim3(blocks), dim3(kBlock), 0, st, mask, (uint32_t)n, ctx->d_sel, ctx->d_nsel);
-    const unsigned grid = std::min<unsigned>(blocks, (unsigned)ctx->n_cu * 8u);
-#define ADJ(ARMV, RV) hipLaunchKernelGGL((adjoint_kernel<ARMV, RV>), dim3(grid), dim3(kBlock), 0, st, a)
+#define ADJ(ARMV, RV) hipLaunchKernelGGL((adjoint_kernel<ARMV, RV>), dim3(blocks), dim3(kBlock), 0, st, a)
     if (arm == SIMC_ARM_HMS) {
         if (tape) ADJ(SIMC_ARM_HMS, TapeStream); else ADJ(SIMC_ARM_HMS, PhiloxStream);
     } else {
         if (tape) ADJ(SIMC_ARM_SHMS, TapeStream); else ADJ(SIMC_ARM_SHMS, PhiloxStream);
     }
 #undef ADJ
+    if (arm == SIMC_ARM_HMS)
+        hipLaunchKernelGGL((adjoint_recon_kernel<SIMC_ARM_HMS>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((adjoint_recon_kernel<SIMC_ARM_SHMS>), dim3(blocks), dim3(kBlock), 0, st, a);
+    const unsigned cgrid = std::min<unsigned>((unsigned)((n + kAdjTile - 1) / kAdjTile), (unsigned)ctx->n_cu * 4u);
+    hipLaunchKernelGGL(adjoint_coef_kernel, dim3(cgrid), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(adjoint_finish, dim3((nc + 255) / 256), dim3(256), 0, st, grad_coef, nc, out9, ctx->d_nsel);
     HIPCHK(hipGetLastError());
     return SIMC_OK;
