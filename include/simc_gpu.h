@@ -215,6 +215,83 @@ int simc_recon_run(simc_ctx *ctx, int arm, int64_t n, const double *x_fp, const 
                    const double *y_fp, const double *dy_fp, const double *fry, double *delta,
                    double *dxdz_tar, double *dydz_tar, double *y_tar, void *stream);
 
+/* ---- Target step: event preparation before an arm ----------------------
+ *
+ * Per event, the part of SIMC's event loop between the vertex and the arm:
+ *   trip_thru_target (target.f:1-246) with sampled losses (enerloss_new,
+ *     typeflag 1) at the vertex energy, as complete_ev does (event.f:1031-1040;
+ *     event.f:194 for the beam), the loss zeroed without using_Eloss;
+ *   target_musc (target.f:548-574) when mc_smear;
+ *   montecarlo()'s delta, angles, TRANSPORT transform, spectrometer offsets
+ *     and drift to z = 0 (simc.f:1376-1441 hadron arm, :1649-1706 electron
+ *     arm; :1364-1369 the beam's angles dang_in).
+ * The outputs of an electron or hadron preparation are the inputs of
+ * simc_arm_run (dpp, x, y, z = 0, dxdz, dydz, fry = x).  Draws come from a
+ * stream of their own: Philox (seed, event0 + i) with the counter word
+ * "TGTS", independent of the arm's stream of the same event; on a tape, the
+ * reference's order (losses: target, Al, air, kevlar, mylar, each drawing one
+ * gauss1(10) when its thickness is > 0; then the two gauss1(3.5) angles). */
+#define SIMC_TARGET_BEAM 1      /* narm of trip_thru_target, target.f:27 */
+#define SIMC_TARGET_ELECTRON 2
+#define SIMC_TARGET_HADRON 3
+
+typedef struct {
+    double A, Z;          /* target nucleus (targ%A, targ%Z)                          */
+    double rho;           /* g/cm^3                                                   */
+    double X0;            /* radiation length, g/cm^2; <= 0: init.f:18-42 from A, Z  */
+    double length;        /* cm                                                       */
+    double angle;         /* target angle, rad                                        */
+    double zoffset;       /* cm                                                       */
+    int32_t can;          /* 1 beer can, 2 pudding can, 3 2017 10 cm cell (liquid: Z < 2.4) */
+    int32_t reserved;
+} simc_target;
+
+typedef struct {
+    int32_t narm;          /* SIMC_TARGET_*                                             */
+    int32_t spectrometer;  /* electron_arm / hadron_arm: 1 HMS, 2 SOS, 3 HRS-R, 4 HRS-L,
+                              5 or 6 SHMS (window materials, target.f:63-97)            */
+    int32_t using_eloss;   /* using_Eloss                                               */
+    int32_t mc_smear;      /* target multiple scattering                                */
+    double p_spec;         /* central momentum, MeV/c                                   */
+    double th_spec;        /* central angle, rad                                        */
+    double phi;            /* spectrometer phi, rad: 3 pi/2 beam left, pi/2 beam right  */
+    double mass;           /* particle mass, MeV (Me for the electron and the beam)     */
+    double coulomb;        /* Coulomb energy (main%target%Coulomb), MeV; electron arm   */
+    double beam_energy;    /* Ebeam, MeV: energy of the beam's loss sampling            */
+    double offset[5];      /* spectrometer offsets x, y, z (cm), xptar, yptar (rad)     */
+} simc_prep_settings;
+
+/* Vertex quantities (device SoA).  x, y, z: target frame, cm, z including
+ * the target's zoffset.  E, p: energy and momentum after radiation (orig),
+ * MeV; E_vertex: before radiation (vertex), NULL = E.  theta: the particle's
+ * physics scattering angle, rad.  delta: orig delta (%), read only by a
+ * hadron arm without using_eloss (NULL otherwise).  xptar, yptar: its
+ * spectrometer angles, rad.  dang_yp, dang_xp: the beam's angles (dang_in),
+ * NULL = 0.  The beam reads z and E (= Ein) only. */
+typedef struct {
+    const double *x, *y, *z, *E, *p, *E_vertex, *theta, *delta, *xptar, *yptar, *dang_yp, *dang_xp;
+} simc_vertex_in;
+
+/* Outputs (device SoA; every pointer may be NULL).  dpp..fry: the arm's
+ * inputs; eloss (MeV) and teff (radiation lengths) of trip_thru_target;
+ * dang1, dang2: the target_musc angles (the beam's dang_in(1), dang_in(2));
+ * used: uniforms drawn, -1 when a tape slot ran out (that event's outputs
+ * are NaN). */
+typedef struct {
+    double *dpp, *x, *y, *z, *dxdz, *dydz, *fry;
+    double *eloss, *teff, *dang1, *dang2;
+    int32_t *used;
+} simc_prep_out;
+
+/* Asynchronous on `stream`. */
+int simc_target_prep(simc_ctx *ctx, const simc_target *t, const simc_prep_settings *s,
+                     const simc_vertex_in *in, simc_prep_out *out, int64_t n, const simc_rng *rng,
+                     void *stream);
+/* The same on host arrays (synchronous); returns SIMC_E_TAPE when an event's
+ * tape slot ran out. */
+int simc_target_prep_host(simc_ctx *ctx, const simc_target *t, const simc_prep_settings *s,
+                          const simc_vertex_in *in, simc_prep_out *out, int64_t n, const simc_rng *rng);
+
 /* STOP counters accumulated by the context since creation / reset, in the
  * order of the reference's common blocks.  Synchronises the device. */
 int simc_counters_get(simc_ctx *ctx, int arm, int64_t *counters, int ncounters);

@@ -220,3 +220,108 @@ def philox_normals(seed, event, n, nsigmax=99.0):
     out = np.zeros(n)
     used = lib().or_philox_normals(seed, event, n, nsigmax, _p(out))
     return out, used
+
+
+# ---- target step ---------------------------------------------------------------------
+class ORTarget(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("A", "Z", "rho", "X0", "length", "angle", "zoffset")] + \
+        [("can", C.c_int)]
+
+
+class ORPrep(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("narm", "spectrometer", "using_eloss", "mc_smear")] + \
+        [(k, C.c_double) for k in ("p_spec", "th_spec", "phi", "mass", "coulomb", "beam_energy")] + \
+        [("offset", C.c_double * 5)]
+
+
+VERTEX_FIELDS = ("x", "y", "z", "E", "p", "E_vertex", "theta", "delta", "xptar", "yptar", "dang_yp",
+                 "dang_xp")
+PREP_OUT_FIELDS = ("dpp", "x", "y", "z", "dxdz", "dydz", "fry", "eloss", "teff", "dang1", "dang2")
+
+
+class ORVertexIn(C.Structure):
+    _fields_ = [(k, _dp) for k in VERTEX_FIELDS]
+
+
+class ORPrepOut(C.Structure):
+    _fields_ = [(k, _dp) for k in PREP_OUT_FIELDS]
+
+
+def _target_lib():
+    L = lib()
+    if not hasattr(L, "_target_ready"):
+        L.or_target_prep.restype = C.c_int
+        L.or_target_prep.argtypes = [C.POINTER(ORTarget), C.POINTER(ORPrep), C.c_int64, C.POINTER(ORVertexIn),
+                                     C.POINTER(ORPrepOut), C.c_int, _dp, C.c_int64, C.c_uint64, C.c_uint64,
+                                     C.c_int, _i32p]
+        L.or_target_X0.restype = C.c_double
+        L.or_target_X0.argtypes = [C.c_double, C.c_double]
+        L._target_ready = True
+    return L
+
+
+def make_target(A=1.0, Z=1.0, rho=0.0723, X0=0.0, length=10.0, angle=0.0, zoffset=0.0, can=3):
+    """Target description; X0 <= 0 takes init.f's value for (A, Z)."""
+    if X0 <= 0.0:
+        X0 = target_X0(A, Z)
+    return ORTarget(A, Z, rho, X0, length, angle, zoffset, can)
+
+
+def target_X0(A, Z):
+    return _target_lib().or_target_X0(A, Z)
+
+
+def make_prep(narm, spectrometer=1, using_eloss=True, mc_smear=True, p_spec=0.0, th_spec=0.0, phi=0.0,
+              mass=0.51099906, coulomb=0.0, beam_energy=0.0, offset=(0.0,) * 5):
+    return ORPrep(narm, spectrometer, int(using_eloss), int(mc_smear), p_spec, th_spec, phi, mass, coulomb,
+                  beam_energy, (C.c_double * 5)(*offset))
+
+
+def target_prep(target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=None, stride=0, nthreads=0):
+    """The oracle's target step over numpy vertex arrays (fields of VERTEX_FIELDS,
+    absent = NULL).  Returns (dict of outputs incl. 'used', return code)."""
+    n = len(vertex["z"])
+    vin = {k: np.ascontiguousarray(vertex[k], dtype=np.float64) for k in VERTEX_FIELDS
+           if vertex.get(k) is not None}
+    out = {k: np.zeros(n) for k in PREP_OUT_FIELDS}
+    used = np.zeros(n, np.int32)
+    V = ORVertexIn(*[_p(vin.get(k)) for k in VERTEX_FIELDS])
+    O = ORPrepOut(*[_p(out[k]) for k in PREP_OUT_FIELDS])
+    if rng == "tape":
+        tape = np.ascontiguousarray(tape, dtype=np.float64)
+        kind, tp = 0, _p(tape)
+    else:
+        kind, tp = 1, None
+    rc = _target_lib().or_target_prep(C.byref(target), C.byref(prep), n, C.byref(V), C.byref(O), kind, tp,
+                                      int(stride), int(seed), int(event0), int(nthreads or os.cpu_count()),
+                                      _p(used, _i32p))
+    out["used"] = used
+    return out, rc
+
+
+def ref_target_step(target, narm, spec, mass, zpos, energy, theta, tape, stride, p_musc=None,
+                    beta_musc=None, typeflag=1):
+    """The reference's trip_thru_target (+ target_musc when p_musc is given)
+    from oracle/_ref over events on tape slots.  Returns dict eloss, teff,
+    dang1, dang2, used."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    if not hasattr(L, "_target_ready"):
+        L.ref_target_step.restype = C.c_int
+        L.ref_target_step.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64,
+                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i32p, _dp, C.c_int64]
+        L._target_ready = True
+    n = len(zpos)
+    t = np.array([target.A, target.Z, target.rho, target.X0, target.length, target.angle, target.zoffset])
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (zpos, energy, theta)]
+    pm = None if p_musc is None else np.ascontiguousarray(p_musc, dtype=np.float64)
+    bm = None if beta_musc is None else np.ascontiguousarray(beta_musc, dtype=np.float64)
+    out = {k: np.zeros(n) for k in ("eloss", "teff", "dang1", "dang2")}
+    used = np.zeros(n, np.int32)
+    tape = np.ascontiguousarray(tape, dtype=np.float64)
+    L.ref_target_step(_p(t), int(target.can), narm, spec, mass, typeflag, n, *[_p(v) for v in a], _p(pm),
+                      _p(bm), *[_p(out[k]) for k in ("eloss", "teff", "dang1", "dang2")], _p(used, _i32p),
+                      _p(tape), int(stride))
+    out["used"] = used
+    return out

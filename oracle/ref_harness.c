@@ -119,3 +119,50 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
     if (chdir(cwd) != 0) return -3;
     return rc;
 }
+
+/* ---- target step: the reference's trip_thru_target and target_musc ---- */
+extern void ref_set_target_(double *a, double *z, double *rho, double *x0, double *tlen,
+                            double *tang, double *zoff, int32_t *ican, int32_t *iearm,
+                            int32_t *iharm);
+extern void trip_thru_target_(int32_t *narm, double *zpos, double *energy, double *theta,
+                              double *eloss, double *radlen, double *mass, int32_t *typeflag);
+extern void target_musc_(double *p, double *beta, double *teff, double *dangles);
+
+/* n events of trip_thru_target(narm, zpos[i], energy[i], theta[i], ..., mass,
+ * typeflag) followed, when p_musc is not NULL, by target_musc(p_musc[i],
+ * beta_musc[i], teff, dangles), each event on its own tape slot.  tgt =
+ * {A, Z, rho, X0 (g/cm^2), length, angle, zoffset}; spec is the arm's
+ * spectrometer number (electron_arm and hadron_arm both set to it).
+ * used[i] = uniforms drawn, -1 on tape overflow. */
+int ref_target_step(const double *tgt, int can, int narm, int spec, double mass, int typeflag,
+                    int64_t n, const double *zpos, const double *energy, const double *theta,
+                    const double *p_musc, const double *beta_musc, double *eloss, double *teff,
+                    double *dang1, double *dang2, int32_t *used, const double *tape,
+                    int64_t stride)
+{
+    double a = tgt[0], z = tgt[1], rho = tgt[2], x0 = tgt[3], len = tgt[4], ang = tgt[5],
+           zoff = tgt[6];
+    int32_t ican = can, arm = spec;
+    ref_set_target_(&a, &z, &rho, &x0, &len, &ang, &zoff, &ican, &arm, &arm);
+    for (int64_t i = 0; i < n; i++) {
+        g_tape = tape + i * stride;
+        g_stride = stride;
+        g_pos = 0;
+        g_used = 0;
+        g_overflow = 0;
+        int32_t na = narm, tf = typeflag;
+        double zp = zpos[i], en = energy[i], th = theta[i], el = 0.0, rl = 0.0, m = mass;
+        trip_thru_target_(&na, &zp, &en, &th, &el, &rl, &m, &tf);
+        double d[2] = {0.0, 0.0};
+        if (p_musc) {
+            double p = p_musc[i], b = beta_musc[i];
+            target_musc_(&p, &b, &rl, d);
+        }
+        eloss[i] = el;
+        teff[i] = rl;
+        dang1[i] = d[0];
+        dang2[i] = d[1];
+        used[i] = g_overflow ? -1 : (int32_t)g_used;
+    }
+    return 0;
+}

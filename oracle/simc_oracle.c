@@ -1,3 +1,4 @@
+#define _GNU_SOURCE
 /*
  * simc_oracle.c -- FP64 CPU restatement of SIMC's spectrometer transport and
  * reconstruction (the oracle).
@@ -452,7 +453,8 @@ double or_grnd(or_rng *g)
     case 1: {
         uint64_t k = (uint64_t)g->pos++;
         if ((k & 3) == 0) {
-            uint32_t ctr[4] = {(uint32_t)(k >> 2), (uint32_t)g->event, (uint32_t)(g->event >> 32), OR_PHILOX_TAG};
+            uint32_t ctr[4] = {(uint32_t)(k >> 2), (uint32_t)g->event, (uint32_t)(g->event >> 32),
+                               g->tag ? g->tag : OR_PHILOX_TAG};
             or_philox4x32_10(ctr, g->key, g->buf);
         }
         return ((double)g->buf[k & 3] + 0.5) * 2.3283064365386963e-10;
@@ -2031,4 +2033,269 @@ void or_ranlux_sequence(int lux, int seed, int32_t isdext[25], double *rvec, int
     or_ranlux_save(r, isdext);
     or_ranlux_fill(r, rvec, n);
     free(r);
+}
+
+/* ========================================================================= */
+/* Target step: the event preparation before an arm                          */
+/* ========================================================================= */
+
+/* Window and wall materials, target.inc:5-28 (X0_cm = X0 / rho there). */
+#define OR_RHO_AL 2.70
+#define OR_Z_AL 13.
+#define OR_A_AL 26.98
+#define OR_X0CM_AL (24.01 / 2.70)
+#define OR_RHO_MYLAR 1.39
+#define OR_Z_MYLAR 4.545
+#define OR_A_MYLAR 8.735
+#define OR_X0CM_MYLAR (39.95 / 1.39)
+#define OR_RHO_KEVLAR 0.74
+#define OR_Z_KEVLAR 2.67
+#define OR_A_KEVLAR 4.67
+#define OR_X0CM_KEVLAR (55.2 / 0.74)
+#define OR_RHO_AIR 0.00121
+#define OR_Z_AIR 7.2
+#define OR_A_AIR 14.4
+#define OR_X0CM_AIR (36.66 / 0.00121)
+#define OR_INCH_CM 2.54
+#define OR_TARGET_PI 3.14159265358979
+
+/* Radiation length of the target nucleus in g/cm^2, init.f:18-42. */
+double or_target_X0(double A, double Z)
+{
+    const double alpha = 1. / 137.0359895; /* constants.inc:44 */
+    double L1 = log(184.15) - log(Z) / 3.0;
+    double L2 = log(1194.) - 2. * log(Z) / 3.0;
+    if (Z == 1) {
+        L1 = 5.31;
+        L2 = 6.144;
+    }
+    double za2 = (Z * alpha) * (Z * alpha);
+    double fc = za2 * (1.202 + za2 * (-1.0369 + za2 * 1.008 / (za2 + 1)));
+    long nA = lround(A);
+    if (nA == 1) return 61.28;
+    if (nA == 2) return 122.4;
+    if (nA == 4) return 94.32;
+    return 716.405 * A / Z / (Z * (L1 - fc) + L2);
+}
+
+/* trip_thru_target, target.f:1-246: path lengths through the target and the
+ * windows for particle narm (1 beam, 2 scattered electron, 3 hadron) seen by
+ * spectrometer `spec` (electron_arm / hadron_arm), the radiation lengths
+ * crossed and the sampled energy loss (enerloss_new, draws in the order
+ * target, Al, air, kevlar, mylar; a zero thickness draws nothing).
+ * Returns 0, or -1 for a spectrometer code the reference does not define. */
+int or_trip_thru_target(or_rng *g, const or_target *T, int narm, int spec, double zpos,
+                        double energy, double theta, double mass, int typeflag, double *eloss,
+                        double *radlen)
+{
+    const double X0_cm = T->X0 / T->rho;
+    const int liquid = T->Z < 2.4;
+    double s_Al = 0.0, s_air = 0.0, s_kevlar = 0.0, s_mylar = 0.0, s_target, forward_path;
+    double e_t, e_al, e_air, e_kev, e_myl;
+    if (narm == 1) { /* target.f:33-50 */
+        s_target = (T->length / 2. + zpos) / fabs(cos(T->angle));
+        if (liquid) {
+            if (T->can == 1) s_Al = s_Al + 0.0028 * OR_INCH_CM;
+            else if (T->can == 2) s_Al = s_Al + 0.0050 * OR_INCH_CM;
+            else if (T->can == 3) s_Al = s_Al + 0.013;
+        }
+        *radlen = s_target / X0_cm + s_Al / OR_X0CM_AL;
+        or_enerloss_new(g, s_target, T->rho, T->Z, T->A, energy, mass, typeflag, &e_t);
+        or_enerloss_new(g, s_Al, OR_RHO_AL, OR_Z_AL, OR_A_AL, energy, mass, typeflag, &e_al);
+        *eloss = e_t + e_al;
+        return 0;
+    }
+    /* target.f:71-101 (electron) and :184-217 (hadron): the same tables */
+    if (spec == 1) {
+        s_Al = 0.016 * OR_INCH_CM; s_air = 15; s_kevlar = 0.015 * OR_INCH_CM;
+        s_mylar = 0.005 * OR_INCH_CM;
+        forward_path = (T->length / 2. - zpos) / fabs(cos(theta + T->angle));
+    } else if (spec == 2) {
+        s_Al = 0.008 * OR_INCH_CM; s_air = 15; s_kevlar = 0.005 * OR_INCH_CM;
+        s_mylar = 0.003 * OR_INCH_CM;
+        forward_path = (T->length / 2. - zpos) / fabs(cos(theta - T->angle));
+    } else if (spec == 3 || spec == 4) {
+        s_Al = 0.013 * OR_INCH_CM; s_air = 15; s_kevlar = 0. * OR_INCH_CM;
+        s_mylar = 0.010 * OR_INCH_CM;
+        forward_path = (T->length / 2. - zpos) / fabs(cos(theta - T->angle));
+    } else if (spec == 5 || spec == 6) {
+        s_Al = (0.02 + 0.01) * OR_INCH_CM; s_air = 57.27; s_kevlar = 0.0; s_mylar = 0.0;
+        forward_path = (T->length / 2. - zpos) / fabs(cos(theta - T->angle));
+    } else {
+        return -1;
+    }
+    s_target = forward_path;
+    if (liquid) {
+        if (T->can == 1) { /* beer can, target.f:105-113 */
+            double side_path = 1.325 * OR_INCH_CM / fabs(sin(theta));
+            if (forward_path < side_path) {
+                s_Al = s_Al + 0.005 * OR_INCH_CM / fabs(cos(theta));
+            } else {
+                s_target = side_path;
+                s_Al = s_Al + 0.005 * OR_INCH_CM / fabs(sin(theta));
+            }
+        } else if (T->can == 2) { /* pudding can, target.f:114-139 */
+            double tt = tan(theta);
+            double t = tt * tt;
+            double atmp = 1 + t;
+            double btmp = -2 * zpos * t;
+            double ctmp = zpos * zpos * t - (T->length / 2.) * (T->length / 2.);
+            double z_can = (-btmp + sqrt(btmp * btmp - 4. * atmp * ctmp)) / 2. / atmp;
+            double side_path = (z_can - zpos) / fabs(cos(theta));
+            s_target = side_path;
+            double costmp = z_can / (T->length / 2.);
+            double th_can = 0.0;
+            if (fabs(costmp) <= 1) th_can = acos(z_can / (T->length / 2.));
+            /* |costmp - 1| <= 1e-6 gives 0; beyond that the reference keeps
+             * the previous event's value (a saved local); 0 here */
+            s_Al = s_Al + 0.0050 * OR_INCH_CM / fabs(sin(OR_TARGET_PI / 2 - (theta - th_can)));
+        } else if (T->can == 3) { /* 2017 cryo cell, target.f:140-165 */
+            double ecir = 1.315 * 2.54;
+            double ecor = (1.315 + 0.0071) * 2.54;
+            double entec = T->length - ecir;
+            double twall = ecor - ecir;
+            double tcm = zpos + T->length / 2.0;
+            double tliquid, tal;
+            if ((tcm + ecir / tan(theta)) < entec) {
+                tliquid = ecir / sin(theta);
+                tal = twall / sin(theta);
+            } else {
+                /* the compiled reference evaluates sin and cos of theta in
+                 * this branch with one sincos call (the compiler merges them),
+                 * whose sine can differ from sin() in the last bit */
+                double sn, cs;
+                sincos(theta, &sn, &cs);
+                double d = T->length - ecir - tcm;
+                double ds = d * sn;
+                tliquid = (sqrt(ecir * ecir - ds * ds) + d * cs);
+                tal = +(sqrt(ecor * ecor - ds * ds) - sqrt(ecir * ecir - ds * ds)) * twall /
+                      (ecor - ecir);
+            }
+            s_Al = s_Al + tal;
+            s_target = tliquid;
+        }
+    }
+    *radlen = s_target / X0_cm + s_Al / OR_X0CM_AL + s_air / OR_X0CM_AIR +
+              s_kevlar / OR_X0CM_KEVLAR + s_mylar / OR_X0CM_MYLAR;
+    or_enerloss_new(g, s_target, T->rho, T->Z, T->A, energy, mass, typeflag, &e_t);
+    or_enerloss_new(g, s_Al, OR_RHO_AL, OR_Z_AL, OR_A_AL, energy, mass, typeflag, &e_al);
+    or_enerloss_new(g, s_air, OR_RHO_AIR, OR_Z_AIR, OR_A_AIR, energy, mass, typeflag, &e_air);
+    or_enerloss_new(g, s_kevlar, OR_RHO_KEVLAR, OR_Z_KEVLAR, OR_A_KEVLAR, energy, mass, typeflag,
+                    &e_kev);
+    or_enerloss_new(g, s_mylar, OR_RHO_MYLAR, OR_Z_MYLAR, OR_A_MYLAR, energy, mass, typeflag,
+                    &e_myl);
+    *eloss = e_t + e_al + e_air + e_kev + e_myl;
+    return 0;
+}
+
+/* target_musc, target.f:548-574: Lynch-Dahl planar angles, each truncated at
+ * 3.5 sigma (gauss1). */
+void or_target_musc(or_rng *g, double p, double beta, double teff, double dangles[2])
+{
+    const double Es = 13.6, epsilon = 0.088, nsig_max = 3.5;
+    double theta_sigma = Es / p / beta * sqrt(teff) * (1 + epsilon * log10(teff / (beta * beta)));
+    dangles[0] = theta_sigma * or_gauss1(g, nsig_max);
+    dangles[1] = theta_sigma * or_gauss1(g, nsig_max);
+}
+
+/* One event of the target step for one particle:
+ *   complete_ev, event.f:1031-1040 (trip_thru_target with typeflag 1 at the
+ *     vertex energy, the loss zeroed without using_Eloss);
+ *   montecarlo, simc.f:1376-1441 (hadron) / :1649-1706 (electron): delta,
+ *     target_musc, the beam angles, the TRANSPORT transform, the offsets and
+ *     the drift to z = 0; simc.f:1364-1369 for the beam (narm 1). */
+static int prep_event(or_rng *g, const or_target *T, const or_prep *P, double cos_th,
+                      double sin_th, double sin_phi, const or_vertex_in *in, int64_t i,
+                      or_prep_out *out)
+{
+    const double tz = in->z[i];
+    const double zpos = tz - T->zoffset;
+    double eloss, teff, dang[2] = {0.0, 0.0};
+    if (P->narm == 1) {
+        if (or_trip_thru_target(g, T, 1, 0, zpos, P->beam_energy, 0.0, P->mass, 1, &eloss, &teff))
+            return -1;
+        if (!P->using_eloss) eloss = 0.0;
+        if (P->mc_smear) or_target_musc(g, in->E[i], 1.0, teff, dang);
+        if (out->eloss) out->eloss[i] = eloss;
+        if (out->teff) out->teff[i] = teff;
+        if (out->dang1) out->dang1[i] = dang[0];
+        if (out->dang2) out->dang2[i] = dang[1];
+        return 0;
+    }
+    const double E = in->E[i], p = in->p[i];
+    const double Ev = in->E_vertex ? in->E_vertex[i] : E;
+    if (or_trip_thru_target(g, T, P->narm, P->spectrometer, zpos, Ev, in->theta[i], P->mass, 1,
+                            &eloss, &teff))
+        return -1;
+    if (!P->using_eloss) eloss = 0.0;
+    double delta;
+    if (P->narm == 3) {
+        if (P->using_eloss)
+            delta = (sqrt(fabs((E - eloss) * (E - eloss) - P->mass * P->mass)) - P->p_spec) /
+                    P->p_spec * 100.;
+        else
+            delta = in->delta[i];
+        if (P->mc_smear) or_target_musc(g, p, p / E, teff, dang);
+    } else {
+        delta = 100 * (E - eloss - P->coulomb - P->p_spec) / P->p_spec;
+        if (P->mc_smear) or_target_musc(g, p, 1.0, teff, dang);
+    }
+    const double din1 = in->dang_yp ? in->dang_yp[i] : 0.0;
+    const double din2 = in->dang_xp ? in->dang_xp[i] : 0.0;
+    const double yptar = in->yptar[i] + dang[0] + din1;
+    const double xptar = in->xptar[i] + dang[1] + din2 * cos_th;
+    const double tx = in->x[i], ty = in->y[i];
+    double x = -ty;
+    double y = -tx * cos_th - tz * sin_th * sin_phi;
+    double z = tz * cos_th + tx * sin_th * sin_phi;
+    x = x - P->offset[0];
+    y = y - P->offset[1];
+    z = z - P->offset[2];
+    const double dx = xptar - P->offset[3];
+    const double dy = yptar - P->offset[4];
+    x = x - z * dx;
+    y = y - z * dy;
+    if (out->dpp) out->dpp[i] = delta;
+    if (out->x) out->x[i] = x;
+    if (out->y) out->y[i] = y;
+    if (out->z) out->z[i] = 0.0;
+    if (out->dxdz) out->dxdz[i] = dx;
+    if (out->dydz) out->dydz[i] = dy;
+    if (out->fry) out->fry[i] = x;
+    if (out->eloss) out->eloss[i] = eloss;
+    if (out->teff) out->teff[i] = teff;
+    if (out->dang1) out->dang1[i] = dang[0];
+    if (out->dang2) out->dang2[i] = dang[1];
+    return 0;
+}
+
+int or_target_prep(const or_target *T, const or_prep *P, int64_t n, const or_vertex_in *in,
+                   or_prep_out *out, int rng_kind, const double *tape, int64_t stride,
+                   uint64_t seed, uint64_t event0, int nthreads, int32_t *used)
+{
+    const double cos_th = cos(P->th_spec), sin_th = sin(P->th_spec), sin_phi = sin(P->phi);
+    if (nthreads <= 0) nthreads = 1;
+    int64_t bad = 0, overflow = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : bad, overflow)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        or_rng g;
+        memset(&g, 0, sizeof g);
+        g.kind = rng_kind;
+        if (rng_kind == 0) {
+            g.tape = tape + i * stride;
+            g.stride = stride;
+        } else {
+            g.key[0] = (uint32_t)seed;
+            g.key[1] = (uint32_t)(seed >> 32);
+            g.event = event0 + (uint64_t)i;
+            g.tag = OR_TARGET_TAG;
+        }
+        if (prep_event(&g, T, P, cos_th, sin_th, sin_phi, in, i, out)) bad++;
+        if (g.overflow) overflow++;
+        if (used) used[i] = (int32_t)g.used;
+    }
+    if (bad) return -1;
+    return overflow ? -2 : 0;
 }

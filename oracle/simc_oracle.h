@@ -62,6 +62,7 @@ typedef struct {
     uint32_t key[2];
     uint64_t event;
     uint32_t buf[4];
+    uint32_t tag;       /* fourth counter word; 0 = the arms' stream (OR_PHILOX_TAG) */
     /* ranlux */
     struct or_ranlux *rl;
     /* bookkeeping */
@@ -223,6 +224,39 @@ void or_ranlux_sequence(int lux, int seed, int32_t isdext[25], double *rvec, int
 int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const double *dx_fp,
                  const double *y_fp, const double *dy_fp, const double *fry, double *delta,
                  double *dxdz, double *dydz, double *ytar);
+
+/* ---- target step (event preparation before an arm) ----
+ * Target of trip_thru_target (target.inc:31-53 fields used there); X0 in
+ * g/cm^2 (or_target_X0 gives init.f's value). */
+typedef struct {
+    double A, Z, rho, X0, length, angle, zoffset;
+    int can;
+} or_target;
+/* One particle's preparation: narm 1 beam, 2 electron, 3 hadron; spectrometer
+ * = electron_arm / hadron_arm code; angles in rad, energies in MeV. */
+typedef struct {
+    int narm, spectrometer, using_eloss, mc_smear;
+    double p_spec, th_spec, phi, mass, coulomb, beam_energy;
+    double offset[5]; /* x, y, z (cm), xptar, yptar (rad) */
+} or_prep;
+typedef struct {
+    /* E, p: after radiation (orig); E_vertex: before (vertex), NULL = E */
+    const double *x, *y, *z, *E, *p, *E_vertex, *theta, *delta, *xptar, *yptar, *dang_yp, *dang_xp;
+} or_vertex_in;
+typedef struct {
+    double *dpp, *x, *y, *z, *dxdz, *dydz, *fry, *eloss, *teff, *dang1, *dang2;
+} or_prep_out;
+#define OR_TARGET_TAG 0x54475453u   /* philox stream of the target step */
+double or_target_X0(double A, double Z);
+int or_trip_thru_target(or_rng *g, const or_target *T, int narm, int spec, double zpos,
+                        double energy, double theta, double mass, int typeflag, double *eloss,
+                        double *radlen);
+void or_target_musc(or_rng *g, double p, double beta, double teff, double dangles[2]);
+/* rng_kind 0 tape, 1 philox (key seed, event event0 + i, tag OR_TARGET_TAG).
+ * Returns 0, -1 for an unknown spectrometer code, -2 on tape overflow. */
+int or_target_prep(const or_target *T, const or_prep *P, int64_t n, const or_vertex_in *in,
+                   or_prep_out *out, int rng_kind, const double *tape, int64_t stride,
+                   uint64_t seed, uint64_t event0, int nthreads, int32_t *used);
 
 #ifdef __cplusplus
 }

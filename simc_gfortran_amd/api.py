@@ -51,6 +51,7 @@ EXPORTS = [
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
+    "simc_target_prep", "simc_target_prep_host",
     "simc_last_error",
     "simc_version",
 ]
@@ -85,6 +86,34 @@ class Rng(C.Structure):
 class Setting(C.Structure):
     _fields_ = [("arm", C.c_int32), ("reserved", C.c_int32), ("s", Settings), ("seed", C.c_uint64),
                 ("event0", C.c_uint64), ("offset", C.c_int64), ("count", C.c_int64)]
+
+
+class Target(C.Structure):
+    """simc_target: the target of trip_thru_target (X0 <= 0: init.f's value)."""
+    _fields_ = [("A", C.c_double), ("Z", C.c_double), ("rho", C.c_double), ("X0", C.c_double),
+                ("length", C.c_double), ("angle", C.c_double), ("zoffset", C.c_double),
+                ("can", C.c_int32), ("reserved", C.c_int32)]
+
+
+class PrepSettings(C.Structure):
+    """simc_prep_settings: one particle's preparation (narm 1 beam, 2 electron, 3 hadron)."""
+    _fields_ = [("narm", C.c_int32), ("spectrometer", C.c_int32), ("using_eloss", C.c_int32),
+                ("mc_smear", C.c_int32), ("p_spec", C.c_double), ("th_spec", C.c_double),
+                ("phi", C.c_double), ("mass", C.c_double), ("coulomb", C.c_double),
+                ("beam_energy", C.c_double), ("offset", C.c_double * 5)]
+
+
+VERTEX_FIELDS = ("x", "y", "z", "E", "p", "E_vertex", "theta", "delta", "xptar", "yptar", "dang_yp",
+                 "dang_xp")
+PREP_OUT_FIELDS = ("dpp", "x", "y", "z", "dxdz", "dydz", "fry", "eloss", "teff", "dang1", "dang2", "used")
+
+
+class VertexIn(C.Structure):
+    _fields_ = [(n, _vp) for n in VERTEX_FIELDS]
+
+
+class PrepOut(C.Structure):
+    _fields_ = [(n, _vp) for n in PREP_OUT_FIELDS]
 
 
 class SimcError(RuntimeError):
@@ -124,6 +153,10 @@ def library(path: str = LIB_PATH):
     L.simc_adjoint_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn), _vp, C.c_int64,
                                    C.POINTER(Rng), _vp, C.POINTER(C.c_double), _vp, _vp, _vp]
     L.simc_recon_coefficients.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.simc_target_prep.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
+                                   C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng), _vp]
+    L.simc_target_prep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
+                                        C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng)]
     L.simc_batch_destroy.argtypes = [_vp]
     L.simc_batch_destroy.restype = None
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
@@ -297,6 +330,52 @@ class Spectrometers:
         _check(library().simc_adjoint_run(self._h, arm, C.byref(S), C.byref(ein), _ptr(mask), n, C.byref(R),
                                           _ptr(coef), off, _ptr(grad_coef), _ptr(out9), C.c_void_p(st)))
         return grad_coef, out9
+
+    # -- target step ---------------------------------------------------------------------
+    def target_prep(self, target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=None, stride=0,
+                    out=None, stream=None):
+        """Target step (simc_target_prep): vertex quantities (dict of float64
+        device tensors, fields of simc_vertex_in; absent = NULL) to the arm's
+        inputs.  target: Target; prep: PrepSettings.  Returns a dict of device
+        tensors dpp, x, y, z, dxdz, dydz, fry, eloss, teff, dang1, dang2 and
+        used (int32), or fills `out`."""
+        n = vertex["z"].numel()
+        vin = {k: _dev_f64(vertex[k], n, k) for k in VERTEX_FIELDS if vertex.get(k) is not None}
+        dev = vin["z"].device
+        if out is None:
+            out = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in PREP_OUT_FIELDS if k != "used"}
+            out["used"] = torch.empty(n, dtype=torch.int32, device=dev)
+        V = VertexIn(*[_ptr(vin.get(k)) for k in VERTEX_FIELDS])
+        O = PrepOut(*[_ptr(out.get(k)) for k in PREP_OUT_FIELDS])
+        if rng == "tape":
+            if tape is None or stride <= 0:
+                raise ValueError("tape rng needs tape and stride")
+            R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _check(library().simc_target_prep(self._h, C.byref(target), C.byref(prep), C.byref(V), C.byref(O), n,
+                                          C.byref(R), C.c_void_p(st)))
+        return out
+
+    def target_prep_host(self, target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=None, stride=0):
+        """target_prep on numpy arrays (synchronous host-pointer entry point)."""
+        n = len(vertex["z"])
+        vin = {k: np.ascontiguousarray(vertex[k], dtype=np.float64) for k in VERTEX_FIELDS
+               if vertex.get(k) is not None}
+        out = {k: np.zeros(n) for k in PREP_OUT_FIELDS if k != "used"}
+        out["used"] = np.zeros(n, np.int32)
+        hp = lambda a: None if a is None else a.ctypes.data_as(_vp)
+        V = VertexIn(*[hp(vin.get(k)) for k in VERTEX_FIELDS])
+        O = PrepOut(*[hp(out.get(k)) for k in PREP_OUT_FIELDS])
+        if rng == "tape":
+            tape = np.ascontiguousarray(tape, dtype=np.float64)
+            R = Rng(RNG_TAPE, 0, 0, event0, hp(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        _check(library().simc_target_prep_host(self._h, C.byref(target), C.byref(prep), C.byref(V), C.byref(O),
+                                               n, C.byref(R)))
+        return out
 
     def recon(self, arm, x_fp, dx_fp, y_fp, dy_fp, fry=None, stream=None):
         """mc_hms_recon / mc_shms_recon over focal-plane tracks."""

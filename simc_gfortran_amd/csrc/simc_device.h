@@ -58,7 +58,8 @@ constexpr double kMk = 493.677;
 // Philox4x32-10 (Salmon et al., SC'11; Random123 constants).  Uniform k of an
 // event is word k%4 of the block with counter (k/4, event lo, event hi, tag),
 // mapped to (w + 1/2) 2^-32.
-constexpr uint32_t kPhiloxTag = 0x53494d43u;
+constexpr uint32_t kPhiloxTag = 0x53494d43u;  // "SIMC"
+constexpr uint32_t kTargetTag = 0x54475453u;  // "TGTS"
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4])
@@ -152,7 +153,10 @@ __device__ __forceinline__ void bm_sincos2pi(double u, double& sn, double& cs)
     cs = ((q + 1) & 2) ? -c0 : c0;
 }
 
-struct PhiloxStream {
+// TAG is the fourth counter word: streams of different tags are independent
+// (the arms use kPhiloxTag, the target step kTargetTag).
+template <uint32_t TAG>
+struct PhiloxStreamT {
     uint32_t k0, k1, e0, e1;
     uint32_t pos;
     uint32_t w[4];
@@ -175,12 +179,12 @@ struct PhiloxStream {
         pos = word & 0x7fffffffu;
         have = (int)(word >> 31);
         pending = pend;
-        if (pos & 3u) philox4x32_10(pos >> 2, e0, e1, kPhiloxTag, k0, k1, w);
+        if (pos & 3u) philox4x32_10(pos >> 2, e0, e1, TAG, k0, k1, w);
     }
     __device__ __forceinline__ double next()
     {
         const uint32_t j = pos & 3u;
-        if (j == 0) philox4x32_10(pos >> 2, e0, e1, kPhiloxTag, k0, k1, w);
+        if (j == 0) philox4x32_10(pos >> 2, e0, e1, TAG, k0, k1, w);
         ++pos;
         const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
         return ((double)v + 0.5) * 2.3283064365386963e-10;
@@ -212,6 +216,7 @@ struct PhiloxStream {
     }
     __device__ __forceinline__ int32_t used() const { return (int32_t)pos; }
 };
+using PhiloxStream = PhiloxStreamT<kPhiloxTag>;
 
 template <class R>
 __device__ __forceinline__ double polar_gauss(R& g, double nsigmax);

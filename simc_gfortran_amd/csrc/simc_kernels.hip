@@ -22,6 +22,7 @@
 #include "simc_device.h"
 #include "simc_maps.h"
 #include "simc_ops.h"
+#include "simc_target.h"
 #include "cosy_gen.dev.inc"
 
 using namespace simcdev;
@@ -1094,6 +1095,89 @@ __global__ __launch_bounds__(256) void recon_kernel(int64_t n, const double* coe
 }
 
 // ---------------------------------------------------------------------------
+// Target step: one event per lane (simc_target.h has the routines).
+// ---------------------------------------------------------------------------
+struct PrepArgs {
+    PrepDev P;
+    uint64_t seed, event0;
+    uint32_t n, stride_bytes;
+    const double* tape;
+    const double *x, *y, *z, *E, *p, *Ev, *theta, *delta, *xptar, *yptar, *dyp, *dxp;
+    double *dpp, *ox, *oy, *oz, *dxdz, *dydz, *fry, *eloss, *teff, *d1, *d2;
+    int32_t* used;
+};
+
+template <class R, int NARM>
+__global__ __launch_bounds__(256) void target_prep_kernel(PrepArgs a)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    const PrepDev& P = a.P;
+    R g;
+    g.init(a.seed, a.event0 + i, a.tape, i * a.stride_bytes, a.stride_bytes);
+    const double tz = a.z[i];
+    const double zpos = tz - P.zoffset;
+    double eloss, teff, d1 = 0.0, d2 = 0.0;
+    if (NARM == 1) {  // simc.f:1364-1369 with event.f:194-196
+        eloss = trip_beam(g, P, zpos, teff);
+        if (!P.using_eloss) eloss = 0.0;
+        if (P.mc_smear) target_musc(g, a.E[i], 1.0, teff, d1, d2);
+        if (g.overflow) eloss = teff = d1 = d2 = __builtin_nan("");
+        if (a.eloss) a.eloss[i] = eloss;
+        if (a.teff) a.teff[i] = teff;
+        if (a.d1) a.d1[i] = d1;
+        if (a.d2) a.d2[i] = d2;
+        if (a.used) a.used[i] = g.overflow ? -1 : g.used();
+        return;
+    }
+    const double E = a.E[i], p = a.p[i];
+    eloss = trip_scattered(g, P, zpos, a.Ev ? a.Ev[i] : E, a.theta[i], teff);  // event.f:1031-1040
+    if (!P.using_eloss) eloss = 0.0;
+    double delta;
+    if (NARM == 3) {  // simc.f:1379-1396
+        if (P.using_eloss) {
+            const double e = E - eloss;
+            delta = (sqrt(fabs(e * e - P.mass * P.mass)) - P.p_spec) / P.p_spec * 100.0;
+        } else {
+            delta = a.delta[i];
+        }
+        if (P.mc_smear) target_musc(g, p, p / E, teff, d1, d2);
+    } else {  // simc.f:1653-1662
+        delta = 100.0 * (E - eloss - P.coulomb - P.p_spec) / P.p_spec;
+        if (P.mc_smear) target_musc(g, p, 1.0, teff, d1, d2);
+    }
+    const double din1 = a.dyp ? a.dyp[i] : 0.0;
+    const double din2 = a.dxp ? a.dxp[i] : 0.0;
+    const double yptar = a.yptar[i] + d1 + din1;
+    const double xptar = a.xptar[i] + d2 + din2 * P.cos_th;
+    // TRANSPORT coordinates, offsets, drift to z = 0 (simc.f:1409-1441)
+    const double tx = a.x[i], ty = a.y[i];
+    double x = -ty;
+    double y = -(tx * P.cos_th) - tz * P.sin_th * P.sin_phi;
+    double z = tz * P.cos_th + tx * P.sin_th * P.sin_phi;
+    x = x - P.off[0];
+    y = y - P.off[1];
+    z = z - P.off[2];
+    double dx = xptar - P.off[3];
+    double dy = yptar - P.off[4];
+    x = x - z * dx;
+    y = y - z * dy;
+    if (g.overflow) delta = x = y = dx = dy = eloss = teff = d1 = d2 = __builtin_nan("");
+    if (a.dpp) a.dpp[i] = delta;
+    if (a.ox) a.ox[i] = x;
+    if (a.oy) a.oy[i] = y;
+    if (a.oz) a.oz[i] = 0.0;
+    if (a.dxdz) a.dxdz[i] = dx;
+    if (a.dydz) a.dydz[i] = dy;
+    if (a.fry) a.fry[i] = x;
+    if (a.eloss) a.eloss[i] = eloss;
+    if (a.teff) a.teff[i] = teff;
+    if (a.d1) a.d1[i] = d1;
+    if (a.d2) a.d2[i] = d2;
+    if (a.used) a.used[i] = g.overflow ? -1 : g.used();
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 thread_local std::string g_err;
@@ -2053,6 +2137,253 @@ int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const 
         hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     HIPCHK(hipGetLastError());
+    return SIMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Target step (host side)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Setting-constant terms of enerloss_new for one material (enerloss_new.f:23-31,51-55)
+MaterialDev material(double dens, double zeff, double aeff)
+{
+    const double me = 0.51099906;
+    const double I = zeff == 1.0 ? 21.8e-06 : (16.0 * std::pow(zeff, 0.9)) * 1.0e-06;
+    const double hnup = 28.816e-06 * std::sqrt(dens * zeff / aeff);
+    MaterialDev m;
+    m.rho = dens;
+    m.k_mp = 0.1536e-03 * zeff / aeff;
+    m.l0 = std::log(me / (I * I)) + 1.063;
+    m.k_log = 0.1536 * zeff / aeff;
+    m.k_chi = 0.307075 / 2.0 * zeff / aeff;
+    m.co = std::log(hnup) - std::log(I) + 0.5;
+    m.co27 = std::fabs(m.co / 27.0);
+    return m;
+}
+
+// Radiation length of the target nucleus, g/cm^2 (init.f:18-42)
+double target_X0(double A, double Z)
+{
+    const double alpha = 1.0 / 137.0359895;  // constants.inc:44
+    double L1 = std::log(184.15) - std::log(Z) / 3.0;
+    double L2 = std::log(1194.) - 2.0 * std::log(Z) / 3.0;
+    if (Z == 1.0) {
+        L1 = 5.31;
+        L2 = 6.144;
+    }
+    const double za2 = (Z * alpha) * (Z * alpha);
+    const double fc = za2 * (1.202 + za2 * (-1.0369 + za2 * 1.008 / (za2 + 1)));
+    const long nA = std::lround(A);
+    if (nA == 1) return 61.28;
+    if (nA == 2) return 122.4;
+    if (nA == 4) return 94.32;
+    return 716.405 * A / Z / (Z * (L1 - fc) + L2);
+}
+
+int make_prep(const simc_target* t, const simc_prep_settings* s, const simc_vertex_in* in, PrepDev& P)
+{
+    if (s->narm < 1 || s->narm > 3) return fail(SIMC_E_ARG, "narm must be 1 (beam), 2 (electron) or 3 (hadron)");
+    if (!(t->rho > 0.0) || !(t->length > 0.0) || !(t->A > 0.0) || !(t->Z > 0.0))
+        return fail(SIMC_E_ARG, "target needs positive A, Z, rho and length");
+    if (!(s->mass > 0.0)) return fail(SIMC_E_ARG, "particle mass must be positive");
+    if (!in->z) return fail(SIMC_E_ARG, "the vertex z array is required");
+    if (s->narm == 1) {
+        if (s->mc_smear && !in->E) return fail(SIMC_E_ARG, "beam multiple scattering needs E (= Ein)");
+        if (!(s->beam_energy > s->mass)) return fail(SIMC_E_ARG, "beam_energy must exceed the mass");
+    } else {
+        if (!in->x || !in->y || !in->E || !in->p || !in->theta || !in->xptar || !in->yptar)
+            return fail(SIMC_E_ARG, "vertex arrays x, y, z, E, p, theta, xptar, yptar are required");
+        if (s->narm == 3 && !s->using_eloss && !in->delta)
+            return fail(SIMC_E_ARG, "a hadron arm without using_eloss reads the vertex delta");
+        if (!(s->p_spec > 0.0)) return fail(SIMC_E_ARG, "p_spec must be positive");
+    }
+    std::memset(&P, 0, sizeof P);
+    P.mat[MAT_TARGET] = material(t->rho, t->Z, t->A);
+    P.mat[MAT_AL] = material(2.70, 13., 26.98);       // target.inc:6-8
+    P.mat[MAT_AIR] = material(0.00121, 7.2, 14.4);    // target.inc:24-26
+    P.mat[MAT_KEVLAR] = material(0.74, 2.67, 4.67);   // target.inc:18-20
+    P.mat[MAT_MYLAR] = material(1.39, 4.545, 8.735);  // target.inc:12-14
+    const double X0 = t->X0 > 0.0 ? t->X0 : target_X0(t->A, t->Z);
+    P.x0_cm[MAT_TARGET] = X0 / t->rho;                // init.f:43
+    P.x0_cm[MAT_AL] = 24.01 / 2.70;
+    P.x0_cm[MAT_AIR] = 36.66 / 0.00121;
+    P.x0_cm[MAT_KEVLAR] = 55.2 / 0.74;
+    P.x0_cm[MAT_MYLAR] = 39.95 / 1.39;
+    P.length = t->length;
+    P.angle = t->angle;
+    P.cos_angle = std::fabs(std::cos(t->angle));
+    P.zoffset = t->zoffset;
+    P.can = t->can;
+    P.liquid = t->Z < 2.4 ? 1 : 0;
+    P.narm = s->narm;
+    P.using_eloss = s->using_eloss != 0;
+    P.mc_smear = s->mc_smear != 0;
+    const double inch_cm = 2.54;
+    if (s->narm == 1) {  // target.f:36-45
+        if (P.liquid) {
+            if (t->can == 1) P.s_al = 0.0028 * inch_cm;
+            else if (t->can == 2) P.s_al = 0.0050 * inch_cm;
+            else if (t->can == 3) P.s_al = 0.013;
+        }
+    } else {  // target.f:71-101, :184-217
+        P.angle_sign = -1.0;
+        switch (s->spectrometer) {
+        case 1:
+            P.s_al = 0.016 * inch_cm; P.s_air = 15; P.s_kevlar = 0.015 * inch_cm; P.s_mylar = 0.005 * inch_cm;
+            P.angle_sign = 1.0;
+            break;
+        case 2:
+            P.s_al = 0.008 * inch_cm; P.s_air = 15; P.s_kevlar = 0.005 * inch_cm; P.s_mylar = 0.003 * inch_cm;
+            break;
+        case 3:
+        case 4:
+            P.s_al = 0.013 * inch_cm; P.s_air = 15; P.s_kevlar = 0. * inch_cm; P.s_mylar = 0.010 * inch_cm;
+            break;
+        case 5:
+        case 6:
+            P.s_al = (0.02 + 0.01) * inch_cm; P.s_air = 57.27; P.s_kevlar = 0.0; P.s_mylar = 0.0;
+            break;
+        default:
+            return fail(SIMC_E_ARG, "spectrometer must be 1 (HMS), 2 (SOS), 3/4 (HRS) or 5/6 (SHMS)");
+        }
+    }
+    P.p_spec = s->p_spec;
+    P.cos_th = std::cos(s->th_spec);
+    P.sin_th = std::sin(s->th_spec);
+    P.sin_phi = std::sin(s->phi);
+    P.mass = s->mass;
+    P.coulomb = s->coulomb;
+    P.beam_energy = s->beam_energy;
+    for (int k = 0; k < 5; ++k) P.off[k] = s->offset[k];
+    P.ln10 = std::log(10.);
+    return SIMC_OK;
+}
+
+void launch_prep(bool tape, int narm, const PrepArgs& a, hipStream_t st)
+{
+    using Ph = PhiloxStreamT<kTargetTag>;
+    const dim3 G((a.n + 255) / 256), B(256);
+    if (tape) {
+        if (narm == 1) hipLaunchKernelGGL((target_prep_kernel<TapeStream, 1>), G, B, 0, st, a);
+        else if (narm == 2) hipLaunchKernelGGL((target_prep_kernel<TapeStream, 2>), G, B, 0, st, a);
+        else hipLaunchKernelGGL((target_prep_kernel<TapeStream, 3>), G, B, 0, st, a);
+    } else {
+        if (narm == 1) hipLaunchKernelGGL((target_prep_kernel<Ph, 1>), G, B, 0, st, a);
+        else if (narm == 2) hipLaunchKernelGGL((target_prep_kernel<Ph, 2>), G, B, 0, st, a);
+        else hipLaunchKernelGGL((target_prep_kernel<Ph, 3>), G, B, 0, st, a);
+    }
+}
+
+}  // namespace
+
+int simc_target_prep(simc_ctx* ctx, const simc_target* t, const simc_prep_settings* s, const simc_vertex_in* in,
+                     simc_prep_out* out, int64_t n, const simc_rng* rng, void* stream)
+{
+    if (!ctx || !t || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_target_prep: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE)
+        return fail(SIMC_E_ARG, "unknown rng kind");
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0 || rng->stride > (1 << 24)))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a stride in [1, 2^24]");
+    PrepArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (int rc = make_prep(t, s, in, a.P)) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t stride_bytes = rng->kind == SIMC_RNG_TAPE ? (uint64_t)rng->stride * 8u : 0u;
+    // one launch covers at most 2^30 events, and a tape slot offset must fit 32 bits
+    int64_t per = (int64_t)1 << 30;
+    if (stride_bytes) per = std::min<int64_t>(per, (int64_t)(0xffffffffull / stride_bytes));
+    a.seed = rng->seed;
+    a.stride_bytes = (uint32_t)stride_bytes;
+    for (int64_t at = 0; at < n; at += per) {
+        const int64_t m = std::min(per, n - at);
+        auto sh = [&](auto* ptr) { return ptr ? ptr + at : ptr; };
+        a.n = (uint32_t)m;
+        a.event0 = rng->event0 + (uint64_t)at;
+        a.tape = rng->kind == SIMC_RNG_TAPE ? rng->tape + at * rng->stride : nullptr;
+        a.x = sh(in->x); a.y = sh(in->y); a.z = sh(in->z); a.E = sh(in->E); a.p = sh(in->p);
+        a.Ev = sh(in->E_vertex); a.theta = sh(in->theta); a.delta = sh(in->delta);
+        a.xptar = sh(in->xptar); a.yptar = sh(in->yptar); a.dyp = sh(in->dang_yp); a.dxp = sh(in->dang_xp);
+        a.dpp = sh(out->dpp); a.ox = sh(out->x); a.oy = sh(out->y); a.oz = sh(out->z);
+        a.dxdz = sh(out->dxdz); a.dydz = sh(out->dydz); a.fry = sh(out->fry); a.eloss = sh(out->eloss);
+        a.teff = sh(out->teff); a.d1 = sh(out->dang1); a.d2 = sh(out->dang2); a.used = sh(out->used);
+        launch_prep(rng->kind == SIMC_RNG_TAPE, s->narm, a, st);
+        HIPCHK(hipGetLastError());
+    }
+    return SIMC_OK;
+}
+
+int simc_target_prep_host(simc_ctx* ctx, const simc_target* t, const simc_prep_settings* s,
+                          const simc_vertex_in* in, simc_prep_out* out, int64_t n, const simc_rng* rng)
+{
+    if (!ctx || !t || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_target_prep_host: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_target_prep_host: at most 2^31 events per call");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0 || rng->stride > (1 << 24)))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a stride in [1, 2^24]");
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    const double* hin[12] = {in->x, in->y, in->z, in->E, in->p, in->E_vertex, in->theta, in->delta,
+                             in->xptar, in->yptar, in->dang_yp, in->dang_xp};
+    double* hout[11] = {out->dpp, out->x, out->y, out->z, out->dxdz, out->dydz, out->fry,
+                        out->eloss, out->teff, out->dang1, out->dang2};
+    const size_t nd = (size_t)n * 8;
+    const size_t tape_bytes = rng->kind == SIMC_RNG_TAPE ? (size_t)n * (size_t)rng->stride * 8 : 0;
+    auto rounded = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t need = 23 * rounded(nd) + rounded(4 * (size_t)n) + rounded(tape_bytes);
+    if (ctx->stage_bytes < need) {
+        if (ctx->d_stage) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(ctx->d_stage));
+            ctx->d_stage = nullptr;
+            ctx->stage_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&ctx->d_stage, need));
+        ctx->stage_bytes = need;
+    }
+    char* base = (char*)ctx->d_stage;
+    size_t at = 0;
+    auto carve = [&](size_t bytes) {
+        char* p = base + at;
+        at += rounded(bytes);
+        return p;
+    };
+    const double* din[12];
+    for (int k = 0; k < 12; ++k) {
+        din[k] = nullptr;
+        if (hin[k]) {
+            double* d = (double*)carve(nd);
+            HIPCHK(hipMemcpy(d, hin[k], nd, hipMemcpyHostToDevice));
+            din[k] = d;
+        }
+    }
+    double* dout[11];
+    for (int k = 0; k < 11; ++k) dout[k] = hout[k] ? (double*)carve(nd) : nullptr;
+    int32_t* d_used = (int32_t*)carve(4 * (size_t)n);  // always staged: tape overflow check
+    simc_rng drng = *rng;
+    if (tape_bytes) {
+        double* d = (double*)carve(tape_bytes);
+        HIPCHK(hipMemcpy(d, rng->tape, tape_bytes, hipMemcpyHostToDevice));
+        drng.tape = d;
+    }
+    if (at > need) return fail(SIMC_E_INTERNAL, "simc_target_prep_host: staging layout exceeds its buffer");
+    simc_vertex_in vin = {din[0], din[1], din[2], din[3], din[4], din[5],
+                          din[6], din[7], din[8], din[9], din[10], din[11]};
+    simc_prep_out pout = {dout[0], dout[1], dout[2], dout[3], dout[4], dout[5], dout[6],
+                          dout[7], dout[8], dout[9], dout[10], d_used};
+    int rc = simc_target_prep(ctx, t, s, &vin, &pout, n, &drng, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    for (int k = 0; k < 11; ++k)
+        if (hout[k]) HIPCHK(hipMemcpy(hout[k], dout[k], nd, hipMemcpyDeviceToHost));
+    std::vector<int32_t> used((size_t)n);
+    HIPCHK(hipMemcpy(used.data(), d_used, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    if (out->used) std::memcpy(out->used, used.data(), 4 * (size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        if (used[(size_t)i] < 0) return fail(SIMC_E_TAPE, "event " + std::to_string(i) + " ran out of its tape slot");
     return SIMC_OK;
 }
 

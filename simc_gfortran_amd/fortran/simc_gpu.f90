@@ -51,7 +51,58 @@ module simc_gpu
     integer(c_int64_t) :: seed = 0, event0 = 0, offset = 0, count = 0
   end type
 
+  ! target step (simc_target_prep): the target, one particle's preparation,
+  ! the vertex arrays and the outputs (the arm's inputs plus loss, radiation
+  ! lengths and scattering angles)
+  integer(c_int32_t), parameter :: SIMC_TARGET_BEAM = 1, SIMC_TARGET_ELECTRON = 2, SIMC_TARGET_HADRON = 3
+
+  type, bind(C) :: simc_target
+    real(c_double)     :: A = 1, Z = 1, rho = 0.0723d0, X0 = 0, length = 10, angle = 0, zoffset = 0
+    integer(c_int32_t) :: can = 3, reserved = 0
+  end type
+
+  type, bind(C) :: simc_prep_settings
+    integer(c_int32_t) :: narm = SIMC_TARGET_HADRON, spectrometer = SIMC_ARM_HMS, using_eloss = 1, mc_smear = 1
+    real(c_double)     :: p_spec = 0, th_spec = 0, phi = 0, mass = 0, coulomb = 0, beam_energy = 0
+    real(c_double)     :: offset(5) = 0
+  end type
+
+  type, bind(C) :: simc_vertex_in
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr, E = c_null_ptr, p = c_null_ptr
+    type(c_ptr) :: E_vertex = c_null_ptr, theta = c_null_ptr, delta = c_null_ptr, xptar = c_null_ptr
+    type(c_ptr) :: yptar = c_null_ptr, dang_yp = c_null_ptr, dang_xp = c_null_ptr
+  end type
+
+  type, bind(C) :: simc_prep_out
+    type(c_ptr) :: dpp = c_null_ptr, x = c_null_ptr, y = c_null_ptr, z = c_null_ptr, dxdz = c_null_ptr
+    type(c_ptr) :: dydz = c_null_ptr, fry = c_null_ptr, eloss = c_null_ptr, teff = c_null_ptr
+    type(c_ptr) :: dang1 = c_null_ptr, dang2 = c_null_ptr, used = c_null_ptr
+  end type
+
   interface
+    integer(c_int) function simc_target_prep(ctx, t, s, vin, pout, n, rng, stream) bind(C, name="simc_target_prep")
+      import :: c_int, c_int64_t, c_ptr, simc_target, simc_prep_settings, simc_vertex_in, simc_prep_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_target), intent(in) :: t
+      type(simc_prep_settings), intent(in) :: s
+      type(simc_vertex_in), intent(in) :: vin
+      type(simc_prep_out), intent(in) :: pout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_target_prep_host(ctx, t, s, vin, pout, n, rng) &
+        bind(C, name="simc_target_prep_host")
+      import :: c_int, c_int64_t, c_ptr, simc_target, simc_prep_settings, simc_vertex_in, simc_prep_out, simc_rng
+      type(c_ptr), value :: ctx
+      type(simc_target), intent(in) :: t
+      type(simc_prep_settings), intent(in) :: s
+      type(simc_vertex_in), intent(in) :: vin
+      type(simc_prep_out), intent(in) :: pout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
     integer(c_int) function simc_ctx_create(map_dir, ctx) bind(C, name="simc_ctx_create")
       import :: c_int, c_char, c_ptr
       character(kind=c_char), dimension(*), intent(in) :: map_dir

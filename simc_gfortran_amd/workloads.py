@@ -110,6 +110,39 @@ def generate_device(w: Workload, n: int, seed: int, device, out: dict | None = N
     return out
 
 
+# The C0 target: liquid hydrogen in the 2017 10 cm cell (init.f:33 X0 value)
+LH2_CELL = dict(A=1.0, Z=1.0, rho=0.0723, X0=61.28, length=10.0, angle=0.0, zoffset=0.0, can=3)
+
+
+def physics_theta(th0: float, phi0: float, dxdz, dydz):
+    """Angle from the beam line of a track with spectrometer angles (dx/dz,
+    dy/dz) (physics_angles, event.f:1572-1620, for phi0 = pi/2 or 3 pi/2)."""
+    r = np.sqrt(1.0 + dxdz ** 2 + dydz ** 2)
+    return np.arccos((math.cos(th0) - dydz * math.sin(th0) * math.sin(phi0)) / r)
+
+
+def generate_vertex(w: Workload, n: int | None = None, seed: int | None = None) -> dict:
+    """Target-frame vertices for the target step (the caller's side of
+    simc.f:1376-1441): position x, y ~ N(0, sigma_xy), z ~ U(z); delta,
+    xptar, yptar uniform as in generate(); p = p_spec (1 + delta/100),
+    E = sqrt(p^2 + m^2) (no radiation: E_vertex = E), theta from
+    physics_theta.  float64 numpy arrays."""
+    n = w.n if n is None else n
+    rng = np.random.default_rng(w.seed if seed is None else seed)
+    xt = rng.normal(0.0, w.sigma_xy, n)
+    yt = rng.normal(0.0, w.sigma_xy, n)
+    zt = rng.uniform(w.z[0], w.z[1], n)
+    delta = rng.uniform(w.dpp[0], w.dpp[1], n)
+    xptar = rng.uniform(w.dxdz[0], w.dxdz[1], n)
+    yptar = rng.uniform(w.dydz[0], w.dydz[1], n)
+    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    p = w.p_spec * (1.0 + delta / 100.0)
+    E = np.sqrt(p * p + w.mass * w.mass)
+    theta = physics_theta(math.radians(w.theta_deg), phi, xptar, yptar)
+    return {"x": xt, "y": yt, "z": zt, "E": E, "p": p, "theta": theta, "delta": delta,
+            "xptar": xptar, "yptar": yptar}
+
+
 def mixed_settings(n_settings: int = 64, seed: int = 2026, lo: float = 1e4, hi: float = 1e7,
                    total: float | None = 1.5e8):
     """The batched configuration: n_settings kinematic settings alternating

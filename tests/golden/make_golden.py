@@ -6,7 +6,10 @@ oracle/_ref/libsimcref.so; this script drives it over small event samples on
 per-event uniform tapes and stores inputs, tape recipe and every output.  The
 tapes are regenerated from their seed by the tests (sha256 recorded here).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # arm goldens (*.npz)
+    python tests/golden/make_golden.py target   # target step (target/*.npz):
+        the reference's trip_thru_target and target_musc (target.f) over
+        vertices on per-event tapes
 """
 import hashlib
 import os
@@ -62,5 +65,62 @@ def main():
         print(name, "accepted", int(out["ok"].sum()), "of", N)
 
 
+# Target step (trip_thru_target + target_musc of the reference's target.f):
+# name, narm, spectrometer, mass, target (A, Z, rho, length, angle, zoffset,
+# can), energy range (MeV), angle range (rad)
+ME_ = 0.51099906
+MP_ = 938.27231
+TARGET_CASES = [
+    ("hadron_hms_lh2_cryo2017", 3, 1, MP_, (1.0, 1.0, 0.0723, 10.0, 0.0, 0.0, 3), (1500., 5000.), (0.2, 0.9)),
+    ("hadron_hms_lh2_beercan", 3, 1, MP_, (1.0, 1.0, 0.0723, 4.0, 0.0, 0.0, 1), (1500., 5000.), (0.2, 0.9)),
+    ("electron_shms_lh2_pudding", 2, 5, ME_, (1.0, 1.0, 0.0723, 4.0, 0.0, 0.0, 2), (2000., 9000.), (0.1, 0.6)),
+    ("electron_shms_ld2_cryo2017", 2, 5, ME_, (2.0, 1.0, 0.167, 10.0, 0.0, 0.5, 3), (2000., 9000.), (0.1, 0.6)),
+    ("electron_sos_lh2_beercan", 2, 2, ME_, (1.0, 1.0, 0.0723, 4.0, 0.0, 0.0, 1), (500., 1700.), (0.3, 1.2)),
+    ("hadron_hrs_lh2_cryo2017", 3, 3, MP_, (1.0, 1.0, 0.0723, 10.0, 0.0, 0.0, 3), (1500., 5000.), (0.2, 0.9)),
+    ("electron_hrsl_lh2_pudding", 2, 4, ME_, (1.0, 1.0, 0.0723, 4.0, 0.0, 0.0, 2), (2000., 9000.), (0.1, 0.6)),
+    ("hadron_shms_carbon_tilted", 3, 5, MP_, (12.0, 6.0, 2.26, 0.3, 0.35, 0.0, 0), (1500., 5000.), (0.2, 0.9)),
+    ("beam_lh2_cryo2017", 1, 1, ME_, (1.0, 1.0, 0.0723, 10.0, 0.0, 0.0, 3), (10600., 10600.), (0.0, 0.0)),
+    ("beam_gold_foil", 1, 1, ME_, (197.0, 79.0, 19.32, 0.05, 0.0, 0.0, 0), (6400., 6400.), (0.0, 0.0)),
+]
+TARGET_N = 1000
+TARGET_STRIDE = 64
+
+
+def target_tape_for(i):
+    return W.uniform_tape(TARGET_N, TARGET_STRIDE, 700 + i)
+
+
+def main_target():
+    out_dir = os.path.join(HERE, "target")
+    os.makedirs(out_dir, exist_ok=True)
+    for i, (name, narm, spec, mass, tg, erange, trange) in enumerate(TARGET_CASES):
+        A, Z, rho, length, angle, zoffset, can = tg
+        T = O.make_target(A=A, Z=Z, rho=rho, length=length, angle=angle, zoffset=zoffset, can=can)
+        rng = np.random.default_rng(4000 + i)
+        zpos = rng.uniform(-0.5, 0.5, TARGET_N) * length
+        energy = rng.uniform(*erange, TARGET_N)
+        theta = rng.uniform(*trange, TARGET_N)
+        e_musc = rng.uniform(*erange, TARGET_N)  # energy after radiation (orig)
+        # the beam's scattering uses Ein itself as the momentum (simc.f:1365)
+        p_musc = e_musc.copy() if narm == 1 else np.sqrt(e_musc ** 2 - mass ** 2)
+        beta = p_musc / e_musc if narm == 3 else np.ones(TARGET_N)
+        tape = target_tape_for(i)
+        r = O.ref_target_step(T, narm, spec, mass, zpos, energy, theta, tape, TARGET_STRIDE, p_musc=p_musc,
+                              beta_musc=beta)
+        assert (r["used"] >= 0).all(), name
+        np.savez_compressed(
+            os.path.join(out_dir, name + ".npz"),
+            narm=narm, spectrometer=spec, mass=mass,
+            target=np.array([A, Z, rho, T.X0, length, angle, zoffset]), can=can,
+            tape_seed=700 + i, stride=TARGET_STRIDE,
+            tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(),
+            zpos=zpos, energy=energy, theta=theta, e_musc=e_musc, p_musc=p_musc,
+            **{k: r[k] for k in ("eloss", "teff", "dang1", "dang2", "used")})
+        print(name, "mean eloss", float(r["eloss"].mean()), "mean teff", float(r["teff"].mean()))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["target"]:
+        main_target()
+    else:
+        main()
