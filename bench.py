@@ -14,6 +14,7 @@ a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import platform
@@ -259,6 +260,45 @@ def bench_adjoint(ctx, n=10_000_000, reps=3):
             "data": "synthetic HMS workload, Philox streams replayed, mask = ok of the forward run"}
 
 
+def bench_target(ctx, n=10_000_000, reps=5):
+    """The target step before the HMS arm (1e7 protons from the 10 cm LH2
+    cell, the HMS workload's vertex distributions): the preparation alone
+    (vertex -> arm inputs, trip_thru_target + target_musc sampled per event)
+    and the chain preparation + transport + reconstruction."""
+    import torch
+    from simc_gfortran_amd import api, workloads as W
+    dev = torch.device("cuda:0")
+    wl = W.HMS_C0
+    v = W.generate_vertex(wl, n, seed=wl.seed)
+    dv = {k: torch.from_numpy(a).to(dev) for k, a in v.items()}
+    del v
+    T = api.Target(**{k: W.LH2_CELL[k] for k in ("A", "Z", "rho", "X0", "length", "angle", "zoffset",
+                                                   "can")}, reserved=0)
+    P = api.PrepSettings(3, 1, 1, 1, wl.p_spec, math.radians(wl.theta_deg), 1.5 * math.pi, wl.mass, 0.0, 0.0,
+                         (api.C.c_double * 5)())
+    prep = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("dpp", "x", "y", "z", "dxdz", "dydz",
+                                                                          "fry")}
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True)
+    out = None
+    tp, tc = [], []
+    for r in range(reps + 1):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record()
+        ctx.target_prep(T, P, dv, rng="philox", seed=1001, out=prep)
+        b.record()
+        out = ctx.run(wl.arm, prep, rng="philox", seed=1001, fit="fp64", out=out, **kw)
+        c.record()
+        torch.cuda.synchronize()
+        if r:
+            tp.append(a.elapsed_time(b))
+            tc.append(a.elapsed_time(c))
+    p_ms, c_ms = float(np.mean(tp)), float(np.mean(tc))
+    return {"workload": "hms_proton_3GeV_lh2_cell_1e7", "events": n, "prep_ms": p_ms,
+            "prep_events_per_s": n / (p_ms * 1e-3), "chain_ms": c_ms, "chain_events_per_s": n / (c_ms * 1e-3),
+            "accepted_fraction": float(out["ok"].float().mean()),
+            "data": "synthetic vertices in a 10 cm LH2 cell (can 3), loss and scattering sampled per event"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,6 +427,7 @@ def main():
         torch.cuda.empty_cache()
         line["batched"] = bench_batched(ctx, line, args)
         line["adjoint"] = bench_adjoint(ctx)
+        line["target_step"] = bench_target(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
