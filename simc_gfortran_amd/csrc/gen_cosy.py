@@ -13,7 +13,9 @@ shms/shms_forward.dat, shms/shms_recon.dat) and writes cosy_gen.inc:
       term are non-zero: mid-plane symmetry),
     - coefficients read from a packed table with compile-time offsets, so the
       loads are uniform scalar loads through the constant address space and
-      coefficient VALUES stay a run-time input;
+      coefficient VALUES stay a run-time input; the reads are split into
+      blocks of COEF_BLOCK coefficients whose loads overlap the arithmetic on
+      the previous block (_block_fences);
 * for the adjoint form: each forward class with two tangent directions
   carried along (same operations for the values), and the recon polynomial
   with all coefficients as run-time values, plus the gradient of its sums
@@ -28,9 +30,14 @@ and different coefficient values (a new tune, an optics fit) needs no rebuild.
 from __future__ import annotations
 
 import os
+import re
 import sys
 
 NVAR = 5
+# coefficients per scalar-load block of the forward and reconstruction
+# polynomials (see _block_fences); 0 = no blocking
+COEF_BLOCK = 32
+coef_block = COEF_BLOCK
 
 
 def _read_records(path):
@@ -162,10 +169,60 @@ def emit_poly(fname, terms, nout, inputs, lines):
     lines.append("{")
     for k in range(nout):
         lines.append(f"  o{k} = 0.0;")
-    lines.extend(body)
+    lines.extend(_block_fences(body, coef_block))
     lines.append("}")
     lines.append("")
     return pack, counter[0]
+
+
+_CREF = re.compile(r"\bc\[(\d+)\]")
+
+
+def _block_fences(body, block):
+    """Split the coefficient reads of a generated body into blocks of `block`
+    consecutive coefficients, read through per-block pointers cb<j>.  Before
+    the arithmetic of block j, one empty asm statement takes every
+    coefficient of block j as an SGPR operand and yields the zero offset of
+    pointer cb<j+1>:
+      * all of block j has arrived there (one wait), and nothing later has
+        been issued yet;
+      * the loads of block j + 1 issue right after it and complete while
+        block j is consumed.
+    Scalar loads complete out of order, so a wait for one outstanding load
+    waits for all of them; without this ordering the compiler issues a
+    block's loads just before their use and waits for them at once.
+    block = 0 leaves the body as is."""
+    if not block:
+        return body
+    refs_of = {}
+    order = []
+    for line in body:
+        for x in _CREF.findall(line):
+            j = int(x) // block
+            if j not in refs_of:
+                order.append(j)
+            refs_of.setdefault(j, []).append(int(x))
+    if order != sorted(order):
+        return body  # blocks not first read in order: leave unfenced
+    out = ["  const cosy_cptr cb0 = c;"]
+    done = set()
+    for line in body:
+        refs = [int(x) for x in _CREF.findall(line)]
+        for idx in refs:
+            j = idx // block
+            if j in done:
+                continue
+            done.add(j)
+            vals = sorted(set(refs_of[j]))
+            ops = " ".join(f"%{k + 1}" for k in range(len(vals)))
+            ins = ", ".join(f'"s"(cb{j}[{i}])' for i in vals)
+            out.append(f"  int z{j + 1};")
+            out.append(f'  asm("s_mov_b32 %0, 0 ; {ops}" : "=s"(z{j + 1}) : {ins});')
+            out.append(f"  const cosy_cptr cb{j + 1} = c + z{j + 1};")
+        if refs:
+            line = _CREF.sub(lambda m: f"cb{int(m.group(1)) // block}[{m.group(1)}]", line)
+        out.append(line)
+    return out
 
 
 def _trie(terms):
@@ -353,7 +410,9 @@ def emit_recon_wgrad(fname, terms, inputs, lines):
     lines.append("")
 
 
-def generate(map_dir, out_path):
+def generate(map_dir, out_path, block=COEF_BLOCK):
+    global coef_block
+    coef_block = block
     specs = [
         ("hms", os.path.join(map_dir, "hms", "forward_cosy.dat"),
          os.path.join(map_dir, "hms", "recon_cosy.dat")),
@@ -363,6 +422,7 @@ def generate(map_dir, out_path):
     dev, host = [], []
     dev.append("// Generated by gen_cosy.py from the COSY map files; do not edit.")
     dev.append("#pragma once")
+    dev.append("typedef const double __attribute__((address_space(4)))* cosy_cptr;")
     host.append("// Generated by gen_cosy.py from the COSY map files; do not edit.")
     host.append("#pragma once")
     host.append("#include <cstdint>")
