@@ -64,6 +64,12 @@ constexpr uint32_t kTargetTag = 0x54475453u;  // "TGTS"
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4])
 {
+    // The key is launch-uniform; without this empty statement the compiler
+    // forms the 20 round keys once per kernel and keeps them in SGPRs across
+    // the whole step loop, where they are spilled to VGPR lanes and read back
+    // (v_readlane) in every block.  Made opaque here, they are recomputed by
+    // scalar adds in each call instead.
+    asm("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -214,6 +220,25 @@ struct PhiloxStreamT {
             if (fabs(g1) <= nsigmax) return g1;
         }
     }
+    // Two consecutive gauss(nsigmax) draws for nsigmax >= 7, where no
+    // Box-Muller value is ever rejected (|g| <= 6.8): with no value pending
+    // that is exactly one pair, cosine value first.
+    __device__ __forceinline__ void gauss_pair(double& a, double& b)
+    {
+        if (have) {
+            a = gauss(99.0);
+            b = gauss(99.0);
+            return;
+        }
+        const double u1 = next();
+        const double u2 = next();
+        const double r = sqrt(-2.0 * bm_log(u1));
+        double sn, cs;
+        bm_sincos2pi(u2, sn, cs);
+        a = r * cs;
+        b = r * sn;
+        pending = b;
+    }
     __device__ __forceinline__ int32_t used() const { return (int32_t)pos; }
 };
 using PhiloxStream = PhiloxStreamT<kPhiloxTag>;
@@ -244,6 +269,11 @@ struct TapeStream {
         return v;
     }
     __device__ __forceinline__ double gauss(double nsigmax) { return polar_gauss(*this, nsigmax); }
+    __device__ __forceinline__ void gauss_pair(double& a, double& b)
+    {
+        a = polar_gauss(*this, 99.0);
+        b = polar_gauss(*this, 99.0);
+    }
     __device__ __forceinline__ int32_t used() const { return (int32_t)(pos / 8u); }
 };
 

@@ -376,9 +376,9 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
             else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
             else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
             double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-            for (int q = 0; q < ng; ++q) {
-                const double v = gauss1(g, 99.0);
-                d0 = d1; d1 = d2; d2 = d3; d3 = v;
+            if (ng) {  // ng = 2 or 4 draws; the last two go to (d2, d3)
+                if (ng == 4) g.gauss_pair(d0, d1);
+                g.gauss_pair(d2, d3);
             }
             switch (code) {
             case OP_PROJECT:
