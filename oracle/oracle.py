@@ -21,6 +21,8 @@ REF_LIB = os.path.join(HERE, "_ref", "libsimcref.so")
 
 H_NCOUNT = 22
 S_NCOUNT = 52
+SO_NCOUNT = 18
+NCOUNT = {1: H_NCOUNT, 2: SO_NCOUNT, 5: S_NCOUNT}
 
 HMS_COUNTERS = [
     "trials", "slit_hor", "slit_vert", "slit_oct", "Q1_in", "Q1_mid", "Q1_out", "Q2_in",
@@ -33,6 +35,9 @@ SHMS_COUNTERS = [
     "Q3_mex", "Q3_out", "D1_in", "D1_flr", "D1_men", "D1_mid1", "D1_mid2", "D1_mid3",
     "D1_mid4", "D1_mid5", "D1_mid6", "D1_mid7", "D1_mex", "D1_out", "BP_in", "BP_out", "hut",
     "dc1", "dc2", "cer1", "cer2", "s1", "s2", "s3", "cal", "stop_id", "coll"]
+SOS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "slit_oct", "quad_in", "quad_mid", "quad_out", "bm01_in",
+    "bm01_out", "bm02_in", "bm02_out", "exit", "hut", "dc1", "dc2", "scin", "cal", "successes"]
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
@@ -146,7 +151,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
         fit_f32=True, wire_sigma=0.0, rng="philox", tape=None, stride=0, seed=0, event0=0,
         nthreads=None):
     """Run the oracle over SoA events (dict of float64 arrays: dpp, x, y, z, dxdz,
-    dydz, fry).  arm: 1 HMS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'."""
+    dydz, fry).  arm: 1 HMS, 2 SOS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'."""
     L = lib()
     n = len(events["dpp"])
     ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
@@ -169,7 +174,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
     rc = L.or_run(maps(), arm, C.byref(S), n, C.byref(ein), C.byref(eout), kind, tp,
                   int(stride), int(seed), int(event0), int(nthreads), _p(cnt, _i64p))
     out["rc"] = rc
-    out["counters"] = cnt[: H_NCOUNT if arm == 1 else S_NCOUNT]
+    out["counters"] = cnt[:NCOUNT[arm]]
     return out
 
 
@@ -192,7 +197,7 @@ def ref_run(arm, events, *, p_spec, m2, tape, stride, ms=True, wcs=True, decay=F
                    _p(out["ok"], _u8p), _p(out["used"], _i32p), _p(tape), int(stride),
                    MAP_DIR.encode(), _p(cnt, _i64p))
     out["rc"] = rc
-    out["counters"] = cnt[: H_NCOUNT if arm == 1 else S_NCOUNT]
+    out["counters"] = cnt[:NCOUNT[arm]]
     return out
 
 

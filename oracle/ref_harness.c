@@ -9,8 +9,8 @@
  *   grnd_()  -- the uniform source, read from a per-event tape (the reference
  *               draws from RANLUX through call_ranlux.f:58-73);
  *   lfit_()  -- the oracle's restatement of cern/lfit.f:11-60.
- * Everything else (mc_hms, mc_hms_hut, mc_hms_recon, mc_shms, mc_shms_hut,
- * mc_shms_recon, transp/transp_init, project, musc, musc_ext, rotate_*,
+ * Everything else (mc_hms, mc_hms_hut, mc_hms_recon, mc_sos, mc_sos_hut,
+ * mc_sos_recon, mc_shms, mc_shms_hut, mc_shms_recon, transp/transp_init, project, musc, musc_ext, rotate_*,
  * gauss1, enerloss_new, loren, the collimator routines) is the reference's
  * own code.
  */
@@ -37,7 +37,13 @@ extern void mc_shms_(double *p_spec, double *th_spec, double *dpp, double *x, do
 
 /* Common blocks (hms/struct_hms.inc:34-42, shms/struct_shms.inc:26-44,
  * simulate.inc:153 /decd/ whose first member is ctau). */
+extern void mc_sos_(double *p_spec, double *th_spec, double *dpp, double *x, double *y, double *z,
+                    double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+                    double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
+                    int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
+                    double *pathlen);
 extern struct { int32_t v[H_NCOUNT]; } spectrometer1_;
+extern struct { int32_t v[SO_NCOUNT]; } spectrometer2_;  /* sos/struct_sos.inc:9-17 */
 extern struct { int32_t v[S_NCOUNT]; } spectrometer_shms_;
 extern struct { double ctau; } decd_;
 
@@ -75,9 +81,10 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return -1;
     if (chdir(mapdir) != 0) return -2;
-    int ncount = arm == 1 ? H_NCOUNT : S_NCOUNT;
+    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : S_NCOUNT;
+    const int32_t *common = arm == 1 ? spectrometer1_.v : arm == 2 ? spectrometer2_.v : spectrometer_shms_.v;
     int32_t before[S_NCOUNT];
-    memcpy(before, arm == 1 ? spectrometer1_.v : spectrometer_shms_.v, ncount * sizeof(int32_t));
+    memcpy(before, common, ncount * sizeof(int32_t));
     decd_.ctau = ctau;
     int rc = 0;
     for (int64_t i = 0; i < n; i++) {
@@ -96,6 +103,9 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
         if (arm == 1)
             mc_hms_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
                     &dcf, &rm, &fry, &okv, &pl, &uc);
+        else if (arm == 2)
+            mc_sos_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
+                    &dcf, &rm, &fry, &okv, &pl);
         else
             mc_shms_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf,
                      &wcf, &dcf, &rm, &fry, &okv, &pl, &spectr, &uc);
@@ -114,7 +124,7 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
         if (used) used[i] = (int32_t)g_used;
         if (g_overflow) rc = OR_E_TAPE;
     }
-    const int32_t *after = arm == 1 ? spectrometer1_.v : spectrometer_shms_.v;
+    const int32_t *after = common;
     for (int c = 0; c < ncount; c++) counters[c] += (int64_t)(after[c] - before[c]);
     if (chdir(cwd) != 0) return -3;
     return rc;

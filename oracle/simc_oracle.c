@@ -261,6 +261,21 @@ or_maps *or_maps_load(const char *dir)
         or_maps_free(M);
         return NULL;
     }
+    /* SOS maps are optional (sos/mc_sos.f:160-165 reads them on first use) */
+    snprintf(p, sizeof p, "%s/sos/forward_cosy.dat", dir);
+    FILE *probe = fopen(p, "r");
+    if (probe) {
+        fclose(probe);
+        M->sos_fwd = (or_fwd_map *)calloc(1, sizeof(or_fwd_map));
+        M->sos_rec = (or_recon_map *)calloc(1, sizeof(or_recon_map));
+        rc |= or_read_forward(p, M->sos_fwd);
+        snprintf(p, sizeof p, "%s/sos/recon_cosy.dat", dir);
+        rc |= or_read_recon(p, M->sos_rec);
+        if (rc || M->sos_fwd->n_classes != 10) {
+            or_maps_free(M);
+            return NULL;
+        }
+    }
     return M;
 }
 
@@ -271,6 +286,8 @@ void or_maps_free(or_maps *M)
     free(M->shms_fwd);
     free(M->hms_rec);
     free(M->shms_rec);
+    free(M->sos_fwd);
+    free(M->sos_rec);
     free(M);
 }
 
@@ -1452,6 +1469,316 @@ void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
 }
 
 /* ========================================================================= */
+/* SOS                                                                        */
+/* ========================================================================= */
+
+/* mc_sos_recon, sos/mc_sos_recon.f:113-137: every hut coordinate, not only
+ * fry, is kept away from 0 (|h| <= 1e-30 -> 1e-30). */
+static void recon_sos(const or_recon_map *m, const or_track *t, double fry, double *delta_p,
+                      double *delta_t, double *delta_phi, double *y_tgt)
+{
+    double hut[5], sum[4];
+    hut[0] = t->xs / 100.;
+    hut[1] = t->dxdzs;
+    hut[2] = t->ys / 100.;
+    hut[3] = t->dydzs;
+    hut[4] = fry / 100.;
+    for (int i = 0; i < 5; i++)
+        if (fabs(hut[i]) <= 1.e-30) hut[i] = 1.e-30;
+    or_recon_eval(m, hut, sum);
+    *delta_phi = sum[0];
+    *y_tgt = sum[1] * 100.;
+    *delta_t = sum[2];
+    *delta_p = sum[3] * 100.;
+}
+
+/* mc_sos_hut, sos/mc_sos_hut.f:1-548 (constants :42-197) */
+static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+                   double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                   int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
+                   double zinit, double *pathlen, int *stop_code, int *fit_done)
+{
+    const double pi = 3.141592654, d_r = pi / 180.;
+    const double sfoil_exit_radlen = 53.3, sfoil_exit_thick = 0.020 * 2.54;
+    const double sfoil_exit_zpos = -3.22, sfoil_exit_ang = 0. * d_r;
+    const double sair_radlen = 30420.;
+    const double sdc_radlen = 16700.0, sdc_thick = 0.61775;
+    const double sdc_wire_radlen = 0.35, sdc_wire_thick = 0.0000354;
+    const double sdc_cath_radlen = 28.7, sdc_cath_thick = 0.0005 * 2.54;
+    const double sscin_radlen = 42.4;
+    const double scer_entr_radlen = 8.90, scer_entr_thick = 0.050;
+    const double scer_radlen = 4810.0;
+    const double scer_mir_radlen = 400.0, scer_mir_thick = 2.0;
+    const double scer_exit_radlen = 8.90, scer_exit_thick = 0.050;
+    const int sdc_nr_cham = 2, sdc_nr_plan = 6;
+    const double sdc_1_zpos = 6.25, sdc_2_zpos = 55.77;
+    const double sdc_del_plane = sdc_thick + sdc_wire_thick + sdc_cath_thick;
+    const double sdc_1_left = 24.0, sdc_1_right = -24.0, sdc_1y_offset = -1.822, sdc_1_top = -32.0,
+                 sdc_1_bot = 32.0, sdc_1x_offset = 8.649;
+    const double sdc_2_left = 24.0, sdc_2_right = -24.0, sdc_2y_offset = -1.976, sdc_2_top = -32.0,
+                 sdc_2_bot = 32.0, sdc_2x_offset = -1.532;
+    const double sscin_1y_zpos = 73.61, sscin_1x_zpos = 97.11, sscin_2y_zpos = 249.51,
+                 sscin_2x_zpos = 290.81;
+    const double sscin_1x_thick = 1.040, sscin_1y_thick = 1.098, sscin_2x_thick = 1.040,
+                 sscin_2y_thick = 1.098;
+    const double sscin_1_left = 18.25, sscin_1_right = -18.25, sscin_1x_offset = 2.8;
+    const double sscin_1_top = -31.75, sscin_1_bot = 31.75, sscin_1y_offset = 2.25;
+    const double sscin_2_left = 18.25, sscin_2_right = -18.25, sscin_2x_offset = 4.9;
+    const double sscin_2_top = -56.25, sscin_2_bot = 56.25, sscin_2y_offset = 2.9;
+    const double scer_zentrance = 130.000, scer_zmirror = 155.000, scer_zexit = 160.000;
+    const double scal_4ta_zpos = 346.01;
+    const double nsig_max = 99.0e0;
+    const int scintrig = 3;
+    const double sigma = S->wire_sigma > 0 ? S->wire_sigma : 0.030; /* sdc_sigma, :112-113 */
+
+    double radw, drift, tmpran1, tmpran2, xt;
+    hut_hits h;
+    int scincount;
+
+    double tmpran = or_grnd(g); /* :253-259 */
+    double tmplim = 0.15;
+    if (tmpran < tmplim) *resmult = 2.0;
+    else *resmult = 1.0;
+    for (int i = 0; i < 12; i++) { h.xdc[i] = 0.; h.ydc[i] = 0.; }
+    scincount = 0;
+
+    /* exit foil, :281-287 */
+    xt = t->xs + t->dxdzs * sfoil_exit_zpos;
+    drift = (sfoil_exit_zpos + xt * tan(sfoil_exit_ang)) - zinit;
+    if (drift <= 0.001) drift = 0.001;
+    project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+    radw = sfoil_exit_thick / sfoil_exit_radlen / cos(sfoil_exit_ang);
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    for (int jchamber = 1; jchamber <= sdc_nr_cham; jchamber++) {
+        if (jchamber == 1) { /* :292-296 */
+            drift = (sdc_1_zpos - 0.5 * sdc_nr_plan * sdc_del_plane) -
+                    (sfoil_exit_zpos + xt * tan(sfoil_exit_ang));
+        } else { /* :346-350 */
+            drift = sdc_2_zpos - sdc_1_zpos - sdc_nr_plan * sdc_del_plane;
+        }
+        radw = drift / sair_radlen;
+        project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        int npl_off = (jchamber - 1) * sdc_nr_plan;
+        for (int iplane = 1; iplane <= sdc_nr_plan; iplane++) {
+            radw = sdc_cath_thick / sdc_cath_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = 0.5 * sdc_thick;
+            radw = drift / sdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + sdc_cath_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            radw = sdc_wire_thick / sdc_wire_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (wcs_flag) {
+                tmpran1 = or_gauss1(g, nsig_max);
+                tmpran2 = or_gauss1(g, nsig_max);
+            } else {
+                tmpran1 = 0.;
+                tmpran2 = 0.;
+            }
+            h.xdc[npl_off + iplane - 1] = t->xs + sigma * tmpran1 * *resmult;
+            h.ydc[npl_off + iplane - 1] = t->ys + sigma * tmpran2 * *resmult;
+            if (iplane == 1 || iplane == 3 || iplane == 5) h.xdc[npl_off + iplane - 1] = 0.;
+            else h.ydc[npl_off + iplane - 1] = 0.;
+            drift = 0.5 * sdc_thick;
+            radw = drift / sdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + sdc_wire_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        }
+        if (jchamber == 1) { /* :335-341 */
+            if (t->xs > (sdc_1_bot - sdc_1x_offset) || t->xs < (sdc_1_top - sdc_1x_offset) ||
+                t->ys > (sdc_1_left - sdc_1y_offset) || t->ys < (sdc_1_right - sdc_1y_offset)) {
+                cnt[SO_DC1]++;
+                *stop_code = STOP_SO_DC1;
+                return 0;
+            }
+        } else { /* :385-391 */
+            if (t->xs > (sdc_2_bot - sdc_2x_offset) || t->xs < (sdc_2_top - sdc_2x_offset) ||
+                t->ys > (sdc_2_left - sdc_2y_offset) || t->ys < (sdc_2_right - sdc_2y_offset)) {
+                cnt[SO_DC2]++;
+                *stop_code = STOP_SO_DC2;
+                return 0;
+            }
+        }
+        radw = sdc_cath_thick / sdc_cath_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    }
+
+    for (int jchamber = 1; jchamber <= sdc_nr_cham; jchamber++) { /* :399-407 */
+        int npl_off = (jchamber - 1) * sdc_nr_plan;
+        for (int iplane = 1; iplane <= sdc_nr_plan; iplane++) {
+            double z0 = jchamber == 1 ? sdc_1_zpos : sdc_2_zpos;
+            h.zdc[npl_off + iplane - 1] = z0 + ((double)iplane - 0.5 - 0.5 * sdc_nr_plan) * sdc_del_plane;
+        }
+    }
+    fit_planes(S, &h, x_fp, dx_fp, y_fp, dy_fp); /* :409-414 */
+    *fit_done = 1;
+
+#define SOS_SCIN_IN(L, R, YO, B, T, XO)                                   \
+    (t->ys < ((L) + (YO)) && t->ys > ((R) + (YO)) && t->xs < ((B) + (XO)) && \
+     t->xs > ((T) + (XO)))
+#define SOS_STEP(DRIFT, RADLEN)                                                          \
+    do {                                                                                 \
+        drift = (DRIFT);                                                                 \
+        radw = drift / (RADLEN);                                                         \
+        project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);      \
+        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
+    } while (0)
+
+    SOS_STEP(sscin_1y_zpos - sdc_2_zpos - 0.5 * sdc_nr_plan * sdc_del_plane, sair_radlen); /* :418-428 */
+    if (SOS_SCIN_IN(sscin_1_left, sscin_1_right, sscin_1y_offset, sscin_1_bot, sscin_1_top, sscin_1x_offset))
+        scincount = scincount + 1;
+    radw = sscin_1y_thick / sscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(sscin_1x_zpos - sscin_1y_zpos, sair_radlen); /* :430-439 */
+    if (SOS_SCIN_IN(sscin_1_left, sscin_1_right, sscin_1y_offset, sscin_1_bot, sscin_1_top, sscin_1x_offset))
+        scincount = scincount + 1;
+    radw = sscin_1x_thick / sscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(scer_zentrance - sscin_1x_zpos, sair_radlen); /* :443-448 */
+    radw = scer_entr_thick / scer_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(scer_zmirror - scer_zentrance, scer_radlen); /* :451-456 */
+    radw = scer_mir_thick / scer_mir_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(scer_zexit - scer_zmirror, scer_radlen); /* :459-464 */
+    radw = scer_exit_thick / scer_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(sscin_2y_zpos - scer_zexit, sair_radlen); /* :469-478 */
+    if (SOS_SCIN_IN(sscin_2_left, sscin_2_right, sscin_2y_offset, sscin_2_bot, sscin_2_top, sscin_2x_offset))
+        scincount = scincount + 1;
+    radw = sscin_2y_thick / sscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    SOS_STEP(sscin_2x_zpos - sscin_2y_zpos, sair_radlen); /* :480-489 */
+    if (SOS_SCIN_IN(sscin_2_left, sscin_2_right, sscin_2y_offset, sscin_2_bot, sscin_2_top, sscin_2x_offset))
+        scincount = scincount + 1;
+    radw = sscin_2x_thick / sscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (scincount < scintrig) { /* :491-494 */
+        cnt[SO_SCIN]++;
+        *stop_code = STOP_SO_SCIN;
+        return 0;
+    }
+    SOS_STEP(scal_4ta_zpos - sscin_2x_zpos, sair_radlen); /* :500-503 */
+#undef SOS_STEP
+#undef SOS_SCIN_IN
+    return 1;
+}
+
+/* mc_sos, sos/mc_sos.f:1-394 (apertures sos/apertures_sos.inc:9-26) */
+void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+               double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+               double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+               double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+               int *stop_code, int *fit_done)
+{
+    const or_fwd_map *F = M->sos_fwd;
+    const double h_entr = 7.201, v_entr = 4.696, h_exit = 7.567, v_exit = 4.935;
+    const double z_entr = 126.3e0, z_exit = z_entr + 6.3e0;
+    const double r2_quad = 163.84, w_bm01 = 8.0, w_bm02 = 8.0;
+    const double t_bm01_in = 41.32, b_bm01_in = -30.52, t_bm01_out = 53.05, b_bm01_out = -65.07;
+    const double t_bm02_in = 51.73, b_bm02_in = -66.38, t_bm02_out = 51.52, b_bm02_out = -55.85;
+    const double w_exit = 8.57, t_exit = 51.52, b_exit = -55.85;
+    const int decay_flag = S->decay_flag, ms_flag = S->ms_flag, wcs_flag = S->wcs_flag;
+
+    double p, xt, yt, zdrift, tmpwidth;
+    double dpp_recon, dth_recon, dph_recon, y_recon;
+    int dflag = 0, ok;
+    *ok_spec = 0;
+    *stop_code = STOP_OK;
+    *fit_done = 0;
+    cnt[SO_TRIALS]++;
+
+    t->xs = *x;
+    t->ys = *y;
+    t->zs = *z;
+    t->dxdzs = *dxdz;
+    t->dydzs = *dydz;
+    t->dpps = *dpp;
+    p = p_spec * (1. + t->dpps / 100.);
+
+#define SO_STOP(C, CODE) do { cnt[C]++; *stop_code = (CODE); return; } while (0)
+    zdrift = z_entr; /* :171-187 */
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (fabs(t->ys) > h_entr) SO_STOP(SO_SLIT_HOR, STOP_SLIT_HOR);
+    if (fabs(t->xs) > v_entr) SO_STOP(SO_SLIT_VERT, STOP_SLIT_VERT);
+    if (fabs(t->xs) > (-v_entr / h_entr * fabs(t->ys) + 3 * v_entr / 2))
+        SO_STOP(SO_SLIT_OCT, STOP_SLIT_OCT);
+    zdrift = z_exit - z_entr; /* :191-204 */
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if (fabs(t->ys) > h_exit) SO_STOP(SO_SLIT_HOR, STOP_SLIT_HOR);
+    if (fabs(t->xs) > v_exit) SO_STOP(SO_SLIT_VERT, STOP_SLIT_VERT);
+    if (fabs(t->xs) > (-v_exit / h_exit * fabs(t->ys) + 3 * v_exit / 2))
+        SO_STOP(SO_SLIT_OCT, STOP_SLIT_OCT);
+
+    zdrift = F->driftdist[0] - z_exit; /* :209-215 */
+    project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r2_quad) SO_STOP(SO_QUAD_IN, STOP_SO_QUAD_IN);
+    transp(F, g, t, S, 2, decay_flag, &dflag, m2, &p, 35.0e0, pathlen); /* :219-223 */
+    if ((t->xs * t->xs + t->ys * t->ys) > r2_quad) SO_STOP(SO_QUAD_MID, STOP_SO_QUAD_MID);
+    transp(F, g, t, S, 3, decay_flag, &dflag, m2, &p, 35.0e0, pathlen); /* :227-231 */
+    if ((t->xs * t->xs + t->ys * t->ys) > r2_quad) SO_STOP(SO_QUAD_OUT, STOP_SO_QUAD_OUT);
+
+    transp(F, g, t, S, 4, decay_flag, &dflag, m2, &p, 80.0e0, pathlen); /* :236-245 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, -45.0e0, &xt, &yt);
+    if ((yt > w_bm01) || (-yt > (w_bm01 - 0.05 * 2.54)) || (-xt > t_bm01_in) || (-xt < b_bm01_in))
+        SO_STOP(SO_BM01_IN, STOP_SO_BM01_IN);
+    transp(F, g, t, S, 5, decay_flag, &dflag, m2, &p, 169.52e0, pathlen); /* :250-257 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 45.0e0, &xt, &yt);
+    if ((fabs(yt) > w_bm01) || (-xt > t_bm01_out) || (-xt < b_bm01_out))
+        SO_STOP(SO_BM01_OUT, STOP_SO_BM01_OUT);
+    transp(F, g, t, S, 6, decay_flag, &dflag, m2, &p, 80.80e0, pathlen); /* :262-269 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 49.0e0, &xt, &yt);
+    if ((fabs(yt) > w_bm02) || (-xt > t_bm02_in) || (-xt < b_bm02_in))
+        SO_STOP(SO_BM02_IN, STOP_SO_BM02_IN);
+    transp(F, g, t, S, 7, decay_flag, &dflag, m2, &p, 77.06e0, pathlen); /* :274-281 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 57.0e0, &xt, &yt);
+    if ((fabs(yt) > w_bm02) || (-xt > t_bm02_out) || (-xt < b_bm02_out))
+        SO_STOP(SO_BM02_OUT, STOP_SO_BM02_OUT);
+
+    transp(F, g, t, S, 8, decay_flag, &dflag, m2, &p, 43.82e0, pathlen); /* :311-319 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 45.0e0, &xt, &yt);
+    if ((fabs(yt) > w_exit) || (-xt > t_exit) || (-xt < b_exit)) SO_STOP(SO_EXIT, STOP_SO_EXIT_FLANGE);
+    transp(F, g, t, S, 9, decay_flag, &dflag, m2, &p, 44.34e0, pathlen); /* :329-336 */
+    tmpwidth = 10.998 + 0.10209 * (t->xs + 37.694);
+    if ((fabs(t->xs) > 37.694) || fabs(t->ys) > tmpwidth) SO_STOP(SO_EXIT, STOP_SO_EXIT_APERTURE);
+    t->xs = t->xs + 7.62 * t->dxdzs; /* :342-350 */
+    t->ys = t->ys + 7.62 * t->dydzs;
+    if ((fabs(t->xs) > 38.1) || fabs(t->ys) > 12.7) SO_STOP(SO_EXIT, STOP_SO_EXIT_WINDOW);
+#undef SO_STOP
+
+    cnt[SO_HUT]++; /* :366-372 */
+    ok = sos_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+                 &dflag, resmult, -3.206267e0, pathlen, stop_code, fit_done);
+    if (!ok) return;
+
+    t->xs = *x_fp;
+    t->ys = *y_fp;
+    t->dxdzs = *dx_fp;
+    t->dydzs = *dy_fp;
+    recon_sos(M->sos_rec, t, fry, &dpp_recon, &dth_recon, &dph_recon, &y_recon);
+    *dpp = dpp_recon;
+    *dxdz = dph_recon;
+    *dydz = dth_recon;
+    *y = y_recon;
+    *ok_spec = 1;
+    cnt[SO_SUCCESSES]++;
+}
+
+/* ========================================================================= */
 /* SHMS                                                                       */
 /* ========================================================================= */
 
@@ -1894,9 +2221,10 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
            uint64_t seed, uint64_t event0, int nthreads, int64_t *counters)
 {
     if (!M || !S || !in || !out || n < 0) return OR_E_ARG;
-    if (arm != 1 && arm != 5) return OR_E_ARG;
+    if (arm != 1 && arm != 2 && arm != 5) return OR_E_ARG;
+    if (arm == 2 && !M->sos_fwd) return OR_E_ARG;
     if (rng_kind == 0 && (!tape || stride <= 0)) return OR_E_TAPE;
-    int ncount = arm == 1 ? H_NCOUNT : S_NCOUNT;
+    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : S_NCOUNT;
     or_ranlux *rl = NULL;
     if (rng_kind == 2) {
         rl = or_ranlux_new(3, (int)(seed & 0x7fffffff));
@@ -1941,6 +2269,10 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
                 or_mc_hms(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
                           &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
                           &fit_done);
+            else if (arm == 2)
+                or_mc_sos(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
+                          &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
+                          &fit_done);
             else
                 or_mc_shms(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
                            &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
@@ -1981,8 +2313,9 @@ int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const
                  const double *y_fp, const double *dy_fp, const double *fry, double *delta,
                  double *dxdz, double *dydz, double *ytar)
 {
-    if (!M || (arm != 1 && arm != 5) || n < 0) return OR_E_ARG;
-    const or_recon_map *R = arm == 1 ? M->hms_rec : M->shms_rec;
+    if (!M || (arm != 1 && arm != 2 && arm != 5) || n < 0) return OR_E_ARG;
+    if (arm == 2 && !M->sos_rec) return OR_E_ARG;
+    const or_recon_map *R = arm == 1 ? M->hms_rec : arm == 2 ? M->sos_rec : M->shms_rec;
     for (int64_t i = 0; i < n; i++) {
         or_track t;
         memset(&t, 0, sizeof t);
@@ -1991,7 +2324,8 @@ int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const
         t.ys = y_fp[i];
         t.dydzs = dy_fp[i];
         double dp, dt, dph, yt;
-        recon(R, &t, fry ? fry[i] : 0.0, &dp, &dt, &dph, &yt);
+        if (arm == 2) recon_sos(R, &t, fry ? fry[i] : 0.0, &dp, &dt, &dph, &yt);
+        else recon(R, &t, fry ? fry[i] : 0.0, &dp, &dt, &dph, &yt);
         delta[i] = dp;
         dxdz[i] = dph;
         dydz[i] = dt;

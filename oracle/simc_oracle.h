@@ -109,6 +109,13 @@ enum {
     S_S3, S_CAL, S_STOP_ID, S_COLL, S_NCOUNT
 };
 
+/* SOS stop counters, same order as common /spectrometer2/ (sos/struct_sos.inc:1-17) */
+enum {
+    SO_TRIALS, SO_SLIT_HOR, SO_SLIT_VERT, SO_SLIT_OCT, SO_QUAD_IN, SO_QUAD_MID, SO_QUAD_OUT,
+    SO_BM01_IN, SO_BM01_OUT, SO_BM02_IN, SO_BM02_OUT, SO_EXIT, SO_HUT, SO_DC1, SO_DC2,
+    SO_SCIN, SO_CAL, SO_SUCCESSES, SO_NCOUNT
+};
+
 /* Stop codes: one unambiguous code per exit point of the path.  The batch
  * drivers write one per event; 0 = accepted. */
 enum {
@@ -129,6 +136,10 @@ enum {
     STOP_S_D1_MID3, STOP_S_D1_MID4, STOP_S_D1_MID5, STOP_S_D1_MID6, STOP_S_D1_MID7,
     STOP_S_D1_MEX, STOP_S_D1_OUT, STOP_S_DC1, STOP_S_DC2, STOP_S_S1X, STOP_S_S1Y,
     STOP_S_S2X, STOP_S_S2Y, STOP_S_CAL, STOP_S_CALFID,
+    /* SOS */
+    STOP_SO_QUAD_IN = 100, STOP_SO_QUAD_MID, STOP_SO_QUAD_OUT, STOP_SO_BM01_IN,
+    STOP_SO_BM01_OUT, STOP_SO_BM02_IN, STOP_SO_BM02_OUT, STOP_SO_EXIT_FLANGE,
+    STOP_SO_EXIT_APERTURE, STOP_SO_EXIT_WINDOW, STOP_SO_DC1, STOP_SO_DC2, STOP_SO_SCIN,
     /* bookkeeping */
     STOP_TAPE_OVERFLOW = 250
 };
@@ -145,6 +156,8 @@ typedef struct {
 typedef struct {
     or_fwd_map *hms_fwd, *shms_fwd;
     or_recon_map *hms_rec, *shms_rec;
+    or_fwd_map *sos_fwd;        /* NULL when dir has no sos/ maps */
+    or_recon_map *sos_rec;
 } or_maps;
 
 /* ---- map files ---- */
@@ -184,6 +197,11 @@ void or_recon_eval(const or_recon_map *m, const double hut[5], double sum[4]);
 
 /* ---- per-event arms (argument lists of hms/mc_hms.f:1-4, shms/mc_shms.f:1-4) ---- */
 void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
+               double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
+               double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+               double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
+               int *stop_code, int *fit_done);
+void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
                double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
                double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,

@@ -11,8 +11,8 @@ montecarlo() does before calling the arm (simc.f:1411-1441):
     (x_arm, y_arm) -= z_arm (dx/dz, dy/dz)        # drift back to z = 0
     fry = x_arm                                    # xtar_init, simc.f:1441
 
-with phi = 3 pi/2 for a left-side spectrometer (HMS) and pi/2 for the SHMS
-(dbase.f sets spec%p%phi / spec%e%phi this way).
+with phi = 3 pi/2 for a left-side spectrometer (HMS) and pi/2 for the SOS and
+the SHMS (dbase.f sets spec%p%phi / spec%e%phi this way).
 """
 from __future__ import annotations
 
@@ -53,6 +53,16 @@ HMS_C0 = Workload("hms_proton_3GeV", 1, 100_000, 3000.0, 20.0, MP, (-12.0, 12.0)
                   (-0.08, 0.08), (-0.05, 0.05), seed=1001)
 SHMS_C1 = Workload("shms_electron_6GeV", 5, 10_000_000, 6000.0, 12.5, ME, (-15.0, 25.0),
                    (-0.06, 0.06), (-0.04, 0.04), wire_sigma=0.030, seed=1002)
+# SOS (arm 2): 1.5 GeV/c protons at 30 degrees, ranges a little beyond the
+# collimator (h 7.2, v 4.7 cm at 126 cm, sos/mc_sos.f:62-65)
+SOS_1p5 = Workload("sos_proton_1.5GeV", 2, 1_000_000, 1500.0, 30.0, MP, (-15.0, 15.0),
+                   (-0.045, 0.045), (-0.07, 0.07), seed=1003)
+
+
+def arm_phi(arm: int) -> float:
+    """Spectrometer phi: 3 pi/2 for HMS and HRS-R (arms 1, 3), pi/2 for SOS,
+    HRS-L and SHMS (2, 4, 5, 6), dbase.f:246-262."""
+    return 1.5 * math.pi if arm in (1, 3) else 0.5 * math.pi
 
 
 def generate(w: Workload, n: int | None = None, seed: int | None = None) -> dict:
@@ -66,7 +76,7 @@ def generate(w: Workload, n: int | None = None, seed: int | None = None) -> dict
     dxdz = rng.uniform(w.dxdz[0], w.dxdz[1], n)
     dydz = rng.uniform(w.dydz[0], w.dydz[1], n)
     th = math.radians(w.theta_deg)
-    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    phi = arm_phi(w.arm)
     cth, sth, sphi = math.cos(th), math.sin(th), math.sin(phi)
     x = -yt
     y = -xt * cth - zt * sth * sphi
@@ -95,7 +105,7 @@ def generate_device(w: Workload, n: int, seed: int, device, out: dict | None = N
     dxdz = uni(*w.dxdz)
     dydz = uni(*w.dydz)
     th = math.radians(w.theta_deg)
-    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    phi = arm_phi(w.arm)
     cth, sth, sphi = math.cos(th), math.sin(th), math.sin(phi)
     x = -yt
     y = -xt * cth - zt * sth * sphi
@@ -135,7 +145,7 @@ def generate_vertex(w: Workload, n: int | None = None, seed: int | None = None) 
     delta = rng.uniform(w.dpp[0], w.dpp[1], n)
     xptar = rng.uniform(w.dxdz[0], w.dxdz[1], n)
     yptar = rng.uniform(w.dydz[0], w.dydz[1], n)
-    phi = 1.5 * math.pi if w.arm == 1 else 0.5 * math.pi
+    phi = arm_phi(w.arm)
     p = w.p_spec * (1.0 + delta / 100.0)
     E = np.sqrt(p * p + w.mass * w.mass)
     theta = physics_theta(math.radians(w.theta_deg), phi, xptar, yptar)

@@ -6,7 +6,8 @@ oracle/_ref/libsimcref.so; this script drives it over small event samples on
 per-event uniform tapes and stores inputs, tape recipe and every output.  The
 tapes are regenerated from their seed by the tests (sha256 recorded here).
 
-    python tests/golden/make_golden.py          # arm goldens (*.npz)
+    python tests/golden/make_golden.py [prefix] # arm goldens (*.npz), optionally
+                                                # only the cases named prefix*
     python tests/golden/make_golden.py target   # target step (target/*.npz):
         the reference's trip_thru_target and target_musc (target.f) over
         vertices on per-event tapes
@@ -34,6 +35,9 @@ CASES = [
     ("shms_electron_no_ms_no_wcs", 5, W.SHMS_C1, dict(p_spec=6000.0, m2=W.ME ** 2, ms=False, wcs=False)),
     ("hms_pion_coll_decay", 1, W.HMS_C0, dict(p_spec=2000.0, m2=MPI ** 2, coll=True, decay=True, ctau=780.45)),
     ("shms_pion_coll_decay", 5, W.SHMS_C1, dict(p_spec=2000.0, m2=MPI ** 2, coll=True, decay=True, ctau=780.45)),
+    ("sos_proton_ms_wcs", 2, W.SOS_1p5, dict(p_spec=1500.0, m2=W.MP ** 2)),
+    ("sos_electron_no_ms_no_wcs", 2, W.SOS_1p5, dict(p_spec=1200.0, m2=W.ME ** 2, ms=False, wcs=False)),
+    ("sos_pion_decay", 2, W.SOS_1p5, dict(p_spec=1000.0, m2=MPI ** 2, decay=True, ctau=780.45)),
 ]
 N = 1500
 STRIDE = 2048
@@ -43,8 +47,10 @@ def tape_for(i):
     return W.uniform_tape(N, STRIDE, 900 + i)
 
 
-def main():
+def main(only=None):
     for i, (name, arm, wl, kw) in enumerate(CASES):
+        if only and not name.startswith(only):
+            continue
         ev = W.generate(wl, N, seed=wl.seed + 17 * i)
         tape = tape_for(i)
         args = dict(ms=True, wcs=True)
@@ -123,4 +129,4 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["target"]:
         main_target()
     else:
-        main()
+        main(sys.argv[1] if sys.argv[1:] else None)

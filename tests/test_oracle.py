@@ -43,7 +43,7 @@ def test_oracle_matches_reference_golden(oracle, path):
     for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
               "resmult", "m2", "ok", "used"):
         np.testing.assert_array_equal(out[k], g["out_" + k], err_msg=k)
-    names = oracle.HMS_COUNTERS if int(g["arm"]) == 1 else oracle.SHMS_COUNTERS
+    names = {1: oracle.HMS_COUNTERS, 2: oracle.SOS_COUNTERS, 5: oracle.SHMS_COUNTERS}[int(g["arm"])]
     for i, name in enumerate(names):
         if name != "stop_id":  # a last-value register, not a counter
             assert out["counters"][i] == g["counters"][i], name
@@ -262,6 +262,9 @@ MAP_SHA = {
     "hms/recon_cosy.dat": "61cf89af0c3a5f7370ca69b84f2a2cd47d8b201c34eb4bbd761afd83e51d80ca",
     "shms/shms_forward.dat": "398595898491378e3d4ade794e1af6b324afd1fcccc20ec9bb5d4456ff75cffc",
     "shms/shms_recon.dat": "7477b37ed142de7743b50080f38d7e45074196e7387ee899f79d04c86bb655b0",
+    # sos/forward_cosy.dat, sos/recon_cosy.dat of the reference tree, copied verbatim
+    "sos/forward_cosy.dat": "ffb83fecf4edadeaed99a6c138a7a555a262922f690f674d2bba77dce922ed7b",
+    "sos/recon_cosy.dat": "0093b61d39ec60aecc0e84d09dd2b03283b1c258dc2ccf56a2691669b82a17ea",
 }
 
 
