@@ -10,6 +10,8 @@
  *   mc_shms(... same ..., pathlen, spectr, using_SHMScoll) shms/mc_shms.f:1-4
  *   mc_hms_recon(delta_p, delta_t, delta_phi, y_tgt, fry) hms/mc_hms_recon.f:1
  *   mc_shms_recon(..., fry, spectr)                       shms/mc_shms_recon.f:1-2
+ *   mc_sos(... as mc_hms without using_HMScoll)            sos/mc_sos.f:1-3
+ *   mc_sos_recon(delta_p, delta_t, delta_phi, y_tgt, fry)  sos/mc_sos_recon.f:1
  *   transp_init(spectr, n_classes)                       shared/transp.f:294
  * called once per event per arm from montecarlo() (simc.f:1464-1488,
  * 1730-1754).  Here one call transports a whole batch of events held in
@@ -44,6 +46,7 @@ extern "C" {
 #define SIMC_E_INTERNAL (-11) /* inconsistent step program (library defect)          */
 
 #define SIMC_ARM_HMS 1        /* spectrometer numbers of transp_init, transp.f:300-313 */
+#define SIMC_ARM_SOS 2
 #define SIMC_ARM_SHMS 5
 
 #define SIMC_RNG_PHILOX 1     /* counter-based: uniform k of event e = Philox4x32-10(key=seed, ctr=(k/4, e, tag)) */
@@ -54,6 +57,7 @@ extern "C" {
 
 #define SIMC_HMS_NCOUNTERS 22  /* common /spectrometer1/,      hms/struct_hms.inc:34-42   */
 #define SIMC_SHMS_NCOUNTERS 52 /* common /spectrometer_shms/, shms/struct_shms.inc:26-44 */
+#define SIMC_SOS_NCOUNTERS 18  /* common /spectrometer2/,     sos/struct_sos.inc:9-17    */
 
 typedef struct simc_ctx simc_ctx;
 
@@ -101,13 +105,18 @@ typedef struct {
     int64_t stride;        /* uniforms per event slot           */
 } simc_rng;
 
-/* transp_init + the recon readers for both arms.  map_dir holds
+/* transp_init + the recon readers for the arms.  map_dir holds
  * hms/forward_cosy.dat, hms/recon_cosy.dat, shms/shms_forward.dat,
- * shms/shms_recon.dat (the reference's relative paths, transp.f:300-309,
- * hms/mc_hms_recon.f:73, shms/mc_shms_recon.f:74). */
+ * shms/shms_recon.dat and, optionally, sos/forward_cosy.dat and
+ * sos/recon_cosy.dat (the reference's relative paths, transp.f:300-309,
+ * hms/mc_hms_recon.f:73, shms/mc_shms_recon.f:74, sos/mc_sos_recon.f:73);
+ * the SOS maps are loaded when present. */
 int simc_ctx_create(const char *map_dir, simc_ctx **out);
 int simc_ctx_create_files(const char *hms_forward, const char *hms_recon,
                           const char *shms_forward, const char *shms_recon, simc_ctx **out);
+/* Load (or replace) one arm's maps: transp_init(spectr) and the recon reader
+ * of that arm (arm = SIMC_ARM_HMS, SIMC_ARM_SOS or SIMC_ARM_SHMS). */
+int simc_ctx_load_arm(simc_ctx *ctx, int arm, const char *forward, const char *recon);
 void simc_ctx_destroy(simc_ctx *ctx);
 
 /* Transport + reconstruct n events through one arm (mc_hms / mc_shms).
@@ -120,6 +129,11 @@ int simc_hms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 int simc_shms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
                   simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+/* mc_sos(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp,
+ *        m2, ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen)
+ * sos/mc_sos.f:1-3 (no collimator stepping: using_coll is ignored). */
+int simc_sos_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
+                 simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(PKG, "libsimc_gpu.so")
 MAP_DIR = os.path.join(REPO, "data", "maps")
 
 HMS = 1
+SOS = 2
 SHMS = 5
 RNG_TAPE = 0
 RNG_PHILOX = 1
@@ -42,11 +43,17 @@ SHMS_COUNTERS = [
     "Q3_mex", "Q3_out", "D1_in", "D1_flr", "D1_men", "D1_mid1", "D1_mid2", "D1_mid3",
     "D1_mid4", "D1_mid5", "D1_mid6", "D1_mid7", "D1_mex", "D1_out", "BP_in", "BP_out", "hut",
     "dc1", "dc2", "cer1", "cer2", "s1", "s2", "s3", "cal", "stop_id", "coll"]
+# common /spectrometer2/ (sos/struct_sos.inc:9-17); "exit" sums the three exit-aperture
+# stops (sos/mc_sos.f:343-358), "scin" counts events lost at the trigger.
+SOS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "slit_oct", "quad_in", "quad_mid", "quad_out", "bm01_in",
+    "bm01_out", "bm02_in", "bm02_out", "exit", "hut", "dc1", "dc2", "scin", "cal", "successes"]
+COUNTERS = {HMS: HMS_COUNTERS, SOS: SOS_COUNTERS, SHMS: SHMS_COUNTERS}
 
 # Every exported C entry point (include/simc_gpu.h).
 EXPORTS = [
-    "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_destroy", "simc_hms_run",
-    "simc_shms_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
+    "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_load_arm", "simc_ctx_destroy",
+    "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
@@ -136,6 +143,7 @@ def library(path: str = LIB_PATH):
     L = C.CDLL(path)
     L.simc_ctx_create.argtypes = [C.c_char_p, C.POINTER(_vp)]
     L.simc_ctx_create_files.argtypes = [C.c_char_p] * 4 + [C.POINTER(_vp)]
+    L.simc_ctx_load_arm.argtypes = [_vp, C.c_int, C.c_char_p, C.c_char_p]
     L.simc_ctx_destroy.argtypes = [_vp]
     L.simc_ctx_destroy.restype = None
     L.simc_arm_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
@@ -205,6 +213,10 @@ class Spectrometers:
         else:
             _check(L.simc_ctx_create(map_dir.encode(), C.byref(h)))
         self._h = h
+
+    def load_arm(self, arm, forward, recon):
+        """Load (or replace) one arm's forward and reconstruction COSY maps."""
+        _check(library().simc_ctx_load_arm(self._h, arm, forward.encode(), recon.encode()))
 
     def close(self):
         if self._h:
@@ -391,7 +403,7 @@ class Spectrometers:
 
     # -- bookkeeping -------------------------------------------------------------------
     def counters(self, arm):
-        names = HMS_COUNTERS if arm == HMS else SHMS_COUNTERS
+        names = COUNTERS[arm]
         buf = (C.c_int64 * len(names))()
         _check(library().simc_counters_get(self._h, arm, buf, len(names)))
         return dict(zip(names, list(buf)))
@@ -451,6 +463,15 @@ def mc_shms(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_fla
                      decay_flag, fry, pathlen, using_SHMScoll, ctau, **kw)
 
 
+def mc_sos(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+           fry, pathlen=None, ctau=0.0, **kw):
+    """Vectorised mc_sos (sos/mc_sos.f:1-3).  The SOS collimator is always the
+    octagon pair (sos/mc_sos.f:173-204): there is no collimator switch."""
+    del th_spec
+    return _arm_call(ctx, SOS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
+                     decay_flag, fry, pathlen, False, ctau, **kw)
+
+
 class Batch:
     """Many kinematic settings transported in one run (simc_batch_*).
 
@@ -505,7 +526,7 @@ class Batch:
         return out
 
     def counters(self, i):
-        names = HMS_COUNTERS if self.settings[i]["arm"] == HMS else SHMS_COUNTERS
+        names = COUNTERS[self.settings[i]["arm"]]
         buf = (C.c_int64 * len(names))()
         _check(library().simc_batch_counters(self._h, i, buf, len(names)))
         return dict(zip(names, list(buf)))

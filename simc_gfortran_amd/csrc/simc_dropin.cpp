@@ -1,18 +1,18 @@
 // Per-event drop-in replacements of the reference's Fortran subroutines
-// mc_hms, mc_shms, mc_hms_recon and mc_shms_recon (gfortran/flang ABI:
+// mc_hms, mc_shms, mc_sos and their recon routines (gfortran/flang ABI:
 // lower-case names with a trailing underscore, every argument by reference,
 // LOGICAL as 4-byte integers), built as libsimc_dropin.so on top of
 // libsimc_gpu.so.  Linking a SIMC executable against this library instead of
-// the objects of hms/mc_hms*.f and shms/mc_shms*.f keeps every call site of
+// the objects of hms/mc_hms*.f, shms/mc_shms*.f and sos/mc_sos*.f keeps every call site of
 // simc.f unchanged; each call transports ONE event on the GPU (synchronous),
 // so this path is for compatibility -- throughput needs the batched entry
 // points of simc_gpu.h.
 //
 // State shared with the caller's program through its common blocks, when
 // they exist (weak references; a program without them still links):
-//   /spectrometer1/, /spectrometer_shms/  STOP counters, incremented per event
-//                                         (hms/struct_hms.inc:34-42,
-//                                          shms/struct_shms.inc:26-44);
+//   /spectrometer1/, /spectrometer_shms/, /spectrometer2/  STOP counters,
+//            incremented per event (hms/struct_hms.inc:34-42,
+//            shms/struct_shms.inc:26-44, sos/struct_sos.inc:22-35);
 //   /track/  xs, ys, zs, dpps, dxdzs, dydzs (spectrometers.inc:47-60): set to
 //            the fitted focal-plane track of an accepted event, as
 //            hms/mc_hms.f:419-423 does, and read by the recon entry points
@@ -35,6 +35,7 @@
 extern "C" {
 extern int32_t spectrometer1_[SIMC_HMS_NCOUNTERS] __attribute__((weak));
 extern int32_t spectrometer_shms_[SIMC_SHMS_NCOUNTERS] __attribute__((weak));
+extern int32_t spectrometer2_[SIMC_SOS_NCOUNTERS] __attribute__((weak));
 extern double track_[6] __attribute__((weak));
 extern double decd_[1] __attribute__((weak));
 }
@@ -66,8 +67,12 @@ simc_ctx* context()
 
 void add_counters(int arm)
 {
-    const int nc = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
-    int32_t* common = arm == SIMC_ARM_HMS ? spectrometer1_ : spectrometer_shms_;
+    const int nc = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS
+                 : arm == SIMC_ARM_SOS ? SIMC_SOS_NCOUNTERS
+                                       : SIMC_SHMS_NCOUNTERS;
+    int32_t* common = arm == SIMC_ARM_HMS ? spectrometer1_
+                    : arm == SIMC_ARM_SOS ? spectrometer2_
+                                          : spectrometer_shms_;
     if (!common) return;
     int64_t c[SIMC_SHMS_NCOUNTERS];
     if (simc_counters_get(g_ctx, arm, c, nc) != SIMC_OK) {
@@ -176,6 +181,26 @@ void mc_shms_(double* p_spec, double* th_spec, double* dpp, double* x, double* y
     }
     arm_event(SIMC_ARM_SHMS, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag,
               wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen, using_coll);
+}
+
+// mc_sos(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp,
+//        m2, ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen),
+// sos/mc_sos.f:1-3 (no collimator switch: the octagons always apply).
+void mc_sos_(double* p_spec, double* th_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+             double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2, int32_t* ms_flag,
+             int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry, int32_t* ok_spec,
+             double* pathlen)
+{
+    (void)th_spec;
+    int32_t no_coll = 0;
+    arm_event(SIMC_ARM_SOS, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag, wcs_flag,
+              decay_flag, resmult, fry, ok_spec, pathlen, &no_coll);
+}
+
+// mc_sos_recon(delta_p, delta_t, delta_phi, y_tgt, fry), sos/mc_sos_recon.f:1
+void mc_sos_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
+{
+    recon_event(SIMC_ARM_SOS, delta_p, delta_t, delta_phi, y_tgt, fry);
 }
 
 // mc_hms_recon(delta_p, delta_t, delta_phi, y_tgt, fry), hms/mc_hms_recon.f:1

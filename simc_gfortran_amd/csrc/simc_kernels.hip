@@ -45,6 +45,10 @@ enum : int {
     ST_S_D1_FLR, ST_S_D1_MEN, ST_S_D1_MID1, ST_S_D1_MID2, ST_S_D1_MID3, ST_S_D1_MID4,
     ST_S_D1_MID5, ST_S_D1_MID6, ST_S_D1_MID7, ST_S_D1_MEX, ST_S_D1_OUT, ST_S_DC1, ST_S_DC2,
     ST_S_S1X, ST_S_S1Y, ST_S_S2X, ST_S_S2Y, ST_S_CAL, ST_S_CALFID,
+    // SOS
+    ST_SO_QUAD_IN = 100, ST_SO_QUAD_MID, ST_SO_QUAD_OUT, ST_SO_BM01_IN, ST_SO_BM01_OUT,
+    ST_SO_BM02_IN, ST_SO_BM02_OUT, ST_SO_EXIT_FLANGE, ST_SO_EXIT_APERTURE, ST_SO_EXIT_WINDOW,
+    ST_SO_DC1, ST_SO_DC2, ST_SO_SCIN,
     ST_TAPE = 250, ST_BADOP = 251
 };
 constexpr int kHistBins = 256;
@@ -128,6 +132,13 @@ __device__ __forceinline__ void poly_fwd(int cls, cdptr c, double r0, double r1,
             case 9: cosy_hms_fwd_c9(c + COSY_HMS_C9_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
             default: cosy_hms_fwd_c11(c + COSY_HMS_C11_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
         }
+    } else if (ARM == SIMC_ARM_SOS) {
+#define OFWD(N) case N: cosy_sos_fwd_c##N(c + COSY_SOS_C##N##_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        switch (cls) {
+            OFWD(2) OFWD(3) OFWD(4) OFWD(5) OFWD(6) OFWD(7) OFWD(8)
+            default: cosy_sos_fwd_c9(c + COSY_SOS_C9_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        }
+#undef OFWD
     } else {
 #define SFWD(N) case N: cosy_shms_fwd_c##N(c + COSY_SHMS_C##N##_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
         switch (cls) {
@@ -144,6 +155,7 @@ __device__ __forceinline__ void poly_recon(cdptr c, double h0, double h1, double
                                            double& s0, double& s1, double& s2, double& s3)
 {
     if (ARM == SIMC_ARM_HMS) cosy_hms_recon(c + COSY_HMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
+    else if (ARM == SIMC_ARM_SOS) cosy_sos_recon(c + COSY_SOS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
     else cosy_shms_recon(c + COSY_SHMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
 }
 
@@ -173,14 +185,17 @@ __device__ __forceinline__ void poly_fwd_tan(int cls, cdptr c, const double (&r)
 #undef TAN_ARGS
 }
 
-// Reconstruction (hms/mc_hms_recon.f:113-137, shms/mc_shms_recon.f:131-160)
+// Reconstruction (hms/mc_hms_recon.f:113-137, shms/mc_shms_recon.f:131-160,
+// sos/mc_sos_recon.f:113-137: the SOS keeps every input away from 0)
 template <int ARM>
 __device__ __forceinline__ void recon(cdptr c, const EvOut& o, double fry, EvOut& r)
 {
-    double h4 = fry / 100.0;
-    if (fabs(h4) <= 1.e-30) h4 = 1.e-30;
+    double h[5] = {o.x_fp / 100.0, o.dx_fp, o.y_fp / 100.0, o.dy_fp, fry / 100.0};
+#pragma unroll
+    for (int k = ARM == SIMC_ARM_SOS ? 0 : 4; k < 5; ++k)
+        if (fabs(h[k]) <= 1.e-30) h[k] = 1.e-30;
     double s0, s1, s2, s3;
-    poly_recon<ARM>(c, o.x_fp / 100.0, o.dx_fp, o.y_fp / 100.0, o.dy_fp, h4, s0, s1, s2, s3);
+    poly_recon<ARM>(c, h[0], h[1], h[2], h[3], h[4], s0, s1, s2, s3);
     r.dxdz = s0;         // delta_phi
     r.y = s1 * 100.0;    // y_tgt
     r.dydz = s2;         // delta_t
@@ -204,8 +219,8 @@ __device__ __forceinline__ bool hms_hit_dipole(double x, double y)
 }
 
 // Track fit of the 12 chamber planes (hms/mc_hms_hut.f:439-455,
-// shms/mc_shms_hut.f:410-426) with LFIT (cern/lfit.f:22-59, KEY = 0).
-// Plane k+1 measures x on k = 0,2,3,5,6,8,9,11 and y on k = 1,4,7,10; the
+// shms/mc_shms_hut.f:410-426, sos/mc_sos_hut.f:409-414) with LFIT
+// (cern/lfit.f:22-59, KEY = 0).  Each plane measures one coordinate; the
 // reference stores the other coordinate as zero, which LFIT's zero test
 // skips, so each fit runs over its own planes in increasing order.  Hits are
 // re-read from LDS in the second pass instead of being held in registers.
@@ -246,23 +261,34 @@ __device__ __forceinline__ void lfit_planes(const int (&pl)[NP], const double* l
     B = (T)(ymed - (double)A * xmed);
 }
 
+template <class T, int NX, int NY>
+__device__ __forceinline__ void fit_planes(const int (&xpl)[NX], const int (&ypl)[NY], const double* lds,
+                                           const double (&zdc)[12], double& x_fp, double& dx_fp, double& y_fp,
+                                           double& dy_fp)
+{
+    T a, b;
+    lfit_planes<T, NX>(xpl, lds, zdc, a, b);
+    dx_fp = (double)a; x_fp = (double)b;
+    lfit_planes<T, NY>(ypl, lds, zdc, a, b);
+    dy_fp = (double)a; y_fp = (double)b;
+}
+
+// The SOS measures y on planes 1, 3, 5 of each chamber (sos/mc_sos_hut.f:324-329),
+// the HMS and SHMS on planes 2 and 5.
+template <int ARM = SIMC_ARM_HMS>
 __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, const double (&zdc)[12],
                                           double& x_fp, double& dx_fp, double& y_fp, double& dy_fp)
 {
-    const int xpl[8] = {0, 2, 3, 5, 6, 8, 9, 11};
-    const int ypl[4] = {1, 4, 7, 10};
-    if (F.fit_f32) {
-        float a, b;
-        lfit_planes<float, 8>(xpl, lds, zdc, a, b);
-        dx_fp = (double)a; x_fp = (double)b;
-        lfit_planes<float, 4>(ypl, lds, zdc, a, b);
-        dy_fp = (double)a; y_fp = (double)b;
+    if constexpr (ARM == SIMC_ARM_SOS) {
+        const int xpl[6] = {1, 3, 5, 7, 9, 11};
+        const int ypl[6] = {0, 2, 4, 6, 8, 10};
+        if (F.fit_f32) fit_planes<float>(xpl, ypl, lds, zdc, x_fp, dx_fp, y_fp, dy_fp);
+        else fit_planes<double>(xpl, ypl, lds, zdc, x_fp, dx_fp, y_fp, dy_fp);
     } else {
-        double a, b;
-        lfit_planes<double, 8>(xpl, lds, zdc, a, b);
-        dx_fp = a; x_fp = b;
-        lfit_planes<double, 4>(ypl, lds, zdc, a, b);
-        dy_fp = a; y_fp = b;
+        const int xpl[8] = {0, 2, 3, 5, 6, 8, 9, 11};
+        const int ypl[4] = {1, 4, 7, 10};
+        if (F.fit_f32) fit_planes<float>(xpl, ypl, lds, zdc, x_fp, dx_fp, y_fp, dy_fp);
+        else fit_planes<double>(xpl, ypl, lds, zdc, x_fp, dx_fp, y_fp, dy_fp);
     }
 }
 
@@ -364,6 +390,19 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
                     st = s;
                 }
                 break;
+            case OP_CHECK_T_BOX:
+                if ((yt > p0) || (-yt > p1) || (-xt > ops[pc].p[2]) || (-xt < ops[pc].p[3])) st = ops[pc].stop;
+                break;
+            case OP_CHECK_TRAP:
+                if ((fabs(t.xs) > p0) || fabs(t.ys) > (p1 + ops[pc].p[2] * (t.xs + p0))) st = ops[pc].stop;
+                break;
+            case OP_SHIFT:
+                t.xs = t.xs + p0 * t.dxdzs;
+                t.ys = t.ys + p0 * t.dydzs;
+                break;
+            case OP_CHECK_ABS:
+                if ((fabs(t.xs) > p0) || fabs(t.ys) > p1) st = ops[pc].stop;
+                break;
             default:
                 st = ST_BADOP;  // not a magnet step (excluded by the host's check)
                 break;
@@ -426,7 +465,7 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
 #pragma unroll
                 for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
                 double x_fp, dx_fp, y_fp, dy_fp;
-                fit_track(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
+                fit_track<ARM>(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
                 st64(a.x_fp, a.n_arr, off, x_fp);
                 st64(a.dx_fp, a.n_arr, off, dx_fp);
                 st64(a.y_fp, a.n_arr, off, y_fp);
@@ -1197,25 +1236,27 @@ int fail(int code, const std::string& msg)
 
 }  // namespace
 
+constexpr int kNumArms = 3;  // HMS, SHMS, SOS (arm_index)
+
 struct simc_ctx {
-    simc::ForwardMap fwd[2];
-    simc::ReconMap rec[2];
-    double* d_coef[2] = {nullptr, nullptr};
-    Op* d_ops[2] = {nullptr, nullptr};
-    ArmTables* d_tab[2] = {nullptr, nullptr};
-    std::vector<Op> ops[2];
-    int n_ops[2] = {0, 0};
-    double zdc[2][12];
-    unsigned long long* d_hist[2] = {nullptr, nullptr};
-    int coll_pc[2] = {0, 0};         // first step after the collimator
-    int hut_pc[2] = {0, 0};          // first step of the detector hut
+    simc::ForwardMap fwd[kNumArms];
+    simc::ReconMap rec[kNumArms];
+    double* d_coef[kNumArms] = {};   // null: the arm's maps are not loaded
+    Op* d_ops[kNumArms] = {};
+    ArmTables* d_tab[kNumArms] = {};
+    std::vector<Op> ops[kNumArms];
+    int n_ops[kNumArms] = {};
+    double zdc[kNumArms][12];
+    unsigned long long* d_hist[kNumArms] = {};
+    int coll_pc[kNumArms] = {};      // first step after the collimator
+    int hut_pc[kNumArms] = {};       // first step of the detector hut
     // survivor queues between the stages (per arm; grown on demand)
-    void* d_queue[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    uint32_t* d_qcount[2] = {nullptr, nullptr};   // the two queue counters
-    uint32_t q_cap[2] = {0, 0};
+    void* d_queue[kNumArms][2] = {};
+    uint32_t* d_qcount[kNumArms] = {};   // the two queue counters
+    uint32_t q_cap[kNumArms] = {};
     std::mutex mu;                   // guards queue growth
     // reconstruction exponents per arm (adjoint form), [n_terms][5]
-    uint8_t* d_rexp[2] = {nullptr, nullptr};
+    uint8_t* d_rexp[kNumArms] = {};
     // adjoint scratch: selected-event list and its counter (grown on demand)
     uint32_t* d_sel = nullptr;
     uint32_t* d_nsel = nullptr;
@@ -1252,7 +1293,16 @@ struct simc_batch {
 
 namespace {
 
-int arm_index(int arm) { return arm == SIMC_ARM_HMS ? 0 : arm == SIMC_ARM_SHMS ? 1 : -1; }
+int arm_index(int arm)
+{
+    return arm == SIMC_ARM_HMS ? 0 : arm == SIMC_ARM_SHMS ? 1 : arm == SIMC_ARM_SOS ? 2 : -1;
+}
+
+// STOP counters of an arm's common block
+int ncounters_of(int arm)
+{
+    return arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : arm == SIMC_ARM_SOS ? SIMC_SOS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+}
 
 // Step-program builder.  All parameters are formed with the reference's own
 // expressions in double precision, so they equal the reference's values.
@@ -1291,9 +1341,11 @@ struct Program {
 const double kInf = HUGE_VAL;
 
 // Wire-chamber planes of one chamber: hms/mc_hms_hut.f:341-370 (the SHMS loop,
-// shms/mc_shms_hut.f:172-205, is the same with its own constants).
+// shms/mc_shms_hut.f:172-205, and the SOS loop, sos/mc_sos_hut.f:298-333, are
+// the same with their own constants).  Bit iplane-1 of y_planes marks the
+// planes that measure y: 2 and 5 in the HMS and SHMS, 1, 3 and 5 in the SOS.
 void chamber(Program& P, int jchamber, double cath_radw, double hdc_thick, double hdc_radlen,
-             double cath_thick, double wire_radw, double wire_thick)
+             double cath_thick, double wire_radw, double wire_thick, unsigned y_planes = 0x12u)
 {
     for (int iplane = 1; iplane <= 6; ++iplane) {
         P.add(OP_MUSC, {cath_radw});
@@ -1302,7 +1354,7 @@ void chamber(Program& P, int jchamber, double cath_radw, double hdc_thick, doubl
         drift = drift + cath_thick;
         P.add(OP_PROJECT, {drift});
         P.add(OP_MUSC, {wire_radw});
-        P.add(OP_WIRE, {}, (jchamber - 1) * 6 + iplane - 1, (iplane == 2 || iplane == 5) ? 1 : 0);
+        P.add(OP_WIRE, {}, (jchamber - 1) * 6 + iplane - 1, (y_planes >> (iplane - 1)) & 1u);
         drift = 0.5 * hdc_thick;
         P.add(OP_MUSC, {drift / hdc_radlen});
         drift = drift + wire_thick;
@@ -1438,6 +1490,126 @@ void build_hms(const simc::ForwardMap& f, Program& P, double zdc[12])
 }
 
 // mc_shms (shms/mc_shms.f:255-1106) followed by mc_shms_hut (shms/mc_shms_hut.f:74-448)
+// mc_sos (sos/mc_sos.f:160-372, apertures sos/apertures_sos.inc:9-26) followed
+// by mc_sos_hut (sos/mc_sos_hut.f:253-503, constants :42-197)
+void build_sos(const simc::ForwardMap& f, Program& P, double zdc[12])
+{
+    const double z_entr = 126.3e0, z_exit = z_entr + 6.3e0;  // :69-70
+    const double r2_quad = 163.84, w_bm01 = 8.0, w_bm02 = 8.0;
+    const double* dd = f.driftdist.data();
+    P.add(OP_PROJECT, {z_entr});                                                          // :171-172
+    P.add(OP_COLLIMATOR, {7.201, 4.696, 7.567, 4.935, 0.0, 0.0, z_exit - z_entr});       // :173-204
+    P.add(OP_PROJECT, {dd[0] - z_exit});  P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_IN);  // :209-215
+    P.add(OP_TRANSP, {35.0e0}, 0, 2);     P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_MID); // :219-223
+    P.add(OP_TRANSP, {35.0e0}, 0, 3);     P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_OUT); // :227-231
+    P.add(OP_TRANSP, {80.0e0}, 0, 4);                                                     // :236-245
+    P.rot(OP_ROT_H, -45.0e0, 0.0, 0.0);
+    P.add(OP_CHECK_T_BOX, {w_bm01, w_bm01 - 0.05 * 2.54, 41.32, -30.52}, ST_SO_BM01_IN);
+    P.add(OP_TRANSP, {169.52e0}, 0, 5);                                                   // :250-257
+    P.rot(OP_ROT_H, 45.0e0, 0.0, 0.0);
+    P.add(OP_CHECK_T_BOX, {w_bm01, w_bm01, 53.05, -65.07}, ST_SO_BM01_OUT);
+    P.add(OP_TRANSP, {80.80e0}, 0, 6);                                                    // :262-269
+    P.rot(OP_ROT_H, 49.0e0, 0.0, 0.0);
+    P.add(OP_CHECK_T_BOX, {w_bm02, w_bm02, 51.73, -66.38}, ST_SO_BM02_IN);
+    P.add(OP_TRANSP, {77.06e0}, 0, 7);                                                    // :274-281
+    P.rot(OP_ROT_H, 57.0e0, 0.0, 0.0);
+    P.add(OP_CHECK_T_BOX, {w_bm02, w_bm02, 51.52, -55.85}, ST_SO_BM02_OUT);
+    P.add(OP_TRANSP, {43.82e0}, 0, 8);                                                    // :311-319
+    P.rot(OP_ROT_H, 45.0e0, 0.0, 0.0);
+    P.add(OP_CHECK_T_BOX, {8.57, 8.57, 51.52, -55.85}, ST_SO_EXIT_FLANGE);
+    P.add(OP_TRANSP, {44.34e0}, 0, 9);                                                    // :329-336
+    P.add(OP_CHECK_TRAP, {37.694, 10.998, 0.10209}, ST_SO_EXIT_APERTURE);
+    P.add(OP_SHIFT, {7.62});                                                              // :342-343
+    P.add(OP_CHECK_ABS, {38.1, 12.7}, ST_SO_EXIT_WINDOW);                                 // :346-350
+
+    // detector hut from z = -3.206267 (:370-371)
+    const double zinit = -3.206267e0;
+    const double sfoil_exit_radlen = 53.3, sfoil_exit_thick = 0.020 * 2.54, sfoil_exit_zpos = -3.22;
+    const double sair_radlen = 30420.;
+    const double sdc_radlen = 16700.0, sdc_thick = 0.61775;
+    const double sdc_wire_radlen = 0.35, sdc_wire_thick = 0.0000354;
+    const double sdc_cath_radlen = 28.7, sdc_cath_thick = 0.0005 * 2.54;
+    const double sscin_radlen = 42.4;
+    const double scer_entr_radlen = 8.90, scer_entr_thick = 0.050, scer_radlen = 4810.0;
+    const double scer_mir_radlen = 400.0, scer_mir_thick = 2.0;
+    const double scer_exit_radlen = 8.90, scer_exit_thick = 0.050;
+    const int sdc_nr_plan = 6;
+    const double sdc_1_zpos = 6.25, sdc_2_zpos = 55.77;
+    const double sdc_del_plane = sdc_thick + sdc_wire_thick + sdc_cath_thick;
+    const double sscin_1y_zpos = 73.61, sscin_1x_zpos = 97.11, sscin_2y_zpos = 249.51, sscin_2x_zpos = 290.81;
+    const double sscin_1x_thick = 1.040, sscin_1y_thick = 1.098, sscin_2x_thick = 1.040, sscin_2y_thick = 1.098;
+    const double scer_zentrance = 130.000, scer_zmirror = 155.000, scer_zexit = 160.000;
+    const double scal_4ta_zpos = 346.01;
+    const double scin1[4] = {18.25 + 2.25, -18.25 + 2.25, 31.75 + 2.8, -31.75 + 2.8};  // :422-425
+    const double scin2[4] = {18.25 + 2.9, -18.25 + 2.9, 56.25 + 4.9, -56.25 + 4.9};    // :472-475
+    double drift;
+
+    P.add(OP_RESMULT_DRAW, {0.15});                                                 // :253-259
+    // Exit foil at sfoil_exit_zpos, tilt angle 0 (:49): the drift
+    // (zpos + xt tan 0) - zinit = -0.013733 is raised to 0.001 (:281-285), and
+    // the air gap to the chambers starts at zpos (:292-293).
+    drift = sfoil_exit_zpos - zinit;
+    if (drift <= 0.001) drift = 0.001;
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC, {sfoil_exit_thick / sfoil_exit_radlen / std::cos(0.0)});
+    drift = (sdc_1_zpos - 0.5 * sdc_nr_plan * sdc_del_plane) - sfoil_exit_zpos;    // :292-296
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    chamber(P, 1, sdc_cath_thick / sdc_cath_radlen, sdc_thick, sdc_radlen, sdc_cath_thick,
+            sdc_wire_thick / sdc_wire_radlen, sdc_wire_thick, 0x15u);                // :298-333
+    P.add(OP_BOX, {32.0 - 8.649, -32.0 - 8.649, 24.0 - -1.822, -24.0 - -1.822}, ST_SO_DC1);  // :335-341
+    P.add(OP_MUSC, {sdc_cath_thick / sdc_cath_radlen});
+    drift = sdc_2_zpos - sdc_1_zpos - sdc_nr_plan * sdc_del_plane;                 // :346-350
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    chamber(P, 2, sdc_cath_thick / sdc_cath_radlen, sdc_thick, sdc_radlen, sdc_cath_thick,
+            sdc_wire_thick / sdc_wire_radlen, sdc_wire_thick, 0x15u);                // :352-383
+    P.add(OP_BOX, {32.0 - -1.532, -32.0 - -1.532, 24.0 - -1.976, -24.0 - -1.976}, ST_SO_DC2);  // :385-391
+    P.add(OP_MUSC, {sdc_cath_thick / sdc_cath_radlen});
+    for (int i = 0; i < 6; ++i) {                                                   // :399-407
+        zdc[i] = sdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * sdc_nr_plan) * sdc_del_plane;
+        zdc[6 + i] = sdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * sdc_nr_plan) * sdc_del_plane;
+    }
+    P.add(OP_FIT);                                                                  // :409-414
+    drift = sscin_1y_zpos - sdc_2_zpos - 0.5 * sdc_nr_plan * sdc_del_plane;         // :418-428
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_SCIN, {scin1[0], scin1[1], scin1[2], scin1[3]});
+    P.add(OP_MUSC, {sscin_1y_thick / sscin_radlen});
+    drift = sscin_1x_zpos - sscin_1y_zpos;                                          // :430-439
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_SCIN, {scin1[0], scin1[1], scin1[2], scin1[3]});
+    P.add(OP_MUSC, {sscin_1x_thick / sscin_radlen});
+    drift = scer_zentrance - sscin_1x_zpos;                                         // :443-448
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_MUSC, {scer_entr_thick / scer_entr_radlen});
+    drift = scer_zmirror - scer_zentrance;                                          // :451-456
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / scer_radlen, drift});
+    P.add(OP_MUSC, {scer_mir_thick / scer_mir_radlen});
+    drift = scer_zexit - scer_zmirror;                                              // :459-464
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / scer_radlen, drift});
+    P.add(OP_MUSC, {scer_exit_thick / scer_exit_radlen});
+    drift = sscin_2y_zpos - scer_zexit;                                             // :469-478
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_SCIN, {scin2[0], scin2[1], scin2[2], scin2[3]});
+    P.add(OP_MUSC, {sscin_2y_thick / sscin_radlen});
+    drift = sscin_2x_zpos - sscin_2y_zpos;                                          // :480-489
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_SCIN, {scin2[0], scin2[1], scin2[2], scin2[3]});
+    P.add(OP_MUSC, {sscin_2x_thick / sscin_radlen});
+    P.add(OP_TRIGGER, {}, ST_SO_SCIN, 3);                                           // :491-494
+    drift = scal_4ta_zpos - sscin_2x_zpos;                                          // :500-503
+    P.add(OP_PROJECT, {drift});
+    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
+    P.add(OP_END);
+}
+
 void build_shms(Program& P, double zdc[12])
 {
     const double zd_hbin = 118.39, zd_hbmen = 17.61, zd_hbmex = 80.0, zd_hbout = 17.61;  // :176-179
@@ -1628,7 +1800,8 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     std::string msg;
     int rc = simc::read_forward_map(fwd_path, ctx->fwd[ai], msg);
     if (rc) return fail(rc, msg);
-    const int want = arm == SIMC_ARM_HMS ? 12 : 32;  // hms/mc_hms.f:188, shms/mc_shms.f:344
+    // hms/mc_hms.f:188, shms/mc_shms.f:344, sos/mc_sos.f:164
+    const int want = arm == SIMC_ARM_HMS ? 12 : arm == SIMC_ARM_SOS ? 10 : 32;
     if (ctx->fwd[ai].n_classes != want)
         return fail(SIMC_E_NCLASS, fwd_path + ": " + std::to_string(ctx->fwd[ai].n_classes) +
                                        " transport classes, expected " + std::to_string(want));
@@ -1639,6 +1812,7 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     if (rc) return fail(rc, msg);
     Program P;
     if (arm == SIMC_ARM_HMS) build_hms(ctx->fwd[ai], P, ctx->zdc[ai]);
+    else if (arm == SIMC_ARM_SOS) build_sos(ctx->fwd[ai], P, ctx->zdc[ai]);
     else build_shms(P, ctx->zdc[ai]);
     ctx->n_ops[ai] = (int)P.ops.size();
     ctx->ops[ai] = P.ops;
@@ -1741,6 +1915,20 @@ void hist_to_counters(int arm, const unsigned long long* h, int64_t* c, int ncou
         c[20] = H(ST_OK);
         c[15] = c[16] + c[17] + c[18] + c[20];
         c[21] = H(ST_COLL);
+    } else if (arm == SIMC_ARM_SOS) {
+        // order of common /spectrometer2/, sos/struct_sos.inc:9-17
+        c[0] = total;
+        c[1] = H(ST_SLIT_HOR);
+        c[2] = H(ST_SLIT_VERT);
+        c[3] = H(ST_SLIT_OCT);
+        for (int k = 0; k < 7; ++k) c[4 + k] = H(ST_SO_QUAD_IN + k);
+        c[11] = H(ST_SO_EXIT_FLANGE) + H(ST_SO_EXIT_APERTURE) + H(ST_SO_EXIT_WINDOW);
+        c[13] = H(ST_SO_DC1);
+        c[14] = H(ST_SO_DC2);
+        c[15] = H(ST_SO_SCIN);
+        c[16] = 0;  // sSTOP_cal: the calorimeter cut is commented out, sos/mc_sos_hut.f:519-538
+        c[17] = H(ST_OK);
+        c[12] = c[13] + c[14] + c[15] + c[17];
     } else {
         // order of common /spectrometer_shms/, shms/struct_shms.inc:26-44
         c[0] = total;
@@ -1803,14 +1991,45 @@ int simc_ctx_create(const char* map_dir, simc_ctx** out)
 {
     if (!map_dir) return fail(SIMC_E_ARG, "simc_ctx_create: NULL map_dir");
     const std::string d(map_dir);
-    return simc_ctx_create_files((d + "/hms/forward_cosy.dat").c_str(), (d + "/hms/recon_cosy.dat").c_str(),
-                                 (d + "/shms/shms_forward.dat").c_str(), (d + "/shms/shms_recon.dat").c_str(), out);
+    int rc = simc_ctx_create_files((d + "/hms/forward_cosy.dat").c_str(), (d + "/hms/recon_cosy.dat").c_str(),
+                                   (d + "/shms/shms_forward.dat").c_str(), (d + "/shms/shms_recon.dat").c_str(),
+                                   out);
+    if (rc) return rc;
+    // the SOS maps are optional: loaded when present (sos/mc_sos.f:160-165)
+    const std::string sf = d + "/sos/forward_cosy.dat", sr = d + "/sos/recon_cosy.dat";
+    if (FILE* f = std::fopen(sf.c_str(), "r")) {
+        std::fclose(f);
+        rc = simc_ctx_load_arm(*out, SIMC_ARM_SOS, sf.c_str(), sr.c_str());
+        if (rc) {
+            std::string keep = g_err;
+            simc_ctx_destroy(*out);
+            *out = nullptr;
+            g_err = keep;
+            return rc;
+        }
+    }
+    return SIMC_OK;
+}
+
+int simc_ctx_load_arm(simc_ctx* ctx, int arm, const char* forward, const char* recon)
+{
+    if (!ctx || !forward || !recon) return fail(SIMC_E_ARG, "simc_ctx_load_arm: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), SIMC_ARM_SOS (2) or SIMC_ARM_SHMS (5)");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipDeviceSynchronize());  // runs of the arm may still use its tables
+    void* old[5] = {ctx->d_coef[ai], ctx->d_ops[ai], ctx->d_tab[ai], ctx->d_rexp[ai], ctx->d_hist[ai]};
+    ctx->d_coef[ai] = nullptr; ctx->d_ops[ai] = nullptr; ctx->d_tab[ai] = nullptr;
+    ctx->d_rexp[ai] = nullptr; ctx->d_hist[ai] = nullptr;
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    return load_arm(ctx, arm, forward, recon);
 }
 
 void simc_ctx_destroy(simc_ctx* ctx)
 {
     if (!ctx) return;
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < kNumArms; ++i) {
         if (ctx->d_coef[i]) (void)hipFree(ctx->d_coef[i]);
         if (ctx->d_ops[i]) (void)hipFree(ctx->d_ops[i]);
         if (ctx->d_tab[i]) (void)hipFree(ctx->d_tab[i]);
@@ -1820,7 +2039,7 @@ void simc_ctx_destroy(simc_ctx* ctx)
         if (ctx->d_qcount[i]) (void)hipFree(ctx->d_qcount[i]);
     }
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < kNumArms; ++i)
         if (ctx->d_rexp[i]) (void)hipFree(ctx->d_rexp[i]);
     if (ctx->d_sel) (void)hipFree(ctx->d_sel);
     if (ctx->d_nsel) (void)hipFree(ctx->d_nsel);
@@ -1834,17 +2053,22 @@ namespace {
 
 int check_settings(int arm, const simc_settings* s)
 {
-    if (arm_index(arm) < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1) or SIMC_ARM_SHMS (5)");
+    if (arm_index(arm) < 0)
+        return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), SIMC_ARM_SOS (2) or SIMC_ARM_SHMS (5)");
     if (!(s->p_spec > 0.0) || !(s->m2 >= 0.0)) return fail(SIMC_E_ARG, "p_spec must be > 0 and m2 >= 0");
     if (s->decay_flag && !(s->ctau > 0.0)) return fail(SIMC_E_ARG, "decay_flag needs ctau > 0");
     return SIMC_OK;
 }
 
 // Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
-// shms/mc_shms.f:503); m2 changes only through a decay.
-bool needs_rare(const simc_settings* s)
+// shms/mc_shms.f:503; the SOS has none); m2 changes only through a decay.
+bool coll_stepping(int arm, const simc_settings* s)
 {
-    return s->decay_flag || (s->using_coll && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
+    return arm != SIMC_ARM_SOS && s->using_coll;
+}
+bool needs_rare(int arm, const simc_settings* s)
+{
+    return s->decay_flag || (coll_stepping(arm, s) && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
 }
 
 SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t event0, uint64_t start,
@@ -1855,10 +2079,11 @@ SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t even
     d.p_spec = s->p_spec;
     d.m2 = s->m2;
     d.ctau = s->ctau;
-    // hdc_sigma: hms/mc_hms_hut.f:143-144, shms/hut.inc:125-126
-    d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_HMS ? 0.030 : 0.020);
+    // hdc_sigma (hms/mc_hms_hut.f:143-144), sdc_sigma (sos/mc_sos_hut.f:112-113),
+    // shms/hut.inc:125-126
+    d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_SHMS ? 0.020 : 0.030);
     d.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
-              (s->using_coll ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
+              (coll_stepping(arm, s) ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
     d.start = start;
     d.seed = seed;
     d.event0 = event0;
@@ -1899,6 +2124,8 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
 #define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
     if (arm == SIMC_ARM_HMS) {
         if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
+    } else if (arm == SIMC_ARM_SOS) {
+        if (tape) LAUNCH1(SIMC_ARM_SOS, TapeStream); else LAUNCH1(SIMC_ARM_SOS, PhiloxStream);
     } else {
         if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
     }
@@ -1915,6 +2142,7 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
     const int ai = arm_index(arm);
     int rc = check_settings(arm, s);
     if (rc) return rc;
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (n < 0) return fail(SIMC_E_ARG, "negative event count");
     if (n == 0) return SIMC_OK;
     if (!outputs_complete(in, out)) return fail(SIMC_E_ARG, "missing event input or output array");
@@ -1933,7 +2161,7 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
     a.ops = ctx->d_ops[ai];
     a.hist = ctx->d_hist[ai];
     const bool tape = rng->kind == SIMC_RNG_TAPE;
-    const bool rare = needs_rare(s);
+    const bool rare = needs_rare(arm, s);
     // Launches are cut so that every byte offset of a launch fits 32 bits
     // and each survivor queue stays at most 2^24 entries (1.7 GB).
     int64_t chunk = (int64_t)1 << 24;
@@ -2116,13 +2344,20 @@ int simc_shms_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* i
     return simc_arm_run(ctx, SIMC_ARM_SHMS, s, in, out, n, rng, stream);
 }
 
+int simc_sos_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+                 int64_t n, const simc_rng* rng, void* stream)
+{
+    return simc_arm_run(ctx, SIMC_ARM_SOS, s, in, out, n, rng, stream);
+}
+
 int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const double* dx_fp,
                    const double* y_fp, const double* dy_fp, const double* fry, double* delta,
                    double* dxdz_tar, double* dydz_tar, double* y_tar, void* stream)
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_recon_run: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (n < 0) return fail(SIMC_E_ARG, "negative event count");
     if (n == 0) return SIMC_OK;
     if (!x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !dxdz_tar || !dydz_tar || !y_tar)
@@ -2132,6 +2367,9 @@ int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const 
     hipStream_t st = (hipStream_t)stream;
     if (arm == SIMC_ARM_HMS)
         hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+                           dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
+    else if (arm == SIMC_ARM_SOS)
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SOS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else
         hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
@@ -2391,8 +2629,9 @@ int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
 {
     if (!ctx || !c) return fail(SIMC_E_ARG, "simc_counters_get: NULL argument");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
-    const int need = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
+    const int need = ncounters_of(arm);
     if (ncounters < need) return fail(SIMC_E_ARG, "counter array too short");
     unsigned long long h[kHistSlots];
     HIPCHK(hipDeviceSynchronize());
@@ -2405,7 +2644,8 @@ int simc_counters_reset(simc_ctx* ctx, int arm)
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_counters_reset: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
     HIPCHK(hipDeviceSynchronize());
@@ -2416,7 +2656,8 @@ int simc_map_class_info(simc_ctx* ctx, int arm, int cls, double* driftdist, int3
 {
     if (!ctx) return fail(SIMC_E_ARG, "NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     const simc::ForwardMap& f = ctx->fwd[ai];
     if (cls < 1 || cls > f.n_classes) return fail(SIMC_E_ARG, "class out of range");
     if (driftdist) *driftdist = f.driftdist[cls - 1];
@@ -2429,7 +2670,8 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
 {
     if (!ctx || !flops) return fail(SIMC_E_ARG, "simc_stop_flops: NULL argument");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (stage < 0 || stage > 3) return fail(SIMC_E_ARG, "stage must be 0 (all), 1, 2 or 3");
     if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
     for (int i = 0; i < nflops; ++i) flops[i] = 0;
@@ -2458,7 +2700,8 @@ int simc_recon_coefficients(simc_ctx* ctx, int arm, double* coef, int32_t* n_ter
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_recon_coefficients: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 or 5");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     const auto& terms = ctx->rec[ai].terms;
     if (n_terms) *n_terms = (int32_t)terms.size();
     if (coef)
@@ -2476,7 +2719,8 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     int rc = check_settings(arm, s);
     if (rc) return rc;
     const int ai = arm_index(arm);
-    if (needs_rare(s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
+    if (arm == SIMC_ARM_SOS) return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
+    if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
     if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");
     if (n <= 0 || n > ((int64_t)1 << 29)) return fail(SIMC_E_ARG, "simc_adjoint_run: n must be in [1, 2^29]");
     if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
@@ -2565,6 +2809,8 @@ int simc_batch_create(simc_ctx* ctx, const simc_setting* settings, int n_setting
         const simc_setting& q = settings[i];
         int rc = check_settings(q.arm, &q.s);
         if (rc) return fail(rc, "setting " + std::to_string(i) + ": " + g_err);
+        if (!ctx->d_coef[arm_index(q.arm)])
+            return fail(SIMC_E_ARG, "setting " + std::to_string(i) + ": the arm's maps are not loaded");
         if (q.offset < 0 || q.count < 0 || q.offset + q.count > ((int64_t)1 << 29))
             return fail(SIMC_E_ARG, "setting " + std::to_string(i) + ": event range outside [0, 2^29)");
         if (q.count) ranges.push_back({q.offset, q.offset + q.count});
@@ -2586,11 +2832,11 @@ int simc_batch_create(simc_ctx* ctx, const simc_setting* settings, int n_setting
         const simc_setting& q0 = settings[i0];
         if (placed[i0] || q0.count == 0) continue;
         const int32_t f0 = make_setdev(q0.arm, &q0.s, 0, 0, 0, 0, 0).flags;
-        const bool r0 = needs_rare(&q0.s);
+        const bool r0 = needs_rare(q0.arm, &q0.s);
         simc_batch::Pass pass{q0.arm, r0, f0, q0.seed, 0, (uint32_t)table.size(), 0};
         for (int i = i0; i < n_settings; ++i) {
             const simc_setting& q = settings[i];
-            if (placed[i] || q.count == 0 || q.arm != q0.arm || needs_rare(&q.s) != r0 ||
+            if (placed[i] || q.count == 0 || q.arm != q0.arm || needs_rare(q.arm, &q.s) != r0 ||
                 make_setdev(q.arm, &q.s, 0, 0, 0, 0, 0).flags != f0 || q.seed != q0.seed)
                 continue;
             placed[i] = true;
@@ -2686,7 +2932,7 @@ int simc_batch_counters(simc_batch* b, int setting, int64_t* c, int ncounters)
     if (!b || !c) return fail(SIMC_E_ARG, "simc_batch_counters: NULL argument");
     if (setting < 0 || setting >= b->n_settings) return fail(SIMC_E_ARG, "setting index out of range");
     const int arm = b->arm_of[setting];
-    const int need = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    const int need = ncounters_of(arm);
     if (ncounters < need) return fail(SIMC_E_ARG, "counter array too short");
     unsigned long long h[kHistSlots];
     HIPCHK(hipDeviceSynchronize());
@@ -2751,9 +2997,13 @@ int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)
     if (!ctx || arm_index(arm) < 0) return 0;
     static const int hms_cls[] = {2, 3, 5, 6, 8, 9, 11, 0};
     static const int shms_cls[] = {3, 7, 8, 12, 13, 17, 18, 23, 24, 25, 26, 27, 28, 29, 30, 0};
+    static const int sos_cls[] = {2, 3, 4, 5, 6, 7, 8, 9, 0};
     int64_t f = 0;
+    if (!ctx->d_coef[arm_index(arm)]) return 0;
     if (arm == SIMC_ARM_HMS)
         for (int c : hms_cls) f += simc::poly_flops(arm, c);
+    else if (arm == SIMC_ARM_SOS)
+        for (int c : sos_cls) f += simc::poly_flops(arm, c);
     else
         for (int c : shms_cls) f += simc::poly_flops(arm, c);
     return f;

@@ -212,6 +212,7 @@ namespace {
 const simc_gen::PolyInfo* polys(int arm, int& n, int& ncoef)
 {
     if (arm == SIMC_ARM_HMS) { n = simc_gen::n_polys_hms; ncoef = simc_gen::n_coef_hms; return simc_gen::polys_hms; }
+    if (arm == SIMC_ARM_SOS) { n = simc_gen::n_polys_sos; ncoef = simc_gen::n_coef_sos; return simc_gen::polys_sos; }
     n = simc_gen::n_polys_shms;
     ncoef = simc_gen::n_coef_shms;
     return simc_gen::polys_shms;

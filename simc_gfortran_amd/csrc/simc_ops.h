@@ -1,6 +1,6 @@
 // Step programs of the spectrometer arms.
 //
-// An arm (mc_hms, mc_shms and their detector huts) is a fixed sequence of
+// An arm (mc_hms, mc_shms, mc_sos and their detector huts) is a fixed sequence of
 // drifts, COSY map segments, aperture tests and detector-hut steps.  The host
 // writes that sequence once per context as an array of Op records; one
 // kernel per arm walks it for every event (all lanes of a wavefront execute
@@ -37,6 +37,11 @@ enum OpCode : int32_t {
     OP_TRIGGER,       // stop if count < arg
     OP_FIT,           // straight-line fit of the 12 planes -> x_fp, x'_fp, y_fp, y'_fp
     OP_FIDUCIAL,      // track at z = p0: stop if yc > p1 or yc < p2 or xc > p3 or xc < p4
+    // magnet-stage steps of the SOS (sos/mc_sos.f:236-358)
+    OP_CHECK_T_BOX,   // stop if yt > p0 or -yt > p1 or -xt > p2 or -xt < p3
+    OP_CHECK_TRAP,    // stop if |xs| > p0 or |ys| > p1 + p2 (xs + p0)
+    OP_SHIFT,         // xs += p0 x', ys += p0 y' (no path length, no decay)
+    OP_CHECK_ABS,     // stop if |xs| > p0 or |ys| > p1
     OP_NCODES
 };
 
@@ -47,9 +52,9 @@ enum OpCode : int32_t {
 enum : int { STAGE_MAGNETS = 1, STAGE_HUT = 2 };
 constexpr int op_stages(int code)
 {
-    return code == OP_END || code == OP_PROJECT ? (STAGE_MAGNETS | STAGE_HUT)
-         : code <= OP_COLLIMATOR               ? STAGE_MAGNETS
-                                               : STAGE_HUT;
+    return code == OP_END || code == OP_PROJECT                ? (STAGE_MAGNETS | STAGE_HUT)
+         : code <= OP_COLLIMATOR || code >= OP_CHECK_T_BOX     ? STAGE_MAGNETS
+                                                               : STAGE_HUT;
 }
 
 struct Op {

@@ -68,6 +68,7 @@ program batch_host
 
   nc = SIMC_HMS_NCOUNTERS
   if (arm == SIMC_ARM_SHMS) nc = SIMC_SHMS_NCOUNTERS
+  if (arm == SIMC_ARM_SOS) nc = SIMC_SOS_NCOUNTERS
   rc = simc_counters_get(ctx, arm, cnt, nc)
   write (*, '(a)', advance='no') 'counters'
   do k = 1, nc

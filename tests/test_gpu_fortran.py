@@ -1,5 +1,5 @@
 """Fortran callers on the GPU: the batch driver through the module simc_gpu
-(simc_arm_run_host) and the per-event drop-in mc_hms / mc_shms called like
+(simc_arm_run_host) and the per-event drop-in mc_hms / mc_sos / mc_shms called like
 montecarlo() does, both against the library's own batched run on the same
 events and Philox streams (bit-identical), including the STOP counters the
 drop-in adds into the program's common blocks.  The programs are built by
@@ -51,7 +51,7 @@ def counters_line(stdout):
     raise AssertionError(stdout)
 
 
-@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1], ids=["hms", "shms"])
+@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1, W.SOS_1p5], ids=["hms", "shms", "sos"])
 def test_fortran_batch_driver_matches_library(ctx, tmp_path, wl):
     n, seed = 5000, 77
     ev = W.generate(wl, n, seed=3)
@@ -71,7 +71,7 @@ def test_fortran_batch_driver_matches_library(ctx, tmp_path, wl):
     assert counters_line(r.stdout) == cnt
 
 
-@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1], ids=["hms", "shms"])
+@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1, W.SOS_1p5], ids=["hms", "shms", "sos"])
 def test_fortran_per_event_dropin_matches_batched(ctx, tmp_path, wl):
     n, seed = 400, 5
     ev = W.generate(wl, n, seed=4)

@@ -41,6 +41,7 @@ def to_host(out):
 CASES = [
     ("hms_c0", W.HMS_C0),
     ("shms_c1", W.SHMS_C1),
+    ("sos_1p5", W.SOS_1p5),
 ]
 
 
@@ -118,7 +119,28 @@ def test_rare_paths_parity(ctx, oracle, arm, coll, decay):
     np.testing.assert_array_equal(g["m2"] != MPI * MPI, ref["m2"] != MPI * MPI)
 
 
-@pytest.mark.parametrize("arm", [1, 5])
+def test_sos_decay_parity(ctx, oracle):
+    """SOS pions decaying in flight (sos/mc_sos.f drifts call project with
+    decay_flag); the SOS has no collimator punch-through."""
+    n, stride = 20000, 2048
+    wl = W.SOS_1p5
+    ev = W.generate(wl, n, seed=6)
+    tape = W.uniform_tape(n, stride, 98)
+    kw = dict(p_spec=1000.0, m2=MPI * MPI, ms=True, wcs=True, decay=True, ctau=780.45)
+    ref = oracle.run(2, ev, rng="tape", tape=tape, stride=stride, fit_f32=False, **kw)
+    out = ctx.run(2, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+                  fit="fp64", **kw)
+    torch.cuda.synchronize()
+    g = to_host(out)
+    rep = parity_report(g, ref, 1e-10)
+    print("sos decay", rep)
+    assert rep["flag_mismatch"] <= 2
+    assert sum(rep["bad"].values()) <= 2, rep
+    assert (g["m2"] != MPI * MPI).any()
+    np.testing.assert_array_equal(g["m2"] != MPI * MPI, ref["m2"] != MPI * MPI)
+
+
+@pytest.mark.parametrize("arm", [1, 2, 5])
 def test_recon_only(ctx, oracle, arm):
     rng = np.random.default_rng(3)
     n = 10000
