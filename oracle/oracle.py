@@ -22,7 +22,8 @@ REF_LIB = os.path.join(HERE, "_ref", "libsimcref.so")
 H_NCOUNT = 22
 S_NCOUNT = 52
 SO_NCOUNT = 18
-NCOUNT = {1: H_NCOUNT, 2: SO_NCOUNT, 5: S_NCOUNT}
+R_NCOUNT = 21
+NCOUNT = {1: H_NCOUNT, 2: SO_NCOUNT, 3: R_NCOUNT, 4: R_NCOUNT, 5: S_NCOUNT}
 
 HMS_COUNTERS = [
     "trials", "slit_hor", "slit_vert", "slit_oct", "Q1_in", "Q1_mid", "Q1_out", "Q2_in",
@@ -38,6 +39,10 @@ SHMS_COUNTERS = [
 SOS_COUNTERS = [
     "trials", "slit_hor", "slit_vert", "slit_oct", "quad_in", "quad_mid", "quad_out", "bm01_in",
     "bm01_out", "bm02_in", "bm02_out", "exit", "hut", "dc1", "dc2", "scin", "cal", "successes"]
+HRS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "Q1_in", "Q1_mid", "Q1_out", "Q2_in", "Q2_mid", "Q2_out",
+    "Q3_in", "Q3_mid", "Q3_out", "D1_in", "D1_out", "hut", "dc1", "dc2", "s1", "s2", "cal",
+    "successes"]
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)

@@ -10,7 +10,8 @@
  *               draws from RANLUX through call_ranlux.f:58-73);
  *   lfit_()  -- the oracle's restatement of cern/lfit.f:11-60.
  * Everything else (mc_hms, mc_hms_hut, mc_hms_recon, mc_sos, mc_sos_hut,
- * mc_sos_recon, mc_shms, mc_shms_hut, mc_shms_recon, transp/transp_init, project, musc, musc_ext, rotate_*,
+ * mc_sos_recon, mc_hrsr*, mc_hrsl*, mc_shms, mc_shms_hut, mc_shms_recon,
+ * transp/transp_init, project, musc, musc_ext, rotate_*,
  * gauss1, enerloss_new, loren, the collimator routines) is the reference's
  * own code.
  */
@@ -42,7 +43,20 @@ extern void mc_sos_(double *p_spec, double *th_spec, double *dpp, double *x, dou
                     double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
                     int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
                     double *pathlen);
+/* mc_hrsr / mc_hrsl share mc_sos's argument list (hrsr/mc_hrsr.f:1-3, hrsl/mc_hrsl.f:1-3) */
+extern void mc_hrsr_(double *p_spec, double *th_spec, double *dpp, double *x, double *y, double *z,
+                     double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+                     double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
+                     int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
+                     double *pathlen);
+extern void mc_hrsl_(double *p_spec, double *th_spec, double *dpp, double *x, double *y, double *z,
+                     double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+                     double *dy_fp, double *m2, int32_t *ms_flag, int32_t *wcs_flag,
+                     int32_t *decay_flag, double *resmult, double *fry, int32_t *ok_spec,
+                     double *pathlen);
 extern struct { int32_t v[H_NCOUNT]; } spectrometer1_;
+extern struct { int32_t v[R_NCOUNT]; } spectrometer3_;  /* hrsr/struct_hrsr.inc:24-40 */
+extern struct { int32_t v[R_NCOUNT]; } spectrometer4_;  /* hrsl/struct_hrsl.inc:24-40 */
 extern struct { int32_t v[SO_NCOUNT]; } spectrometer2_;  /* sos/struct_sos.inc:9-17 */
 extern struct { int32_t v[S_NCOUNT]; } spectrometer_shms_;
 extern struct { double ctau; } decd_;
@@ -66,7 +80,7 @@ void lfit_(float *X, float *Y, int32_t *L, int32_t *KEY, float *A, float *B, flo
     or_lfit_f32(X, Y, *L, *KEY, A, B, E);
 }
 
-/* Runs n events through mc_hms (arm 1) or mc_shms (arm 5).  Same SoA
+/* Runs n events through mc_hms (arm 1), mc_sos (2), mc_hrsr (3), mc_hrsl (4) or mc_shms (5).  Same SoA
  * conventions as or_run; counters receive the change of the reference's
  * STOP counters over the call.  mapdir must hold hms/ and shms/ map files
  * (the reference opens them relative to the working directory). */
@@ -81,8 +95,10 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return -1;
     if (chdir(mapdir) != 0) return -2;
-    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : S_NCOUNT;
-    const int32_t *common = arm == 1 ? spectrometer1_.v : arm == 2 ? spectrometer2_.v : spectrometer_shms_.v;
+    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : arm == 5 ? S_NCOUNT : R_NCOUNT;
+    const int32_t *common = arm == 1 ? spectrometer1_.v : arm == 2 ? spectrometer2_.v
+                          : arm == 3 ? spectrometer3_.v : arm == 4 ? spectrometer4_.v
+                                     : spectrometer_shms_.v;
     int32_t before[S_NCOUNT];
     memcpy(before, common, ncount * sizeof(int32_t));
     decd_.ctau = ctau;
@@ -106,6 +122,12 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
         else if (arm == 2)
             mc_sos_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
                     &dcf, &rm, &fry, &okv, &pl);
+        else if (arm == 3)
+            mc_hrsr_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
+                     &dcf, &rm, &fry, &okv, &pl);
+        else if (arm == 4)
+            mc_hrsl_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf, &wcf,
+                     &dcf, &rm, &fry, &okv, &pl);
         else
             mc_shms_(&ps, &th, &d, &xx, &yy, &zz, &dx, &dy, &xf, &dxf, &yf, &dyf, &m2, &msf,
                      &wcf, &dcf, &rm, &fry, &okv, &pl, &spectr, &uc);

@@ -276,6 +276,25 @@ or_maps *or_maps_load(const char *dir)
             return NULL;
         }
     }
+    /* HRS maps are optional too (hrs?/mc_hrs?.f:148-153): hrsr/ for arm 3, hrsl/ for arm 4 */
+    for (int k = 0; k < 2; k++) {
+        const char *sub = k ? "hrsl" : "hrsr";
+        snprintf(p, sizeof p, "%s/%s/hrs_forward_cosy.dat", dir, sub);
+        probe = fopen(p, "r");
+        if (!probe) continue;
+        fclose(probe);
+        or_fwd_map *f = (or_fwd_map *)calloc(1, sizeof(or_fwd_map));
+        or_recon_map *r = (or_recon_map *)calloc(1, sizeof(or_recon_map));
+        if (k) { M->hrsl_fwd = f; M->hrsl_rec = r; }
+        else { M->hrsr_fwd = f; M->hrsr_rec = r; }
+        rc |= or_read_forward(p, f);
+        snprintf(p, sizeof p, "%s/%s/hrs_recon_cosy.dat", dir, sub);
+        rc |= or_read_recon(p, r);
+        if (rc || f->n_classes != 12) {
+            or_maps_free(M);
+            return NULL;
+        }
+    }
     return M;
 }
 
@@ -288,6 +307,10 @@ void or_maps_free(or_maps *M)
     free(M->shms_rec);
     free(M->sos_fwd);
     free(M->sos_rec);
+    free(M->hrsr_fwd);
+    free(M->hrsr_rec);
+    free(M->hrsl_fwd);
+    free(M->hrsl_rec);
     free(M);
 }
 
@@ -1779,6 +1802,315 @@ void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
 }
 
 /* ========================================================================= */
+/* HRS (HRS-R = spectrometer 3, HRS-L = spectrometer 4)                      */
+/* ========================================================================= */
+
+/* mc_hrsl_hut / mc_hrsr_hut, hrsl/mc_hrsl_hut.f:1-498 (constants :37-185); the two
+ * huts are the same code with their own counters.  resmult is 1 (:231). */
+static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+                   double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
+                   int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
+                   double zinit, double *pathlen, int *stop_code, int *fit_done)
+{
+    const double hfoil_exit_radlen = 3.56, hfoil_exit_thick = 0.01;
+    const double hair_radlen = 30420.;
+    const double hdc_entr_radlen = 34.4, hdc_entr_thick = 0.00018 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 1.5;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
+    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
+    const double hdc_exit_radlen = 34.4, hdc_exit_thick = 0.00018 * 2.54;
+    const double hscin_radlen = 42.4;
+    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54;
+    const double hcer_radlen = 36620.0;
+    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
+    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_cham = 2, hdc_nr_plan = 6;
+    const double hdc_1_zpos = -25.0 + 25.0, hdc_2_zpos = 25.0 + 25.0;
+    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
+    const double hdc_left = 14.4, hdc_right = -14.4, hdc_top = -105.6, hdc_bot = 105.6;
+    const double hdc_x_offset = 0.000, hdc_y_offset = 0.000;
+    const double hscin_1x_zpos = 95.0 + 25.0, hscin_2x_zpos = 288.3 + 25.0;
+    const double hscin_1x_thick = 0.5 * 1.067, hscin_2x_thick = 0.5 * 1.067;
+    const double hscin_1x_left = 18.0, hscin_1x_right = -18.0;
+    const double hscin_2x_left = 30.0, hscin_2x_right = -30.0;
+    const double hcer_zentrance = 137.0 + 25.0, hcer_zmirror = 197.0 + 25.0, hcer_zexit = 237.0 + 25.0;
+    const double hcal_4ta_zpos = 407.3 + 25.0;
+    const double nsig_max = 99.0e0;
+    const double sigma = S->wire_sigma > 0 ? S->wire_sigma : 0.0225; /* hdc_sigma, :113-114 */
+
+    double radw, drift, tmpran1, tmpran2, xt, yt;
+    hut_hits h;
+
+    for (int i = 0; i < 12; i++) { h.xdc[i] = 0.; h.ydc[i] = 0.; }
+    *resmult = 1.0;
+
+    radw = hfoil_exit_thick / hfoil_exit_radlen; /* exit foil at zinit, :240-241 */
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+
+    for (int jchamber = 1; jchamber <= hdc_nr_cham; jchamber++) {
+        if (jchamber == 1) /* :247-250 */
+            drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - zinit;
+        else /* :310-313 */
+            drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;
+        radw = drift / hair_radlen;
+        project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        radw = hdc_entr_thick / hdc_entr_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) { /* :256-285, :319-348 */
+            radw = hdc_cath_thick / hdc_cath_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_cath_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+            radw = hdc_wire_thick / hdc_wire_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (wcs_flag) {
+                tmpran1 = or_gauss1(g, nsig_max);
+                tmpran2 = or_gauss1(g, nsig_max);
+            } else {
+                tmpran1 = 0.;
+                tmpran2 = 0.;
+            }
+            h.xdc[npl_off + iplane - 1] = t->xs + sigma * tmpran1 * *resmult;
+            h.ydc[npl_off + iplane - 1] = t->ys + sigma * tmpran2 * *resmult;
+            if (iplane == 1 || iplane == 3 || iplane == 5) h.xdc[npl_off + iplane - 1] = 0.;
+            else h.ydc[npl_off + iplane - 1] = 0.;
+            drift = 0.5 * hdc_thick;
+            radw = drift / hdc_radlen;
+            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            drift = drift + hdc_wire_thick;
+            project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
+        }
+        radw = hdc_exit_thick / hdc_exit_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        /* the VDC frame, rotated by 45 degrees (:291-304, :354-367) */
+        xt = t->xs;
+        yt = t->ys;
+        or_rotate_haxis(t, 45.0e0, &xt, &yt);
+        if (xt > (hdc_bot - hdc_x_offset) || xt < (hdc_top - hdc_x_offset) ||
+            yt > (hdc_left - hdc_y_offset) || yt < (hdc_right - hdc_y_offset)) {
+            cnt[jchamber == 1 ? R_DC1 : R_DC2]++;
+            *stop_code = jchamber == 1 ? STOP_R_DC1 : STOP_R_DC2;
+            return 0;
+        }
+        radw = hdc_cath_thick / hdc_cath_radlen;
+        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    }
+
+    for (int jchamber = 1; jchamber <= hdc_nr_cham; jchamber++) { /* :373-381 */
+        int npl_off = (jchamber - 1) * hdc_nr_plan;
+        for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
+            double z0 = jchamber == 1 ? hdc_1_zpos : hdc_2_zpos;
+            h.zdc[npl_off + iplane - 1] = z0 + ((double)iplane - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        }
+    }
+    fit_planes(S, &h, x_fp, dx_fp, y_fp, dy_fp); /* :383-389 */
+    *fit_done = 1;
+
+#define HRS_STEP(DRIFT, RADLEN)                                                          \
+    do {                                                                                 \
+        drift = (DRIFT);                                                                 \
+        radw = drift / (RADLEN);                                                         \
+        project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);      \
+        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
+    } while (0)
+
+    HRS_STEP(hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane, hair_radlen); /* :393-405 */
+    if (t->ys > hscin_1x_left || t->ys < hscin_1x_right) {
+        cnt[R_S1]++;
+        *stop_code = STOP_R_S1;
+        return 0;
+    }
+    radw = hscin_1x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    HRS_STEP(hcer_zentrance - hscin_1x_zpos, hair_radlen); /* :410-416 */
+    radw = hcer_entr_thick / hcer_entr_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    HRS_STEP(hcer_zmirror - hcer_zentrance, hcer_radlen); /* :418-424 */
+    radw = hcer_mir_thick / hcer_mir_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    HRS_STEP(hcer_zexit - hcer_zmirror, hcer_radlen); /* :426-432 */
+    radw = hcer_exit_thick / hcer_exit_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    HRS_STEP(hscin_2x_zpos - hcer_zexit, hair_radlen); /* :436-448 */
+    if (t->ys > hscin_2x_left || t->ys < hscin_2x_right) {
+        cnt[R_S2]++;
+        *stop_code = STOP_R_S2;
+        return 0;
+    }
+    radw = hscin_2x_thick / hscin_radlen;
+    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    HRS_STEP(hcal_4ta_zpos - hscin_2x_zpos, hair_radlen); /* :453-456 */
+#undef HRS_STEP
+    return 1;
+}
+
+/* mc_hrsr / mc_hrsl, hrsr/mc_hrsr.f:1-550, hrsl/mc_hrsl.f:1-551 (apertures
+ * hrsl/apertures_hrsl.inc:6-8).  The two arms differ in the slit position,
+ * the exit slit's half width and the VDC offset subtracted from y_fp. */
+void or_mc_hrs(const or_maps *M, int arm, const or_settings *S, or_rng *g, or_track *t,
+               int64_t *cnt, double p_spec, double *dpp, double *x, double *y, double *z,
+               double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+               double *dy_fp, double *m2, double *resmult, double fry, int *ok_spec,
+               double *pathlen, int *stop_code, int *fit_done)
+{
+    const int left = arm == 4;
+    const or_fwd_map *F = left ? M->hrsl_fwd : M->hrsr_fwd;
+    const or_recon_map *R = left ? M->hrsl_rec : M->hrsr_rec;
+    const double h_entr = 3.145, v_entr = 6.090, v_exit = 6.485;
+    const double h_exit = left ? 3.335 : 3.340;
+    const double y_off = 0.0, x_off = 0.0, z_off = 0.0;
+    const double z_entr = (left ? 110.9e0 : 110.0e0) + z_off, z_exit = z_entr + 8.0e0;
+    const double r_Q1 = 15.0, r_Q2 = 30.22, r_Q3 = 30.22;
+    const double vdc_offset = left ? 0.78 : 0.48;
+    const int decay_flag = S->decay_flag, ms_flag = S->ms_flag, wcs_flag = S->wcs_flag;
+
+    double p, xt, yt, zdrift, ztmp;
+    double dpp_recon, dth_recon, dph_recon, y_recon;
+    int dflag = 0, ok;
+    *ok_spec = 0;
+    *stop_code = STOP_OK;
+    *fit_done = 0;
+    cnt[R_TRIALS]++;
+
+    t->xs = *x;
+    t->ys = *y;
+    t->zs = *z;
+    t->dxdzs = *dxdz;
+    t->dydzs = *dydz;
+    t->dpps = *dpp;
+    p = p_spec * (1. + t->dpps / 100.);
+
+#define R_STOP(C, CODE) do { cnt[C]++; *stop_code = (CODE); return; } while (0)
+#define R_DRIFT(D) project(g, t, S, &t->xs, &t->ys, (D), decay_flag, &dflag, m2, &p, pathlen)
+#define R_RAD (sqrt(t->xs * t->xs + t->ys * t->ys))
+    zdrift = 65.686; /* circular apertures before the slit box, :159-179 */
+    ztmp = zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 7.3787) R_STOP(R_SLIT_HOR, STOP_R_PIPE_65);
+    zdrift = 80.436 - ztmp;
+    ztmp = 80.436;
+    R_DRIFT(zdrift);
+    if (R_RAD > 7.4092) R_STOP(R_SLIT_HOR, STOP_R_PIPE_80);
+
+    zdrift = z_entr - ztmp; /* fixed slit, :184-218 */
+    R_DRIFT(zdrift);
+    if (fabs(t->ys - y_off) > h_entr) R_STOP(R_SLIT_HOR, STOP_SLIT_HOR);
+    if (fabs(t->xs - x_off) > v_entr) R_STOP(R_SLIT_VERT, STOP_SLIT_VERT);
+    zdrift = z_exit - z_entr;
+    R_DRIFT(zdrift);
+    if (fabs(t->ys - y_off) > h_exit) R_STOP(R_SLIT_HOR, STOP_SLIT_HOR);
+    if (fabs(t->xs - x_off) > v_exit) R_STOP(R_SLIT_VERT, STOP_SLIT_VERT);
+
+    ztmp = 135.064; /* :222-230 */
+    zdrift = ztmp - z_exit;
+    R_DRIFT(zdrift);
+    if (R_RAD > 12.5222) R_STOP(R_Q1_IN, STOP_R_Q1_PRE);
+    zdrift = F->driftdist[0] - ztmp; /* Q1, :234-265 */
+    R_DRIFT(zdrift);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q1 * r_Q1) R_STOP(R_Q1_IN, STOP_R_Q1_IN);
+    transp(F, g, t, S, 2, decay_flag, &dflag, m2, &p, 62.75333333e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q1 * r_Q1) R_STOP(R_Q1_MID, STOP_R_Q1_MID);
+    transp(F, g, t, S, 3, decay_flag, &dflag, m2, &p, 31.37666667e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q1 * r_Q1) R_STOP(R_Q1_OUT, STOP_R_Q1_OUT);
+
+    zdrift = 300.464 - 253.16; /* :268-290 */
+    ztmp = zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 14.9225) R_STOP(R_Q1_OUT, STOP_R_Q1_POST);
+    zdrift = 314.464 - 300.464;
+    ztmp = ztmp + zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 20.9550) R_STOP(R_Q2_IN, STOP_R_Q2_PRE);
+    zdrift = F->driftdist[3] - ztmp; /* Q2, :293-324 */
+    R_DRIFT(zdrift);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q2 * r_Q2) R_STOP(R_Q2_IN, STOP_R_Q2_IN);
+    transp(F, g, t, S, 5, decay_flag, &dflag, m2, &p, 121.77333333e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q2 * r_Q2) R_STOP(R_Q2_MID, STOP_R_Q2_MID);
+    transp(F, g, t, S, 6, decay_flag, &dflag, m2, &p, 60.88666667e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q2 * r_Q2) R_STOP(R_Q2_OUT, STOP_R_Q2_OUT);
+
+    zdrift = 609.664 - 553.020; /* :327-359 */
+    ztmp = zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 30.0073) R_STOP(R_Q2_OUT, STOP_R_Q2_POST1);
+    zdrift = 641.800 - 609.664;
+    ztmp = ztmp + zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 30.0073) R_STOP(R_Q2_OUT, STOP_R_Q2_POST2);
+    zdrift = 819.489 - 641.800;
+    ztmp = ztmp + zdrift;
+    R_DRIFT(zdrift);
+    if (fabs(t->xs) > 50.0 || fabs(t->ys) > 15.0) R_STOP(R_D1_IN, STOP_R_D1_PRE);
+
+    zdrift = F->driftdist[6] - ztmp; /* D1 entrance, :361-383 */
+    R_DRIFT(zdrift);
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, -30.0, &xt, &yt);
+    if (fabs(xt - 2.500) > 52.5) R_STOP(R_D1_IN, STOP_R_D1_IN);
+    if ((fabs(yt) + 0.01861 * xt) > 12.5) R_STOP(R_D1_IN, STOP_R_D1_IN);
+    transp(F, g, t, S, 8, decay_flag, &dflag, m2, &p, 659.73445725e0, pathlen); /* :385-405 */
+    xt = t->xs;
+    yt = t->ys;
+    or_rotate_haxis(t, 30.0, &xt, &yt);
+    if (fabs(xt - 2.500) > 52.5) R_STOP(R_D1_OUT, STOP_R_D1_OUT);
+    if ((fabs(yt) + 0.01861 * xt) > 12.5) R_STOP(R_D1_OUT, STOP_R_D1_OUT);
+
+    zdrift = 1745.33546 - 1655.83446; /* :409-437 */
+    ztmp = zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 30.3276) R_STOP(R_D1_OUT, STOP_R_D1_POST_R);
+    if (fabs(t->xs) > 50.0 || fabs(t->ys) > 15.0) R_STOP(R_D1_OUT, STOP_R_D1_POST_BOX);
+    zdrift = 1759.00946 - 1745.33546;
+    ztmp = ztmp + zdrift;
+    R_DRIFT(zdrift);
+    if (R_RAD > 30.3276) R_STOP(R_Q3_IN, STOP_R_Q3_PRE);
+    zdrift = F->driftdist[8] - ztmp; /* Q3, :441-472 */
+    R_DRIFT(zdrift);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q3 * r_Q3) R_STOP(R_Q3_IN, STOP_R_Q3_IN);
+    transp(F, g, t, S, 10, decay_flag, &dflag, m2, &p, 121.7866667e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q3 * r_Q3) R_STOP(R_Q3_MID, STOP_R_Q3_MID);
+    transp(F, g, t, S, 11, decay_flag, &dflag, m2, &p, 60.89333333e0, pathlen);
+    if ((t->xs * t->xs + t->ys * t->ys) > r_Q3 * r_Q3) R_STOP(R_Q3_OUT, STOP_R_Q3_OUT);
+
+    zdrift = 2080.38746 - 1997.76446; /* :475-499 */
+    ztmp = zdrift;
+    R_DRIFT(zdrift);
+    if (fabs(t->xs) > 35.56 || fabs(t->ys) > 17.145) R_STOP(R_Q3_OUT, STOP_R_Q3_POST);
+    zdrift = 2327.47246 - 2080.38746;
+    ztmp = ztmp + zdrift;
+    R_DRIFT(zdrift);
+    if (fabs(t->xs) > 99.76635 || fabs(t->ys) > 17.145) R_STOP(R_Q3_OUT, STOP_R_WINDOW);
+#undef R_RAD
+#undef R_DRIFT
+#undef R_STOP
+
+    cnt[R_HUT]++; /* :503-510 */
+    zdrift = F->driftdist[11] - ztmp;
+    ok = hrs_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+                 &dflag, resmult, -zdrift, pathlen, stop_code, fit_done);
+    if (!ok) return;
+
+    t->xs = *x_fp; /* :512-535: recon from the unshifted track, y_fp shifted by the VDC offset */
+    t->ys = *y_fp;
+    t->dxdzs = *dx_fp;
+    t->dydzs = *dy_fp;
+    *y_fp = *y_fp - vdc_offset;
+    recon(R, t, fry, &dpp_recon, &dth_recon, &dph_recon, &y_recon);
+    *dpp = dpp_recon;
+    *dxdz = dph_recon;
+    *dydz = dth_recon;
+    *y = y_recon;
+    *ok_spec = 1;
+    cnt[R_SUCCESSES]++;
+}
+
+/* ========================================================================= */
 /* SHMS                                                                       */
 /* ========================================================================= */
 
@@ -2221,10 +2553,11 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
            uint64_t seed, uint64_t event0, int nthreads, int64_t *counters)
 {
     if (!M || !S || !in || !out || n < 0) return OR_E_ARG;
-    if (arm != 1 && arm != 2 && arm != 5) return OR_E_ARG;
+    if (arm < 1 || arm > 5) return OR_E_ARG;
     if (arm == 2 && !M->sos_fwd) return OR_E_ARG;
+    if ((arm == 3 && !M->hrsr_fwd) || (arm == 4 && !M->hrsl_fwd)) return OR_E_ARG;
     if (rng_kind == 0 && (!tape || stride <= 0)) return OR_E_TAPE;
-    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : S_NCOUNT;
+    int ncount = arm == 1 ? H_NCOUNT : arm == 2 ? SO_NCOUNT : arm == 5 ? S_NCOUNT : R_NCOUNT;
     or_ranlux *rl = NULL;
     if (rng_kind == 2) {
         rl = or_ranlux_new(3, (int)(seed & 0x7fffffff));
@@ -2273,6 +2606,10 @@ int or_run(const or_maps *M, int arm, const or_settings *S, int64_t n, const or_
                 or_mc_sos(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
                           &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
                           &fit_done);
+            else if (arm == 3 || arm == 4)
+                or_mc_hrs(M, arm, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz,
+                          &x_fp, &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
+                          &fit_done);
             else
                 or_mc_shms(M, S, &g, &t, cnt, S->p_spec, &dpp, &x, &y, &z, &dxdz, &dydz, &x_fp,
                            &dx_fp, &y_fp, &dy_fp, &m2, &resmult, fry, &ok, &pathlen, &stop,
@@ -2313,9 +2650,10 @@ int or_recon_run(const or_maps *M, int arm, int64_t n, const double *x_fp, const
                  const double *y_fp, const double *dy_fp, const double *fry, double *delta,
                  double *dxdz, double *dydz, double *ytar)
 {
-    if (!M || (arm != 1 && arm != 2 && arm != 5) || n < 0) return OR_E_ARG;
-    if (arm == 2 && !M->sos_rec) return OR_E_ARG;
-    const or_recon_map *R = arm == 1 ? M->hms_rec : arm == 2 ? M->sos_rec : M->shms_rec;
+    if (!M || arm < 1 || arm > 5 || n < 0) return OR_E_ARG;
+    const or_recon_map *R = arm == 1 ? M->hms_rec : arm == 2 ? M->sos_rec : arm == 3 ? M->hrsr_rec
+                          : arm == 4 ? M->hrsl_rec : M->shms_rec;
+    if (!R) return OR_E_ARG;
     for (int64_t i = 0; i < n; i++) {
         or_track t;
         memset(&t, 0, sizeof t);

@@ -116,6 +116,14 @@ enum {
     SO_SCIN, SO_CAL, SO_SUCCESSES, SO_NCOUNT
 };
 
+/* HRS stop counters, same order as common /spectrometer3/ (HRS-R, hrsr/struct_hrsr.inc:24-40)
+ * and /spectrometer4/ (HRS-L, hrsl/struct_hrsl.inc:24-40) */
+enum {
+    R_TRIALS, R_SLIT_HOR, R_SLIT_VERT, R_Q1_IN, R_Q1_MID, R_Q1_OUT, R_Q2_IN, R_Q2_MID,
+    R_Q2_OUT, R_Q3_IN, R_Q3_MID, R_Q3_OUT, R_D1_IN, R_D1_OUT, R_HUT, R_DC1, R_DC2, R_S1,
+    R_S2, R_CAL, R_SUCCESSES, R_NCOUNT
+};
+
 /* Stop codes: one unambiguous code per exit point of the path.  The batch
  * drivers write one per event; 0 = accepted. */
 enum {
@@ -140,6 +148,12 @@ enum {
     STOP_SO_QUAD_IN = 100, STOP_SO_QUAD_MID, STOP_SO_QUAD_OUT, STOP_SO_BM01_IN,
     STOP_SO_BM01_OUT, STOP_SO_BM02_IN, STOP_SO_BM02_OUT, STOP_SO_EXIT_FLANGE,
     STOP_SO_EXIT_APERTURE, STOP_SO_EXIT_WINDOW, STOP_SO_DC1, STOP_SO_DC2, STOP_SO_SCIN,
+    /* HRS (both arms) */
+    STOP_R_PIPE_65 = 120, STOP_R_PIPE_80, STOP_R_Q1_PRE, STOP_R_Q1_IN, STOP_R_Q1_MID,
+    STOP_R_Q1_OUT, STOP_R_Q1_POST, STOP_R_Q2_PRE, STOP_R_Q2_IN, STOP_R_Q2_MID, STOP_R_Q2_OUT,
+    STOP_R_Q2_POST1, STOP_R_Q2_POST2, STOP_R_D1_PRE, STOP_R_D1_IN, STOP_R_D1_OUT,
+    STOP_R_D1_POST_R, STOP_R_D1_POST_BOX, STOP_R_Q3_PRE, STOP_R_Q3_IN, STOP_R_Q3_MID,
+    STOP_R_Q3_OUT, STOP_R_Q3_POST, STOP_R_WINDOW, STOP_R_DC1, STOP_R_DC2, STOP_R_S1, STOP_R_S2,
     /* bookkeeping */
     STOP_TAPE_OVERFLOW = 250
 };
@@ -158,12 +172,14 @@ typedef struct {
     or_recon_map *hms_rec, *shms_rec;
     or_fwd_map *sos_fwd;        /* NULL when dir has no sos/ maps */
     or_recon_map *sos_rec;
+    or_fwd_map *hrsr_fwd, *hrsl_fwd;   /* NULL when dir has no hrsr/ (hrsl/) maps */
+    or_recon_map *hrsr_rec, *hrsl_rec;
 } or_maps;
 
 /* ---- map files ---- */
 int or_read_forward(const char *path, or_fwd_map *m);
 int or_read_recon(const char *path, or_recon_map *m);
-or_maps *or_maps_load(const char *dir);   /* dir holds hms/ and shms/ */
+or_maps *or_maps_load(const char *dir);   /* dir holds hms/ and shms/ (+ optional sos/, hrsr/, hrsl/) */
 void or_maps_free(or_maps *m);
 
 /* ---- random numbers ---- */
@@ -206,6 +222,12 @@ void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
                double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                double *m2, double *resmult, double fry, int *ok_spec, double *pathlen,
                int *stop_code, int *fit_done);
+/* mc_hrsr (arm 3) / mc_hrsl (arm 4), hrsr/mc_hrsr.f:1-3, hrsl/mc_hrsl.f:1-3 */
+void or_mc_hrs(const or_maps *M, int arm, const or_settings *S, or_rng *g, or_track *t,
+               int64_t *cnt, double p_spec, double *dpp, double *x, double *y, double *z,
+               double *dxdz, double *dydz, double *x_fp, double *dx_fp, double *y_fp,
+               double *dy_fp, double *m2, double *resmult, double fry, int *ok_spec,
+               double *pathlen, int *stop_code, int *fit_done);
 void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
                 double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
                 double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
@@ -213,14 +235,14 @@ void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, 
                 int *stop_code, int *fit_done);
 
 /* ---- batch driver ----
- * arm: 1 = HMS, 5 = SHMS.  rng_kind: 0 tape (tape[i*stride + k]), 1 philox
+ * arm: 1 = HMS, 2 = SOS, 3 = HRS-R, 4 = HRS-L, 5 = SHMS.  rng_kind: 0 tape (tape[i*stride + k]), 1 philox
  * (key = seed, stream = global event id = event0 + i), 2 ranlux (one global
  * stream in event order; forces a single thread).
  * Inputs are SoA; outputs follow the reference's INOUT conventions:
  *   dpp/y/dxdz/dydz: recon values on success, input values otherwise;
  *   x_fp..dy_fp: the track fit when it was reached, else left untouched;
  *   pathlen: input + accumulated path.
- * counters: H_NCOUNT or S_NCOUNT int64, accumulated (not cleared). */
+ * counters: the arm's H_/SO_/R_/S_NCOUNT int64, accumulated (not cleared). */
 typedef struct {
     const double *dpp, *x, *y, *z, *dxdz, *dydz, *fry;
 } or_events_in;

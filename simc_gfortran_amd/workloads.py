@@ -29,7 +29,7 @@ MPI = 139.57018        # constants.inc:23
 @dataclass
 class Workload:
     name: str
-    arm: int                 # 1 = HMS, 5 = SHMS
+    arm: int                 # 1 = HMS, 2 = SOS, 3 = HRS-R, 4 = HRS-L, 5 = SHMS
     n: int
     p_spec: float            # MeV/c
     theta_deg: float
@@ -57,6 +57,13 @@ SHMS_C1 = Workload("shms_electron_6GeV", 5, 10_000_000, 6000.0, 12.5, ME, (-15.0
 # collimator (h 7.2, v 4.7 cm at 126 cm, sos/mc_sos.f:62-65)
 SOS_1p5 = Workload("sos_proton_1.5GeV", 2, 1_000_000, 1500.0, 30.0, MP, (-15.0, 15.0),
                    (-0.045, 0.045), (-0.07, 0.07), seed=1003)
+# HRS-R (arm 3) protons and HRS-L (arm 4) electrons: ranges a little beyond the
+# slit (h 3.1, v 6.1 cm at 110-111 cm, hrsl/mc_hrsl.f:54-57) and the +-4.5 %
+# momentum bite
+HRSR_2 = Workload("hrsr_proton_2GeV", 3, 1_000_000, 2000.0, 20.0, MP, (-6.0, 6.0),
+                  (-0.07, 0.07), (-0.04, 0.04), seed=1004)
+HRSL_3 = Workload("hrsl_electron_3GeV", 4, 1_000_000, 3000.0, 15.0, ME, (-6.0, 6.0),
+                  (-0.07, 0.07), (-0.04, 0.04), seed=1005)
 
 
 def arm_phi(arm: int) -> float:

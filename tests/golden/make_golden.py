@@ -1,6 +1,7 @@
 """Write the golden vectors of tests/golden/*.npz from the reference itself.
 
-The reference's Fortran hot path (mc_hms, mc_shms and everything they call)
+The reference's Fortran hot path (mc_hms, mc_shms, mc_sos, mc_hrsr, mc_hrsl and
+everything they call)
 is compiled from its own sources by `make -C oracle ref` into
 oracle/_ref/libsimcref.so; this script drives it over small event samples on
 per-event uniform tapes and stores inputs, tape recipe and every output.  The
@@ -38,6 +39,10 @@ CASES = [
     ("sos_proton_ms_wcs", 2, W.SOS_1p5, dict(p_spec=1500.0, m2=W.MP ** 2)),
     ("sos_electron_no_ms_no_wcs", 2, W.SOS_1p5, dict(p_spec=1200.0, m2=W.ME ** 2, ms=False, wcs=False)),
     ("sos_pion_decay", 2, W.SOS_1p5, dict(p_spec=1000.0, m2=MPI ** 2, decay=True, ctau=780.45)),
+    ("hrsr_proton_ms_wcs", 3, W.HRSR_2, dict(p_spec=2000.0, m2=W.MP ** 2)),
+    ("hrsl_electron_ms_wcs", 4, W.HRSL_3, dict(p_spec=3000.0, m2=W.ME ** 2)),
+    ("hrsl_electron_no_ms_no_wcs", 4, W.HRSL_3, dict(p_spec=3000.0, m2=W.ME ** 2, ms=False, wcs=False)),
+    ("hrsr_pion_decay", 3, W.HRSR_2, dict(p_spec=1500.0, m2=MPI ** 2, decay=True, ctau=780.45)),
 ]
 N = 1500
 STRIDE = 2048

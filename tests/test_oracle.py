@@ -43,7 +43,8 @@ def test_oracle_matches_reference_golden(oracle, path):
     for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
               "resmult", "m2", "ok", "used"):
         np.testing.assert_array_equal(out[k], g["out_" + k], err_msg=k)
-    names = {1: oracle.HMS_COUNTERS, 2: oracle.SOS_COUNTERS, 5: oracle.SHMS_COUNTERS}[int(g["arm"])]
+    names = {1: oracle.HMS_COUNTERS, 2: oracle.SOS_COUNTERS, 3: oracle.HRS_COUNTERS, 4: oracle.HRS_COUNTERS,
+             5: oracle.SHMS_COUNTERS}[int(g["arm"])]
     for i, name in enumerate(names):
         if name != "stop_id":  # a last-value register, not a counter
             assert out["counters"][i] == g["counters"][i], name
@@ -265,6 +266,12 @@ MAP_SHA = {
     # sos/forward_cosy.dat, sos/recon_cosy.dat of the reference tree, copied verbatim
     "sos/forward_cosy.dat": "ffb83fecf4edadeaed99a6c138a7a555a262922f690f674d2bba77dce922ed7b",
     "sos/recon_cosy.dat": "0093b61d39ec60aecc0e84d09dd2b03283b1c258dc2ccf56a2691669b82a17ea",
+    # hrsr/ and hrsl/ hrs_forward_cosy.dat, hrs_recon_cosy.dat of the reference tree, copied
+    # verbatim (the two arms ship identical files)
+    "hrsr/hrs_forward_cosy.dat": "8bfd3cfafdd5aebaad2a60f9901e918193c64f929191e075a4c6711fd1d943f3",
+    "hrsr/hrs_recon_cosy.dat": "e43c5816a93d9e3cbdd86681a3acb2c8e3a3c1a58d52f8eb3a0f5d8bd71b787d",
+    "hrsl/hrs_forward_cosy.dat": "8bfd3cfafdd5aebaad2a60f9901e918193c64f929191e075a4c6711fd1d943f3",
+    "hrsl/hrs_recon_cosy.dat": "e43c5816a93d9e3cbdd86681a3acb2c8e3a3c1a58d52f8eb3a0f5d8bd71b787d",
 }
 
 
@@ -287,7 +294,10 @@ class ReconMap(C.Structure):
 
 class Maps(C.Structure):
     _fields_ = [("hms_fwd", C.POINTER(FwdMap)), ("shms_fwd", C.POINTER(FwdMap)),
-                ("hms_rec", C.POINTER(ReconMap)), ("shms_rec", C.POINTER(ReconMap))]
+                ("hms_rec", C.POINTER(ReconMap)), ("shms_rec", C.POINTER(ReconMap)),
+                ("sos_fwd", C.POINTER(FwdMap)), ("sos_rec", C.POINTER(ReconMap)),
+                ("hrsr_fwd", C.POINTER(FwdMap)), ("hrsl_fwd", C.POINTER(FwdMap)),
+                ("hrsr_rec", C.POINTER(ReconMap)), ("hrsl_rec", C.POINTER(ReconMap))]
 
 
 @pytest.fixture(scope="module")
@@ -347,10 +357,32 @@ def test_drift_class_is_a_field_free_drift(oracle, maps):
             assert abs(-out[4] - want_dl) < 1e-6 * Ld * (xp * xp + yp * yp) + 1e-9
 
 
+# Central path lengths mc_hrsr / mc_hrsl pass to transp (hrsl/mc_hrsl.f:244-259,
+# :303-318, :388, :451-466).  transp compares each with the length the map's comments
+# give and warns beyond 0.01 (shared/transp.f:125-130): the lengths agree.
+HRS_LENGTHS = {2: 62.75333333, 3: 31.37666667, 5: 121.77333333, 6: 60.88666667, 8: 659.73445725,
+               10: 121.7866667, 11: 60.89333333}
+
+
+def test_hrs_maps_match_the_central_lengths(maps):
+    for m in (maps.hrsr_fwd.contents, maps.hrsl_fwd.contents):
+        assert m.n_classes == 12
+        for cls, zd in HRS_LENGTHS.items():
+            assert not m.adrift[cls - 1]
+            assert m.length[cls - 1] > 0.01 and abs(m.length[cls - 1] - zd) <= 0.01, cls
+        for cls in (1, 4, 7, 9, 12):  # the drifts mc_hrs? checks with adrift (:235, :294, :364, :442, :505)
+            assert m.adrift[cls - 1], cls
+    for r in (maps.hrsr_rec.contents, maps.hrsl_rec.contents):
+        e = np.array(r.expon[:5 * r.n_terms]).reshape(-1, 5)
+        # every monomial of degree <= 5 in 5 variables once (C(10, 5) = 252), the constant included
+        assert r.n_terms == 252 and e.sum(axis=1).max() == 5
+        assert len({tuple(x) for x in e}) == 252
+
+
 def test_central_ray_stays_on_axis(oracle, maps):
     L = oracle.lib()
     L.or_transp_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-    for ptr, n in ((maps.hms_fwd, 12), (maps.shms_fwd, 32)):
+    for ptr, n in ((maps.hms_fwd, 12), (maps.shms_fwd, 32), (maps.sos_fwd, 10), (maps.hrsr_fwd, 12)):
         for cls in range(1, n + 1):
             out = (C.c_double * 5)()
             L.or_transp_eval(C.cast(ptr, C.c_void_p), cls, (C.c_double * 5)(0, 0, 0, 0, 0), out)
