@@ -47,6 +47,8 @@ extern "C" {
 
 #define SIMC_ARM_HMS 1        /* spectrometer numbers of transp_init, transp.f:300-313 */
 #define SIMC_ARM_SOS 2
+#define SIMC_ARM_HRSR 3       /* HRS-R (the reference's former hadron arm)               */
+#define SIMC_ARM_HRSL 4       /* HRS-L (the reference's former electron arm)             */
 #define SIMC_ARM_SHMS 5
 
 #define SIMC_RNG_PHILOX 1     /* counter-based: uniform k of event e = Philox4x32-10(key=seed, ctr=(k/4, e, tag)) */
@@ -58,6 +60,8 @@ extern "C" {
 #define SIMC_HMS_NCOUNTERS 22  /* common /spectrometer1/,      hms/struct_hms.inc:34-42   */
 #define SIMC_SHMS_NCOUNTERS 52 /* common /spectrometer_shms/, shms/struct_shms.inc:26-44 */
 #define SIMC_SOS_NCOUNTERS 18  /* common /spectrometer2/,     sos/struct_sos.inc:9-17    */
+#define SIMC_HRS_NCOUNTERS 21  /* common /spectrometer3/ (HRS-R), /spectrometer4/ (HRS-L),
+                                  hrsr/struct_hrsr.inc:24-40, hrsl/struct_hrsl.inc:24-40  */
 
 typedef struct simc_ctx simc_ctx;
 
@@ -107,15 +111,17 @@ typedef struct {
 
 /* transp_init + the recon readers for the arms.  map_dir holds
  * hms/forward_cosy.dat, hms/recon_cosy.dat, shms/shms_forward.dat,
- * shms/shms_recon.dat and, optionally, sos/forward_cosy.dat and
- * sos/recon_cosy.dat (the reference's relative paths, transp.f:300-309,
- * hms/mc_hms_recon.f:73, shms/mc_shms_recon.f:74, sos/mc_sos_recon.f:73);
- * the SOS maps are loaded when present. */
+ * shms/shms_recon.dat and, optionally, sos/forward_cosy.dat,
+ * sos/recon_cosy.dat, hrsr/hrs_forward_cosy.dat, hrsr/hrs_recon_cosy.dat,
+ * hrsl/hrs_forward_cosy.dat and hrsl/hrs_recon_cosy.dat (the reference's
+ * relative paths, transp.f:300-309, hms/mc_hms_recon.f:73,
+ * shms/mc_shms_recon.f:74, sos/mc_sos_recon.f:73, hrsr/mc_hrsr_recon.f:74,
+ * hrsl/mc_hrsl_recon.f:74); the optional arms are loaded when present. */
 int simc_ctx_create(const char *map_dir, simc_ctx **out);
 int simc_ctx_create_files(const char *hms_forward, const char *hms_recon,
                           const char *shms_forward, const char *shms_recon, simc_ctx **out);
 /* Load (or replace) one arm's maps: transp_init(spectr) and the recon reader
- * of that arm (arm = SIMC_ARM_HMS, SIMC_ARM_SOS or SIMC_ARM_SHMS). */
+ * of that arm (arm = SIMC_ARM_HMS, _SOS, _HRSR, _HRSL or _SHMS). */
 int simc_ctx_load_arm(simc_ctx *ctx, int arm, const char *forward, const char *recon);
 void simc_ctx_destroy(simc_ctx *ctx);
 
@@ -134,6 +140,15 @@ int simc_shms_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *i
  * sos/mc_sos.f:1-3 (no collimator stepping: using_coll is ignored). */
 int simc_sos_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+/* mc_hrsr / mc_hrsl (same argument list as mc_sos), hrsr/mc_hrsr.f:1-3,
+ * hrsl/mc_hrsl.f:1-3 (no collimator stepping: using_coll is ignored).  y_fp of
+ * an accepted event is the fitted value minus the VDC offset (0.48 cm HRS-R,
+ * 0.78 cm HRS-L, hrsr/mc_hrsr.f:524, hrsl/mc_hrsl.f:525); the reconstruction
+ * uses the unshifted track, as the reference does. */
+int simc_hrsr_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
+                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+int simc_hrsl_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *in,
+                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 
@@ -175,7 +190,7 @@ int simc_arm_run_timed(simc_ctx *ctx, int arm, const simc_settings *s, const sim
  * sharing a seed need disjoint event-id ranges; settings are launched
  * together when they share arm, flags and seed. */
 typedef struct {
-    int32_t arm;            /* SIMC_ARM_HMS or SIMC_ARM_SHMS                    */
+    int32_t arm;            /* SIMC_ARM_HMS, _SOS, _HRSR, _HRSL or _SHMS        */
     int32_t reserved;
     simc_settings s;        /* this setting's scalars                           */
     uint64_t seed;          /* Philox key of this setting's streams             */

@@ -26,6 +26,8 @@ MAP_DIR = os.path.join(REPO, "data", "maps")
 
 HMS = 1
 SOS = 2
+HRSR = 3
+HRSL = 4
 SHMS = 5
 RNG_TAPE = 0
 RNG_PHILOX = 1
@@ -48,12 +50,18 @@ SHMS_COUNTERS = [
 SOS_COUNTERS = [
     "trials", "slit_hor", "slit_vert", "slit_oct", "quad_in", "quad_mid", "quad_out", "bm01_in",
     "bm01_out", "bm02_in", "bm02_out", "exit", "hut", "dc1", "dc2", "scin", "cal", "successes"]
-COUNTERS = {HMS: HMS_COUNTERS, SOS: SOS_COUNTERS, SHMS: SHMS_COUNTERS}
+# common /spectrometer3/ (HRS-R) and /spectrometer4/ (HRS-L), hrsr/struct_hrsr.inc:24-40
+HRS_COUNTERS = [
+    "trials", "slit_hor", "slit_vert", "Q1_in", "Q1_mid", "Q1_out", "Q2_in", "Q2_mid", "Q2_out",
+    "Q3_in", "Q3_mid", "Q3_out", "D1_in", "D1_out", "hut", "dc1", "dc2", "s1", "s2", "cal",
+    "successes"]
+COUNTERS = {HMS: HMS_COUNTERS, SOS: SOS_COUNTERS, HRSR: HRS_COUNTERS, HRSL: HRS_COUNTERS,
+            SHMS: SHMS_COUNTERS}
 
 # Every exported C entry point (include/simc_gpu.h).
 EXPORTS = [
     "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_load_arm", "simc_ctx_destroy",
-    "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
+    "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_hrsr_run", "simc_hrsl_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
@@ -202,8 +210,10 @@ def _dev_f64(t, n, name):
 
 
 class Spectrometers:
-    """A context: the COSY maps of both arms on the device plus stop counters
-    (transp_init and the recon readers, shared/transp.f:294-478)."""
+    """A context: the COSY maps of the arms on the device plus stop counters
+    (transp_init and the recon readers, shared/transp.f:294-478).  HMS and SHMS
+    maps are required; SOS, HRS-R and HRS-L maps are loaded when map_dir has
+    them."""
 
     def __init__(self, map_dir: str = MAP_DIR, files=None):
         L = library()
@@ -461,6 +471,24 @@ def mc_shms(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_fla
         raise SimcError(-6, "mc_shms: spectr must be 5 (the LSA tune is no longer supported)")
     return _arm_call(ctx, SHMS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
                      decay_flag, fry, pathlen, using_SHMScoll, ctau, **kw)
+
+
+def mc_hrsr(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+            fry, pathlen=None, ctau=0.0, **kw):
+    """Vectorised mc_hrsr (hrsr/mc_hrsr.f:1-3); y_fp of accepted events carries
+    the VDC offset (hrsr/mc_hrsr.f:524)."""
+    del th_spec
+    return _arm_call(ctx, HRSR, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
+                     decay_flag, fry, pathlen, False, ctau, **kw)
+
+
+def mc_hrsl(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,
+            fry, pathlen=None, ctau=0.0, **kw):
+    """Vectorised mc_hrsl (hrsl/mc_hrsl.f:1-3); y_fp of accepted events carries
+    the VDC offset (hrsl/mc_hrsl.f:525)."""
+    del th_spec
+    return _arm_call(ctx, HRSL, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
+                     decay_flag, fry, pathlen, False, ctau, **kw)
 
 
 def mc_sos(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag, decay_flag,

@@ -1,9 +1,10 @@
 """Build-time generator of the COSY polynomial device code.
 
 Reads the forward and reconstruction map files (the same files mc_hms /
-mc_shms / mc_sos load at run time: hms/forward_cosy.dat, hms/recon_cosy.dat,
-shms/shms_forward.dat, shms/shms_recon.dat, sos/forward_cosy.dat,
-sos/recon_cosy.dat) and writes cosy_gen.inc:
+mc_shms / mc_sos / mc_hrsr / mc_hrsl load at run time: hms/forward_cosy.dat,
+hms/recon_cosy.dat, shms/shms_forward.dat, shms/shms_recon.dat,
+sos/forward_cosy.dat, sos/recon_cosy.dat, hrsr/hrs_forward_cosy.dat,
+hrsr/hrs_recon_cosy.dat -- the HRS-L files are the same) and writes cosy_gen.inc:
 
 * one __device__ function per non-drift forward class and per recon map that
   evaluates  out_k = sum_i c_{k,i} prod_v r_v^{e_{v,i}}  (shared/transp.f:205-218,
@@ -103,6 +104,7 @@ TRANSP_CLASSES = {
     "hms": [2, 3, 5, 6, 8, 9, 11],
     "shms": [3, 7, 8, 12, 13, 17, 18, 23, 24, 25, 26, 27, 28, 29, 30],
     "sos": [2, 3, 4, 5, 6, 7, 8, 9],          # sos/mc_sos.f:219-329
+    "hrs": [2, 3, 5, 6, 8, 10, 11],           # hrsl/mc_hrsl.f:244-466 (HRS-R and HRS-L share the map)
 }
 
 
@@ -422,6 +424,8 @@ def generate(map_dir, out_path, block=COEF_BLOCK):
          os.path.join(map_dir, "shms", "shms_recon.dat")),
         ("sos", os.path.join(map_dir, "sos", "forward_cosy.dat"),
          os.path.join(map_dir, "sos", "recon_cosy.dat")),
+        ("hrs", os.path.join(map_dir, "hrsr", "hrs_forward_cosy.dat"),
+         os.path.join(map_dir, "hrsr", "hrs_recon_cosy.dat")),
     ]
     dev, host = [], []
     dev.append("// Generated by gen_cosy.py from the COSY map files; do not edit.")

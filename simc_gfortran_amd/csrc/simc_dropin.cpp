@@ -1,22 +1,26 @@
 // Per-event drop-in replacements of the reference's Fortran subroutines
-// mc_hms, mc_shms, mc_sos and their recon routines (gfortran/flang ABI:
+// mc_hms, mc_shms, mc_sos, mc_hrsr, mc_hrsl and their recon routines (gfortran/flang ABI:
 // lower-case names with a trailing underscore, every argument by reference,
 // LOGICAL as 4-byte integers), built as libsimc_dropin.so on top of
 // libsimc_gpu.so.  Linking a SIMC executable against this library instead of
-// the objects of hms/mc_hms*.f, shms/mc_shms*.f and sos/mc_sos*.f keeps every call site of
+// the objects of hms/mc_hms*.f, shms/mc_shms*.f, sos/mc_sos*.f and hrs?/mc_hrs?*.f keeps every call site of
 // simc.f unchanged; each call transports ONE event on the GPU (synchronous),
 // so this path is for compatibility -- throughput needs the batched entry
 // points of simc_gpu.h.
 //
 // State shared with the caller's program through its common blocks, when
 // they exist (weak references; a program without them still links):
-//   /spectrometer1/, /spectrometer_shms/, /spectrometer2/  STOP counters,
-//            incremented per event (hms/struct_hms.inc:34-42,
-//            shms/struct_shms.inc:26-44, sos/struct_sos.inc:22-35);
+//   /spectrometer1/, /spectrometer_shms/, /spectrometer2/, /spectrometer3/,
+//   /spectrometer4/  STOP counters, incremented per event (hms/struct_hms.inc:34-42,
+//            shms/struct_shms.inc:26-44, sos/struct_sos.inc:22-35,
+//            hrsr/struct_hrsr.inc:24-40, hrsl/struct_hrsl.inc:24-40);
 //   /track/  xs, ys, zs, dpps, dxdzs, dydzs (spectrometers.inc:47-60): set to
 //            the fitted focal-plane track of an accepted event, as
 //            hms/mc_hms.f:419-423 does, and read by the recon entry points
-//            (hms/mc_hms_recon.f:113-118);
+//            (hms/mc_hms_recon.f:113-118).  The HRS arms return y_fp minus the
+//            VDC offset while /track/ keeps the fitted y (hrsl/mc_hrsl.f:512-525);
+//            the drop-in adds the offset back, so /track/ ys may differ from the
+//            fitted value by one rounding;
 //   /decd/   ctau, its first member (simulate.inc:153).
 //
 // Random numbers: a counter-based Philox4x32-10 stream per event, keyed by
@@ -36,6 +40,8 @@ extern "C" {
 extern int32_t spectrometer1_[SIMC_HMS_NCOUNTERS] __attribute__((weak));
 extern int32_t spectrometer_shms_[SIMC_SHMS_NCOUNTERS] __attribute__((weak));
 extern int32_t spectrometer2_[SIMC_SOS_NCOUNTERS] __attribute__((weak));
+extern int32_t spectrometer3_[SIMC_HRS_NCOUNTERS] __attribute__((weak));
+extern int32_t spectrometer4_[SIMC_HRS_NCOUNTERS] __attribute__((weak));
 extern double track_[6] __attribute__((weak));
 extern double decd_[1] __attribute__((weak));
 }
@@ -69,10 +75,13 @@ void add_counters(int arm)
 {
     const int nc = arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS
                  : arm == SIMC_ARM_SOS ? SIMC_SOS_NCOUNTERS
-                                       : SIMC_SHMS_NCOUNTERS;
+                 : arm == SIMC_ARM_SHMS ? SIMC_SHMS_NCOUNTERS
+                                        : SIMC_HRS_NCOUNTERS;
     int32_t* common = arm == SIMC_ARM_HMS ? spectrometer1_
                     : arm == SIMC_ARM_SOS ? spectrometer2_
-                                          : spectrometer_shms_;
+                    : arm == SIMC_ARM_HRSR ? spectrometer3_
+                    : arm == SIMC_ARM_HRSL ? spectrometer4_
+                                           : spectrometer_shms_;
     if (!common) return;
     int64_t c[SIMC_SHMS_NCOUNTERS];
     if (simc_counters_get(g_ctx, arm, c, nc) != SIMC_OK) {
@@ -121,7 +130,7 @@ void arm_event(int arm, double* p_spec, double* dpp, double* x, double* y, doubl
     *ok_spec = ok ? 1 : 0;
     if (ok && track_) {  // hms/mc_hms.f:419-423
         track_[0] = *x_fp;
-        track_[1] = *y_fp;
+        track_[1] = arm == SIMC_ARM_HRSR ? *y_fp + 0.48 : arm == SIMC_ARM_HRSL ? *y_fp + 0.78 : *y_fp;
         track_[4] = *dx_fp;
         track_[5] = *dy_fp;
     }
@@ -201,6 +210,41 @@ void mc_sos_(double* p_spec, double* th_spec, double* dpp, double* x, double* y,
 void mc_sos_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
 {
     recon_event(SIMC_ARM_SOS, delta_p, delta_t, delta_phi, y_tgt, fry);
+}
+
+// mc_hrsr / mc_hrsl (same arguments as mc_sos), hrsr/mc_hrsr.f:1-3, hrsl/mc_hrsl.f:1-3
+void mc_hrsr_(double* p_spec, double* th_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+              double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2,
+              int32_t* ms_flag, int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry,
+              int32_t* ok_spec, double* pathlen)
+{
+    (void)th_spec;
+    int32_t no_coll = 0;
+    arm_event(SIMC_ARM_HRSR, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag,
+              wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen, &no_coll);
+}
+
+void mc_hrsl_(double* p_spec, double* th_spec, double* dpp, double* x, double* y, double* z, double* dxdz,
+              double* dydz, double* x_fp, double* dx_fp, double* y_fp, double* dy_fp, double* m2,
+              int32_t* ms_flag, int32_t* wcs_flag, int32_t* decay_flag, double* resmult, double* fry,
+              int32_t* ok_spec, double* pathlen)
+{
+    (void)th_spec;
+    int32_t no_coll = 0;
+    arm_event(SIMC_ARM_HRSL, p_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2, ms_flag,
+              wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen, &no_coll);
+}
+
+// mc_hrsr_recon / mc_hrsl_recon(delta_p, delta_t, delta_phi, y_tgt, fry),
+// hrsr/mc_hrsr_recon.f:1, hrsl/mc_hrsl_recon.f:1
+void mc_hrsr_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
+{
+    recon_event(SIMC_ARM_HRSR, delta_p, delta_t, delta_phi, y_tgt, fry);
+}
+
+void mc_hrsl_recon_(double* delta_p, double* delta_t, double* delta_phi, double* y_tgt, double* fry)
+{
+    recon_event(SIMC_ARM_HRSL, delta_p, delta_t, delta_phi, y_tgt, fry);
 }
 
 // mc_hms_recon(delta_p, delta_t, delta_phi, y_tgt, fry), hms/mc_hms_recon.f:1

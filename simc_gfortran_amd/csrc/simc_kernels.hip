@@ -49,6 +49,11 @@ enum : int {
     ST_SO_QUAD_IN = 100, ST_SO_QUAD_MID, ST_SO_QUAD_OUT, ST_SO_BM01_IN, ST_SO_BM01_OUT,
     ST_SO_BM02_IN, ST_SO_BM02_OUT, ST_SO_EXIT_FLANGE, ST_SO_EXIT_APERTURE, ST_SO_EXIT_WINDOW,
     ST_SO_DC1, ST_SO_DC2, ST_SO_SCIN,
+    // HRS (both arms)
+    ST_R_PIPE_65 = 120, ST_R_PIPE_80, ST_R_Q1_PRE, ST_R_Q1_IN, ST_R_Q1_MID, ST_R_Q1_OUT,
+    ST_R_Q1_POST, ST_R_Q2_PRE, ST_R_Q2_IN, ST_R_Q2_MID, ST_R_Q2_OUT, ST_R_Q2_POST1, ST_R_Q2_POST2,
+    ST_R_D1_PRE, ST_R_D1_IN, ST_R_D1_OUT, ST_R_D1_POST_R, ST_R_D1_POST_BOX, ST_R_Q3_PRE, ST_R_Q3_IN,
+    ST_R_Q3_MID, ST_R_Q3_OUT, ST_R_Q3_POST, ST_R_WINDOW, ST_R_DC1, ST_R_DC2, ST_R_S1, ST_R_S2,
     ST_TAPE = 250, ST_BADOP = 251
 };
 constexpr int kHistBins = 256;
@@ -63,6 +68,9 @@ struct ArmTables {
     // straight-line fit as a linear map of the hits (all planes of the
     // plane's coordinate present): slope = sum fw_a h, intercept = sum fw_b h
     double fw_a[12], fw_b[12];
+    // subtracted from y_fp of an accepted event after the reconstruction
+    // (the HRS VDC offset, hrsl/mc_hrsl.f:525, hrsr/mc_hrsr.f:524)
+    double yfp_shift;
 };
 
 // One kinematic setting: the by-value arguments of mc_hms / mc_shms and the
@@ -139,6 +147,13 @@ __device__ __forceinline__ void poly_fwd(int cls, cdptr c, double r0, double r1,
             default: cosy_sos_fwd_c9(c + COSY_SOS_C9_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
         }
 #undef OFWD
+    } else if (ARM == SIMC_ARM_HRSR) {  // both HRS arms (one map structure)
+#define RFWD(N) case N: cosy_hrs_fwd_c##N(c + COSY_HRS_C##N##_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        switch (cls) {
+            RFWD(2) RFWD(3) RFWD(5) RFWD(6) RFWD(8) RFWD(10)
+            default: cosy_hrs_fwd_c11(c + COSY_HRS_C11_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
+        }
+#undef RFWD
     } else {
 #define SFWD(N) case N: cosy_shms_fwd_c##N(c + COSY_SHMS_C##N##_OFF, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4); break;
         switch (cls) {
@@ -156,6 +171,7 @@ __device__ __forceinline__ void poly_recon(cdptr c, double h0, double h1, double
 {
     if (ARM == SIMC_ARM_HMS) cosy_hms_recon(c + COSY_HMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
     else if (ARM == SIMC_ARM_SOS) cosy_sos_recon(c + COSY_SOS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
+    else if (ARM == SIMC_ARM_HRSR) cosy_hrs_recon(c + COSY_HRS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
     else cosy_shms_recon(c + COSY_SHMS_RECON_OFF, h0, h1, h2, h3, h4, s0, s1, s2, s3);
 }
 
@@ -186,7 +202,8 @@ __device__ __forceinline__ void poly_fwd_tan(int cls, cdptr c, const double (&r)
 }
 
 // Reconstruction (hms/mc_hms_recon.f:113-137, shms/mc_shms_recon.f:131-160,
-// sos/mc_sos_recon.f:113-137: the SOS keeps every input away from 0)
+// hrsl/mc_hrsl_recon.f:113-137, sos/mc_sos_recon.f:113-137: the SOS keeps
+// every input away from 0, the others only x_tar)
 template <int ARM>
 __device__ __forceinline__ void recon(cdptr c, const EvOut& o, double fry, EvOut& r)
 {
@@ -273,13 +290,14 @@ __device__ __forceinline__ void fit_planes(const int (&xpl)[NX], const int (&ypl
     dy_fp = (double)a; y_fp = (double)b;
 }
 
-// The SOS measures y on planes 1, 3, 5 of each chamber (sos/mc_sos_hut.f:324-329),
-// the HMS and SHMS on planes 2 and 5.
+// The SOS and the HRS measure y on planes 1, 3, 5 of each chamber
+// (sos/mc_sos_hut.f:324-329, hrsl/mc_hrsl_hut.f:275-279), the HMS and SHMS on
+// planes 2 and 5.
 template <int ARM = SIMC_ARM_HMS>
 __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, const double (&zdc)[12],
                                           double& x_fp, double& dx_fp, double& y_fp, double& dy_fp)
 {
-    if constexpr (ARM == SIMC_ARM_SOS) {
+    if constexpr (ARM == SIMC_ARM_SOS || ARM == SIMC_ARM_HRSR) {
         const int xpl[6] = {1, 3, 5, 7, 9, 11};
         const int ypl[6] = {0, 2, 4, 6, 8, 10};
         if (F.fit_f32) fit_planes<float>(xpl, ypl, lds, zdc, x_fp, dx_fp, y_fp, dy_fp);
@@ -403,6 +421,12 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
             case OP_CHECK_ABS:
                 if ((fabs(t.xs) > p0) || fabs(t.ys) > p1) st = ops[pc].stop;
                 break;
+            case OP_CHECK_R:
+                if (sqrt(t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
+                break;
+            case OP_CHECK_T_SLAB:
+                if ((fabs(xt - p0) > p1) || ((fabs(yt) + ops[pc].p[2] * xt) > ops[pc].p[3])) st = ops[pc].stop;
+                break;
             default:
                 st = ST_BADOP;  // not a magnet step (excluded by the host's check)
                 break;
@@ -470,6 +494,14 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
                 st64(a.dx_fp, a.n_arr, off, dx_fp);
                 st64(a.y_fp, a.n_arr, off, y_fp);
                 st64(a.dy_fp, a.n_arr, off, dy_fp);
+                break;
+            }
+            case OP_BOX_ROT_H: {
+                const Rot r{p0, p1, ops[pc].p[2]};
+                double xr = t.xs, yr = t.ys;
+                rotate_haxis(t, r, xr, yr);
+                if (xr > ops[pc].p[3] || xr < ops[pc].p[4] || yr > ops[pc].p[5] || yr < ops[pc].p[6])
+                    st = ops[pc].stop;
                 break;
             }
             case OP_FIDUCIAL: {
@@ -572,6 +604,7 @@ __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int s
         o.y_fp = ld64(a.y_fp, n, off);
         o.dy_fp = ld64(a.dy_fp, n, off);
         recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
+        if (ARM == SIMC_ARM_HRSR) st64(a.y_fp, n, off, o.y_fp - ((ctptr)a.tab)->yfp_shift);
         st64(a.o_dpp, n, off, r.dpp);
         st64(a.o_y, n, off, r.y);
         st64(a.o_dxdz, n, off, r.dxdz);
@@ -1236,7 +1269,7 @@ int fail(int code, const std::string& msg)
 
 }  // namespace
 
-constexpr int kNumArms = 3;  // HMS, SHMS, SOS (arm_index)
+constexpr int kNumArms = 5;  // HMS, SHMS, SOS, HRS-R, HRS-L (arm_index)
 
 struct simc_ctx {
     simc::ForwardMap fwd[kNumArms];
@@ -1295,13 +1328,17 @@ namespace {
 
 int arm_index(int arm)
 {
-    return arm == SIMC_ARM_HMS ? 0 : arm == SIMC_ARM_SHMS ? 1 : arm == SIMC_ARM_SOS ? 2 : -1;
+    return arm == SIMC_ARM_HMS ? 0 : arm == SIMC_ARM_SHMS ? 1 : arm == SIMC_ARM_SOS ? 2
+         : arm == SIMC_ARM_HRSR ? 3 : arm == SIMC_ARM_HRSL ? 4 : -1;
 }
+
+bool is_hrs(int arm) { return arm == SIMC_ARM_HRSR || arm == SIMC_ARM_HRSL; }
 
 // STOP counters of an arm's common block
 int ncounters_of(int arm)
 {
-    return arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : arm == SIMC_ARM_SOS ? SIMC_SOS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
+    return arm == SIMC_ARM_HMS ? SIMC_HMS_NCOUNTERS : arm == SIMC_ARM_SOS ? SIMC_SOS_NCOUNTERS
+         : is_hrs(arm)          ? SIMC_HRS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
 }
 
 // Step-program builder.  All parameters are formed with the reference's own
@@ -1610,6 +1647,130 @@ void build_sos(const simc::ForwardMap& f, Program& P, double zdc[12])
     P.add(OP_END);
 }
 
+// mc_hrsr / mc_hrsl (hrsr/mc_hrsr.f:155-526, hrsl/mc_hrsl.f:155-527, apertures
+// hrsl/apertures_hrsl.inc:6-8) followed by their hut (hrsl/mc_hrsl_hut.f:229-456,
+// constants :37-185).  The arms differ in the slit position and exit half width
+// (hrsr/mc_hrsr.f:58,67; hrsl/mc_hrsl.f:58,67); their huts are the same.
+void build_hrs(int arm, const simc::ForwardMap& f, Program& P, double zdc[12])
+{
+    const bool left = arm == SIMC_ARM_HRSL;
+    const double h_entr = 3.145, v_entr = 6.090, v_exit = 6.485, h_exit = left ? 3.335 : 3.340;
+    const double z_off = 0.0, z_entr = (left ? 110.9e0 : 110.0e0) + z_off, z_exit = z_entr + 8.0e0;
+    const double rq1 = 15.0 * 15.0, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;
+    const double* dd = f.driftdist.data();
+    double ztmp, zdrift;
+    auto slit = [&](double h, double v) {  // |ys - y_off| > h, then |xs - x_off| > v (offsets 0)
+        P.add(OP_CHECK_ABS, {kInf, h}, ST_SLIT_HOR);
+        P.add(OP_CHECK_ABS, {v, kInf}, ST_SLIT_VERT);
+    };
+    auto d1_plane = [&](double ang, int stop) {  // rotated D1 aperture plane, :367-383, :389-405
+        P.rot(OP_ROT_H, ang, 0.0, 0.0);
+        P.add(OP_CHECK_T_SLAB, {2.500, 52.5, 0.01861, 12.5}, stop);
+    };
+    zdrift = 65.686; ztmp = zdrift;                                                   // :159-168
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {7.3787}, ST_R_PIPE_65);
+    zdrift = 80.436 - ztmp; ztmp = 80.436;                                            // :170-179
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {7.4092}, ST_R_PIPE_80);
+    P.add(OP_PROJECT, {z_entr - ztmp});  slit(h_entr, v_entr);                       // :184-199
+    P.add(OP_PROJECT, {z_exit - z_entr});  slit(h_exit, v_exit);                     // :203-218
+    ztmp = 135.064;                                                                   // :222-230
+    P.add(OP_PROJECT, {ztmp - z_exit});  P.add(OP_CHECK_R, {12.5222}, ST_R_Q1_PRE);
+    P.add(OP_PROJECT, {dd[0] - ztmp});  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_IN);       // :234-243
+    P.add(OP_TRANSP, {62.75333333e0}, 0, 2);  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_MID); // :247-254
+    P.add(OP_TRANSP, {31.37666667e0}, 0, 3);  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_OUT); // :258-265
+    zdrift = 300.464 - 253.16; ztmp = zdrift;                                         // :270-278
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {14.9225}, ST_R_Q1_POST);
+    zdrift = 314.464 - 300.464; ztmp = ztmp + zdrift;                                 // :281-289
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {20.9550}, ST_R_Q2_PRE);
+    P.add(OP_PROJECT, {dd[3] - ztmp});  P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_IN);       // :293-302
+    P.add(OP_TRANSP, {121.77333333e0}, 0, 5);  P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_MID);
+    P.add(OP_TRANSP, {60.88666667e0}, 0, 6);   P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_OUT);
+    zdrift = 609.664 - 553.020; ztmp = zdrift;                                        // :329-337
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.0073}, ST_R_Q2_POST1);
+    zdrift = 641.800 - 609.664; ztmp = ztmp + zdrift;                                 // :340-348
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.0073}, ST_R_Q2_POST2);
+    zdrift = 819.489 - 641.800; ztmp = ztmp + zdrift;                                 // :351-359
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {50.0, 15.0}, ST_R_D1_PRE);
+    P.add(OP_PROJECT, {dd[6] - ztmp});  d1_plane(-30.0, ST_R_D1_IN);                 // :364-383
+    P.add(OP_TRANSP, {659.73445725e0}, 0, 8);  d1_plane(30.0, ST_R_D1_OUT);          // :388-405
+    zdrift = 1745.33546 - 1655.83446; ztmp = zdrift;                                  // :411-426
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.3276}, ST_R_D1_POST_R);
+    P.add(OP_CHECK_ABS, {50.0, 15.0}, ST_R_D1_POST_BOX);
+    zdrift = 1759.00946 - 1745.33546; ztmp = ztmp + zdrift;                           // :428-437
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.3276}, ST_R_Q3_PRE);
+    P.add(OP_PROJECT, {dd[8] - ztmp});  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_IN);       // :442-451
+    P.add(OP_TRANSP, {121.7866667e0}, 0, 10);  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_MID);
+    P.add(OP_TRANSP, {60.89333333e0}, 0, 11);  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_OUT);
+    zdrift = 2080.38746 - 1997.76446; ztmp = zdrift;                                  // :477-486
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {35.56, 17.145}, ST_R_Q3_POST);
+    zdrift = 2327.47246 - 2080.38746; ztmp = ztmp + zdrift;                           // :490-499
+    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {99.76635, 17.145}, ST_R_WINDOW);
+
+    // the hut from zinit = -(driftdist(12) - ztmp) (:505-510)
+    const double zinit = -(dd[11] - ztmp);
+    const double hfoil_exit_radlen = 3.56, hfoil_exit_thick = 0.01;
+    const double hair_radlen = 30420.;
+    const double hdc_entr_radlen = 34.4, hdc_entr_thick = 0.00018 * 2.54;
+    const double hdc_radlen = 16700.0, hdc_thick = 1.5;
+    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
+    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
+    const double hdc_exit_radlen = 34.4, hdc_exit_thick = 0.00018 * 2.54;
+    const double hscin_radlen = 42.4;
+    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54, hcer_radlen = 36620.0;
+    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
+    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
+    const int hdc_nr_plan = 6;
+    const double hdc_1_zpos = -25.0 + 25.0, hdc_2_zpos = 25.0 + 25.0;
+    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
+    const double hdc_left = 14.4, hdc_right = -14.4, hdc_top = -105.6, hdc_bot = 105.6;
+    const double hdc_x_offset = 0.000, hdc_y_offset = 0.000;
+    const double hscin_1x_zpos = 95.0 + 25.0, hscin_2x_zpos = 288.3 + 25.0;
+    const double hscin_1x_thick = 0.5 * 1.067, hscin_2x_thick = 0.5 * 1.067;
+    const double hcer_zentrance = 137.0 + 25.0, hcer_zmirror = 197.0 + 25.0, hcer_zexit = 237.0 + 25.0;
+    const double hcal_4ta_zpos = 407.3 + 25.0;
+    const double raddeg = 0.017453292, r45 = 45.0e0 * raddeg;  // shared/rotate_haxis.f:41-51
+    double drift;
+    auto air = [&](double d, double radlen) {
+        P.add(OP_PROJECT, {d});
+        P.add(OP_MUSC_EXT, {d / radlen, d});
+    };
+
+    P.add(OP_RESMULT_ONE);                                                          // :231
+    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});                         // :240-241
+    for (int j = 1; j <= 2; ++j) {
+        drift = j == 1 ? (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - zinit   // :247-250
+                       : hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;     // :310-313
+        air(drift, hair_radlen);
+        P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});
+        chamber(P, j, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
+                hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick, 0x15u);            // :256-285
+        P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
+        P.add(OP_BOX_ROT_H, {std::tan(r45), std::sin(r45), std::cos(r45), hdc_bot - hdc_x_offset,
+                             hdc_top - hdc_x_offset, hdc_left - hdc_y_offset, hdc_right - hdc_y_offset},
+              j == 1 ? ST_R_DC1 : ST_R_DC2);                                        // :289-304, :352-367
+        P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
+    }
+    for (int i = 0; i < 6; ++i) {                                                   // :373-381
+        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
+    }
+    P.add(OP_FIT);                                                                  // :383-389
+    air(hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane, hair_radlen);  // :393-405
+    P.add(OP_BOX, {kInf, -kInf, 18.0, -18.0}, ST_R_S1);
+    P.add(OP_MUSC, {hscin_1x_thick / hscin_radlen});
+    air(hcer_zentrance - hscin_1x_zpos, hair_radlen);                               // :410-416
+    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
+    air(hcer_zmirror - hcer_zentrance, hcer_radlen);                                // :418-424
+    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
+    air(hcer_zexit - hcer_zmirror, hcer_radlen);                                    // :426-432
+    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
+    air(hscin_2x_zpos - hcer_zexit, hair_radlen);                                   // :436-448
+    P.add(OP_BOX, {kInf, -kInf, 30.0, -30.0}, ST_R_S2);
+    P.add(OP_MUSC, {hscin_2x_thick / hscin_radlen});
+    air(hcal_4ta_zpos - hscin_2x_zpos, hair_radlen);                                // :453-456
+    P.add(OP_END);
+}
+
 void build_shms(Program& P, double zdc[12])
 {
     const double zd_hbin = 118.39, zd_hbmen = 17.61, zd_hbmex = 80.0, zd_hbout = 17.61;  // :176-179
@@ -1775,6 +1936,8 @@ ArmTables make_tables(const simc_ctx* ctx, int arm)
         T.coll = CollGeom{4.575, 11.646, 4.759, 12.114, 0.000, +0.028, 0.41753, 6.30 / 20};  // hms/mc_hms_coll.f:22-53
     else
         T.coll = CollGeom{8.50, 12.50, 8.65, 12.85, 0.000, +0.000, 0.42084, 6.35 / 20};      // shms/mc_shms_coll.f:23-54
+    // VDC offsets: hrsr/mc_hrsr.f:524, hrsl/mc_hrsl.f:525
+    T.yfp_shift = arm == SIMC_ARM_HRSR ? 0.48 : arm == SIMC_ARM_HRSL ? 0.78 : 0.0;
     std::memcpy(T.zdc, ctx->zdc[arm_index(arm)], sizeof T.zdc);
     // LFIT (cern/lfit.f:22-59) over x planes 1,3,4,6,7,9,10,12 and y planes
     // 2,5,8,11: A = sum (z - zm)(h - hm) / sum (z - zm)^2, B = hm - A zm.
@@ -1800,8 +1963,8 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     std::string msg;
     int rc = simc::read_forward_map(fwd_path, ctx->fwd[ai], msg);
     if (rc) return fail(rc, msg);
-    // hms/mc_hms.f:188, shms/mc_shms.f:344, sos/mc_sos.f:164
-    const int want = arm == SIMC_ARM_HMS ? 12 : arm == SIMC_ARM_SOS ? 10 : 32;
+    // hms/mc_hms.f:188, shms/mc_shms.f:344, sos/mc_sos.f:164, hrsl/mc_hrsl.f:151
+    const int want = arm == SIMC_ARM_HMS || is_hrs(arm) ? 12 : arm == SIMC_ARM_SOS ? 10 : 32;
     if (ctx->fwd[ai].n_classes != want)
         return fail(SIMC_E_NCLASS, fwd_path + ": " + std::to_string(ctx->fwd[ai].n_classes) +
                                        " transport classes, expected " + std::to_string(want));
@@ -1813,6 +1976,7 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     Program P;
     if (arm == SIMC_ARM_HMS) build_hms(ctx->fwd[ai], P, ctx->zdc[ai]);
     else if (arm == SIMC_ARM_SOS) build_sos(ctx->fwd[ai], P, ctx->zdc[ai]);
+    else if (is_hrs(arm)) build_hrs(arm, ctx->fwd[ai], P, ctx->zdc[ai]);
     else build_shms(P, ctx->zdc[ai]);
     ctx->n_ops[ai] = (int)P.ops.size();
     ctx->ops[ai] = P.ops;
@@ -1825,10 +1989,18 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
             return fail(SIMC_E_INTERNAL, "step " + std::to_string(k) + " outside its stage");
     }
     ctx->hut_pc[ai] = hut;
+    // Stage 1 ends after the collimator; an arm without one (the HRS: slit
+    // apertures only) ends it before its first COSY segment.
     int coll = 0;
     while (coll < hut && P.ops[coll].code != OP_COLLIMATOR) ++coll;
-    if (coll == hut) return fail(SIMC_E_INTERNAL, "no collimator step before the hut");
-    ctx->coll_pc[ai] = coll + 1;
+    if (coll < hut) {
+        ctx->coll_pc[ai] = coll + 1;
+    } else {
+        int first = 0;
+        while (first < hut && P.ops[first].code != OP_TRANSP) ++first;
+        if (first == 0 || first == hut) return fail(SIMC_E_INTERNAL, "no stage split before the hut");
+        ctx->coll_pc[ai] = first;
+    }
     HIPCHK(hipMalloc(&ctx->d_coef[ai], packed.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_coef[ai], packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&ctx->d_ops[ai], P.ops.size() * sizeof(Op)));
@@ -1929,6 +2101,29 @@ void hist_to_counters(int arm, const unsigned long long* h, int64_t* c, int ncou
         c[16] = 0;  // sSTOP_cal: the calorimeter cut is commented out, sos/mc_sos_hut.f:519-538
         c[17] = H(ST_OK);
         c[12] = c[13] + c[14] + c[15] + c[17];
+    } else if (is_hrs(arm)) {
+        // order of common /spectrometer3/ and /spectrometer4/, hrsr/struct_hrsr.inc:24-40
+        c[0] = total;
+        c[1] = H(ST_R_PIPE_65) + H(ST_R_PIPE_80) + H(ST_SLIT_HOR);
+        c[2] = H(ST_SLIT_VERT);
+        c[3] = H(ST_R_Q1_PRE) + H(ST_R_Q1_IN);
+        c[4] = H(ST_R_Q1_MID);
+        c[5] = H(ST_R_Q1_OUT) + H(ST_R_Q1_POST);
+        c[6] = H(ST_R_Q2_PRE) + H(ST_R_Q2_IN);
+        c[7] = H(ST_R_Q2_MID);
+        c[8] = H(ST_R_Q2_OUT) + H(ST_R_Q2_POST1) + H(ST_R_Q2_POST2);
+        c[9] = H(ST_R_Q3_PRE) + H(ST_R_Q3_IN);
+        c[10] = H(ST_R_Q3_MID);
+        c[11] = H(ST_R_Q3_OUT) + H(ST_R_Q3_POST) + H(ST_R_WINDOW);
+        c[12] = H(ST_R_D1_PRE) + H(ST_R_D1_IN);
+        c[13] = H(ST_R_D1_OUT) + H(ST_R_D1_POST_R) + H(ST_R_D1_POST_BOX);
+        c[15] = H(ST_R_DC1);
+        c[16] = H(ST_R_DC2);
+        c[17] = H(ST_R_S1);
+        c[18] = H(ST_R_S2);
+        c[19] = 0;  // the calorimeter cuts are commented out, hrsl/mc_hrsl_hut.f:463-490
+        c[20] = H(ST_OK);
+        c[14] = c[15] + c[16] + c[17] + c[18] + c[20];
     } else {
         // order of common /spectrometer_shms/, shms/struct_shms.inc:26-44
         c[0] = total;
@@ -1995,11 +2190,19 @@ int simc_ctx_create(const char* map_dir, simc_ctx** out)
                                    (d + "/shms/shms_forward.dat").c_str(), (d + "/shms/shms_recon.dat").c_str(),
                                    out);
     if (rc) return rc;
-    // the SOS maps are optional: loaded when present (sos/mc_sos.f:160-165)
-    const std::string sf = d + "/sos/forward_cosy.dat", sr = d + "/sos/recon_cosy.dat";
-    if (FILE* f = std::fopen(sf.c_str(), "r")) {
+    // the SOS and HRS maps are optional: loaded when present (sos/mc_sos.f:160-165,
+    // hrsl/mc_hrsl.f:148-153; the reference reads them on an arm's first call)
+    const struct { int arm; const char *fwd, *rec; } optional[] = {
+        {SIMC_ARM_SOS, "/sos/forward_cosy.dat", "/sos/recon_cosy.dat"},
+        {SIMC_ARM_HRSR, "/hrsr/hrs_forward_cosy.dat", "/hrsr/hrs_recon_cosy.dat"},
+        {SIMC_ARM_HRSL, "/hrsl/hrs_forward_cosy.dat", "/hrsl/hrs_recon_cosy.dat"},
+    };
+    for (const auto& o : optional) {
+        const std::string sf = d + o.fwd, sr = d + o.rec;
+        FILE* f = std::fopen(sf.c_str(), "r");
+        if (!f) continue;
         std::fclose(f);
-        rc = simc_ctx_load_arm(*out, SIMC_ARM_SOS, sf.c_str(), sr.c_str());
+        rc = simc_ctx_load_arm(*out, o.arm, sf.c_str(), sr.c_str());
         if (rc) {
             std::string keep = g_err;
             simc_ctx_destroy(*out);
@@ -2015,7 +2218,7 @@ int simc_ctx_load_arm(simc_ctx* ctx, int arm, const char* forward, const char* r
 {
     if (!ctx || !forward || !recon) return fail(SIMC_E_ARG, "simc_ctx_load_arm: NULL argument");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), SIMC_ARM_SOS (2) or SIMC_ARM_SHMS (5)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), _SOS (2), _HRSR (3), _HRSL (4) or _SHMS (5)");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipDeviceSynchronize());  // runs of the arm may still use its tables
     void* old[5] = {ctx->d_coef[ai], ctx->d_ops[ai], ctx->d_tab[ai], ctx->d_rexp[ai], ctx->d_hist[ai]};
@@ -2054,17 +2257,18 @@ namespace {
 int check_settings(int arm, const simc_settings* s)
 {
     if (arm_index(arm) < 0)
-        return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), SIMC_ARM_SOS (2) or SIMC_ARM_SHMS (5)");
+        return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), _SOS (2), _HRSR (3), _HRSL (4) or _SHMS (5)");
     if (!(s->p_spec > 0.0) || !(s->m2 >= 0.0)) return fail(SIMC_E_ARG, "p_spec must be > 0 and m2 >= 0");
     if (s->decay_flag && !(s->ctau > 0.0)) return fail(SIMC_E_ARG, "decay_flag needs ctau > 0");
     return SIMC_OK;
 }
 
 // Collimator stepping applies to m2 in (100^2, 200^2) (hms/mc_hms.f:206,
-// shms/mc_shms.f:503; the SOS has none); m2 changes only through a decay.
+// shms/mc_shms.f:503; the SOS and the HRS have none); m2 changes only through
+// a decay.
 bool coll_stepping(int arm, const simc_settings* s)
 {
-    return arm != SIMC_ARM_SOS && s->using_coll;
+    return (arm == SIMC_ARM_HMS || arm == SIMC_ARM_SHMS) && s->using_coll;
 }
 bool needs_rare(int arm, const simc_settings* s)
 {
@@ -2079,9 +2283,10 @@ SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t even
     d.p_spec = s->p_spec;
     d.m2 = s->m2;
     d.ctau = s->ctau;
-    // hdc_sigma (hms/mc_hms_hut.f:143-144), sdc_sigma (sos/mc_sos_hut.f:112-113),
-    // shms/hut.inc:125-126
-    d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma : (arm == SIMC_ARM_SHMS ? 0.020 : 0.030);
+    // hdc_sigma (hms/mc_hms_hut.f:143-144, hrsl/mc_hrsl_hut.f:113-114), sdc_sigma
+    // (sos/mc_sos_hut.f:112-113), shms/hut.inc:125-126
+    d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma
+            : arm == SIMC_ARM_SHMS ? 0.020 : is_hrs(arm) ? 0.0225 : 0.030;
     d.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
               (coll_stepping(arm, s) ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
     d.start = start;
@@ -2126,6 +2331,8 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
         if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
     } else if (arm == SIMC_ARM_SOS) {
         if (tape) LAUNCH1(SIMC_ARM_SOS, TapeStream); else LAUNCH1(SIMC_ARM_SOS, PhiloxStream);
+    } else if (is_hrs(arm)) {  // one kernel family for both HRS arms (their tables differ)
+        if (tape) LAUNCH1(SIMC_ARM_HRSR, TapeStream); else LAUNCH1(SIMC_ARM_HRSR, PhiloxStream);
     } else {
         if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
     }
@@ -2350,13 +2557,25 @@ int simc_sos_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in
     return simc_arm_run(ctx, SIMC_ARM_SOS, s, in, out, n, rng, stream);
 }
 
+int simc_hrsr_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+                  int64_t n, const simc_rng* rng, void* stream)
+{
+    return simc_arm_run(ctx, SIMC_ARM_HRSR, s, in, out, n, rng, stream);
+}
+
+int simc_hrsl_run(simc_ctx* ctx, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
+                  int64_t n, const simc_rng* rng, void* stream)
+{
+    return simc_arm_run(ctx, SIMC_ARM_HRSL, s, in, out, n, rng, stream);
+}
+
 int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const double* dx_fp,
                    const double* y_fp, const double* dy_fp, const double* fry, double* delta,
                    double* dxdz_tar, double* dydz_tar, double* y_tar, void* stream)
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_recon_run: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (n < 0) return fail(SIMC_E_ARG, "negative event count");
     if (n == 0) return SIMC_OK;
@@ -2370,6 +2589,9 @@ int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const 
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else if (arm == SIMC_ARM_SOS)
         hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SOS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+                           dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
+    else if (is_hrs(arm))
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HRSR>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else
         hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
@@ -2629,7 +2851,7 @@ int simc_counters_get(simc_ctx* ctx, int arm, int64_t* c, int ncounters)
 {
     if (!ctx || !c) return fail(SIMC_E_ARG, "simc_counters_get: NULL argument");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     const int need = ncounters_of(arm);
     if (ncounters < need) return fail(SIMC_E_ARG, "counter array too short");
@@ -2644,7 +2866,7 @@ int simc_counters_reset(simc_ctx* ctx, int arm)
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_counters_reset: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemset(ctx->d_hist[ai], 0, kHistSlots * sizeof(unsigned long long)));
@@ -2656,7 +2878,7 @@ int simc_map_class_info(simc_ctx* ctx, int arm, int cls, double* driftdist, int3
 {
     if (!ctx) return fail(SIMC_E_ARG, "NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     const simc::ForwardMap& f = ctx->fwd[ai];
     if (cls < 1 || cls > f.n_classes) return fail(SIMC_E_ARG, "class out of range");
@@ -2670,7 +2892,7 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
 {
     if (!ctx || !flops) return fail(SIMC_E_ARG, "simc_stop_flops: NULL argument");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (stage < 0 || stage > 3) return fail(SIMC_E_ARG, "stage must be 0 (all), 1, 2 or 3");
     if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
@@ -2700,7 +2922,7 @@ int simc_recon_coefficients(simc_ctx* ctx, int arm, double* coef, int32_t* n_ter
 {
     if (!ctx) return fail(SIMC_E_ARG, "simc_recon_coefficients: NULL context");
     const int ai = arm_index(arm);
-    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS) or 5 (SHMS)");
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     const auto& terms = ctx->rec[ai].terms;
     if (n_terms) *n_terms = (int32_t)terms.size();
@@ -2719,7 +2941,8 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     int rc = check_settings(arm, s);
     if (rc) return rc;
     const int ai = arm_index(arm);
-    if (arm == SIMC_ARM_SOS) return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
+    if (arm != SIMC_ARM_HMS && arm != SIMC_ARM_SHMS)
+        return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
     if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
     if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");
     if (n <= 0 || n > ((int64_t)1 << 29)) return fail(SIMC_E_ARG, "simc_adjoint_run: n must be in [1, 2^29]");
@@ -2998,12 +3221,15 @@ int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)
     static const int hms_cls[] = {2, 3, 5, 6, 8, 9, 11, 0};
     static const int shms_cls[] = {3, 7, 8, 12, 13, 17, 18, 23, 24, 25, 26, 27, 28, 29, 30, 0};
     static const int sos_cls[] = {2, 3, 4, 5, 6, 7, 8, 9, 0};
+    static const int hrs_cls[] = {2, 3, 5, 6, 8, 10, 11, 0};
     int64_t f = 0;
     if (!ctx->d_coef[arm_index(arm)]) return 0;
     if (arm == SIMC_ARM_HMS)
         for (int c : hms_cls) f += simc::poly_flops(arm, c);
     else if (arm == SIMC_ARM_SOS)
         for (int c : sos_cls) f += simc::poly_flops(arm, c);
+    else if (is_hrs(arm))
+        for (int c : hrs_cls) f += simc::poly_flops(arm, c);
     else
         for (int c : shms_cls) f += simc::poly_flops(arm, c);
     return f;

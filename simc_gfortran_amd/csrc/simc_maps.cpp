@@ -213,6 +213,11 @@ const simc_gen::PolyInfo* polys(int arm, int& n, int& ncoef)
 {
     if (arm == SIMC_ARM_HMS) { n = simc_gen::n_polys_hms; ncoef = simc_gen::n_coef_hms; return simc_gen::polys_hms; }
     if (arm == SIMC_ARM_SOS) { n = simc_gen::n_polys_sos; ncoef = simc_gen::n_coef_sos; return simc_gen::polys_sos; }
+    if (arm == SIMC_ARM_HRSR || arm == SIMC_ARM_HRSL) {
+        n = simc_gen::n_polys_hrs;
+        ncoef = simc_gen::n_coef_hrs;
+        return simc_gen::polys_hrs;
+    }
     n = simc_gen::n_polys_shms;
     ncoef = simc_gen::n_coef_shms;
     return simc_gen::polys_shms;

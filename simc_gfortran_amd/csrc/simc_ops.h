@@ -1,6 +1,6 @@
 // Step programs of the spectrometer arms.
 //
-// An arm (mc_hms, mc_shms, mc_sos and their detector huts) is a fixed sequence of
+// An arm (mc_hms, mc_shms, mc_sos, mc_hrsr, mc_hrsl and their detector huts) is a fixed sequence of
 // drifts, COSY map segments, aperture tests and detector-hut steps.  The host
 // writes that sequence once per context as an array of Op records; one
 // kernel per arm walks it for every event (all lanes of a wavefront execute
@@ -42,6 +42,12 @@ enum OpCode : int32_t {
     OP_CHECK_TRAP,    // stop if |xs| > p0 or |ys| > p1 + p2 (xs + p0)
     OP_SHIFT,         // xs += p0 x', ys += p0 y' (no path length, no decay)
     OP_CHECK_ABS,     // stop if |xs| > p0 or |ys| > p1
+    // magnet-stage steps of the HRS (hrsl/mc_hrsl.f:157-499)
+    OP_CHECK_R,       // stop if sqrt(xs^2 + ys^2) > p0
+    OP_CHECK_T_SLAB,  // stop if |xt - p0| > p1 or |yt| + p2 xt > p3
+    // hut steps of the HRS (hrsl/mc_hrsl_hut.f:289-304)
+    OP_BOX_ROT_H,     // (xt,yt) = rotate_haxis(xs,ys; tan p0, sin p1, cos p2);
+                      // stop if xt > p3 or xt < p4 or yt > p5 or yt < p6
     OP_NCODES
 };
 
@@ -52,9 +58,9 @@ enum OpCode : int32_t {
 enum : int { STAGE_MAGNETS = 1, STAGE_HUT = 2 };
 constexpr int op_stages(int code)
 {
-    return code == OP_END || code == OP_PROJECT                ? (STAGE_MAGNETS | STAGE_HUT)
-         : code <= OP_COLLIMATOR || code >= OP_CHECK_T_BOX     ? STAGE_MAGNETS
-                                                               : STAGE_HUT;
+    return code == OP_END || code == OP_PROJECT                                  ? (STAGE_MAGNETS | STAGE_HUT)
+         : code <= OP_COLLIMATOR || (code >= OP_CHECK_T_BOX && code <= OP_CHECK_T_SLAB) ? STAGE_MAGNETS
+                                                                                 : STAGE_HUT;
 }
 
 struct Op {

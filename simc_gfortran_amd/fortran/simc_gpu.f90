@@ -13,10 +13,12 @@ module simc_gpu
   private :: c_strlen
 
   integer(c_int), parameter :: SIMC_OK = 0
-  integer(c_int), parameter :: SIMC_ARM_HMS = 1, SIMC_ARM_SOS = 2, SIMC_ARM_SHMS = 5
+  integer(c_int), parameter :: SIMC_ARM_HMS = 1, SIMC_ARM_SOS = 2, SIMC_ARM_HRSR = 3, SIMC_ARM_HRSL = 4
+  integer(c_int), parameter :: SIMC_ARM_SHMS = 5
   integer(c_int32_t), parameter :: SIMC_RNG_TAPE = 0, SIMC_RNG_PHILOX = 1
   integer(c_int32_t), parameter :: SIMC_FIT_FP64 = 0, SIMC_FIT_REF_F32 = 1
   integer(c_int), parameter :: SIMC_HMS_NCOUNTERS = 22, SIMC_SHMS_NCOUNTERS = 52, SIMC_SOS_NCOUNTERS = 18
+  integer(c_int), parameter :: SIMC_HRS_NCOUNTERS = 21
 
   type, bind(C) :: simc_settings
     real(c_double)     :: p_spec = 0, m2 = 0, ctau = 0
@@ -160,6 +162,26 @@ module simc_gpu
     end function
 
     integer(c_int) function simc_sos_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_sos_run")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_hrsr_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_hrsr_run")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_hrsl_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_hrsl_run")
       import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
       type(c_ptr), value :: ctx, stream
       type(simc_settings), intent(in) :: s

@@ -69,6 +69,7 @@ program batch_host
   nc = SIMC_HMS_NCOUNTERS
   if (arm == SIMC_ARM_SHMS) nc = SIMC_SHMS_NCOUNTERS
   if (arm == SIMC_ARM_SOS) nc = SIMC_SOS_NCOUNTERS
+  if (arm == SIMC_ARM_HRSR .or. arm == SIMC_ARM_HRSL) nc = SIMC_HRS_NCOUNTERS
   rc = simc_counters_get(ctx, arm, cnt, nc)
   write (*, '(a)', advance='no') 'counters'
   do k = 1, nc

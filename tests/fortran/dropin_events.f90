@@ -1,6 +1,6 @@
 ! The per-event drop-in: a program written like simc.f's montecarlo() that
-! calls mc_hms / mc_sos / mc_shms once per event, with the common blocks the reference
-! declares for the STOP counters (/spectrometer1/, /spectrometer2/, /spectrometer_shms/),
+! calls mc_hms / mc_sos / mc_hrsr / mc_hrsl / mc_shms once per event, with the common blocks the reference
+! declares for the STOP counters (/spectrometer1/ to /spectrometer4/, /spectrometer_shms/),
 ! the track (/track/) and ctau (/decd/), linked against libsimc_dropin.so.
 ! Reads n events (SoA float64 dpp, x, y, z, dxdz, dydz), writes the per-event
 ! outputs (SoA float64 dpp, y, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, pathlen,
@@ -9,9 +9,11 @@
 !   usage: dropin_events ARM N P_SPEC M2 SEED IN_FILE OUT_FILE  (SIMC_MAP_DIR set)
 program dropin_events
   implicit none
-  integer*4 hstop(22), sstop(52), ostop(18)
+  integer*4 hstop(22), sstop(52), ostop(18), rstop(21), lstop(21)
   common /spectrometer1/ hstop
   common /spectrometer2/ ostop
+  common /spectrometer3/ rstop
+  common /spectrometer4/ lstop
   common /spectrometer_shms/ sstop
   real*8 trk(80)
   common /track/ trk
@@ -43,6 +45,8 @@ program dropin_events
   hstop = 0
   sstop = 0
   ostop = 0
+  rstop = 0
+  lstop = 0
   ctau = 0
   call simc_dropin_seed(seed)
   ms_flag = .true.
@@ -64,6 +68,12 @@ program dropin_events
     else if (arm == 2) then
       call mc_sos(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2e, &
                   ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen)
+    else if (arm == 3) then
+      call mc_hrsr(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2e, &
+                   ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen)
+    else if (arm == 4) then
+      call mc_hrsl(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2e, &
+                   ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen)
     else
       call mc_shms(p_spec, th_spec, dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp, m2e, &
                    ms_flag, wcs_flag, decay_flag, resmult, fry, ok_spec, pathlen, spectr, using_coll)
@@ -73,6 +83,10 @@ program dropin_events
         call mc_hms_recon(r_dpp, r_dth, r_dph, r_y, fry)
       else if (arm == 2) then
         call mc_sos_recon(r_dpp, r_dth, r_dph, r_y, fry)
+      else if (arm == 3) then
+        call mc_hrsr_recon(r_dpp, r_dth, r_dph, r_y, fry)
+      else if (arm == 4) then
+        call mc_hrsl_recon(r_dpp, r_dth, r_dph, r_y, fry)
       else
         call mc_shms_recon(r_dpp, r_dth, r_dph, r_y, fry, spectr)
       end if
@@ -92,6 +106,14 @@ program dropin_events
   else if (arm == 2) then
     do k = 1, 18
       write (*, '(1x,i0)', advance='no') ostop(k)
+    end do
+  else if (arm == 3) then
+    do k = 1, 21
+      write (*, '(1x,i0)', advance='no') rstop(k)
+    end do
+  else if (arm == 4) then
+    do k = 1, 21
+      write (*, '(1x,i0)', advance='no') lstop(k)
     end do
   else
     do k = 1, 52

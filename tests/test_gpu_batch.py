@@ -37,11 +37,13 @@ def mixed(counts, gap=17):
         dict(arm=5, p_spec=4200.0, m2=W.MP ** 2),
         dict(arm=1, p_spec=2000.0, m2=MPI ** 2, decay=True, ctau=780.45),
         dict(arm=2, p_spec=1200.0, m2=W.MP ** 2, fit="fp64"),
+        dict(arm=3, p_spec=2000.0, m2=W.MP ** 2, fit="fp64"),
+        dict(arm=4, p_spec=3000.0, m2=W.ME ** 2),
     ]
     settings, parts, off = [], [], 0
     for i, cnt in enumerate(counts):
         d = dict(specs[i % len(specs)])
-        wl = {1: W.HMS_C0, 2: W.SOS_1p5, 5: W.SHMS_C1}[d["arm"]]
+        wl = {1: W.HMS_C0, 2: W.SOS_1p5, 3: W.HRSR_2, 4: W.HRSL_3, 5: W.SHMS_C1}[d["arm"]]
         d.update(offset=off, count=cnt, seed=11, event0=(i + 1) << 32)
         settings.append(d)
         ev = W.generate(wl, cnt, seed=100 + i)
@@ -68,7 +70,7 @@ def plain(ctx, d, events):
 
 def test_batch_equals_plain_runs(ctx):
     from simc_gfortran_amd import api
-    counts = [20000, 3000, 5, 0, 45000, 1, 12000, 9000]
+    counts = [20000, 3000, 5, 0, 45000, 1, 12000, 9000, 7000, 6000]
     settings, events = mixed(counts)
     b = api.Batch(ctx, settings)
     out = b.run(events)

@@ -42,6 +42,8 @@ CASES = [
     ("hms_c0", W.HMS_C0),
     ("shms_c1", W.SHMS_C1),
     ("sos_1p5", W.SOS_1p5),
+    ("hrsr_2", W.HRSR_2),
+    ("hrsl_3", W.HRSL_3),
 ]
 
 
@@ -119,28 +121,29 @@ def test_rare_paths_parity(ctx, oracle, arm, coll, decay):
     np.testing.assert_array_equal(g["m2"] != MPI * MPI, ref["m2"] != MPI * MPI)
 
 
-def test_sos_decay_parity(ctx, oracle):
-    """SOS pions decaying in flight (sos/mc_sos.f drifts call project with
-    decay_flag); the SOS has no collimator punch-through."""
+@pytest.mark.parametrize("wl", [W.SOS_1p5, W.HRSR_2, W.HRSL_3], ids=["sos", "hrsr", "hrsl"])
+def test_decay_parity_arms_without_collimator_stepping(ctx, oracle, wl):
+    """Pions decaying in flight through the SOS and the HRS (their drifts and
+    COSY segments call project / transp with decay_flag); these arms have no
+    collimator punch-through."""
     n, stride = 20000, 2048
-    wl = W.SOS_1p5
     ev = W.generate(wl, n, seed=6)
     tape = W.uniform_tape(n, stride, 98)
     kw = dict(p_spec=1000.0, m2=MPI * MPI, ms=True, wcs=True, decay=True, ctau=780.45)
-    ref = oracle.run(2, ev, rng="tape", tape=tape, stride=stride, fit_f32=False, **kw)
-    out = ctx.run(2, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+    ref = oracle.run(wl.arm, ev, rng="tape", tape=tape, stride=stride, fit_f32=False, **kw)
+    out = ctx.run(wl.arm, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
                   fit="fp64", **kw)
     torch.cuda.synchronize()
     g = to_host(out)
     rep = parity_report(g, ref, 1e-10)
-    print("sos decay", rep)
+    print(wl.name, "decay", rep)
     assert rep["flag_mismatch"] <= 2
     assert sum(rep["bad"].values()) <= 2, rep
     assert (g["m2"] != MPI * MPI).any()
     np.testing.assert_array_equal(g["m2"] != MPI * MPI, ref["m2"] != MPI * MPI)
 
 
-@pytest.mark.parametrize("arm", [1, 2, 5])
+@pytest.mark.parametrize("arm", [1, 2, 3, 4, 5])
 def test_recon_only(ctx, oracle, arm):
     rng = np.random.default_rng(3)
     n = 10000
@@ -161,7 +164,7 @@ def test_errors(ctx):
     from simc_gfortran_amd import api
     ev = to_dev(W.generate(W.HMS_C0, 16))
     with pytest.raises(api.SimcError) as e:
-        ctx.run(3, ev, p_spec=3000.0, m2=1.0)
+        ctx.run(6, ev, p_spec=3000.0, m2=1.0)
     assert e.value.code == -6
     with pytest.raises(api.SimcError):
         ctx.run(1, ev, p_spec=-1.0, m2=1.0)

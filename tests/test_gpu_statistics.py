@@ -26,7 +26,8 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1, W.SOS_1p5], ids=["hms", "shms", "sos"])
+@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1, W.SOS_1p5, W.HRSR_2, W.HRSL_3],
+                         ids=["hms", "shms", "sos", "hrsr", "hrsl"])
 def test_gpu_full_stream_histograms_match_reference_stream(ctx, oracle, wl):
     kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma)
     ref = oracle.run(wl.arm, W.generate(wl, N, seed=41), rng="ranlux", seed=1234567, nthreads=1,
