@@ -335,3 +335,113 @@ def ref_target_step(target, narm, spec, mass, zpos, energy, theta, tape, stride,
                       _p(tape), int(stride))
     out["used"] = used
     return out
+
+
+# ---------------------------------------------------------------------------
+# target-field tracking (trg_track.f)
+# ---------------------------------------------------------------------------
+FIELD_MAP = os.path.join(REPO, "data", "fields", "trg_field_map.dat")
+_FIELD_BYTES = 8 * (2 * 51 * 51 + 2 * 51 + 4)
+
+
+def _field_lib():
+    L = lib()
+    if not hasattr(L, "_field_ready"):
+        L.or_trg_init.restype = C.c_int
+        L.or_trg_init.argtypes = [C.c_void_p, C.c_char_p] + [C.c_double] * 4
+        L.or_trg_field.restype = None
+        L.or_trg_field.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.or_track_from_tgt_run.restype = None
+        L.or_track_from_tgt_run.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 6 + [C.c_double, C.c_int, _u8p,
+                                                                                C.c_int]
+        L.or_track_to_tgt_run.restype = None
+        L.or_track_to_tgt_run.argtypes = ([C.c_void_p, C.c_void_p, C.c_int, C.c_int64] + [_dp] * 11 +
+                                          [C.c_double, C.c_double, C.c_double, C.c_int, _u8p, C.c_int])
+        L._field_ready = True
+    return L
+
+
+def field(map_path=FIELD_MAP, theta_e=0.0, theta_p=0.0):
+    """trgInit: map_path None = the uniform 5 T test field, "0" = zero field;
+    theta_e / theta_p: field angle relative to the e / p arm (deg)."""
+    buf = C.create_string_buffer(_FIELD_BYTES)
+    rc = _field_lib().or_trg_init(buf, None if map_path is None else map_path.encode(), theta_e, 0.0,
+                                  theta_p, 0.0)
+    if rc:
+        raise RuntimeError("oracle: cannot read field map " + str(map_path))
+    return buf
+
+
+def field_at(F, x, spect):
+    B = np.zeros(3)
+    _field_lib().or_trg_field(F, _p(np.ascontiguousarray(x, dtype=np.float64)), _p(B), spect)
+    return B
+
+
+def track_from_tgt(F, ev, mom, mass, spect, nthreads=0):
+    """Per-event track_from_tgt over dict ev (x, y, z, dx, dy); returns new
+    arrays and ok (uint8)."""
+    n = len(ev["x"])
+    out = {k: np.array(ev[k], dtype=np.float64, copy=True) for k in ("x", "y", "z", "dx", "dy")}
+    ok = np.zeros(n, np.uint8)
+    m = np.ascontiguousarray(mom, dtype=np.float64)
+    _field_lib().or_track_from_tgt_run(F, n, *[_p(out[k]) for k in ("x", "y", "z", "dx", "dy")], _p(m), mass,
+                                       spect, _p(ok, _u8p), int(nthreads or os.cpu_count()))
+    out["ok"] = ok
+    return out
+
+
+def track_to_tgt(F, arm, trk, first, frx, fry, mom, mass, ctheta, stheta, spect, nthreads=0):
+    """Per-event track_to_tgt: trk = focal-plane track dict (x_fp, dx_fp,
+    y_fp, dy_fp), first = first-guess recon dict (delta, y, dx, dy)."""
+    n = len(trk["x_fp"])
+    out = {k: np.array(first[k], dtype=np.float64, copy=True) for k in ("delta", "y", "dx", "dy")}
+    ok = np.zeros(n, np.uint8)
+    a = [np.ascontiguousarray(trk[k], dtype=np.float64) for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")]
+    f = [np.ascontiguousarray(v, dtype=np.float64) for v in (frx, fry, mom)]
+    _field_lib().or_track_to_tgt_run(F, C.c_void_p(maps()), arm, n, *[_p(v) for v in a],
+                                     *[_p(out[k]) for k in ("delta", "y", "dx", "dy")], *[_p(v) for v in f],
+                                     mass, ctheta, stheta, spect, _p(ok, _u8p), int(nthreads or os.cpu_count()))
+    out["ok"] = ok
+    return out
+
+
+def ref_track_from_tgt(map_path, theta_e, theta_p, ev, mom, mass, spect):
+    """The reference's trgInit + track_from_tgt (oracle/_ref)."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    L.ref_trg_init.restype = None
+    L.ref_trg_init.argtypes = [C.c_char_p] + [C.c_double] * 4
+    L.ref_track_from_tgt.restype = None
+    L.ref_track_from_tgt.argtypes = [C.c_int64] + [_dp] * 6 + [C.c_double, C.c_int, _u8p]
+    L.ref_trg_init(None if map_path is None else map_path.encode(), theta_e, 0.0, theta_p, 0.0)
+    n = len(ev["x"])
+    out = {k: np.array(ev[k], dtype=np.float64, copy=True) for k in ("x", "y", "z", "dx", "dy")}
+    ok = np.zeros(n, np.uint8)
+    L.ref_track_from_tgt(n, *[_p(out[k]) for k in ("x", "y", "z", "dx", "dy")],
+                         _p(np.ascontiguousarray(mom, dtype=np.float64)), mass, spect, _p(ok, _u8p))
+    out["ok"] = ok
+    return out
+
+
+def ref_track_to_tgt(map_path, theta_e, theta_p, arm, trk, first, frx, fry, mom, mass, ctheta, stheta, spect):
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    L.ref_trg_init.restype = None
+    L.ref_trg_init.argtypes = [C.c_char_p] + [C.c_double] * 4
+    L.ref_track_to_tgt.restype = C.c_int
+    L.ref_track_to_tgt.argtypes = [C.c_int, C.c_int64] + [_dp] * 11 + [C.c_double] * 3 + [C.c_int, _u8p,
+                                                                                         C.c_char_p]
+    L.ref_trg_init(None if map_path is None else map_path.encode(), theta_e, 0.0, theta_p, 0.0)
+    n = len(trk["x_fp"])
+    out = {k: np.array(first[k], dtype=np.float64, copy=True) for k in ("delta", "y", "dx", "dy")}
+    ok = np.zeros(n, np.uint8)
+    a = [np.ascontiguousarray(trk[k], dtype=np.float64) for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")]
+    f = [np.ascontiguousarray(v, dtype=np.float64) for v in (frx, fry, mom)]
+    rc = L.ref_track_to_tgt(arm, n, *[_p(v) for v in a], *[_p(out[k]) for k in ("delta", "y", "dx", "dy")],
+                            *[_p(v) for v in f], mass, ctheta, stheta, spect, _p(ok, _u8p), MAP_DIR.encode())
+    assert rc == 0, rc
+    out["ok"] = ok
+    return out

@@ -198,3 +198,56 @@ int ref_target_step(const double *tgt, int can, int narm, int spec, double mass,
     }
     return 0;
 }
+
+/* ---- target-field tracking: the reference's trg_track.f ---- */
+extern void trginit_(const char *map, double *theta_e, double *phi_e, double *theta_p,
+                     double *phi_p, size_t map_len);
+extern void track_from_tgt_(double *x, double *y, double *z, double *dx, double *dy,
+                            double *mom, double *mass, int32_t *spect, int32_t *ok);
+extern void track_to_tgt_(double *delta, double *y, double *dx, double *dy, double *frx,
+                          double *fry, double *mom, double *mass, double *ctheta,
+                          double *stheta, int32_t *spect, int32_t *ok, int32_t *arm);
+extern struct { double xs, ys, zs, dpps, dxdzs, dydzs; } track_;  /* spectrometers.inc:47 */
+
+/* trgInit(map, ...): map NULL = the uniform test field (' '), "0" = zero field. */
+void ref_trg_init(const char *map, double theta_e, double phi_e, double theta_p, double phi_p)
+{
+    const char *m = map ? map : " ";
+    trginit_(m, &theta_e, &phi_e, &theta_p, &phi_p, strlen(m));
+}
+
+void ref_track_from_tgt(int64_t n, double *x, double *y, double *z, double *dx, double *dy,
+                        const double *mom, double mass, int spect, uint8_t *ok)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double m = mom[i], ms = mass;
+        int32_t sp = spect, o = 0;
+        track_from_tgt_(&x[i], &y[i], &z[i], &dx[i], &dy[i], &m, &ms, &sp, &o);
+        ok[i] = (uint8_t)(o != 0);
+    }
+}
+
+/* track_to_tgt per event, with /track/ set to the event's focal-plane track;
+ * mapdir holds the recon maps the reference opens on first use. */
+int ref_track_to_tgt(int arm, int64_t n, const double *x_fp, const double *dx_fp,
+                     const double *y_fp, const double *dy_fp, double *delta, double *y,
+                     double *dx, double *dy, const double *frx, const double *fry,
+                     const double *mom, double mass, double ctheta, double stheta, int spect,
+                     uint8_t *ok, const char *mapdir)
+{
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd)) return -1;
+    if (chdir(mapdir) != 0) return -2;
+    for (int64_t i = 0; i < n; i++) {
+        track_.xs = x_fp[i];
+        track_.ys = y_fp[i];
+        track_.dxdzs = dx_fp[i];
+        track_.dydzs = dy_fp[i];
+        double fx = frx[i], fy = fry[i], m = mom[i], ms = mass, ct = ctheta, st = stheta;
+        int32_t sp = spect, o = 1, a = arm;
+        track_to_tgt_(&delta[i], &y[i], &dx[i], &dy[i], &fx, &fy, &m, &ms, &ct, &st, &sp, &o, &a);
+        ok[i] = (uint8_t)(o != 0);
+    }
+    if (chdir(cwd) != 0) return -3;
+    return 0;
+}

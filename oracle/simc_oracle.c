@@ -2971,3 +2971,293 @@ int or_target_prep(const or_target *T, const or_prep *P, int64_t n, const or_ver
     if (bad) return -1;
     return overflow ? -2 : 0;
 }
+
+/* ========================================================================= */
+/* Target-field tracking (trg_track.f)                                        */
+/* ========================================================================= */
+
+/* trgInit, trg_track.f:243-325: 51 x 51 (z, r) grid at 2 cm spacing; the
+ * map file's lines run over z fastest, columns 3 and 4 are B_z and B_r. */
+int or_trg_init(or_field *F, const char *map, double theta_e, double phi_e, double theta_p,
+                double phi_p)
+{
+    const double pi180 = 3.141592653 / 180.;
+    (void)phi_e;
+    (void)phi_p; /* "for performance reasons B_phi is always treated 0" (:255-256, :375) */
+    F->stht[0] = sin(theta_e * pi180);
+    F->ctht[0] = cos(theta_e * pi180);
+    F->stht[1] = sin(theta_p * pi180);
+    F->ctht[1] = cos(theta_p * pi180);
+    for (int ir = 0; ir < OR_TRG_N; ir++) {
+        F->rr[ir] = 2. * (double)ir;
+        F->zz[ir] = 2. * (double)ir;
+    }
+    if (map && strcmp(map, "0") == 0) { /* :290-299 */
+        memset(F->bz, 0, sizeof F->bz);
+        memset(F->br, 0, sizeof F->br);
+    } else if (map) { /* :300-310 */
+        FILE *f = fopen(map, "r");
+        if (!f) return -1;
+        char line[512];
+        for (int ir = 0; ir < OR_TRG_N; ir++)
+            for (int iz = 0; iz < OR_TRG_N; iz++) {
+                double v[7];
+                if (!fgets(line, sizeof line, f) ||
+                    sscanf(line, "%lf %lf %lf %lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3], &v[4],
+                           &v[5], &v[6]) != 7) {
+                    fclose(f);
+                    return -1;
+                }
+                F->bz[ir][iz] = v[2];
+                F->br[ir][iz] = v[3];
+            }
+        fclose(f);
+    } else { /* uniform 5 T over 26 cm in z and 16 cm in r, :311-323 */
+        for (int ir = 0; ir < OR_TRG_N; ir++)
+            for (int iz = 0; iz < OR_TRG_N; iz++) {
+                F->br[ir][iz] = 0.;
+                F->bz[ir][iz] = (F->rr[ir] <= 16. && F->zz[iz] <= 26.) ? 5.0 : 0.0;
+            }
+    }
+    return 0;
+}
+
+/* trgField, trg_track.f:328-418: rotate into the field frame (z' along the
+ * field), bilinear interpolation in (|z'|, r), rotate back. */
+void or_trg_field(const or_field *F, const double x_[3], double B_[3], int spect)
+{
+    const int k = spect == -1 ? 0 : 1;
+    const double stht = F->stht[k], ctht = F->ctht[k];
+    double x[3], B[3], z, r, az, ar, a0, a1;
+    x[0] = x_[0];
+    x[1] = stht * x_[2] + ctht * x_[1];
+    x[2] = ctht * x_[2] - stht * x_[1];
+    z = fabs(x[2]);
+    r = sqrt(x[0] * x[0] + x[1] * x[1]);
+    const int i = (int)((z - F->zz[0]) / (F->zz[1] - F->zz[0])) + 1;
+    const int j = (int)((r - F->rr[0]) / (F->rr[1] - F->rr[0])) + 1;
+    if ((i + 1 > OR_TRG_N) || (i < 1) || (j + 1 > OR_TRG_N) || (j < 1)) {
+        B_[0] = 0.;
+        B_[1] = 0.;
+        B_[2] = 0.;
+        return;
+    }
+    const int i0 = i - 1, j0 = j - 1; /* B(i, j) = b[j0][i0] */
+    az = ((z - F->zz[i0]) / (F->zz[1] - F->zz[0]));
+    ar = ((r - F->rr[j0]) / (F->rr[1] - F->rr[0]));
+    a0 = az * (F->bz[j0][i0 + 1] - F->bz[j0][i0]) + F->bz[j0][i0];
+    a1 = az * (F->bz[j0 + 1][i0 + 1] - F->bz[j0 + 1][i0]) + F->bz[j0 + 1][i0];
+    B[2] = (ar * (a1 - a0) + a0);
+    if (r > 0.) {
+        a0 = az * (F->br[j0][i0 + 1] - F->br[j0][i0]) + F->br[j0][i0];
+        a1 = az * (F->br[j0 + 1][i0 + 1] - F->br[j0 + 1][i0]) + F->br[j0 + 1][i0];
+        B[1] = (ar * (a1 - a0) + a0) / r;
+        if (x[2] < 0.) B[1] = -B[1];
+        B[0] = B[1] * x[0];
+        B[1] = B[1] * x[1];
+        B_[0] = B[0];
+        B_[1] = -stht * B[2] + ctht * B[1];
+        B_[2] = ctht * B[2] + stht * B[1];
+    } else {
+        B_[0] = 0.;
+        B_[1] = -stht * B[2];
+        B_[2] = ctht * B[2];
+    }
+}
+
+/* trgDeriv, trg_track.f:423-459 (the u(7..9) terms are not used by trgRK4) */
+static void trg_deriv(const or_field *F, const double u[9], double dudt[9], int spect,
+                      double factor)
+{
+    double B[3];
+    or_trg_field(F, u, B, spect);
+    dudt[0] = u[3];
+    dudt[1] = u[4];
+    dudt[2] = u[5];
+    dudt[6] = u[4] * B[2] - u[5] * B[1];
+    dudt[7] = u[5] * B[0] - u[3] * B[2];
+    dudt[8] = u[3] * B[1] - u[4] * B[0];
+    dudt[3] = dudt[6] * factor;
+    dudt[4] = dudt[7] * factor;
+    dudt[5] = dudt[8] * factor;
+}
+
+/* trgRK4, trg_track.f:461-498 (Numerical Recipes rk4 on u(1..6)) */
+void or_trg_rk4(const or_field *F, const double u0[9], double u1[9], double h, int spect,
+                double factor)
+{
+    double ut[9], dudt[9], dut[9], dum[9], hh, h6;
+    hh = h * 0.5;
+    h6 = h / 6.;
+    trg_deriv(F, u0, dudt, spect, factor);
+    for (int i = 0; i < 6; i++) ut[i] = u0[i] + hh * dudt[i];
+    trg_deriv(F, ut, dut, spect, factor);
+    for (int i = 0; i < 6; i++) ut[i] = u0[i] + hh * dut[i];
+    trg_deriv(F, ut, dum, spect, factor);
+    for (int i = 0; i < 6; i++) {
+        ut[i] = u0[i] + h * dum[i];
+        dum[i] = dut[i] + dum[i];
+    }
+    trg_deriv(F, ut, dut, spect, factor);
+    for (int i = 0; i < 6; i++) u1[i] = u0[i] + h6 * (dudt[i] + dut[i] + 2. * dum[i]);
+}
+
+/* trgTrackToPlane, trg_track.f:154-238: step until the plane
+ * a x + b y + c z + d = 0 is crossed, then interpolate linearly.  The first
+ * step only chooses the direction; with no crossing in it the loop restarts
+ * from it and compares against the starting distance (:211-224). */
+void or_trg_track_to_plane(const or_field *F, double u[9], double E, double dl, double a,
+                           double b, double c, double d, int *ok, int spect)
+{
+    double ts, n, an, bn, cn, dn, maxdist, dist0, dist1, u0[9], u1[9];
+    int steps, max_steps;
+    if (!*ok) return;
+    n = 1 / sqrt(a * a + b * b + c * c);
+    an = a * n;
+    bn = b * n;
+    cn = c * n;
+    dn = d * n;
+    const double factor = 90. / E;
+    ts = -dl / sqrt(u[3] * u[3] + u[4] * u[4] + u[5] * u[5]);
+    dist0 = u[0] * an + u[1] * bn + u[2] * cn + dn;
+    maxdist = fmax(fabs(dist0) * 4., 1.0);
+    or_trg_rk4(F, u, u1, ts, spect, factor);
+    dist1 = u1[0] * an + u1[1] * bn + u1[2] * cn + dn;
+    if ((copysign(1., dist0) == copysign(1., dist1)) && (fabs(dist0) < fabs(dist1))) ts = -ts;
+    steps = 0;
+    max_steps = (int)(fmax(dist0, 10. * dl) / dl) * 10;
+    if (copysign(1., dist0) == copysign(1., dist1)) {
+        dist1 = dist0;
+        while ((copysign(1., dist0) == copysign(1., dist1)) && *ok) {
+            or_trg_rk4(F, u1, u0, ts, spect, factor);
+            dist0 = u0[0] * an + u0[1] * bn + u0[2] * cn + dn;
+            if (copysign(1., dist0) == copysign(1., dist1)) {
+                or_trg_rk4(F, u0, u1, ts, spect, factor);
+                dist1 = u1[0] * an + u1[1] * bn + u1[2] * cn + dn;
+            }
+            *ok = (fabs(dist1) < maxdist) && steps < max_steps;
+            steps = steps + 1;
+        }
+    } else {
+        for (int i = 0; i < 6; i++) u0[i] = u[i];
+    }
+    if (*ok)
+        for (int i = 0; i < 6; i++) u[i] = u0[i] + (u1[i] - u0[i]) * dist0 / (dist0 - dist1);
+}
+
+/* track_from_tgt, trg_track.f:591-658: from the vertex to the plane z = 0,
+ * then to z = 100 cm (field free); the first call's status is not kept. */
+void or_track_from_tgt(const or_field *F, double *x, double *y, double *z, double *dx,
+                       double *dy, double mom, double mass, int spect, int *ok)
+{
+    const double cc = 29.9792458;
+    double vT[9] = {0};
+    const double vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+    const double eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+    vT[0] = *x;
+    vT[1] = *y;
+    vT[2] = *z;
+    vT[5] = vel / sqrt(1 + *dx * *dx + *dy * *dy);
+    vT[3] = *dx * vT[5];
+    vT[4] = *dy * vT[5];
+    *ok = 1;
+    or_trg_track_to_plane(F, vT, eng, 1., 0., 0., 1., 0., ok, spect);
+    *ok = 1;
+    or_trg_track_to_plane(F, vT, eng, 1., 0., 0., 1., -100., ok, spect);
+    *x = vT[0];
+    *y = vT[1];
+    *z = vT[2];
+    *dx = vT[3] / vT[5];
+    *dy = vT[4] / vT[5];
+}
+
+/* track_to_tgt, trg_track.f:738-876: iterate field tracking from z = 100 cm
+ * back to the beam plane and the arm's reconstruction with the corrected
+ * vertical position, at most 10 times. */
+void or_track_to_tgt(const or_field *F, const or_maps *M, int arm, const double track[4],
+                     double *delta, double *y, double *dx, double *dy, double frx, double fry,
+                     double mom, double mass, double ctheta, double stheta, int spect, int *ok)
+{
+    const double cc = 29.9792458;
+    double vT[9] = {0}, vTx[9] = {0}, xx, delx, xxd, vel, eng, mom_0;
+    int n;
+    xx = -fry;
+    vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+    eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+    mom_0 = mom / (1.e0 + *delta / 100.e0);
+    vT[0] = -fry + 100. * *dx;
+    vT[1] = *y + 100. * *dy;
+    vT[2] = 100.;
+    vT[5] = vel / sqrt(1 + *dy * *dy + *dx * *dx);
+    vT[3] = *dx * vT[5];
+    vT[4] = *dy * vT[5];
+    or_trg_track_to_plane(F, vT, eng, 1., 0., -ctheta, stheta, frx, ok, spect);
+    n = 0;
+    delx = 1.;
+    while ((delx > .0001) && (n < 10) && *ok) {
+        delx = fabs(-fry - vT[0]);
+        vTx[0] = -fry;
+        for (int i = 1; i < 6; i++) vTx[i] = vT[i];
+        or_trg_track_to_plane(F, vT, eng, 1., 0., 0., 1., 0., ok, spect);
+        or_trg_track_to_plane(F, vTx, eng, 1., 0., 0., 1., 0., ok, spect);
+        xx = xx + fmin(1., fmax(-1., (vTx[0] - vT[0])));
+        xxd = xx;
+        /* mc_*_recon(delta, dy, dx, y, xxd) on /track/ (:836-846) */
+        double dl_, dxdz_, dydz_, yt_;
+        or_recon_run(M, arm, 1, &track[0], &track[1], &track[2], &track[3], &xxd, &dl_, &dxdz_,
+                     &dydz_, &yt_);
+        *delta = dl_;
+        *dy = dydz_;
+        *dx = dxdz_;
+        *y = yt_;
+        mom = mom_0 * (1.e0 + *delta / 100.e0);
+        vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+        eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+        vT[0] = xx + 100. * *dx;
+        vT[1] = *y + 100. * *dy;
+        vT[2] = 100.;
+        vT[5] = vel / sqrt(1 + *dy * *dy + *dx * *dx);
+        vT[3] = *dx * vT[5];
+        vT[4] = *dy * vT[5];
+        or_trg_track_to_plane(F, vT, eng, 1., 0., -ctheta, stheta, frx, ok, spect);
+        n = n + 1;
+    }
+    if (delx > .2) *ok = 0;
+    *dy = vT[4] / vT[5];
+    *dx = vT[3] / vT[5];
+    *y = vT[1];
+}
+
+void or_track_from_tgt_run(const or_field *F, int64_t n, double *x, double *y, double *z,
+                           double *dx, double *dy, const double *mom, double mass, int spect,
+                           uint8_t *ok, int nthreads)
+{
+    if (nthreads <= 0) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        int o = 0;
+        or_track_from_tgt(F, &x[i], &y[i], &z[i], &dx[i], &dy[i], mom[i], mass, spect, &o);
+        ok[i] = (uint8_t)o;
+    }
+}
+
+void or_track_to_tgt_run(const or_field *F, const or_maps *M, int arm, int64_t n,
+                         const double *x_fp, const double *dx_fp, const double *y_fp,
+                         const double *dy_fp, double *delta, double *y, double *dx, double *dy,
+                         const double *frx, const double *fry, const double *mom, double mass,
+                         double ctheta, double stheta, int spect, uint8_t *ok, int nthreads)
+{
+    if (nthreads <= 0) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        const double tr[4] = {x_fp[i], dx_fp[i], y_fp[i], dy_fp[i]};
+        int o = 1;
+        or_track_to_tgt(F, M, arm, tr, &delta[i], &y[i], &dx[i], &dy[i], frx[i], fry[i], mom[i],
+                        mass, ctheta, stheta, spect, &o);
+        ok[i] = (uint8_t)o;
+    }
+}

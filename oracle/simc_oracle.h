@@ -298,6 +298,42 @@ int or_target_prep(const or_target *T, const or_prep *P, int64_t n, const or_ver
                    or_prep_out *out, int rng_kind, const double *tape, int64_t stride,
                    uint64_t seed, uint64_t event0, int nthreads, int32_t *used);
 
+/* ---- target-field tracking (trg_track.f) ---- */
+#define OR_TRG_N 51
+typedef struct {
+    double bz[OR_TRG_N][OR_TRG_N];   /* B_field_z(iz, ir) at bz[ir][iz] (T) */
+    double br[OR_TRG_N][OR_TRG_N];
+    double zz[OR_TRG_N], rr[OR_TRG_N];
+    double stht[2], ctht[2];         /* [0]: spect -1 (electron arm), [1]: spect +1 */
+} or_field;
+/* trgInit: map = NULL for the uniform 5 T test field, "0" for zero field,
+ * else the 2601-line map file; angles in degrees.  Returns 0, -1 when the
+ * file cannot be read. */
+int or_trg_init(or_field *F, const char *map, double theta_e, double phi_e, double theta_p,
+                double phi_p);
+void or_trg_field(const or_field *F, const double x[3], double B[3], int spect);
+void or_trg_rk4(const or_field *F, const double u0[9], double u1[9], double h, int spect,
+                double factor);
+void or_trg_track_to_plane(const or_field *F, double u[9], double E, double dl, double a,
+                           double b, double c, double d, int *ok, int spect);
+void or_track_from_tgt(const or_field *F, double *x, double *y, double *z, double *dx,
+                       double *dy, double mom, double mass, int spect, int *ok);
+/* batch forms (OpenMP over events): per-event mom (signed), one mass and
+ * spect per call; ok as bytes */
+void or_track_from_tgt_run(const or_field *F, int64_t n, double *x, double *y, double *z,
+                           double *dx, double *dy, const double *mom, double mass, int spect,
+                           uint8_t *ok, int nthreads);
+void or_track_to_tgt_run(const or_field *F, const or_maps *M, int arm, int64_t n,
+                         const double *x_fp, const double *dx_fp, const double *y_fp,
+                         const double *dy_fp, double *delta, double *y, double *dx, double *dy,
+                         const double *frx, const double *fry, const double *mom, double mass,
+                         double ctheta, double stheta, int spect, uint8_t *ok, int nthreads);
+/* track_to_tgt with the arm's reconstruction of the focal-plane track
+ * (x_fp, dx_fp, y_fp, dy_fp: the /track/ values the recon reads). */
+void or_track_to_tgt(const or_field *F, const or_maps *M, int arm, const double track[4],
+                     double *delta, double *y, double *dx, double *dy, double frx, double fry,
+                     double mom, double mass, double ctheta, double stheta, int spect, int *ok);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,8 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
     python tests/golden/make_golden.py target   # target step (target/*.npz):
         the reference's trip_thru_target and target_musc (target.f) over
         vertices on per-event tapes
+    python tests/golden/make_golden.py field    # target-field tracking (field/*.npz):
+        the reference's trgInit, track_from_tgt and track_to_tgt (trg_track.f)
 """
 import hashlib
 import os
@@ -130,8 +132,65 @@ def main_target():
         print(name, "mean eloss", float(r["eloss"].mean()), "mean teff", float(r["teff"].mean()))
 
 
+# Target-field tracking (trg_track.f): track_from_tgt over vertices, and
+# track_to_tgt over the accepted events of an oracle run of the arm.
+# name, map (None = uniform test field, "0" = zero field), theta_e, theta_p (deg)
+FIELD_FROM_CASES = [
+    ("from_map_proton", "map", 0.0, 10.0, +1, MP_, (800., 3000.)),
+    ("from_map_electron", "map", -6.0, 0.0, -1, ME_, (800., 4000.)),
+    ("from_uniform_proton", None, 0.0, 25.0, +1, MP_, (600., 2000.)),
+    ("from_zero_field", "0", 0.0, 0.0, +1, MP_, (1000., 2000.)),
+]
+# name, arm, workload, spect, mass, charge sign, arm angle (deg), map angles (deg)
+FIELD_TO_CASES = [
+    ("to_hms_proton", 1, W.HMS_C0, +1, MP_, +1.0, 20.0, (10.0, 10.0)),
+    ("to_shms_electron", 5, W.SHMS_C1, -1, ME_, -1.0, 12.5, (-8.0, 0.0)),
+    ("to_hrsl_electron", 4, W.HRSL_3, -1, ME_, -1.0, 15.0, (5.0, 0.0)),
+]
+NF = 1000
+
+
+def main_field():
+    out_dir = os.path.join(HERE, "field")
+    os.makedirs(out_dir, exist_ok=True)
+    for i, (name, mp, te, tp, spect, mass, prange) in enumerate(FIELD_FROM_CASES):
+        rng = np.random.default_rng(700 + i)
+        ev = dict(x=rng.uniform(-0.5, 0.5, NF), y=rng.uniform(-0.5, 0.5, NF), z=rng.uniform(-1.5, 1.5, NF),
+                  dx=rng.uniform(-0.06, 0.06, NF), dy=rng.uniform(-0.04, 0.04, NF))
+        mom = spect * rng.uniform(*prange, NF)
+        path = O.FIELD_MAP if mp == "map" else mp
+        r = O.ref_track_from_tgt(path, te, tp, ev, mom, mass, spect)
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), kind="from", map=str(mp), theta_e=te,
+                            theta_p=tp, spect=spect, mass=mass, mom=mom,
+                            **{"in_" + k: ev[k] for k in ev}, **{"out_" + k: r[k] for k in r})
+        print(name, "ok", int(r["ok"].sum()), "of", NF)
+    for i, (name, arm, wl, spect, mass, sgn, th_deg, (te, tp)) in enumerate(FIELD_TO_CASES):
+        rng = np.random.default_rng(800 + i)
+        ev = W.generate(wl, 4 * NF, seed=900 + i)
+        res = O.run(arm, ev, p_spec=wl.p_spec, m2=mass * mass, rng="philox", seed=31 + i, fit_f32=False)
+        sel = np.flatnonzero(res["ok"] == 1)[:NF]
+        trk = {k: res[k][sel] for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")}
+        if arm in (3, 4):  # /track/ holds the fitted y, y_fp carries the VDC offset
+            trk["y_fp"] = trk["y_fp"] + (0.48 if arm == 3 else 0.78)
+        first = dict(delta=res["dpp"][sel], y=res["y"][sel], dx=res["dxdz"][sel], dy=res["dydz"][sel])
+        m = len(sel)
+        frx = rng.uniform(-0.2, 0.2, m)
+        fry = rng.uniform(-0.2, 0.2, m)
+        mom = sgn * wl.p_spec * (1 + first["delta"] / 100)
+        th = np.radians(th_deg)
+        st = np.sin(th) if arm in (1, 3) else -np.sin(th)  # simc.f:1573-1584
+        r = O.ref_track_to_tgt(O.FIELD_MAP, te, tp, arm, trk, first, frx, -fry, mom, mass, np.cos(th), st, spect)
+        np.savez_compressed(os.path.join(out_dir, name + ".npz"), kind="to", map="map", theta_e=te, theta_p=tp,
+                            arm=arm, spect=spect, mass=mass, mom=mom, frx=frx, fry=-fry, ctheta=np.cos(th),
+                            stheta=st, **{"trk_" + k: trk[k] for k in trk},
+                            **{"in_" + k: first[k] for k in first}, **{"out_" + k: r[k] for k in r})
+        print(name, "ok", int(r["ok"].sum()), "of", m)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["target"]:
         main_target()
+    elif sys.argv[1:] == ["field"]:
+        main_field()
     else:
         main(sys.argv[1] if sys.argv[1:] else None)
