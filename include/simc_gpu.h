@@ -321,6 +321,45 @@ int simc_target_prep(simc_ctx *ctx, const simc_target *t, const simc_prep_settin
 int simc_target_prep_host(simc_ctx *ctx, const simc_target *t, const simc_prep_settings *s,
                           const simc_vertex_in *in, simc_prep_out *out, int64_t n, const simc_rng *rng);
 
+/* ---- Target-field tracking (polarized-target field, trg_track.f) ----
+ * simc_field_create = trgInit(map, theta_e, phi_e, theta_p, phi_p)
+ * (trg_track.f:243-325): map = NULL or blank for the uniform 5 T test field,
+ * "0" for no field, else the 51 x 51 field-map file (e.g. trg_field_map.dat); angles
+ * in degrees between the field and the electron (e) / hadron (p) arm, phi
+ * ignored as in the reference.  The map is copied to the current device. */
+typedef struct simc_field simc_field;
+int simc_field_create(const char *map, double theta_e, double phi_e, double theta_p, double phi_p,
+                      simc_field **out);
+void simc_field_destroy(simc_field *f);
+/* track_from_tgt(x, y, z, dx, dy, mom, mass, spect, ok), trg_track.f:591-658,
+ * for n events (device arrays, asynchronous on `stream`): x..dy INOUT, from
+ * the vertex to the field-free plane z = 100 cm; mom signed (MeV/c, < 0 for
+ * negative particles), spect -1 (electron-arm field angle) or +1 (hadron
+ * arm); ok may be NULL. */
+int simc_track_from_tgt(const simc_field *f, int spect, int64_t n, double *x, double *y, double *z,
+                        double *dx, double *dy, const double *mom, double mass, uint8_t *ok,
+                        void *stream);
+/* track_to_tgt(delta, y, dx, dy, frx, fry, mom, mass, ctheta, stheta, spect,
+ * ok, arm), trg_track.f:738-876, for n events: delta, y, dx, dy INOUT (first
+ * guess from the arm's reconstruction in, field-corrected reconstruction
+ * out); x_fp..dy_fp the focal-plane track the arm's reconstruction reads
+ * (/track/: for the HRS the fitted y, i.e. y_fp + VDC offset); fry as the
+ * reference receives it (simc.f passes -fry); ok INOUT (NULL: all true in,
+ * not written). */
+int simc_track_to_tgt(simc_ctx *ctx, const simc_field *f, int arm, int spect, int64_t n,
+                      const double *x_fp, const double *dx_fp, const double *y_fp,
+                      const double *dy_fp, double *delta, double *y, double *dx, double *dy,
+                      const double *frx, const double *fry, const double *mom, double mass,
+                      double ctheta, double stheta, uint8_t *ok, void *stream);
+/* The same two on host arrays (synchronous; calls on one field are serialised). */
+int simc_track_from_tgt_host(simc_field *f, int spect, int64_t n, double *x, double *y, double *z,
+                             double *dx, double *dy, const double *mom, double mass, uint8_t *ok);
+int simc_track_to_tgt_host(simc_ctx *ctx, simc_field *f, int arm, int spect, int64_t n,
+                           const double *x_fp, const double *dx_fp, const double *y_fp,
+                           const double *dy_fp, double *delta, double *y, double *dx, double *dy,
+                           const double *frx, const double *fry, const double *mom, double mass,
+                           double ctheta, double stheta, uint8_t *ok);
+
 /* STOP counters accumulated by the context since creation / reset, in the
  * order of the reference's common blocks.  Synchronises the device. */
 int simc_counters_get(simc_ctx *ctx, int arm, int64_t *counters, int ncounters);

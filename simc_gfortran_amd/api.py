@@ -67,6 +67,8 @@ EXPORTS = [
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
     "simc_target_prep", "simc_target_prep_host",
+    "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
+    "simc_track_from_tgt_host", "simc_track_to_tgt_host",
     "simc_last_error",
     "simc_version",
 ]
@@ -173,6 +175,15 @@ def library(path: str = LIB_PATH):
                                    C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng), _vp]
     L.simc_target_prep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
                                         C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng)]
+    L.simc_field_create.argtypes = [C.c_char_p] + [C.c_double] * 4 + [C.POINTER(_vp)]
+    L.simc_field_destroy.argtypes = [_vp]
+    L.simc_field_destroy.restype = None
+    L.simc_track_from_tgt.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 6 + [C.c_double, _vp, _vp]
+    L.simc_track_to_tgt.argtypes = ([_vp, _vp, C.c_int, C.c_int, C.c_int64] + [_vp] * 11 +
+                                    [C.c_double] * 3 + [_vp, _vp])
+    L.simc_track_from_tgt_host.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 6 + [C.c_double, _vp]
+    L.simc_track_to_tgt_host.argtypes = ([_vp, _vp, C.c_int, C.c_int, C.c_int64] + [_vp] * 11 +
+                                         [C.c_double] * 3 + [_vp])
     L.simc_batch_destroy.argtypes = [_vp]
     L.simc_batch_destroy.restype = None
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
@@ -498,6 +509,76 @@ def mc_sos(ctx, p_spec, th_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag
     del th_spec
     return _arm_call(ctx, SOS, p_spec, dpp, x, y, z, dxdz, dydz, m2, ms_flag, wcs_flag,
                      decay_flag, fry, pathlen, False, ctau, **kw)
+
+
+FIELD_MAP = os.path.join(REPO, "data", "fields", "trg_field_map.dat")
+
+
+class TargetField:
+    """A polarized-target field on the device (simc_field_create = trgInit,
+    trg_track.f:243-325).  map_path None = the uniform 5 T test field, "0" =
+    no field; theta_e / theta_p: field angle relative to the electron / hadron
+    arm in degrees."""
+
+    def __init__(self, map_path=FIELD_MAP, theta_e=0.0, theta_p=0.0):
+        h = _vp()
+        _check(library().simc_field_create(None if map_path is None else map_path.encode(), theta_e, 0.0,
+                                           theta_p, 0.0, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            library().simc_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def track_from_tgt(self, ev, mom, mass, spect, ok=None, stream=None):
+        """track_from_tgt (trg_track.f:591-658) in place on the float64 device
+        tensors ev["x"], ["y"], ["z"], ["dx"], ["dy"]; mom signed per event.
+        Returns ok (uint8)."""
+        n = ev["x"].numel()
+        t = [_dev_f64(ev[k], n, k) for k in ("x", "y", "z", "dx", "dy")]
+        m = _dev_f64(mom, n, "mom")
+        ok = torch.zeros(n, dtype=torch.uint8, device=t[0].device) if ok is None else ok
+        st = stream if stream is not None else torch.cuda.current_stream(t[0].device).cuda_stream
+        _check(library().simc_track_from_tgt(self._h, spect, n, *[_ptr(v) for v in t], _ptr(m), mass, _ptr(ok),
+                                             C.c_void_p(st)))
+        return ok
+
+    def track_to_tgt(self, ctx, arm, track, rec, frx, fry, mom, mass, ctheta, stheta, spect, ok=None,
+                     stream=None):
+        """track_to_tgt (trg_track.f:738-876) in place on rec["delta"], ["y"],
+        ["dx"], ["dy"] (device tensors), with the arm's reconstruction of
+        track["x_fp"], ["dx_fp"], ["y_fp"], ["dy_fp"] (the /track/ values).
+        ok: uint8 device tensor INOUT (None: all true).  Returns ok."""
+        n = rec["delta"].numel()
+        tr = [_dev_f64(track[k], n, k) for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")]
+        io = [_dev_f64(rec[k], n, k) for k in ("delta", "y", "dx", "dy")]
+        f = [_dev_f64(v, n, k) for v, k in ((frx, "frx"), (fry, "fry"), (mom, "mom"))]
+        ok = torch.ones(n, dtype=torch.uint8, device=io[0].device) if ok is None else ok
+        st = stream if stream is not None else torch.cuda.current_stream(io[0].device).cuda_stream
+        _check(library().simc_track_to_tgt(ctx._h, self._h, arm, spect, n, *[_ptr(v) for v in tr],
+                                           *[_ptr(v) for v in io], *[_ptr(v) for v in f], mass, ctheta, stheta,
+                                           _ptr(ok), C.c_void_p(st)))
+        return ok
+
+    def track_from_tgt_host(self, ev, mom, mass, spect):
+        """The same on numpy arrays (copies returned)."""
+        import numpy as np
+        out = {k: np.array(ev[k], dtype=np.float64, copy=True) for k in ("x", "y", "z", "dx", "dy")}
+        n = len(out["x"])
+        m = np.ascontiguousarray(mom, dtype=np.float64)
+        ok = np.zeros(n, np.uint8)
+        hp = lambda a: a.ctypes.data_as(_vp)  # noqa: E731
+        _check(library().simc_track_from_tgt_host(self._h, spect, n, *[hp(out[k]) for k in ("x", "y", "z", "dx", "dy")],
+                                                  hp(m), mass, hp(ok)))
+        out["ok"] = ok
+        return out
 
 
 class Batch:

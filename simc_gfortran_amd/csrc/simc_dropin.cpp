@@ -22,6 +22,8 @@
 //            the drop-in adds the offset back, so /track/ ys may differ from the
 //            fitted value by one rounding;
 //   /decd/   ctau, its first member (simulate.inc:153).
+// trgInit / track_from_tgt / track_to_tgt (trg_track.f) are replaced too: the
+// target field lives on the GPU, one synchronous launch per call.
 //
 // Random numbers: a counter-based Philox4x32-10 stream per event, keyed by
 // (seed, call number); seed from SIMC_SEED (default 0) or simc_dropin_seed.
@@ -33,6 +35,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <string>
 
 #include "../../include/simc_gpu.h"
 
@@ -54,6 +57,7 @@ bool g_failed = false;
 uint64_t g_seed = 0;
 uint64_t g_event = 0;
 bool g_seed_set = false;
+simc_field* g_field = nullptr;
 
 simc_ctx* context()
 {
@@ -261,6 +265,63 @@ void mc_shms_recon_(double* delta_p, double* delta_t, double* delta_phi, double*
     recon_event(SIMC_ARM_SHMS, delta_p, delta_t, delta_phi, y_tgt, fry);
 }
 
+// ---- target-field tracking (trg_track.f) ----
+// trgInit(map, theta_e, phi_e, theta_p, phi_p), trg_track.f:243: map is a
+// blank-padded CHARACTER (hidden length argument): all blanks = the uniform
+// test field, '0' = no field, else the map file (opened relative to the
+// working directory, like the reference).
+void trginit_(const char* map, double* theta_e, double* phi_e, double* theta_p, double* phi_p, size_t map_len)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    std::string m(map, map_len);
+    while (!m.empty() && (m.back() == ' ' || m.back() == '\0')) m.pop_back();
+    if (g_field) simc_field_destroy(g_field);
+    g_field = nullptr;
+    if (simc_field_create(m.empty() ? nullptr : m.c_str(), *theta_e, *phi_e, *theta_p, *phi_p, &g_field) != SIMC_OK)
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+}
+
+// track_from_tgt(x, y, z, dx, dy, mom, mass, spect, ok), trg_track.f:591
+void track_from_tgt_(double* x, double* y, double* z, double* dx, double* dy, double* mom, double* mass,
+                     int32_t* spect, int32_t* ok)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    *ok = 0;
+    if (!g_field) {
+        std::fprintf(stderr, "simc_dropin: track_from_tgt before trgInit\n");
+        return;
+    }
+    uint8_t o = 0;
+    if (simc_track_from_tgt_host(g_field, *spect, 1, x, y, z, dx, dy, mom, *mass, &o) != SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        return;
+    }
+    *ok = o ? 1 : 0;
+}
+
+// track_to_tgt(delta, y, dx, dy, frx, fry, mom, mass, ctheta, stheta, spect,
+// ok, arm), trg_track.f:738: the arm's reconstruction reads the focal-plane
+// track from /track/ (arm 6 is the SHMS, :845).
+void track_to_tgt_(double* delta, double* y, double* dx, double* dy, double* frx, double* fry, double* mom,
+                   double* mass, double* ctheta, double* stheta, int32_t* spect, int32_t* ok, int32_t* arm)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g_field || !track_ || !context()) {
+        std::fprintf(stderr, "simc_dropin: track_to_tgt needs trgInit, /track/ and the maps\n");
+        *ok = 0;
+        return;
+    }
+    const int a = *arm == 6 ? SIMC_ARM_SHMS : *arm;
+    uint8_t o = *ok ? 1 : 0;
+    if (simc_track_to_tgt_host(g_ctx, g_field, a, *spect, 1, &track_[0], &track_[4], &track_[1], &track_[5], delta,
+                               y, dx, dy, frx, fry, mom, *mass, *ctheta, *stheta, &o) != SIMC_OK) {
+        std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
+        *ok = 0;
+        return;
+    }
+    *ok = o ? 1 : 0;
+}
+
 // Seed of the per-event streams (Fortran: call simc_dropin_seed(seed) with an
 // integer*8 seed); restarts the event numbering.
 void simc_dropin_seed_(int64_t* seed)
@@ -277,6 +338,8 @@ void simc_dropin_close_()
     std::lock_guard<std::mutex> lock(g_mu);
     if (g_ctx) simc_ctx_destroy(g_ctx);
     g_ctx = nullptr;
+    if (g_field) simc_field_destroy(g_field);
+    g_field = nullptr;
     g_failed = false;
 }
 

@@ -23,6 +23,7 @@
 #include "simc_maps.h"
 #include "simc_ops.h"
 #include "simc_target.h"
+#include "simc_field.h"
 #include "cosy_gen.dev.inc"
 
 using namespace simcdev;
@@ -1247,6 +1248,114 @@ __global__ __launch_bounds__(256) void target_prep_kernel(PrepArgs a)
     if (a.d1) a.d1[i] = d1;
     if (a.d2) a.d2[i] = d2;
     if (a.used) a.used[i] = g.overflow ? -1 : g.used();
+}
+
+// ---------------------------------------------------------------------------
+// Target-field tracking (trg_track.f).  Each workgroup stages the field map
+// (2 x 51 x 51 doubles, 41.6 KB) in LDS; one event per lane.
+// ---------------------------------------------------------------------------
+struct FieldFromArgs {
+    const double* map;  // B_z nodes, then B_r nodes
+    double stht, ctht;
+    uint32_t n;
+    double *x, *y, *z, *dx, *dy;
+    const double* mom;
+    double mass;
+    uint8_t* ok;
+};
+
+struct FieldToArgs {
+    const double* map;
+    double stht, ctht;
+    uint32_t n;
+    const double* coef;  // the arm's packed coefficients (reconstruction)
+    const double *x_fp, *dx_fp, *y_fp, *dy_fp;
+    double *delta, *y, *dx, *dy;
+    const double *frx, *fry, *mom;
+    double mass, ctheta, stheta;
+    uint8_t* ok;
+};
+
+__device__ __forceinline__ void stage_field_map(double* s, const double* map)
+{
+    for (int k = threadIdx.x; k < 2 * kTrgNodes; k += blockDim.x) s[k] = map[k];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void field_from_kernel(FieldFromArgs a)
+{
+    __shared__ double s[2 * kTrgNodes];
+    stage_field_map(s, a.map);
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    const TrgMap M{s, s + kTrgNodes, a.stht, a.ctht};
+    double x = a.x[i], y = a.y[i], z = a.z[i], dx = a.dx[i], dy = a.dy[i];
+    const bool ok = track_from_tgt(M, x, y, z, dx, dy, a.mom[i], a.mass);
+    a.x[i] = x;
+    a.y[i] = y;
+    a.z[i] = z;
+    a.dx[i] = dx;
+    a.dy[i] = dy;
+    if (a.ok) a.ok[i] = ok ? 1 : 0;
+}
+
+// track_to_tgt, trg_track.f:738-876, with the arm's reconstruction
+// (recon<ARM>, the mc_*_recon of :836-846) of the event's focal-plane track
+// at the corrected vertical position.
+template <int ARM>
+__global__ __launch_bounds__(256) void field_to_kernel(FieldToArgs a)
+{
+    __shared__ double s[2 * kTrgNodes];
+    stage_field_map(s, a.map);
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    const TrgMap M{s, s + kTrgNodes, a.stht, a.ctht};
+    const double cc = 29.9792458;
+    EvOut o;
+    o.x_fp = a.x_fp[i];
+    o.dx_fp = a.dx_fp[i];
+    o.y_fp = a.y_fp[i];
+    o.dy_fp = a.dy_fp[i];
+    double delta = a.delta[i], y = a.y[i], dx = a.dx[i], dy = a.dy[i];
+    const double frx = a.frx[i], fry = a.fry[i], mass = a.mass;
+    double mom = a.mom[i];
+    bool ok = a.ok ? a.ok[i] != 0 : true;
+    double xx = -fry;
+    double vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+    double eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+    const double mom_0 = mom / (1.e0 + delta / 100.e0);
+    double vT[6], vTx[6];
+    trg_start(vT, -fry + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
+    trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, frx, ok);
+    int n = 0;
+    double delx = 1.;
+    while ((delx > .0001) && (n < 10) && ok) {
+        delx = fabs(-fry - vT[0]);
+        vTx[0] = -fry;
+#pragma unroll
+        for (int k = 1; k < 6; ++k) vTx[k] = vT[k];
+        trg_track_to_plane(M, vT, eng, 1., 0., 0., 1., 0., ok);
+        trg_track_to_plane(M, vTx, eng, 1., 0., 0., 1., 0., ok);
+        xx = xx + fmin(1., fmax(-1., (vTx[0] - vT[0])));
+        EvOut r;
+        recon<ARM>((cdptr)a.coef, o, xx, r);
+        delta = r.dpp;
+        dy = r.dydz;
+        dx = r.dxdz;
+        y = r.y;
+        mom = mom_0 * (1.e0 + delta / 100.e0);
+        vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+        eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+        trg_start(vT, xx + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
+        trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, frx, ok);
+        n = n + 1;
+    }
+    if (delx > .2) ok = false;
+    a.delta[i] = delta;
+    a.dy[i] = vT[4] / vT[5];
+    a.dx[i] = vT[3] / vT[5];
+    a.y[i] = vT[1];
+    if (a.ok) a.ok[i] = ok ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2844,6 +2953,243 @@ int simc_target_prep_host(simc_ctx* ctx, const simc_target* t, const simc_prep_s
     if (out->used) std::memcpy(out->used, used.data(), 4 * (size_t)n);
     for (int64_t i = 0; i < n; ++i)
         if (used[(size_t)i] < 0) return fail(SIMC_E_TAPE, "event " + std::to_string(i) + " ran out of its tape slot");
+    return SIMC_OK;
+}
+
+}  // extern "C"
+
+// A loaded target field (trgInit): the map on the device and the field
+// angles of the two arms.
+struct simc_field {
+    double* d_map = nullptr;   // B_z nodes, then B_r nodes (2 x 2601)
+    double stht[2] = {}, ctht[2] = {};   // [0]: spect -1 (electron arm), [1]: spect +1
+    int device = 0;
+    std::mutex mu;             // guards the host-pointer staging
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+};
+
+namespace {
+
+int field_angles(const simc_field* f, int spect, double& st, double& ct)
+{
+    if (spect != -1 && spect != 1) return fail(SIMC_E_ARG, "spect must be -1 (electron arm) or +1 (hadron arm)");
+    st = f->stht[spect == -1 ? 0 : 1];
+    ct = f->ctht[spect == -1 ? 0 : 1];
+    return SIMC_OK;
+}
+
+// Staging of host arrays for the host-pointer entry points.
+struct Stager {
+    simc_field* f;
+    size_t at = 0;
+    char* base = nullptr;
+    static size_t rounded(size_t b) { return (b + 255) & ~(size_t)255; }
+    int reserve(size_t need)
+    {
+        if (f->stage_bytes < need) {
+            if (f->d_stage) {
+                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipFree(f->d_stage));
+                f->d_stage = nullptr;
+                f->stage_bytes = 0;
+            }
+            HIPCHK(hipMalloc(&f->d_stage, need));
+            f->stage_bytes = need;
+        }
+        base = (char*)f->d_stage;
+        return SIMC_OK;
+    }
+    template <class T>
+    T* carve(size_t bytes)
+    {
+        T* p = (T*)(base + at);
+        at += rounded(bytes);
+        return p;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int simc_field_create(const char* map, double theta_e, double phi_e, double theta_p, double phi_p,
+                      simc_field** out)
+{
+    if (!out) return fail(SIMC_E_ARG, "simc_field_create: NULL out");
+    *out = nullptr;
+    (void)phi_e;
+    (void)phi_p;  // B_phi is treated as 0 (trg_track.f:255-256)
+    std::vector<double> nodes(2 * kTrgNodes, 0.0);
+    bool blank = true;  // trgInit compares the blank-padded name with ' ' (:300)
+    for (const char* c = map; c && *c; ++c) blank = blank && *c == ' ';
+    if (blank) {  // uniform 5 T over 26 cm in z and 16 cm in r (trg_track.f:311-323)
+        for (int ir = 0; ir < kTrgN; ++ir)
+            for (int iz = 0; iz < kTrgN; ++iz)
+                nodes[ir * kTrgN + iz] = (2. * ir <= 16. && 2. * iz <= 26.) ? 5.0 : 0.0;
+    } else if (std::strcmp(map, "0") != 0) {  // the map file, z fastest (trg_track.f:300-310)
+        FILE* fp = std::fopen(map, "r");
+        if (!fp) return fail(SIMC_E_ARG, std::string("cannot open the field map ") + map);
+        char line[512];
+        for (int k = 0; k < kTrgNodes; ++k) {
+            double v[7];
+            if (!std::fgets(line, sizeof line, fp) ||
+                std::sscanf(line, "%lf %lf %lf %lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                            &v[6]) != 7) {
+                std::fclose(fp);
+                return fail(SIMC_E_MAPFMT, std::string("field map line ") + std::to_string(k + 1) + " of " + map +
+                                               ": expected 7 numbers");
+            }
+            nodes[k] = v[2];
+            nodes[kTrgNodes + k] = v[3];
+        }
+        std::fclose(fp);
+    }
+    simc_field* f = new (std::nothrow) simc_field();
+    if (!f) return fail(SIMC_E_NOMEM, "out of host memory");
+    const double pi180 = 3.141592653 / 180.;  // trg_track.f:266, :271-281
+    f->stht[0] = std::sin(theta_e * pi180);
+    f->ctht[0] = std::cos(theta_e * pi180);
+    f->stht[1] = std::sin(theta_p * pi180);
+    f->ctht[1] = std::cos(theta_p * pi180);
+    if (hipGetDevice(&f->device) != hipSuccess ||
+        hipMalloc(&f->d_map, nodes.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(f->d_map, nodes.data(), nodes.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        if (f->d_map) (void)hipFree(f->d_map);
+        delete f;
+        return fail(SIMC_E_HIP, "simc_field_create: device allocation or copy failed");
+    }
+    *out = f;
+    return SIMC_OK;
+}
+
+void simc_field_destroy(simc_field* f)
+{
+    if (!f) return;
+    if (f->d_map) (void)hipFree(f->d_map);
+    if (f->d_stage) (void)hipFree(f->d_stage);
+    delete f;
+}
+
+int simc_track_from_tgt(const simc_field* f, int spect, int64_t n, double* x, double* y, double* z, double* dx,
+                        double* dy, const double* mom, double mass, uint8_t* ok, void* stream)
+{
+    if (!f || !x || !y || !z || !dx || !dy || !mom) return fail(SIMC_E_ARG, "simc_track_from_tgt: NULL argument");
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_from_tgt: n must be in [0, 2^31]");
+    if (n == 0) return SIMC_OK;
+    FieldFromArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (int rc = field_angles(f, spect, a.stht, a.ctht)) return rc;
+    HIPCHK(hipSetDevice(f->device));
+    a.map = f->d_map;
+    a.n = (uint32_t)n;
+    a.x = x; a.y = y; a.z = z; a.dx = dx; a.dy = dy;
+    a.mom = mom;
+    a.mass = mass;
+    a.ok = ok;
+    hipLaunchKernelGGL(field_from_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
+    HIPCHK(hipGetLastError());
+    return SIMC_OK;
+}
+
+int simc_track_to_tgt(simc_ctx* ctx, const simc_field* f, int arm, int spect, int64_t n, const double* x_fp,
+                      const double* dx_fp, const double* y_fp, const double* dy_fp, double* delta, double* y,
+                      double* dx, double* dy, const double* frx, const double* fry, const double* mom, double mass,
+                      double ctheta, double stheta, uint8_t* ok, void* stream)
+{
+    if (!ctx || !f || !x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !y || !dx || !dy || !frx || !fry || !mom)
+        return fail(SIMC_E_ARG, "simc_track_to_tgt: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_to_tgt: n must be in [0, 2^31]");
+    if (n == 0) return SIMC_OK;
+    FieldToArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (int rc = field_angles(f, spect, a.stht, a.ctht)) return rc;
+    HIPCHK(hipSetDevice(f->device));
+    a.map = f->d_map;
+    a.n = (uint32_t)n;
+    a.coef = ctx->d_coef[ai];
+    a.x_fp = x_fp; a.dx_fp = dx_fp; a.y_fp = y_fp; a.dy_fp = dy_fp;
+    a.delta = delta; a.y = y; a.dx = dx; a.dy = dy;
+    a.frx = frx; a.fry = fry; a.mom = mom;
+    a.mass = mass; a.ctheta = ctheta; a.stheta = stheta;
+    a.ok = ok;
+    const dim3 G((uint32_t)((n + 255) / 256)), B(256);
+    hipStream_t st = (hipStream_t)stream;
+    if (arm == SIMC_ARM_HMS) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_HMS>, G, B, 0, st, a);
+    else if (arm == SIMC_ARM_SOS) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_SOS>, G, B, 0, st, a);
+    else if (is_hrs(arm)) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_HRSR>, G, B, 0, st, a);
+    else hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_SHMS>, G, B, 0, st, a);
+    HIPCHK(hipGetLastError());
+    return SIMC_OK;
+}
+
+int simc_track_from_tgt_host(simc_field* f, int spect, int64_t n, double* x, double* y, double* z, double* dx,
+                             double* dy, const double* mom, double mass, uint8_t* ok)
+{
+    if (!f || !x || !y || !z || !dx || !dy || !mom) return fail(SIMC_E_ARG, "simc_track_from_tgt_host: NULL argument");
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_from_tgt_host: n must be in [0, 2^31]");
+    if (n == 0) return SIMC_OK;
+    std::lock_guard<std::mutex> lock(f->mu);
+    HIPCHK(hipSetDevice(f->device));
+    const size_t nd = (size_t)n * 8;
+    Stager S{f};
+    if (int rc = S.reserve(6 * Stager::rounded(nd) + Stager::rounded((size_t)n))) return rc;
+    double* h[5] = {x, y, z, dx, dy};
+    double* d[5];
+    for (int k = 0; k < 5; ++k) {
+        d[k] = S.carve<double>(nd);
+        HIPCHK(hipMemcpy(d[k], h[k], nd, hipMemcpyHostToDevice));
+    }
+    double* dm = S.carve<double>(nd);
+    HIPCHK(hipMemcpy(dm, mom, nd, hipMemcpyHostToDevice));
+    uint8_t* dok = S.carve<uint8_t>((size_t)n);
+    int rc = simc_track_from_tgt(f, spect, n, d[0], d[1], d[2], d[3], d[4], dm, mass, dok, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    for (int k = 0; k < 5; ++k) HIPCHK(hipMemcpy(h[k], d[k], nd, hipMemcpyDeviceToHost));
+    if (ok) HIPCHK(hipMemcpy(ok, dok, (size_t)n, hipMemcpyDeviceToHost));
+    return SIMC_OK;
+}
+
+int simc_track_to_tgt_host(simc_ctx* ctx, simc_field* f, int arm, int spect, int64_t n, const double* x_fp,
+                           const double* dx_fp, const double* y_fp, const double* dy_fp, double* delta, double* y,
+                           double* dx, double* dy, const double* frx, const double* fry, const double* mom,
+                           double mass, double ctheta, double stheta, uint8_t* ok)
+{
+    if (!ctx || !f || !x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !y || !dx || !dy || !frx || !fry || !mom)
+        return fail(SIMC_E_ARG, "simc_track_to_tgt_host: NULL argument");
+    if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_to_tgt_host: n must be in [0, 2^31]");
+    if (n == 0) return SIMC_OK;
+    std::lock_guard<std::mutex> lock(f->mu);
+    HIPCHK(hipSetDevice(f->device));
+    const size_t nd = (size_t)n * 8;
+    Stager S{f};
+    if (int rc = S.reserve(11 * Stager::rounded(nd) + Stager::rounded((size_t)n))) return rc;
+    const double* hin[7] = {x_fp, dx_fp, y_fp, dy_fp, frx, fry, mom};
+    const double* din[7];
+    for (int k = 0; k < 7; ++k) {
+        double* dd = S.carve<double>(nd);
+        HIPCHK(hipMemcpy(dd, hin[k], nd, hipMemcpyHostToDevice));
+        din[k] = dd;
+    }
+    double* hio[4] = {delta, y, dx, dy};
+    double* dio[4];
+    for (int k = 0; k < 4; ++k) {
+        dio[k] = S.carve<double>(nd);
+        HIPCHK(hipMemcpy(dio[k], hio[k], nd, hipMemcpyHostToDevice));
+    }
+    uint8_t* dok = S.carve<uint8_t>((size_t)n);
+    if (ok) HIPCHK(hipMemcpy(dok, ok, (size_t)n, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(dok, 1, (size_t)n));
+    int rc = simc_track_to_tgt(ctx, f, arm, spect, n, din[0], din[1], din[2], din[3], dio[0], dio[1], dio[2], dio[3],
+                               din[4], din[5], din[6], mass, ctheta, stheta, dok, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    for (int k = 0; k < 4; ++k) HIPCHK(hipMemcpy(hio[k], dio[k], nd, hipMemcpyDeviceToHost));
+    if (ok) HIPCHK(hipMemcpy(ok, dok, (size_t)n, hipMemcpyDeviceToHost));
     return SIMC_OK;
 }
 

@@ -105,6 +105,63 @@ module simc_gpu
       type(simc_rng), intent(in) :: rng
     end function
 
+    ! target-field tracking (trg_track.f): map = c_null_char-terminated file name,
+    ! '0' for no field, blanks for the uniform test field (as trgInit)
+    integer(c_int) function simc_field_create(map, theta_e, phi_e, theta_p, phi_p, f) &
+        bind(C, name="simc_field_create")
+      import :: c_int, c_char, c_double, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: map
+      real(c_double), value :: theta_e, phi_e, theta_p, phi_p
+      type(c_ptr), intent(out) :: f
+    end function
+
+    subroutine simc_field_destroy(f) bind(C, name="simc_field_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: f
+    end subroutine
+
+    integer(c_int) function simc_track_from_tgt(f, spect, n, x, y, z, dx, dy, mom, mass, ok, stream) &
+        bind(C, name="simc_track_from_tgt")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: f, x, y, z, dx, dy, mom, ok, stream
+      integer(c_int), value :: spect
+      integer(c_int64_t), value :: n
+      real(c_double), value :: mass
+    end function
+
+    integer(c_int) function simc_track_to_tgt(ctx, f, arm, spect, n, x_fp, dx_fp, y_fp, dy_fp, delta, y, dx, dy, &
+        frx, fry, mom, mass, ctheta, stheta, ok, stream) bind(C, name="simc_track_to_tgt")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, f, x_fp, dx_fp, y_fp, dy_fp, delta, y, dx, dy, frx, fry, mom, ok, stream
+      integer(c_int), value :: arm, spect
+      integer(c_int64_t), value :: n
+      real(c_double), value :: mass, ctheta, stheta
+    end function
+
+    integer(c_int) function simc_track_from_tgt_host(f, spect, n, x, y, z, dx, dy, mom, mass, ok) &
+        bind(C, name="simc_track_from_tgt_host")
+      import :: c_int, c_int64_t, c_double, c_int8_t, c_ptr
+      type(c_ptr), value :: f
+      integer(c_int), value :: spect
+      integer(c_int64_t), value :: n
+      real(c_double), dimension(*), intent(inout) :: x, y, z, dx, dy
+      real(c_double), dimension(*), intent(in) :: mom
+      real(c_double), value :: mass
+      integer(c_int8_t), dimension(*), intent(out) :: ok
+    end function
+
+    integer(c_int) function simc_track_to_tgt_host(ctx, f, arm, spect, n, x_fp, dx_fp, y_fp, dy_fp, delta, y, dx, &
+        dy, frx, fry, mom, mass, ctheta, stheta, ok) bind(C, name="simc_track_to_tgt_host")
+      import :: c_int, c_int64_t, c_double, c_int8_t, c_ptr
+      type(c_ptr), value :: ctx, f
+      integer(c_int), value :: arm, spect
+      integer(c_int64_t), value :: n
+      real(c_double), dimension(*), intent(in) :: x_fp, dx_fp, y_fp, dy_fp, frx, fry, mom
+      real(c_double), dimension(*), intent(inout) :: delta, y, dx, dy
+      real(c_double), value :: mass, ctheta, stheta
+      integer(c_int8_t), dimension(*), intent(inout) :: ok
+    end function
+
     integer(c_int) function simc_ctx_create(map_dir, ctx) bind(C, name="simc_ctx_create")
       import :: c_int, c_char, c_ptr
       character(kind=c_char), dimension(*), intent(in) :: map_dir

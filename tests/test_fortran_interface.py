@@ -31,5 +31,6 @@ def test_module_compiles(tmp_path):
 def test_dropin_exports_reference_symbols():
     lib = ctypes.CDLL(B.DROPIN)
     for name in ("mc_hms_", "mc_shms_", "mc_sos_", "mc_hrsr_", "mc_hrsl_", "mc_hms_recon_", "mc_shms_recon_",
-                 "mc_sos_recon_", "mc_hrsr_recon_", "mc_hrsl_recon_", "simc_dropin_seed_", "simc_dropin_close_"):
+                 "mc_sos_recon_", "mc_hrsr_recon_", "mc_hrsl_recon_", "trginit_", "track_from_tgt_",
+                 "track_to_tgt_", "simc_dropin_seed_", "simc_dropin_close_"):
         assert hasattr(lib, name), name
