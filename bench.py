@@ -299,6 +299,83 @@ def bench_target(ctx, n=10_000_000, reps=5):
             "data": "synthetic vertices in a 10 cm LH2 cell (can 3), loss and scattering sampled per event"}
 
 
+# FP64 operations of one RK4 step of the target-field tracking (trgRK4,
+# trg_track.f:461-498): four field evaluations (rotation 6, radius 4, grid
+# indices and fractions 8, two bilinear interpolations 18, B_r / r and signs 4,
+# back-rotation 6: 46, counting sqrt and divisions as one) each followed by the
+# cross product and scaling (9), plus the stage combinations (72).
+RK4_FLOP = 4 * (46 + 9) + 72
+
+
+def bench_field(ctx, n=10_000_000, reps=5):
+    """Target-field tracking (the polarized-target map, field at 10 degrees to
+    the hadron arm): track_from_tgt for 1e7 HMS-workload protons from a 3 cm
+    cell to z = 100 cm, and track_to_tgt (reconstruction through the field,
+    up to 10 iterations of three plane searches and the HMS reconstruction)
+    for the accepted events of an HMS run."""
+    import torch
+    from simc_gfortran_amd import api, workloads as W
+    dev = torch.device("cuda:0")
+    wl = W.HMS_C0
+    g = torch.Generator(device=dev)
+    g.manual_seed(77)
+    u = lambda lo, hi: torch.rand(n, dtype=torch.float64, device=dev, generator=g) * (hi - lo) + lo  # noqa: E731
+    ev0 = {"x": u(-0.5, 0.5), "y": u(-0.5, 0.5), "z": u(-1.5, 1.5), "dx": u(*wl.dxdz), "dy": u(*wl.dydz)}
+    mom = wl.p_spec * (1 + u(*wl.dpp) / 100)
+    F = api.TargetField(api.FIELD_MAP, 10.0, 10.0)
+    ev = {k: torch.empty_like(v) for k, v in ev0.items()}
+    ok = torch.empty(n, dtype=torch.uint8, device=dev)
+    st = torch.cuda.current_stream(dev)
+    tf = []
+    for r in range(reps + 1):
+        for k in ev:
+            ev[k].copy_(ev0[k])
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        F.track_from_tgt(ev, mom, wl.mass, 1, ok=ok)
+        b.record(st)
+        torch.cuda.synchronize()
+        if r:
+            tf.append(a.elapsed_time(b))
+    f_ms = float(np.mean(tf))
+    # RK4 steps: the first search crosses |z0| (plus its direction step), the
+    # second the 100 cm to the field-free plane, 1 cm of path per step
+    slope = torch.sqrt(1 + ev0["dx"] ** 2 + ev0["dy"] ** 2)
+    steps = float((((ev0["z"].abs() + 100.0) * slope).ceil() + 4).mean())
+    flop = n * steps * RK4_FLOP
+    # reconstruction through the field for the accepted events of an HMS run
+    run = ctx.run(wl.arm, W.generate_device(wl, 2_000_000, seed=wl.seed, device=dev), p_spec=wl.p_spec,
+                  m2=wl.m2, rng="philox", seed=3, fit="fp64")
+    sel = run["ok"] == 1
+    m = int(sel.sum())
+    trk = {k: run[k][sel].contiguous() for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")}
+    first = {"delta": run["dpp"][sel], "y": run["y"][sel], "dx": run["dxdz"][sel], "dy": run["dydz"][sel]}
+    fr = torch.zeros(m, dtype=torch.float64, device=dev)
+    pm = wl.p_spec * (1 + first["delta"] / 100)
+    th = math.radians(wl.theta_deg)
+    tt = []
+    for r in range(reps + 1):
+        rec = {k: v.clone() for k, v in first.items()}
+        ok2 = torch.ones(m, dtype=torch.uint8, device=dev)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        F.track_to_tgt(ctx, wl.arm, trk, rec, fr, fr, pm, wl.mass, math.cos(th), math.sin(th), 1, ok=ok2)
+        b.record(st)
+        torch.cuda.synchronize()
+        if r:
+            tt.append(a.elapsed_time(b))
+    t_ms = float(np.mean(tt))
+    F.close()
+    return {"workload": "hms_proton_3GeV_polarized_target_field_1e7", "events": n, "from_tgt_ms": f_ms,
+            "from_tgt_events_per_s": n / (f_ms * 1e-3), "rk4_steps_per_event": steps,
+            "rk4_flop_per_step": RK4_FLOP, "from_tgt_TFLOPs": flop / (f_ms * 1e-3) / 1e12,
+            "from_tgt_ok_fraction": float(ok.float().mean()),
+            "to_tgt_events": m, "to_tgt_ms": t_ms, "to_tgt_events_per_s": m / (t_ms * 1e-3),
+            "to_tgt_ok_fraction": float(ok2.float().mean()),
+            "data": "synthetic: vertices uniform in a 3 cm cell, HMS angles and momenta; the reference's "
+                    "trg_field_map.dat at 10 degrees; track_to_tgt on the accepted events of 2e6 HMS events"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,6 +505,7 @@ def main():
         line["batched"] = bench_batched(ctx, line, args)
         line["adjoint"] = bench_adjoint(ctx)
         line["target_step"] = bench_target(ctx)
+        line["target_field"] = bench_field(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
