@@ -360,6 +360,28 @@ int simc_track_to_tgt_host(simc_ctx *ctx, simc_field *f, int arm, int spect, int
                            const double *frx, const double *fry, const double *mom, double mass,
                            double ctheta, double stheta, uint8_t *ok);
 
+/* ---- Output formats ----
+ * The .bin ntuple (simc.f:71-74): a Fortran unformatted sequential file with
+ * 4-byte record lengths around every record -- the column count (INTEGER*4)
+ * and the 16-character column names (NtupleInit.f:351-355), then one REAL*8
+ * record per column of every event written (results_write.f:262-264).
+ * tags: ncols (<= 80) names as NtupleInit.f gives them for the run's mode. */
+typedef struct simc_ntuple simc_ntuple;
+int simc_ntuple_open(const char *path, int ncols, const char *const *tags, simc_ntuple **out);
+/* nrows events from a host array, row-major (nrows x ncols). */
+int simc_ntuple_write(simc_ntuple *nt, const double *rows, int64_t nrows);
+/* The events i with select[i] != 0 (select NULL: all n) of ncols DEVICE
+ * columns cols[c][i], in event order: compacted and formatted on the GPU,
+ * appended to the file before returning (the stream is synchronised);
+ * *written (may be NULL) receives the number of events written. */
+int simc_ntuple_write_device(simc_ntuple *nt, const double *const *cols, const uint8_t *select, int64_t n,
+                             int64_t *written, void *stream);
+int simc_ntuple_close(simc_ntuple *nt);
+/* The STOP-counter report of simc.f:462-530 (the .geni file) from the
+ * context's counters, for the arms electron_arm and hadron_arm (codes as in
+ * the reference, 6 = SHMS too), in gfortran's list-directed layout. */
+int simc_stop_report(simc_ctx *ctx, const char *path, int electron_arm, int hadron_arm);
+
 /* STOP counters accumulated by the context since creation / reset, in the
  * order of the reference's common blocks.  Synchronises the device. */
 int simc_counters_get(simc_ctx *ctx, int arm, int64_t *counters, int ncounters);

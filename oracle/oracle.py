@@ -445,3 +445,29 @@ def ref_track_to_tgt(map_path, theta_e, theta_p, arm, trk, first, frx, fry, mom,
     assert rc == 0, rc
     out["ok"] = ok
     return out
+
+
+# ---------------------------------------------------------------------------
+# the .bin ntuple written by the reference's NtupleInit / NtupleClose
+# ---------------------------------------------------------------------------
+NTUPLE_MODES = {"elastic": 1, "pion": 2, "semi": 3}
+
+
+def ref_ntuple(path, mode, tgt_field, rows):
+    """Writes `path` with the reference's NtupleInit (for `mode` and
+    using_tgt_field) and one record per value of each row, as
+    results_ntu_write does; closes it with NtupleClose."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    m = C.c_int32(NTUPLE_MODES[mode])
+    t = C.c_int32(1 if tgt_field else 0)
+    L.ref_ntuple_mode_(C.byref(m), C.byref(t))
+    name = path.encode().ljust(80)
+    if len(name) > 80:
+        raise ValueError("path longer than CHARACTER*80")
+    L.ntupleinit_(C.c_char_p(name), C.c_size_t(80))
+    for r in np.ascontiguousarray(rows, dtype=np.float64):
+        r = np.ascontiguousarray(r)
+        L.ref_ntuple_event_(r.ctypes.data_as(C.c_void_p))
+    L.ntupleclose_(C.c_char_p(name), C.c_size_t(80))

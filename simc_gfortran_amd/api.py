@@ -69,6 +69,7 @@ EXPORTS = [
     "simc_target_prep", "simc_target_prep_host",
     "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
     "simc_track_from_tgt_host", "simc_track_to_tgt_host",
+    "simc_ntuple_open", "simc_ntuple_write", "simc_ntuple_write_device", "simc_ntuple_close", "simc_stop_report",
     "simc_last_error",
     "simc_version",
 ]
@@ -184,6 +185,11 @@ def library(path: str = LIB_PATH):
     L.simc_track_from_tgt_host.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 6 + [C.c_double, _vp]
     L.simc_track_to_tgt_host.argtypes = ([_vp, _vp, C.c_int, C.c_int, C.c_int64] + [_vp] * 11 +
                                          [C.c_double] * 3 + [_vp])
+    L.simc_ntuple_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(_vp)]
+    L.simc_ntuple_write.argtypes = [_vp, _vp, C.c_int64]
+    L.simc_ntuple_write_device.argtypes = [_vp, C.POINTER(_vp), _vp, C.c_int64, C.POINTER(C.c_int64), _vp]
+    L.simc_ntuple_close.argtypes = [_vp]
+    L.simc_stop_report.argtypes = [_vp, C.c_char_p, C.c_int, C.c_int]
     L.simc_batch_destroy.argtypes = [_vp]
     L.simc_batch_destroy.restype = None
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
@@ -579,6 +585,120 @@ class TargetField:
                                                   hp(m), mass, hp(ok)))
         out["ok"] = ok
         return out
+
+
+# Column names of the .bin ntuple per analysis mode, NtupleInit.f:36-341
+# (the names are part of the format that util/root_tree and analyses read).
+_NT_COMMON = ["hsdelta", "hsyptar", "hsxptar", "hsytar", "hsxfp", "hsxpfp", "hsyfp", "hsypfp", "hsdeltai",
+              "hsyptari", "hsxptari", "hsytari", "ssdelta", "ssyptar", "ssxptar", "ssytar", "ssxfp", "ssxpfp",
+              "ssyfp", "ssypfp", "ssdeltai", "ssyptari", "ssxptari", "ssytari", "q", "nu", "Q2", "W", "epsilon",
+              "Em", "Pm", "thetapq", "phipq"]
+_NT_TGT_FIELD = ["th_tarq", "phitarq", "beta", "phis", "phic", "betai", "phisi", "phici"]
+_NT_PIZERO = ["xcal_gamma1", "ycal_gamma1", "Egamma1", "Pgamma1x", "Pgamma1y", "Pgamma1z", "xcal_gamma2",
+              "ycal_gamma2", "Egamma2", "Pgamma2x", "Pgamma2y", "Pgamma2z"]
+
+
+def ntuple_tags(mode, tgt_field=False, kaon=False, rho=False, pizero=False):
+    """The ntuple's column names for mode 'elastic' (doing_hyd_elast,
+    doing_deuterium, doing_heavy), 'pion' (doing_pion, doing_kaon,
+    doing_delta) or 'semi' (doing_semi, doing_rho), NtupleInit.f:36-341."""
+    t = list(_NT_COMMON)
+    if mode == "pion":
+        t += ["missmass", "mmnuc", "phad", "t", "pmpar", "pmper", "pmoop", "fry", "radphot", "pfermi", "siglab",
+              "sigcm", "Weight", "decdist", "Mhadron", "pdotqhat", "Q2i", "Wi", "ti", "phipqi"]
+        t += _NT_TGT_FIELD if tgt_field else []
+        t += ["saghai", "factor"] if kaon else []
+        t += _NT_PIZERO if pizero else []
+    elif mode == "semi":
+        t += ["missmass", "ppi", "t", "fry", "radphot", "siglab", "sigcent", "Weight", "decdist", "Mhadron", "z",
+              "zi", "pt2", "pt2i", "xbj", "xbji", "thqi", "sighad", "jacobian", "centjac", "pfermi", "xfermi",
+              "phipqi"]
+        t += _NT_TGT_FIELD if tgt_field else []
+        t += ["Mrho", "Thrho", "mmnuc"] if rho else []
+        t += _NT_PIZERO if pizero else []
+    elif mode == "elastic":
+        t += ["corrsing", "Pmx", "Pmy", "Pmz", "PmPar", "PmPer", "PmOop", "fry", "radphot", "sigcc", "Weight",
+              "theta_e", "theta_p"]
+    else:
+        raise ValueError("mode must be 'elastic', 'pion' or 'semi'")
+    return t
+
+
+class Ntuple:
+    """The reference's .bin ntuple (simc_ntuple_*): open with the column
+    names, append events from host rows or selected events of device
+    columns, close."""
+
+    def __init__(self, path, tags):
+        self.tags = list(tags)
+        arr = (C.c_char_p * len(self.tags))(*[t.encode() for t in self.tags])
+        h = _vp()
+        _check(library().simc_ntuple_open(path.encode(), len(self.tags), arr, C.byref(h)))
+        self._h = h
+
+    def write(self, rows):
+        import numpy as np
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        if rows.ndim != 2 or rows.shape[1] != len(self.tags):
+            raise ValueError(f"rows must be (n, {len(self.tags)})")
+        _check(library().simc_ntuple_write(self._h, rows.ctypes.data_as(_vp), rows.shape[0]))
+
+    def write_device(self, cols, select=None, stream=None):
+        """cols: one float64 device tensor per column; select: uint8 device
+        tensor (None: all events).  Returns the number of events written."""
+        if len(cols) != len(self.tags):
+            raise ValueError(f"expected {len(self.tags)} columns")
+        n = cols[0].numel()
+        cs = [_dev_f64(c, n, f"column {k}") for k, c in enumerate(cols)]
+        ptrs = (_vp * len(cs))(*[c.data_ptr() for c in cs])
+        if select is not None and (select.dtype != torch.uint8 or select.numel() != n or not select.is_cuda):
+            raise TypeError("select: expected a uint8 device tensor of the columns' length")
+        st = stream if stream is not None else torch.cuda.current_stream(cs[0].device).cuda_stream
+        w = C.c_int64()
+        _check(library().simc_ntuple_write_device(self._h, ptrs, _ptr(select), n, C.byref(w), C.c_void_p(st)))
+        return w.value
+
+    def close(self):
+        if self._h:
+            _check(library().simc_ntuple_close(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_ntuple(path):
+    """(tags, rows) of a .bin ntuple: rows is an (events, columns) float64 array."""
+    import numpy as np
+    raw = open(path, "rb").read()
+    pos = 0
+
+    def record():
+        nonlocal pos
+        n = int.from_bytes(raw[pos:pos + 4], "little", signed=True)
+        body = raw[pos + 4:pos + 4 + n]
+        if int.from_bytes(raw[pos + 4 + n:pos + 8 + n], "little", signed=True) != n:
+            raise ValueError(f"{path}: record framing broken at byte {pos}")
+        pos += 8 + n
+        return body
+    ncols = int.from_bytes(record(), "little", signed=True)
+    tags = [record().decode().rstrip() for _ in range(ncols)]
+    rest = np.frombuffer(raw, dtype=np.uint8, offset=pos)
+    if rest.size % (16 * ncols):
+        raise ValueError(f"{path}: truncated event records")
+    words = rest.view("<i4").reshape(-1, 4)
+    if not ((words[:, 0] == 8) & (words[:, 3] == 8)).all():
+        raise ValueError(f"{path}: event records are not 8-byte records")
+    vals = rest.reshape(-1, 16)[:, 4:12].copy().view("<f8").reshape(-1, ncols)
+    return tags, vals
+
+
+def stop_report(ctx, path, electron_arm, hadron_arm):
+    """The STOP-counter report of simc.f:462-530 (.geni) from ctx's counters."""
+    _check(library().simc_stop_report(ctx._h, path.encode(), electron_arm, hadron_arm))
 
 
 class Batch:

@@ -3,7 +3,8 @@
     python -m simc_gfortran_amd.build
 
 Steps: generate the COSY polynomial code from data/maps (gen_cosy.py), then
-compile the HIP kernels and the host map reader into one shared library;
+compile the HIP kernels, the output formats and the host map reader into one
+shared library;
 build the per-event Fortran-ABI drop-in libsimc_dropin.so on top of it; and,
 when a Fortran compiler is present, compile the Fortran module simc_gpu and
 the Fortran test programs of tests/fortran against both libraries.
@@ -42,7 +43,8 @@ def build(verbose: bool = True) -> str:
     summary = gen_cosy.generate(MAP_DIR, os.path.join(GEN, "cosy_gen"))
     if verbose:
         print("gen_cosy:", summary)
-    srcs = [os.path.join(CSRC, "simc_kernels.hip"), os.path.join(CSRC, "simc_maps.cpp")]
+    srcs = [os.path.join(CSRC, "simc_kernels.hip"), os.path.join(CSRC, "simc_io.hip"),
+            os.path.join(CSRC, "simc_maps.cpp")]
     cmd = [HIPCC, *HIP_FLAGS, "-I", GEN, "-I", CSRC, "-I", os.path.join(REPO, "include"),
            *srcs, "-o", LIB + ".tmp"]
     if verbose:

@@ -1378,6 +1378,10 @@ int fail(int code, const std::string& msg)
 
 }  // namespace
 
+namespace simc {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace simc
+
 constexpr int kNumArms = 5;  // HMS, SHMS, SOS, HRS-R, HRS-L (arm_index)
 
 struct simc_ctx {

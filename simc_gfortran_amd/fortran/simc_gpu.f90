@@ -162,6 +162,44 @@ module simc_gpu
       integer(c_int8_t), dimension(*), intent(inout) :: ok
     end function
 
+    ! output formats: the .bin ntuple (tags: array of c_loc of null-terminated
+    ! names) and the STOP-counter report
+    integer(c_int) function simc_ntuple_open(path, ncols, tags, nt) bind(C, name="simc_ntuple_open")
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: ncols
+      type(c_ptr), dimension(*), intent(in) :: tags
+      type(c_ptr), intent(out) :: nt
+    end function
+
+    integer(c_int) function simc_ntuple_write(nt, rows, nrows) bind(C, name="simc_ntuple_write")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: nt
+      real(c_double), dimension(*), intent(in) :: rows
+      integer(c_int64_t), value :: nrows
+    end function
+
+    integer(c_int) function simc_ntuple_write_device(nt, cols, sel, n, written, stream) &
+        bind(C, name="simc_ntuple_write_device")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: nt, sel, stream
+      type(c_ptr), dimension(*), intent(in) :: cols
+      integer(c_int64_t), value :: n
+      integer(c_int64_t), intent(out) :: written
+    end function
+
+    integer(c_int) function simc_ntuple_close(nt) bind(C, name="simc_ntuple_close")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nt
+    end function
+
+    integer(c_int) function simc_stop_report(ctx, path, electron_arm, hadron_arm) bind(C, name="simc_stop_report")
+      import :: c_int, c_char, c_ptr
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int), value :: electron_arm, hadron_arm
+    end function
+
     integer(c_int) function simc_ctx_create(map_dir, ctx) bind(C, name="simc_ctx_create")
       import :: c_int, c_char, c_ptr
       character(kind=c_char), dimension(*), intent(in) :: map_dir
