@@ -1264,18 +1264,6 @@ struct FieldFromArgs {
     uint8_t* ok;
 };
 
-struct FieldToArgs {
-    const double* map;
-    double stht, ctht;
-    uint32_t n;
-    const double* coef;  // the arm's packed coefficients (reconstruction)
-    const double *x_fp, *dx_fp, *y_fp, *dy_fp;
-    double *delta, *y, *dx, *dy;
-    const double *frx, *fry, *mom;
-    double mass, ctheta, stheta;
-    uint8_t* ok;
-};
-
 __device__ __forceinline__ void stage_field_map(double* s, const double* map)
 {
     for (int k = threadIdx.x; k < 2 * kTrgNodes; k += blockDim.x) s[k] = map[k];
@@ -1299,64 +1287,148 @@ __global__ __launch_bounds__(256) void field_from_kernel(FieldFromArgs a)
     if (a.ok) a.ok[i] = ok ? 1 : 0;
 }
 
-// track_to_tgt, trg_track.f:738-876, with the arm's reconstruction
-// (recon<ARM>, the mc_*_recon of :836-846) of the event's focal-plane track
-// at the corrected vertical position.
-template <int ARM>
-__global__ __launch_bounds__(256) void field_to_kernel(FieldToArgs a)
+// track_to_tgt (trg_track.f:738-876) in rounds.  The reference's loop
+// alternates plane searches (RK4, a few hundred steps) with the arm's
+// reconstruction, and events leave it after 1 to 10 iterations.  One lane
+// per event through the whole loop would run every wave as long as its
+// slowest lane (3.3 times the mean path on HMS events), so the loop is cut
+// at the reconstruction: a start kernel does the first search and the
+// iteration's searches up to the reconstruction; then, per round, a
+// reconstruction kernel and a search kernel run over the list of events
+// still iterating, the search kernel appending the events that continue to
+// the next round's list.  Per event the operations and their order are the
+// reference's, so the rounds change nothing in the results.
+struct FieldToArgs {
+    const double* map;
+    double stht, ctht;
+    uint32_t n;
+    const double* coef;  // the arm's packed coefficients (reconstruction)
+    const double *x_fp, *dx_fp, *y_fp, *dy_fp;
+    double *delta, *y, *dx, *dy;
+    const double *frx, *fry, *mom;
+    double mass, ctheta, stheta;
+    uint8_t* ok;         // INOUT (nullptr: all true, not written)
+    // per-event loop state (workspace)
+    double *xx, *mom0, *delx;
+    uint8_t* st_ok;
+    uint8_t* iter;
+    const uint32_t* list_in;   // events of this round
+    const uint32_t* count_in;
+    uint32_t* list_out;        // events of the next round
+    uint32_t* count_out;
+};
+
+__device__ __forceinline__ void field_energy(double mom, double mass, double& vel, double& eng)
 {
-    __shared__ double s[2 * kTrgNodes];
-    stage_field_map(s, a.map);
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.n) return;
-    const TrgMap M{s, s + kTrgNodes, a.stht, a.ctht};
     const double cc = 29.9792458;
-    EvOut o;
-    o.x_fp = a.x_fp[i];
-    o.dx_fp = a.dx_fp[i];
-    o.y_fp = a.y_fp[i];
-    o.dy_fp = a.dy_fp[i];
-    double delta = a.delta[i], y = a.y[i], dx = a.dx[i], dy = a.dy[i];
-    const double frx = a.frx[i], fry = a.fry[i], mass = a.mass;
-    double mom = a.mom[i];
-    bool ok = a.ok ? a.ok[i] != 0 : true;
-    double xx = -fry;
-    double vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
-    double eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
-    const double mom_0 = mom / (1.e0 + delta / 100.e0);
-    double vT[6], vTx[6];
-    trg_start(vT, -fry + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
-    trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, frx, ok);
-    int n = 0;
-    double delx = 1.;
-    while ((delx > .0001) && (n < 10) && ok) {
+    vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
+    eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
+}
+
+// The loop test and, while it holds, the two searches to z = 0 and the new
+// vertical position (trg_track.f:803-821); otherwise the final values
+// (:864-873).  vT is the track at the beam plane.
+__device__ __forceinline__ void field_to_iterate(const TrgMap& M, const FieldToArgs& a, uint32_t i, double (&vT)[6],
+                                                 double eng, double xx, double delx, int n, bool ok)
+{
+    const double fry = a.fry[i];
+    if ((delx > .0001) && (n < 10) && ok) {
         delx = fabs(-fry - vT[0]);
+        double vTx[6];
         vTx[0] = -fry;
 #pragma unroll
         for (int k = 1; k < 6; ++k) vTx[k] = vT[k];
         trg_track_to_plane(M, vT, eng, 1., 0., 0., 1., 0., ok);
         trg_track_to_plane(M, vTx, eng, 1., 0., 0., 1., 0., ok);
         xx = xx + fmin(1., fmax(-1., (vTx[0] - vT[0])));
-        EvOut r;
-        recon<ARM>((cdptr)a.coef, o, xx, r);
-        delta = r.dpp;
-        dy = r.dydz;
-        dx = r.dxdz;
-        y = r.y;
-        mom = mom_0 * (1.e0 + delta / 100.e0);
-        vel = fabs(mom) / sqrt(mom * mom + mass * mass) * cc;
-        eng = copysign(1., mom) * sqrt(mom * mom + mass * mass);
-        trg_start(vT, xx + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
-        trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, frx, ok);
-        n = n + 1;
+        a.xx[i] = xx;
+        a.delx[i] = delx;
+        a.iter[i] = (uint8_t)n;
+        a.st_ok[i] = ok ? 1 : 0;
+        // append to the next round's list: one atomic per wave (the lanes
+        // active here), each lane at its rank among them
+        const uint64_t act = __ballot(1);
+        const int lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(a.count_out, (uint32_t)__popcll(act));
+        base = __shfl(base, leader);
+        a.list_out[base + rank] = i;
+        return;
     }
     if (delx > .2) ok = false;
-    a.delta[i] = delta;
     a.dy[i] = vT[4] / vT[5];
     a.dx[i] = vT[3] / vT[5];
     a.y[i] = vT[1];
     if (a.ok) a.ok[i] = ok ? 1 : 0;
 }
+
+// Round 0: the first search from z = 100 cm to the beam plane (:768-787).
+__global__ __launch_bounds__(256) void field_to_start(FieldToArgs a)
+{
+    __shared__ double s[2 * kTrgNodes];
+    stage_field_map(s, a.map);
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    const TrgMap M{s, s + kTrgNodes, a.stht, a.ctht};
+    const double delta = a.delta[i], y = a.y[i], dx = a.dx[i], dy = a.dy[i];
+    const double fry = a.fry[i];
+    const double mom = a.mom[i];
+    bool ok = a.ok ? a.ok[i] != 0 : true;
+    double vel, eng;
+    field_energy(mom, a.mass, vel, eng);
+    a.mom0[i] = mom / (1.e0 + delta / 100.e0);
+    double vT[6];
+    trg_start(vT, -fry + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
+    trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, a.frx[i], ok);
+    field_to_iterate(M, a, i, vT, eng, -fry, 1., 0, ok);
+}
+
+// A round's reconstruction, mc_*_recon(delta, dy, dx, y, xx) on the event's
+// focal-plane track (:836-846).
+template <int ARM>
+__global__ __launch_bounds__(256) void field_to_recon(FieldToArgs a)
+{
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= *a.count_in) return;
+    const uint32_t i = a.list_in[j];
+    EvOut o, r;
+    o.x_fp = a.x_fp[i];
+    o.dx_fp = a.dx_fp[i];
+    o.y_fp = a.y_fp[i];
+    o.dy_fp = a.dy_fp[i];
+    recon<ARM>((cdptr)a.coef, o, a.xx[i], r);
+    a.delta[i] = r.dpp;
+    a.dy[i] = r.dydz;
+    a.dx[i] = r.dxdz;
+    a.y[i] = r.y;
+}
+
+// A round's search back to the beam plane from the reconstructed track
+// (:848-865), then the loop test and the next searches.
+__global__ __launch_bounds__(256) void field_to_search(FieldToArgs a)
+{
+    __shared__ double s[2 * kTrgNodes];
+    const uint32_t count = *a.count_in;
+    if (blockIdx.x * 256u >= count) return;  // the whole workgroup is past the list
+    stage_field_map(s, a.map);
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= count) return;
+    const uint32_t i = a.list_in[j];
+    const TrgMap M{s, s + kTrgNodes, a.stht, a.ctht};
+    const double delta = a.delta[i], y = a.y[i], dx = a.dx[i], dy = a.dy[i], xx = a.xx[i];
+    bool ok = a.st_ok[i] != 0;
+    const double mom = a.mom0[i] * (1.e0 + delta / 100.e0);
+    double vel, eng;
+    field_energy(mom, a.mass, vel, eng);
+    double vT[6];
+    trg_start(vT, xx + 100. * dx, y + 100. * dy, 100., dx, dy, vel, false);
+    trg_track_to_plane(M, vT, eng, 1., 0., -a.ctheta, a.stheta, a.frx[i], ok);
+    field_to_iterate(M, a, i, vT, eng, xx, a.delx[i], (int)a.iter[i] + 1, ok);
+}
+
+__global__ void zero_u32(uint32_t* p) { p[0] = 0u; }
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -2971,6 +3043,10 @@ struct simc_field {
     std::mutex mu;             // guards the host-pointer staging
     void* d_stage = nullptr;
     size_t stage_bytes = 0;
+    // track_to_tgt loop state and event lists (grown on demand; calls on one
+    // field must be ordered)
+    void* d_work = nullptr;
+    int64_t work_cap = 0;
 };
 
 namespace {
@@ -3112,6 +3188,36 @@ int simc_track_to_tgt(simc_ctx* ctx, const simc_field* f, int arm, int spect, in
     std::memset(&a, 0, sizeof a);
     if (int rc = field_angles(f, spect, a.stht, a.ctht)) return rc;
     HIPCHK(hipSetDevice(f->device));
+    simc_field* fw = const_cast<simc_field*>(f);
+    // workspace: xx, mom0, delx (doubles), two event lists, two counters, ok, iteration
+    auto rounded = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t nn = (size_t)n;
+    const size_t need = 3 * rounded(8 * nn) + 2 * rounded(4 * nn) + 256 + 2 * rounded(nn);
+    if (fw->work_cap < n) {
+        if (fw->d_work) {
+            HIPCHK(hipDeviceSynchronize());  // earlier calls may still use it
+            HIPCHK(hipFree(fw->d_work));
+            fw->d_work = nullptr;
+            fw->work_cap = 0;
+        }
+        HIPCHK(hipMalloc(&fw->d_work, need));
+        fw->work_cap = n;
+    }
+    char* w = (char*)fw->d_work;
+    size_t at = 0;
+    auto carve = [&](size_t b) {
+        char* p = w + at;
+        at += rounded(b);
+        return p;
+    };
+    a.xx = (double*)carve(8 * nn);
+    a.mom0 = (double*)carve(8 * nn);
+    a.delx = (double*)carve(8 * nn);
+    uint32_t* list[2] = {(uint32_t*)carve(4 * nn), (uint32_t*)carve(4 * nn)};
+    uint32_t* count = (uint32_t*)carve(256);
+    a.st_ok = (uint8_t*)carve(nn);
+    a.iter = (uint8_t*)carve(nn);
+    if (at > need) return fail(SIMC_E_INTERNAL, "simc_track_to_tgt: workspace layout exceeds its buffer");
     a.map = f->d_map;
     a.n = (uint32_t)n;
     a.coef = ctx->d_coef[ai];
@@ -3122,10 +3228,25 @@ int simc_track_to_tgt(simc_ctx* ctx, const simc_field* f, int arm, int spect, in
     a.ok = ok;
     const dim3 G((uint32_t)((n + 255) / 256)), B(256);
     hipStream_t st = (hipStream_t)stream;
-    if (arm == SIMC_ARM_HMS) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_HMS>, G, B, 0, st, a);
-    else if (arm == SIMC_ARM_SOS) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_SOS>, G, B, 0, st, a);
-    else if (is_hrs(arm)) hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_HRSR>, G, B, 0, st, a);
-    else hipLaunchKernelGGL(field_to_kernel<SIMC_ARM_SHMS>, G, B, 0, st, a);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, count);
+    a.list_out = list[0];
+    a.count_out = count;
+    hipLaunchKernelGGL(field_to_start, G, B, 0, st, a);
+    // at most 10 iterations (trg_track.f:800): after round r every event
+    // still listed has made r + 1 of them
+    for (int r = 0; r < 10; ++r) {
+        const int in = r & 1;
+        a.list_in = list[in];
+        a.count_in = count + in;
+        a.list_out = list[in ^ 1];
+        a.count_out = count + (in ^ 1);
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(1), 0, st, a.count_out);
+        if (arm == SIMC_ARM_HMS) hipLaunchKernelGGL(field_to_recon<SIMC_ARM_HMS>, G, B, 0, st, a);
+        else if (arm == SIMC_ARM_SOS) hipLaunchKernelGGL(field_to_recon<SIMC_ARM_SOS>, G, B, 0, st, a);
+        else if (is_hrs(arm)) hipLaunchKernelGGL(field_to_recon<SIMC_ARM_HRSR>, G, B, 0, st, a);
+        else hipLaunchKernelGGL(field_to_recon<SIMC_ARM_SHMS>, G, B, 0, st, a);
+        hipLaunchKernelGGL(field_to_search, G, B, 0, st, a);
+    }
     HIPCHK(hipGetLastError());
     return SIMC_OK;
 }
