@@ -299,6 +299,34 @@ def bench_target(ctx, n=10_000_000, reps=5):
             "data": "synthetic vertices in a 10 cm LH2 cell (can 3), loss and scattering sampled per event"}
 
 
+def bench_arms(ctx, n=10_000_000, reps=5):
+    """The other spectrometers at 1e7 events each (Philox streams, FP64 fit):
+    HMS protons (3 GeV/c), SOS protons (1.5 GeV/c), HRS-R protons (2 GeV/c),
+    HRS-L electrons (3 GeV/c); per-stage times from HIP events."""
+    import torch
+    from simc_gfortran_amd import workloads as W
+    dev = torch.device("cuda:0")
+    res = {}
+    for wl in (W.HMS_C0, W.SOS_1p5, W.HRSR_2, W.HRSL_3):
+        ev = W.generate_device(wl, n, seed=wl.seed, device=dev)
+        out = None
+        st = []
+        for r in range(reps + 1):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            for x in e:
+                x.record()
+            out = ctx.run(wl.arm, ev, p_spec=wl.p_spec, m2=wl.m2, rng="philox", seed=11 + r, fit="fp64", out=out,
+                          stage_events=e)
+            torch.cuda.synchronize()
+            if r:
+                st.append([e[k].elapsed_time(e[k + 1]) for k in range(3)])
+        ms = np.mean(np.array(st), axis=0)
+        res[wl.name] = {"arm": wl.arm, "events": n, "ms": float(ms.sum()), "events_per_s": n / (ms.sum() * 1e-3),
+                        "stage_ms": [float(x) for x in ms], "accepted_fraction": float(out["ok"].float().mean())}
+        del ev, out
+    return res
+
+
 # FP64 operations of one RK4 step of the target-field tracking (trgRK4,
 # trg_track.f:461-498): four field evaluations (rotation 6, radius 4, grid
 # indices and fractions 8, two bilinear interpolations 18, B_r / r and signs 4,
@@ -506,6 +534,7 @@ def main():
         line["adjoint"] = bench_adjoint(ctx)
         line["target_step"] = bench_target(ctx)
         line["target_field"] = bench_field(ctx)
+        line["other_arms"] = bench_arms(ctx)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
