@@ -86,6 +86,8 @@ def build_fortran(verbose: bool = True) -> bool:
                  "-L", PKG, "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "batch_host")])
     cmds.append([FORTRAN, "-O2", os.path.join(tdir, "dropin_events.f90"),
                  "-L", PKG, "-lsimc_dropin", "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "dropin_events")])
+    cmds.append([FORTRAN, "-O2", os.path.join(tdir, "field_events.f90"),
+                 "-L", PKG, "-lsimc_dropin", "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "field_events")])
     for c in cmds:
         if verbose:
             print(" ".join(c))

@@ -161,9 +161,10 @@ int simc_ntuple_write_device(simc_ntuple* nt, const double* const* cols, const u
 {
     if (!nt || !cols) return io_fail(SIMC_E_ARG, "simc_ntuple_write_device: NULL argument");
     if (n < 0 || n > ((int64_t)1 << 31)) return io_fail(SIMC_E_ARG, "simc_ntuple_write_device: n must be in [0, 2^31]");
+    if (written) *written = 0;
+    if (n == 0) return SIMC_OK;
     for (int c = 0; c < nt->ncols; ++c)
         if (!cols[c]) return io_fail(SIMC_E_ARG, "simc_ntuple_write_device: NULL column");
-    if (written) *written = 0;
     hipStream_t st = (hipStream_t)stream;
     const size_t rec_bytes = (size_t)nt->ncols * 16;
     // chunks of at most 1 GiB of records

@@ -3154,9 +3154,10 @@ void simc_field_destroy(simc_field* f)
 int simc_track_from_tgt(const simc_field* f, int spect, int64_t n, double* x, double* y, double* z, double* dx,
                         double* dy, const double* mom, double mass, uint8_t* ok, void* stream)
 {
-    if (!f || !x || !y || !z || !dx || !dy || !mom) return fail(SIMC_E_ARG, "simc_track_from_tgt: NULL argument");
+    if (!f) return fail(SIMC_E_ARG, "simc_track_from_tgt: NULL field");
     if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_from_tgt: n must be in [0, 2^31]");
     if (n == 0) return SIMC_OK;
+    if (!x || !y || !z || !dx || !dy || !mom) return fail(SIMC_E_ARG, "simc_track_from_tgt: NULL argument");
     FieldFromArgs a;
     std::memset(&a, 0, sizeof a);
     if (int rc = field_angles(f, spect, a.stht, a.ctht)) return rc;
@@ -3177,13 +3178,14 @@ int simc_track_to_tgt(simc_ctx* ctx, const simc_field* f, int arm, int spect, in
                       double* dx, double* dy, const double* frx, const double* fry, const double* mom, double mass,
                       double ctheta, double stheta, uint8_t* ok, void* stream)
 {
-    if (!ctx || !f || !x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !y || !dx || !dy || !frx || !fry || !mom)
-        return fail(SIMC_E_ARG, "simc_track_to_tgt: NULL argument");
+    if (!ctx || !f) return fail(SIMC_E_ARG, "simc_track_to_tgt: NULL context or field");
     const int ai = arm_index(arm);
     if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
     if (n < 0 || n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_track_to_tgt: n must be in [0, 2^31]");
     if (n == 0) return SIMC_OK;
+    if (!x_fp || !dx_fp || !y_fp || !dy_fp || !delta || !y || !dx || !dy || !frx || !fry || !mom)
+        return fail(SIMC_E_ARG, "simc_track_to_tgt: NULL argument");
     FieldToArgs a;
     std::memset(&a, 0, sizeof a);
     if (int rc = field_angles(f, spect, a.stht, a.ctht)) return rc;

@@ -157,3 +157,15 @@ def test_field_errors(ctx):
         F.track_from_tgt(ev, dev(np.ones(4)), 1.0, 0)
     assert e.value.code == -6
     F.close()
+
+
+def test_empty_inputs(ctx):
+    from simc_gfortran_amd import api
+    F = api.TargetField(api.FIELD_MAP, 0.0, 0.0)
+    e = {k: torch.zeros(0, dtype=torch.float64, device="cuda") for k in ("x", "y", "z", "dx", "dy")}
+    assert F.track_from_tgt(e, torch.zeros(0, dtype=torch.float64, device="cuda"), 938.0, 1).numel() == 0
+    z = torch.zeros(0, dtype=torch.float64, device="cuda")
+    trk = {k: z for k in ("x_fp", "dx_fp", "y_fp", "dy_fp")}
+    rec = {k: z.clone() for k in ("delta", "y", "dx", "dy")}
+    assert F.track_to_tgt(ctx, 1, trk, rec, z, z, z, 938.0, 1.0, 0.0, 1).numel() == 0
+    F.close()

@@ -76,3 +76,15 @@ def test_stop_report_layout_and_counts(ctx, tmp_path):
     assert lines[0] == " HRSr Trials:          " + "%12d" % r["trials"]
     assert lines[1].split()[-2:] == [str(r["slit_vert"]), str(r["slit_hor"])]
     assert lines[8] == " S1, S2, Cal\t    " + "".join("%12d" % r[k] for k in ("s1", "s2", "cal"))
+
+
+def test_device_ntuple_empty_and_no_selection(ctx, tmp_path):
+    from simc_gfortran_amd import api
+    z = torch.zeros(0, dtype=torch.float64, device="cuda")
+    with api.Ntuple(str(tmp_path / "e.bin"), ["a", "b"]) as nt:
+        assert nt.write_device([z, z]) == 0
+        none = torch.zeros(5, dtype=torch.uint8, device="cuda")
+        c = torch.arange(5, dtype=torch.float64, device="cuda")
+        assert nt.write_device([c, c], none) == 0
+    t, v = api.read_ntuple(str(tmp_path / "e.bin"))
+    assert t == ["a", "b"] and v.shape == (0, 2)

@@ -100,3 +100,51 @@ def test_fortran_per_event_dropin_matches_batched(ctx, tmp_path, wl):
     worst = float([l for l in r.stdout.splitlines() if l.startswith("recon_worst")][0].split()[1])
     # the HRS drop-in restores the fitted y from y_fp + VDC offset (one rounding)
     assert worst == 0.0 if wl.arm not in (3, 4) else worst < 1e-12
+
+
+def test_fortran_field_dropins_match_library(ctx, tmp_path):
+    """trgInit / track_from_tgt / track_to_tgt called per event from Fortran
+    (the drop-ins, /track/ set like mc_hms does) equal the batched device
+    entry points on the same events."""
+    from simc_gfortran_amd import api
+    n = 300
+    wl = W.HMS_C0
+    rng = np.random.default_rng(12)
+    a = np.stack([rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n), rng.uniform(-1.5, 1.5, n),
+                  rng.uniform(-0.05, 0.05, n), rng.uniform(-0.04, 0.04, n), rng.uniform(2500, 3500, n)], axis=1)
+    ev = W.generate(wl, 2000, seed=13)
+    res = ctx.run_host(wl.arm, ev, p_spec=wl.p_spec, m2=wl.m2, fit="fp64", seed=21)
+    sel = np.flatnonzero(res["ok"] == 1)[:n]
+    assert len(sel) == n
+    b = np.stack([res["x_fp"][sel], res["dx_fp"][sel], res["y_fp"][sel], res["dy_fp"][sel], res["dpp"][sel],
+                  res["y"][sel], res["dxdz"][sel], res["dydz"][sel], rng.uniform(-0.2, 0.2, n),
+                  rng.uniform(-0.2, 0.2, n), wl.p_spec * (1 + res["dpp"][sel] / 100)], axis=1)
+    np.concatenate([a.T.ravel(), b.T.ravel()]).astype(np.float64).tofile(tmp_path / "in.bin")
+    th = np.radians(wl.theta_deg)
+    env = dict(os.environ, SIMC_MAP_DIR=MAPS)
+    print("field_events", n, flush=True)
+    r = subprocess.run([os.path.join(BIN, "field_events"), api.FIELD_MAP, "10.0", "10.0", "1", "1", repr(wl.mass),
+                        repr(float(np.cos(th))), repr(float(np.sin(th))), str(n), str(tmp_path / "in.bin"),
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(tmp_path / "out.bin", dtype=np.uint8)
+    oa = raw[:5 * n * 8].view(np.float64).reshape(5, n)
+    ob = raw[5 * n * 8:9 * n * 8].view(np.float64).reshape(4, n)
+    ok = raw[9 * n * 8:].view(np.int32).reshape(2, n)
+    F = api.TargetField(api.FIELD_MAP, 10.0, 10.0)
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()  # noqa: E731
+    d = {k: dev(a[:, i]) for i, k in enumerate(("x", "y", "z", "dx", "dy"))}
+    oka = F.track_from_tgt(d, dev(a[:, 5]), wl.mass, 1)
+    rec = {k: dev(b[:, 4 + i]) for i, k in enumerate(("delta", "y", "dx", "dy"))}
+    trk = {k: dev(b[:, i]) for i, k in enumerate(("x_fp", "dx_fp", "y_fp", "dy_fp"))}
+    okb = F.track_to_tgt(ctx, 1, trk, rec, dev(b[:, 8]), dev(b[:, 9]), dev(b[:, 10]), wl.mass, np.cos(th),
+                         np.sin(th), 1)
+    torch.cuda.synchronize()
+    for i, k in enumerate(("x", "y", "z", "dx", "dy")):
+        np.testing.assert_array_equal(oa[i], d[k].cpu().numpy(), err_msg=k)
+    for i, k in enumerate(("delta", "y", "dx", "dy")):
+        np.testing.assert_array_equal(ob[i], rec[k].cpu().numpy(), err_msg=k)
+    np.testing.assert_array_equal(ok[0], oka.cpu().numpy())
+    np.testing.assert_array_equal(ok[1], okb.cpu().numpy())
+    assert ok[1].mean() > 0.9
+    F.close()
