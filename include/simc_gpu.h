@@ -321,6 +321,57 @@ int simc_target_prep(simc_ctx *ctx, const simc_target *t, const simc_prep_settin
 int simc_target_prep_host(simc_ctx *ctx, const simc_target *t, const simc_prep_settings *s,
                           const simc_vertex_in *in, simc_prep_out *out, int64_t n, const simc_rng *rng);
 
+/* ---- Event generator: H(e,e'p) elastic scattering, radiation off -------
+ *
+ * Per event, SIMC's generate (event.f:126-428) for doing_hyd_elast without
+ * using_rad: beam position (gauss1(3) sigma widths, event.f:157-158), raster
+ * (fr_pattern 1 bedpost, 2 circle, 3 flat square; :165-183), vertex z
+ * (:187), the beam's sampled loss (trip_thru_target narm 1, :194-196), the
+ * Coulomb energy (:197-203), Ein (:204-205), electron yptar, xptar uniform
+ * in the generation limits (:275-277), physics_angles; then complete_ev's
+ * hydrogen-elastic branch (event.f:497-563: E' = Ein Mh / (Mh + Ein (1 -
+ * cos theta_e)), nu, Q2, q, the proton along q, spectrometer_angles,
+ * :1013-1014 jacobian) and complete_main's weight (event.f:1558: jacobian
+ * x sigep, physics_proton.f:1-23).  The scattered particles' losses of
+ * complete_ev (event.f:1031-1040) belong to simc_target_prep (narm 2, 3),
+ * which takes these outputs as its vertex.  Draws come from a stream of their
+ * own: Philox (seed, event0 + i) with the counter word "GNEH"; on a tape,
+ * the reference's order (gauss1 x, gauss1 y, raster 2, z 1, the beam's
+ * losses, Ein 1, yptar 1, xptar 1). */
+typedef struct {
+    double Ebeam, dEbeam;           /* MeV: beam energy, full width of its flat spread   */
+    double xwid, ywid;              /* gen%xwid, gen%ywid: beam sigma, cm                 */
+    double xoffset, yoffset;        /* targ%xoffset, targ%yoffset, cm                     */
+    double fr1, fr2;                /* targ%fr1, targ%fr2: raster half-widths or radii    */
+    double coulomb;                 /* targ%Coulomb_constant, MeV                         */
+    double e_yptar[2], e_xptar[2];  /* gen%e%yptar, gen%e%xptar (min, max), rad           */
+    double e_theta, e_phi, e_P;     /* spec%e: central angle, phi (pi/2 or 3 pi/2), MeV/c */
+    double p_theta, p_phi, p_P;     /* spec%p                                             */
+    double Mh;                      /* hadron mass, MeV (Mp for hydrogen)                 */
+    int32_t fr_pattern;             /* 0 none, 1, 2, 3                                    */
+    int32_t using_eloss, using_coulomb, reserved;
+} simc_hep_settings;
+
+/* Outputs (device SoA, every pointer may be NULL): main%target x, y, z (cm,
+ * z with zoffset), rasterx, rastery; Ein; the beam's eloss (MeV) and teff
+ * (radiation lengths); the electron's and the proton's xptar, yptar, theta,
+ * phi (rad), E and P (MeV), delta (%); Q2 (MeV^2), nu, q; jacobian, sigcc
+ * (microbarn/sr), weight; ok: complete_ev's success; used: uniforms drawn,
+ * -1 when a tape slot ran out (that event's outputs are NaN). */
+typedef struct {
+    double *x, *y, *z, *rasterx, *rastery, *Ein, *eloss, *teff;
+    double *e_xptar, *e_yptar, *e_theta, *e_phi, *e_E, *e_delta;
+    double *p_xptar, *p_yptar, *p_theta, *p_phi, *p_P, *p_E, *p_delta;
+    double *Q2, *nu, *q, *jacobian, *sigcc, *weight;
+    uint8_t *ok;
+    int32_t *used;
+} simc_hep_out;
+
+/* Asynchronous on `stream`; the target description gives the beam's path
+ * (length, angle, zoffset, can, material). */
+int simc_gen_hep(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h, simc_hep_out *out,
+                 int64_t n, const simc_rng *rng, void *stream);
+
 /* ---- Target-field tracking (polarized-target field, trg_track.f) ----
  * simc_field_create = trgInit(map, theta_e, phi_e, theta_p, phi_p)
  * (trg_track.f:243-325): map = NULL or blank for the uniform 5 T test field,

@@ -471,3 +471,153 @@ def ref_ntuple(path, mode, tgt_field, rows):
         r = np.ascontiguousarray(r)
         L.ref_ntuple_event_(r.ctypes.data_as(C.c_void_p))
     L.ntupleclose_(C.c_char_p(name), C.c_size_t(80))
+
+
+# ---- event generator: H(e,e'p) elastic, radiation off -------------------------
+HEP_OUT_FIELDS = ("x", "y", "z", "rasterx", "rastery", "Ein", "eloss", "teff",
+                  "e_xptar", "e_yptar", "e_theta", "e_phi", "e_E", "e_delta",
+                  "p_xptar", "p_yptar", "p_theta", "p_phi", "p_P", "p_E", "p_delta",
+                  "Q2", "nu", "q", "jacobian", "sigcc", "weight")
+
+
+class ORHep(C.Structure):
+    _fields_ = [("Ebeam", C.c_double), ("dEbeam", C.c_double), ("xwid", C.c_double), ("ywid", C.c_double),
+                ("xoffset", C.c_double), ("yoffset", C.c_double), ("fr1", C.c_double), ("fr2", C.c_double),
+                ("coulomb", C.c_double), ("e_yptar", C.c_double * 2), ("e_xptar", C.c_double * 2),
+                ("e_theta", C.c_double), ("e_phi", C.c_double), ("e_P", C.c_double),
+                ("p_theta", C.c_double), ("p_phi", C.c_double), ("p_P", C.c_double), ("Mh", C.c_double),
+                ("fr_pattern", C.c_int), ("using_eloss", C.c_int), ("using_coulomb", C.c_int)]
+
+
+class ORHepOut(C.Structure):
+    _fields_ = [(k, _dp) for k in HEP_OUT_FIELDS] + [("ok", _u8p)]
+
+
+def make_hep(**kw):
+    """Generator settings (the or_hep fields; e_yptar, e_xptar as (min, max))."""
+    h = ORHep()
+    for k, v in kw.items():
+        if k in ("e_yptar", "e_xptar"):
+            getattr(h, k)[0], getattr(h, k)[1] = v
+        elif k in ("fr_pattern", "using_eloss", "using_coulomb"):
+            setattr(h, k, int(v))
+        else:
+            setattr(h, k, float(v))
+    return h
+
+
+def _gen_lib():
+    L = _target_lib()
+    if not hasattr(L, "_gen_ready"):
+        L.or_gen_hep.restype = C.c_int
+        L.or_gen_hep.argtypes = [C.POINTER(ORTarget), C.POINTER(ORHep), C.c_int64, C.POINTER(ORHepOut), C.c_int,
+                                 _dp, C.c_int64, C.c_uint64, C.c_uint64, C.c_int, _i32p]
+        L.or_physics_angles.restype = None
+        L.or_physics_angles.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)] * 2
+        L.or_spectrometer_angles.restype = None
+        L.or_spectrometer_angles.argtypes = [C.c_double] * 2 + [C.POINTER(C.c_double)] * 2 + [C.c_double] * 2
+        L.or_fofa_best_fit.restype = None
+        L.or_fofa_best_fit.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_sigep.restype = C.c_double
+        L.or_sigep.argtypes = [C.c_double] * 4
+        L.or_sigmott.restype = C.c_double
+        L.or_sigmott.argtypes = [C.c_double] * 3
+        L._gen_ready = True
+    return L
+
+
+def gen_hep(target, hep, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0, nthreads=0):
+    """The oracle's H(e,e'p) generator for n events.  Returns (dict of
+    HEP_OUT_FIELDS + 'ok' + 'used', return code)."""
+    out = {k: np.zeros(n) for k in HEP_OUT_FIELDS}
+    out["ok"] = np.zeros(n, np.uint8)
+    used = np.zeros(n, np.int32)
+    O = ORHepOut(*([_p(out[k]) for k in HEP_OUT_FIELDS] + [_p(out["ok"], _u8p)]))
+    if rng == "tape":
+        tape = np.ascontiguousarray(tape, dtype=np.float64)
+        kind, tp = 0, _p(tape)
+    else:
+        kind, tp = 1, None
+    rc = _gen_lib().or_gen_hep(C.byref(target), C.byref(hep), n, C.byref(O), kind, tp, int(stride), int(seed),
+                               int(event0), int(nthreads or os.cpu_count()), _p(used, _i32p))
+    out["used"] = used
+    return out, rc
+
+
+def physics_angles(theta0, phi0, dx, dy):
+    th, ph = C.c_double(), C.c_double()
+    _gen_lib().or_physics_angles(theta0, phi0, dx, dy, C.byref(th), C.byref(ph))
+    return th.value, ph.value
+
+
+def spectrometer_angles(theta0, phi0, theta, phi):
+    dx, dy = C.c_double(), C.c_double()
+    _gen_lib().or_spectrometer_angles(theta0, phi0, C.byref(dx), C.byref(dy), theta, phi)
+    return dx.value, dy.value
+
+
+def fofa_best_fit(qsquar):
+    ge, gm = C.c_double(), C.c_double()
+    _gen_lib().or_fofa_best_fit(qsquar, C.byref(ge), C.byref(gm))
+    return ge.value, gm.value
+
+
+def sigep(Q2, E, theta, Ein):
+    return _gen_lib().or_sigep(Q2, E, theta, Ein)
+
+
+def _ref_gen_lib():
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    if not hasattr(L, "_gen_ready"):
+        _d = C.POINTER(C.c_double)
+        L.physics_angles_.restype = None
+        L.physics_angles_.argtypes = [_d] * 6
+        L.spectrometer_angles_.restype = None
+        L.spectrometer_angles_.argtypes = [_d] * 6
+        L.ref_sigep_.restype = C.c_double
+        L.ref_sigep_.argtypes = [_d] * 4
+        L.ref_fofa_.restype = None
+        L.ref_fofa_.argtypes = [_d] * 3
+        L._gen_ready = True
+    return L
+
+
+def ref_physics_angles(theta0, phi0, dx, dy):
+    """The reference's physics_angles (event.f) from oracle/_ref."""
+    a = [C.c_double(v) for v in (theta0, phi0, dx, dy, 0.0, 0.0)]
+    _ref_gen_lib().physics_angles_(*[C.byref(v) for v in a])
+    return a[4].value, a[5].value
+
+
+def ref_spectrometer_angles(theta0, phi0, theta, phi):
+    """The reference's spectrometer_angles (event.f) from oracle/_ref."""
+    a = [C.c_double(v) for v in (theta0, phi0, 0.0, 0.0, theta, phi)]
+    _ref_gen_lib().spectrometer_angles_(*[C.byref(v) for v in a])
+    return a[2].value, a[3].value
+
+
+def ref_sigep(Q2, E, theta, Ein):
+    """The reference's sigep (physics_proton.f) from oracle/_ref."""
+    a = [C.c_double(v) for v in (Q2, E, theta, Ein)]
+    return _ref_gen_lib().ref_sigep_(*[C.byref(v) for v in a])
+
+
+def ref_fofa(qsquar):
+    a = [C.c_double(v) for v in (qsquar, 0.0, 0.0)]
+    _ref_gen_lib().ref_fofa_(*[C.byref(v) for v in a])
+    return a[1].value, a[2].value
+
+
+def gauss1_tape(tape, nsigmax):
+    """gauss1 (gauss1.f) on the uniforms of `tape`: (value, uniforms used)."""
+    L = lib()
+    if not hasattr(L, "_g1_ready"):
+        L.or_gauss1_tape.restype = C.c_double
+        L.or_gauss1_tape.argtypes = [_dp, C.c_int64, C.c_double, _i64p]
+        L._g1_ready = True
+    t = np.ascontiguousarray(tape, dtype=np.float64)
+    u = np.zeros(1, np.int64)
+    v = L.or_gauss1_tape(_p(t), len(t), float(nsigmax), _p(u, _i64p))
+    return v, int(u[0])

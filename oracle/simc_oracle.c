@@ -3261,3 +3261,183 @@ void or_track_to_tgt_run(const or_field *F, const or_maps *M, int arm, int64_t n
         ok[i] = (uint8_t)o;
     }
 }
+
+/* ========================================================================= */
+/* Event generator: H(e,e'p) elastic, radiation off                          */
+/* ========================================================================= */
+
+/* constants.inc:17-44 */
+#define OR_C_ME 0.51099906
+#define OR_C_MP 938.27231
+#define OR_C_HBARC 197.327053
+#define OR_C_PI 3.141592653589793
+
+/* physics_angles, event.f:1572-1616: spectrometer slopes (dx = xptar,
+ * dy = yptar) about the central angles to the physics theta and phi. */
+void or_physics_angles(double theta0, double phi0, double dx, double dy, double *theta,
+                       double *phi)
+{
+    const double costh = cos(theta0), sinth = sin(theta0), sinph = sin(phi0);
+    const double r = sqrt(1. + dx * dx + dy * dy);
+    *theta = acos((costh - dy * sinth * sinph) / r);
+    if (dx != 0.0) {
+        double p = atan((dy * costh + sinth * sinph) / dx);
+        if (p <= 0) p = p + OR_C_PI;
+        if (sinph < 0.) p = p + OR_C_PI;
+        *phi = p;
+    } else {
+        *phi = phi0;
+    }
+}
+
+/* spectrometer_angles, event.f:1618-1647: the inverse of physics_angles */
+void or_spectrometer_angles(double theta0, double phi0, double *dx, double *dy, double theta,
+                            double phi)
+{
+    const double x = sin(theta) * cos(phi), y = sin(theta) * sin(phi), z = cos(theta);
+    const double x0 = sin(theta0) * cos(phi0), y0 = sin(theta0) * sin(phi0), z0 = cos(theta0);
+    const double cos_dtheta = x * x0 + y * y0 + z * z0;
+    *dx = x / cos_dtheta;
+    double d = sqrt(1 / (cos_dtheta * cos_dtheta) - 1. - *dx * *dx);
+    const double y_event = y / cos_dtheta;
+    if (y_event < y0) d = -d;
+    *dy = d;
+}
+
+/* fofa_best_fit, physics_proton.f:137-171 (Bosted's fit, Eqs. 4 and 5).
+ * The reference's compiler evaluates x**2. and x**3. as products (x*x*x)
+ * and x**4., x**5. as pow calls; the same here. */
+void or_fofa_best_fit(double qsquar, double *GE, double *GM)
+{
+    const double mu_p = 2.793;
+    const double Q2 = -qsquar * (OR_C_HBARC * OR_C_HBARC) * 1.e-6;
+    const double Q = sqrt(fmax(Q2, 0.e0));
+    const double Q3 = Q * Q * Q, Q4 = pow(Q, 4.), Q5 = pow(Q, 5.);
+    double denom = 1. + 0.62 * Q + 0.68 * Q2 + 2.8 * Q3 + 0.83 * Q4;
+    *GE = 1. / denom;
+    denom = 1. + 0.35 * Q + 2.44 * Q2 + 0.5 * Q3 + 1.04 * Q4 + 0.34 * Q5;
+    *GM = mu_p / denom;
+}
+
+/* sigMott, physics_proton.f:176-190: microbarn/sr */
+double or_sigmott(double e0, double theta, double Q2)
+{
+    const double alpha = 1. / 137.0359895;
+    const double t = 2. * alpha * OR_C_HBARC * e0 * cos(theta / 2.) / Q2;
+    const double sig = t * t;
+    return sig * 1.e4;
+}
+
+/* sigep, physics_proton.f:1-23 */
+double or_sigep(double Q2, double E, double theta, double Ein)
+{
+    const double Mp2 = OR_C_MP * OR_C_MP;
+    double GE, GM;
+    or_fofa_best_fit(-Q2 / (OR_C_HBARC * OR_C_HBARC), &GE, &GM);
+    const double qmu4mp = Q2 / 4. / Mp2;
+    const double W1p = GM * GM * qmu4mp;
+    const double W2p = (GE * GE + GM * GM * qmu4mp) / (1.0 + qmu4mp);
+    const double t = tan(theta / 2.);
+    const double Wp = W2p + 2. * W1p * (t * t);
+    return or_sigmott(E, theta, Q2) * E / Ein * Wp;
+}
+
+/* One event of generate (event.f:157-212, :275-277, :322-325, :381-404) with
+ * complete_ev's hydrogen-elastic branch (:497-563, :1013-1014) and
+ * complete_main's weight (:1397-1398, :1450-1452, :1558). */
+static void gen_hep_event(or_rng *g, const or_target *T, const or_hep *H, int64_t i,
+                          or_hep_out *o)
+{
+    const double nsig_max = 3.0;
+    double x = or_gauss1(g, nsig_max) * H->xwid + H->xoffset;
+    double y = or_gauss1(g, nsig_max) * H->ywid + H->yoffset;
+    double t3, t4, t5 = 0.0, t6 = 0.0;
+    if (H->fr_pattern == 1) {
+        t3 = or_grnd(g) * OR_C_PI;
+        t4 = or_grnd(g) * OR_C_PI;
+        t5 = cos(t3) * H->fr1;
+        t6 = cos(t4) * H->fr2;
+    } else if (H->fr_pattern == 2) {
+        t3 = or_grnd(g) * 2. * OR_C_PI;
+        t4 = sqrt(or_grnd(g)) * (H->fr2 - H->fr1) + H->fr1;
+        t5 = cos(t3) * t4;
+        t6 = sin(t3) * t4;
+    } else if (H->fr_pattern == 3) {
+        t3 = 2. * or_grnd(g) - 1.0;
+        t4 = 2. * or_grnd(g) - 1.0;
+        t5 = H->fr1 * t3;
+        t6 = H->fr2 * t4;
+    }
+    x = x + t5;
+    y = y + t6;
+    const double z = (0.5 - or_grnd(g)) * T->length + T->zoffset;
+    double eloss, teff;
+    or_trip_thru_target(g, T, 1, 0, z - T->zoffset, H->Ebeam, 0.0, OR_C_ME, 1, &eloss, &teff);
+    if (!H->using_eloss) eloss = 0.0;
+    const double coul = H->using_coulomb ? H->coulomb : 0.0;
+    const double Ein = H->Ebeam + (or_grnd(g) - 0.5) * H->dEbeam + coul - eloss;
+    const double e_yptar = H->e_yptar[0] + or_grnd(g) * (H->e_yptar[1] - H->e_yptar[0]);
+    const double e_xptar = H->e_xptar[0] + or_grnd(g) * (H->e_xptar[1] - H->e_xptar[0]);
+    double e_theta, e_phi;
+    or_physics_angles(H->e_theta, H->e_phi, e_xptar, e_yptar, &e_theta, &e_phi);
+    /* complete_ev */
+    const double uex = sin(e_theta) * cos(e_phi), uey = sin(e_theta) * sin(e_phi);
+    const double uez = cos(e_theta);
+    const double E = Ein * H->Mh / (H->Mh + Ein * (1. - uez));
+    const int ok = !(E > Ein);
+    const double e_delta = (E - H->e_P) * 100. / H->e_P;
+    const double nu = Ein - E;
+    const double Q2 = 2 * Ein * E * (1. - uez);
+    const double q = sqrt(Q2 + nu * nu);
+    const double uqx = -(E * uex / q), uqy = -(E * uey / q), uqz = (Ein - E * uez) / q;
+    const double p_theta = acos(uqz);
+    double p_phi = atan2(uqy, uqx);
+    if (p_phi < 0.) p_phi = p_phi + 2. * OR_C_PI;
+    double p_xptar, p_yptar;
+    or_spectrometer_angles(H->p_theta, H->p_phi, &p_xptar, &p_yptar, p_theta, p_phi);
+    const double p_E = sqrt(q * q + H->Mh * H->Mh);
+    const double p_delta = (q - H->p_P) * 100. / H->p_P;
+    const double r = sqrt(1. + e_yptar * e_yptar + e_xptar * e_xptar);
+    const double jacobian = 1.0 / (r * r * r);
+    const double sigcc = or_sigep(Q2, E, e_theta, Ein);
+    const double weight = jacobian * sigcc;
+#define OR_PUT(f, v) if (o->f) o->f[i] = (v)
+    OR_PUT(x, x); OR_PUT(y, y); OR_PUT(z, z); OR_PUT(rasterx, t5); OR_PUT(rastery, t6);
+    OR_PUT(Ein, Ein); OR_PUT(eloss, eloss); OR_PUT(teff, teff);
+    OR_PUT(e_xptar, e_xptar); OR_PUT(e_yptar, e_yptar); OR_PUT(e_theta, e_theta);
+    OR_PUT(e_phi, e_phi); OR_PUT(e_E, E); OR_PUT(e_delta, e_delta);
+    OR_PUT(p_xptar, p_xptar); OR_PUT(p_yptar, p_yptar); OR_PUT(p_theta, p_theta);
+    OR_PUT(p_phi, p_phi); OR_PUT(p_P, q); OR_PUT(p_E, p_E); OR_PUT(p_delta, p_delta);
+    OR_PUT(Q2, Q2); OR_PUT(nu, nu); OR_PUT(q, q); OR_PUT(jacobian, jacobian);
+    OR_PUT(sigcc, sigcc); OR_PUT(weight, weight); OR_PUT(ok, (uint8_t)ok);
+#undef OR_PUT
+}
+
+int or_gen_hep(const or_target *T, const or_hep *H, int64_t n, or_hep_out *out, int rng_kind,
+               const double *tape, int64_t stride, uint64_t seed, uint64_t event0, int nthreads,
+               int32_t *used)
+{
+    if (nthreads <= 0) nthreads = 1;
+    int64_t overflow = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : overflow)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        or_rng g;
+        memset(&g, 0, sizeof g);
+        g.kind = rng_kind;
+        if (rng_kind == 0) {
+            g.tape = tape + i * stride;
+            g.stride = stride;
+        } else {
+            g.key[0] = (uint32_t)seed;
+            g.key[1] = (uint32_t)(seed >> 32);
+            g.event = event0 + (uint64_t)i;
+            g.tag = OR_GEN_TAG;
+        }
+        gen_hep_event(&g, T, H, i, out);
+        if (g.overflow) overflow++;
+        if (used) used[i] = (int32_t)g.used;
+    }
+    return overflow ? -2 : 0;
+}

@@ -334,6 +334,45 @@ void or_track_to_tgt(const or_field *F, const or_maps *M, int arm, const double 
                      double *delta, double *y, double *dx, double *dy, double frx, double fry,
                      double mom, double mass, double ctheta, double stheta, int spect, int *ok);
 
+/* ---- event generator: H(e,e'p) elastic scattering, radiation off ----
+ * generate (event.f:126-428) for doing_hyd_elast without using_rad,
+ * complete_ev's hydrogen-elastic branch (event.f:432-1052) up to the
+ * scattered particles' target losses (the target step does those), and
+ * complete_main's weight (event.f:1393-1560) with sigep. */
+typedef struct {
+    double Ebeam, dEbeam;            /* MeV */
+    double xwid, ywid;               /* gen%xwid, gen%ywid: beam sigma, cm */
+    double xoffset, yoffset;         /* targ%xoffset, targ%yoffset, cm */
+    double fr1, fr2;                 /* raster half-widths or radii, cm */
+    double coulomb;                  /* targ%Coulomb_constant, MeV */
+    double e_yptar[2], e_xptar[2];   /* gen%e%yptar, gen%e%xptar (min, max), rad */
+    double e_theta, e_phi, e_P;      /* spec%e: central angles (rad), momentum (MeV/c) */
+    double p_theta, p_phi, p_P;      /* spec%p */
+    double Mh;                       /* hadron mass, MeV (Mp) */
+    int fr_pattern, using_eloss, using_coulomb;
+} or_hep;
+typedef struct {
+    double *x, *y, *z, *rasterx, *rastery, *Ein, *eloss, *teff;
+    double *e_xptar, *e_yptar, *e_theta, *e_phi, *e_E, *e_delta;
+    double *p_xptar, *p_yptar, *p_theta, *p_phi, *p_P, *p_E, *p_delta;
+    double *Q2, *nu, *q, *jacobian, *sigcc, *weight;
+    uint8_t *ok;
+} or_hep_out;
+#define OR_GEN_TAG 0x48454E47u   /* philox stream of the generator, "GNEH" */
+void or_physics_angles(double theta0, double phi0, double dx, double dy, double *theta,
+                       double *phi);
+void or_spectrometer_angles(double theta0, double phi0, double *dx, double *dy, double theta,
+                            double phi);
+void or_fofa_best_fit(double qsquar, double *GE, double *GM);
+double or_sigmott(double e0, double theta, double Q2);
+/* sigep of a vertex with these Q2, e%E, e%theta, Ein */
+double or_sigep(double Q2, double E, double theta, double Ein);
+/* rng_kind 0 tape, 1 philox (key seed, event event0 + i, tag OR_GEN_TAG).
+ * Returns 0, -2 on tape overflow. */
+int or_gen_hep(const or_target *T, const or_hep *H, int64_t n, or_hep_out *out, int rng_kind,
+               const double *tape, int64_t stride, uint64_t seed, uint64_t event0, int nthreads,
+               int32_t *used);
+
 #ifdef __cplusplus
 }
 #endif

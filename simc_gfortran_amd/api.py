@@ -66,7 +66,7 @@ EXPORTS = [
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
-    "simc_target_prep", "simc_target_prep_host",
+    "simc_target_prep", "simc_target_prep_host", "simc_gen_hep",
     "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
     "simc_track_from_tgt_host", "simc_track_to_tgt_host",
     "simc_ntuple_open", "simc_ntuple_write", "simc_ntuple_write_device", "simc_ntuple_close", "simc_stop_report",
@@ -134,6 +134,41 @@ class PrepOut(C.Structure):
     _fields_ = [(n, _vp) for n in PREP_OUT_FIELDS]
 
 
+class HepSettings(C.Structure):
+    """simc_hep_settings: H(e,e'p) elastic generation (event.f generate,
+    complete_ev, complete_main for doing_hyd_elast, radiation off)."""
+    _fields_ = [("Ebeam", C.c_double), ("dEbeam", C.c_double), ("xwid", C.c_double), ("ywid", C.c_double),
+                ("xoffset", C.c_double), ("yoffset", C.c_double), ("fr1", C.c_double), ("fr2", C.c_double),
+                ("coulomb", C.c_double), ("e_yptar", C.c_double * 2), ("e_xptar", C.c_double * 2),
+                ("e_theta", C.c_double), ("e_phi", C.c_double), ("e_P", C.c_double),
+                ("p_theta", C.c_double), ("p_phi", C.c_double), ("p_P", C.c_double), ("Mh", C.c_double),
+                ("fr_pattern", C.c_int32), ("using_eloss", C.c_int32), ("using_coulomb", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+def hep_settings(**kw):
+    """HepSettings from keywords (e_yptar, e_xptar as (min, max))."""
+    h = HepSettings()
+    for k, v in kw.items():
+        if k in ("e_yptar", "e_xptar"):
+            getattr(h, k)[0], getattr(h, k)[1] = v
+        elif k in ("fr_pattern", "using_eloss", "using_coulomb"):
+            setattr(h, k, int(v))
+        else:
+            setattr(h, k, float(v))
+    return h
+
+
+HEP_OUT_FIELDS = ("x", "y", "z", "rasterx", "rastery", "Ein", "eloss", "teff",
+                  "e_xptar", "e_yptar", "e_theta", "e_phi", "e_E", "e_delta",
+                  "p_xptar", "p_yptar", "p_theta", "p_phi", "p_P", "p_E", "p_delta",
+                  "Q2", "nu", "q", "jacobian", "sigcc", "weight")
+
+
+class HepOut(C.Structure):
+    _fields_ = [(n, _vp) for n in HEP_OUT_FIELDS] + [("ok", _vp), ("used", _vp)]
+
+
 class SimcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"simc error {code}: {msg}")
@@ -172,6 +207,8 @@ def library(path: str = LIB_PATH):
     L.simc_adjoint_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn), _vp, C.c_int64,
                                    C.POINTER(Rng), _vp, C.POINTER(C.c_double), _vp, _vp, _vp]
     L.simc_recon_coefficients.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.simc_gen_hep.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(HepOut), C.c_int64,
+                               C.POINTER(Rng), _vp]
     L.simc_target_prep.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
                                    C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng), _vp]
     L.simc_target_prep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
@@ -395,6 +432,37 @@ class Spectrometers:
         st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(library().simc_target_prep(self._h, C.byref(target), C.byref(prep), C.byref(V), C.byref(O), n,
                                           C.byref(R), C.c_void_p(st)))
+        return out
+
+    # -- event generator -----------------------------------------------------------------
+    def gen_hep(self, target, hep, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0, out=None,
+                device=None, stream=None):
+        """H(e,e'p) elastic events (simc_gen_hep): n vertices with their
+        kinematics and weight.  target: Target; hep: HepSettings.  Returns a
+        dict of float64 device tensors (HEP_OUT_FIELDS), ok (uint8) and used
+        (int32), or fills `out` (absent keys are not written)."""
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if out is None:
+            out = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in HEP_OUT_FIELDS}
+            out["ok"] = torch.empty(n, dtype=torch.uint8, device=dev)
+            out["used"] = torch.empty(n, dtype=torch.int32, device=dev)
+        for k in HEP_OUT_FIELDS:
+            if out.get(k) is not None:
+                _dev_f64(out[k], n, k)
+        for k, dt in (("ok", torch.uint8), ("used", torch.int32)):
+            t = out.get(k)
+            if t is not None and (not t.is_cuda or t.dtype != dt or t.numel() != n or not t.is_contiguous()):
+                raise ValueError(f"{k}: expected {n} contiguous {dt} elements on a HIP device")
+        O = HepOut(*[_ptr(out.get(k)) for k in HEP_OUT_FIELDS + ("ok", "used")])
+        if rng == "tape":
+            if tape is None or stride <= 0:
+                raise ValueError("tape rng needs tape and stride")
+            R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _check(library().simc_gen_hep(self._h, C.byref(target), C.byref(hep), C.byref(O), int(n), C.byref(R),
+                                      C.c_void_p(st)))
         return out
 
     def target_prep_host(self, target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=None, stride=0):

@@ -24,6 +24,7 @@
 #include "simc_ops.h"
 #include "simc_target.h"
 #include "simc_field.h"
+#include "simc_gen.h"
 #include "cosy_gen.dev.inc"
 
 using namespace simcdev;
@@ -1247,6 +1248,44 @@ __global__ __launch_bounds__(256) void target_prep_kernel(PrepArgs a)
     if (a.teff) a.teff[i] = teff;
     if (a.d1) a.d1[i] = d1;
     if (a.d2) a.d2[i] = d2;
+    if (a.used) a.used[i] = g.overflow ? -1 : g.used();
+}
+
+// ---------------------------------------------------------------------------
+// Event generator, H(e,e'p) elastic (simc_gen.h): one event per lane; every
+// output column may be absent.
+// ---------------------------------------------------------------------------
+constexpr int kHepCols = 27;
+
+struct GenArgs {
+    HepDev H;
+    PrepDev P;
+    uint64_t seed, event0;
+    uint32_t n, stride_bytes;
+    const double* tape;
+    double* col[kHepCols];
+    uint8_t* ok;
+    int32_t* used;
+};
+
+template <class R>
+__global__ __launch_bounds__(256) void gen_hep_kernel(GenArgs a)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    R g;
+    g.init(a.seed, a.event0 + i, a.tape, i * a.stride_bytes, a.stride_bytes);
+    HepEvent v;
+    gen_hep(g, a.H, a.P, v);
+    const double c[kHepCols] = {v.x, v.y, v.z, v.rasterx, v.rastery, v.Ein, v.eloss, v.teff,
+                                v.e_xptar, v.e_yptar, v.e_theta, v.e_phi, v.e_E, v.e_delta,
+                                v.p_xptar, v.p_yptar, v.p_theta, v.p_phi, v.p_P, v.p_E, v.p_delta,
+                                v.Q2, v.nu, v.q, v.jacobian, v.sigcc, v.weight};
+    const double nan = __builtin_nan("");
+#pragma unroll
+    for (int k = 0; k < kHepCols; ++k)
+        if (a.col[k]) a.col[k][i] = g.overflow ? nan : c[k];
+    if (a.ok) a.ok[i] = (v.ok && !g.overflow) ? 1 : 0;
     if (a.used) a.used[i] = g.overflow ? -1 : g.used();
 }
 
@@ -3029,6 +3068,90 @@ int simc_target_prep_host(simc_ctx* ctx, const simc_target* t, const simc_prep_s
     if (out->used) std::memcpy(out->used, used.data(), 4 * (size_t)n);
     for (int64_t i = 0; i < n; ++i)
         if (used[(size_t)i] < 0) return fail(SIMC_E_TAPE, "event " + std::to_string(i) + " ran out of its tape slot");
+    return SIMC_OK;
+}
+
+int simc_gen_hep(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h, simc_hep_out* out, int64_t n,
+                 const simc_rng* rng, void* stream)
+{
+    if (!ctx || !t || !h || !out || !rng) return fail(SIMC_E_ARG, "simc_gen_hep: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE)
+        return fail(SIMC_E_ARG, "unknown rng kind");
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0 || rng->stride > (1 << 24)))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a stride in [1, 2^24]");
+    if (h->fr_pattern < 0 || h->fr_pattern > 3)
+        return fail(SIMC_E_ARG, "fr_pattern must be 0 (none), 1 (bedpost), 2 (circle) or 3 (flat square)");
+    if (!(h->Mh > 0.0) || !(h->e_P > 0.0) || !(h->p_P > 0.0))
+        return fail(SIMC_E_ARG, "Mh and the central momenta must be positive");
+    GenArgs a;
+    std::memset(&a, 0, sizeof a);
+    // the beam's trip_thru_target (narm 1, energy Ebeam, electron mass, event.f:194-196)
+    simc_prep_settings s;
+    std::memset(&s, 0, sizeof s);
+    s.narm = SIMC_TARGET_BEAM;
+    s.spectrometer = 1;
+    s.using_eloss = h->using_eloss;
+    s.mass = kMe;
+    s.beam_energy = h->Ebeam;
+    static const double z_present = 0.0;  // make_prep checks that a z column exists
+    simc_vertex_in vin;
+    std::memset(&vin, 0, sizeof vin);
+    vin.z = &z_present;
+    if (int rc = make_prep(t, &s, &vin, a.P)) return rc;
+    HepDev& H = a.H;
+    H.Ebeam = h->Ebeam;
+    H.dEbeam = h->dEbeam;
+    H.xwid = h->xwid;
+    H.ywid = h->ywid;
+    H.xoffset = h->xoffset;
+    H.yoffset = h->yoffset;
+    H.fr1 = h->fr1;
+    H.fr2 = h->fr2;
+    H.coulomb = h->using_coulomb ? h->coulomb : 0.0;
+    H.e_yp0 = h->e_yptar[0];
+    H.e_ypw = h->e_yptar[1] - h->e_yptar[0];
+    H.e_xp0 = h->e_xptar[0];
+    H.e_xpw = h->e_xptar[1] - h->e_xptar[0];
+    H.e_cos = std::cos(h->e_theta);
+    H.e_sin = std::sin(h->e_theta);
+    H.e_sinph = std::sin(h->e_phi);
+    H.e_phi = h->e_phi;
+    H.e_P = h->e_P;
+    H.p_x0 = std::sin(h->p_theta) * std::cos(h->p_phi);
+    H.p_y0 = std::sin(h->p_theta) * std::sin(h->p_phi);
+    H.p_z0 = std::cos(h->p_theta);
+    H.p_P = h->p_P;
+    H.Mh = h->Mh;
+    H.Mh2 = h->Mh * h->Mh;
+    H.fr_pattern = h->fr_pattern;
+    H.using_eloss = h->using_eloss != 0;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t stride_bytes = rng->kind == SIMC_RNG_TAPE ? (uint64_t)rng->stride * 8u : 0u;
+    int64_t per = (int64_t)1 << 30;
+    if (stride_bytes) per = std::min<int64_t>(per, (int64_t)(0xffffffffull / stride_bytes));
+    a.seed = rng->seed;
+    a.stride_bytes = (uint32_t)stride_bytes;
+    double* const cols[kHepCols] = {out->x, out->y, out->z, out->rasterx, out->rastery, out->Ein, out->eloss,
+                                    out->teff, out->e_xptar, out->e_yptar, out->e_theta, out->e_phi, out->e_E,
+                                    out->e_delta, out->p_xptar, out->p_yptar, out->p_theta, out->p_phi,
+                                    out->p_P, out->p_E, out->p_delta, out->Q2, out->nu, out->q,
+                                    out->jacobian, out->sigcc, out->weight};
+    for (int64_t at = 0; at < n; at += per) {
+        const int64_t m = std::min(per, n - at);
+        a.n = (uint32_t)m;
+        a.event0 = rng->event0 + (uint64_t)at;
+        a.tape = rng->kind == SIMC_RNG_TAPE ? rng->tape + at * rng->stride : nullptr;
+        for (int k = 0; k < kHepCols; ++k) a.col[k] = cols[k] ? cols[k] + at : nullptr;
+        a.ok = out->ok ? out->ok + at : nullptr;
+        a.used = out->used ? out->used + at : nullptr;
+        const dim3 G((unsigned)((m + 255) / 256)), B(256);
+        if (rng->kind == SIMC_RNG_TAPE) hipLaunchKernelGGL((gen_hep_kernel<TapeStream>), G, B, 0, st, a);
+        else hipLaunchKernelGGL((gen_hep_kernel<PhiloxStreamT<kGenTag>>), G, B, 0, st, a);
+        HIPCHK(hipGetLastError());
+    }
     return SIMC_OK;
 }
 

@@ -81,7 +81,38 @@ module simc_gpu
     type(c_ptr) :: dang1 = c_null_ptr, dang2 = c_null_ptr, used = c_null_ptr
   end type
 
+  ! event generator (simc_gen_hep): H(e,e'p) elastic, radiation off
+  type, bind(C) :: simc_hep_settings
+    real(c_double)     :: Ebeam = 0, dEbeam = 0, xwid = 0, ywid = 0, xoffset = 0, yoffset = 0
+    real(c_double)     :: fr1 = 0, fr2 = 0, coulomb = 0
+    real(c_double)     :: e_yptar(2) = 0, e_xptar(2) = 0
+    real(c_double)     :: e_theta = 0, e_phi = 0, e_P = 0, p_theta = 0, p_phi = 0, p_P = 0
+    real(c_double)     :: Mh = 938.27231d0
+    integer(c_int32_t) :: fr_pattern = 0, using_eloss = 1, using_coulomb = 0, reserved = 0
+  end type
+
+  type, bind(C) :: simc_hep_out
+    type(c_ptr) :: x = c_null_ptr, y = c_null_ptr, z = c_null_ptr, rasterx = c_null_ptr, rastery = c_null_ptr
+    type(c_ptr) :: Ein = c_null_ptr, eloss = c_null_ptr, teff = c_null_ptr
+    type(c_ptr) :: e_xptar = c_null_ptr, e_yptar = c_null_ptr, e_theta = c_null_ptr, e_phi = c_null_ptr
+    type(c_ptr) :: e_E = c_null_ptr, e_delta = c_null_ptr
+    type(c_ptr) :: p_xptar = c_null_ptr, p_yptar = c_null_ptr, p_theta = c_null_ptr, p_phi = c_null_ptr
+    type(c_ptr) :: p_P = c_null_ptr, p_E = c_null_ptr, p_delta = c_null_ptr
+    type(c_ptr) :: Q2 = c_null_ptr, nu = c_null_ptr, q = c_null_ptr, jacobian = c_null_ptr
+    type(c_ptr) :: sigcc = c_null_ptr, weight = c_null_ptr, ok = c_null_ptr, used = c_null_ptr
+  end type
+
   interface
+    integer(c_int) function simc_gen_hep(ctx, t, h, hout, n, rng, stream) bind(C, name="simc_gen_hep")
+      import :: c_int, c_int64_t, c_ptr, simc_target, simc_hep_settings, simc_hep_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_target), intent(in) :: t
+      type(simc_hep_settings), intent(in) :: h
+      type(simc_hep_out), intent(in) :: hout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
     integer(c_int) function simc_target_prep(ctx, t, s, vin, pout, n, rng, stream) bind(C, name="simc_target_prep")
       import :: c_int, c_int64_t, c_ptr, simc_target, simc_prep_settings, simc_vertex_in, simc_prep_out, simc_rng
       type(c_ptr), value :: ctx, stream

@@ -14,6 +14,9 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         vertices on per-event tapes
     python tests/golden/make_golden.py field    # target-field tracking (field/*.npz):
         the reference's trgInit, track_from_tgt and track_to_tgt (trg_track.f)
+    python tests/golden/make_golden.py gen      # generator pieces (gen/*.npz):
+        the reference's physics_angles, spectrometer_angles (event.f), sigep and
+        fofa_best_fit (physics_proton.f) over sampled arguments
 """
 import hashlib
 import os
@@ -187,8 +190,48 @@ def main_field():
         print(name, "ok", int(r["ok"].sum()), "of", m)
 
 
+NG = 4000
+
+
+def main_gen():
+    out_dir = os.path.join(HERE, "gen")
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(1200)
+    # central angles of an arm on either side of the beam (phi0 pi/2 or 3 pi/2),
+    # slopes inside and beyond the spectrometers' acceptances, and dx = 0
+    th0 = rng.uniform(0.05, 1.4, NG)
+    ph0 = np.where(rng.random(NG) < 0.5, np.pi / 2, 1.5 * np.pi)
+    dx = rng.uniform(-0.15, 0.15, NG)
+    dx[::97] = 0.0
+    dy = rng.uniform(-0.1, 0.1, NG)
+    th = np.zeros(NG)
+    ph = np.zeros(NG)
+    sdx = np.zeros(NG)
+    sdy = np.zeros(NG)
+    for i in range(NG):
+        th[i], ph[i] = O.ref_physics_angles(th0[i], ph0[i], dx[i], dy[i])
+        sdx[i], sdy[i] = O.ref_spectrometer_angles(th0[i], ph0[i], th[i], ph[i])
+    np.savez_compressed(os.path.join(out_dir, "angles.npz"), theta0=th0, phi0=ph0, dx=dx, dy=dy, theta=th,
+                        phi=ph, back_dx=sdx, back_dy=sdy)
+    # elastic kinematics over beam energies 0.5-11 GeV and electron angles 3-80 deg
+    ein = rng.uniform(500.0, 11000.0, NG)
+    the = rng.uniform(0.05, 1.4, NG)
+    mp = 938.27231
+    e = ein * mp / (mp + ein * (1 - np.cos(the)))
+    q2 = 2 * ein * e * (1 - np.cos(the))
+    sig = np.array([O.ref_sigep(q2[i], e[i], the[i], ein[i]) for i in range(NG)])
+    qsq = -rng.uniform(0.0, 25.0, NG) * 1e6 / 197.327053 ** 2
+    qsq[::101] = 0.0
+    ff = np.array([O.ref_fofa(v) for v in qsq])
+    np.savez_compressed(os.path.join(out_dir, "sigep.npz"), Q2=q2, E=e, theta=the, Ein=ein, sigep=sig, qsquar=qsq,
+                        GE=ff[:, 0], GM=ff[:, 1])
+    print("gen goldens", NG, "angles,", NG, "sigep")
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["target"]:
+    if sys.argv[1:] == ["gen"]:
+        main_gen()
+    elif sys.argv[1:] == ["target"]:
         main_target()
     elif sys.argv[1:] == ["field"]:
         main_field()

@@ -1,0 +1,143 @@
+"""H(e,e'p) elastic generator on the GPU (simc_gen_hep) against the oracle.
+
+The device evaluates the oracle's operation order; its sin, cos, acos, atan,
+atan2, tan, pow and log may differ from the host's libm in the last bit, so
+coordinates, energies and weights agree within 1e-12 relative (floor 1e-12
+absolute); flags and the uniforms drawn per event are identical.  The oracle
+is pinned to the reference's physics_angles, spectrometer_angles and sigep
+(tests/golden/gen) and to elastic kinematics (tests/test_gen_oracle.py).
+"""
+import numpy as np
+import pytest
+
+from simc_gfortran_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+TOL = 1e-12
+COLS = ("x", "y", "z", "rasterx", "rastery", "Ein", "eloss", "teff", "e_xptar", "e_yptar", "e_theta", "e_phi",
+        "e_E", "e_delta", "p_xptar", "p_yptar", "p_theta", "p_phi", "p_P", "p_E", "p_delta", "Q2", "nu", "q",
+        "jacobian", "sigcc", "weight")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from simc_gfortran_amd import api
+    c = api.Spectrometers()
+    yield c
+    c.close()
+
+
+def _settings(api, oracle, **over):
+    kw = dict(W.HEP_ELASTIC)
+    kw.update(over)
+    return api.hep_settings(**kw), oracle.make_hep(**kw)
+
+
+def _target(api, oracle):
+    return api.Target(**W.LH2_CELL), oracle.make_target(**W.LH2_CELL)
+
+
+def _compare(gpu, ref, n):
+    okg = gpu["ok"].cpu().numpy()
+    np.testing.assert_array_equal(okg, ref["ok"])
+    fails = {}
+    for k in COLS:
+        g = gpu[k].cpu().numpy()
+        r = ref[k]
+        tol = TOL * (np.abs(r) + 1.0)
+        if k == "p_yptar":
+            # spectrometer_angles forms dy = sqrt(1/cos^2 - 1 - dx^2): a
+            # last-bit difference in cos (libm vs device sin/cos) moves dy
+            # by about eps / |dy|, so near dy = 0 the bound widens as 2e-15 / |dy|
+            tol = tol + 2e-15 / np.maximum(np.abs(r), 1e-300)
+        bad = ~(np.abs(g - r) <= tol)
+        if bad.any():
+            fails[k] = (int(bad.sum()), float(np.max(np.abs(g - r))))
+    assert not fails, fails
+
+
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+def test_gen_parity_philox(ctx, oracle, pattern):
+    from simc_gfortran_amd import api
+    n = 200_000
+    H, Ho = _settings(api, oracle, fr_pattern=pattern, using_coulomb=1, coulomb=0.3)
+    T, To = _target(api, oracle)
+    print("gen philox pattern", pattern, n, flush=True)
+    out = ctx.gen_hep(T, H, n, seed=17, event0=5)
+    torch.cuda.synchronize()
+    ref, rc = oracle.gen_hep(To, Ho, n, rng="philox", seed=17, event0=5)
+    assert rc == 0
+    _compare(out, ref, n)
+    np.testing.assert_array_equal(out["used"].cpu().numpy(), ref["used"])
+
+
+def test_gen_parity_tape(ctx, oracle):
+    """On the reference's draw order (a tape of uniforms per event)."""
+    from simc_gfortran_amd import api
+    n, stride = 20_000, 48
+    tape = W.uniform_tape(n, stride, 61)
+    H, Ho = _settings(api, oracle, fr_pattern=2, fr1=0.05, fr2=0.15)
+    T, To = _target(api, oracle)
+    print("gen tape", n, flush=True)
+    out = ctx.gen_hep(T, H, n, rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride)
+    torch.cuda.synchronize()
+    ref, rc = oracle.gen_hep(To, Ho, n, rng="tape", tape=tape, stride=stride)
+    assert rc == 0
+    _compare(out, ref, n)
+    np.testing.assert_array_equal(out["used"].cpu().numpy(), ref["used"])
+
+
+def test_gen_tape_overflow_marks_events(ctx):
+    from simc_gfortran_amd import api
+    n, stride = 1000, 6  # too few uniforms for any event
+    H = api.hep_settings(**W.HEP_ELASTIC)
+    T = api.Target(**W.LH2_CELL)
+    out = ctx.gen_hep(T, H, n, rng="tape", tape=torch.full((n * stride,), 0.25, dtype=torch.float64,
+                                                           device="cuda"), stride=stride)
+    torch.cuda.synchronize()
+    assert (out["used"].cpu().numpy() == -1).all()
+    assert torch.isnan(out["weight"]).all() and (out["ok"] == 0).all()
+
+
+def test_gen_large_properties(ctx):
+    """1e7 events: every event succeeds, weights positive and finite, the
+    elastic relation E_p = nu + Mh holds, slopes stay in the generation
+    limits; a second launch with another event0 continues the streams."""
+    from simc_gfortran_amd import api
+    n = 10_000_000
+    H = api.hep_settings(**W.HEP_ELASTIC)
+    T = api.Target(**W.LH2_CELL)
+    print("gen large", n, flush=True)
+    out = ctx.gen_hep(T, H, n, seed=3)
+    torch.cuda.synchronize()
+    assert bool((out["ok"] == 1).all())
+    w = out["weight"]
+    assert bool(torch.isfinite(w).all()) and bool((w > 0).all())
+    rel = (out["p_E"] - out["nu"] - W.MP).abs() / out["p_E"]
+    assert float(rel.max()) < 1e-12
+    lo, hi = W.HEP_ELASTIC["e_yptar"]
+    assert float(out["e_yptar"].min()) >= lo and float(out["e_yptar"].max()) <= hi
+    tail = ctx.gen_hep(T, H, 1000, seed=3, event0=n - 1000)
+    torch.cuda.synchronize()
+    for k in ("Ein", "weight", "p_yptar"):
+        assert torch.equal(tail[k], out[k][n - 1000:])
+
+
+def test_gen_partial_outputs_and_errors(ctx):
+    from simc_gfortran_amd import api
+    H = api.hep_settings(**W.HEP_ELASTIC)
+    T = api.Target(**W.LH2_CELL)
+    full = ctx.gen_hep(T, H, 5000, seed=8)
+    only = {"weight": torch.empty(5000, dtype=torch.float64, device="cuda")}
+    ctx.gen_hep(T, H, 5000, seed=8, out=only)
+    torch.cuda.synchronize()
+    assert torch.equal(only["weight"], full["weight"])
+    assert ctx.gen_hep(T, H, 0, out={}) == {}
+    with pytest.raises(api.SimcError):
+        ctx.gen_hep(T, api.hep_settings(**dict(W.HEP_ELASTIC, fr_pattern=7)), 10)
+    with pytest.raises(api.SimcError):
+        ctx.gen_hep(T, api.hep_settings(**dict(W.HEP_ELASTIC, Mh=0.0)), 10)
