@@ -621,3 +621,33 @@ def gauss1_tape(tape, nsigmax):
     u = np.zeros(1, np.int64)
     v = L.or_gauss1_tape(_p(t), len(t), float(nsigmax), _p(u, _i64p))
     return v, int(u[0])
+
+
+def ref_gen_hep(target, hep, n, tape, stride, earm=5, harm=1):
+    """The reference's generate + complete_main (event.f, H(e,e'p) elastic,
+    radiation off) from oracle/_ref on per-event tape slots.  Returns
+    (dict of HEP_OUT_FIELDS + 'ok' + 'used', return code); the uniforms
+    used include complete_ev's losses of the two scattered particles."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    if not hasattr(L, "_gen_hep_ready"):
+        L.ref_gen_hep.restype = C.c_int
+        L.ref_gen_hep.argtypes = [_dp, C.c_int32, C.c_int32, C.c_int32, _dp, _i32p, C.c_int64, _dp, C.c_int64,
+                                  _dp, _i32p, _i32p]
+        L._gen_hep_ready = True
+    t = np.array([target.A, target.Z, target.rho, target.X0, target.length, target.angle, target.zoffset])
+    par = np.array([hep.Ebeam, hep.dEbeam, hep.xwid, hep.ywid, hep.xoffset, hep.yoffset, hep.fr1, hep.fr2,
+                    hep.coulomb, hep.e_yptar[0], hep.e_yptar[1], hep.e_xptar[0], hep.e_xptar[1], hep.e_theta,
+                    hep.e_phi, hep.e_P, hep.p_theta, hep.p_phi, hep.p_P, hep.Mh])
+    ipar = np.array([hep.fr_pattern, hep.using_eloss, hep.using_coulomb], np.int32)
+    tape = np.ascontiguousarray(tape, dtype=np.float64)
+    out = np.zeros((n, 27))
+    ok = np.zeros(n, np.int32)
+    used = np.zeros(n, np.int32)
+    rc = L.ref_gen_hep(_p(t), int(target.can), earm, harm, _p(par), _p(ipar, _i32p), n, _p(tape), int(stride),
+                       _p(out), _p(ok, _i32p), _p(used, _i32p))
+    res = {k: out[:, j].copy() for j, k in enumerate(HEP_OUT_FIELDS)}
+    res["ok"] = ok.astype(np.uint8)
+    res["used"] = used
+    return res, rc

@@ -251,3 +251,31 @@ int ref_track_to_tgt(int arm, int64_t n, const double *x_fp, const double *dx_fp
     if (chdir(cwd) != 0) return -3;
     return 0;
 }
+
+/* ---- generator: the reference's generate + complete_main (event.f) ---- */
+void ref_gen_one_(const double *par, const int32_t *ipar, double *out, int32_t *ok);
+
+/* n events of H(e,e'p) elastic generation without radiation, one tape slot
+ * each; target = A, Z, rho, X0, length, angle, zoffset; can; the electron
+ * and hadron arms' spectrometer codes (their windows enter complete_ev's
+ * losses, drawn after the generator's uniforms).  out: 27 values per event. */
+int ref_gen_hep(const double *target, int32_t can, int32_t earm, int32_t harm, const double *par,
+                const int32_t *ipar, int64_t n, const double *tape, int64_t stride, double *out,
+                int32_t *ok, int32_t *used)
+{
+    double a = target[0], z = target[1], rho = target[2], x0 = target[3], len = target[4];
+    double ang = target[5], zoff = target[6];
+    ref_set_target_(&a, &z, &rho, &x0, &len, &ang, &zoff, &can, &earm, &harm);
+    int rc = 0;
+    for (int64_t i = 0; i < n; i++) {
+        g_tape = tape + i * stride;
+        g_stride = stride;
+        g_pos = 0;
+        g_used = 0;
+        g_overflow = 0;
+        ref_gen_one_(par, ipar, out + 27 * i, ok + i);
+        used[i] = (int32_t)g_used;
+        if (g_overflow) rc = -2;
+    }
+    return rc;
+}

@@ -14,9 +14,10 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         vertices on per-event tapes
     python tests/golden/make_golden.py field    # target-field tracking (field/*.npz):
         the reference's trgInit, track_from_tgt and track_to_tgt (trg_track.f)
-    python tests/golden/make_golden.py gen      # generator pieces (gen/*.npz):
+    python tests/golden/make_golden.py gen      # generator (gen/*.npz):
         the reference's physics_angles, spectrometer_angles (event.f), sigep and
-        fofa_best_fit (physics_proton.f) over sampled arguments
+        fofa_best_fit (physics_proton.f) over sampled arguments, and its
+        generate + complete_main for H(e,e'p) elastic on per-event tapes
 """
 import hashlib
 import os
@@ -226,6 +227,24 @@ def main_gen():
     np.savez_compressed(os.path.join(out_dir, "sigep.npz"), Q2=q2, E=e, theta=the, Ein=ein, sigep=sig, qsquar=qsq,
                         GE=ff[:, 0], GM=ff[:, 1])
     print("gen goldens", NG, "angles,", NG, "sigep")
+    # the reference's generate + complete_main (H(e,e'p) elastic, radiation
+    # off) on per-event tapes, one case per raster pattern
+    tgt = O.make_target(**W.LH2_CELL)
+    for pat in (0, 1, 2, 3):
+        kw = dict(W.HEP_ELASTIC)
+        kw.update(fr_pattern=pat, fr1=0.1, fr2=0.2, using_coulomb=1, coulomb=0.7, using_eloss=int(pat != 1))
+        tape = W.uniform_tape(GEN_N, GEN_STRIDE, GEN_SEED + pat)
+        r, rc = O.ref_gen_hep(tgt, O.make_hep(**kw), GEN_N, tape, GEN_STRIDE)
+        assert rc == 0
+        np.savez_compressed(os.path.join(out_dir, f"hep_pattern{pat}.npz"), tape_seed=GEN_SEED + pat,
+                            stride=GEN_STRIDE, tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(),
+                            settings=np.array(repr(kw)), **{"out_" + k: v for k, v in r.items()})
+        print("hep_pattern", pat, "ok", int(r["ok"].sum()), "of", GEN_N)
+
+
+GEN_N = 2000
+GEN_STRIDE = 96
+GEN_SEED = 1300
 
 
 if __name__ == "__main__":

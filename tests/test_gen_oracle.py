@@ -13,9 +13,13 @@ Pins:
   the cube of the cosine to the central ray, weight = jacobian x sigep;
 * the draw order (beam x, y, raster, z, beam losses, Ein, yptar, xptar)
   against the uniforms placed on a tape, and the raster patterns' geometry.
-generate's full chain (complete_ev calls init.f's radc_init_ev, which the
-available flang does not compile) is not run from the reference itself.
+* the reference's own generate + complete_main (event.f with init.f's
+  radc_init_ev, radc.f, brem.f, target.f, physics_proton.f) on per-event
+  tapes for every raster pattern (tests/golden/gen/hep_pattern*.npz): all
+  27 outputs and the success flag bit for bit.
 """
+import ast
+import hashlib
 import math
 import os
 
@@ -225,3 +229,21 @@ def test_eloss_and_coulomb_switches(O):
     assert np.abs(a["Ein"] - 6400.0).max() <= 1.6 + 1e-9
     b, _ = O.gen_hep(T, _hep(O, using_eloss=0, using_coulomb=1, coulomb=5.0), 2000, seed=4)
     np.testing.assert_allclose(b["Ein"] - a["Ein"], 5.0, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+def test_oracle_matches_reference_generate_golden(O, pattern):
+    g = np.load(os.path.join(GOLD, f"hep_pattern{pattern}.npz"))
+    kw = ast.literal_eval(str(g["settings"]))
+    n = len(g["out_ok"])
+    stride = int(g["stride"])
+    tape = W.uniform_tape(n, stride, int(g["tape_seed"]))
+    assert hashlib.sha256(tape.tobytes()).hexdigest() == str(g["tape_sha256"])
+    out, rc = O.gen_hep(_target(O), O.make_hep(**kw), n, rng="tape", tape=tape, stride=stride)
+    assert rc == 0
+    np.testing.assert_array_equal(out["ok"], g["out_ok"])
+    for k in O.HEP_OUT_FIELDS:
+        np.testing.assert_array_equal(out[k], g["out_" + k], err_msg=k)
+    # the reference draws further uniforms after the generator's (complete_ev's
+    # losses of the two scattered particles, which the target step takes)
+    assert (out["used"] < g["out_used"]).all()

@@ -4,9 +4,15 @@ The device evaluates the oracle's operation order; its sin, cos, acos, atan,
 atan2, tan, pow and log may differ from the host's libm in the last bit, so
 coordinates, energies and weights agree within 1e-12 relative (floor 1e-12
 absolute); flags and the uniforms drawn per event are identical.  The oracle
-is pinned to the reference's physics_angles, spectrometer_angles and sigep
-(tests/golden/gen) and to elastic kinematics (tests/test_gen_oracle.py).
+is pinned to the reference's generate + complete_main run on tapes, its
+physics_angles, spectrometer_angles and sigep (tests/golden/gen) and to
+elastic kinematics (tests/test_gen_oracle.py); the GPU is also checked
+against the reference's generated events directly.
 """
+import ast
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -16,6 +22,8 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 TOL = 1e-12
+HERE = os.path.dirname(os.path.abspath(__file__))
+GEN_GOLD = sorted(glob.glob(os.path.join(HERE, "golden", "gen", "hep_pattern*.npz")))
 COLS = ("x", "y", "z", "rasterx", "rastery", "Ein", "eloss", "teff", "e_xptar", "e_yptar", "e_theta", "e_phi",
         "e_E", "e_delta", "p_xptar", "p_yptar", "p_theta", "p_phi", "p_P", "p_E", "p_delta", "Q2", "nu", "q",
         "jacobian", "sigcc", "weight")
@@ -141,3 +149,20 @@ def test_gen_partial_outputs_and_errors(ctx):
         ctx.gen_hep(T, api.hep_settings(**dict(W.HEP_ELASTIC, fr_pattern=7)), 10)
     with pytest.raises(api.SimcError):
         ctx.gen_hep(T, api.hep_settings(**dict(W.HEP_ELASTIC, Mh=0.0)), 10)
+
+
+@pytest.mark.parametrize("path", GEN_GOLD, ids=[os.path.basename(p)[:-4] for p in GEN_GOLD])
+def test_gpu_matches_reference_generate_golden(ctx, path):
+    from simc_gfortran_amd import api
+    g = np.load(path)
+    kw = ast.literal_eval(str(g["settings"]))
+    n = len(g["out_ok"])
+    stride = int(g["stride"])
+    tape = W.uniform_tape(n, stride, int(g["tape_seed"]))
+    print(os.path.basename(path), n, flush=True)
+    out = ctx.gen_hep(api.Target(**W.LH2_CELL), api.hep_settings(**kw), n, rng="tape",
+                      tape=torch.from_numpy(tape).cuda(), stride=stride)
+    torch.cuda.synchronize()
+    ref = {k: g["out_" + k] for k in COLS}
+    ref["ok"] = g["out_ok"]
+    _compare(out, ref, n)
