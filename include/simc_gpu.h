@@ -371,6 +371,10 @@ typedef struct {
  * (length, angle, zoffset, can, material). */
 int simc_gen_hep(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h, simc_hep_out *out,
                  int64_t n, const simc_rng *rng, void *stream);
+/* The same on host arrays (synchronous); returns SIMC_E_TAPE when an event's
+ * tape slot ran out. */
+int simc_gen_hep_host(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h, simc_hep_out *out,
+                      int64_t n, const simc_rng *rng);
 
 /* ---- Target-field tracking (polarized-target field, trg_track.f) ----
  * simc_field_create = trgInit(map, theta_e, phi_e, theta_p, phi_p)

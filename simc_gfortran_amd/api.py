@@ -66,7 +66,7 @@ EXPORTS = [
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
-    "simc_target_prep", "simc_target_prep_host", "simc_gen_hep",
+    "simc_target_prep", "simc_target_prep_host", "simc_gen_hep", "simc_gen_hep_host",
     "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
     "simc_track_from_tgt_host", "simc_track_to_tgt_host",
     "simc_ntuple_open", "simc_ntuple_write", "simc_ntuple_write_device", "simc_ntuple_close", "simc_stop_report",
@@ -209,6 +209,8 @@ def library(path: str = LIB_PATH):
     L.simc_recon_coefficients.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.simc_gen_hep.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(HepOut), C.c_int64,
                                C.POINTER(Rng), _vp]
+    L.simc_gen_hep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(HepOut), C.c_int64,
+                                    C.POINTER(Rng)]
     L.simc_target_prep.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
                                    C.POINTER(PrepOut), C.c_int64, C.POINTER(Rng), _vp]
     L.simc_target_prep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
@@ -463,6 +465,20 @@ class Spectrometers:
         st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _check(library().simc_gen_hep(self._h, C.byref(target), C.byref(hep), C.byref(O), int(n), C.byref(R),
                                       C.c_void_p(st)))
+        return out
+
+    def gen_hep_host(self, target, hep, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0):
+        """simc_gen_hep_host: the generator into numpy arrays (synchronous)."""
+        out = {k: np.zeros(n) for k in HEP_OUT_FIELDS}
+        out["ok"] = np.zeros(n, np.uint8)
+        out["used"] = np.zeros(n, np.int32)
+        O = HepOut(*[C.c_void_p(out[k].ctypes.data) for k in HEP_OUT_FIELDS + ("ok", "used")])
+        if rng == "tape":
+            tape = np.ascontiguousarray(tape, dtype=np.float64)
+            R = Rng(RNG_TAPE, 0, 0, event0, C.c_void_p(tape.ctypes.data), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        _check(library().simc_gen_hep_host(self._h, C.byref(target), C.byref(hep), C.byref(O), int(n), C.byref(R)))
         return out
 
     def target_prep_host(self, target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=None, stride=0):

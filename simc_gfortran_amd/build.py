@@ -84,6 +84,8 @@ def build_fortran(verbose: bool = True) -> bool:
     rpath = "-Wl,-rpath,$ORIGIN/../../../simc_gfortran_amd"
     cmds.append([FORTRAN, "-O2", "-I", FBUILD, os.path.join(tdir, "batch_host.f90"), mod_obj,
                  "-L", PKG, "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "batch_host")])
+    cmds.append([FORTRAN, "-O2", "-I", FBUILD, os.path.join(tdir, "gen_events.f90"), mod_obj,
+                 "-L", PKG, "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "gen_events")])
     cmds.append([FORTRAN, "-O2", os.path.join(tdir, "dropin_events.f90"),
                  "-L", PKG, "-lsimc_dropin", "-lsimc_gpu", rpath, "-o", os.path.join(FBIN, "dropin_events")])
     cmds.append([FORTRAN, "-O2", os.path.join(tdir, "field_events.f90"),

@@ -3155,6 +3155,71 @@ int simc_gen_hep(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h
     return SIMC_OK;
 }
 
+int simc_gen_hep_host(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h, simc_hep_out* out,
+                      int64_t n, const simc_rng* rng)
+{
+    if (!ctx || !t || !h || !out || !rng) return fail(SIMC_E_ARG, "simc_gen_hep_host: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n > ((int64_t)1 << 31)) return fail(SIMC_E_ARG, "simc_gen_hep_host: at most 2^31 events per call");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0 || rng->stride > (1 << 24)))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a stride in [1, 2^24]");
+    std::lock_guard<std::mutex> lock(ctx->host_mu);
+    double* hcol[kHepCols] = {out->x, out->y, out->z, out->rasterx, out->rastery, out->Ein, out->eloss,
+                              out->teff, out->e_xptar, out->e_yptar, out->e_theta, out->e_phi, out->e_E,
+                              out->e_delta, out->p_xptar, out->p_yptar, out->p_theta, out->p_phi,
+                              out->p_P, out->p_E, out->p_delta, out->Q2, out->nu, out->q,
+                              out->jacobian, out->sigcc, out->weight};
+    const size_t nd = (size_t)n * 8;
+    const size_t tape_bytes = rng->kind == SIMC_RNG_TAPE ? (size_t)n * (size_t)rng->stride * 8 : 0;
+    auto rounded = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t need = kHepCols * rounded(nd) + rounded((size_t)n) + rounded(4 * (size_t)n) + rounded(tape_bytes);
+    if (ctx->stage_bytes < need) {
+        if (ctx->d_stage) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(ctx->d_stage));
+            ctx->d_stage = nullptr;
+            ctx->stage_bytes = 0;
+        }
+        HIPCHK(hipMalloc(&ctx->d_stage, need));
+        ctx->stage_bytes = need;
+    }
+    char* base = (char*)ctx->d_stage;
+    size_t at = 0;
+    auto carve = [&](size_t bytes) {
+        char* p = base + at;
+        at += rounded(bytes);
+        return p;
+    };
+    double* dcol[kHepCols];
+    for (int k = 0; k < kHepCols; ++k) dcol[k] = hcol[k] ? (double*)carve(nd) : nullptr;
+    uint8_t* d_ok = out->ok ? (uint8_t*)carve((size_t)n) : nullptr;
+    int32_t* d_used = (int32_t*)carve(4 * (size_t)n);  // always staged: tape overflow check
+    simc_rng drng = *rng;
+    if (tape_bytes) {
+        double* d = (double*)carve(tape_bytes);
+        HIPCHK(hipMemcpy(d, rng->tape, tape_bytes, hipMemcpyHostToDevice));
+        drng.tape = d;
+    }
+    if (at > need) return fail(SIMC_E_INTERNAL, "simc_gen_hep_host: staging layout exceeds its buffer");
+    simc_hep_out dout = {dcol[0], dcol[1], dcol[2], dcol[3], dcol[4], dcol[5], dcol[6], dcol[7], dcol[8],
+                         dcol[9], dcol[10], dcol[11], dcol[12], dcol[13], dcol[14], dcol[15], dcol[16], dcol[17],
+                         dcol[18], dcol[19], dcol[20], dcol[21], dcol[22], dcol[23], dcol[24], dcol[25],
+                         dcol[26], d_ok, d_used};
+    int rc = simc_gen_hep(ctx, t, h, &dout, n, &drng, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    for (int k = 0; k < kHepCols; ++k)
+        if (hcol[k]) HIPCHK(hipMemcpy(hcol[k], dcol[k], nd, hipMemcpyDeviceToHost));
+    if (out->ok) HIPCHK(hipMemcpy(out->ok, d_ok, (size_t)n, hipMemcpyDeviceToHost));
+    std::vector<int32_t> used((size_t)n);
+    HIPCHK(hipMemcpy(used.data(), d_used, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    if (out->used) std::memcpy(out->used, used.data(), 4 * (size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        if (used[(size_t)i] < 0) return fail(SIMC_E_TAPE, "event " + std::to_string(i) + " ran out of its tape slot");
+    return SIMC_OK;
+}
+
 }  // extern "C"
 
 // A loaded target field (trgInit): the map on the device and the field

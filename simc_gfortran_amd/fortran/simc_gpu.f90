@@ -113,6 +113,16 @@ module simc_gpu
       type(simc_rng), intent(in) :: rng
     end function
 
+    integer(c_int) function simc_gen_hep_host(ctx, t, h, hout, n, rng) bind(C, name="simc_gen_hep_host")
+      import :: c_int, c_int64_t, c_ptr, simc_target, simc_hep_settings, simc_hep_out, simc_rng
+      type(c_ptr), value :: ctx
+      type(simc_target), intent(in) :: t
+      type(simc_hep_settings), intent(in) :: h
+      type(simc_hep_out), intent(in) :: hout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
     integer(c_int) function simc_target_prep(ctx, t, s, vin, pout, n, rng, stream) bind(C, name="simc_target_prep")
       import :: c_int, c_int64_t, c_ptr, simc_target, simc_prep_settings, simc_vertex_in, simc_prep_out, simc_rng
       type(c_ptr), value :: ctx, stream

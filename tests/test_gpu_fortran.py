@@ -148,3 +148,32 @@ def test_fortran_field_dropins_match_library(ctx, tmp_path):
     np.testing.assert_array_equal(ok[1], okb.cpu().numpy())
     assert ok[1].mean() > 0.9
     F.close()
+
+
+def test_fortran_generator_matches_library(ctx, tmp_path):
+    """simc_gen_hep_host called from Fortran through the module: the same
+    events as the library's device entry point, bit for bit."""
+    from simc_gfortran_amd import api
+    H = dict(W.HEP_ELASTIC)
+    par = np.array([H["Ebeam"], H["dEbeam"], H["xwid"], H["ywid"], H["xoffset"], H["yoffset"], H["fr1"], H["fr2"],
+                    H["coulomb"], *H["e_yptar"], *H["e_xptar"], H["e_theta"], H["e_phi"], H["e_P"], H["p_theta"],
+                    H["p_phi"], H["p_P"], H["Mh"]], np.float64)
+    ipar = np.array([H["fr_pattern"], H["using_eloss"], H["using_coulomb"]], np.int32)
+    with open(tmp_path / "set.bin", "wb") as f:
+        f.write(par.tobytes() + ipar.tobytes())
+    n, seed = 20000, 31
+    print("gen_events", n, flush=True)
+    r = subprocess.run([os.path.join(BIN, "gen_events"), MAPS, str(n), str(seed), str(tmp_path / "set.bin"),
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(tmp_path / "out.bin", dtype=np.uint8)
+    cols = raw[:27 * n * 8].view(np.float64).reshape(27, n)
+    ok = raw[27 * n * 8:]
+    ref = ctx.gen_hep(api.Target(**W.LH2_CELL), api.hep_settings(**H), n, seed=seed)
+    torch.cuda.synchronize()
+    assert (ok == ref["ok"].cpu().numpy()).all()
+    for j, k in enumerate(api.HEP_OUT_FIELDS):
+        np.testing.assert_array_equal(cols[j], ref[k].cpu().numpy(), err_msg=k)
+    host = ctx.gen_hep_host(api.Target(**W.LH2_CELL), api.hep_settings(**H), n, seed=seed)
+    for j, k in enumerate(api.HEP_OUT_FIELDS):
+        np.testing.assert_array_equal(host[k], cols[j], err_msg=k)
