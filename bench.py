@@ -299,19 +299,19 @@ def bench_target(ctx, n=10_000_000, reps=5):
             "data": "synthetic vertices in a 10 cm LH2 cell (can 3), loss and scattering sampled per event"}
 
 
-def bench_gen(ctx, n=10_000_000, reps=5):
-    """H(e,e'p) elastic events (6.4 GeV beam, 10 cm LH2 cell, electron in the
-    SHMS at 18 deg, proton in the HMS; simc_gen_hep): the generator alone
-    (all 27 output columns), and the coincidence chain generator -> target
-    step of each particle -> SHMS and HMS transport + reconstruction."""
+def bench_gen(ctx, n=10_000_000, chunks=10, reps=5):
+    """H(e,e'p) elastic events (10.6 GeV beam, 10 cm LH2 cell, electron in the
+    SHMS at 12.5 deg, proton in the HMS; simc_gen_hep, radiation off): the
+    generator alone (1e7 events, all 27 output columns), and the coincidence
+    chain generator -> target step of each particle -> SHMS and HMS transport
+    + reconstruction over chunks x n = 1e8 generated events (each chunk's
+    event0 continues the streams)."""
     import torch
     from simc_gfortran_amd import api, workloads as W
     dev = torch.device("cuda:0")
     Hs = W.HEP_ELASTIC
     H = api.hep_settings(**Hs)
     T = api.Target(**W.LH2_CELL)
-    g = None
-    tg, tc = [], []
     Pe = api.PrepSettings(2, 5, 1, 1, Hs["e_P"], Hs["e_theta"], Hs["e_phi"], W.ME, 0.0, Hs["Ebeam"],
                           (api.C.c_double * 5)())
     Pp = api.PrepSettings(3, 1, 1, 1, Hs["p_P"], Hs["p_theta"], Hs["p_phi"], W.MP, 0.0, Hs["Ebeam"],
@@ -319,34 +319,54 @@ def bench_gen(ctx, n=10_000_000, reps=5):
     cols = ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")
     pe = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in cols}
     pp = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in cols}
-    oe = op = None
+    g = None
+    tg = []
     for r in range(reps + 1):
-        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
         a.record()
         g = ctx.gen_hep(T, H, n, seed=2001, out=g)
         b.record()
+        torch.cuda.synchronize()
+        if r:
+            tg.append(a.elapsed_time(b))
+    oe = op = None
+
+    def chunk(k):
+        nonlocal g, oe, op
+        e0 = k * n
+        g = ctx.gen_hep(T, H, n, seed=2001, event0=e0, out=g)
         ve = {"x": g["x"], "y": g["y"], "z": g["z"], "E": g["e_E"], "p": g["e_E"], "theta": g["e_theta"],
               "xptar": g["e_xptar"], "yptar": g["e_yptar"]}
         vp = {"x": g["x"], "y": g["y"], "z": g["z"], "E": g["p_E"], "p": g["p_P"], "theta": g["p_theta"],
               "delta": g["p_delta"], "xptar": g["p_xptar"], "yptar": g["p_yptar"]}
-        ctx.target_prep(T, Pe, ve, rng="philox", seed=2001, out=pe)
-        ctx.target_prep(T, Pp, vp, rng="philox", seed=2002, out=pp)
-        oe = ctx.run(5, pe, rng="philox", seed=2003, fit="fp64", out=oe, p_spec=Hs["e_P"], m2=W.ME ** 2)
-        op = ctx.run(1, pp, rng="philox", seed=2004, fit="fp64", out=op, p_spec=Hs["p_P"], m2=W.MP ** 2)
-        c.record()
-        torch.cuda.synchronize()
-        if r:
-            tg.append(a.elapsed_time(b))
-            tc.append(a.elapsed_time(c))
-    g_ms, c_ms = float(np.mean(tg)), float(np.mean(tc))
-    both = (oe["ok"] == 1) & (op["ok"] == 1)
+        ctx.target_prep(T, Pe, ve, rng="philox", seed=2001, event0=e0, out=pe)
+        ctx.target_prep(T, Pp, vp, rng="philox", seed=2002, event0=e0, out=pp)
+        oe = ctx.run(5, pe, rng="philox", seed=2003, event0=e0, fit="fp64", out=oe, p_spec=Hs["e_P"],
+                     m2=W.ME ** 2, wire_sigma=0.03)
+        op = ctx.run(1, pp, rng="philox", seed=2004, event0=e0, fit="fp64", out=op, p_spec=Hs["p_P"],
+                     m2=W.MP ** 2)
+        both = (oe["ok"] == 1) & (op["ok"] == 1)
+        return both.sum(), torch.where(both, g["weight"], torch.zeros_like(g["weight"])).sum()
+
+    chunk(0)  # warmup
+    torch.cuda.synchronize()
+    a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    a.record()
+    sums = [chunk(k) for k in range(chunks)]
+    b.record()
+    torch.cuda.synchronize()
+    c_ms = a.elapsed_time(b)
+    g_ms = float(np.mean(tg))
+    ncoin = int(sum(int(c) for c, _ in sums))
+    wsum = float(sum(float(w) for _, w in sums))
     bytes_out = n * (27 * 8 + 1 + 4)
-    return {"workload": "hep_elastic_6p4GeV_shms18_hms_lh2_1e7", "events": n, "gen_ms": g_ms,
+    total = n * chunks
+    return {"workload": "hep_elastic_10p6GeV_shms12p5_hms_lh2", "gen_events": n, "gen_ms": g_ms,
             "gen_events_per_s": n / (g_ms * 1e-3), "gen_write_GBps": bytes_out / (g_ms * 1e-3) / 1e9,
-            "chain_ms": c_ms, "chain_events_per_s": n / (c_ms * 1e-3),
-            "coincidence_fraction": float(both.float().mean()),
-            "weighted_coincidences": float(torch.where(both, g["weight"], torch.zeros_like(g["weight"])).sum()),
-            "data": "synthetic: generated per event (Philox), radiation off"}
+            "chain_events": total, "chain_ms": c_ms, "chain_events_per_s": total / (c_ms * 1e-3),
+            "coincidences": ncoin, "coincidence_fraction": ncoin / total, "weighted_coincidences": wsum,
+            "data": "synthetic: generated per event (Philox), radiation off; beam spread uniform with the "
+                    "RMS of a 5e-4 Gaussian"}
 
 
 def bench_arms(ctx, n=10_000_000, reps=5):

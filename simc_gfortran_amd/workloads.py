@@ -205,11 +205,14 @@ def _elastic_central(ebeam: float, th_e_deg: float):
     return e1, math.hypot(px, pz), math.atan2(-px, pz)
 
 
-# H(e,e'p) elastic in Hall C: 6.4 GeV beam on the 10 cm LH2 cell, electron in
-# the SHMS at 18 deg, proton in the HMS at the conjugate angle (the settings
-# of simc_hep_settings; rastered 2 mm x 2 mm flat square, 0.05 % energy spread)
-_E1, _PP, _THP = _elastic_central(6400.0, 18.0)
-HEP_ELASTIC = dict(Ebeam=6400.0, dEbeam=3.2, xwid=0.01, ywid=0.01, xoffset=0.0, yoffset=0.0, fr1=0.1, fr2=0.1,
-                   coulomb=0.0, e_yptar=(-0.035, 0.035), e_xptar=(-0.05, 0.05), e_theta=math.radians(18.0),
-                   e_phi=arm_phi(5), e_P=_E1, p_theta=_THP, p_phi=arm_phi(1), p_P=_PP, Mh=MP, fr_pattern=3,
-                   using_eloss=1, using_coulomb=0)
+# H(e,e'p) elastic coincidence in Hall C (the coincidence configuration):
+# 10.6 GeV beam on the 10 cm LH2 cell, electron in the SHMS at 12.5 deg,
+# proton in the HMS at the conjugate angle, flat square raster +-1 mm.  The
+# reference spreads the beam energy uniformly over dEbeam (event.f:204); a
+# Gaussian of 5e-4 relative sigma is replaced by the uniform width with the
+# same RMS, dEbeam = sqrt(12) * 5e-4 * Ebeam.
+_E1, _PP, _THP = _elastic_central(10600.0, 12.5)
+HEP_ELASTIC = dict(Ebeam=10600.0, dEbeam=math.sqrt(12.0) * 5e-4 * 10600.0, xwid=0.01, ywid=0.01, xoffset=0.0,
+                   yoffset=0.0, fr1=0.1, fr2=0.1, coulomb=0.0, e_yptar=(-0.035, 0.035), e_xptar=(-0.05, 0.05),
+                   e_theta=math.radians(12.5), e_phi=arm_phi(5), e_P=_E1, p_theta=_THP, p_phi=arm_phi(1), p_P=_PP,
+                   Mh=MP, fr_pattern=3, using_eloss=1, using_coulomb=0)

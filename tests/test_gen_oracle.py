@@ -122,11 +122,11 @@ def test_elastic_kinematics_and_weight(O):
     # the generated slopes fill the generation limits
     assert out["e_yptar"].min() >= H["e_yptar"][0] and out["e_yptar"].max() <= H["e_yptar"][1]
     assert abs(out["e_xptar"].mean()) < 0.002
-    # +-2 deg in the electron's angle moves the elastic proton's momentum by
-    # up to about 15 %; its out-of-plane angle is about E' / p times the
-    # electron's (0.05 rad -> 0.1 rad)
-    assert np.abs(out["p_delta"]).max() < 20.0
-    assert np.abs(np.arccos(up @ zp)).max() < math.radians(10.0)
+    # +-2 deg about 12.5 deg in the electron's angle moves the elastic
+    # proton's momentum by up to about 22 %; its out-of-plane angle is about
+    # E' / p times the electron's (0.05 rad -> 0.14 rad, 8 deg)
+    assert np.abs(out["p_delta"]).max() < 30.0
+    assert np.abs(np.arccos(up @ zp)).max() < math.radians(14.0)
 
 
 def _tape(n, stride, seed):
@@ -226,7 +226,8 @@ def test_eloss_and_coulomb_switches(O):
     T = _target(O)
     a, _ = O.gen_hep(T, _hep(O, using_eloss=0, using_coulomb=0, coulomb=5.0), 2000, seed=4)
     assert (a["eloss"] == 0.0).all()
-    assert np.abs(a["Ein"] - 6400.0).max() <= 1.6 + 1e-9
+    H = W.HEP_ELASTIC
+    assert np.abs(a["Ein"] - H["Ebeam"]).max() <= H["dEbeam"] / 2 + 1e-9
     b, _ = O.gen_hep(T, _hep(O, using_eloss=0, using_coulomb=1, coulomb=5.0), 2000, seed=4)
     np.testing.assert_allclose(b["Ein"] - a["Ein"], 5.0, rtol=0, atol=1e-9)
 
