@@ -240,6 +240,17 @@ def main_gen():
                             stride=GEN_STRIDE, tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(),
                             settings=np.array(repr(kw)), **{"out_" + k: v for k, v in r.items()})
         print("hep_pattern", pat, "ok", int(r["ok"].sum()), "of", GEN_N)
+    # degenerate settings: xptar range of zero width (physics_angles' dx = 0
+    # branch), no beam spread or spot, an annulus of zero width, losses off
+    kw = dict(W.HEP_ELASTIC)
+    kw.update(fr_pattern=2, fr1=0.1, fr2=0.1, xwid=0.0, ywid=0.0, dEbeam=0.0, e_xptar=(0.0, 0.0), using_eloss=0)
+    tape = W.uniform_tape(GEN_N, GEN_STRIDE, GEN_SEED + 10)
+    r, rc = O.ref_gen_hep(tgt, O.make_hep(**kw), GEN_N, tape, GEN_STRIDE)
+    assert rc == 0
+    np.savez_compressed(os.path.join(out_dir, "hep_pattern2_edges.npz"), tape_seed=GEN_SEED + 10, stride=GEN_STRIDE,
+                        tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(), settings=np.array(repr(kw)),
+                        **{"out_" + k: v for k, v in r.items()})
+    print("hep_pattern2_edges ok", int(r["ok"].sum()), "of", GEN_N)
 
 
 GEN_N = 2000

@@ -232,7 +232,7 @@ def test_eloss_and_coulomb_switches(O):
     np.testing.assert_allclose(b["Ein"] - a["Ein"], 5.0, rtol=0, atol=1e-9)
 
 
-@pytest.mark.parametrize("pattern", [0, 1, 2, 3])
+@pytest.mark.parametrize("pattern", ["0", "1", "2", "3", "2_edges"])
 def test_oracle_matches_reference_generate_golden(O, pattern):
     g = np.load(os.path.join(GOLD, f"hep_pattern{pattern}.npz"))
     kw = ast.literal_eval(str(g["settings"]))
@@ -248,3 +248,13 @@ def test_oracle_matches_reference_generate_golden(O, pattern):
     # the reference draws further uniforms after the generator's (complete_ev's
     # losses of the two scattered particles, which the target step takes)
     assert (out["used"] < g["out_used"]).all()
+
+
+def test_degenerate_settings_golden_covers_its_branches(O):
+    """The edge case golden exercises what it is for: xptar = 0 (phi = phi0
+    from physics_angles), no spread or spot, raster on its circle."""
+    g = np.load(os.path.join(GOLD, "hep_pattern2_edges.npz"))
+    kw = ast.literal_eval(str(g["settings"]))
+    assert (g["out_e_xptar"] == 0.0).all() and (g["out_e_phi"] == kw["e_phi"]).all()
+    assert (g["out_Ein"] == kw["Ebeam"]).all() and (g["out_eloss"] == 0.0).all()
+    np.testing.assert_allclose(np.hypot(g["out_x"], g["out_y"]), 0.1, rtol=0, atol=1e-15)
