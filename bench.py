@@ -330,6 +330,11 @@ def bench_gen(ctx, n=10_000_000, chunks=10, reps=5):
         if r:
             tg.append(a.elapsed_time(b))
     oe = op = None
+    # the electron side (target step + SHMS) and the proton side (target step
+    # + HMS) are independent once the generator has run: two streams (default
+    # priority), joined before the next chunk reuses the buffers
+    main = torch.cuda.current_stream(dev)
+    s_e, s_p = torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)
 
     def chunk(k):
         nonlocal g, oe, op
@@ -339,12 +344,18 @@ def bench_gen(ctx, n=10_000_000, chunks=10, reps=5):
               "xptar": g["e_xptar"], "yptar": g["e_yptar"]}
         vp = {"x": g["x"], "y": g["y"], "z": g["z"], "E": g["p_E"], "p": g["p_P"], "theta": g["p_theta"],
               "delta": g["p_delta"], "xptar": g["p_xptar"], "yptar": g["p_yptar"]}
-        ctx.target_prep(T, Pe, ve, rng="philox", seed=2001, event0=e0, out=pe)
-        ctx.target_prep(T, Pp, vp, rng="philox", seed=2002, event0=e0, out=pp)
-        oe = ctx.run(5, pe, rng="philox", seed=2003, event0=e0, fit="fp64", out=oe, p_spec=Hs["e_P"],
-                     m2=W.ME ** 2, wire_sigma=0.03)
-        op = ctx.run(1, pp, rng="philox", seed=2004, event0=e0, fit="fp64", out=op, p_spec=Hs["p_P"],
-                     m2=W.MP ** 2)
+        s_e.wait_stream(main)
+        s_p.wait_stream(main)
+        with torch.cuda.stream(s_e):
+            ctx.target_prep(T, Pe, ve, rng="philox", seed=2001, event0=e0, out=pe)
+            oe = ctx.run(5, pe, rng="philox", seed=2003, event0=e0, fit="fp64", out=oe, p_spec=Hs["e_P"],
+                         m2=W.ME ** 2, wire_sigma=0.03)
+        with torch.cuda.stream(s_p):
+            ctx.target_prep(T, Pp, vp, rng="philox", seed=2002, event0=e0, out=pp)
+            op = ctx.run(1, pp, rng="philox", seed=2004, event0=e0, fit="fp64", out=op, p_spec=Hs["p_P"],
+                         m2=W.MP ** 2)
+        main.wait_stream(s_e)
+        main.wait_stream(s_p)
         both = (oe["ok"] == 1) & (op["ok"] == 1)
         return both.sum(), torch.where(both, g["weight"], torch.zeros_like(g["weight"])).sum()
 
