@@ -89,3 +89,40 @@ def test_coincidence_chain_matches_oracle(ctx, oracle):
     assert both_g.mean() > 0.1
     w = gd["weight"].cpu().numpy()
     np.testing.assert_allclose(w[both_g].sum(), go["weight"][both_r].sum(), rtol=1e-12)
+
+
+def test_two_arms_on_two_streams_match_sequential_runs(ctx):
+    """One context, SHMS and HMS issued on two streams at once: outputs and
+    STOP counters bit-identical to the same runs one after the other (the
+    survivor queues and counters are per arm)."""
+    from simc_gfortran_amd import api  # noqa: F401
+    n = 2_000_000
+    wls = (W.SHMS_C1, W.HMS_C0)
+    evs = [{k: torch.from_numpy(v).cuda() for k, v in W.generate(wl, n, seed=50 + i).items()}
+           for i, wl in enumerate(wls)]
+    kws = [dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma, rng="philox", seed=60,
+                fit="fp64") for wl in wls]
+    print("two streams", n, flush=True)
+    seq, seq_cnt = [], []
+    for wl, ev, kw in zip(wls, evs, kws):
+        ctx.reset_counters(wl.arm)
+        seq.append(ctx.run(wl.arm, ev, **kw))
+        torch.cuda.synchronize()
+        seq_cnt.append(list(ctx.counters(wl.arm).values()))
+    for wl in wls:
+        ctx.reset_counters(wl.arm)
+    main = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    par = []
+    for s, wl, ev, kw in zip(streams, wls, evs, kws):
+        s.wait_stream(main)
+        with torch.cuda.stream(s):
+            par.append(ctx.run(wl.arm, ev, **kw))
+    for s in streams:
+        main.wait_stream(s)
+    torch.cuda.synchronize()
+    for i, wl in enumerate(wls):
+        for k, t in seq[i].items():
+            if t is not None:
+                assert torch.equal(t, par[i][k]), (wl.name, k)
+        assert list(ctx.counters(wl.arm).values()) == seq_cnt[i]
