@@ -132,7 +132,10 @@ def ref_lib():
         L.ref_run.restype = C.c_int
         L.ref_run.argtypes = ([C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                C.c_double, C.c_int, C.c_int64] + [_dp] * 7 + [_dp] * 11 +
-                              [_u8p, _i32p, _dp, C.c_int64, C.c_char_p, _i64p])
+                              [_u8p, _i32p, _dp, C.c_int64, C.c_char_p, _i64p,
+                               C.POINTER(C.c_uint64)])
+        L.ref_lfit.restype = None
+        L.ref_lfit.argtypes = [C.c_int64, C.c_int32] + [C.POINTER(C.c_float)] * 5
         _ref = L
     return _ref
 
@@ -195,15 +198,52 @@ def ref_run(arm, events, *, p_spec, m2, tape, stride, ms=True, wcs=True, decay=F
     out = _prep_outputs(n, ev)
     tape = np.ascontiguousarray(tape, dtype=np.float64)
     cnt = np.zeros(64, np.int64)
+    mask = np.zeros(n, np.uint64)
     rc = L.ref_run(arm, p_spec, m2, int(ms), int(wcs), int(decay), ctau, int(coll), n,
                    *[_p(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")],
                    *[_p(out[k]) for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp",
                                           "y_fp", "dy_fp", "pathlen", "resmult", "m2")],
                    _p(out["ok"], _u8p), _p(out["used"], _i32p), _p(tape), int(stride),
-                   MAP_DIR.encode(), _p(cnt, _i64p))
+                   MAP_DIR.encode(), _p(cnt, _i64p), _p(mask, C.POINTER(C.c_uint64)))
     out["rc"] = rc
     out["counters"] = cnt[:NCOUNT[arm]]
+    # bit c set: the event moved STOP counter c of the arm's common block
+    out["stop_mask"] = mask
     return out
+
+
+def ref_lfit(X, Y):
+    """The reference's LFIT (cern/lfit.f, compiled into oracle/_ref) over the
+    rows of X and Y (float32, n x L), KEY = 0.  Returns A, B, E (float32)."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    n, npts = X.shape
+    A, B, E = (np.zeros(n, np.float32) for _ in range(3))
+    fp = C.POINTER(C.c_float)
+    L.ref_lfit(n, npts, _p(X, fp), _p(Y, fp), _p(A, fp), _p(B, fp), _p(E, fp))
+    return A, B, E
+
+
+def lfit_f32(X, Y):
+    """The oracle's LFIT restatement (REAL*4 points, REAL*8 sums) over the rows
+    of X and Y, KEY = 0.  Returns A, B, E (float32)."""
+    L = lib()
+    fn = L.or_lfit_f32
+    fp = C.POINTER(C.c_float)
+    fn.restype = None
+    fn.argtypes = [fp, fp, C.c_int, C.c_int, fp, fp, fp]
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    n, npts = X.shape
+    A, B, E = (np.zeros(n, np.float32) for _ in range(3))
+    a, b, e = C.c_float(), C.c_float(), C.c_float()
+    for i in range(n):
+        fn(X[i].ctypes.data_as(fp), Y[i].ctypes.data_as(fp), npts, 0, C.byref(a), C.byref(b), C.byref(e))
+        A[i], B[i], E[i] = a.value, b.value, e.value
+    return A, B, E
 
 
 def philox_block(ctr, key):

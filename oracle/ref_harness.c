@@ -3,15 +3,13 @@
  * oracle/Makefile from the reference's own sources into oracle/_ref/) event
  * by event.  TEST INFRASTRUCTURE ONLY.
  *
- * Two routines of the path cannot be compiled from the reference tree in
- * this image (cern/lfit.f and cern/ranlux.f include CERNLIB headers that the
- * tree does not ship), so this harness supplies:
+ * The harness supplies one routine of the path:
  *   grnd_()  -- the uniform source, read from a per-event tape (the reference
- *               draws from RANLUX through call_ranlux.f:58-73);
- *   lfit_()  -- the oracle's restatement of cern/lfit.f:11-60.
+ *               draws from RANLUX through call_ranlux.f:58-73).
  * Everything else (mc_hms, mc_hms_hut, mc_hms_recon, mc_sos, mc_sos_hut,
  * mc_sos_recon, mc_hrsr*, mc_hrsl*, mc_shms, mc_shms_hut, mc_shms_recon,
- * transp/transp_init, project, musc, musc_ext, rotate_*,
+ * transp/transp_init, project, musc, musc_ext, rotate_*, LFIT (cern/lfit.f,
+ * compiled with its CERNLIB '#' lines removed as gfortran ignores them),
  * gauss1, enerloss_new, loren, the collimator routines) is the reference's
  * own code.
  */
@@ -75,9 +73,15 @@ double grnd_(void)
     return g_tape[g_pos++];
 }
 
-void lfit_(float *X, float *Y, int32_t *L, int32_t *KEY, float *A, float *B, float *E)
+/* The reference's LFIT (cern/lfit.f:11-60) over n point sets of L points
+ * each (rows of X and Y), KEY = 0: A, B, E per set. */
+extern void lfit_(float *X, float *Y, int32_t *L, int32_t *KEY, float *A, float *B, float *E);
+void ref_lfit(int64_t n, int32_t L, const float *X, const float *Y, float *A, float *B, float *E)
 {
-    or_lfit_f32(X, Y, *L, *KEY, A, B, E);
+    for (int64_t i = 0; i < n; i++) {
+        int32_t l = L, key = 0;
+        lfit_((float *)(X + i * L), (float *)(Y + i * L), &l, &key, A + i, B + i, E + i);
+    }
 }
 
 /* Runs n events through mc_hms (arm 1), mc_sos (2), mc_hrsr (3), mc_hrsl (4) or mc_shms (5).  Same SoA
@@ -90,7 +94,8 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
             const double *dydz_in, const double *fry_in, double *dpp, double *y, double *dxdz,
             double *dydz, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
             double *pathlen, double *resmult, double *m2_out, uint8_t *ok, int32_t *used,
-            const double *tape, int64_t stride, const char *mapdir, int64_t *counters)
+            const double *tape, int64_t stride, const char *mapdir, int64_t *counters,
+            uint64_t *stop_mask)
 {
     char cwd[4096];
     if (!getcwd(cwd, sizeof cwd)) return -1;
@@ -103,7 +108,9 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
     memcpy(before, common, ncount * sizeof(int32_t));
     decd_.ctau = ctau;
     int rc = 0;
+    int32_t prev[S_NCOUNT];
     for (int64_t i = 0; i < n; i++) {
+        memcpy(prev, common, ncount * sizeof(int32_t));
         g_tape = tape + i * stride;
         g_stride = stride;
         g_pos = 0;
@@ -145,6 +152,15 @@ int ref_run(int arm, double p_spec, double m2_in, int ms_flag, int wcs_flag, int
         ok[i] = (uint8_t)(okv != 0);
         if (used) used[i] = (int32_t)g_used;
         if (g_overflow) rc = OR_E_TAPE;
+        /* which of the reference's STOP counters this event moved (bit c for
+         * counter c of the common block): its exit point as the reference
+         * records it */
+        if (stop_mask) {
+            uint64_t m = 0;
+            for (int c = 0; c < ncount && c < 64; c++)
+                if (common[c] != prev[c]) m |= (uint64_t)1 << c;
+            stop_mask[i] = m;
+        }
     }
     const int32_t *after = common;
     for (int c = 0; c < ncount; c++) counters[c] += (int64_t)(after[c] - before[c]);

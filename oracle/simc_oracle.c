@@ -639,7 +639,9 @@ double or_gauss1(or_rng *g, double nsigmax)
 }
 
 /* LFIT, cern/lfit.f:11-60.  X, Y, A, B, E are REAL*4; the sums are the
- * implicit REAL type, 8-byte under -fdefault-real-8. */
+ * implicit REAL type, 8-byte under -fdefault-real-8.  Pinned bit for bit
+ * against the reference's own lfit.f (built into oracle/_ref) on seeded REAL*4
+ * point sets (tests/golden/lfit/ref_lfit.npz). */
 void or_lfit_f32(const float *X, const float *Y, int L, int KEY, float *A, float *B, float *E)
 {
     if (L - 2 < 0) goto l25;

@@ -196,6 +196,57 @@ def uniform_tape(n: int, stride: int, seed: int) -> np.ndarray:
     return t
 
 
+def lfit_point_sets(n: int, seed: int, npts: int = 12):
+    """Point sets for the straight-line fit LFIT (cern/lfit.f:11-60) with
+    KEY = 0: X are plane positions, Y hits, both REAL*4 (float32).
+
+    Most sets look like a wire chamber's: 12 planes in two chambers of six
+    (z within +-60 cm, irregular spacing), hits on a line with 0.02-0.3 cm
+    smearing, none of the values exactly representable in few bits; some
+    planes read 0.0 (a y plane in an x fit, or a hit exactly 0), which LFIT
+    skips.  The rest are edge cases: no, one or two non-zero hits, all planes
+    at one position (SUMXX = 0), large and tiny magnitudes, negative zero.
+    """
+    rng = np.random.default_rng(seed)
+    z0 = rng.uniform(-60.0, -40.0, (n, 1)) + np.sort(rng.uniform(0.0, 12.0, (n, npts // 2)), axis=1)
+    z1 = rng.uniform(30.0, 45.0, (n, 1)) + np.sort(rng.uniform(0.0, 12.0, (n, npts - npts // 2)), axis=1)
+    X = np.concatenate([z0, z1], axis=1)
+    slope = rng.uniform(-0.1, 0.1, (n, 1))
+    icpt = rng.uniform(-40.0, 40.0, (n, 1))
+    Y = slope * X + icpt + rng.normal(0.0, 1.0, (n, 1)) * rng.uniform(0.02, 0.3, (n, 1)) * \
+        rng.standard_normal((n, npts))
+    # 0-4 planes read exactly zero
+    nz = rng.integers(0, 5, n)
+    for i in np.nonzero(nz)[0]:
+        Y[i, rng.choice(npts, nz[i], replace=False)] = 0.0
+    k = n // 20  # edge cases in the last 5 % of sets, 1/8 of them each
+    e = n - k
+    kinds = np.arange(k) % 8
+    for j, kind in enumerate(kinds):
+        i = e + j
+        if kind == 0:
+            Y[i] = 0.0                                     # no points
+        elif kind == 1:
+            Y[i] = 0.0
+            Y[i, rng.integers(npts)] = rng.uniform(-5, 5)  # one point
+        elif kind == 2:
+            keep = rng.choice(npts, 2, replace=False)      # two points: E = 0 branch
+            m = np.zeros(npts, bool)
+            m[keep] = True
+            Y[i, ~m] = 0.0
+        elif kind == 3:
+            X[i] = X[i, 0]                                 # all planes at one z: SUMXX = 0
+        elif kind == 4:
+            Y[i] *= 1e4                                    # large hits
+        elif kind == 5:
+            Y[i] *= 1e-6                                   # tiny hits
+        elif kind == 6:
+            Y[i, ::3] = -0.0                               # negative zeros are skipped too
+        else:
+            Y[i] = rng.uniform(-1.0, 1.0) + 1e-3 * rng.standard_normal(npts)  # nearly flat
+    return X.astype(np.float32), Y.astype(np.float32)
+
+
 def _elastic_central(ebeam: float, th_e_deg: float):
     """E' and the proton's momentum and angle of H(e,e'p) at the central
     electron angle (two-body elastic kinematics)."""

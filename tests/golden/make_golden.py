@@ -14,6 +14,8 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         vertices on per-event tapes
     python tests/golden/make_golden.py field    # target-field tracking (field/*.npz):
         the reference's trgInit, track_from_tgt and track_to_tgt (trg_track.f)
+    python tests/golden/make_golden.py lfit     # track fit (lfit/ref_lfit.npz): the
+        reference's LFIT (cern/lfit.f) over seeded REAL*4 point sets
     python tests/golden/make_golden.py gen      # generator (gen/*.npz):
         the reference's physics_angles, spectrometer_angles (event.f), sigep and
         fofa_best_fit (physics_proton.f) over sampled arguments, and its
@@ -78,7 +80,7 @@ def main(only=None):
             **{"in_" + k: ev[k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")},
             **{"out_" + k: out[k] for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp",
                                              "dy_fp", "pathlen", "resmult", "m2", "ok", "used")},
-            counters=out["counters"])
+            counters=out["counters"], stop_mask=out["stop_mask"])
         print(name, "accepted", int(out["ok"].sum()), "of", N)
 
 
@@ -257,6 +259,19 @@ GEN_N = 2000
 GEN_STRIDE = 96
 GEN_SEED = 1300
 
+# LFIT (cern/lfit.f, compiled into oracle/_ref as gfortran sees it): the
+# reference's A, B, E over seeded REAL*4 point sets (W.lfit_point_sets).
+LFIT_N = 12000
+LFIT_SEED = 1500
+
+
+def main_lfit():
+    X, Y = W.lfit_point_sets(LFIT_N, LFIT_SEED)
+    A, B, E = O.ref_lfit(X, Y)
+    np.savez_compressed(os.path.join(HERE, "lfit", "ref_lfit.npz"), seed=LFIT_SEED, n=LFIT_N,
+                        points_sha256=hashlib.sha256(X.tobytes() + Y.tobytes()).hexdigest(), A=A, B=B, E=E)
+    print("lfit", LFIT_N, "sets; A = B = 0 for", int(((A == 0) & (B == 0)).sum()))
+
 
 if __name__ == "__main__":
     if sys.argv[1:] == ["gen"]:
@@ -265,5 +280,7 @@ if __name__ == "__main__":
         main_target()
     elif sys.argv[1:] == ["field"]:
         main_field()
+    elif sys.argv[1:] == ["lfit"]:
+        main_lfit()
     else:
         main(sys.argv[1] if sys.argv[1:] else None)

@@ -190,26 +190,35 @@ GOLDEN = sorted(__import__("glob").glob(__import__("os").path.join(
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
-def test_gpu_matches_reference_golden(ctx, path):
+def test_gpu_matches_reference_golden(ctx, oracle, path):
     """The GPU against the reference's own outputs (REAL*4 track fit as in the
-    reference): identical flags and uniform counts, coordinates within 1e-12
-    except events whose REAL*4 rounding flips (counted, rare)."""
+    reference): identical acceptance, uniform counts and STOP counters;
+    per-event stop codes identical to the oracle's, whose per-event exit
+    points equal the reference's (tests/test_oracle.py); coordinates within
+    1e-12 except events whose REAL*4 rounding flips (counted, rare)."""
     g = np.load(path)
     stride = int(g["stride"])
     n = len(g["in_dpp"])
     tape = W.uniform_tape(n, stride, int(g["tape_seed"]))
     p_spec, m2, ms, wcs, decay, coll, ctau = g["settings"]
+    arm = int(g["arm"])
     ev = {k: g["in_" + k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
-    out = ctx.run(int(g["arm"]), to_dev(ev), p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs),
-                  decay=bool(decay), coll=bool(coll), ctau=ctau, rng="tape",
-                  tape=torch.from_numpy(tape).cuda(), stride=stride, fit="ref_f32")
+    kw = dict(p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs), decay=bool(decay), coll=bool(coll), ctau=ctau)
+    ctx.reset_counters(arm)
+    out = ctx.run(arm, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
+                  fit="ref_f32", **kw)
     got = to_host(out)
     ref = {k: g["out_" + k] for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp",
                                       "pathlen", "ok", "used")}
-    ref["stop"] = np.where(ref["ok"] == 1, 0, 1)
-    got_stop = np.where(got["ok"] == 1, 0, 1)
-    assert (got_stop == ref["stop"]).all()
-    assert (got["used"] == ref["used"]).all()
-    rep = parity_report({**got, "stop": got_stop}, ref, 1e-12)
+    np.testing.assert_array_equal(got["ok"], ref["ok"])
+    np.testing.assert_array_equal(got["used"], ref["used"])
+    orc = oracle.run(arm, ev, rng="tape", tape=tape, stride=stride, fit_f32=True, **kw)
+    np.testing.assert_array_equal(got["stop"], orc["stop"])
+    cnt = ctx.counters(arm)
+    for i, k in enumerate(cnt):
+        if k != "stop_id":  # a last-value register, not a counter
+            assert cnt[k] == g["counters"][i], (k, cnt[k], g["counters"][i])
+    ref["stop"] = got["stop"]
+    rep = parity_report(got, ref, 1e-12)
     print(path.split("/")[-1], rep)
     assert max(rep["bad"].values()) <= max(2, n // 1000), rep

@@ -50,6 +50,35 @@ def test_oracle_matches_reference_golden(oracle, path):
             assert out["counters"][i] == g["counters"][i], name
 
 
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_exit_points_match_reference_per_event(oracle, path):
+    """Each event's exit point: the STOP counters the oracle moves for that
+    event alone equal the counters the reference moved for it (recorded per
+    event in the golden file), for every event."""
+    g = np.load(path)
+    stride = int(g["stride"])
+    n = len(g["in_dpp"])
+    tape = W.uniform_tape(n, stride, int(g["tape_seed"])).reshape(n, stride)
+    p_spec, m2, ms, wcs, decay, coll, ctau = g["settings"]
+    ev = {k: g["in_" + k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
+    names = {1: oracle.HMS_COUNTERS, 2: oracle.SOS_COUNTERS, 3: oracle.HRS_COUNTERS, 4: oracle.HRS_COUNTERS,
+             5: oracle.SHMS_COUNTERS}[int(g["arm"])]
+    skip = names.index("stop_id") if "stop_id" in names else -1
+    bad = []
+    for i in range(n):
+        out = oracle.run(int(g["arm"]), {k: v[i:i + 1] for k, v in ev.items()}, p_spec=p_spec, m2=m2,
+                         ms=bool(ms), wcs=bool(wcs), decay=bool(decay), coll=bool(coll), ctau=ctau,
+                         rng="tape", tape=tape[i], stride=stride, fit_f32=True, nthreads=1)
+        m = 0
+        for c, v in enumerate(out["counters"]):
+            if v and c != skip:
+                m |= 1 << c
+        ref = int(g["stop_mask"][i]) & ~((1 << skip) if skip >= 0 else 0)
+        if m != ref:
+            bad.append((i, int(out["stop"][0]), hex(m), hex(ref)))
+    assert not bad, bad[:10]
+
+
 # ---------------------------------------------------------------------------
 # random numbers
 # ---------------------------------------------------------------------------
@@ -231,6 +260,82 @@ def test_lfit_exact_line_and_zero_skip(oracle, f32):
     assert (a2, b2) == (a, b)
     # fewer than two points: A = B = 0
     assert lfit(oracle, [1.0, 2.0], [0.0, 5.0], f32)[:2] == (0.0, 0.0)
+
+
+def _lfit_golden():
+    g = np.load(os.path.join(HERE, "golden", "lfit", "ref_lfit.npz"))
+    X, Y = W.lfit_point_sets(int(g["n"]), int(g["seed"]))
+    assert hashlib.sha256(X.tobytes() + Y.tobytes()).hexdigest() == str(g["points_sha256"])
+    return X, Y, g
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def test_lfit_matches_reference_vectors(oracle):
+    """or_lfit_f32 against the reference's own LFIT (cern/lfit.f:11-60, built
+    as gfortran compiles it) on 12,000 REAL*4 point sets: A, B and E
+    identical bit for bit (non-representable hits, skipped zero hits, the
+    edge cases of the routine's branches)."""
+    X, Y, g = _lfit_golden()
+    A, B, E = oracle.lfit_f32(X, Y)
+    for name, got, ref in (("A", A, g["A"]), ("B", B, g["B"]), ("E", E, g["E"])):
+        np.testing.assert_array_equal(_bits(got), _bits(ref), err_msg=name)
+    # the vectors reach every branch: no fit (A = B = E = 0), two points (E = 0),
+    # a full fit
+    assert ((g["A"] == 0) & (g["B"] == 0)).sum() > 100
+    assert ((g["E"] == 0) & (g["A"] != 0)).sum() > 30
+    assert (g["E"] > 0).sum() > 10000
+
+
+def test_lfit_sensitivity_of_the_vectors(oracle):
+    """The vectors tell the REAL*4-in / REAL*8-sums semantics from the
+    all-float and the all-double variants of the same formulas."""
+    X, Y, g = _lfit_golden()
+    old_err = np.seterr(divide="ignore", invalid="ignore")  # sets with SUMXX = 0 give no fit
+    sel = slice(None)
+    x, y = X[sel].astype(np.float64), Y[sel].astype(np.float64)
+    keep = y != 0.0
+    cnt = keep.sum(axis=1)
+    ok = cnt > 1
+    x, y, keep, cnt = x[ok], y[ok], keep[ok], cnt[ok]
+    refA, refB = g["A"][sel][ok], g["B"][sel][ok]
+
+    def fit(t):
+        xs, ys = x.astype(t), y.astype(t)
+        c = cnt.astype(t)
+        xm = (xs * keep).sum(axis=1, dtype=t) / c
+        ym = (ys * keep).sum(axis=1, dtype=t) / c
+        sx = (xs - xm[:, None]) * keep
+        sy = (ys - ym[:, None]) * keep
+        a = ((sx * sy).sum(axis=1, dtype=t) / (sx * sx).sum(axis=1, dtype=t)).astype(np.float32)
+        b = (ym - a.astype(t) * xm).astype(np.float32)
+        return a, b
+
+    a32, b32 = fit(np.float32)
+    assert (_bits(a32) != _bits(refA)).sum() + (_bits(b32) != _bits(refB)).sum() > 100
+    # B from the double slope instead of the REAL*4 A
+    xm = (x * keep).sum(axis=1) / cnt
+    ym = (y * keep).sum(axis=1) / cnt
+    sx = (x - xm[:, None]) * keep
+    sy = (y - ym[:, None]) * keep
+    a64 = (sx * sy).sum(axis=1) / (sx * sx).sum(axis=1)
+    b_dbl = (ym - a64 * xm).astype(np.float32)
+    np.seterr(**old_err)
+    assert (_bits(b_dbl) != _bits(refB)).sum() > 50
+
+
+def test_lfit_matches_live_reference(oracle):
+    """The same comparison against the reference library itself on fresh
+    point sets (only where oracle/_ref is built)."""
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    X, Y = W.lfit_point_sets(20000, 77)
+    ref = oracle.ref_lfit(X, Y)
+    got = oracle.lfit_f32(X, Y)
+    for name, a, b in zip("ABE", got, ref):
+        np.testing.assert_array_equal(_bits(a), _bits(b), err_msg=name)
 
 
 def test_pion_absorption_law(oracle):
