@@ -154,10 +154,13 @@ def bench_batched(ctx, line, args):
     settings, off = [], 0
     for w in sets:
         W.generate_device(w, w.n, seed=w.seed, device=dev, out=ev, at=off)
+        # each setting is one SIMC run with its own seed (seed*1000 + s) and
+        # its events numbered from 0
         settings.append(dict(arm=w.arm, p_spec=w.p_spec, m2=w.m2, wire_sigma=w.wire_sigma, fit="fp64",
-                             seed=1234, event0=off, offset=off, count=w.n))
+                             seed=w.seed, event0=0, offset=off, count=w.n))
         off += w.n + gap
     b = api.Batch(ctx, settings)
+    n_passes, n_lane = b.passes()
     out = b.new_outputs(ev)
     chunk, tune_ms = b.tune(ev, out)
     # capture the launch sequence and time graph replays
@@ -191,7 +194,7 @@ def bench_batched(ctx, line, args):
         t0.record()
         for d in settings:
             sl = slice(d["offset"], d["offset"] + d["count"])
-            ctx.run(d["arm"], {k: v[sl] for k, v in ev.items()}, rng="philox", seed=1234,
+            ctx.run(d["arm"], {k: v[sl] for k, v in ev.items()}, rng="philox", seed=d["seed"],
                     event0=d["event0"], out={k: v[sl] for k, v in outs.items()},
                     **{k: d[k] for k in ("p_spec", "m2", "wire_sigma", "fit")})
         t1.record()
@@ -207,6 +210,7 @@ def bench_batched(ctx, line, args):
     r_shms = line["value"]
     single_mix = total / (n_hms / r_hms + n_shms / r_shms)
     return {"settings": len(sets), "events": total, "hms_events": n_hms, "shms_events": n_shms,
+            "passes": n_passes, "passes_keyed_per_setting": n_lane,
             "events_per_s": batch_rate, "ms": batch_ms, "hip_graph": True, "tuned_chunk": chunk,
             "tune_best_ms": tune_ms, "accepted_fraction": accepted / total,
             "per_setting_plain_events_per_s": total / (per_setting_ms * 1e-3),
@@ -215,7 +219,7 @@ def bench_batched(ctx, line, args):
             "single_setting_mix_events_per_s": single_mix,
             "vs_single_setting": batch_rate / single_mix,
             "data": "synthetic: 64 settings alternating HMS protons / SHMS electrons, p0 U(1,9) GeV/c, "
-                    "counts log-uniform in [1e4,1e7] scaled to 1.5e8 total"}
+                    "counts log-uniform in [1e4,1e7] scaled to 1.5e8 total, Philox seed 2026*1000+s per setting"}
 
 
 def bench_adjoint(ctx, n=10_000_000, reps=3):

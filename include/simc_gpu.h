@@ -187,8 +187,14 @@ int simc_arm_run_timed(simc_ctx *ctx, int arm, const simc_settings *s, const sim
  * simc_batch_run only launches kernels on `stream` (it may be captured into
  * a HIP graph).  Counters are kept per setting.  Philox streams only:
  * event j of a setting draws from stream (seed, event0 + j), so settings
- * sharing a seed need disjoint event-id ranges; settings are launched
- * together when they share arm, flags and seed. */
+ * sharing a seed need disjoint event-id ranges.  Settings are launched
+ * together (one pass) when they share arm, kernel variant and flags; when
+ * their seeds differ, each event's lane keys its own setting's stream, so a
+ * SIMC run's per-setting seed (simc.f:107, sgrnd) splits no pass.
+ * The batch owns its survivor queues (sized by the launch tile): runs of the
+ * context do not disturb a captured batch run.  simc_batch_set_chunk and
+ * simc_batch_tune may replace them, and simc_ctx_load_arm replaces the arm's
+ * tables: graphs captured before either call must be captured again. */
 typedef struct {
     int32_t arm;            /* SIMC_ARM_HMS, _SOS, _HRSR, _HRSL or _SHMS        */
     int32_t reserved;
@@ -215,6 +221,9 @@ int simc_batch_set_chunk(simc_batch *b, int64_t chunk);
 int simc_batch_tune(simc_batch *b, const simc_events_in *in, simc_events_out *out, void *stream,
                     int64_t *best_chunk, double *best_ms);
 void simc_batch_destroy(simc_batch *b);
+/* How simc_batch_create grouped the settings: number of passes, and how many
+ * of them key their lanes per setting (no counterpart in the reference). */
+int simc_batch_passes(simc_batch *b, int32_t *n_passes, int32_t *n_lane_key);
 
 /* ---- Adjoint form: gradients for optics fits ---------------------------
  *

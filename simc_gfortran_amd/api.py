@@ -65,7 +65,7 @@ EXPORTS = [
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
-    "simc_batch_tune", "simc_batch_destroy", "simc_adjoint_run", "simc_recon_coefficients",
+    "simc_batch_tune", "simc_batch_destroy", "simc_batch_passes", "simc_adjoint_run", "simc_recon_coefficients",
     "simc_target_prep", "simc_target_prep_host", "simc_gen_hep", "simc_gen_hep_host",
     "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
     "simc_track_from_tgt_host", "simc_track_to_tgt_host",
@@ -90,6 +90,8 @@ class EventsIn(C.Structure):
 
 OUT_FIELDS = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
               "resmult", "m2", "ok", "stop", "used")
+# the outputs a run always writes (resmult, m2 and used are optional)
+RUN_REQUIRED = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "ok", "stop")
 
 
 class EventsOut(C.Structure):
@@ -231,6 +233,7 @@ def library(path: str = LIB_PATH):
     L.simc_stop_report.argtypes = [_vp, C.c_char_p, C.c_int, C.c_int]
     L.simc_batch_destroy.argtypes = [_vp]
     L.simc_batch_destroy.restype = None
+    L.simc_batch_passes.argtypes = [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.simc_arm_run_host.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
                                     C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng)]
     L.simc_recon_run_host.argtypes = [_vp, C.c_int, C.c_int64] + [_vp] * 9
@@ -263,6 +266,40 @@ def _dev_f64(t, n, name):
     if t.numel() != n or not t.is_contiguous():
         raise ValueError(f"{name}: expected {n} contiguous elements")
     return t
+
+
+# dtype of every event output column (simc_events_out)
+OUT_DTYPES = {k: torch.float64 for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp",
+                                          "pathlen", "resmult", "m2")}
+OUT_DTYPES.update(ok=torch.uint8, stop=torch.uint8, used=torch.int32)
+
+
+def _dev_tensor(t, n, name, dtype, at_least=False):
+    """A device tensor of `dtype` with n contiguous elements (or at least n):
+    what a kernel may address through its pointer."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dtype:
+        raise TypeError(f"{name}: expected a {dtype} tensor on a HIP device")
+    if not t.is_contiguous() or (t.numel() < n if at_least else t.numel() != n):
+        raise ValueError(f"{name}: expected {'at least ' if at_least else ''}{n} contiguous elements")
+    return t
+
+
+def _check_outputs(out, n, required, at_least=False):
+    """Validate caller-supplied event outputs: every present column has its
+    dtype, lives on a HIP device and holds n elements; `required` ones must
+    be present."""
+    for k in required:
+        if out.get(k) is None:
+            raise ValueError(f"out: column {k} is required")
+    for k, dt in OUT_DTYPES.items():
+        if out.get(k) is not None:
+            _dev_tensor(out[k], n, "out[" + k + "]", dt, at_least)
+
+
+def _check_tape(tape, n, stride):
+    if stride <= 0:
+        raise ValueError("tape rng needs a positive stride")
+    _dev_tensor(tape, n * int(stride), "tape", torch.float64, at_least=True)
 
 
 class Spectrometers:
@@ -319,13 +356,16 @@ class Spectrometers:
             out["ok"] = torch.zeros(n, dtype=torch.uint8, device=dev)
             out["stop"] = torch.zeros(n, dtype=torch.uint8, device=dev)
             out["used"] = torch.zeros(n, dtype=torch.int32, device=dev)
+        else:
+            _check_outputs(out, n, RUN_REQUIRED)
         S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
                      FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
         ein = EventsIn(*[_ptr(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
         eout = EventsOut(*[_ptr(out.get(k)) for k in OUT_FIELDS])
         if rng == "tape":
-            if tape is None or stride <= 0:
+            if tape is None:
                 raise ValueError("tape rng needs tape and stride")
+            _check_tape(tape, n, stride)
             R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
         else:
             R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
@@ -423,6 +463,10 @@ class Spectrometers:
         if out is None:
             out = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in PREP_OUT_FIELDS if k != "used"}
             out["used"] = torch.empty(n, dtype=torch.int32, device=dev)
+        else:
+            for k in PREP_OUT_FIELDS:
+                if out.get(k) is not None:
+                    _dev_tensor(out[k], n, "out[" + k + "]", torch.int32 if k == "used" else torch.float64)
         V = VertexIn(*[_ptr(vin.get(k)) for k in VERTEX_FIELDS])
         O = PrepOut(*[_ptr(out.get(k)) for k in PREP_OUT_FIELDS])
         if rng == "tape":
@@ -811,13 +855,13 @@ class Batch:
 
     def _structs(self, events, out):
         n = self.n_arr
-        ev = {k: _dev_f64(events[k], events[k].numel(), k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
-        if any(t.numel() < n for t in ev.values()):
-            raise ValueError(f"event arrays must hold at least {n} elements")
+        ev = {k: _dev_tensor(events[k], n, k, torch.float64, at_least=True)
+              for k in ("dpp", "x", "y", "dxdz", "dydz")}
         fry = events.get("fry")
-        ein = EventsIn(*[_ptr(ev.get(k) if k not in ("z", "fry") else None) for k in
-                         ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
-        ein.fry = _ptr(fry)
+        if fry is not None:
+            ev["fry"] = _dev_tensor(fry, n, "fry", torch.float64, at_least=True)
+        _check_outputs(out, n, RUN_REQUIRED, at_least=True)
+        ein = EventsIn(*[_ptr(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
         eout = EventsOut(*[_ptr(out.get(k)) for k in OUT_FIELDS])
         return ein, eout
 
@@ -846,6 +890,12 @@ class Batch:
 
     def set_chunk(self, chunk):
         _check(library().simc_batch_set_chunk(self._h, int(chunk)))
+
+    def passes(self):
+        """(number of passes, passes whose lanes key their own setting's stream)."""
+        n, k = C.c_int32(), C.c_int32()
+        _check(library().simc_batch_passes(self._h, C.byref(n), C.byref(k)))
+        return n.value, k.value
 
     def tune(self, events, out, stream=None):
         """Autotune the launch tile; returns (chunk, ms)."""

@@ -61,15 +61,18 @@ constexpr double kMk = 493.677;
 constexpr uint32_t kPhiloxTag = 0x53494d43u;  // "SIMC"
 constexpr uint32_t kTargetTag = 0x54475453u;  // "TGTS"
 
+template <bool LANE_KEY = false>
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4])
 {
-    // The key is launch-uniform; without this empty statement the compiler
-    // forms the 20 round keys once per kernel and keeps them in SGPRs across
-    // the whole step loop, where they are spilled to VGPR lanes and read back
+    // A launch-uniform key: without this empty statement the compiler forms
+    // the 20 round keys once per kernel and keeps them in SGPRs across the
+    // whole step loop, where they are spilled to VGPR lanes and read back
     // (v_readlane) in every block.  Made opaque here, they are recomputed by
-    // scalar adds in each call instead.
-    asm("" : "+s"(k0), "+s"(k1));
+    // scalar adds in each call instead.  A per-lane key (events of several
+    // settings in one wavefront) is kept opaque in VGPRs for the same reason.
+    if (LANE_KEY) asm("" : "+v"(k0), "+v"(k1));
+    else asm("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
@@ -161,8 +164,10 @@ __device__ __forceinline__ void bm_sincos2pi(double u, double& sn, double& cs)
 
 // TAG is the fourth counter word: streams of different tags are independent
 // (the arms use kPhiloxTag, the target step kTargetTag).
-template <uint32_t TAG>
+template <uint32_t TAG, bool LANE_KEY = false>
 struct PhiloxStreamT {
+    // LANE_KEY: the key (seed) may differ between the lanes of a wavefront
+    static constexpr bool kLaneKey = LANE_KEY;
     uint32_t k0, k1, e0, e1;
     uint32_t pos;
     uint32_t w[4];
@@ -185,12 +190,12 @@ struct PhiloxStreamT {
         pos = word & 0x7fffffffu;
         have = (int)(word >> 31);
         pending = pend;
-        if (pos & 3u) philox4x32_10(pos >> 2, e0, e1, TAG, k0, k1, w);
+        if (pos & 3u) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
     }
     __device__ __forceinline__ double next()
     {
         const uint32_t j = pos & 3u;
-        if (j == 0) philox4x32_10(pos >> 2, e0, e1, TAG, k0, k1, w);
+        if (j == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
         ++pos;
         const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
         return ((double)v + 0.5) * 2.3283064365386963e-10;
@@ -242,6 +247,7 @@ struct PhiloxStreamT {
     __device__ __forceinline__ int32_t used() const { return (int32_t)pos; }
 };
 using PhiloxStream = PhiloxStreamT<kPhiloxTag>;
+using PhiloxLaneStream = PhiloxStreamT<kPhiloxTag, true>;
 
 template <class R>
 __device__ __forceinline__ double polar_gauss(R& g, double nsigmax);
@@ -249,6 +255,7 @@ __device__ __forceinline__ double polar_gauss(R& g, double nsigmax);
 // Tape of the launch: event slot at byte offset `base`, `stride` bytes long.
 // Normal deviates use the reference's Marsaglia polar method (gauss1.f).
 struct TapeStream {
+    static constexpr bool kLaneKey = false;
     const double* tape;
     uint32_t base, stride, pos;
     int overflow;

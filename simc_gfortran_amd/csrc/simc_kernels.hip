@@ -96,7 +96,8 @@ struct KArgs {
     const SetDev* set;    // n_set settings sorted by start, or nullptr
     uint32_t n_set;
     int32_t flags;        // KF_* bits of every setting of the launch (wave-uniform control flow)
-    uint64_t seed;        // Philox key of every setting of the launch (uniform round keys)
+    uint64_t seed;        // Philox key of every setting of the launch (uniform round keys); a
+                          // launch of settings with different keys reads each setting's own
     uint32_t count;       // events in this launch: pass indices pass_base + [0, count)
     uint64_t pass_base;
     uint64_t ix_base;     // element of the event arrays that the base pointers address
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             s = bits >> 1;
             const SetDev S = setting_of(a, s);
             const EvId e = event_id(a, S, s, li);
-            g.init(a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
+            g.init(R::kLaneKey ? S.seed : a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
             g.resume(ld32(qin.rng, c, o4), ld64(qin.pend, c, o8));
             t.xs = ld64(qin.xs, c, o8);
             t.ys = ld64(qin.ys, c, o8);
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
         const Flags F = flags_of(a, S);
         if (!FROM_Q) {
             const uint32_t n = a.n_arr, off = e.off;
-            g.init(a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
+            g.init(R::kLaneKey ? S.seed : a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
             t.xs = ld64(a.x, n, off);
             t.ys = ld64(a.y, n, off);
             t.dxdzs = ld64(a.dxdz, n, off);
@@ -1495,6 +1496,14 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 constexpr int kNumArms = 5;  // HMS, SHMS, SOS, HRS-R, HRS-L (arm_index)
 
+// The two survivor queues of one arm and their counters (device memory,
+// grown on demand; the owner frees them).
+struct QueuePair {
+    void* d_queue[2] = {};
+    uint32_t* d_qcount = nullptr;   // the two queue counters
+    uint32_t cap = 0;
+};
+
 struct simc_ctx {
     simc::ForwardMap fwd[kNumArms];
     simc::ReconMap rec[kNumArms];
@@ -1507,10 +1516,8 @@ struct simc_ctx {
     unsigned long long* d_hist[kNumArms] = {};
     int coll_pc[kNumArms] = {};      // first step after the collimator
     int hut_pc[kNumArms] = {};       // first step of the detector hut
-    // survivor queues between the stages (per arm; grown on demand)
-    void* d_queue[kNumArms][2] = {};
-    uint32_t* d_qcount[kNumArms] = {};   // the two queue counters
-    uint32_t q_cap[kNumArms] = {};
+    // survivor queues between the stages of plain runs (per arm)
+    QueuePair qp[kNumArms];
     std::mutex mu;                   // guards queue growth
     // reconstruction exponents per arm (adjoint form), [n_terms][5]
     uint8_t* d_rexp[kNumArms] = {};
@@ -1534,11 +1541,16 @@ struct simc_batch {
         int arm;
         bool rare;
         int32_t flags;      // KF_* bits shared by the pass's settings
-        uint64_t seed;      // Philox key shared by the pass's settings
+        uint64_t seed;      // Philox key of the pass's first setting
+        bool lane_key;      // its settings' keys differ: each lane reads its setting's key
         uint64_t total;     // events of the pass
         uint32_t first, n;  // its settings in d_set
     };
     simc_ctx* ctx = nullptr;
+    // the batch's own survivor queues (per arm), sized for its launch tile:
+    // a captured run keeps addressing them while the context's plain runs
+    // grow or replace theirs
+    QueuePair qp[kNumArms];
     int n_settings = 0;
     std::vector<int> arm_of;               // arm of each setting (input order)
     std::vector<Pass> passes;
@@ -2262,25 +2274,40 @@ void queue_view(void* base, uint32_t c, uint32_t* count, Queue& q)
     q.cap = c;
 }
 
+// Queue views of a pair that holds at least `cap` entries per queue (grown
+// here; growing waits for the device, whose earlier launches may use them).
+int ensure_pair(QueuePair& qp, uint32_t cap, Queue& q1, Queue& q2)
+{
+    if (!qp.d_qcount) HIPCHK(hipMalloc(&qp.d_qcount, 256));
+    if (qp.cap < cap) {
+        if (qp.d_queue[0] || qp.d_queue[1]) {
+            HIPCHK(hipDeviceSynchronize());
+            for (int j = 0; j < 2; ++j) {
+                if (qp.d_queue[j]) HIPCHK(hipFree(qp.d_queue[j]));
+                qp.d_queue[j] = nullptr;
+            }
+            qp.cap = 0;
+        }
+        for (int j = 0; j < 2; ++j) HIPCHK(hipMalloc(&qp.d_queue[j], (size_t)cap * kQueueEntryBytes));
+        qp.cap = cap;
+    }
+    queue_view(qp.d_queue[0], qp.cap, qp.d_qcount, q1);
+    queue_view(qp.d_queue[1], qp.cap, qp.d_qcount + 1, q2);
+    return SIMC_OK;
+}
+
+void free_pair(QueuePair& qp)
+{
+    for (int j = 0; j < 2; ++j)
+        if (qp.d_queue[j]) (void)hipFree(qp.d_queue[j]);
+    if (qp.d_qcount) (void)hipFree(qp.d_qcount);
+    qp = QueuePair();
+}
+
 int ensure_queues(simc_ctx* ctx, int ai, uint32_t cap, Queue& q1, Queue& q2)
 {
     std::lock_guard<std::mutex> lock(ctx->mu);
-    if (!ctx->d_qcount[ai]) HIPCHK(hipMalloc(&ctx->d_qcount[ai], 256));
-    if (ctx->q_cap[ai] < cap) {
-        if (ctx->d_queue[ai][0] || ctx->d_queue[ai][1]) {
-            HIPCHK(hipDeviceSynchronize());  // earlier launches may still use them
-            for (int j = 0; j < 2; ++j) {
-                if (ctx->d_queue[ai][j]) HIPCHK(hipFree(ctx->d_queue[ai][j]));
-                ctx->d_queue[ai][j] = nullptr;
-            }
-            ctx->q_cap[ai] = 0;
-        }
-        for (int j = 0; j < 2; ++j) HIPCHK(hipMalloc(&ctx->d_queue[ai][j], (size_t)cap * kQueueEntryBytes));
-        ctx->q_cap[ai] = cap;
-    }
-    queue_view(ctx->d_queue[ai][0], ctx->q_cap[ai], ctx->d_qcount[ai], q1);
-    queue_view(ctx->d_queue[ai][1], ctx->q_cap[ai], ctx->d_qcount[ai] + 1, q2);
-    return SIMC_OK;
+    return ensure_pair(ctx->qp[ai], cap, q1, q2);
 }
 
 }  // namespace
@@ -2461,9 +2488,7 @@ void simc_ctx_destroy(simc_ctx* ctx)
         if (ctx->d_ops[i]) (void)hipFree(ctx->d_ops[i]);
         if (ctx->d_tab[i]) (void)hipFree(ctx->d_tab[i]);
         if (ctx->d_hist[i]) (void)hipFree(ctx->d_hist[i]);
-        for (int j = 0; j < 2; ++j)
-            if (ctx->d_queue[i][j]) (void)hipFree(ctx->d_queue[i][j]);
-        if (ctx->d_qcount[i]) (void)hipFree(ctx->d_qcount[i]);
+        free_pair(ctx->qp[i]);
     }
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     for (int i = 0; i < kNumArms; ++i)
@@ -2531,7 +2556,7 @@ bool outputs_complete(const simc_events_in* in, const simc_events_out* out)
 // One launch chunk: counter reset and the three stages on `st`; `ev` (four
 // events or nullptr) marks the stage boundaries.
 int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, const Queue& q1, const Queue& q2,
-                  hipStream_t st, hipEvent_t* ev)
+                  hipStream_t st, hipEvent_t* ev, bool lane_key = false)
 {
     const int ai = arm_index(arm);
     const int32_t coll = ctx->coll_pc[ai], hut = ctx->hut_pc[ai];
@@ -2551,15 +2576,17 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
         if (ev) HIPCHK(hipEventRecord(ev[3], st));                                                          \
     } while (0)
 #define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
-    if (arm == SIMC_ARM_HMS) {
-        if (tape) LAUNCH1(SIMC_ARM_HMS, TapeStream); else LAUNCH1(SIMC_ARM_HMS, PhiloxStream);
-    } else if (arm == SIMC_ARM_SOS) {
-        if (tape) LAUNCH1(SIMC_ARM_SOS, TapeStream); else LAUNCH1(SIMC_ARM_SOS, PhiloxStream);
-    } else if (is_hrs(arm)) {  // one kernel family for both HRS arms (their tables differ)
-        if (tape) LAUNCH1(SIMC_ARM_HRSR, TapeStream); else LAUNCH1(SIMC_ARM_HRSR, PhiloxStream);
-    } else {
-        if (tape) LAUNCH1(SIMC_ARM_SHMS, TapeStream); else LAUNCH1(SIMC_ARM_SHMS, PhiloxStream);
-    }
+#define LAUNCHR(ARMV)                                                                                       \
+    do {                                                                                                    \
+        if (tape) LAUNCH1(ARMV, TapeStream);                                                                \
+        else if (lane_key) LAUNCH1(ARMV, PhiloxLaneStream);                                                 \
+        else LAUNCH1(ARMV, PhiloxStream);                                                                   \
+    } while (0)
+    if (arm == SIMC_ARM_HMS) LAUNCHR(SIMC_ARM_HMS);
+    else if (arm == SIMC_ARM_SOS) LAUNCHR(SIMC_ARM_SOS);
+    else if (is_hrs(arm)) LAUNCHR(SIMC_ARM_HRSR);  // one kernel family for both HRS arms (their tables differ)
+    else LAUNCHR(SIMC_ARM_SHMS);
+#undef LAUNCHR
 #undef LAUNCH1
 #undef LAUNCH3
     HIPCHK(hipGetLastError());
@@ -3710,20 +3737,23 @@ int simc_batch_create(simc_ctx* ctx, const simc_setting* settings, int n_setting
     for (int i = 0; i < n_settings; ++i) b->arm_of.push_back(settings[i].arm);
     std::vector<SetDev> table;
     uint64_t max_total = 0;
-    // passes: one per (arm, kernel variant, flag set, seed), settings in input order
+    // passes: one per (arm, kernel variant, flag set), settings in input
+    // order; settings with different Philox keys share a pass and its lanes
+    // read their own setting's key
     std::vector<bool> placed(n_settings, false);
     for (int i0 = 0; i0 < n_settings; ++i0) {
         const simc_setting& q0 = settings[i0];
         if (placed[i0] || q0.count == 0) continue;
         const int32_t f0 = make_setdev(q0.arm, &q0.s, 0, 0, 0, 0, 0).flags;
         const bool r0 = needs_rare(q0.arm, &q0.s);
-        simc_batch::Pass pass{q0.arm, r0, f0, q0.seed, 0, (uint32_t)table.size(), 0};
+        simc_batch::Pass pass{q0.arm, r0, f0, q0.seed, false, 0, (uint32_t)table.size(), 0};
         for (int i = i0; i < n_settings; ++i) {
             const simc_setting& q = settings[i];
             if (placed[i] || q.count == 0 || q.arm != q0.arm || needs_rare(q.arm, &q.s) != r0 ||
-                make_setdev(q.arm, &q.s, 0, 0, 0, 0, 0).flags != f0 || q.seed != q0.seed)
+                make_setdev(q.arm, &q.s, 0, 0, 0, 0, 0).flags != f0)
                 continue;
             placed[i] = true;
+            if (q.seed != q0.seed) pass.lane_key = true;
             table.push_back(make_setdev(q.arm, &q.s, q.seed, q.event0, pass.total, (uint64_t)q.offset, (uint32_t)i));
             pass.total += (uint64_t)q.count;
             ++pass.n;
@@ -3762,7 +3792,7 @@ int simc_batch_set_chunk(simc_batch* b, int64_t chunk)
     Queue q1, q2;
     for (const auto& pass : b->passes) {
         const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)chunk, pass.total);
-        int rc = ensure_queues(b->ctx, arm_index(pass.arm), cap, q1, q2);
+        int rc = ensure_pair(b->qp[arm_index(pass.arm)], cap, q1, q2);
         if (rc) return rc;
     }
     b->chunk = chunk;
@@ -3779,11 +3809,12 @@ int simc_batch_run(simc_batch* b, const simc_events_in* in, simc_events_out* out
     hipLaunchKernelGGL(zero_u64, dim3((nh + 255) / 256), dim3(256), 0, st, b->d_hist, nh);
     for (const auto& pass : b->passes) {
         const int ai = arm_index(pass.arm);
-        if (ctx->q_cap[ai] < std::min<uint64_t>((uint64_t)b->chunk, pass.total))
+        const QueuePair& qp = b->qp[ai];
+        if (qp.cap < std::min<uint64_t>((uint64_t)b->chunk, pass.total))
             return fail(SIMC_E_INTERNAL, "simc_batch_run: queues smaller than the chunk");
         Queue q1, q2;
-        queue_view(ctx->d_queue[ai][0], ctx->q_cap[ai], ctx->d_qcount[ai], q1);
-        queue_view(ctx->d_queue[ai][1], ctx->q_cap[ai], ctx->d_qcount[ai] + 1, q2);
+        queue_view(qp.d_queue[0], qp.cap, qp.d_qcount, q1);
+        queue_view(qp.d_queue[1], qp.cap, qp.d_qcount + 1, q2);
         KArgs a;
         std::memset(&a, 0, sizeof a);
         a.set = (const SetDev*)b->d_set + pass.first;
@@ -3804,10 +3835,20 @@ int simc_batch_run(simc_batch* b, const simc_events_in* in, simc_events_out* out
         for (uint64_t c0 = 0; c0 < pass.total; c0 += (uint64_t)b->chunk) {
             a.pass_base = c0;
             a.count = (uint32_t)std::min<uint64_t>((uint64_t)b->chunk, pass.total - c0);
-            int rc = launch_stages(ctx, pass.arm, pass.rare, false, a, q1, q2, st, nullptr);
+            int rc = launch_stages(ctx, pass.arm, pass.rare, false, a, q1, q2, st, nullptr, pass.lane_key);
             if (rc) return rc;
         }
     }
+    return SIMC_OK;
+}
+
+int simc_batch_passes(simc_batch* b, int32_t* n_passes, int32_t* n_lane_key)
+{
+    if (!b) return fail(SIMC_E_ARG, "simc_batch_passes: NULL batch");
+    int32_t lk = 0;
+    for (const auto& pass : b->passes) lk += pass.lane_key ? 1 : 0;
+    if (n_passes) *n_passes = (int32_t)b->passes.size();
+    if (n_lane_key) *n_lane_key = lk;
     return SIMC_OK;
 }
 
@@ -3873,6 +3914,7 @@ void simc_batch_destroy(simc_batch* b)
     if (!b) return;
     if (b->d_set) (void)hipFree(b->d_set);
     if (b->d_hist) (void)hipFree(b->d_hist);
+    for (int i = 0; i < kNumArms; ++i) free_pair(b->qp[i]);
     delete b;
 }
 

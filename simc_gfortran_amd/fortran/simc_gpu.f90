@@ -461,6 +461,12 @@ module simc_gpu
       type(c_ptr), value :: b
     end subroutine
 
+    integer(c_int) function simc_batch_passes(b, n_passes, n_lane_key) bind(C, name="simc_batch_passes")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int32_t), intent(out) :: n_passes, n_lane_key
+    end function
+
     integer(c_int) function simc_counters_get(ctx, arm, c, nc) bind(C, name="simc_counters_get")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: ctx
