@@ -40,11 +40,35 @@ def cpu_model():
     return platform.processor()
 
 
+# Host cores a one-GPU job may use on the GPU hosts this runs on (their CPU
+# share per GPU); the machine's nproc is reported beside it.
+CPU_SHARE = 16
+
+
+def host_threads():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return min(avail, CPU_SHARE), avail
+
+
+def reference_calibration():
+    """Ratio reference rate / oracle rate measured where the reference tree is
+    (oracle/calibrate.py), or None."""
+    path = os.path.join(REPO, "profiles", "cpu_calibration_reference_vs_oracle.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path))
+
+
 def cpu_baseline(wl, seconds=12.0):
-    """The FP64 oracle over a bounded sample of the workload on the host cores."""
+    """The FP64 oracle over a bounded sample of the workload on the host
+    cores (all the job may use), and on one thread (the reference's execution
+    model: simc.f:169-351 loops over events one at a time)."""
     from oracle import oracle as O
     from simc_gfortran_amd import workloads as W
-    threads = min(os.cpu_count() or 1, 16)
+    threads, avail = host_threads()
     chunk = 200_000
     kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma,
               fit_f32=False)
@@ -56,15 +80,57 @@ def cpu_baseline(wl, seconds=12.0):
         t_used += time.perf_counter() - t0
         done += chunk
         k += 1
-    # one thread, the reference's execution model, on a smaller sample
     ev = W.generate(wl, 40_000, seed=wl.seed + 999)
     t0 = time.perf_counter()
     O.run(wl.arm, ev, rng="philox", seed=99, nthreads=1, **kw)
     t1 = time.perf_counter() - t0
-    return {"value": done / t_used, "unit": "events/s", "cores": threads, "kind": "port",
-            "sample": f"{done} events of the same workload ({k} x {chunk}), FP64 oracle "
-                      f"(oracle/simc_oracle.c), OpenMP over {threads} threads, {cpu_model()}",
-            "single_thread_events_per_s": 40_000 / t1}
+    out = {"value": done / t_used, "unit": "events/s", "cores": threads, "kind": "port",
+           "sample": f"{done} events of the same workload ({k} x {chunk}), FP64 oracle "
+                     f"(oracle/simc_oracle.c), OpenMP over {threads} threads",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": avail,
+           "cpu_share_per_gpu": CPU_SHARE,
+           "single_thread_events_per_s": 40_000 / t1}
+    cal = reference_calibration()
+    if cal:
+        r = cal["reference_over_oracle"]
+        out["reference_over_oracle"] = r
+        out["reference_estimate_events_per_s"] = out["value"] * r
+        out["reference_estimate_single_thread_events_per_s"] = out["single_thread_events_per_s"] * r
+        out["calibration"] = ("profiles/cpu_calibration_reference_vs_oracle.json: the flang-built reference "
+                              "timed against the oracle on identical events, one thread, " + cal["cpu"])
+    return out
+
+
+def cpu_c0(seconds=8.0):
+    """The HMS configuration of 1e5 events on the CPU: the oracle's target
+    step (trip_thru_target, target_musc, transform) and HMS transport +
+    reconstruction, on one thread and on the host cores."""
+    from oracle import oracle as O
+    from simc_gfortran_amd import workloads as W
+    wl = W.HMS_C0
+    n = 100_000
+    v = W.generate_vertex(wl, n, seed=wl.seed)
+    T = O.make_target(**W.LH2_CELL)
+    P = O.make_prep(3, spectrometer=1, p_spec=wl.p_spec, th_spec=math.radians(wl.theta_deg),
+                    phi=1.5 * math.pi, mass=wl.mass)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, fit_f32=False)
+    threads, _ = host_threads()
+    rates = {}
+    for nt in (1, threads):
+        m, t, k = 0, 0.0, 0
+        while t < seconds / 2 or k == 0:
+            sub = n if nt > 1 else 20_000
+            vv = {key: a[:sub] for key, a in v.items()}
+            t0 = time.perf_counter()
+            prep, rc = O.target_prep(T, P, vv, rng="philox", seed=1001, event0=m, nthreads=nt)
+            assert rc == 0
+            O.run(wl.arm, prep, rng="philox", seed=1001, event0=m, nthreads=nt, **kw)
+            t += time.perf_counter() - t0
+            m += sub
+            k += 1
+        rates[nt] = m / t
+    return {"workload": "hms_proton_3GeV_lh2_cell_target_step_plus_arm", "events": n,
+            "cpu_events_per_s_1_thread": rates[1], "cpu_events_per_s": rates[threads], "cores": threads}
 
 
 STAGE_NAMES = ["stage1_target_to_collimator", "stage2_magnets", "stage3_hut_fit_recon"]
@@ -262,6 +328,40 @@ def bench_adjoint(ctx, n=10_000_000, reps=3):
             "adjoint_events_per_s": n / (adj * 1e-3), "loss": float(res[0]),
             "gradients": int(coef.numel()) + 6,
             "data": "synthetic HMS workload, Philox streams replayed, mask = ok of the forward run"}
+
+
+def bench_c0(ctx, reps=20):
+    """The HMS configuration as specified: 1e5 protons at 3 GeV/c from the
+    10 cm LH2 cell through the target step and mc_hms + hut + reconstruction
+    (one launch sequence per step; at this size launch latency matters)."""
+    import torch
+    from simc_gfortran_amd import api, workloads as W
+    dev = torch.device("cuda:0")
+    wl = W.HMS_C0
+    n = 100_000
+    v = W.generate_vertex(wl, n, seed=wl.seed)
+    dv = {k: torch.from_numpy(a).to(dev) for k, a in v.items()}
+    T = api.Target(**{k: W.LH2_CELL[k] for k in ("A", "Z", "rho", "X0", "length", "angle", "zoffset",
+                                                   "can")}, reserved=0)
+    P = api.PrepSettings(3, 1, 1, 1, wl.p_spec, math.radians(wl.theta_deg), 1.5 * math.pi, wl.mass, 0.0, 0.0,
+                         (api.C.c_double * 5)())
+    prep = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in ("dpp", "x", "y", "z", "dxdz", "dydz",
+                                                                          "fry")}
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True)
+    out = None
+    ts = []
+    for r in range(reps + 2):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ctx.target_prep(T, P, dv, rng="philox", seed=1001, event0=r * n, out=prep)
+        out = ctx.run(wl.arm, prep, rng="philox", seed=1001, event0=r * n, fit="fp64", out=out, **kw)
+        b.record()
+        torch.cuda.synchronize()
+        if r >= 2:
+            ts.append(a.elapsed_time(b))
+    ms = float(np.mean(ts))
+    return {"workload": "hms_proton_3GeV_lh2_cell_1e5 (target step + arm)", "events": n, "ms": ms,
+            "events_per_s": n / (ms * 1e-3), "accepted_fraction": float(out["ok"].float().mean())}
 
 
 def bench_target(ctx, n=10_000_000, reps=5):
@@ -624,7 +724,7 @@ def main():
                    "p_spec_MeV": wl.p_spec, "theta_deg": wl.theta_deg,
                    "wire_sigma_cm": wl.wire_sigma, "ms": True, "wcs": True, "fit": "fp64",
                    "rng": "philox4x32-10 per event"},
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64_valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": names[dom], "kernel_ms": stage_ms[dom],
                      "algorithmic_flop_per_launch": stage_flops[dom]},
@@ -635,6 +735,7 @@ def main():
     if not args.no_forms:
         del d_ev, out
         torch.cuda.empty_cache()
+        line["c0"] = bench_c0(ctx)
         line["batched"] = bench_batched(ctx, line, args)
         line["adjoint"] = bench_adjoint(ctx)
         line["target_step"] = bench_target(ctx)
@@ -646,6 +747,13 @@ def main():
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = value / cb["value"]
+        if "reference_estimate_events_per_s" in cb:
+            line["gpu_over_reference_estimate"] = value / cb["reference_estimate_events_per_s"]
+        if "c0" in line:
+            c0 = cpu_c0()
+            line["c0"].update(c0)
+            line["c0"]["gpu_over_cpu"] = line["c0"]["events_per_s"] / c0["cpu_events_per_s"]
+            line["c0"]["gpu_over_cpu_1_thread"] = line["c0"]["events_per_s"] / c0["cpu_events_per_s_1_thread"]
     print(json.dumps(line))
 
 

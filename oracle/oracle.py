@@ -71,6 +71,16 @@ _ref = None
 _maps = None
 
 
+def default_threads():
+    """OpenMP threads for the oracle: the CPUs this process may use, at most
+    16 (a one-GPU job's share of the host on the GPU machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
 def build():
     """Compile liboracle.so (and oracle/_ref when the reference tree exists)."""
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
@@ -178,7 +188,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
         tape = np.ascontiguousarray(tape, dtype=np.float64)
         tp = _p(tape)
     if nthreads is None:
-        nthreads = os.cpu_count() or 1
+        nthreads = default_threads()
     rc = L.or_run(maps(), arm, C.byref(S), n, C.byref(ein), C.byref(eout), kind, tp,
                   int(stride), int(seed), int(event0), int(nthreads), _p(cnt, _i64p))
     out["rc"] = rc
@@ -343,7 +353,7 @@ def target_prep(target, prep, vertex, *, rng="philox", seed=0, event0=0, tape=No
     else:
         kind, tp = 1, None
     rc = _target_lib().or_target_prep(C.byref(target), C.byref(prep), n, C.byref(V), C.byref(O), kind, tp,
-                                      int(stride), int(seed), int(event0), int(nthreads or os.cpu_count()),
+                                      int(stride), int(seed), int(event0), int(nthreads or default_threads()),
                                       _p(used, _i32p))
     out["used"] = used
     return out, rc
@@ -426,7 +436,7 @@ def track_from_tgt(F, ev, mom, mass, spect, nthreads=0):
     ok = np.zeros(n, np.uint8)
     m = np.ascontiguousarray(mom, dtype=np.float64)
     _field_lib().or_track_from_tgt_run(F, n, *[_p(out[k]) for k in ("x", "y", "z", "dx", "dy")], _p(m), mass,
-                                       spect, _p(ok, _u8p), int(nthreads or os.cpu_count()))
+                                       spect, _p(ok, _u8p), int(nthreads or default_threads()))
     out["ok"] = ok
     return out
 
@@ -441,7 +451,7 @@ def track_to_tgt(F, arm, trk, first, frx, fry, mom, mass, ctheta, stheta, spect,
     f = [np.ascontiguousarray(v, dtype=np.float64) for v in (frx, fry, mom)]
     _field_lib().or_track_to_tgt_run(F, C.c_void_p(maps()), arm, n, *[_p(v) for v in a],
                                      *[_p(out[k]) for k in ("delta", "y", "dx", "dy")], *[_p(v) for v in f],
-                                     mass, ctheta, stheta, spect, _p(ok, _u8p), int(nthreads or os.cpu_count()))
+                                     mass, ctheta, stheta, spect, _p(ok, _u8p), int(nthreads or default_threads()))
     out["ok"] = ok
     return out
 
@@ -579,7 +589,7 @@ def gen_hep(target, hep, n, *, rng="philox", seed=0, event0=0, tape=None, stride
     else:
         kind, tp = 1, None
     rc = _gen_lib().or_gen_hep(C.byref(target), C.byref(hep), n, C.byref(O), kind, tp, int(stride), int(seed),
-                               int(event0), int(nthreads or os.cpu_count()), _p(used, _i32p))
+                               int(event0), int(nthreads or default_threads()), _p(used, _i32p))
     out["used"] = used
     return out, rc
 

@@ -201,7 +201,9 @@ def test_batch_queues_survive_context_runs(ctx):
         ctx.run(arm, big, p_spec=wl.p_spec, m2=wl.m2, rng="philox", seed=3)
     torch.cuda.synchronize()
     for k in ("dpp", "y", "dxdz", "dydz"):
-        out[k].copy_(events[k])
+        out[k].copy_(events[k])  # INOUT fields back to their inputs
+    for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen"):
+        out[k].zero_()           # pathlen accumulates
     torch.cuda.synchronize()
     g.replay()
     torch.cuda.synchronize()
