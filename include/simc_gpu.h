@@ -152,6 +152,20 @@ int simc_hrsl_run(simc_ctx *ctx, const simc_settings *s, const simc_events_in *i
 int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                  simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 
+/* simc_arm_run without any allocation (no counterpart in the reference: for
+ * callers that cannot allocate device memory, such as code under HIP graph
+ * capture or a tight serving loop).  The survivor queues live in a static
+ * device array of the library (about 100 MiB per device, held until the
+ * process ends, its address set up by simc_ctx_create); the call allocates
+ * nothing and only launches kernels on `stream`.  At most
+ * SIMC_NOALLOC_MAX_EVENTS events per call (larger n: SIMC_E_ARG).  Every call
+ * on one device shares that array: calls that overlap on one device (other
+ * streams, other contexts, other host threads) race and must be ordered by
+ * the caller.  Results and counters equal simc_arm_run's. */
+#define SIMC_NOALLOC_MAX_EVENTS 524288
+int simc_arm_run_noalloc(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
+                         simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
+
 /* simc_arm_run on HOST arrays (the same structs, host pointers; the tape too):
  * the library copies the passed arrays to device staging memory it owns,
  * runs on the default stream and copies every passed output back before

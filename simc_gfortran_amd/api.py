@@ -61,7 +61,8 @@ COUNTERS = {HMS: HMS_COUNTERS, SOS: SOS_COUNTERS, HRSR: HRS_COUNTERS, HRSL: HRS_
 # Every exported C entry point (include/simc_gpu.h).
 EXPORTS = [
     "simc_ctx_create", "simc_ctx_create_files", "simc_ctx_load_arm", "simc_ctx_destroy",
-    "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_hrsr_run", "simc_hrsl_run", "simc_arm_run", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
+    "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_hrsr_run", "simc_hrsl_run", "simc_arm_run",
+    "simc_arm_run_noalloc", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
@@ -196,6 +197,8 @@ def library(path: str = LIB_PATH):
     L.simc_ctx_destroy.restype = None
     L.simc_arm_run.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
                                C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp]
+    L.simc_arm_run_noalloc.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
+                                       C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp]
     L.simc_arm_run_timed.argtypes = [_vp, C.c_int, C.POINTER(Settings), C.POINTER(EventsIn),
                                      C.POINTER(EventsOut), C.c_int64, C.POINTER(Rng), _vp,
                                      C.POINTER(_vp)]

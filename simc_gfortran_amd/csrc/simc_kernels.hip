@@ -2312,6 +2312,55 @@ int ensure_queues(simc_ctx* ctx, int ai, uint32_t cap, Queue& q1, Queue& q2)
 
 }  // namespace
 
+// Scratch of the allocation-free entry point (simc_arm_run_noalloc): the two
+// survivor queues of one launch of up to SIMC_NOALLOC_MAX_EVENTS events, after
+// 256 bytes that hold their two counters.  Each device holds its own copy for
+// the life of the process; every call on that device shares it.
+constexpr size_t kScratchQueueBytes = (size_t)SIMC_NOALLOC_MAX_EVENTS * (11 * 8 + 3 * 4);
+static_assert(kScratchQueueBytes % 256 == 0, "queue arrays stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[256 + 2 * kScratchQueueBytes];
+
+namespace {
+
+static_assert(kScratchQueueBytes == (size_t)SIMC_NOALLOC_MAX_EVENTS * kQueueEntryBytes, "queue entry layout");
+
+// Device address of `scratch` per device, taken once with the device current
+// (the array's name in host code is a stand-in, not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(SIMC_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SIMC_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SIMC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return SIMC_OK;
+}
+
+// Queue views of the scratch array of `device` (scratch_init has run).
+int scratch_queues(int device, Queue& q1, Queue& q2)
+{
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (device >= 0 && device < kMaxDevices) p = g_scratch_ptr[device];
+    }
+    if (!p) return fail(SIMC_E_INTERNAL, "scratch address not set up for this device");
+    char* base = (char*)p;
+    uint32_t* counts = (uint32_t*)base;
+    queue_view(base + 256, SIMC_NOALLOC_MAX_EVENTS, counts, q1);
+    queue_view(base + 256 + kScratchQueueBytes, SIMC_NOALLOC_MAX_EVENTS, counts + 1, q2);
+    return SIMC_OK;
+}
+
+}  // namespace
+
 namespace {
 
 // Stop-code histogram (kHistSlots bins) -> the STOP counters of the arm's
@@ -2421,7 +2470,8 @@ int simc_ctx_create_files(const char* hms_forward, const char* hms_recon, const 
         delete ctx;
         return fail(SIMC_E_HIP, "cannot query the current HIP device");
     }
-    int rc = load_arm(ctx, SIMC_ARM_HMS, hms_forward, hms_recon);
+    int rc = scratch_init(ctx->device);
+    if (!rc) rc = load_arm(ctx, SIMC_ARM_HMS, hms_forward, hms_recon);
     if (!rc) rc = load_arm(ctx, SIMC_ARM_SHMS, shms_forward, shms_recon);
     if (rc) {
         std::string keep = g_err;
@@ -2593,8 +2643,10 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
     return SIMC_OK;
 }
 
+// fixed: the queue pair to use (room for n events), or nullptr for the
+// context's queues of the arm (grown here as needed).
 int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in, simc_events_out* out,
-            int64_t n, const simc_rng* rng, void* stream, void* const* stage_events)
+            int64_t n, const simc_rng* rng, void* stream, void* const* stage_events, const Queue* fixed = nullptr)
 {
     if (!ctx || !s || !in || !out || !rng) return fail(SIMC_E_ARG, "simc_arm_run: NULL argument");
     const int ai = arm_index(arm);
@@ -2631,8 +2683,15 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
         a.stride_bytes = (uint32_t)sb;
     }
     Queue q1, q2;
-    rc = ensure_queues(ctx, ai, (uint32_t)(n < chunk ? n : chunk), q1, q2);
-    if (rc) return rc;
+    if (fixed) {
+        if ((int64_t)fixed[0].cap < std::min<int64_t>(n, chunk))
+            return fail(SIMC_E_INTERNAL, "arm_run: fixed queues smaller than the launch");
+        q1 = fixed[0];
+        q2 = fixed[1];
+    } else {
+        rc = ensure_queues(ctx, ai, (uint32_t)(n < chunk ? n : chunk), q1, q2);
+        if (rc) return rc;
+    }
     hipStream_t st = (hipStream_t)stream;
     for (int64_t b = 0; b < n; b += chunk) {
         const int64_t cnt = n - b < chunk ? n - b : chunk;
@@ -2667,6 +2726,19 @@ int simc_arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_even
                  simc_events_out* out, int64_t n, const simc_rng* rng, void* stream)
 {
     return arm_run(ctx, arm, s, in, out, n, rng, stream, nullptr);
+}
+
+int simc_arm_run_noalloc(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,
+                         simc_events_out* out, int64_t n, const simc_rng* rng, void* stream)
+{
+    if (!ctx) return fail(SIMC_E_ARG, "simc_arm_run_noalloc: NULL context");
+    if (n > (int64_t)SIMC_NOALLOC_MAX_EVENTS)
+        return fail(SIMC_E_ARG, "simc_arm_run_noalloc: " + std::to_string(n) + " events exceed the scratch capacity of " +
+                                    std::to_string(SIMC_NOALLOC_MAX_EVENTS) + " events per call");
+    Queue q[2];
+    int rc = scratch_queues(ctx->device, q[0], q[1]);
+    if (rc) return rc;
+    return arm_run(ctx, arm, s, in, out, n, rng, stream, nullptr, q);
 }
 
 int simc_arm_run_timed(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in* in,

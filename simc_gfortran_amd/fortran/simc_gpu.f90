@@ -277,6 +277,19 @@ module simc_gpu
       type(simc_rng), intent(in) :: rng
     end function
 
+    ! allocation-free variant: at most SIMC_NOALLOC_MAX_EVENTS (524288) events per call
+    integer(c_int) function simc_arm_run_noalloc(ctx, arm, s, evin, evout, n, rng, stream) &
+        bind(C, name="simc_arm_run_noalloc")
+      import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: arm
+      type(simc_settings), intent(in) :: s
+      type(simc_events_in), intent(in) :: evin
+      type(simc_events_out), intent(in) :: evout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
     integer(c_int) function simc_hms_run(ctx, s, evin, evout, n, rng, stream) bind(C, name="simc_hms_run")
       import :: c_int, c_int64_t, c_ptr, simc_settings, simc_events_in, simc_events_out, simc_rng
       type(c_ptr), value :: ctx, stream
