@@ -701,3 +701,128 @@ def ref_gen_hep(target, hep, n, tape, stride, earm=5, harm=1):
     res["ok"] = ok.astype(np.uint8)
     res["used"] = used
     return res, rc
+
+
+# ---- radiative tails (radc.f, brem.f) --------------------------------------------------
+class ORRad(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("rad_flag", "extrad_flag", "intcor_mode", "use_expon",
+                                       "use_offshell_rad")] + \
+        [("doing_tail", C.c_int * 3), ("e_arm", C.c_int), ("p_arm", C.c_int)] + \
+        [(k, C.c_double) for k in ("dE_edge_test", "Egamma_gen_max")] + \
+        [(k, C.c_double * 2) for k in ("edge_e_E", "edge_p_E", "edge_Em", "vertex_Trec")] + \
+        [(k, C.c_double) for k in ("Egamma_tot_max", "etatzai", "targ_A")]
+
+
+RAD_OUT_FIELDS = ("Egamma1", "Egamma2", "Egamma3", "gen_weight", "orig_Ein", "orig_e_E", "orig_e_delta",
+                  "orig_p_E", "orig_p_P", "orig_p_delta", "eloss2", "teff2", "eloss3", "teff3", "hardcorfac")
+
+
+class ORRadOut(C.Structure):
+    _fields_ = [(k, _dp) for k in RAD_OUT_FIELDS] + [("ntail", _i32p)]
+
+
+def make_rad(**kw):
+    """Radiative settings (the or_rad fields; pairs as (min, max), doing_tail
+    as three flags)."""
+    r = ORRad()
+    for k, v in kw.items():
+        if k == "doing_tail":
+            for i in range(3):
+                r.doing_tail[i] = int(v[i])
+        elif k in ("edge_e_E", "edge_p_E", "edge_Em", "vertex_Trec"):
+            getattr(r, k)[0], getattr(r, k)[1] = v
+        elif k in ("rad_flag", "extrad_flag", "intcor_mode", "use_expon", "use_offshell_rad", "e_arm", "p_arm"):
+            setattr(r, k, int(v))
+        else:
+            setattr(r, k, float(v))
+    return r
+
+
+def _rad_lib():
+    L = _gen_lib()
+    if not hasattr(L, "_rad_ready"):
+        L.or_gen_hep_rad.restype = C.c_int
+        L.or_gen_hep_rad.argtypes = [C.POINTER(ORTarget), C.POINTER(ORHep), C.POINTER(ORRad), C.c_int64,
+                                     C.POINTER(ORHepOut), C.POINTER(ORRadOut), C.c_int, _dp, C.c_int64,
+                                     C.c_uint64, C.c_uint64, C.c_int, _i32p]
+        L.or_rad_etatzai.restype = C.c_double
+        L.or_rad_etatzai.argtypes = [C.c_double]
+        L.or_rad_gamma.restype = C.c_double
+        L.or_rad_gamma.argtypes = [C.c_double]
+        L.or_lambda_dave.restype = C.c_double
+        L.or_lambda_dave.argtypes = [C.c_int] * 3 + [C.c_double] * 5
+        d = C.POINTER(C.c_double)
+        L.or_brem.restype = C.c_double
+        L.or_brem.argtypes = [C.POINTER(ORRad)] + [C.c_double] * 3 + [C.c_int, d, d, d]
+        L.or_bremos.restype = C.c_double
+        L.or_bremos.argtypes = [C.POINTER(ORRad)] + [C.c_double] * 14 + [C.c_int, d, d, d]
+        L._rad_ready = True
+    return L
+
+
+def rad_etatzai(Z):
+    return _rad_lib().or_rad_etatzai(float(Z))
+
+
+def gen_hep_rad(target, hep, rad, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0, nthreads=0):
+    """The oracle's H(e,e'p) generator with radiative tails for n events.
+    Returns (dict of HEP_OUT_FIELDS + RAD_OUT_FIELDS + 'ntail', 'ok', 'used',
+    return code)."""
+    out = {k: np.zeros(n) for k in HEP_OUT_FIELDS + RAD_OUT_FIELDS}
+    out["ok"] = np.zeros(n, np.uint8)
+    out["ntail"] = np.zeros(n, np.int32)
+    used = np.zeros(n, np.int32)
+    O = ORHepOut(*([_p(out[k]) for k in HEP_OUT_FIELDS] + [_p(out["ok"], _u8p)]))
+    RO = ORRadOut(*([_p(out[k]) for k in RAD_OUT_FIELDS] + [_p(out["ntail"], _i32p)]))
+    if rng == "tape":
+        tape = np.ascontiguousarray(tape, dtype=np.float64)
+        kind, tp = 0, _p(tape)
+    else:
+        kind, tp = 1, None
+    rc = _rad_lib().or_gen_hep_rad(C.byref(target), C.byref(hep), C.byref(rad), n, C.byref(O), C.byref(RO), kind,
+                                   tp, int(stride), int(seed), int(event0), int(nthreads or default_threads()),
+                                   _p(used, _i32p))
+    out["used"] = used
+    return out, rc
+
+
+def ref_gen_hep_rad(target, hep, rad, n, tape, stride):
+    """The reference's generate with using_rad (generate_rad, radc.f) +
+    complete_main from oracle/_ref on per-event tape slots.  Returns (dict of
+    HEP_OUT_FIELDS + RAD_OUT_FIELDS + 'ntail', 'etatzai', 'ok', 'used',
+    return code)."""
+    L = ref_lib()
+    if L is None:
+        raise FileNotFoundError(REF_LIB)
+    if not hasattr(L, "_gen_rad_ready"):
+        L.ref_gen_hep_rad.restype = C.c_int
+        L.ref_gen_hep_rad.argtypes = [_dp, C.c_int32, C.c_int32, C.c_int32, _dp, _i32p, _dp, _i32p, C.c_int64, _dp,
+                                      C.c_int64, _dp, _i32p, _i32p]
+        L._gen_rad_ready = True
+    t = np.array([target.A, target.Z, target.rho, target.X0, target.length, target.angle, target.zoffset])
+    par = np.array([hep.Ebeam, hep.dEbeam, hep.xwid, hep.ywid, hep.xoffset, hep.yoffset, hep.fr1, hep.fr2,
+                    hep.coulomb, hep.e_yptar[0], hep.e_yptar[1], hep.e_xptar[0], hep.e_xptar[1], hep.e_theta,
+                    hep.e_phi, hep.e_P, hep.p_theta, hep.p_phi, hep.p_P, hep.Mh])
+    ipar = np.array([hep.fr_pattern, hep.using_eloss, hep.using_coulomb], np.int32)
+    rpar = np.array([rad.dE_edge_test, rad.Egamma_gen_max, rad.edge_e_E[0], rad.edge_e_E[1], rad.edge_p_E[0],
+                     rad.edge_p_E[1], rad.edge_Em[0], rad.edge_Em[1], rad.vertex_Trec[0], rad.vertex_Trec[1],
+                     rad.Egamma_tot_max])
+    irpar = np.array([rad.rad_flag, rad.extrad_flag, rad.intcor_mode, rad.use_expon, rad.use_offshell_rad,
+                      rad.doing_tail[0], rad.doing_tail[1], rad.doing_tail[2]], np.int32)
+    tape = np.ascontiguousarray(tape, dtype=np.float64)
+    out = np.zeros((n, 44))
+    ok = np.zeros(n, np.int32)
+    used = np.zeros(n, np.int32)
+    rc = L.ref_gen_hep_rad(_p(t), int(target.can), int(rad.e_arm), int(rad.p_arm), _p(par), _p(ipar, _i32p),
+                           _p(rpar), _p(irpar, _i32p), n, _p(tape), int(stride), _p(out), _p(ok, _i32p),
+                           _p(used, _i32p))
+    res = {k: out[:, j].copy() for j, k in enumerate(HEP_OUT_FIELDS)}
+    for j, k in enumerate(("Egamma1", "Egamma2", "Egamma3")):
+        res[k] = out[:, 27 + j].copy()
+    res["ntail"] = out[:, 30].astype(np.int32)
+    for j, k in enumerate(("gen_weight", "orig_Ein", "orig_e_E", "orig_e_delta", "orig_p_E", "orig_p_P",
+                           "orig_p_delta", "eloss2", "teff2", "eloss3", "teff3", "hardcorfac", "etatzai")):
+        res[k] = out[:, 31 + j].copy()
+    res["ok"] = ok.astype(np.uint8)
+    res["used"] = used
+    return res, rc

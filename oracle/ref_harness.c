@@ -295,3 +295,86 @@ int ref_gen_hep(const double *target, int32_t can, int32_t earm, int32_t harm, c
     }
     return rc;
 }
+
+/* ---- radiative tails: the reference's generate with using_rad ---- */
+void ref_gen_rad_one_(const double *par, const int32_t *ipar, const double *rpar, const int32_t *irpar,
+                      double *out, int32_t *ok);
+
+/* n events of H(e,e'p) elastic generation with radiation (generate_rad),
+ * one tape slot each; out: 44 values per event (ref_gen.f). */
+int ref_gen_hep_rad(const double *target, int32_t can, int32_t earm, int32_t harm, const double *par,
+                    const int32_t *ipar, const double *rpar, const int32_t *irpar, int64_t n,
+                    const double *tape, int64_t stride, double *out, int32_t *ok, int32_t *used)
+{
+    double a = target[0], z = target[1], rho = target[2], x0 = target[3], len = target[4];
+    double ang = target[5], zoff = target[6];
+    ref_set_target_(&a, &z, &rho, &x0, &len, &ang, &zoff, &can, &earm, &harm);
+    int rc = 0;
+    for (int64_t i = 0; i < n; i++) {
+        g_tape = tape + i * stride;
+        g_stride = stride;
+        g_pos = 0;
+        g_used = 0;
+        g_overflow = 0;
+        ref_gen_rad_one_(par, ipar, rpar, irpar, out + 44 * i, ok + i);
+        used[i] = (int32_t)g_used;
+        if (g_overflow) rc = -2;
+    }
+    return rc;
+}
+
+/* the reference's brem, bremos, gamma and lambda_dave on given arguments */
+extern void ref_rad_setflags_(int32_t *iexpon);
+extern double brem_(double *ein, double *eout, double *egamma, int32_t *radiate_proton, double *bsoft,
+                    double *bhard, double *dbsoft);
+extern double bremos_(double *egamma, double *kix, double *kiy, double *kiz, double *kfx, double *kfy,
+                      double *kfz, double *pix, double *piy, double *piz, double *pfx, double *pfy,
+                      double *pfz, double *pfe, int32_t *radiate_proton, double *bsoft, double *bhard,
+                      double *dbsoft);
+extern double gamma_(double *x);
+extern double lambda_dave_(int32_t *itail, int32_t *plus_flag, int32_t *doing_proton, double *e1,
+                           double *e2, double *e3, double *p3, double *th);
+
+/* out: r, bsoft, bhard, dbsoft per call (4 values) */
+void ref_brem(int64_t n, int32_t use_expon, const double *ein, const double *eout, const double *egamma,
+              const int32_t *radiate_proton, double *out)
+{
+    ref_rad_setflags_(&use_expon);
+    for (int64_t i = 0; i < n; i++) {
+        double a = ein[i], b = eout[i], c = egamma[i];
+        int32_t rp = radiate_proton[i];
+        out[4 * i] = brem_(&a, &b, &c, &rp, &out[4 * i + 1], &out[4 * i + 2], &out[4 * i + 3]);
+    }
+}
+
+/* args: 14 doubles per call (egamma, k_i, k_f, p_i, p_f, p_fe); out as ref_brem */
+void ref_bremos(int64_t n, int32_t use_expon, const double *args, const int32_t *radiate_proton, double *out)
+{
+    ref_rad_setflags_(&use_expon);
+    for (int64_t i = 0; i < n; i++) {
+        double v[14];
+        memcpy(v, args + 14 * i, sizeof v);
+        int32_t rp = radiate_proton[i];
+        out[4 * i] = bremos_(&v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10],
+                             &v[11], &v[12], &v[13], &rp, &out[4 * i + 1], &out[4 * i + 2], &out[4 * i + 3]);
+    }
+}
+
+void ref_rad_gamma(int64_t n, const double *x, double *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double v = x[i];
+        out[i] = gamma_(&v);
+    }
+}
+
+/* args: itail, plus_flag, doing_proton as ints; e1, e2, e3, p3, th as doubles (5 per call) */
+void ref_lambda_dave(int64_t n, const int32_t *iargs, const double *dargs, double *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        int32_t it = iargs[3 * i], pf = iargs[3 * i + 1], dp = iargs[3 * i + 2];
+        double d[5];
+        memcpy(d, dargs + 5 * i, sizeof d);
+        out[i] = lambda_dave_(&it, &pf, &dp, &d[0], &d[1], &d[2], &d[3], &d[4]);
+    }
+}

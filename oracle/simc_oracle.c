@@ -3443,3 +3443,703 @@ int or_gen_hep(const or_target *T, const or_hep *H, int64_t n, or_hep_out *out, 
     }
     return overflow ? -2 : 0;
 }
+
+/* ========================================================================= */
+/* Radiative tails: H(e,e'p) elastic with using_rad                           */
+/* ========================================================================= */
+
+/* constants.inc:38-44 (alpi = alpha/pi of the rounded alpha) */
+#define OR_C_AMU 931.49432
+static const double or_c_alpi = (1. / 137.0359895) / 3.141592653589793;
+static const double or_c_euler = 0.577215665;
+
+/* gamma: the callers (basicrad_init_ev init.f:744-790, peaked_rad_weight
+ * radc.f:532,640-642, extrad_phi radc.f:675,713) declare `real*8 gamma`
+ * without EXTERNAL, so the Fortran 2008 intrinsic GAMMA is called (libm
+ * tgamma in the compiled reference), not the polynomial function gamma of
+ * radc.f:94-116, which no caller reaches. */
+double or_rad_gamma(double x) { return tgamma(x); }
+
+/* lambda_dave, radc.f:763-812: effective bt of internal radiation per tail
+ * (extended peaking approximation with plus_flag = 1). */
+double or_lambda_dave(int itail, int plus_flag, int doing_proton, double e1, double e2, double e3, double p3,
+                      double th)
+{
+    double lam = 0.0, plus_term = 0.0;
+    if (plus_flag == 1 && itail < 3) {
+        plus_term = log((1. - cos(th)) / 2.);
+        if (doing_proton) plus_term = plus_term + 2. * log(e1 / e2);
+    }
+    if (itail == 1) {
+        lam = or_c_alpi * ((2. * log(2. * e1 / OR_C_ME) - 1.) + plus_term);
+    } else if (itail == 2) {
+        lam = or_c_alpi * ((2. * log(2. * e2 / OR_C_ME) - 1.) + plus_term);
+    } else if (itail == 3) {
+        if (doing_proton) {
+            lam = or_c_alpi * ((e3 / p3) * log((e3 + p3) / (e3 - p3)) - 2.);
+            if (lam < 0) lam = 0.0;
+        } else {
+            lam = 0.0;
+        }
+    }
+    return lam;
+}
+
+/* spence, brem.f:287-322 (the reference's approximate form) */
+static double rad_spence(double ax)
+{
+    const double bx = fabs(ax);
+    if (bx <= 1) return 0.0;
+    const double l = log(bx);
+    return -0.5 * (l * l);
+}
+
+/* inter, brem.f:218-248 (calculate_spence = .true., init.f:648) */
+static double rad_inter(double alpha, double ar1, double ar2, double e1, double e2, double de)
+{
+    const double de2 = e1 - e2;
+    const double amult = -1. / (alpha * (ar1 - ar2));
+    double r = log(fabs((e2 / de) + ar1 * (de2 / de))) * log(fabs((ar1 - 1.) / ar1)) -
+               log(fabs((e2 / de) + ar2 * (de2 / de))) * log(fabs((ar2 - 1.) / ar2));
+    const double arg1 = (de2 / (e2 + ar1 * de2)) * (ar1 - 1.);
+    const double arg2 = (de2 / (e2 + ar1 * de2)) * (ar1);
+    const double arg3 = (de2 / (e2 + ar2 * de2)) * (ar2 - 1.);
+    const double arg4 = (de2 / (e2 + ar2 * de2)) * (ar2);
+    r = (((r - rad_spence(arg1)) + rad_spence(arg2)) + rad_spence(arg3)) - rad_spence(arg4);
+    return r * amult / (OR_C_PI);
+}
+
+/* inter_prime, brem.f:581-598 */
+static double rad_inter_prime(double alpha, double ar1, double ar2, double de)
+{
+    const double amult = -1. / (alpha * (ar1 - ar2));
+    return (-1. / de) * amult / OR_C_PI * (log(fabs((ar1 - 1.) / ar1)) - log(fabs((ar2 - 1.) / ar2)));
+}
+
+/* brem, brem.f:6-216 (include_hard = calculate_spence = .true.,
+ * exponentiate = use_expon, init.f:641-648).  Energies in MeV; returns the
+ * radiative factor, with bsoft, bhard and dbsoft (1/MeV).  The compiled
+ * reference evaluates x**0.5 as a square root (no pow call in brem.o). */
+double or_brem(const or_rad *R, double ein, double eout, double egamma, int radiate_proton, double *bsoft,
+               double *bhard, double *dbsoft)
+{
+    const double pi = 3.141592653589793, am = .93827231, ame = .00051099906, e2 = 1. / 137.0359895;
+    const double ame2 = ame * ame, ame4 = ame2 * ame2, am2 = am * am;
+    const double ak = ein / 1000., akp = eout / 1000., de = egamma / 1000.;
+    const double eang = 2. * asin(sqrt(am / (2. * ak) * (ak / akp - 1.)));
+    const double se = sin(eang / 2.);
+    const double q2 = 4. * ak * akp * (se * se);
+    const double ape = am + ak - akp;
+    const double ap = sqrt(ape * ape - am2);
+    const double pang = acos((ak - akp * cos(eang)) / ap);
+    double aprod = 1.e0;
+    const double bei = aprod * (-1. / (2. * pi)) * log(ak / de);
+    const double dbei = aprod * (1. / (2. * pi * de));
+    const double bef = aprod * (-1. / (2. * pi)) * log(akp / de);
+    const double dbef = aprod * (1. / (2. * pi * de));
+    aprod = -1.e0;
+    double adot = ak * akp * (1. - cos(eang));
+    double alpha = 2. * ame2 - 2. * adot;
+    double ar1 = 0.5 + sqrt(adot * adot - ame4) / alpha;
+    double ar2 = 0.5 - sqrt(adot * adot - ame4) / alpha;
+    const double bee = aprod * adot * rad_inter(alpha, ar1, ar2, ak, akp, de);
+    const double dbee = aprod * adot / (pi * alpha * (ar1 - ar2) * de) *
+                        (log((ar1 - 1) / ar1) - log((ar2 - 1) / ar2));
+    double bpi = 0, bpf = 0, bpp = 0, bepii = 0, bepff = 0, bepif = 0, bepfi = 0;
+    double dbpi = 0, dbpf = 0, dbpp = 0, dbepii = 0, dbepff = 0, dbepif = 0, dbepfi = 0;
+    if (radiate_proton) {
+        const double ame_am2 = (ame * am) * (ame * am);
+        aprod = 1.e0;
+        bpi = aprod * (-1. / (2. * pi)) * log(am / de);
+        dbpi = aprod * (1. / (2. * pi * de));
+        bpf = aprod * (-1. / (2. * pi)) * log(ape / de);
+        dbpf = aprod * (1 / (2. * pi * de));
+        aprod = -1.e0;
+        adot = am * ape;
+        alpha = 2. * am2 - 2. * adot;
+        ar1 = 0.5 + sqrt(adot * adot - am2 * am2) / alpha;
+        ar2 = 0.5 - sqrt(adot * adot - am2 * am2) / alpha;
+        bpp = aprod * adot * rad_inter(alpha, ar1, ar2, am, ape, de);
+        dbpp = aprod * adot / (pi * alpha * (ar1 - ar2) * de) * (log((ar1 - 1) / ar1) - log((ar2 - 1) / ar2));
+        /* the four electron-proton interference terms share one form */
+        const double c_adot[4] = {ak * am, akp * ape - akp * ap * cos(eang + pang), ak * ape - ak * ap * cos(pang),
+                                  akp * am};
+        const double c_e1[4] = {ak, akp, ak, akp}, c_e2[4] = {am, ape, ape, am};
+        const double c_prod[4] = {-1.e0, -1.e0, 1.e0, 1.e0};
+        double bb[4], dbb[4];
+        for (int k = 0; k < 4; ++k) {
+            aprod = c_prod[k];
+            adot = c_adot[k];
+            alpha = am2 + ame2 - 2. * adot;
+            ar1 = (am2 - adot + sqrt(adot * adot - ame_am2)) / alpha;
+            ar2 = (am2 - adot - sqrt(adot * adot - ame_am2)) / alpha;
+            bb[k] = aprod * adot * rad_inter(alpha, ar1, ar2, c_e1[k], c_e2[k], de);
+            dbb[k] = aprod * adot / (pi * alpha * (ar1 - ar2) * de) * (log((ar1 - 1) / ar1) - log((ar2 - 1) / ar2));
+        }
+        bepii = bb[0]; bepff = bb[1]; bepif = bb[2]; bepfi = bb[3];
+        dbepii = dbb[0]; dbepff = dbb[1]; dbepif = dbb[2]; dbepfi = dbb[3];
+    }
+    const double b = 2. * e2 * (bei + bef + bee);
+    double bzz = 0.0, bz = 0.0, dbzz = 0.0, dbz = 0.0;
+    if (radiate_proton) {
+        bzz = 2. * e2 * (bpi + bpf + bpp);
+        bz = 2. * e2 * (bepii + bepff + bepif + bepfi);
+    }
+    *bsoft = b + bz + bzz;
+    *bhard = -1. * (e2 / pi) * (-28 / 9. + 13. / 6. * log(q2 / ame2));
+    const double db = 2. * e2 * (dbei + dbef + dbee);
+    if (radiate_proton) {
+        dbzz = 2. * e2 * (dbpi + dbpf + dbpp);
+        dbz = 2. * e2 * (dbepii + dbepff + dbepif + dbepfi);
+    }
+    *dbsoft = db + dbz + dbzz;
+    *dbsoft = *dbsoft / 1000.;
+    double r = R->use_expon == 1 ? -*dbsoft / exp(*bsoft) : 1. - *dbsoft;
+    return r * (1. - *bhard);
+}
+
+/* bremos, brem.f:344-578: the same with off-shell protons (four-vectors,
+ * MeV in; the initial proton at rest with the proton mass). */
+double or_bremos(const or_rad *R, double egamma, double k_ix, double k_iy, double k_iz, double k_fx,
+                 double k_fy, double k_fz, double p_ix, double p_iy, double p_iz, double p_fx, double p_fy,
+                 double p_fz, double p_fe, int radiate_proton, double *bsoft, double *bhard, double *dbsoft)
+{
+    const double pi = 3.141592653589793, twopi = 2. * 3.141592653589793, ame = .00051099906;
+    const double e2 = 1. / 137.0359895, mp = .93827231;
+    const double ame2 = ame * ame, ame4 = ame2 * ame2;
+    const double de = egamma / 1000.;
+    const double kix = k_ix / 1000., kiy = k_iy / 1000., kiz = k_iz / 1000.;
+    const double kfx = k_fx / 1000., kfy = k_fy / 1000., kfz = k_fz / 1000.;
+    const double pix = p_ix / 1000., piy = p_iy / 1000., piz = p_iz / 1000.;
+    const double pfe = p_fe / 1000., pfx = p_fx / 1000., pfy = p_fy / 1000., pfz = p_fz / 1000.;
+    const double kie = sqrt(kix * kix + kiy * kiy + kiz * kiz + ame2);
+    const double kfe = sqrt(kfx * kfx + kfy * kfy + kfz * kfz + ame2);
+    const double pie = mp;
+    const double dke = kfe - kie, dkx = kfx - kix, dky = kfy - kiy, dkz = kfz - kiz;
+    const double q2 = -1. * (dke * dke - dkx * dkx - dky * dky - dkz * dkz);
+    const double ami = mp;
+    const double amf = sqrt(pfe * pfe - pfx * pfx - pfy * pfy - pfz * pfz);
+    double aprod = 1.e0;
+    const double bei = aprod * (-1. / twopi) * log(kie / de);
+    const double dbei = aprod * (-1. / twopi) * (-1. / de);
+    const double bef = aprod * (-1. / twopi) * log(kfe / de);
+    const double dbef = aprod * (-1. / twopi) * (-1. / de);
+    aprod = -1.e0;
+    double adot = kie * kfe - kix * kfx - kiy * kfy - kiz * kfz;
+    double alpha = 2. * ame2 - 2. * adot;
+    double ar1 = 0.5 + sqrt(4. * (adot * adot) - 4. * ame4) / (2. * alpha);
+    double ar2 = 0.5 - sqrt(4. * (adot * adot) - 4. * ame4) / (2. * alpha);
+    const double bee = aprod * adot * rad_inter(alpha, ar1, ar2, kie, kfe, de);
+    const double dbee = aprod * adot * rad_inter_prime(alpha, ar1, ar2, de);
+    double bpi = 0, bpf = 0, bpp = 0, bepii = 0, bepff = 0, bepif = 0, bepfi = 0;
+    double dbpi = 0, dbpf = 0, dbpp = 0, dbepii = 0, dbepff = 0, dbepif = 0, dbepfi = 0;
+    if (radiate_proton) {
+        aprod = 1.e0;
+        bpi = aprod * (-1. / twopi) * log(pie / de);
+        dbpi = aprod * (-1. / twopi) * (-1. / de);
+        bpf = aprod * (-1. / twopi) * log(pfe / de);
+        dbpf = aprod * (-1. / twopi) * (-1. / de);
+        aprod = -1.e0;
+        adot = pie * pfe - pix * pfx - piy * pfy - piz * pfz;
+        alpha = ami * ami + amf * amf - 2. * adot;
+        {
+            const double m = ami * amf;
+            ar1 = (2. * (amf * amf) - 2. * adot + sqrt(4. * (adot * adot) - 4. * (m * m))) / (2. * alpha);
+            ar2 = (2. * (amf * amf) - 2. * adot - sqrt(4. * (adot * adot) - 4. * (m * m))) / (2. * alpha);
+        }
+        bpp = aprod * adot * rad_inter(alpha, ar1, ar2, pie, pfe, de);
+        dbpp = aprod * adot * rad_inter_prime(alpha, ar1, ar2, de);
+        /* interference terms: (electron leg, proton leg, sign, proton mass) */
+        const double c_adot[4] = {kie * pie - kix * pix - kiy * piy - kiz * piz,
+                                  kfe * pfe - kfx * pfx - kfy * pfy - kfz * pfz,
+                                  kie * pfe - kix * pfx - kiy * pfy - kiz * pfz,
+                                  kfe * pie - kfx * pix - kfy * piy - kfz * piz};
+        const double c_e1[4] = {kie, kfe, kie, kfe}, c_e2[4] = {pie, pfe, pfe, pie};
+        const double c_m[4] = {ami, amf, amf, ami};
+        const double c_prod[4] = {-1.e0, -1.e0, 1.e0, 1.e0};
+        double bb[4], dbb[4];
+        for (int k = 0; k < 4; ++k) {
+            aprod = c_prod[k];
+            adot = c_adot[k];
+            const double M = c_m[k], m = ame * M;
+            alpha = M * M + ame2 - 2. * adot;
+            ar1 = (2. * (M * M) - 2. * adot + sqrt(4. * (adot * adot) - 4. * (m * m))) / (2. * alpha);
+            ar2 = (2. * (M * M) - 2. * adot - sqrt(4. * (adot * adot) - 4. * (m * m))) / (2. * alpha);
+            bb[k] = aprod * adot * rad_inter(alpha, ar1, ar2, c_e1[k], c_e2[k], de);
+            dbb[k] = aprod * adot * rad_inter_prime(alpha, ar1, ar2, de);
+        }
+        bepii = bb[0]; bepff = bb[1]; bepif = bb[2]; bepfi = bb[3];
+        dbepii = dbb[0]; dbepff = dbb[1]; dbepif = dbb[2]; dbepfi = dbb[3];
+    }
+    const double b = 2. * e2 * (bei + bef + bee);
+    double bzz = 0.0, bz = 0.0, dbzz = 0.0, dbz = 0.0;
+    if (radiate_proton) {
+        bzz = 2. * e2 * (bpi + bpf + bpp);
+        bz = 2. * e2 * (bepii + bepff + bepif + bepfi);
+    }
+    *bsoft = b + bz + bzz;
+    *bhard = -1. * (e2 / pi) * (-28 / 9. + 13. / 6. * log(q2 / ame2));
+    const double db = 2. * e2 * (dbei + dbef + dbee);
+    if (radiate_proton) {
+        dbzz = 2. * e2 * (dbpi + dbpf + dbpp);
+        dbz = 2. * e2 * (dbepii + dbepff + dbepif + dbepfi);
+    }
+    *dbsoft = db + dbz + dbzz;
+    *dbsoft = *dbsoft / 1000.;
+    double r = R->use_expon == 1 ? -*dbsoft / exp(*bsoft) : 1. - *dbsoft;
+    return r * (1. - *bhard);
+}
+
+/* spen, radc.f:747-761 */
+static double rad_spen(double x)
+{
+    double y = 1.0, s = 0.0;
+    int i = 0;
+    while (i <= 100 && fabs(y) > fabs(s) * 1.e-4) {
+        i = i + 1;
+        y = x * y;
+        s = s + y / (double)(i * i);
+    }
+    return s;
+}
+
+/* The event fields generate_rad and its helpers read and change (the
+ * hydrogen-elastic members of type(event) and type(event_main)). */
+typedef struct {
+    double Ein, e_E, e_P, e_delta, e_theta, e_phi, e_xptar, e_yptar;
+    double uex, uey, uez;
+    double p_P, p_E, p_delta, p_theta, p_phi, p_xptar, p_yptar;
+    double upx, upy, upz;
+    double Q2, nu, q, Em, Trec;
+    double jacobian, z;
+    double eloss[3], teff[3];
+} rad_event;
+
+/* The per-event radiative constants of /radccom/ (radc.inc:5-14). */
+typedef struct {
+    double bt[2], lambda[3], g[5], c[5], c_int[4], c_ext[4], g_int, g_ext, frac[3];
+    double hardcorfac, etta;
+    int rad_proton_this_ev;
+} rad_state;
+
+/* schwinger, radc.f:721-745 (include_hard = .true.) */
+static void rad_schwinger(const or_rad *R, const rad_state *S, const rad_event *v, double Ecutoff, double *dsoft,
+                          double *dhard)
+{
+    const double lq = log(v->Q2 / (OR_C_ME * OR_C_ME)) - 1.0;
+    const double sn = sin(v->e_theta / 2.);
+    const double s2 = sn * sn;
+    const double b = 1. + 2. * v->nu * s2 / (R->targ_A * OR_C_AMU);
+    const double spence = rad_spen(1. - s2) - 2.5893784;
+    *dsoft = or_c_alpi * lq * log(v->Ein / (S->etta * S->etta) * v->e_E * b / (Ecutoff * Ecutoff));
+    const double l = log(v->Ein / v->e_E);
+    *dhard = -or_c_alpi * (2.166666 * lq + spence - (l * l) / 2.0);
+}
+
+/* basicrad_init_ev, init.f:732-804 */
+static void rad_basicrad_init_ev(rad_state *S, double e1, double e2, double e3)
+{
+    const double one = 1.;
+    const double e[3] = {e1, e2, e3};
+    const double *lam = S->lambda, *bt = S->bt;
+    double *g = S->g, *c = S->c, *ci = S->c_int, *ce = S->c_ext;
+    g[1] = lam[0] + bt[0];
+    g[2] = lam[1] + bt[1];
+    g[3] = lam[2];
+    g[0] = g[1] + g[2] + g[3];
+    ci[1] = lam[0] / pow(e[0] * e[1], lam[0] / 2.);
+    ci[2] = lam[1] / pow(e[0] * e[1], lam[1] / 2.);
+    ci[3] = lam[2] / pow(OR_C_MP * e[2], lam[2] / 2.);
+    for (int i = 1; i <= 3; ++i) ci[i] = ci[i] * exp(-or_c_euler * lam[i - 1]) / tgamma(one + lam[i - 1]);
+    S->g_int = lam[0] + lam[1] + lam[2];
+    ci[0] = ci[1] * ci[2] * S->g_int / lam[0] / lam[1];
+    if (lam[2] > 0) ci[0] = ci[0] * ci[3] / lam[2];
+    ci[0] = ci[0] * tgamma(one + lam[0]) * tgamma(one + lam[1]) * tgamma(one + lam[2]) /
+            tgamma(one + S->g_int);
+    for (int i = 1; i <= 2; ++i) ce[i] = bt[i - 1] / pow(e[i - 1], bt[i - 1]) / tgamma(one + bt[i - 1]);
+    ce[3] = 0.0;
+    S->g_ext = bt[0] + bt[1];
+    ce[0] = ce[1] * ce[2] * S->g_ext / bt[0] / bt[1];
+    ce[0] = ce[0] * tgamma(one + bt[0]) * tgamma(one + bt[1]) / tgamma(one + S->g_ext);
+    for (int i = 1; i <= 2; ++i)
+        c[i] = ci[i] * ce[i] * g[i] / lam[i - 1] / bt[i - 1] * tgamma(one + lam[i - 1]) *
+               tgamma(one + bt[i - 1]) / tgamma(one + g[i]);
+    c[3] = ci[3];
+    c[0] = c[1] * c[2] * g[0] / g[1] / g[2];
+    if (g[3] > 0) c[0] = c[0] * c[3] / g[3];
+    c[0] = c[0] * tgamma(one + g[1]) * tgamma(one + g[2]) * tgamma(one + g[3]) /
+           tgamma(one + g[0]);
+    c[4] = g[4] / pow(e1 * e2, g[4]) / tgamma(one + g[4]);
+    if (g[3] > 0) c[4] = c[4] / pow(e3, g[4]);
+}
+
+/* radc_init_ev, init.f:655-720 */
+static void rad_init_ev(const or_rad *R, rad_state *S, const rad_event *v)
+{
+    S->etta = 1.0;
+    for (int i = 0; i < 2; ++i) S->bt[i] = R->etatzai * v->teff[i];
+    for (int i = 1; i <= 3; ++i)
+        S->lambda[i - 1] = or_lambda_dave(i, 1, R->doing_tail[2], v->Ein, v->e_E, v->p_E, v->p_P, v->e_theta);
+    S->rad_proton_this_ev = S->lambda[2] > 0;
+    const double Ecutoff = 450.;
+    double dsoft, dhard, dsoft_prime = 0.0;  /* schwinger leaves dsoft_prime at its initial 0 */
+    if (R->intcor_mode == 0) {
+        rad_schwinger(R, S, v, Ecutoff, &dsoft, &dhard);
+    } else if (!R->use_offshell_rad) {
+        or_brem(R, v->Ein, v->e_E, Ecutoff, S->rad_proton_this_ev, &dsoft, &dhard, &dsoft_prime);
+    } else {
+        or_bremos(R, Ecutoff, 0.0, 0.0, v->Ein, v->e_P * v->uex, v->e_P * v->uey, v->e_P * v->uez, 0.0, 0.0, 0.0,
+                  v->p_P * v->upx, v->p_P * v->upy, v->p_P * v->upz, v->p_E, S->rad_proton_this_ev, &dsoft,
+                  &dhard, &dsoft_prime);
+    }
+    S->hardcorfac = 1. / (1. - dhard);
+    S->g[4] = -dsoft_prime * Ecutoff + S->bt[0] + S->bt[1];
+    rad_basicrad_init_ev(S, v->Ein, v->e_E, v->p_E);
+    for (int i = 1; i <= 3; ++i) S->frac[i - 1] = S->g[i] / S->g[0];
+}
+
+/* basicrad, radc.f:3-91: Egamma from g Egamma^(g-1) / (hi^g - lo^g) over
+ * [lo, hi] (one uniform); itail 0 is the combined tail (4). */
+static void rad_basicrad(or_rng *gr, const rad_state *S, int itail, double lo, double hi, double *Egamma,
+                         double *weight, double *val_reciprocal)
+{
+    *Egamma = 0.0;
+    *weight = 0.0;
+    *val_reciprocal = 0.0;
+    if (itail == 0) itail = 4;
+    const double g = S->g[itail];
+    if (g <= 0) {
+        *weight = 1.0;
+        return;
+    }
+    if (hi <= lo || hi <= 0) return;
+    const double power_hi = pow(hi, g);
+    double power_lo = 0.0;
+    if (lo > 0) power_lo = pow(lo, g);
+    const double ymin = power_lo / power_hi;
+    const double y = ymin + or_grnd(gr) * (1. - ymin);
+    const double x = pow(y, 1. / g);
+    *Egamma = x * hi;
+    if (*Egamma > 0) *val_reciprocal = pow(*Egamma, 1. - g) * (power_hi - power_lo) / g;
+    *weight = S->c[itail] / g * (power_hi - power_lo);
+}
+
+/* extrad_phi, radc.f:668-718 (extrad_flag 3 with itail 0 stops the
+ * reference; not reached from supported settings) */
+static double rad_extrad_phi(const or_rad *R, const rad_state *S, int extrad_flag, int itail, double E1, double E2,
+                             double Egamma)
+{
+    const double E[2] = {E1, E2};
+    double phi = 1.0;
+    if (extrad_flag == 2) {
+        if (itail == 0)
+            phi = 1. - (S->bt[0] / E[0] + S->bt[1] / E[1]) / (S->g[1] + S->g[2]) * Egamma;
+        else if (itail == 1 || itail == 2)
+            phi = 1. - S->bt[itail - 1] / E[itail - 1] / S->g[itail] * Egamma;
+    } else if (extrad_flag == 3) {
+        if (itail == 1 || itail == 2) {
+            const double x = Egamma / E[itail - 1];
+            const double t = S->bt[itail - 1] / R->etatzai;
+            phi = phi * (1. - x + (x * x) / R->etatzai) *
+                  exp(t * ((R->etatzai - 0.5) - R->etatzai * x + (x * x) / 2.)) * tgamma(1. + S->bt[itail - 1]);
+        }
+    }
+    return phi;
+}
+
+/* peaked_rad_weight, radc.f:523-650 (rad_flag 0 or 1) */
+static double rad_peaked_weight(const or_rad *R, const rad_state *S, int extrad_flag, const rad_event *v,
+                                double Egamma, double emin, double emax, double basicrad_weight)
+{
+    const double ein = v->Ein, eout = v->e_E, eul = 0.577215665;
+    if (extrad_flag <= 2) {
+        const double phi_ext = rad_extrad_phi(R, S, extrad_flag, 0, ein, eout, Egamma);
+        if (R->rad_flag == 1) return basicrad_weight * phi_ext;
+    }
+    double dsoft_intmin = 1.0, dsoft_intmax, dhard, dpm, dpx;
+    if (!R->use_offshell_rad) {
+        if (emin > 0) or_brem(R, ein, eout, emin, S->rad_proton_this_ev, &dsoft_intmin, &dhard, &dpm);
+        or_brem(R, ein, eout, emax, S->rad_proton_this_ev, &dsoft_intmax, &dhard, &dpx);
+    } else {
+        const double kx = v->e_E * v->uex, ky = v->e_E * v->uey, kz = v->e_E * v->uez;
+        const double px = v->p_P * v->upx, py = v->p_P * v->upy, pz = v->p_P * v->upz;
+        if (emin > 0)
+            or_bremos(R, emin, 0.0, 0.0, ein, kx, ky, kz, 0.0, 0.0, 0.0, px, py, pz, v->p_E, S->rad_proton_this_ev,
+                      &dsoft_intmin, &dhard, &dpm);
+        or_bremos(R, emax, 0.0, 0.0, ein, kx, ky, kz, 0.0, 0.0, 0.0, px, py, pz, v->p_E, S->rad_proton_this_ev,
+                  &dsoft_intmax, &dhard, &dpx);
+    }
+    double w;
+    if (emin > 0)
+        w = S->c_ext[0] / S->g_ext *
+            (exp(-dsoft_intmax) * pow(emax, S->g_ext) - exp(-dsoft_intmin) * pow(emin, S->g_ext));
+    else
+        w = S->c_ext[0] / S->g_ext * (exp(-dsoft_intmax) * pow(emax, S->g_ext));
+    const double *g = S->g, *bt = S->bt;
+    w = w * exp(-eul * g[4]) / tgamma(1. + g[4]) * tgamma(1. + g[4] - bt[0] - bt[1]) *
+        tgamma(1. + bt[0]) * tgamma(1. + bt[1]) / tgamma(1. + g[4]);
+    if (w < 0) w = 0;
+    return w;
+}
+
+double or_rad_etatzai(double Z)
+{
+    double L1 = log(184.15) - log(Z) / 3.0, L2 = log(1194.) - 2. * log(Z) / 3.0;  /* init.f:19-24 */
+    if (Z == 1) {
+        L1 = 5.31;
+        L2 = 6.144;
+    }
+    return (12.0 + (Z + 1.) / (Z * L1 + L2)) / 9.0;  /* init.f:634 */
+}
+
+/* complete_ev for hydrogen elastic (event.f:497-563, :1013-1014) and the
+ * scattered particles' target losses (:1031-1040), then radc_init_ev
+ * (:1045).  Returns 0 when E' > Ein (complete_ev's failure). */
+static int rad_complete_ev(or_rng *gr, const or_target *T, const or_hep *H, const or_rad *R, rad_state *S,
+                           rad_event *v)
+{
+    v->jacobian = 1.0;
+    v->uex = sin(v->e_theta) * cos(v->e_phi);
+    v->uey = sin(v->e_theta) * sin(v->e_phi);
+    v->uez = cos(v->e_theta);
+    v->e_E = v->Ein * H->Mh / (H->Mh + v->Ein * (1. - v->uez));
+    if (v->e_E > v->Ein) return 0;
+    v->e_P = v->e_E;
+    v->e_delta = (v->e_P - H->e_P) * 100. / H->e_P;
+    v->nu = v->Ein - v->e_E;
+    v->Q2 = 2 * v->Ein * v->e_E * (1. - v->uez);
+    v->q = sqrt(v->Q2 + v->nu * v->nu);
+    const double uqx = -v->e_P * v->uex / v->q, uqy = -v->e_P * v->uey / v->q;
+    const double uqz = (v->Ein - v->e_P * v->uez) / v->q;
+    v->Em = 0.0;
+    v->upx = uqx;
+    v->upy = uqy;
+    v->upz = uqz;
+    v->p_P = v->q;
+    v->p_theta = acos(v->upz);
+    v->p_phi = atan2(v->upy, v->upx);
+    if (v->p_phi < 0.) v->p_phi = v->p_phi + 2. * OR_C_PI;
+    or_spectrometer_angles(H->p_theta, H->p_phi, &v->p_xptar, &v->p_yptar, v->p_theta, v->p_phi);
+    v->p_E = sqrt(v->p_P * v->p_P + H->Mh * H->Mh);
+    v->p_delta = (v->p_P - H->p_P) * 100. / H->p_P;
+    v->Trec = 0.0;
+    const double r = sqrt(1. + v->e_yptar * v->e_yptar + v->e_xptar * v->e_xptar);
+    v->jacobian = v->jacobian / (r * r * r);
+    or_trip_thru_target(gr, T, 2, R->e_arm, v->z - T->zoffset, v->e_E, v->e_theta, OR_C_ME, 1, &v->eloss[1],
+                        &v->teff[1]);
+    or_trip_thru_target(gr, T, 3, R->p_arm, v->z - T->zoffset, v->p_E, v->p_theta, H->Mh, 1, &v->eloss[2],
+                        &v->teff[2]);
+    if (!H->using_eloss) {
+        v->eloss[1] = 0.0;
+        v->eloss[2] = 0.0;
+    }
+    rad_init_ev(R, S, v);
+    return 1;
+}
+
+/* One event of generate with using_rad (event.f:157-212, :275-277,
+ * :322-325, :381-404) and generate_rad (radc.f:120-517) for doing_hyd_elast,
+ * then complete_main's weight (event.f:1397-1398, :1450-1452, :1558). */
+static void gen_hep_rad_event(or_rng *gr, const or_target *T, const or_hep *H, const or_rad *R, int64_t i,
+                              or_hep_out *o, or_rad_out *ro)
+{
+    const double nsig_max = 3.0;
+    rad_event v;
+    memset(&v, 0, sizeof v);
+    rad_state S;
+    memset(&S, 0, sizeof S);
+    double x = or_gauss1(gr, nsig_max) * H->xwid + H->xoffset;
+    double y = or_gauss1(gr, nsig_max) * H->ywid + H->yoffset;
+    double t3, t4, t5 = 0.0, t6 = 0.0;
+    if (H->fr_pattern == 1) {
+        t3 = or_grnd(gr) * OR_C_PI;
+        t4 = or_grnd(gr) * OR_C_PI;
+        t5 = cos(t3) * H->fr1;
+        t6 = cos(t4) * H->fr2;
+    } else if (H->fr_pattern == 2) {
+        t3 = or_grnd(gr) * 2. * OR_C_PI;
+        t4 = sqrt(or_grnd(gr)) * (H->fr2 - H->fr1) + H->fr1;
+        t5 = cos(t3) * t4;
+        t6 = sin(t3) * t4;
+    } else if (H->fr_pattern == 3) {
+        t3 = 2. * or_grnd(gr) - 1.0;
+        t4 = 2. * or_grnd(gr) - 1.0;
+        t5 = H->fr1 * t3;
+        t6 = H->fr2 * t4;
+    }
+    x = x + t5;
+    y = y + t6;
+    v.z = (0.5 - or_grnd(gr)) * T->length + T->zoffset;
+    or_trip_thru_target(gr, T, 1, 0, v.z - T->zoffset, H->Ebeam, 0.0, OR_C_ME, 1, &v.eloss[0], &v.teff[0]);
+    if (!H->using_eloss) v.eloss[0] = 0.0;
+    const double coul = H->using_coulomb ? H->coulomb : 0.0;
+    v.Ein = H->Ebeam + (or_grnd(gr) - 0.5) * H->dEbeam + coul - v.eloss[0];
+    v.e_yptar = H->e_yptar[0] + or_grnd(gr) * (H->e_yptar[1] - H->e_yptar[0]);
+    v.e_xptar = H->e_xptar[0] + or_grnd(gr) * (H->e_xptar[1] - H->e_xptar[0]);
+    or_physics_angles(H->e_theta, H->e_phi, v.e_xptar, v.e_yptar, &v.e_theta, &v.e_phi);
+    int ok = rad_complete_ev(gr, T, H, R, &S, &v);
+    double gen_weight = 1.0, Eg[3] = {0.0, 0.0, 0.0};
+    double orig_Ein = 0, orig_e_E = 0, orig_e_delta = 0, orig_p_E = 0, orig_p_P = 0, orig_p_delta = 0;
+    int ntail = 0;
+    if (ok) {
+        /* generate_rad */
+        ok = 0;
+        const int extrad_flag = R->extrad_flag != 0 ? R->extrad_flag : (R->rad_flag == 0 ? 3 : 1);
+        double rad_weight = 1;
+        int peaked_basis_flag = 1;
+        if (R->rad_flag <= 1) {
+            peaked_basis_flag = 0;
+            const double u = or_grnd(gr);
+            if (u >= S.frac[0] + S.frac[1]) ntail = 3;
+            else if (u >= S.frac[0]) ntail = 2;
+            else ntail = 1;
+        } else if (R->rad_flag == 2) {
+            ntail = (int)(or_grnd(gr) * 3.) + 1;
+            if (ntail == 4) ntail = 3;
+        } else {
+            ntail = 0;
+        }
+        const double Mp = OR_C_MP;
+        const double max_delta_Trec = fmax(v.Trec - R->vertex_Trec[0], R->vertex_Trec[1] - v.Trec);
+        const double Egamma1_max = R->doing_tail[0] ? R->Egamma_tot_max : 0.0;
+        const int hardwired = R->Egamma_gen_max > 0.01;
+        double Emin[3] = {0, 0, 0}, Emax[3] = {0, 0, 0}, bw = 0.0, bvr = 0.0;
+        int fail = 0;
+        if (R->doing_tail[0] && (ntail == 0 || ntail == 1)) {
+            double ebeam_max = Mp * R->edge_e_E[1] / (Mp - R->edge_e_E[1] * (1. - v.uez));
+            if (ebeam_max < 0) ebeam_max = 1.e10;
+            const double ebeam_min = Mp * R->edge_e_E[0] / (Mp - R->edge_e_E[0] * (1. - v.uez));
+            Emin[0] = v.Ein - ebeam_max;
+            Emax[0] = v.Ein - ebeam_min;
+            Emax[0] = fmin(Emax[0], R->edge_Em[1]);
+            Emax[0] = fmin(Emax[0], Egamma1_max);
+            Emin[0] = Emin[0] - R->dE_edge_test;
+            Emax[0] = Emax[0] + R->dE_edge_test;
+            if (hardwired) Emax[0] = R->Egamma_gen_max;
+            rad_basicrad(gr, &S, 1 * peaked_basis_flag, Emin[0], Emax[0], &Eg[0], &bw, &bvr);
+            if (bw <= 0) {
+                fail = 1;
+            } else {
+                v.Ein = v.Ein - Eg[0];
+                if (!rad_complete_ev(gr, T, H, R, &S, &v)) {
+                    fail = 1;
+                } else {
+                    rad_weight = rad_weight * bw;
+                    if (R->rad_flag <= 1)
+                        rad_weight = rad_peaked_weight(R, &S, extrad_flag, &v, Eg[0], Emin[0], Emax[0], bw);
+                    else
+                        rad_weight = rad_weight * rad_extrad_phi(R, &S, extrad_flag, 1, v.Ein, v.e_E, Eg[0]);
+                }
+            }
+        }
+        if (!fail && R->doing_tail[1] && (ntail == 0 || ntail == 2)) {
+            Emin[1] = v.e_E - R->edge_e_E[1];
+            Emax[1] = v.e_E - R->edge_e_E[0];
+            /* doing_eep */
+            Emax[1] = fmin(Emax[1], (R->edge_Em[1] - v.Em) - Eg[0] + max_delta_Trec);
+            if (ntail != 0 || !S.rad_proton_this_ev)
+                Emin[1] = fmax(Emin[1], (R->edge_Em[0] - v.Em) - Eg[0] - max_delta_Trec);
+            Emax[1] = fmin(Emax[1], R->Egamma_tot_max - Eg[0]);
+            Emin[1] = Emin[1] - R->dE_edge_test;
+            Emax[1] = Emax[1] + R->dE_edge_test;
+            if (hardwired) Emax[1] = R->Egamma_gen_max;
+            rad_basicrad(gr, &S, 2 * peaked_basis_flag, Emin[1], Emax[1], &Eg[1], &bw, &bvr);
+            if (bw <= 0) {
+                fail = 1;
+            } else {
+                rad_weight = rad_weight * bw;
+                if (R->rad_flag <= 1)
+                    rad_weight = rad_peaked_weight(R, &S, extrad_flag, &v, Eg[1], Emin[1], Emax[1], bw);
+                else
+                    rad_weight = rad_weight * rad_extrad_phi(R, &S, extrad_flag, 2, v.Ein, v.e_E, Eg[1]);
+            }
+        }
+        if (!fail && S.rad_proton_this_ev && (ntail == 0 || ntail == 3)) {
+            Emin[2] = v.p_E - R->edge_p_E[1];
+            Emax[2] = v.p_E - R->edge_p_E[0];
+            Emax[2] = fmin(Emax[2], (R->edge_Em[1] - v.Em) - Eg[0] - Eg[1] + max_delta_Trec);
+            Emin[2] = fmax(Emin[2], (R->edge_Em[0] - v.Em) - Eg[0] - Eg[1] - max_delta_Trec);
+            Emax[2] = fmin(Emax[2], R->Egamma_tot_max - Eg[0] - Eg[1]);
+            Emin[2] = Emin[2] - R->dE_edge_test;
+            Emax[2] = Emax[2] + R->dE_edge_test;
+            if (hardwired) Emax[2] = R->Egamma_gen_max;
+            rad_basicrad(gr, &S, 3 * peaked_basis_flag, Emin[2], Emax[2], &Eg[2], &bw, &bvr);
+            if (bw <= 0) {
+                fail = 1;
+            } else {
+                rad_weight = rad_weight * bw;
+                if (R->rad_flag <= 1)
+                    rad_weight = rad_peaked_weight(R, &S, extrad_flag, &v, Eg[2], Emin[2], Emax[2], bw);
+                else
+                    rad_weight = rad_weight * rad_extrad_phi(R, &S, extrad_flag, 3, v.Ein, v.e_E, Eg[2]);
+            }
+        }
+        if (!fail) {
+            orig_Ein = v.Ein + Eg[0];
+            orig_e_E = v.e_E - Eg[1];
+            if (orig_e_E > 0e0) {
+                orig_e_delta = (orig_e_E - H->e_P) / H->e_P * 100.;
+                orig_p_E = v.p_E - Eg[2];
+                if (orig_p_E > H->Mh) {
+                    orig_p_P = sqrt(orig_p_E * orig_p_E - H->Mh * H->Mh);
+                    orig_p_delta = (orig_p_P - H->p_P) / H->p_P * 100.;
+                    gen_weight = gen_weight * rad_weight / S.hardcorfac;
+                    ok = 1;
+                }
+            }
+        }
+    }
+    const double sigcc = or_sigep(v.Q2, v.e_E, v.e_theta, v.Ein);
+    const double weight = v.jacobian * gen_weight * sigcc;
+#define OR_PUT(f, val) if (o->f) o->f[i] = (val)
+    OR_PUT(x, x); OR_PUT(y, y); OR_PUT(z, v.z); OR_PUT(rasterx, t5); OR_PUT(rastery, t6);
+    OR_PUT(Ein, v.Ein); OR_PUT(eloss, v.eloss[0]); OR_PUT(teff, v.teff[0]);
+    OR_PUT(e_xptar, v.e_xptar); OR_PUT(e_yptar, v.e_yptar); OR_PUT(e_theta, v.e_theta);
+    OR_PUT(e_phi, v.e_phi); OR_PUT(e_E, v.e_E); OR_PUT(e_delta, v.e_delta);
+    OR_PUT(p_xptar, v.p_xptar); OR_PUT(p_yptar, v.p_yptar); OR_PUT(p_theta, v.p_theta);
+    OR_PUT(p_phi, v.p_phi); OR_PUT(p_P, v.p_P); OR_PUT(p_E, v.p_E); OR_PUT(p_delta, v.p_delta);
+    OR_PUT(Q2, v.Q2); OR_PUT(nu, v.nu); OR_PUT(q, v.q); OR_PUT(jacobian, v.jacobian);
+    OR_PUT(sigcc, sigcc); OR_PUT(weight, weight); OR_PUT(ok, (uint8_t)ok);
+#undef OR_PUT
+#define OR_PUT(f, val) if (ro->f) ro->f[i] = (val)
+    OR_PUT(Egamma1, Eg[0]); OR_PUT(Egamma2, Eg[1]); OR_PUT(Egamma3, Eg[2]); OR_PUT(gen_weight, gen_weight);
+    OR_PUT(orig_Ein, orig_Ein); OR_PUT(orig_e_E, orig_e_E); OR_PUT(orig_e_delta, orig_e_delta);
+    OR_PUT(orig_p_E, orig_p_E); OR_PUT(orig_p_P, orig_p_P); OR_PUT(orig_p_delta, orig_p_delta);
+    OR_PUT(eloss2, v.eloss[1]); OR_PUT(teff2, v.teff[1]); OR_PUT(eloss3, v.eloss[2]); OR_PUT(teff3, v.teff[2]);
+    OR_PUT(hardcorfac, S.hardcorfac); OR_PUT(ntail, (int32_t)ntail);
+#undef OR_PUT
+}
+
+int or_gen_hep_rad(const or_target *T, const or_hep *H, const or_rad *R, int64_t n, or_hep_out *out,
+                   or_rad_out *rout, int rng_kind, const double *tape, int64_t stride, uint64_t seed,
+                   uint64_t event0, int nthreads, int32_t *used)
+{
+    if (R->rad_flag < 0 || R->rad_flag > 3 || R->extrad_flag < 0 || R->extrad_flag > 3) return -3;
+    /* rad_flag 1 with the Friedrich form reads stale saved locals in the
+     * reference (radc.f:610-636); rad_flag >= 2 with extrad_flag 3 stops
+     * there for the combined tail only, which the peaked basis alone uses */
+    if (R->rad_flag == 1 && R->extrad_flag == 3) return -3;
+    if (nthreads <= 0) nthreads = 1;
+    int64_t overflow = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : overflow)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        or_rng g;
+        memset(&g, 0, sizeof g);
+        g.kind = rng_kind;
+        if (rng_kind == 0) {
+            g.tape = tape + i * stride;
+            g.stride = stride;
+        } else {
+            g.key[0] = (uint32_t)seed;
+            g.key[1] = (uint32_t)(seed >> 32);
+            g.event = event0 + (uint64_t)i;
+            g.tag = OR_GEN_TAG;
+        }
+        gen_hep_rad_event(&g, T, H, R, i, out, rout);
+        if (g.overflow) overflow++;
+        if (used) used[i] = (int32_t)g.used;
+    }
+    return overflow ? -2 : 0;
+}

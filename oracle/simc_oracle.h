@@ -373,6 +373,52 @@ int or_gen_hep(const or_target *T, const or_hep *H, int64_t n, or_hep_out *out, 
                const double *tape, int64_t stride, uint64_t seed, uint64_t event0, int nthreads,
                int32_t *used);
 
+/* ---- radiative tails of H(e,e'p) elastic (radc.f, brem.f, init.f) ----
+ * Run constants of radc_init (init.f:576-651) and limits_init
+ * (init.f:91-470) that generate_rad reads, as explicit inputs. */
+typedef struct {
+    int rad_flag;          /* 0..3 (radc.f:141-159) */
+    int extrad_flag;       /* 1..3; 0 = radc_init's default (init.f:621-629) */
+    int intcor_mode;       /* 0 schwinger, 1 brem / bremos (init.f:690-702) */
+    int use_expon;         /* brem's exponentiate (init.f:641-645) */
+    int use_offshell_rad;  /* bremos instead of brem */
+    int doing_tail[3];     /* dbase.f:494-511 from one_tail */
+    int e_arm, p_arm;      /* spectrometers of complete_ev's window losses */
+    double dE_edge_test;
+    double Egamma_gen_max; /* hardwired limit when > 0.01 (dbase.f:514-518) */
+    double edge_e_E[2], edge_p_E[2], edge_Em[2], vertex_Trec[2];  /* min, max */
+    double Egamma_tot_max; /* init.f:409-421 */
+    double etatzai;        /* init.f:634 */
+    double targ_A;         /* schwinger's recoil factor (radc.f:729) */
+} or_rad;
+typedef struct {
+    double *Egamma1, *Egamma2, *Egamma3;   /* Egamma_used(1..3) */
+    double *gen_weight;                    /* main%gen_weight = rad_weight / hardcorfac */
+    double *orig_Ein, *orig_e_E, *orig_e_delta, *orig_p_E, *orig_p_P, *orig_p_delta;
+    double *eloss2, *teff2, *eloss3, *teff3;  /* complete_ev's losses (last call) */
+    double *hardcorfac;
+    int32_t *ntail;                        /* ntup%radarm */
+} or_rad_out;
+/* radc_init's etatzai for a target of charge Z (target_init's L1, L2). */
+double or_rad_etatzai(double Z);
+/* generate with using_rad for doing_hyd_elast (event.f:126-428 calling
+ * generate_rad, radc.f:120-517) and complete_main's weight; hep outputs
+ * hold the VERTEX kinematics (after tail 1), rad outputs the photons, the
+ * ORIG kinematics and complete_ev's losses.  Returns 0, -2 on tape
+ * overflow, -3 for an unsupported setting. */
+int or_gen_hep_rad(const or_target *T, const or_hep *H, const or_rad *R, int64_t n, or_hep_out *out,
+                   or_rad_out *rout, int rng_kind, const double *tape, int64_t stride, uint64_t seed,
+                   uint64_t event0, int nthreads, int32_t *used);
+/* pieces, for pins (or_rad_gamma: the GAMMA the reference's callers reach) */
+double or_rad_gamma(double x);
+double or_brem(const or_rad *R, double ein, double eout, double egamma, int radiate_proton, double *bsoft,
+               double *bhard, double *dbsoft);
+double or_bremos(const or_rad *R, double egamma, double k_ix, double k_iy, double k_iz, double k_fx,
+                 double k_fy, double k_fz, double p_ix, double p_iy, double p_iz, double p_fx, double p_fy,
+                 double p_fz, double p_fe, int radiate_proton, double *bsoft, double *bhard, double *dbsoft);
+double or_lambda_dave(int itail, int plus_flag, int doing_proton, double e1, double e2, double e3, double p3,
+                      double th);
+
 #ifdef __cplusplus
 }
 #endif

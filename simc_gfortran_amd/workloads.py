@@ -267,3 +267,51 @@ HEP_ELASTIC = dict(Ebeam=10600.0, dEbeam=math.sqrt(12.0) * 5e-4 * 10600.0, xwid=
                    yoffset=0.0, fr1=0.1, fr2=0.1, coulomb=0.0, e_yptar=(-0.035, 0.035), e_xptar=(-0.05, 0.05),
                    e_theta=math.radians(12.5), e_phi=arm_phi(5), e_P=_E1, p_theta=_THP, p_phi=arm_phi(1), p_P=_PP,
                    Mh=MP, fr_pattern=3, using_eloss=1, using_coulomb=0)
+
+
+def hep_rad_limits(hep: dict, *, e_delta=(-10.0, 22.0), p_delta=(-10.0, 10.0), Em_cut=(0.0, 200.0),
+                   eloss_max=(4.0, 4.0, 8.0), slop_delta=(0.0, 0.0), dE_edge_test=0.0,
+                   Em_edge_from_cuts=True) -> dict:
+    """Edges of an H(e,e'p) run that generate_rad reads, formed with
+    limits_init's formulas (init.f:206-226 true edges, :282-302 Em slop,
+    :405-421 Egamma_tot_max) from the spectrometer acceptances (SPedge delta,
+    %), the Em cut window (MeV), the largest target losses of the beam, the
+    electron and the proton (MeV; the smallest are taken as 0) and the
+    reconstruction slops of delta (%).  No Coulomb correction.
+
+    Em_edge_from_cuts: the reference's limits_init forms edge%Em from the cut
+    window only for A >= 3 (init.f:297-302) and leaves it 0 for hydrogen, which
+    limits tails 1 and 2 to photons below dE_edge_test; True applies the
+    A >= 3 formula (the evident intent), False reproduces the zeros."""
+    e_P, p_P, Mh = hep["e_P"], hep["p_P"], hep["Mh"]
+    sd_e, sd_p = slop_delta
+    e_min = (1. + (e_delta[0] - sd_e) / 100.) * e_P - dE_edge_test
+    e_max = (1. + (e_delta[1] + sd_e) / 100.) * e_P + dE_edge_test
+    pp_min = max(0.001, (1. + (p_delta[0] - sd_p) / 100.) * p_P - dE_edge_test)
+    pp_max = (1. + (p_delta[1] + sd_p) / 100.) * p_P + dE_edge_test
+    pE_min = math.sqrt(pp_min ** 2 + Mh * Mh)
+    pE_max = math.sqrt(pp_max ** 2 + Mh * Mh)
+    e_max += eloss_max[1]
+    pE_max += eloss_max[2]
+    ebeam_max = hep["Ebeam"] + hep["dEbeam"] / 2.
+    if Em_edge_from_cuts:
+        slop_ebeam = hep["dEbeam"] / 2. + eloss_max[0]
+        slop_ee = sd_e / 100. * e_P + eloss_max[1]
+        r = math.sqrt(pE_max ** 2 - Mh * Mh)
+        slop_ep = math.sqrt((r + sd_p / 100. * p_P) ** 2 + Mh * Mh) - pE_max + eloss_max[2]
+        slop = slop_ebeam + slop_ee + slop_ep + dE_edge_test
+        Em = (max(0.0, Em_cut[0] - slop), Em_cut[1] + slop)
+    else:
+        Em = (0.0, 0.0)
+    tot = ebeam_max + Mh - e_min - pE_min
+    return dict(edge_e_E=(e_min, e_max), edge_p_E=(pE_min, pE_max), edge_Em=Em, vertex_Trec=(0.0, 0.0),
+                Egamma_tot_max=tot, dE_edge_test=dE_edge_test)
+
+
+# The radiative settings of the reference's H(e,e'p) example input
+# (infiles/extra_hydrogen.inp: rad_flag 0, extrad_flag 2, intcor_mode 1,
+# use_offshell_rad 1, one_tail 0, use_expon 0) for the coincidence
+# configuration, electron in the SHMS (5) and proton in the HMS (1).
+HEP_RAD = dict(rad_flag=0, extrad_flag=2, intcor_mode=1, use_expon=0, use_offshell_rad=1,
+               doing_tail=(1, 1, 1), e_arm=5, p_arm=1, Egamma_gen_max=0.0,
+               **hep_rad_limits(HEP_ELASTIC))

@@ -14,6 +14,9 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         vertices on per-event tapes
     python tests/golden/make_golden.py field    # target-field tracking (field/*.npz):
         the reference's trgInit, track_from_tgt and track_to_tgt (trg_track.f)
+    python tests/golden/make_golden.py rad      # radiative tails (gen/rad_*.npz): the
+        reference's generate with using_rad (generate_rad, brem, bremos,
+        radc_init_ev) + complete_main on per-event tapes, one process per case
     python tests/golden/make_golden.py lfit     # track fit (lfit/ref_lfit.npz): the
         reference's LFIT (cern/lfit.f) over seeded REAL*4 point sets
     python tests/golden/make_golden.py gen      # generator (gen/*.npz):
@@ -259,6 +262,60 @@ GEN_N = 2000
 GEN_STRIDE = 96
 GEN_SEED = 1300
 
+# Radiative tails (generate with using_rad: generate_rad of radc.f, brem.f,
+# radc_init_ev of init.f): name, radiative settings over W.HEP_RAD, generator
+# settings over W.HEP_ELASTIC.  Each case runs in a fresh process: with
+# intcor_mode 0 the reference's radc_init_ev reads a saved local
+# (dsoft_prime) that only brem/bremos set, so an earlier case would leak
+# into it; in a fresh process it is 0.
+RAD_CASES = [
+    ("rad_default", {}, {}),
+    ("rad_onshell_brem", dict(use_offshell_rad=0), {}),
+    ("rad_schwinger", dict(intcor_mode=0), {}),
+    ("rad_flag1", dict(rad_flag=1), {}),
+    ("rad_flag2_friedrich", dict(rad_flag=2, extrad_flag=3), {}),
+    ("rad_flag3", dict(rad_flag=3), {}),
+    ("rad_extrad3", dict(extrad_flag=3), {}),
+    ("rad_expon", dict(use_expon=1), {}),
+    ("rad_hardwired", dict(Egamma_gen_max=50.0), {}),
+    ("rad_no_proton_tail", dict(doing_tail=(1, 1, 0)), {}),
+    ("rad_edges_as_built", dict(W.hep_rad_limits(W.HEP_ELASTIC, Em_edge_from_cuts=False, dE_edge_test=5.0)), {}),
+    ("rad_raster1_no_eloss", {}, dict(fr_pattern=1, using_eloss=0)),
+]
+RAD_N = 800
+RAD_STRIDE = 160
+RAD_SEED = 1600
+
+
+def rad_settings(i):
+    name, rkw, hkw = RAD_CASES[i]
+    rad = dict(W.HEP_RAD)
+    rad.update(rkw)
+    rad.update(etatzai=O.rad_etatzai(W.LH2_CELL["Z"]), targ_A=W.LH2_CELL["A"])
+    hep = dict(W.HEP_ELASTIC)
+    hep.update(hkw)
+    return name, rad, hep
+
+
+def main_rad_case(i):
+    name, rad, hep = rad_settings(i)
+    out_dir = os.path.join(HERE, "gen")
+    tgt = O.make_target(**W.LH2_CELL)
+    tape = W.uniform_tape(RAD_N, RAD_STRIDE, RAD_SEED + i)
+    r, rc = O.ref_gen_hep_rad(tgt, O.make_hep(**hep), O.make_rad(**rad), RAD_N, tape, RAD_STRIDE)
+    assert rc == 0, (name, rc)
+    np.savez_compressed(os.path.join(out_dir, name + ".npz"), case=i, tape_seed=RAD_SEED + i, stride=RAD_STRIDE,
+                        tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(),
+                        settings=np.array(repr((rad, hep))), **{"out_" + k: v for k, v in r.items()})
+    print(name, "ok", int(r["ok"].sum()), "of", RAD_N, "tails", np.bincount(r["ntail"][r["ok"] == 1], minlength=4))
+
+
+def main_rad():
+    import subprocess
+    for i in range(len(RAD_CASES)):
+        subprocess.run([sys.executable, __file__, "rad_case", str(i)], check=True)
+
+
 # LFIT (cern/lfit.f, compiled into oracle/_ref as gfortran sees it): the
 # reference's A, B, E over seeded REAL*4 point sets (W.lfit_point_sets).
 LFIT_N = 12000
@@ -282,5 +339,9 @@ if __name__ == "__main__":
         main_field()
     elif sys.argv[1:] == ["lfit"]:
         main_lfit()
+    elif sys.argv[1:] == ["rad"]:
+        main_rad()
+    elif sys.argv[1:2] == ["rad_case"]:
+        main_rad_case(int(sys.argv[2]))
     else:
         main(sys.argv[1] if sys.argv[1:] else None)
