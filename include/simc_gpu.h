@@ -399,6 +399,53 @@ int simc_gen_hep(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h
 int simc_gen_hep_host(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h, simc_hep_out *out,
                       int64_t n, const simc_rng *rng);
 
+/* ---- Radiative tails of H(e,e'p) elastic (radc.f, brem.f, init.f) ----
+ * simc_gen_hep_rad is generate with using_rad for doing_hyd_elast
+ * (event.f:126-428 calling generate_rad, radc.f:120-517): simc_gen_hep's
+ * event, then complete_ev's target losses of the scattered electron and
+ * proton (event.f:1031-1040, in the arms' windows), radc_init_ev
+ * (init.f:655-720), the tail choice and the photons of generate_rad with
+ * basicrad, peaked_rad_weight / extrad_phi and brem / bremos / schwinger,
+ * and complete_main's weight = jacobian x gen_weight x sigep.  The run
+ * constants that radc_init (init.f:576-651) and limits_init (init.f:91-470)
+ * give the reference are settings here. */
+typedef struct {
+    int32_t rad_flag;         /* 0..3 (radc.f:141-159)                          */
+    int32_t extrad_flag;      /* 1..3; 0 = radc_init's default (init.f:621-629) */
+    int32_t intcor_mode;      /* 0 schwinger, 1 brem / bremos                   */
+    int32_t use_expon;        /* brem's exponentiate (init.f:641-645)           */
+    int32_t use_offshell_rad; /* bremos instead of brem                         */
+    int32_t doing_tail[3];    /* from one_tail, dbase.f:494-511                 */
+    int32_t e_arm, p_arm;     /* spectrometers whose windows the electron / proton cross */
+    double dE_edge_test;      /* MeV                                             */
+    double Egamma_gen_max;    /* hardwired photon limit when > 0.01 (dbase.f:514) */
+    double edge_e_E[2];       /* edge%e%E min, max (MeV)                         */
+    double edge_p_E[2];       /* edge%p%E                                        */
+    double edge_Em[2];        /* edge%Em (limits_init leaves 0 for hydrogen)     */
+    double vertex_Trec[2];    /* VERTEXedge%Trec                                 */
+    double Egamma_tot_max;    /* init.f:409-421                                  */
+    double etatzai;           /* radc_init, init.f:634; <= 0: from the target's Z */
+} simc_rad_settings;
+
+/* Radiative outputs (device, each may be NULL): Egamma_used(1..3) (MeV),
+ * gen_weight = rad_weight / hardcorfac, the ORIG kinematics the arms see
+ * (Ein, e E and delta, p E, P and delta), complete_ev's last losses and
+ * radiation lengths of the electron (2) and the proton (3), hardcorfac,
+ * and the radiating tail (ntup%radarm: 1..3, 0 = all tails possible). */
+typedef struct {
+    double *Egamma1, *Egamma2, *Egamma3, *gen_weight;
+    double *orig_Ein, *orig_e_E, *orig_e_delta, *orig_p_E, *orig_p_P, *orig_p_delta;
+    double *eloss2, *teff2, *eloss3, *teff3, *hardcorfac;
+    int32_t *ntail;
+} simc_rad_out;
+
+/* out holds the VERTEX kinematics (after tail 1's photon: Ein is the beam
+ * energy at the vertex); ok = generate's success.  On a tape, the uniforms
+ * are consumed in the reference's order. */
+int simc_gen_hep_rad(simc_ctx *ctx, const simc_target *t, const simc_hep_settings *h,
+                     const simc_rad_settings *r, simc_hep_out *out, simc_rad_out *rout, int64_t n,
+                     const simc_rng *rng, void *stream);
+
 /* ---- Target-field tracking (polarized-target field, trg_track.f) ----
  * simc_field_create = trgInit(map, theta_e, phi_e, theta_p, phi_p)
  * (trg_track.f:243-325): map = NULL or blank for the uniform 5 T test field,

@@ -3448,6 +3448,22 @@ int or_gen_hep(const or_target *T, const or_hep *H, int64_t n, or_hep_out *out, 
 /* Radiative tails: H(e,e'p) elastic with using_rad                           */
 /* ========================================================================= */
 
+/* OR_RAD_PERTURB (a conditioning probe for the tests, never the checker):
+ * every elementary function of the radiative code returns its result moved
+ * by one ulp, to measure how far last-bit differences of a libm travel. */
+#ifdef OR_RAD_PERTURB
+static double or_ulp_up(double x) { return nextafter(x, INFINITY); }
+#define log(x) or_ulp_up(log(x))
+#define exp(x) or_ulp_up(exp(x))
+#define pow(x, y) or_ulp_up(pow(x, y))
+#define tgamma(x) or_ulp_up(tgamma(x))
+#define sin(x) or_ulp_up(sin(x))
+#define cos(x) or_ulp_up(cos(x))
+#define asin(x) or_ulp_up(asin(x))
+#define acos(x) or_ulp_up(acos(x))
+#define atan2(x, y) or_ulp_up(atan2(x, y))
+#endif
+
 /* constants.inc:38-44 (alpi = alpha/pi of the rounded alpha) */
 #define OR_C_AMU 931.49432
 static const double or_c_alpi = (1. / 137.0359895) / 3.141592653589793;
@@ -4143,3 +4159,15 @@ int or_gen_hep_rad(const or_target *T, const or_hep *H, const or_rad *R, int64_t
     }
     return overflow ? -2 : 0;
 }
+
+#ifdef OR_RAD_PERTURB
+#undef log
+#undef exp
+#undef pow
+#undef tgamma
+#undef sin
+#undef cos
+#undef asin
+#undef acos
+#undef atan2
+#endif

@@ -67,7 +67,7 @@ EXPORTS = [
     "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_batch_passes", "simc_adjoint_run", "simc_recon_coefficients",
-    "simc_target_prep", "simc_target_prep_host", "simc_gen_hep", "simc_gen_hep_host",
+    "simc_target_prep", "simc_target_prep_host", "simc_gen_hep", "simc_gen_hep_host", "simc_gen_hep_rad",
     "simc_field_create", "simc_field_destroy", "simc_track_from_tgt", "simc_track_to_tgt",
     "simc_track_from_tgt_host", "simc_track_to_tgt_host",
     "simc_ntuple_open", "simc_ntuple_write", "simc_ntuple_write_device", "simc_ntuple_close", "simc_stop_report",
@@ -172,6 +172,44 @@ class HepOut(C.Structure):
     _fields_ = [(n, _vp) for n in HEP_OUT_FIELDS] + [("ok", _vp), ("used", _vp)]
 
 
+class RadSettings(C.Structure):
+    """simc_rad_settings: the radiative options and run constants that
+    generate_rad reads (radc.f; radc_init and limits_init of init.f)."""
+    _fields_ = [("rad_flag", C.c_int32), ("extrad_flag", C.c_int32), ("intcor_mode", C.c_int32),
+                ("use_expon", C.c_int32), ("use_offshell_rad", C.c_int32), ("doing_tail", C.c_int32 * 3),
+                ("e_arm", C.c_int32), ("p_arm", C.c_int32), ("dE_edge_test", C.c_double),
+                ("Egamma_gen_max", C.c_double), ("edge_e_E", C.c_double * 2), ("edge_p_E", C.c_double * 2),
+                ("edge_Em", C.c_double * 2), ("vertex_Trec", C.c_double * 2), ("Egamma_tot_max", C.c_double),
+                ("etatzai", C.c_double)]
+
+
+def rad_settings(**kw):
+    """RadSettings from keywords (pairs as (min, max), doing_tail as three
+    flags); etatzai absent or <= 0: from the target's Z."""
+    r = RadSettings()
+    for k, v in kw.items():
+        if k == "doing_tail":
+            for i in range(3):
+                r.doing_tail[i] = int(v[i])
+        elif k in ("edge_e_E", "edge_p_E", "edge_Em", "vertex_Trec"):
+            getattr(r, k)[0], getattr(r, k)[1] = v
+        elif k in ("rad_flag", "extrad_flag", "intcor_mode", "use_expon", "use_offshell_rad", "e_arm", "p_arm"):
+            setattr(r, k, int(v))
+        elif k in ("dE_edge_test", "Egamma_gen_max", "Egamma_tot_max", "etatzai"):
+            setattr(r, k, float(v))
+        elif k != "targ_A":  # the target carries A
+            raise KeyError(k)
+    return r
+
+
+RAD_OUT_FIELDS = ("Egamma1", "Egamma2", "Egamma3", "gen_weight", "orig_Ein", "orig_e_E", "orig_e_delta",
+                  "orig_p_E", "orig_p_P", "orig_p_delta", "eloss2", "teff2", "eloss3", "teff3", "hardcorfac")
+
+
+class RadOut(C.Structure):
+    _fields_ = [(n, _vp) for n in RAD_OUT_FIELDS] + [("ntail", _vp)]
+
+
 class SimcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"simc error {code}: {msg}")
@@ -214,6 +252,8 @@ def library(path: str = LIB_PATH):
     L.simc_recon_coefficients.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.simc_gen_hep.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(HepOut), C.c_int64,
                                C.POINTER(Rng), _vp]
+    L.simc_gen_hep_rad.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(RadSettings),
+                                   C.POINTER(HepOut), C.POINTER(RadOut), C.c_int64, C.POINTER(Rng), _vp]
     L.simc_gen_hep_host.argtypes = [_vp, C.POINTER(Target), C.POINTER(HepSettings), C.POINTER(HepOut), C.c_int64,
                                     C.POINTER(Rng)]
     L.simc_target_prep.argtypes = [_vp, C.POINTER(Target), C.POINTER(PrepSettings), C.POINTER(VertexIn),
@@ -484,6 +524,38 @@ class Spectrometers:
         return out
 
     # -- event generator -----------------------------------------------------------------
+    def gen_hep_rad(self, target, hep, rad, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0, out=None,
+                    device=None, stream=None):
+        """H(e,e'p) elastic events with radiative tails (simc_gen_hep_rad):
+        the VERTEX kinematics (HEP_OUT_FIELDS), the photons and ORIG
+        kinematics (RAD_OUT_FIELDS), ntail (int32), ok (uint8), used
+        (int32) as device tensors, or fills `out`."""
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if out is None:
+            out = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in HEP_OUT_FIELDS + RAD_OUT_FIELDS}
+            out["ntail"] = torch.empty(n, dtype=torch.int32, device=dev)
+            out["ok"] = torch.empty(n, dtype=torch.uint8, device=dev)
+            out["used"] = torch.empty(n, dtype=torch.int32, device=dev)
+        for k in HEP_OUT_FIELDS + RAD_OUT_FIELDS:
+            if out.get(k) is not None:
+                _dev_tensor(out[k], n, k, torch.float64)
+        for k, dt in (("ok", torch.uint8), ("used", torch.int32), ("ntail", torch.int32)):
+            if out.get(k) is not None:
+                _dev_tensor(out[k], n, k, dt)
+        O = HepOut(*[_ptr(out.get(k)) for k in HEP_OUT_FIELDS + ("ok", "used")])
+        RO = RadOut(*[_ptr(out.get(k)) for k in RAD_OUT_FIELDS + ("ntail",)])
+        if rng == "tape":
+            if tape is None:
+                raise ValueError("tape rng needs tape and stride")
+            _check_tape(tape, n, stride)
+            R = Rng(RNG_TAPE, 0, 0, event0, _ptr(tape), int(stride))
+        else:
+            R = Rng(RNG_PHILOX, 0, int(seed), int(event0), None, 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _check(library().simc_gen_hep_rad(self._h, C.byref(target), C.byref(hep), C.byref(rad), C.byref(O),
+                                          C.byref(RO), int(n), C.byref(R), C.c_void_p(st)))
+        return out
+
     def gen_hep(self, target, hep, n, *, rng="philox", seed=0, event0=0, tape=None, stride=0, out=None,
                 device=None, stream=None):
         """H(e,e'p) elastic events (simc_gen_hep): n vertices with their

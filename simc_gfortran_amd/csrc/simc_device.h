@@ -174,7 +174,9 @@ struct PhiloxStreamT {
     int overflow;
     int have;        // a second Box-Muller normal is pending
     double pending;
-    __device__ void init(uint64_t seed, uint64_t event, const double*, uint32_t, uint32_t)
+    // inlined: an outlined init would leave the key in memory, no longer
+    // provably uniform for the SGPR constraint of philox4x32_10
+    __device__ __forceinline__ void init(uint64_t seed, uint64_t event, const double*, uint32_t, uint32_t)
     {
         k0 = (uint32_t)seed; k1 = (uint32_t)(seed >> 32);
         e0 = (uint32_t)event; e1 = (uint32_t)(event >> 32);
@@ -261,7 +263,7 @@ struct TapeStream {
     int overflow;
     int have;          // unused (kept for a common state layout)
     double pending;
-    __device__ void init(uint64_t, uint64_t, const double* t, uint32_t slot_off, uint32_t s)
+    __device__ __forceinline__ void init(uint64_t, uint64_t, const double* t, uint32_t slot_off, uint32_t s)
     {
         tape = t; base = slot_off; stride = s; pos = 0; overflow = 0; have = 0; pending = 0.0;
     }

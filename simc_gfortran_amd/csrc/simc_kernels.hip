@@ -25,6 +25,7 @@
 #include "simc_target.h"
 #include "simc_field.h"
 #include "simc_gen.h"
+#include "simc_rad.h"
 #include "cosy_gen.dev.inc"
 
 using namespace simcdev;
@@ -1286,6 +1287,54 @@ __global__ __launch_bounds__(256) void gen_hep_kernel(GenArgs a)
 #pragma unroll
     for (int k = 0; k < kHepCols; ++k)
         if (a.col[k]) a.col[k][i] = g.overflow ? nan : c[k];
+    if (a.ok) a.ok[i] = (v.ok && !g.overflow) ? 1 : 0;
+    if (a.used) a.used[i] = g.overflow ? -1 : g.used();
+}
+
+// The generator with radiative tails: the beam's path (Pb), the electron's
+// and the proton's window losses (Pe, Pp), the radiative run constants.
+constexpr int kRadCols = 15;
+struct GenRadArgs {
+    HepDev H;
+    PrepDev Pb, Pe, Pp;
+    RadDev Rd;
+    uint64_t seed, event0;
+    uint32_t n, stride_bytes;
+    const double* tape;
+    double* col[kHepCols];
+    double* rcol[kRadCols];
+    int32_t* ntail;
+    uint8_t* ok;
+    int32_t* used;
+};
+
+template <class R>
+__global__ __launch_bounds__(256) void gen_hep_rad_kernel(GenRadArgs a)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    R g;
+    g.init(a.seed, a.event0 + i, a.tape, i * a.stride_bytes, a.stride_bytes);
+    HepEvent v;
+    RadOut ro;
+    RadEvent ev;
+    double hardcorfac;
+    gen_hep_rad(g, a.H, a.Pb, a.Pe, a.Pp, a.Rd, v, ro, ev, hardcorfac);
+    const double nan = __builtin_nan("");
+    const double c[kHepCols] = {v.x, v.y, v.z, v.rasterx, v.rastery, v.Ein, v.eloss, v.teff,
+                                v.e_xptar, v.e_yptar, v.e_theta, v.e_phi, v.e_E, v.e_delta,
+                                v.p_xptar, v.p_yptar, v.p_theta, v.p_phi, v.p_P, v.p_E, v.p_delta,
+                                v.Q2, v.nu, v.q, v.jacobian, v.sigcc, v.weight};
+#pragma unroll
+    for (int k = 0; k < kHepCols; ++k)
+        if (a.col[k]) a.col[k][i] = g.overflow ? nan : c[k];
+    const double r[kRadCols] = {ro.Eg[0], ro.Eg[1], ro.Eg[2], ro.gen_weight, ro.orig_Ein, ro.orig_e_E,
+                                ro.orig_e_delta, ro.orig_p_E, ro.orig_p_P, ro.orig_p_delta, ev.eloss[1], ev.teff[1],
+                                ev.eloss[2], ev.teff[2], hardcorfac};
+#pragma unroll
+    for (int k = 0; k < kRadCols; ++k)
+        if (a.rcol[k]) a.rcol[k][i] = g.overflow ? nan : r[k];
+    if (a.ntail) a.ntail[i] = ro.ntail;
     if (a.ok) a.ok[i] = (v.ok && !g.overflow) ? 1 : 0;
     if (a.used) a.used[i] = g.overflow ? -1 : g.used();
 }
@@ -3058,6 +3107,69 @@ void launch_prep(bool tape, int narm, const PrepArgs& a, hipStream_t st)
     }
 }
 
+// simc_hep_settings -> the device form (setting constants with the
+// reference's libm calls)
+void make_hep(const simc_hep_settings* h, HepDev& H)
+{
+    std::memset(&H, 0, sizeof H);
+    H.Ebeam = h->Ebeam;
+    H.dEbeam = h->dEbeam;
+    H.xwid = h->xwid;
+    H.ywid = h->ywid;
+    H.xoffset = h->xoffset;
+    H.yoffset = h->yoffset;
+    H.fr1 = h->fr1;
+    H.fr2 = h->fr2;
+    H.coulomb = h->using_coulomb ? h->coulomb : 0.0;
+    H.e_yp0 = h->e_yptar[0];
+    H.e_ypw = h->e_yptar[1] - h->e_yptar[0];
+    H.e_xp0 = h->e_xptar[0];
+    H.e_xpw = h->e_xptar[1] - h->e_xptar[0];
+    H.e_cos = std::cos(h->e_theta);
+    H.e_sin = std::sin(h->e_theta);
+    H.e_sinph = std::sin(h->e_phi);
+    H.e_phi = h->e_phi;
+    H.e_P = h->e_P;
+    H.p_x0 = std::sin(h->p_theta) * std::cos(h->p_phi);
+    H.p_y0 = std::sin(h->p_theta) * std::sin(h->p_phi);
+    H.p_z0 = std::cos(h->p_theta);
+    H.p_P = h->p_P;
+    H.Mh = h->Mh;
+    H.Mh2 = h->Mh * h->Mh;
+    H.fr_pattern = h->fr_pattern;
+    H.using_eloss = h->using_eloss != 0;
+}
+
+void hep_columns(const simc_hep_out* out, double* cols[kHepCols])
+{
+    double* const c[kHepCols] = {out->x, out->y, out->z, out->rasterx, out->rastery, out->Ein, out->eloss,
+                                 out->teff, out->e_xptar, out->e_yptar, out->e_theta, out->e_phi, out->e_E,
+                                 out->e_delta, out->p_xptar, out->p_yptar, out->p_theta, out->p_phi,
+                                 out->p_P, out->p_E, out->p_delta, out->Q2, out->nu, out->q,
+                                 out->jacobian, out->sigcc, out->weight};
+    for (int k = 0; k < kHepCols; ++k) cols[k] = c[k];
+}
+
+// The window losses of a scattered particle for complete_ev's
+// trip_thru_target(narm, ...) (target.f:51-246): the spectrometer's windows
+// and the particle's mass.
+int scattered_prep(const simc_target* t, int narm, int spectrometer, double mass, double p_spec, int using_eloss,
+                   PrepDev& P)
+{
+    simc_prep_settings s;
+    std::memset(&s, 0, sizeof s);
+    s.narm = narm;
+    s.spectrometer = spectrometer;
+    s.using_eloss = using_eloss;
+    s.mass = mass;
+    s.p_spec = p_spec;
+    static const double present = 0.0;  // make_prep checks that the vertex columns exist
+    simc_vertex_in vin;
+    std::memset(&vin, 0, sizeof vin);
+    vin.x = vin.y = vin.z = vin.E = vin.p = vin.theta = vin.xptar = vin.yptar = vin.delta = &present;
+    return make_prep(t, &s, &vin, P);
+}
+
 }  // namespace
 
 int simc_target_prep(simc_ctx* ctx, const simc_target* t, const simc_prep_settings* s, const simc_vertex_in* in,
@@ -3199,33 +3311,7 @@ int simc_gen_hep(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h
     std::memset(&vin, 0, sizeof vin);
     vin.z = &z_present;
     if (int rc = make_prep(t, &s, &vin, a.P)) return rc;
-    HepDev& H = a.H;
-    H.Ebeam = h->Ebeam;
-    H.dEbeam = h->dEbeam;
-    H.xwid = h->xwid;
-    H.ywid = h->ywid;
-    H.xoffset = h->xoffset;
-    H.yoffset = h->yoffset;
-    H.fr1 = h->fr1;
-    H.fr2 = h->fr2;
-    H.coulomb = h->using_coulomb ? h->coulomb : 0.0;
-    H.e_yp0 = h->e_yptar[0];
-    H.e_ypw = h->e_yptar[1] - h->e_yptar[0];
-    H.e_xp0 = h->e_xptar[0];
-    H.e_xpw = h->e_xptar[1] - h->e_xptar[0];
-    H.e_cos = std::cos(h->e_theta);
-    H.e_sin = std::sin(h->e_theta);
-    H.e_sinph = std::sin(h->e_phi);
-    H.e_phi = h->e_phi;
-    H.e_P = h->e_P;
-    H.p_x0 = std::sin(h->p_theta) * std::cos(h->p_phi);
-    H.p_y0 = std::sin(h->p_theta) * std::sin(h->p_phi);
-    H.p_z0 = std::cos(h->p_theta);
-    H.p_P = h->p_P;
-    H.Mh = h->Mh;
-    H.Mh2 = h->Mh * h->Mh;
-    H.fr_pattern = h->fr_pattern;
-    H.using_eloss = h->using_eloss != 0;
+    make_hep(h, a.H);
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     const uint64_t stride_bytes = rng->kind == SIMC_RNG_TAPE ? (uint64_t)rng->stride * 8u : 0u;
@@ -3233,11 +3319,8 @@ int simc_gen_hep(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h
     if (stride_bytes) per = std::min<int64_t>(per, (int64_t)(0xffffffffull / stride_bytes));
     a.seed = rng->seed;
     a.stride_bytes = (uint32_t)stride_bytes;
-    double* const cols[kHepCols] = {out->x, out->y, out->z, out->rasterx, out->rastery, out->Ein, out->eloss,
-                                    out->teff, out->e_xptar, out->e_yptar, out->e_theta, out->e_phi, out->e_E,
-                                    out->e_delta, out->p_xptar, out->p_yptar, out->p_theta, out->p_phi,
-                                    out->p_P, out->p_E, out->p_delta, out->Q2, out->nu, out->q,
-                                    out->jacobian, out->sigcc, out->weight};
+    double* cols[kHepCols];
+    hep_columns(out, cols);
     for (int64_t at = 0; at < n; at += per) {
         const int64_t m = std::min(per, n - at);
         a.n = (uint32_t)m;
@@ -3249,6 +3332,110 @@ int simc_gen_hep(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h
         const dim3 G((unsigned)((m + 255) / 256)), B(256);
         if (rng->kind == SIMC_RNG_TAPE) hipLaunchKernelGGL((gen_hep_kernel<TapeStream>), G, B, 0, st, a);
         else hipLaunchKernelGGL((gen_hep_kernel<PhiloxStreamT<kGenTag>>), G, B, 0, st, a);
+        HIPCHK(hipGetLastError());
+    }
+    return SIMC_OK;
+}
+
+int simc_gen_hep_rad(simc_ctx* ctx, const simc_target* t, const simc_hep_settings* h, const simc_rad_settings* r,
+                     simc_hep_out* out, simc_rad_out* rout, int64_t n, const simc_rng* rng, void* stream)
+{
+    if (!ctx || !t || !h || !r || !out || !rng) return fail(SIMC_E_ARG, "simc_gen_hep_rad: NULL argument");
+    if (n < 0) return fail(SIMC_E_ARG, "negative event count");
+    if (n == 0) return SIMC_OK;
+    if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE)
+        return fail(SIMC_E_ARG, "unknown rng kind");
+    if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0 || rng->stride > (1 << 24)))
+        return fail(SIMC_E_ARG, "tape rng needs a tape and a stride in [1, 2^24]");
+    if (h->fr_pattern < 0 || h->fr_pattern > 3)
+        return fail(SIMC_E_ARG, "fr_pattern must be 0 (none), 1 (bedpost), 2 (circle) or 3 (flat square)");
+    if (!(h->Mh > 0.0) || !(h->e_P > 0.0) || !(h->p_P > 0.0))
+        return fail(SIMC_E_ARG, "Mh and the central momenta must be positive");
+    if (r->rad_flag < 0 || r->rad_flag > 3) return fail(SIMC_E_ARG, "rad_flag must be 0..3 (radc.f:141-159)");
+    if (r->extrad_flag < 0 || r->extrad_flag > 3) return fail(SIMC_E_ARG, "extrad_flag must be 0..3");
+    if (r->intcor_mode != 0 && r->intcor_mode != 1) return fail(SIMC_E_ARG, "intcor_mode must be 0 or 1");
+    // radc_init's default (init.f:621-629)
+    const int extrad = r->extrad_flag != 0 ? r->extrad_flag : (r->rad_flag == 0 ? 3 : 1);
+    // peaked_rad_weight with rad_flag 1 and the Friedrich form reads saved
+    // locals that no call of that run sets (radc.f:560-637)
+    if (r->rad_flag == 1 && extrad == 3)
+        return fail(SIMC_E_ARG, "rad_flag 1 with extrad_flag 3 reads undefined values in the reference");
+    GenRadArgs a;
+    std::memset(&a, 0, sizeof a);
+    {
+        simc_prep_settings s;
+        std::memset(&s, 0, sizeof s);
+        s.narm = SIMC_TARGET_BEAM;
+        s.spectrometer = 1;
+        s.using_eloss = h->using_eloss;
+        s.mass = kMe;
+        s.beam_energy = h->Ebeam;
+        static const double z_present = 0.0;
+        simc_vertex_in vin;
+        std::memset(&vin, 0, sizeof vin);
+        vin.z = &z_present;
+        if (int rc = make_prep(t, &s, &vin, a.Pb)) return rc;
+    }
+    if (int rc = scattered_prep(t, SIMC_TARGET_ELECTRON, r->e_arm, kMe, h->e_P, h->using_eloss, a.Pe)) return rc;
+    if (int rc = scattered_prep(t, SIMC_TARGET_HADRON, r->p_arm, h->Mh, h->p_P, h->using_eloss, a.Pp)) return rc;
+    make_hep(h, a.H);
+    RadDev& D = a.Rd;
+    D.rad_flag = r->rad_flag;
+    D.extrad_flag = extrad;
+    D.intcor_mode = r->intcor_mode;
+    D.use_expon = r->use_expon;
+    D.use_offshell_rad = r->use_offshell_rad != 0;
+    for (int k = 0; k < 3; ++k) D.doing_tail[k] = r->doing_tail[k] != 0;
+    D.hardwired = r->Egamma_gen_max > 0.01;
+    D.dE_edge_test = r->dE_edge_test;
+    D.Egamma_gen_max = r->Egamma_gen_max;
+    for (int k = 0; k < 2; ++k) {
+        D.edge_e_E[k] = r->edge_e_E[k];
+        D.edge_p_E[k] = r->edge_p_E[k];
+        D.edge_Em[k] = r->edge_Em[k];
+        D.vertex_Trec[k] = r->vertex_Trec[k];
+    }
+    D.Egamma_tot_max = r->Egamma_tot_max;
+    D.Egamma1_max = D.doing_tail[0] ? r->Egamma_tot_max : 0.0;  // init.f:422-424
+    if (r->etatzai > 0.0) {
+        D.etatzai = r->etatzai;
+    } else {  // radc_init with target_init's L1, L2 (init.f:19-24, :634)
+        double L1 = std::log(184.15) - std::log(t->Z) / 3.0, L2 = std::log(1194.) - 2. * std::log(t->Z) / 3.0;
+        if (t->Z == 1) {
+            L1 = 5.31;
+            L2 = 6.144;
+        }
+        D.etatzai = (12.0 + (t->Z + 1.) / (t->Z * L1 + L2)) / 9.0;
+    }
+    D.targ_A = t->A;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t stride_bytes = rng->kind == SIMC_RNG_TAPE ? (uint64_t)rng->stride * 8u : 0u;
+    int64_t per = (int64_t)1 << 30;
+    if (stride_bytes) per = std::min<int64_t>(per, (int64_t)(0xffffffffull / stride_bytes));
+    a.seed = rng->seed;
+    a.stride_bytes = (uint32_t)stride_bytes;
+    double* cols[kHepCols];
+    hep_columns(out, cols);
+    simc_rad_out none;
+    std::memset(&none, 0, sizeof none);
+    const simc_rad_out* ro = rout ? rout : &none;
+    double* const rcols[kRadCols] = {ro->Egamma1, ro->Egamma2, ro->Egamma3, ro->gen_weight, ro->orig_Ein,
+                                     ro->orig_e_E, ro->orig_e_delta, ro->orig_p_E, ro->orig_p_P, ro->orig_p_delta,
+                                     ro->eloss2, ro->teff2, ro->eloss3, ro->teff3, ro->hardcorfac};
+    for (int64_t at = 0; at < n; at += per) {
+        const int64_t m = std::min(per, n - at);
+        a.n = (uint32_t)m;
+        a.event0 = rng->event0 + (uint64_t)at;
+        a.tape = rng->kind == SIMC_RNG_TAPE ? rng->tape + at * rng->stride : nullptr;
+        for (int k = 0; k < kHepCols; ++k) a.col[k] = cols[k] ? cols[k] + at : nullptr;
+        for (int k = 0; k < kRadCols; ++k) a.rcol[k] = rcols[k] ? rcols[k] + at : nullptr;
+        a.ntail = ro->ntail ? ro->ntail + at : nullptr;
+        a.ok = out->ok ? out->ok + at : nullptr;
+        a.used = out->used ? out->used + at : nullptr;
+        const dim3 G((unsigned)((m + 255) / 256)), B(256);
+        if (rng->kind == SIMC_RNG_TAPE) hipLaunchKernelGGL((gen_hep_rad_kernel<TapeStream>), G, B, 0, st, a);
+        else hipLaunchKernelGGL((gen_hep_rad_kernel<PhiloxStreamT<kGenTag>>), G, B, 0, st, a);
         HIPCHK(hipGetLastError());
     }
     return SIMC_OK;

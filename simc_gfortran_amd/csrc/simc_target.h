@@ -90,10 +90,11 @@ __device__ __forceinline__ double eloss_material(R& g, const MaterialDev& m, con
     return E;
 }
 
-// trip_thru_target for narm 2/3 (target.f:71-246); returns the loss, sets radlen
+// trip_thru_target for narm 2/3 (target.f:71-246); returns the loss, sets
+// radlen.  Inlined: the stream's key must stay provably uniform.
 template <class R>
-__device__ double trip_scattered(R& g, const PrepDev& P, double zpos, double energy, double theta,
-                                 double& radlen)
+__device__ __forceinline__ double trip_scattered(R& g, const PrepDev& P, double zpos, double energy, double theta,
+                                                 double& radlen)
 {
     const double forward_path = (P.length / 2.0 - zpos) / fabs(cos(theta + P.angle_sign * P.angle));
     double s_target = forward_path;
@@ -154,7 +155,7 @@ __device__ double trip_scattered(R& g, const PrepDev& P, double zpos, double ene
 
 // trip_thru_target for the incoming beam (narm 1, target.f:33-50)
 template <class R>
-__device__ double trip_beam(R& g, const PrepDev& P, double zpos, double& radlen)
+__device__ __forceinline__ double trip_beam(R& g, const PrepDev& P, double zpos, double& radlen)
 {
     const double s_target = (P.length / 2.0 + zpos) / P.cos_angle;  // cos_angle = |cos(angle)|
     radlen = s_target / P.x0_cm[MAT_TARGET] + P.s_al / P.x0_cm[MAT_AL];

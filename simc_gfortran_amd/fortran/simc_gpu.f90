@@ -102,6 +102,23 @@ module simc_gpu
     type(c_ptr) :: sigcc = c_null_ptr, weight = c_null_ptr, ok = c_null_ptr, used = c_null_ptr
   end type
 
+  ! generate with using_rad (radc.f, brem.f): simc_rad_settings, simc_rad_out
+  type, bind(C) :: simc_rad_settings
+    integer(c_int32_t) :: rad_flag = 0, extrad_flag = 0, intcor_mode = 1, use_expon = 0
+    integer(c_int32_t) :: use_offshell_rad = 0, doing_tail(3) = 1, e_arm = 5, p_arm = 1
+    real(c_double)     :: dE_edge_test = 0, Egamma_gen_max = 0
+    real(c_double)     :: edge_e_E(2) = 0, edge_p_E(2) = 0, edge_Em(2) = 0, vertex_Trec(2) = 0
+    real(c_double)     :: Egamma_tot_max = 0, etatzai = 0
+  end type
+
+  type, bind(C) :: simc_rad_out
+    type(c_ptr) :: Egamma1 = c_null_ptr, Egamma2 = c_null_ptr, Egamma3 = c_null_ptr, gen_weight = c_null_ptr
+    type(c_ptr) :: orig_Ein = c_null_ptr, orig_e_E = c_null_ptr, orig_e_delta = c_null_ptr
+    type(c_ptr) :: orig_p_E = c_null_ptr, orig_p_P = c_null_ptr, orig_p_delta = c_null_ptr
+    type(c_ptr) :: eloss2 = c_null_ptr, teff2 = c_null_ptr, eloss3 = c_null_ptr, teff3 = c_null_ptr
+    type(c_ptr) :: hardcorfac = c_null_ptr, ntail = c_null_ptr
+  end type
+
   interface
     integer(c_int) function simc_gen_hep(ctx, t, h, hout, n, rng, stream) bind(C, name="simc_gen_hep")
       import :: c_int, c_int64_t, c_ptr, simc_target, simc_hep_settings, simc_hep_out, simc_rng
@@ -109,6 +126,20 @@ module simc_gpu
       type(simc_target), intent(in) :: t
       type(simc_hep_settings), intent(in) :: h
       type(simc_hep_out), intent(in) :: hout
+      integer(c_int64_t), value :: n
+      type(simc_rng), intent(in) :: rng
+    end function
+
+    integer(c_int) function simc_gen_hep_rad(ctx, t, h, r, hout, rout, n, rng, stream) &
+        bind(C, name="simc_gen_hep_rad")
+      import :: c_int, c_int64_t, c_ptr, simc_target, simc_hep_settings, simc_rad_settings, simc_hep_out, &
+                simc_rad_out, simc_rng
+      type(c_ptr), value :: ctx, stream
+      type(simc_target), intent(in) :: t
+      type(simc_hep_settings), intent(in) :: h
+      type(simc_rad_settings), intent(in) :: r
+      type(simc_hep_out), intent(in) :: hout
+      type(simc_rad_out), intent(in) :: rout
       integer(c_int64_t), value :: n
       type(simc_rng), intent(in) :: rng
     end function

@@ -32,3 +32,19 @@ def parity_report(gpu, ref, rtol, keys=None):
         worst[k] = float(d.max()) if d.size else 0.0
     return {"flag_mismatch": flag_mismatch, "n_both_ok": int(both.sum()), "bad": bad,
             "worst": worst}
+
+
+# Tolerances of the radiative generator's columns (relative, floor at the
+# column's scale 1): 4 x the largest change that moving every elementary
+# function of the radiative code (log, exp, pow, tgamma, sin, cos, asin,
+# acos, atan2) by one ulp produces in the oracle over 5e4 events
+# (tests/test_rad_oracle.py::test_rad_conditioning_envelope measures it).
+# The photon energies are hi * y**(1/g) with g ~ 0.02-0.05 built from
+# soft-photon deltas with cancellation, so last-bit differences of a libm
+# reach them amplified ~1e7; the vertex kinematics after tail 1 and the
+# deltas (E - P0)/P0 inherit them.  Columns not listed: 1e-12.
+RAD_TOL = {"Egamma1": 2e-8, "Egamma2": 2e-8, "Egamma3": 2e-8, "gen_weight": 2e-9,
+           "e_delta": 4e-9, "p_delta": 4e-9, "orig_e_delta": 4e-9, "orig_p_delta": 6e-9,
+           "Ein": 4e-11, "e_E": 4e-11, "p_P": 6e-11, "p_E": 6e-11, "Q2": 8e-11, "nu": 8e-11, "q": 6e-11,
+           "orig_e_E": 6e-11, "orig_p_E": 1.2e-10, "orig_p_P": 1.2e-10, "p_yptar": 2e-10, "p_theta": 1e-11,
+           "p_xptar": 4e-12, "eloss3": 1e-11, "weight": 4e-12}

@@ -127,3 +127,44 @@ def test_rad_tail_choice_follows_the_tail_fractions(O):
     f = np.bincount(nt, minlength=4)[1:] / n
     assert f[1] > f[0] > f[2] > 0
     assert abs(f.sum() - 1) < 1e-12
+
+
+def test_rad_conditioning_envelope(O, tmp_path):
+    """How far last-bit differences of the elementary functions travel: the
+    oracle built with every log, exp, pow, tgamma, sin, cos, asin, acos and
+    atan2 of the radiative code moved up by one ulp, against the oracle, over
+    5e4 events.  Flags and tails never change; each column's largest change
+    lies within a quarter of the GPU tolerance RAD_TOL (so that tolerance is
+    4 x this envelope), and the photon energies do move by more than 1e-10
+    (the tolerance is needed, not slack)."""
+    import ctypes as C
+    import subprocess
+    from tests._compare import RAD_TOL
+    src = os.path.join(os.path.dirname(HERE), "oracle", "simc_oracle.c")
+    lib = str(tmp_path / "liboracle_perturb.so")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-fopenmp", "-shared",
+                    "-DOR_RAD_PERTURB", "-o", lib, src, "-lm"], check=True)
+    T, H = O.make_target(**W.LH2_CELL), O.make_hep(**W.HEP_ELASTIC)
+    R = O.make_rad(**dict(W.HEP_RAD, etatzai=O.rad_etatzai(1.0), targ_A=1.0))
+    n = 50000
+    a, rc = O.gen_hep_rad(T, H, R, n, rng="philox", seed=23)
+    assert rc == 0
+    L = C.CDLL(lib)
+    L.or_gen_hep_rad.restype = C.c_int
+    L.or_gen_hep_rad.argtypes = O._rad_lib().or_gen_hep_rad.argtypes
+    b = {k: np.zeros(n) for k in O.HEP_OUT_FIELDS + O.RAD_OUT_FIELDS}
+    b["ok"] = np.zeros(n, np.uint8)
+    b["ntail"] = np.zeros(n, np.int32)
+    used = np.zeros(n, np.int32)
+    Oo = O.ORHepOut(*([O._p(b[k]) for k in O.HEP_OUT_FIELDS] + [O._p(b["ok"], O._u8p)]))
+    RO = O.ORRadOut(*([O._p(b[k]) for k in O.RAD_OUT_FIELDS] + [O._p(b["ntail"], O._i32p)]))
+    assert L.or_gen_hep_rad(C.byref(T), C.byref(H), C.byref(R), n, C.byref(Oo), C.byref(RO), 1, None, 0, 23, 0,
+                            O.default_threads(), O._p(used, O._i32p)) == 0
+    np.testing.assert_array_equal(a["ok"], b["ok"])
+    ok = a["ok"] == 1
+    np.testing.assert_array_equal(a["ntail"][ok], b["ntail"][ok])
+    env = {}
+    for k in O.HEP_OUT_FIELDS + O.RAD_OUT_FIELDS:
+        env[k] = float(np.max(np.abs(a[k][ok] - b[k][ok]) / (np.abs(a[k][ok]) + 1.0)))
+        assert env[k] <= RAD_TOL.get(k, 1e-12) / 4, (k, env[k])
+    assert min(env["Egamma1"], env["Egamma2"]) > 1e-10, env
