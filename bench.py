@@ -484,6 +484,145 @@ def bench_gen(ctx, n=10_000_000, chunks=10, reps=5):
                     "RMS of a 5e-4 Gaussian"}
 
 
+def _rad_vertices(g, particle, ok):
+    """The target step's inputs of one scattered particle with radiation: ORIG
+    energy and momentum (photons removed), the vertex energy for its losses,
+    the vertex angles; events the generator rejected get their vertex values
+    (they never count)."""
+    import torch
+
+    def pick(orig, vert):
+        return torch.where(ok, g[orig], g[vert])
+    if particle == "e":
+        e = pick("orig_e_E", "e_E")
+        return {"x": g["x"], "y": g["y"], "z": g["z"], "E": e, "p": e, "E_vertex": g["e_E"],
+                "theta": g["e_theta"], "xptar": g["e_xptar"], "yptar": g["e_yptar"]}
+    return {"x": g["x"], "y": g["y"], "z": g["z"], "E": pick("orig_p_E", "p_E"), "p": pick("orig_p_P", "p_P"),
+            "E_vertex": g["p_E"], "theta": g["p_theta"], "delta": pick("orig_p_delta", "p_delta"),
+            "xptar": g["p_xptar"], "yptar": g["p_yptar"]}
+
+
+def bench_c2(ctx, n=10_000_000, chunks=10, reps=3):
+    """The coincidence configuration as specified, radiative tails included:
+    H(e,e'p) elastic at 10.6 GeV (10 cm LH2, electron in the SHMS at 12.5 deg,
+    proton in the HMS), simc_gen_hep_rad with the reference's H(e,e'p)
+    radiative options (rad_flag 0, extrad_flag 2, off-shell bremos) -> target
+    step of the electron and the proton on their ORIG kinematics -> SHMS and
+    HMS transport + reconstruction, over chunks x n = 1e8 generated events."""
+    import torch
+    from simc_gfortran_amd import api, workloads as W
+    dev = torch.device("cuda:0")
+    Hs = W.HEP_ELASTIC
+    H = api.hep_settings(**Hs)
+    Rs = api.rad_settings(**W.HEP_RAD)
+    T = api.Target(**W.LH2_CELL)
+    Pe = api.PrepSettings(2, 5, 1, 1, Hs["e_P"], Hs["e_theta"], Hs["e_phi"], W.ME, 0.0, Hs["Ebeam"],
+                          (api.C.c_double * 5)())
+    Pp = api.PrepSettings(3, 1, 1, 1, Hs["p_P"], Hs["p_theta"], Hs["p_phi"], W.MP, 0.0, Hs["Ebeam"],
+                          (api.C.c_double * 5)())
+    cols = ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")
+    pe = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in cols}
+    pp = {k: torch.empty(n, dtype=torch.float64, device=dev) for k in cols}
+    g = None
+    tg = []
+    for r in range(reps + 1):
+        a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+        a.record()
+        g = ctx.gen_hep_rad(T, H, Rs, n, seed=2001, out=g)
+        b.record()
+        torch.cuda.synchronize()
+        if r:
+            tg.append(a.elapsed_time(b))
+    gen_ok = float(g["ok"].float().mean())
+    oe = op = None
+    main = torch.cuda.current_stream(dev)
+    s_e, s_p = torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)
+
+    def chunk(k):
+        nonlocal g, oe, op
+        e0 = k * n
+        g = ctx.gen_hep_rad(T, H, Rs, n, seed=2001, event0=e0, out=g)
+        ok = g["ok"] == 1
+        ve, vp = _rad_vertices(g, "e", ok), _rad_vertices(g, "p", ok)
+        s_e.wait_stream(main)
+        s_p.wait_stream(main)
+        with torch.cuda.stream(s_e):
+            ctx.target_prep(T, Pe, ve, rng="philox", seed=2001, event0=e0, out=pe)
+            oe = ctx.run(5, pe, rng="philox", seed=2003, event0=e0, fit="fp64", out=oe, p_spec=Hs["e_P"],
+                         m2=W.ME ** 2, wire_sigma=0.03)
+        with torch.cuda.stream(s_p):
+            ctx.target_prep(T, Pp, vp, rng="philox", seed=2002, event0=e0, out=pp)
+            op = ctx.run(1, pp, rng="philox", seed=2004, event0=e0, fit="fp64", out=op, p_spec=Hs["p_P"],
+                         m2=W.MP ** 2)
+        main.wait_stream(s_e)
+        main.wait_stream(s_p)
+        both = ok & (oe["ok"] == 1) & (op["ok"] == 1)
+        return both.sum(), torch.where(both, g["weight"], torch.zeros_like(g["weight"])).sum()
+
+    chunk(0)  # warmup
+    torch.cuda.synchronize()
+    a, b = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    a.record()
+    sums = [chunk(k) for k in range(chunks)]
+    b.record()
+    torch.cuda.synchronize()
+    c_ms = a.elapsed_time(b)
+    g_ms = float(np.mean(tg))
+    ncoin = int(sum(int(c) for c, _ in sums))
+    wsum = float(sum(float(w) for _, w in sums))
+    total = n * chunks
+    return {"workload": "hep_elastic_rad_10p6GeV_shms12p5_hms_lh2", "gen_events": n, "gen_ms": g_ms,
+            "gen_events_per_s": n / (g_ms * 1e-3), "gen_ok_fraction": gen_ok,
+            "chain_events": total, "chain_ms": c_ms, "chain_events_per_s": total / (c_ms * 1e-3),
+            "coincidences": ncoin, "coincidence_fraction": ncoin / total, "weighted_coincidences": wsum,
+            "data": "synthetic: generated per event (Philox), radiative tails (rad_flag 0, extrad_flag 2, "
+                    "bremos), beam spread uniform with the RMS of a 5e-4 Gaussian, Em edge from a 0-200 MeV cut"}
+
+
+def cpu_c2(seconds=10.0):
+    """The radiated coincidence chain on the CPU: the oracle's generator with
+    tails, target step of both particles and both arms, one thread and the
+    host cores."""
+    from oracle import oracle as O
+    from simc_gfortran_amd import workloads as W
+    Hs = W.HEP_ELASTIC
+    T, H = O.make_target(**W.LH2_CELL), O.make_hep(**Hs)
+    R = O.make_rad(**dict(W.HEP_RAD, etatzai=O.rad_etatzai(W.LH2_CELL["Z"]), targ_A=W.LH2_CELL["A"]))
+    Pe = O.make_prep(2, 5, True, True, p_spec=Hs["e_P"], th_spec=Hs["e_theta"], phi=Hs["e_phi"], mass=W.ME,
+                     beam_energy=Hs["Ebeam"])
+    Pp = O.make_prep(3, 1, True, True, p_spec=Hs["p_P"], th_spec=Hs["p_theta"], phi=Hs["p_phi"], mass=W.MP,
+                     beam_energy=Hs["Ebeam"])
+    threads, _ = host_threads()
+
+    def vert(g, particle):
+        ok = g["ok"] == 1
+        if particle == "e":
+            e = np.where(ok, g["orig_e_E"], g["e_E"])
+            return {"x": g["x"], "y": g["y"], "z": g["z"], "E": e, "p": e, "E_vertex": g["e_E"], "theta": g["e_theta"],
+                    "xptar": g["e_xptar"], "yptar": g["e_yptar"]}
+        return {"x": g["x"], "y": g["y"], "z": g["z"], "E": np.where(ok, g["orig_p_E"], g["p_E"]),
+                "p": np.where(ok, g["orig_p_P"], g["p_P"]), "E_vertex": g["p_E"], "theta": g["p_theta"],
+                "delta": np.where(ok, g["orig_p_delta"], g["p_delta"]), "xptar": g["p_xptar"], "yptar": g["p_yptar"]}
+
+    rates = {}
+    for nt in (1, threads):
+        m, t = 0, 0.0
+        sub = 20_000 if nt == 1 else 200_000
+        while t < seconds / 2 or m == 0:
+            t0 = time.perf_counter()
+            g, rc = O.gen_hep_rad(T, H, R, sub, rng="philox", seed=2001, event0=m, nthreads=nt)
+            for part, P, arm, p0, m2, wire, sd in (("e", Pe, 5, Hs["e_P"], W.ME ** 2, 0.03, 2003),
+                                                   ("p", Pp, 1, Hs["p_P"], W.MP ** 2, 0.0, 2004)):
+                ev, rc2 = O.target_prep(T, P, vert(g, part), rng="philox", seed=sd - 2, event0=m, nthreads=nt)
+                O.run(arm, ev, p_spec=p0, m2=m2, wire_sigma=wire, fit_f32=False, rng="philox", seed=sd, event0=m,
+                      nthreads=nt)
+            t += time.perf_counter() - t0
+            m += sub
+        rates[nt] = m / t
+    return {"cpu_chain_events_per_s_1_thread": rates[1], "cpu_chain_events_per_s": rates[threads],
+            "cpu_cores": threads}
+
+
 def bench_host_entry(ctx, wl, ev, reps=3):
     """The headline workload through the host-array entry point
     (simc_arm_run_host): inputs copied from pageable host memory, outputs
@@ -742,6 +881,7 @@ def main():
         line["target_field"] = bench_field(ctx)
         line["other_arms"] = bench_arms(ctx)
         line["generator"] = bench_gen(ctx)
+        line["c2_radiative_chain"] = bench_c2(ctx)
         line["host_entry"] = bench_host_entry(ctx, wl, ev)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
@@ -749,6 +889,12 @@ def main():
         line["gpu_over_cpu"] = value / cb["value"]
         if "reference_estimate_events_per_s" in cb:
             line["gpu_over_reference_estimate"] = value / cb["reference_estimate_events_per_s"]
+        if "c2_radiative_chain" in line:
+            c2 = cpu_c2()
+            line["c2_radiative_chain"].update(c2)
+            r = line["c2_radiative_chain"]
+            r["gpu_over_cpu"] = r["chain_events_per_s"] / c2["cpu_chain_events_per_s"]
+            r["gpu_over_cpu_1_thread"] = r["chain_events_per_s"] / c2["cpu_chain_events_per_s_1_thread"]
         if "c0" in line:
             c0 = cpu_c0()
             line["c0"].update(c0)
