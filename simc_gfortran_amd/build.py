@@ -43,6 +43,7 @@ def build(verbose: bool = True) -> str:
     summary = gen_cosy.generate(MAP_DIR, os.path.join(GEN, "cosy_gen"))
     if verbose:
         print("gen_cosy:", summary)
+    gen_hut(verbose)
     srcs = [os.path.join(CSRC, "simc_kernels.hip"), os.path.join(CSRC, "simc_io.hip"),
             os.path.join(CSRC, "simc_maps.cpp")]
     cmd = [HIPCC, *HIP_FLAGS, "-I", GEN, "-I", CSRC, "-I", os.path.join(REPO, "include"),
@@ -54,6 +55,20 @@ def build(verbose: bool = True) -> str:
     build_dropin(verbose)
     build_fortran(verbose)
     return LIB
+
+
+def gen_hut(verbose: bool = True) -> str:
+    """_gen/hut_gen.inc: the detector huts of the step programs as
+    straight-line device code (gen_hut.cpp, built and run on the host)."""
+    exe = os.path.join(GEN, "gen_hut")
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"),
+           os.path.join(CSRC, "gen_hut.cpp"), "-o", exe]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    out = os.path.join(GEN, "hut_gen.inc")
+    subprocess.run([exe, out], check=True)
+    return out
 
 
 def build_dropin(verbose: bool = True) -> str:

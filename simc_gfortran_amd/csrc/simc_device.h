@@ -71,8 +71,16 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     // (v_readlane) in every block.  Made opaque here, they are recomputed by
     // scalar adds in each call instead.  A per-lane key (events of several
     // settings in one wavefront) is kept opaque in VGPRs for the same reason.
-    if (LANE_KEY) asm("" : "+v"(k0), "+v"(k1));
-    else asm("" : "+s"(k0), "+s"(k1));
+    // A launch-uniform key read back from memory (a stream passed to an
+    // outlined function) arrives in VGPRs: readfirstlane moves it to SGPRs
+    // (a copy where it is already there).
+    if (LANE_KEY) {
+        asm("" : "+v"(k0), "+v"(k1));
+    } else {
+        k0 = __builtin_amdgcn_readfirstlane(k0);
+        k1 = __builtin_amdgcn_readfirstlane(k1);
+        asm("" : "+s"(k0), "+s"(k1));
+    }
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;

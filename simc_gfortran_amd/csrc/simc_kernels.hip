@@ -22,43 +22,18 @@
 #include "simc_device.h"
 #include "simc_maps.h"
 #include "simc_ops.h"
+#include "simc_programs.h"
 #include "simc_target.h"
 #include "simc_field.h"
 #include "simc_gen.h"
 #include "simc_rad.h"
 #include "cosy_gen.dev.inc"
+#include "hut_gen.inc"
 
 using namespace simcdev;
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Stop codes (one per exit point; 0 = accepted)
-// ---------------------------------------------------------------------------
-enum : int {
-    ST_OK = 0, ST_SLIT_HOR = 1, ST_SLIT_VERT = 2, ST_SLIT_OCT = 3, ST_COLL = 4,
-    // HMS
-    ST_H_Q1_IN = 10, ST_H_Q1_MID, ST_H_Q1_OUT, ST_H_Q2_IN, ST_H_Q2_MID, ST_H_Q2_OUT,
-    ST_H_Q3_IN, ST_H_Q3_MID, ST_H_Q3_OUT, ST_H_D1_IN, ST_H_D1_OUT, ST_H_PIPE0, ST_H_PIPE1,
-    ST_H_PIPE2, ST_H_PIPE3, ST_H_DC1, ST_H_DC2, ST_H_SCIN,
-    // SHMS
-    ST_S_HB_IN = 40, ST_S_HB_MEN, ST_S_HB_MEX, ST_S_HB_OUT, ST_S_Q1_IN, ST_S_Q1_MEN,
-    ST_S_Q1_MID, ST_S_Q1_MEX, ST_S_Q1_OUT, ST_S_Q2_IN, ST_S_Q2_MEN, ST_S_Q2_MID, ST_S_Q2_MEX,
-    ST_S_Q2_OUT, ST_S_Q3_IN, ST_S_Q3_MEN, ST_S_Q3_MID, ST_S_Q3_MEX, ST_S_Q3_OUT, ST_S_D1_IN,
-    ST_S_D1_FLR, ST_S_D1_MEN, ST_S_D1_MID1, ST_S_D1_MID2, ST_S_D1_MID3, ST_S_D1_MID4,
-    ST_S_D1_MID5, ST_S_D1_MID6, ST_S_D1_MID7, ST_S_D1_MEX, ST_S_D1_OUT, ST_S_DC1, ST_S_DC2,
-    ST_S_S1X, ST_S_S1Y, ST_S_S2X, ST_S_S2Y, ST_S_CAL, ST_S_CALFID,
-    // SOS
-    ST_SO_QUAD_IN = 100, ST_SO_QUAD_MID, ST_SO_QUAD_OUT, ST_SO_BM01_IN, ST_SO_BM01_OUT,
-    ST_SO_BM02_IN, ST_SO_BM02_OUT, ST_SO_EXIT_FLANGE, ST_SO_EXIT_APERTURE, ST_SO_EXIT_WINDOW,
-    ST_SO_DC1, ST_SO_DC2, ST_SO_SCIN,
-    // HRS (both arms)
-    ST_R_PIPE_65 = 120, ST_R_PIPE_80, ST_R_Q1_PRE, ST_R_Q1_IN, ST_R_Q1_MID, ST_R_Q1_OUT,
-    ST_R_Q1_POST, ST_R_Q2_PRE, ST_R_Q2_IN, ST_R_Q2_MID, ST_R_Q2_OUT, ST_R_Q2_POST1, ST_R_Q2_POST2,
-    ST_R_D1_PRE, ST_R_D1_IN, ST_R_D1_OUT, ST_R_D1_POST_R, ST_R_D1_POST_BOX, ST_R_Q3_PRE, ST_R_Q3_IN,
-    ST_R_Q3_MID, ST_R_Q3_OUT, ST_R_Q3_POST, ST_R_WINDOW, ST_R_DC1, ST_R_DC2, ST_R_S1, ST_R_S2,
-    ST_TAPE = 250, ST_BADOP = 251
-};
 constexpr int kHistBins = 256;
 constexpr int kBlock = 256;  // threads per workgroup (one event each)
 constexpr int kHistSlots = kHistBins + 3;  // + per-step collimator slit hits (hor, vert, oct)
@@ -315,19 +290,14 @@ __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, con
 }
 
 // ---------------------------------------------------------------------------
-// The step-program interpreter: one event of arm ARM, steps [pc_begin,
-// pc_end) of one stage.  The magnet stage's switch holds the drifts, COSY
-// segments, apertures and the collimator; the hut stage's switch the
-// materials, wire planes, fit and trigger.
-//
-// Only the track, the random stream, the tilted-plane point (xt, yt), resmult
-// and the scintillator count are carried from step to step.  Wire hits go to
-// LDS (hits[plane][lane]); the fitted focal-plane track is written to its
-// output arrays at the fit and read back where it is used again.
+// The step-program interpreter of the magnet stages: one event of arm ARM,
+// steps [pc_begin, pc_end) (drifts, COSY segments, apertures and the
+// collimator).  Only the track, the random stream and the tilted-plane point
+// (xt, yt) are carried from step to step.
 // ---------------------------------------------------------------------------
-template <int ARM, class R, bool RARE, int STAGE>
-__device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off, double* lds_hits,
-                           unsigned* coll_cnt, double& resmult, int pc_begin, int pc_end)
+template <int ARM, class R, bool RARE>
+__device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, unsigned* coll_cnt,
+                                           int pc_begin, int pc_end)
 {
     // The in-flight decay and the collimator stepping of pions/muons are
     // compiled only into the RARE variant, so the common variant does not
@@ -341,188 +311,219 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
     const coptr ops = (coptr)a.ops;
     const ctptr tab = (ctptr)a.tab;
     double xt = 0.0, yt = 0.0;
-    int scincount = 0;
     int st = ST_OK;
-    ThetaCache ms;
     for (int pc = pc_begin; pc < pc_end; ++pc) {
         const int code = ops[pc].code;
         if (code == OP_END) break;
         const double p0 = ops[pc].p[0], p1 = ops[pc].p[1];
-        if constexpr (STAGE == STAGE_MAGNETS) {
-            switch (code) {
-            case OP_PROJECT:
-                project(g, t, F, p0);
-                break;
-            case OP_TRANSP: {
-                const int cls = ops[pc].arg;
-                transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
-                                        double& o1, double& o2, double& o3, double& o4) {
-                    poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
-                });
-                break;
-            }
-            case OP_CHECK_R2:
-                if ((t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
-                break;
-            case OP_ROT_H:
-            case OP_ROT_V: {
-                const Rot r{p0, p1, ops[pc].p[2]};
-                xt = t.xs;
-                yt = t.ys;
-                if (code == OP_ROT_H) rotate_haxis(t, r, xt, yt);
-                else rotate_vaxis(t, r, xt, yt);
-                xt = xt + ops[pc].p[3];
-                yt = yt + ops[pc].p[4];
-                break;
-            }
-            case OP_CHECK_T_R2:
-                if ((xt * xt + yt * yt) > p0) st = ops[pc].stop;
-                break;
-            case OP_CHECK_T_HB:
-                if ((xt * xt > p0) || (yt > p1) || (yt < ops[pc].p[2])) st = ops[pc].stop;
-                break;
-            case OP_HIT_DIPOLE_T:
-                if (hms_hit_dipole(xt, yt)) st = ops[pc].stop;
-                break;
-            case OP_PIPE_T: {
-                const double dx = xt - p1, dy = yt - ops[pc].p[2];
-                if (((dx * dx + dy * dy) > p0) || (fabs(dy) > ops[pc].p[3])) st = ops[pc].stop;
-                break;
-            }
-            case OP_PIPE_S: {
-                const double dx = t.xs - p1, dy = t.ys - ops[pc].p[2];
-                if ((dx * dx + dy * dy) > p0) st = ops[pc].stop;
-                break;
-            }
-            case OP_COLLIMATOR:
-                if (F.coll && t.m2 > 100.0 * 100.0 && t.m2 < 200.0 * 200.0) {
-                    CollGeom cg;
-                    cg.h_entr = tab->coll.h_entr; cg.v_entr = tab->coll.v_entr;
-                    cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
-                    cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
-                    cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
-                    if (!coll_step(g, t, F, cg, F.p_spec, coll_cnt)) st = ST_COLL;
-                } else {
-                    const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
-                    int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
-                    if (!s) {
-                        project(g, t, F, ops[pc].p[6]);
-                        s = slit(t.xs, t.ys, ops[pc].p[2], ops[pc].p[3], x_off, y_off);
-                    }
-                    st = s;
+        switch (code) {
+        case OP_PROJECT:
+            project(g, t, F, p0);
+            break;
+        case OP_TRANSP: {
+            const int cls = ops[pc].arg;
+            transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
+                                    double& o1, double& o2, double& o3, double& o4) {
+                poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
+            });
+            break;
+        }
+        case OP_CHECK_R2:
+            if ((t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
+            break;
+        case OP_ROT_H:
+        case OP_ROT_V: {
+            const Rot r{p0, p1, ops[pc].p[2]};
+            xt = t.xs;
+            yt = t.ys;
+            if (code == OP_ROT_H) rotate_haxis(t, r, xt, yt);
+            else rotate_vaxis(t, r, xt, yt);
+            xt = xt + ops[pc].p[3];
+            yt = yt + ops[pc].p[4];
+            break;
+        }
+        case OP_CHECK_T_R2:
+            if ((xt * xt + yt * yt) > p0) st = ops[pc].stop;
+            break;
+        case OP_CHECK_T_HB:
+            if ((xt * xt > p0) || (yt > p1) || (yt < ops[pc].p[2])) st = ops[pc].stop;
+            break;
+        case OP_HIT_DIPOLE_T:
+            if (hms_hit_dipole(xt, yt)) st = ops[pc].stop;
+            break;
+        case OP_PIPE_T: {
+            const double dx = xt - p1, dy = yt - ops[pc].p[2];
+            if (((dx * dx + dy * dy) > p0) || (fabs(dy) > ops[pc].p[3])) st = ops[pc].stop;
+            break;
+        }
+        case OP_PIPE_S: {
+            const double dx = t.xs - p1, dy = t.ys - ops[pc].p[2];
+            if ((dx * dx + dy * dy) > p0) st = ops[pc].stop;
+            break;
+        }
+        case OP_COLLIMATOR:
+            if (F.coll && t.m2 > 100.0 * 100.0 && t.m2 < 200.0 * 200.0) {
+                CollGeom cg;
+                cg.h_entr = tab->coll.h_entr; cg.v_entr = tab->coll.v_entr;
+                cg.h_exit = tab->coll.h_exit; cg.v_exit = tab->coll.v_exit;
+                cg.x_off = tab->coll.x_off; cg.y_off = tab->coll.y_off;
+                cg.coll_radl = tab->coll.coll_radl; cg.step_size = tab->coll.step_size;
+                if (!coll_step(g, t, F, cg, F.p_spec, coll_cnt)) st = ST_COLL;
+            } else {
+                const double x_off = ops[pc].p[4], y_off = ops[pc].p[5];
+                int s = slit(t.xs, t.ys, p0, p1, x_off, y_off);
+                if (!s) {
+                    project(g, t, F, ops[pc].p[6]);
+                    s = slit(t.xs, t.ys, ops[pc].p[2], ops[pc].p[3], x_off, y_off);
                 }
-                break;
-            case OP_CHECK_T_BOX:
-                if ((yt > p0) || (-yt > p1) || (-xt > ops[pc].p[2]) || (-xt < ops[pc].p[3])) st = ops[pc].stop;
-                break;
-            case OP_CHECK_TRAP:
-                if ((fabs(t.xs) > p0) || fabs(t.ys) > (p1 + ops[pc].p[2] * (t.xs + p0))) st = ops[pc].stop;
-                break;
-            case OP_SHIFT:
-                t.xs = t.xs + p0 * t.dxdzs;
-                t.ys = t.ys + p0 * t.dydzs;
-                break;
-            case OP_CHECK_ABS:
-                if ((fabs(t.xs) > p0) || fabs(t.ys) > p1) st = ops[pc].stop;
-                break;
-            case OP_CHECK_R:
-                if (sqrt(t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
-                break;
-            case OP_CHECK_T_SLAB:
-                if ((fabs(xt - p0) > p1) || ((fabs(yt) + ops[pc].p[2] * xt) > ops[pc].p[3])) st = ops[pc].stop;
-                break;
-            default:
-                st = ST_BADOP;  // not a magnet step (excluded by the host's check)
-                break;
+                st = s;
             }
-        } else {
-            // Gaussian draws of this step, in call order (one gauss1 instance
-            // serves musc, musc_ext and the wire smearing).
-            int ng = 0;
-            if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
-            else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
-            else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-            if (ng) {  // ng = 2 or 4 draws; the last two go to (d2, d3)
-                if (ng == 4) g.gauss_pair(d0, d1);
-                g.gauss_pair(d2, d3);
-            }
-            switch (code) {
-            case OP_PROJECT:
-                project(g, t, F, p0);
-                break;
-            case OP_RESMULT_DRAW:
-                resmult = g.next() < p0 ? 2.0 : 1.0;
-                break;
-            case OP_RESMULT_ONE:
-                resmult = 1.0;
-                break;
-            case OP_MUSC:
-                if (ng) {  // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
-                    const double ts = ms.theta(t, p1, ops[pc].p[2]);
-                    t.dydzs = t.dydzs + ts * d2;
-                    t.dxdzs = t.dxdzs + ts * d3;
-                }
-                break;
-            case OP_MUSC_EXT:
-                if (ng) {  // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
-                    const double ts = ms.theta(t, ops[pc].p[2], ops[pc].p[3]);
-                    const double s12 = 3.4641016151377544;  // sqrt(12.)
-                    t.dxdzs = t.dxdzs + ts * d0;
-                    t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
-                    t.dydzs = t.dydzs + ts * d2;
-                    t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
-                }
-                break;
-            case OP_WIRE: {  // plane ops[pc].stop (0-based); y plane when arg = 1
-                const double h = ops[pc].arg ? t.ys + F.sigma * d3 * resmult : t.xs + F.sigma * d2 * resmult;
-                lds_hits[ops[pc].stop * kBlock + threadIdx.x] = h;
-                break;
-            }
-            case OP_BOX:
-                if (t.xs > p0 || t.xs < p1 || t.ys > ops[pc].p[2] || t.ys < ops[pc].p[3]) st = ops[pc].stop;
-                break;
-            case OP_SCIN:
-                if (t.ys < p0 && t.ys > p1 && t.xs < ops[pc].p[2] && t.xs > ops[pc].p[3]) ++scincount;
-                break;
-            case OP_TRIGGER:
-                if (scincount < ops[pc].arg) st = ops[pc].stop;
-                break;
-            case OP_FIT: {
-                double zdc[12];
-#pragma unroll
-                for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
-                double x_fp, dx_fp, y_fp, dy_fp;
-                fit_track<ARM>(F, lds_hits, zdc, x_fp, dx_fp, y_fp, dy_fp);
-                st64(a.x_fp, a.n_arr, off, x_fp);
-                st64(a.dx_fp, a.n_arr, off, dx_fp);
-                st64(a.y_fp, a.n_arr, off, y_fp);
-                st64(a.dy_fp, a.n_arr, off, dy_fp);
-                break;
-            }
-            case OP_BOX_ROT_H: {
-                const Rot r{p0, p1, ops[pc].p[2]};
-                double xr = t.xs, yr = t.ys;
-                rotate_haxis(t, r, xr, yr);
-                if (xr > ops[pc].p[3] || xr < ops[pc].p[4] || yr > ops[pc].p[5] || yr < ops[pc].p[6])
-                    st = ops[pc].stop;
-                break;
-            }
-            case OP_FIDUCIAL: {
-                const double xcal = ld64(a.x_fp, a.n_arr, off) + ld64(a.dx_fp, a.n_arr, off) * p0;
-                const double ycal = ld64(a.y_fp, a.n_arr, off) + ld64(a.dy_fp, a.n_arr, off) * p0;
-                if (ycal > p1 || ycal < ops[pc].p[2] || xcal > ops[pc].p[3] || xcal < ops[pc].p[4])
-                    st = ops[pc].stop;
-                break;
-            }
-            default:
-                st = ST_BADOP;  // not a hut step (excluded by the host's check)
-                break;
-            }
+            break;
+        case OP_CHECK_T_BOX:
+            if ((yt > p0) || (-yt > p1) || (-xt > ops[pc].p[2]) || (-xt < ops[pc].p[3])) st = ops[pc].stop;
+            break;
+        case OP_CHECK_TRAP:
+            if ((fabs(t.xs) > p0) || fabs(t.ys) > (p1 + ops[pc].p[2] * (t.xs + p0))) st = ops[pc].stop;
+            break;
+        case OP_SHIFT:
+            t.xs = t.xs + p0 * t.dxdzs;
+            t.ys = t.ys + p0 * t.dydzs;
+            break;
+        case OP_CHECK_ABS:
+            if ((fabs(t.xs) > p0) || fabs(t.ys) > p1) st = ops[pc].stop;
+            break;
+        case OP_CHECK_R:
+            if (sqrt(t.xs * t.xs + t.ys * t.ys) > p0) st = ops[pc].stop;
+            break;
+        case OP_CHECK_T_SLAB:
+            if ((fabs(xt - p0) > p1) || ((fabs(yt) + ops[pc].p[2] * xt) > ops[pc].p[3])) st = ops[pc].stop;
+            break;
+        default:
+            st = ST_BADOP;  // not a magnet step (excluded by the host's check)
+            break;
         }
         if (st) return st;
     }
     return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The detector hut, compiled.  gen_hut.cpp writes the hut steps of each arm
+// family as straight-line code (hut_gen.inc): one step<code, arg, stop>(k,
+// p0..p7) call per step k of the program, its parameters as literals where
+// they do not depend on the loaded maps.  The library checks every loaded
+// program against those tables (load_arm).  Below are the steps' semantics
+// (the materials of musc / musc_ext, the wire planes, boxes, scintillators,
+// trigger, fit and fiducial cut of the mc_*_hut routines).
+//
+// Carried from step to step: the track, the random stream, resmult, the
+// scintillator count and the fitted track.  Wire hits go to LDS
+// (hits[plane][lane]).
+// ---------------------------------------------------------------------------
+template <int>
+constexpr bool kNotHutStep = false;
+
+template <int ARM, class R, bool RARE>
+struct Hut {
+    R& g;
+    Track& t;
+    const KArgs& a;
+    Flags F;
+    uint32_t off;
+    double* lds_hits;
+    double& resmult;
+    int scincount;
+    ThetaCache ms;
+    double fp[4];  // fitted x_fp, x'_fp, y_fp, y'_fp
+
+    // parameter j of program step k, for the parameters that depend on the maps
+    __device__ __forceinline__ double runtime_p(int k, int j) const { return ((coptr)a.ops)[k].p[j]; }
+
+    template <int CODE, int ARG, int STOP>
+    __device__ __forceinline__ int step(int, double p0, double p1, double p2, double p3, double p4, double p5,
+                                        double p6, double)
+    {
+        if constexpr (CODE == OP_PROJECT) {
+            project(g, t, F, p0);
+        } else if constexpr (CODE == OP_RESMULT_DRAW) {
+            resmult = g.next() < p0 ? 2.0 : 1.0;
+        } else if constexpr (CODE == OP_RESMULT_ONE) {
+            resmult = 1.0;
+        } else if constexpr (CODE == OP_MUSC) {
+            // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
+            if (F.ms && p0 != 0.0) {
+                double d2, d3;
+                g.gauss_pair(d2, d3);
+                const double ts = ms.theta(t, p1, p2);
+                t.dydzs = t.dydzs + ts * d2;
+                t.dxdzs = t.dxdzs + ts * d3;
+            }
+        } else if constexpr (CODE == OP_MUSC_EXT) {
+            // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
+            if (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) {
+                double d0, d1, d2, d3;
+                g.gauss_pair(d0, d1);
+                g.gauss_pair(d2, d3);
+                const double ts = ms.theta(t, p2, p3);
+                const double s12 = 3.4641016151377544;  // sqrt(12.)
+                t.dxdzs = t.dxdzs + ts * d0;
+                t.xs = t.xs + ts * p1 * d1 / s12 + ts * p1 * d0 / 2.0;
+                t.dydzs = t.dydzs + ts * d2;
+                t.ys = t.ys + ts * p1 * d3 / s12 + ts * p1 * d2 / 2.0;
+            }
+        } else if constexpr (CODE == OP_WIRE) {
+            // plane STOP (0-based); a y plane when ARG = 1
+            double d2 = 0.0, d3 = 0.0;
+            if (F.wcs) g.gauss_pair(d2, d3);
+            const double h = ARG ? t.ys + F.sigma * d3 * resmult : t.xs + F.sigma * d2 * resmult;
+            lds_hits[STOP * kBlock + threadIdx.x] = h;
+        } else if constexpr (CODE == OP_BOX) {
+            if (t.xs > p0 || t.xs < p1 || t.ys > p2 || t.ys < p3) return STOP;
+        } else if constexpr (CODE == OP_SCIN) {
+            if (t.ys < p0 && t.ys > p1 && t.xs < p2 && t.xs > p3) ++scincount;
+        } else if constexpr (CODE == OP_TRIGGER) {
+            if (scincount < ARG) return STOP;
+        } else if constexpr (CODE == OP_FIT) {
+            const ctptr tab = (ctptr)a.tab;
+            double zdc[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) zdc[k] = tab->zdc[k];
+            fit_track<ARM>(F, lds_hits, zdc, fp[0], fp[1], fp[2], fp[3]);
+            st64(a.x_fp, a.n_arr, off, fp[0]);
+            st64(a.dx_fp, a.n_arr, off, fp[1]);
+            st64(a.y_fp, a.n_arr, off, fp[2]);
+            st64(a.dy_fp, a.n_arr, off, fp[3]);
+        } else if constexpr (CODE == OP_BOX_ROT_H) {
+            const Rot r{p0, p1, p2};
+            double xr = t.xs, yr = t.ys;
+            rotate_haxis(t, r, xr, yr);
+            if (xr > p3 || xr < p4 || yr > p5 || yr < p6) return STOP;
+        } else if constexpr (CODE == OP_FIDUCIAL) {
+            // track of the fit at z = p0
+            const double xcal = fp[0] + fp[1] * p0;
+            const double ycal = fp[2] + fp[3] * p0;
+            if (ycal > p1 || ycal < p2 || xcal > p3 || xcal < p4) return STOP;
+        } else {
+            static_assert(kNotHutStep<CODE>, "not a detector-hut step");
+        }
+        return ST_OK;
+    }
+};
+
+// One event through its arm's compiled hut: ST_OK or the stop code.
+template <int ARM, class R, bool RARE>
+__device__ __forceinline__ int run_hut(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off,
+                                       double* lds_hits, double& resmult)
+{
+    Hut<ARM, R, RARE> h{g, t, a, FA, off, lds_hits, resmult, 0, ThetaCache{}, {0.0, 0.0, 0.0, 0.0}};
+    if (!RARE) {  // in-flight decay only in the RARE variant (as in the magnets)
+        h.F.decay = 0;
+        h.F.coll = 0;
+    }
+    if constexpr (ARM == SIMC_ARM_HMS) return hut_steps_hms(h);
+    else if constexpr (ARM == SIMC_ARM_SHMS) return hut_steps_shms(h);
+    else if constexpr (ARM == SIMC_ARM_SOS) return hut_steps_sos(h);
+    else return hut_steps_hrs(h);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,24 +596,36 @@ __device__ __forceinline__ EvId event_id(const KArgs& a, const SetDev& S, uint32
     return e;
 }
 
-// Outputs of a finished event (byte offset off in the event arrays).
-template <int ARM, class R>
+// The reconstruction of an accepted event from the fitted focal plane in the
+// output arrays (byte offset off): the target quantities, and the HRS VDC
+// offset taken off y_fp afterwards.
+template <int ARM>
+__device__ __forceinline__ void reconstruct_event(const KArgs& a, uint32_t off)
+{
+    const uint32_t n = a.n_arr;
+    EvOut o, r;
+    o.x_fp = ld64(a.x_fp, n, off);
+    o.dx_fp = ld64(a.dx_fp, n, off);
+    o.y_fp = ld64(a.y_fp, n, off);
+    o.dy_fp = ld64(a.dy_fp, n, off);
+    recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
+    if (ARM == SIMC_ARM_HRSR) st64(a.y_fp, n, off, o.y_fp - ((ctptr)a.tab)->yfp_shift);
+    st64(a.o_dpp, n, off, r.dpp);
+    st64(a.o_y, n, off, r.y);
+    st64(a.o_dxdz, n, off, r.dxdz);
+    st64(a.o_dydz, n, off, r.dydz);
+}
+
+// Outputs of a finished event (byte offset off in the event arrays); an
+// accepted event is reconstructed here unless RECON_LATER (the hut stage
+// hands accepted events to recon_list_kernel).
+template <int ARM, class R, bool RECON_LATER = false>
 __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int st, const R& g, const Track& t,
                                              double resmult)
 {
     const uint32_t n = a.n_arr;
     if (st == ST_OK) {
-        EvOut o, r;
-        o.x_fp = ld64(a.x_fp, n, off);
-        o.dx_fp = ld64(a.dx_fp, n, off);
-        o.y_fp = ld64(a.y_fp, n, off);
-        o.dy_fp = ld64(a.dy_fp, n, off);
-        recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
-        if (ARM == SIMC_ARM_HRSR) st64(a.y_fp, n, off, o.y_fp - ((ctptr)a.tab)->yfp_shift);
-        st64(a.o_dpp, n, off, r.dpp);
-        st64(a.o_y, n, off, r.y);
-        st64(a.o_dxdz, n, off, r.dxdz);
-        st64(a.o_dydz, n, off, r.dydz);
+        if (!RECON_LATER) reconstruct_event<ARM>(a, off);
     } else {
         // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440)
         st64(a.o_dpp, n, off, ld64(a.dpp, n, off));
@@ -639,7 +652,9 @@ __device__ __forceinline__ void count_stop(const KArgs& a, unsigned* hist, uint3
 }
 
 // Stage kernel.  FROM_Q: events come from queue qin (else from the inputs);
-// TO_Q: survivors go to queue qout (else every event is finished here).
+// TO_Q: survivors go to queue qout (else every event is finished here); in
+// the hut stage, TO_Q lists the accepted events (index and setting) for
+// recon_list_kernel instead of reconstructing them here.
 // The common variants are held to 168 VGPRs (three wavefronts per SIMD),
 // which the hut stage would exceed slightly on its own.
 template <int ARM, class R, bool RARE, int STAGE, bool FROM_Q, bool TO_Q>
@@ -713,10 +728,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             t.dflag = 0;
             resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
         }
-        int st = run_program<ARM, R, RARE, STAGE>(g, t, a, F, e.off, lds_hits, coll_cnt, resmult, pc_begin,
-                                                   pc_end);
+        int st;
+        if constexpr (STAGE == STAGE_HUT) st = run_hut<ARM, R, RARE>(g, t, a, F, e.off, lds_hits, resmult);
+        else st = run_program<ARM, R, RARE>(g, t, a, F, coll_cnt, pc_begin, pc_end);
         if (g.overflow) st = ST_TAPE;
-        if (TO_Q && st == ST_OK) {
+        if (STAGE == STAGE_HUT) {
+            // TO_Q: accepted events go to the reconstruction list
+            finish_event<ARM, R, TO_Q>(a, e.off, st, g, t, resmult);
+            count_stop(a, hist, s_block, s, st, 1u);
+            survivor = TO_Q && st == ST_OK;
+        } else if (TO_Q && st == ST_OK) {
             survivor = true;
         } else {
             finish_event<ARM>(a, e.off, st, g, t, resmult);
@@ -747,25 +768,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             for (int w = 0; w < wave; ++w) slot += wave_n[w];
             const uint32_t c = qout.cap, o8 = slot * 8u, o4 = slot * 4u;
             st32(qout.idx, c, o4, li);
-            st32(qout.rng, c, o4, g.state_word());
             st32(qout.bits, c, o4, (uint32_t)t.dflag | (s << 1));
-            st64(qout.pend, c, o8, g.pending);
-            st64(qout.xs, c, o8, t.xs);
-            st64(qout.ys, c, o8, t.ys);
-            st64(qout.dxdzs, c, o8, t.dxdzs);
-            st64(qout.dydzs, c, o8, t.dydzs);
-            st64(qout.dpps, c, o8, t.dpps);
-            st64(qout.p, c, o8, t.p);
-            st64(qout.m2, c, o8, t.m2);
-            st64(qout.pathlen, c, o8, t.pathlen);
-            st64(qout.decdist, c, o8, t.decdist);
-            st64(qout.resmult, c, o8, resmult);
+            if (STAGE != STAGE_HUT) {
+                st32(qout.rng, c, o4, g.state_word());
+                st64(qout.pend, c, o8, g.pending);
+                st64(qout.xs, c, o8, t.xs);
+                st64(qout.ys, c, o8, t.ys);
+                st64(qout.dxdzs, c, o8, t.dxdzs);
+                st64(qout.dydzs, c, o8, t.dydzs);
+                st64(qout.dpps, c, o8, t.dpps);
+                st64(qout.p, c, o8, t.p);
+                st64(qout.m2, c, o8, t.m2);
+                st64(qout.pathlen, c, o8, t.pathlen);
+                st64(qout.decdist, c, o8, t.decdist);
+                st64(qout.resmult, c, o8, resmult);
+            }
         }
     }
     __syncthreads();
     const uint32_t hist_index = setting_of(a, s_block).hist;
     for (int k = threadIdx.x; k < kHistSlots; k += blockDim.x)
         if (hist[k]) atomicAdd(&a.hist[(size_t)hist_index * kHistSlots + k], (unsigned long long)hist[k]);
+}
+
+// Reconstruction of the accepted events the hut stage listed (one per
+// lane): the COSY reconstruction polynomial alone needs few registers, so it
+// runs at full occupancy here instead of holding the hut stage's.
+template <int ARM>
+__global__ __launch_bounds__(kBlock) void recon_list_kernel(KArgs a, Queue acc)
+{
+    const uint32_t n_in = *acc.count;
+    if (blockIdx.x * kBlock >= n_in) return;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_in) return;
+    const uint32_t li = ld32(acc.idx, acc.cap, i * 4u);
+    const uint32_t s = ld32(acc.bits, acc.cap, i * 4u) >> 1;
+    const SetDev S = setting_of(a, s);
+    reconstruct_event<ARM>(a, event_id(a, S, s, li).off);
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,9 +1183,10 @@ __global__ void adjoint_finish(double* g_coef, uint32_t n_coef, double* g_out, c
 
 // Queue counter reset (a kernel rather than a memset, so that a captured
 // launch sequence replays correctly).
-__global__ void zero_u32x2(uint32_t* p)
+// the two survivor queues' counters and the accepted list's
+__global__ void zero_queue_counts(uint32_t* p)
 {
-    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+    if (threadIdx.x < 3) p[threadIdx.x] = 0u;
 }
 
 __global__ void zero_u64(unsigned long long* p, uint32_t n)
@@ -1626,592 +1666,6 @@ int ncounters_of(int arm)
          : is_hrs(arm)          ? SIMC_HRS_NCOUNTERS : SIMC_SHMS_NCOUNTERS;
 }
 
-// Step-program builder.  All parameters are formed with the reference's own
-// expressions in double precision, so they equal the reference's values.
-struct Program {
-    std::vector<Op> ops;
-    Op& add(int code, std::initializer_list<double> p = {}, int stop = 0, int arg = 0)
-    {
-        Op o;
-        std::memset(&o, 0, sizeof o);
-        o.code = code;
-        o.stop = stop;
-        o.arg = arg;
-        int k = 0;
-        for (double v : p) o.p[k++] = v;
-        // multiple scattering steps also carry sqrt(rad_len) and log10(rad_len)
-        if (code == OP_MUSC || code == OP_MUSC_EXT) {
-            const double rl = o.p[0];
-            const int at = code == OP_MUSC ? 1 : 2;
-            o.p[at] = rl > 0.0 ? std::sqrt(rl) : 0.0;
-            o.p[at + 1] = rl > 0.0 ? std::log10(rl) : 0.0;
-        }
-        ops.push_back(o);
-        return ops.back();
-    }
-    // rotate_haxis / rotate_vaxis of the track into a tilted aperture plane
-    // (shared/rotate_haxis.f:41-51: truncated degree constant, libm trig of
-    // the product), then the aperture-centre offsets.
-    void rot(int code, double rotang, double xoff, double yoff)
-    {
-        const double raddeg = 0.017453292;
-        const double r = rotang * raddeg;
-        add(code, {std::tan(r), std::sin(r), std::cos(r), xoff, yoff});
-    }
-};
-
-const double kInf = HUGE_VAL;
-
-// Wire-chamber planes of one chamber: hms/mc_hms_hut.f:341-370 (the SHMS loop,
-// shms/mc_shms_hut.f:172-205, and the SOS loop, sos/mc_sos_hut.f:298-333, are
-// the same with their own constants).  Bit iplane-1 of y_planes marks the
-// planes that measure y: 2 and 5 in the HMS and SHMS, 1, 3 and 5 in the SOS.
-void chamber(Program& P, int jchamber, double cath_radw, double hdc_thick, double hdc_radlen,
-             double cath_thick, double wire_radw, double wire_thick, unsigned y_planes = 0x12u)
-{
-    for (int iplane = 1; iplane <= 6; ++iplane) {
-        P.add(OP_MUSC, {cath_radw});
-        double drift = 0.5 * hdc_thick;
-        P.add(OP_MUSC, {drift / hdc_radlen});
-        drift = drift + cath_thick;
-        P.add(OP_PROJECT, {drift});
-        P.add(OP_MUSC, {wire_radw});
-        P.add(OP_WIRE, {}, (jchamber - 1) * 6 + iplane - 1, (y_planes >> (iplane - 1)) & 1u);
-        drift = 0.5 * hdc_thick;
-        P.add(OP_MUSC, {drift / hdc_radlen});
-        drift = drift + wire_thick;
-        P.add(OP_PROJECT, {drift});
-    }
-}
-
-// mc_hms (hms/mc_hms.f:150-441) followed by mc_hms_hut (hms/mc_hms_hut.f:292-601)
-void build_hms(const simc::ForwardMap& f, Program& P, double zdc[12])
-{
-    const double z_off = +40.17, z_entr = 126.2e0 + z_off, z_exit = z_entr + 6.3e0;  // :84-89
-    const double z_dip1 = 64.77e0, z_dip2 = z_dip1 + 297.18e0, z_dip3 = z_dip2 + 115.57e0;  // :92-94
-    const double rq1 = 20.50 * 20.50, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;  // apertures_hms.inc:5-7
-    const double* dd = f.driftdist.data();  // driftdist(spectr, k) = dd[k-1]
-    P.add(OP_PROJECT, {z_entr});                                                        // :201-202
-    P.add(OP_COLLIMATOR, {4.575, 11.646, 4.759, 12.114, 0.000, +0.028, z_exit - z_entr});  // :206-254
-    P.add(OP_PROJECT, {dd[0] - z_exit});                                                // :259-260
-    P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_IN);
-    P.add(OP_TRANSP, {125.233e0}, 0, 2);  P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_MID);    // :268-272
-    P.add(OP_TRANSP, {62.617e0}, 0, 3);   P.add(OP_CHECK_R2, {rq1}, ST_H_Q1_OUT);    // :276-280
-    P.add(OP_PROJECT, {dd[3]});           P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_IN);     // :285-290
-    P.add(OP_TRANSP, {143.90e0}, 0, 5);   P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_MID);
-    P.add(OP_TRANSP, {71.95e0}, 0, 6);    P.add(OP_CHECK_R2, {rq2}, ST_H_Q2_OUT);
-    P.add(OP_PROJECT, {dd[6]});           P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_IN);     // :311-316
-    P.add(OP_TRANSP, {143.8e0}, 0, 8);    P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_MID);
-    P.add(OP_TRANSP, {71.9e0}, 0, 9);     P.add(OP_CHECK_R2, {rq3}, ST_H_Q3_OUT);
-    P.add(OP_PROJECT, {dd[9]});                                                         // :338-339
-    P.rot(OP_ROT_H, -6.0e0, 0.0, 0.0);    P.add(OP_HIT_DIPOLE_T, {}, ST_H_D1_IN);    // :340-346
-    P.add(OP_TRANSP, {526.053e0}, 0, 11);                                               // :351
-    P.rot(OP_ROT_H, 6.0e0, 0.0, 0.0);     P.add(OP_HIT_DIPOLE_T, {}, ST_H_D1_OUT);   // :352-358
-    P.add(OP_PIPE_T, {30.48 * 30.48, 2.8, 0.0, 20.5232}, ST_H_PIPE0);                  // :363-367
-    P.add(OP_PROJECT, {z_dip1});           P.add(OP_PIPE_S, {1145.518, 2.8, 0.0}, ST_H_PIPE1);
-    P.add(OP_PROJECT, {z_dip2 - z_dip1});  P.add(OP_PIPE_S, {1512.2299, 2.8, 0.0}, ST_H_PIPE2);
-    P.add(OP_PROJECT, {z_dip3 - z_dip2});  P.add(OP_PIPE_S, {2162.9383, 2.8, 0.0}, ST_H_PIPE3);
-
-    // detector hut, hms/mc_hms_hut.f (constants :57-232)
-    const double zinit = -(dd[11] - z_dip3);  // mc_hms.f:413-415
-    const double hfoil_exit_radlen = 8.90, hfoil_exit_thick = 0.011 * 2.54, hair_radlen = 30420.;
-    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
-    const double hdc_radlen = 16700.0, hdc_thick = 1.8;
-    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
-    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
-    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
-    const double haer_entr_radlen = 8.90, haer_entr_thick = 0.15, haer_radlen = 150.0, haer_thick = 9.0;
-    const double haer_air_radlen = 30420.0, haer_air_thick = 16.0, haer_exit_radlen = 8.90, haer_exit_thick = 0.1;
-    const double hscin_radlen = 42.4, hscin_thick = 1.067;
-    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54, hcer_radlen = 9620.0;
-    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
-    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
-    const int hdc_nr_plan = 6;
-    const double hdc_1_zpos = -52.1084, hdc_2_zpos = 29.2608;
-    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
-    const double haer_zentrance = 35.699, haer_zexit = 60.949;
-    const double hscin_1x_zpos = 77.830, hscin_1y_zpos = 97.520, hscin_2x_zpos = 298.820, hscin_2y_zpos = 318.510;
-    const double hcer_zentrance = 110.000, hcer_zmirror = 230.000, hcer_zexit = 265.000;
-    const double hcal_4ta_zpos = 371.69;
-    // scintillator box of :487-490 (reused for the second hodoscope, :537-551)
-    const double scin[4] = {37.75 + -1.3, -37.75 + -1.3, 60.25 + -1.3, -60.25 + -1.3};
-    double drift;
-
-    P.add(OP_RESMULT_DRAW, {0.15});                                               // :292-298
-    P.add(OP_PROJECT, {(hdc_1_zpos - 25.000) - zinit});                           // :322-323
-    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});
-    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos - 25.000);
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 1, :337-381
-    chamber(P, 1, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
-            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
-    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
-    P.add(OP_BOX, {56.5 - 1.670, -56.5 - 1.670, 26.0 - 1.443, -26.0 - 1.443}, ST_H_DC1);
-    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
-    drift = (hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos + 0.5 * hdc_nr_plan * hdc_del_plane);
-    P.add(OP_PROJECT, {drift});                                                   // :385-389
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 2, :391-436
-    chamber(P, 2, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
-            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
-    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
-    P.add(OP_BOX, {56.5 - 2.758, -56.5 - 2.758, 26.0 - 2.753, -26.0 - 2.753}, ST_H_DC2);
-    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
-    for (int i = 0; i < 6; ++i) {                                                 // :439-447
-        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-    }
-    P.add(OP_FIT);                                                                // :449-455
-    drift = haer_zentrance - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;      // :459-462
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {haer_entr_thick / haer_entr_radlen});
-    P.add(OP_PROJECT, {haer_thick});
-    P.add(OP_MUSC_EXT, {haer_thick / haer_radlen, haer_thick});
-    P.add(OP_PROJECT, {haer_air_thick});
-    P.add(OP_MUSC_EXT, {haer_air_thick / haer_air_radlen, haer_air_thick});
-    P.add(OP_MUSC, {haer_exit_thick / haer_exit_radlen});
-    drift = hscin_1x_zpos - haer_zexit;                                           // :482-492
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hscin_1y_zpos - hscin_1x_zpos;                                        // :494-503
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hcer_zentrance - hscin_1y_zpos;                                       // :507-513
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
-    drift = hcer_zmirror - hcer_zentrance;                                        // :515-521
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hcer_radlen, drift});
-    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
-    drift = hcer_zexit - hcer_zmirror;                                            // :523-529
-    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
-    drift = hscin_2x_zpos - hcer_zexit;                                           // :533-542
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hscin_2y_zpos - hscin_2x_zpos;                                        // :544-553
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_SCIN, {scin[0], scin[1], scin[2], scin[3]});
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    P.add(OP_TRIGGER, {}, ST_H_SCIN, 3);                                          // :556-559
-    drift = hcal_4ta_zpos - hscin_2y_zpos;                                        // :563-566
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_END);
-}
-
-// mc_shms (shms/mc_shms.f:255-1106) followed by mc_shms_hut (shms/mc_shms_hut.f:74-448)
-// mc_sos (sos/mc_sos.f:160-372, apertures sos/apertures_sos.inc:9-26) followed
-// by mc_sos_hut (sos/mc_sos_hut.f:253-503, constants :42-197)
-void build_sos(const simc::ForwardMap& f, Program& P, double zdc[12])
-{
-    const double z_entr = 126.3e0, z_exit = z_entr + 6.3e0;  // :69-70
-    const double r2_quad = 163.84, w_bm01 = 8.0, w_bm02 = 8.0;
-    const double* dd = f.driftdist.data();
-    P.add(OP_PROJECT, {z_entr});                                                          // :171-172
-    P.add(OP_COLLIMATOR, {7.201, 4.696, 7.567, 4.935, 0.0, 0.0, z_exit - z_entr});       // :173-204
-    P.add(OP_PROJECT, {dd[0] - z_exit});  P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_IN);  // :209-215
-    P.add(OP_TRANSP, {35.0e0}, 0, 2);     P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_MID); // :219-223
-    P.add(OP_TRANSP, {35.0e0}, 0, 3);     P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_OUT); // :227-231
-    P.add(OP_TRANSP, {80.0e0}, 0, 4);                                                     // :236-245
-    P.rot(OP_ROT_H, -45.0e0, 0.0, 0.0);
-    P.add(OP_CHECK_T_BOX, {w_bm01, w_bm01 - 0.05 * 2.54, 41.32, -30.52}, ST_SO_BM01_IN);
-    P.add(OP_TRANSP, {169.52e0}, 0, 5);                                                   // :250-257
-    P.rot(OP_ROT_H, 45.0e0, 0.0, 0.0);
-    P.add(OP_CHECK_T_BOX, {w_bm01, w_bm01, 53.05, -65.07}, ST_SO_BM01_OUT);
-    P.add(OP_TRANSP, {80.80e0}, 0, 6);                                                    // :262-269
-    P.rot(OP_ROT_H, 49.0e0, 0.0, 0.0);
-    P.add(OP_CHECK_T_BOX, {w_bm02, w_bm02, 51.73, -66.38}, ST_SO_BM02_IN);
-    P.add(OP_TRANSP, {77.06e0}, 0, 7);                                                    // :274-281
-    P.rot(OP_ROT_H, 57.0e0, 0.0, 0.0);
-    P.add(OP_CHECK_T_BOX, {w_bm02, w_bm02, 51.52, -55.85}, ST_SO_BM02_OUT);
-    P.add(OP_TRANSP, {43.82e0}, 0, 8);                                                    // :311-319
-    P.rot(OP_ROT_H, 45.0e0, 0.0, 0.0);
-    P.add(OP_CHECK_T_BOX, {8.57, 8.57, 51.52, -55.85}, ST_SO_EXIT_FLANGE);
-    P.add(OP_TRANSP, {44.34e0}, 0, 9);                                                    // :329-336
-    P.add(OP_CHECK_TRAP, {37.694, 10.998, 0.10209}, ST_SO_EXIT_APERTURE);
-    P.add(OP_SHIFT, {7.62});                                                              // :342-343
-    P.add(OP_CHECK_ABS, {38.1, 12.7}, ST_SO_EXIT_WINDOW);                                 // :346-350
-
-    // detector hut from z = -3.206267 (:370-371)
-    const double zinit = -3.206267e0;
-    const double sfoil_exit_radlen = 53.3, sfoil_exit_thick = 0.020 * 2.54, sfoil_exit_zpos = -3.22;
-    const double sair_radlen = 30420.;
-    const double sdc_radlen = 16700.0, sdc_thick = 0.61775;
-    const double sdc_wire_radlen = 0.35, sdc_wire_thick = 0.0000354;
-    const double sdc_cath_radlen = 28.7, sdc_cath_thick = 0.0005 * 2.54;
-    const double sscin_radlen = 42.4;
-    const double scer_entr_radlen = 8.90, scer_entr_thick = 0.050, scer_radlen = 4810.0;
-    const double scer_mir_radlen = 400.0, scer_mir_thick = 2.0;
-    const double scer_exit_radlen = 8.90, scer_exit_thick = 0.050;
-    const int sdc_nr_plan = 6;
-    const double sdc_1_zpos = 6.25, sdc_2_zpos = 55.77;
-    const double sdc_del_plane = sdc_thick + sdc_wire_thick + sdc_cath_thick;
-    const double sscin_1y_zpos = 73.61, sscin_1x_zpos = 97.11, sscin_2y_zpos = 249.51, sscin_2x_zpos = 290.81;
-    const double sscin_1x_thick = 1.040, sscin_1y_thick = 1.098, sscin_2x_thick = 1.040, sscin_2y_thick = 1.098;
-    const double scer_zentrance = 130.000, scer_zmirror = 155.000, scer_zexit = 160.000;
-    const double scal_4ta_zpos = 346.01;
-    const double scin1[4] = {18.25 + 2.25, -18.25 + 2.25, 31.75 + 2.8, -31.75 + 2.8};  // :422-425
-    const double scin2[4] = {18.25 + 2.9, -18.25 + 2.9, 56.25 + 4.9, -56.25 + 4.9};    // :472-475
-    double drift;
-
-    P.add(OP_RESMULT_DRAW, {0.15});                                                 // :253-259
-    // Exit foil at sfoil_exit_zpos, tilt angle 0 (:49): the drift
-    // (zpos + xt tan 0) - zinit = -0.013733 is raised to 0.001 (:281-285), and
-    // the air gap to the chambers starts at zpos (:292-293).
-    drift = sfoil_exit_zpos - zinit;
-    if (drift <= 0.001) drift = 0.001;
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC, {sfoil_exit_thick / sfoil_exit_radlen / std::cos(0.0)});
-    drift = (sdc_1_zpos - 0.5 * sdc_nr_plan * sdc_del_plane) - sfoil_exit_zpos;    // :292-296
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    chamber(P, 1, sdc_cath_thick / sdc_cath_radlen, sdc_thick, sdc_radlen, sdc_cath_thick,
-            sdc_wire_thick / sdc_wire_radlen, sdc_wire_thick, 0x15u);                // :298-333
-    P.add(OP_BOX, {32.0 - 8.649, -32.0 - 8.649, 24.0 - -1.822, -24.0 - -1.822}, ST_SO_DC1);  // :335-341
-    P.add(OP_MUSC, {sdc_cath_thick / sdc_cath_radlen});
-    drift = sdc_2_zpos - sdc_1_zpos - sdc_nr_plan * sdc_del_plane;                 // :346-350
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    chamber(P, 2, sdc_cath_thick / sdc_cath_radlen, sdc_thick, sdc_radlen, sdc_cath_thick,
-            sdc_wire_thick / sdc_wire_radlen, sdc_wire_thick, 0x15u);                // :352-383
-    P.add(OP_BOX, {32.0 - -1.532, -32.0 - -1.532, 24.0 - -1.976, -24.0 - -1.976}, ST_SO_DC2);  // :385-391
-    P.add(OP_MUSC, {sdc_cath_thick / sdc_cath_radlen});
-    for (int i = 0; i < 6; ++i) {                                                   // :399-407
-        zdc[i] = sdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * sdc_nr_plan) * sdc_del_plane;
-        zdc[6 + i] = sdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * sdc_nr_plan) * sdc_del_plane;
-    }
-    P.add(OP_FIT);                                                                  // :409-414
-    drift = sscin_1y_zpos - sdc_2_zpos - 0.5 * sdc_nr_plan * sdc_del_plane;         // :418-428
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_SCIN, {scin1[0], scin1[1], scin1[2], scin1[3]});
-    P.add(OP_MUSC, {sscin_1y_thick / sscin_radlen});
-    drift = sscin_1x_zpos - sscin_1y_zpos;                                          // :430-439
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_SCIN, {scin1[0], scin1[1], scin1[2], scin1[3]});
-    P.add(OP_MUSC, {sscin_1x_thick / sscin_radlen});
-    drift = scer_zentrance - sscin_1x_zpos;                                         // :443-448
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_MUSC, {scer_entr_thick / scer_entr_radlen});
-    drift = scer_zmirror - scer_zentrance;                                          // :451-456
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / scer_radlen, drift});
-    P.add(OP_MUSC, {scer_mir_thick / scer_mir_radlen});
-    drift = scer_zexit - scer_zmirror;                                              // :459-464
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / scer_radlen, drift});
-    P.add(OP_MUSC, {scer_exit_thick / scer_exit_radlen});
-    drift = sscin_2y_zpos - scer_zexit;                                             // :469-478
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_SCIN, {scin2[0], scin2[1], scin2[2], scin2[3]});
-    P.add(OP_MUSC, {sscin_2y_thick / sscin_radlen});
-    drift = sscin_2x_zpos - sscin_2y_zpos;                                          // :480-489
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_SCIN, {scin2[0], scin2[1], scin2[2], scin2[3]});
-    P.add(OP_MUSC, {sscin_2x_thick / sscin_radlen});
-    P.add(OP_TRIGGER, {}, ST_SO_SCIN, 3);                                           // :491-494
-    drift = scal_4ta_zpos - sscin_2x_zpos;                                          // :500-503
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / sair_radlen, drift});
-    P.add(OP_END);
-}
-
-// mc_hrsr / mc_hrsl (hrsr/mc_hrsr.f:155-526, hrsl/mc_hrsl.f:155-527, apertures
-// hrsl/apertures_hrsl.inc:6-8) followed by their hut (hrsl/mc_hrsl_hut.f:229-456,
-// constants :37-185).  The arms differ in the slit position and exit half width
-// (hrsr/mc_hrsr.f:58,67; hrsl/mc_hrsl.f:58,67); their huts are the same.
-void build_hrs(int arm, const simc::ForwardMap& f, Program& P, double zdc[12])
-{
-    const bool left = arm == SIMC_ARM_HRSL;
-    const double h_entr = 3.145, v_entr = 6.090, v_exit = 6.485, h_exit = left ? 3.335 : 3.340;
-    const double z_off = 0.0, z_entr = (left ? 110.9e0 : 110.0e0) + z_off, z_exit = z_entr + 8.0e0;
-    const double rq1 = 15.0 * 15.0, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;
-    const double* dd = f.driftdist.data();
-    double ztmp, zdrift;
-    auto slit = [&](double h, double v) {  // |ys - y_off| > h, then |xs - x_off| > v (offsets 0)
-        P.add(OP_CHECK_ABS, {kInf, h}, ST_SLIT_HOR);
-        P.add(OP_CHECK_ABS, {v, kInf}, ST_SLIT_VERT);
-    };
-    auto d1_plane = [&](double ang, int stop) {  // rotated D1 aperture plane, :367-383, :389-405
-        P.rot(OP_ROT_H, ang, 0.0, 0.0);
-        P.add(OP_CHECK_T_SLAB, {2.500, 52.5, 0.01861, 12.5}, stop);
-    };
-    zdrift = 65.686; ztmp = zdrift;                                                   // :159-168
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {7.3787}, ST_R_PIPE_65);
-    zdrift = 80.436 - ztmp; ztmp = 80.436;                                            // :170-179
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {7.4092}, ST_R_PIPE_80);
-    P.add(OP_PROJECT, {z_entr - ztmp});  slit(h_entr, v_entr);                       // :184-199
-    P.add(OP_PROJECT, {z_exit - z_entr});  slit(h_exit, v_exit);                     // :203-218
-    ztmp = 135.064;                                                                   // :222-230
-    P.add(OP_PROJECT, {ztmp - z_exit});  P.add(OP_CHECK_R, {12.5222}, ST_R_Q1_PRE);
-    P.add(OP_PROJECT, {dd[0] - ztmp});  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_IN);       // :234-243
-    P.add(OP_TRANSP, {62.75333333e0}, 0, 2);  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_MID); // :247-254
-    P.add(OP_TRANSP, {31.37666667e0}, 0, 3);  P.add(OP_CHECK_R2, {rq1}, ST_R_Q1_OUT); // :258-265
-    zdrift = 300.464 - 253.16; ztmp = zdrift;                                         // :270-278
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {14.9225}, ST_R_Q1_POST);
-    zdrift = 314.464 - 300.464; ztmp = ztmp + zdrift;                                 // :281-289
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {20.9550}, ST_R_Q2_PRE);
-    P.add(OP_PROJECT, {dd[3] - ztmp});  P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_IN);       // :293-302
-    P.add(OP_TRANSP, {121.77333333e0}, 0, 5);  P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_MID);
-    P.add(OP_TRANSP, {60.88666667e0}, 0, 6);   P.add(OP_CHECK_R2, {rq2}, ST_R_Q2_OUT);
-    zdrift = 609.664 - 553.020; ztmp = zdrift;                                        // :329-337
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.0073}, ST_R_Q2_POST1);
-    zdrift = 641.800 - 609.664; ztmp = ztmp + zdrift;                                 // :340-348
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.0073}, ST_R_Q2_POST2);
-    zdrift = 819.489 - 641.800; ztmp = ztmp + zdrift;                                 // :351-359
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {50.0, 15.0}, ST_R_D1_PRE);
-    P.add(OP_PROJECT, {dd[6] - ztmp});  d1_plane(-30.0, ST_R_D1_IN);                 // :364-383
-    P.add(OP_TRANSP, {659.73445725e0}, 0, 8);  d1_plane(30.0, ST_R_D1_OUT);          // :388-405
-    zdrift = 1745.33546 - 1655.83446; ztmp = zdrift;                                  // :411-426
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.3276}, ST_R_D1_POST_R);
-    P.add(OP_CHECK_ABS, {50.0, 15.0}, ST_R_D1_POST_BOX);
-    zdrift = 1759.00946 - 1745.33546; ztmp = ztmp + zdrift;                           // :428-437
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_R, {30.3276}, ST_R_Q3_PRE);
-    P.add(OP_PROJECT, {dd[8] - ztmp});  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_IN);       // :442-451
-    P.add(OP_TRANSP, {121.7866667e0}, 0, 10);  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_MID);
-    P.add(OP_TRANSP, {60.89333333e0}, 0, 11);  P.add(OP_CHECK_R2, {rq3}, ST_R_Q3_OUT);
-    zdrift = 2080.38746 - 1997.76446; ztmp = zdrift;                                  // :477-486
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {35.56, 17.145}, ST_R_Q3_POST);
-    zdrift = 2327.47246 - 2080.38746; ztmp = ztmp + zdrift;                           // :490-499
-    P.add(OP_PROJECT, {zdrift});  P.add(OP_CHECK_ABS, {99.76635, 17.145}, ST_R_WINDOW);
-
-    // the hut from zinit = -(driftdist(12) - ztmp) (:505-510)
-    const double zinit = -(dd[11] - ztmp);
-    const double hfoil_exit_radlen = 3.56, hfoil_exit_thick = 0.01;
-    const double hair_radlen = 30420.;
-    const double hdc_entr_radlen = 34.4, hdc_entr_thick = 0.00018 * 2.54;
-    const double hdc_radlen = 16700.0, hdc_thick = 1.5;
-    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000049;
-    const double hdc_cath_radlen = 7.2, hdc_cath_thick = 0.000177;
-    const double hdc_exit_radlen = 34.4, hdc_exit_thick = 0.00018 * 2.54;
-    const double hscin_radlen = 42.4;
-    const double hcer_entr_radlen = 8.90, hcer_entr_thick = 0.040 * 2.54, hcer_radlen = 36620.0;
-    const double hcer_mir_radlen = 400.0, hcer_mir_thick = 2.0;
-    const double hcer_exit_radlen = 8.90, hcer_exit_thick = 0.040 * 2.54;
-    const int hdc_nr_plan = 6;
-    const double hdc_1_zpos = -25.0 + 25.0, hdc_2_zpos = 25.0 + 25.0;
-    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;
-    const double hdc_left = 14.4, hdc_right = -14.4, hdc_top = -105.6, hdc_bot = 105.6;
-    const double hdc_x_offset = 0.000, hdc_y_offset = 0.000;
-    const double hscin_1x_zpos = 95.0 + 25.0, hscin_2x_zpos = 288.3 + 25.0;
-    const double hscin_1x_thick = 0.5 * 1.067, hscin_2x_thick = 0.5 * 1.067;
-    const double hcer_zentrance = 137.0 + 25.0, hcer_zmirror = 197.0 + 25.0, hcer_zexit = 237.0 + 25.0;
-    const double hcal_4ta_zpos = 407.3 + 25.0;
-    const double raddeg = 0.017453292, r45 = 45.0e0 * raddeg;  // shared/rotate_haxis.f:41-51
-    double drift;
-    auto air = [&](double d, double radlen) {
-        P.add(OP_PROJECT, {d});
-        P.add(OP_MUSC_EXT, {d / radlen, d});
-    };
-
-    P.add(OP_RESMULT_ONE);                                                          // :231
-    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});                         // :240-241
-    for (int j = 1; j <= 2; ++j) {
-        drift = j == 1 ? (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - zinit   // :247-250
-                       : hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;     // :310-313
-        air(drift, hair_radlen);
-        P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});
-        chamber(P, j, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
-                hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick, 0x15u);            // :256-285
-        P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
-        P.add(OP_BOX_ROT_H, {std::tan(r45), std::sin(r45), std::cos(r45), hdc_bot - hdc_x_offset,
-                             hdc_top - hdc_x_offset, hdc_left - hdc_y_offset, hdc_right - hdc_y_offset},
-              j == 1 ? ST_R_DC1 : ST_R_DC2);                                        // :289-304, :352-367
-        P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
-    }
-    for (int i = 0; i < 6; ++i) {                                                   // :373-381
-        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-    }
-    P.add(OP_FIT);                                                                  // :383-389
-    air(hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane, hair_radlen);  // :393-405
-    P.add(OP_BOX, {kInf, -kInf, 18.0, -18.0}, ST_R_S1);
-    P.add(OP_MUSC, {hscin_1x_thick / hscin_radlen});
-    air(hcer_zentrance - hscin_1x_zpos, hair_radlen);                               // :410-416
-    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
-    air(hcer_zmirror - hcer_zentrance, hcer_radlen);                                // :418-424
-    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
-    air(hcer_zexit - hcer_zmirror, hcer_radlen);                                    // :426-432
-    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
-    air(hscin_2x_zpos - hcer_zexit, hair_radlen);                                   // :436-448
-    P.add(OP_BOX, {kInf, -kInf, 30.0, -30.0}, ST_R_S2);
-    P.add(OP_MUSC, {hscin_2x_thick / hscin_radlen});
-    air(hcal_4ta_zpos - hscin_2x_zpos, hair_radlen);                                // :453-456
-    P.add(OP_END);
-}
-
-void build_shms(Program& P, double zdc[12])
-{
-    const double zd_hbin = 118.39, zd_hbmen = 17.61, zd_hbmex = 80.0, zd_hbout = 17.61;  // :176-179
-    const double z_entr = 25.189, z_thick = 6.35;                                        // :181-182
-    const double zd_q1in = 58.39, zd_q1men = 28.35, zd_q1mid = 93.65, zd_q1mex = 93.65, zd_q1out = 28.35;
-    const double zd_q2in = 25.55, zd_q2men = 39.1, zd_q2mid = 79.35, zd_q2mex = 79.35, zd_q2out = 39.1;
-    const double zd_q3in = 28.10, zd_q3men = 39.1, zd_q3mid = 79.35, zd_q3mex = 79.35, zd_q3out = 39.1;
-    const double zd_q3d1trans = 18.00, zd_d1flare = 30.10, zd_d1men = 39.47, zd_d1mid = 36.406263;
-    const double zd_d1out = 60.68, zd_fp = 307.95;
-    // shms/apertures_shms.inc:9-28
-    const double rhb = 11.2 * 11.2, rq1 = 20.00 * 20.00, rq2 = 30.00 * 30.00, rq3 = 30.00 * 30.00,
-                 rd1 = 30.00 * 30.00;
-    P.add(OP_PROJECT, {zd_hbin});                                                 // :422-423
-    P.rot(OP_ROT_V, 1.5, 0.0, 1.51);
-    P.add(OP_CHECK_T_HB, {rhb, 11.75, -4.13}, ST_S_HB_IN);
-    P.add(OP_PROJECT, {zd_hbmen});                                                // :441-455
-    P.rot(OP_ROT_V, 1.5, 0.0, 0.98);
-    P.add(OP_CHECK_T_HB, {rhb, 11.74, -5.45}, ST_S_HB_MEN);
-    P.add(OP_TRANSP, {zd_hbmex}, 0, 3);                                           // :462-476
-    P.rot(OP_ROT_V, -1.5, 0.0, 0.98);
-    P.add(OP_CHECK_T_HB, {rhb, 11.71, -10.25}, ST_S_HB_MEX);
-    P.add(OP_PROJECT, {zd_hbout});                                                // :479-495
-    P.rot(OP_ROT_V, -1.5, 0.0, 1.51);
-    P.add(OP_CHECK_T_HB, {rhb, 11.70, -11.71}, ST_S_HB_OUT);
-    P.add(OP_PROJECT, {z_entr});                                                  // :505-506, 527-528
-    P.add(OP_COLLIMATOR, {8.5, 12.5, 8.65, 12.85, 0.00, 0.00, z_thick});          // :500-568
-    P.add(OP_PROJECT, {zd_q1in - z_entr - z_thick});                              // :573-585
-    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_IN);
-    P.add(OP_PROJECT, {zd_q1men});         P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MEN);
-    P.add(OP_TRANSP, {zd_q1mid}, 0, 7);    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MID);
-    P.add(OP_TRANSP, {zd_q1mex}, 0, 8);    P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_MEX);
-    P.add(OP_PROJECT, {zd_q1out});         P.add(OP_CHECK_R2, {rq1}, ST_S_Q1_OUT);
-    P.add(OP_PROJECT, {zd_q2in});          P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_IN);   // :644-712
-    P.add(OP_PROJECT, {zd_q2men});         P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MEN);
-    P.add(OP_TRANSP, {zd_q2mid}, 0, 12);   P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MID);
-    P.add(OP_TRANSP, {zd_q2mex}, 0, 13);   P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_MEX);
-    P.add(OP_PROJECT, {zd_q2out});         P.add(OP_CHECK_R2, {rq2}, ST_S_Q2_OUT);
-    P.add(OP_PROJECT, {zd_q3in});          P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_IN);   // :714-781
-    P.add(OP_PROJECT, {zd_q3men});         P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MEN);
-    P.add(OP_TRANSP, {zd_q3mid}, 0, 17);   P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MID);
-    P.add(OP_TRANSP, {zd_q3mex}, 0, 18);   P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_MEX);
-    P.add(OP_PROJECT, {zd_q3out});         P.add(OP_CHECK_R2, {rq3}, ST_S_Q3_OUT);
-    P.add(OP_PROJECT, {zd_q3d1trans});     P.add(OP_CHECK_R2, {rd1}, ST_S_D1_IN);   // :815-827
-    P.add(OP_PROJECT, {zd_d1flare});                                              // :832-846
-    P.rot(OP_ROT_H, 9.200, -3.5, 0.0);     P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_FLR);
-    P.add(OP_PROJECT, {zd_d1men});                                                // :851-865
-    P.rot(OP_ROT_H, 9.200, 2.82, 0.0);     P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_MEN);
-    const double ang[8] = {6.9, 4.6, 2.3, 0.0, -2.3, -4.6, -6.9, -9.2};          // :871-1011
-    const double off[8] = {8.05, 11.75, 13.96, 14.70, 13.96, 11.75, 8.05, 2.82};
-    for (int k = 0; k < 8; ++k) {
-        P.add(OP_TRANSP, {zd_d1mid}, 0, 23 + k);
-        P.rot(OP_ROT_H, ang[k], off[k], 0.0);
-        P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_MID1 + k);
-    }
-    P.add(OP_PROJECT, {zd_d1out});                                                // :1015-1031
-    P.rot(OP_ROT_H, -9.20, -6.88, 0.0);    P.add(OP_CHECK_T_R2, {rd1}, ST_S_D1_OUT);
-    const double hcer_1_zentrance = -291.700, hcer_1_zmirror = -84.900, hcer_1_zexit = -61.700;
-    P.add(OP_PROJECT, {zd_fp + hcer_1_zentrance});                                // :1036-1047
-
-    // detector hut, shms/mc_shms_hut.f with shms/hut.inc
-    const double hfoil_exit_radlen = 8.89, hfoil_exit_thick = 0.020 * 2.54, hair_radlen = 30420.;
-    const double hdc_entr_radlen = 28.7, hdc_entr_thick = 0.001 * 2.54;
-    const double hdc_radlen = 16700.0, hdc_thick = 0.125 * 2.54;
-    const double hdc_wire_radlen = 0.35, hdc_wire_thick = 0.0000354;
-    const double hdc_cath_radlen = 28.6, hdc_cath_thick = 0.001 * 2.54;
-    const double hdc_exit_radlen = 28.7, hdc_exit_thick = 0.001 * 2.54;
-    const double hscin_radlen = 42.4, hscin_thick = 1.000 * 1.067;
-    const double hcer_entr_radlen = 19.63, hcer_entr_thick = 0.002 * 2.54, hcer_1_radlen = 11700.0;
-    const double hcer_mirglass_radlen = 12.29, hcer_mirglass_thick = 0.3;
-    const double hcer_exit_radlen = 19.63, hcer_exit_thick = 0.002 * 2.54;
-    const double hcer_2_entr_radlen = 8.90, hcer_2_entr_thick = 0.040 * 2.54, hcer_2_radlen = 1202.5;
-    const double hcer_mir_radlen = 400., hcer_mir_thick = 2.00;
-    const double hcer_2_exit_radlen = 8.90, hcer_2_exit_thick = 0.040 * 2.54;
-    const int hdc_nr_plan = 6;
-    const double hdc_1_zpos = -40.656, hdc_2_zpos = 39.332;
-    const double hscin_1x_zpos = 52.1, hscin_1y_zpos = 61.7, hscin_2x_zpos = 271.4, hscin_2y_zpos = 282.4;
-    const double hcer_2_zentrance = 72.600, hcer_2_zmirror = 179.400, hcer_2_zexit = 202.600;
-    const double hcal_4ta_zpos = 341.0;
-    const double hcal_left = 63.00, hcal_right = -63.00, hcal_top = -70.00, hcal_bottom = 70.00;
-    const double hdc_del_plane = hdc_thick + hdc_wire_thick + hdc_cath_thick;     // :74
-    double drift;
-
-    P.add(OP_RESMULT_ONE);                                                        // :80
-    P.add(OP_MUSC, {hfoil_exit_thick / hfoil_exit_radlen});                       // :99-120
-    P.add(OP_MUSC, {hcer_entr_thick / hcer_entr_radlen});
-    drift = hcer_1_zmirror - hcer_1_zentrance - hcer_mirglass_thick / 2;
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hcer_1_radlen, drift});
-    P.add(OP_MUSC, {hcer_mirglass_thick / hcer_mirglass_radlen});
-    drift = hcer_1_zexit - hcer_1_zmirror - hcer_mirglass_thick / 2;
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hcer_1_radlen, drift});
-    P.add(OP_MUSC, {hcer_exit_thick / hcer_exit_radlen});
-    drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - hcer_1_zexit;      // :160-165
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 1, :168-222
-    chamber(P, 1, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
-            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
-    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
-    P.add(OP_BOX, {40. - 0.0, -40. - 0.0, 40.0 - 0.0, -40.0 - 0.0}, ST_S_DC1);
-    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
-    drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;                // :227-234
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hdc_entr_thick / hdc_entr_radlen});                           // chamber 2, :236-287
-    chamber(P, 2, hdc_cath_thick / hdc_cath_radlen, hdc_thick, hdc_radlen, hdc_cath_thick,
-            hdc_wire_thick / hdc_wire_radlen, hdc_wire_thick);
-    P.add(OP_MUSC, {hdc_exit_thick / hdc_exit_radlen});
-    P.add(OP_BOX, {40. - 0., -40. - 0., 40.0 - 0.0, -40.0 - 0.0}, ST_S_DC2);
-    P.add(OP_MUSC, {hdc_cath_thick / hdc_cath_radlen});
-    drift = hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;       // :291-306
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_BOX, {kInf, -kInf, 50. + 0.0, -50. + 0.0}, ST_S_S1X);
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hscin_1y_zpos - hscin_1x_zpos;                                        // :307-321
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_BOX, {45. + 0.0, -45. + 0.0, kInf, -kInf}, ST_S_S1Y);
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hcer_2_zentrance - hscin_1y_zpos;                                     // :326-332
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_MUSC, {hcer_2_entr_thick / hcer_2_entr_radlen});
-    drift = hcer_2_zmirror - hcer_2_zentrance;                                    // :334-341
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hcer_2_radlen, drift});
-    P.add(OP_MUSC, {hcer_mir_thick / hcer_mir_radlen});
-    drift = hcer_2_zexit - hcer_2_zmirror;                                        // :343-349
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hcer_2_radlen, drift});
-    P.add(OP_MUSC, {hcer_2_exit_thick / hcer_2_exit_radlen});
-    drift = hscin_2x_zpos - hcer_2_zexit;                                         // :353-367
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_BOX, {kInf, -kInf, 55. + 0., -55. + 0.}, ST_S_S2X);
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hscin_2y_zpos - hscin_2x_zpos;                                        // :368-382
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_BOX, {62.5 + 0., -62.5 + 0., kInf, -kInf}, ST_S_S2Y);
-    P.add(OP_MUSC, {hscin_thick / hscin_radlen});
-    drift = hcal_4ta_zpos - hscin_2y_zpos;                                        // :390-401
-    P.add(OP_PROJECT, {drift});
-    P.add(OP_MUSC_EXT, {drift / hair_radlen, drift});
-    P.add(OP_BOX, {hcal_bottom, hcal_top, hcal_left, hcal_right}, ST_S_CAL);
-    for (int i = 0; i < 6; ++i) {                                                 // :410-418
-        zdc[i] = hdc_1_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-        zdc[6 + i] = hdc_2_zpos + ((double)(i + 1) - 0.5 - 0.5 * hdc_nr_plan) * hdc_del_plane;
-    }
-    P.add(OP_FIT);                                                                // :419-426
-    P.add(OP_FIDUCIAL, {hcal_4ta_zpos, hcal_left - 5.0, hcal_right + 5.0, hcal_bottom - 5.0,
-                        hcal_top + 5.0}, ST_S_CALFID);                            // :435-444
-    P.add(OP_END);
-}
 
 ArmTables make_tables(const simc_ctx* ctx, int arm)
 {
@@ -2242,6 +1696,29 @@ ArmTables make_tables(const simc_ctx* ctx, int arm)
     return T;
 }
 
+// The hut stage runs the compiled hut of the arm's kernel family
+// (hut_gen.inc): the loaded program's hut steps must be that sequence, with
+// the same literal parameters.
+int check_compiled_hut(int arm, const simcprog::Program& P, int hut)
+{
+    const int family = is_hrs(arm) ? SIMC_ARM_HRSR : arm;
+    const HutGenTable* g = nullptr;
+    for (const HutGenTable& x : kHutGen)
+        if (x.kernel_arm == family) g = &x;
+    if (!g || g->first != hut || g->first + g->n != (int)P.ops.size())
+        return fail(SIMC_E_INTERNAL, "the arm's program does not match its compiled detector hut");
+    for (int k = 0; k < g->n; ++k) {
+        const Op& o = P.ops[g->first + k];
+        bool same = o.code == g->seq[k].code && o.arg == g->seq[k].arg && o.stop == g->seq[k].stop;
+        for (int j = 0; j < 8; ++j)
+            if (g->lit_mask[k] & (1u << j)) same = same && std::memcmp(&o.p[j], &g->lit[k][j], sizeof(double)) == 0;
+        if (!same)
+            return fail(SIMC_E_INTERNAL, "step " + std::to_string(g->first + k) +
+                                             " of the arm's program differs from its compiled detector hut");
+    }
+    return SIMC_OK;
+}
+
 int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::string& rec_path)
 {
     const int ai = arm_index(arm);
@@ -2258,22 +1735,20 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     std::vector<double> packed;
     rc = simc::pack_arm(arm, ctx->fwd[ai], ctx->rec[ai], packed, msg);
     if (rc) return fail(rc, msg);
-    Program P;
-    if (arm == SIMC_ARM_HMS) build_hms(ctx->fwd[ai], P, ctx->zdc[ai]);
-    else if (arm == SIMC_ARM_SOS) build_sos(ctx->fwd[ai], P, ctx->zdc[ai]);
-    else if (is_hrs(arm)) build_hrs(arm, ctx->fwd[ai], P, ctx->zdc[ai]);
-    else build_shms(P, ctx->zdc[ai]);
+    simcprog::Program P;
+    simcprog::build_program(arm, ctx->fwd[ai], P, ctx->zdc[ai]);
     ctx->n_ops[ai] = (int)P.ops.size();
     ctx->ops[ai] = P.ops;
     // Split point of the two stages; every step must belong to its stage.
-    int hut = 0;
-    while (hut < (int)P.ops.size() && op_stages(P.ops[hut].code) != STAGE_HUT) ++hut;
+    const int hut = simcprog::hut_begin(P);
     for (int k = 0; k < (int)P.ops.size(); ++k) {
         const int want = k < hut ? STAGE_MAGNETS : STAGE_HUT;
         if (!(op_stages(P.ops[k].code) & want))
             return fail(SIMC_E_INTERNAL, "step " + std::to_string(k) + " outside its stage");
     }
     ctx->hut_pc[ai] = hut;
+    rc = check_compiled_hut(arm, P, hut);
+    if (rc) return rc;
     // Stage 1 ends after the collimator; an arm without one (the HRS: slit
     // apertures only) ends it before its first COSY segment.
     int coll = 0;
@@ -2660,7 +2135,11 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
     const int ai = arm_index(arm);
     const int32_t coll = ctx->coll_pc[ai], hut = ctx->hut_pc[ai];
     const dim3 G((a.count + kBlock - 1) / kBlock), B(kBlock);
-    hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, q1.count);
+    // the accepted list of the hut stage reuses queue 1 (consumed by then)
+    // with the third counter
+    Queue acc = q1;
+    acc.count = q1.count + 2;
+    hipLaunchKernelGGL(zero_queue_counts, dim3(1), dim3(64), 0, st, q1.count);
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
 #define LAUNCH3(ARMV, RV, RARE)                                                                             \
     do {                                                                                                    \
@@ -2670,8 +2149,9 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
         hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, true, true>), G, B, 0, st, a, q1, q2,  \
                            coll, hut);                                                                      \
         if (ev) HIPCHK(hipEventRecord(ev[2], st));                                                          \
-        hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_HUT, true, false>), G, B, 0, st, a, q2, q1,     \
+        hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_HUT, true, true>), G, B, 0, st, a, q2, acc,     \
                            hut, 0x7fffffff);                                                                \
+        hipLaunchKernelGGL((recon_list_kernel<ARMV>), G, B, 0, st, a, acc);                                 \
         if (ev) HIPCHK(hipEventRecord(ev[3], st));                                                          \
     } while (0)
 #define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
@@ -3948,7 +3428,7 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     const uint32_t nc = (uint32_t)n_terms * 4;
     hipLaunchKernelGGL(zero_u64, dim3((nc + 255) / 256), dim3(256), 0, st, (unsigned long long*)grad_coef, nc);
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, st, (unsigned long long*)out9, 9u);
-    hipLaunchKernelGGL(zero_u32x2, dim3(1), dim3(64), 0, st, ctx->d_nsel);
+    hipLaunchKernelGGL(zero_queue_counts, dim3(1), dim3(64), 0, st, ctx->d_nsel);
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(select_events, dim3(blocks), dim3(kBlock), 0, st, mask, (uint32_t)n, ctx->d_sel, ctx->d_nsel);
 #define ADJ(ARMV, RV) hipLaunchKernelGGL((adjoint_kernel<ARMV, RV>), dim3(blocks), dim3(kBlock), 0, st, a)

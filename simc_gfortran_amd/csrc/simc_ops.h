@@ -2,11 +2,13 @@
 //
 // An arm (mc_hms, mc_shms, mc_sos, mc_hrsr, mc_hrsl and their detector huts) is a fixed sequence of
 // drifts, COSY map segments, aperture tests and detector-hut steps.  The host
-// writes that sequence once per context as an array of Op records; one
-// kernel per arm walks it for every event (all lanes of a wavefront execute
-// the same op, so dispatch is uniform and the code of each step exists once).
-// Every numeric parameter is evaluated on the host with the reference's own
-// expressions (cited where the table is built, simc_kernels.hip).
+// writes that sequence once per context as an array of Op records
+// (simc_programs.h, every numeric parameter formed with the reference's own
+// expressions, cited there).  The magnet stages walk the records (all lanes
+// of a wavefront execute the same op, so dispatch is uniform and the code of
+// each step exists once); the detector hut runs as straight-line code
+// generated from the same builders (gen_hut.cpp), checked against the loaded
+// records.
 #pragma once
 
 #include <stdint.h>
@@ -69,6 +71,39 @@ struct Op {
     int32_t stop;
     int32_t pad;
     double p[8];
+};
+
+// ---------------------------------------------------------------------------
+// Stop codes (one per exit point; 0 = accepted)
+// ---------------------------------------------------------------------------
+enum : int {
+    ST_OK = 0, ST_SLIT_HOR = 1, ST_SLIT_VERT = 2, ST_SLIT_OCT = 3, ST_COLL = 4,
+    // HMS
+    ST_H_Q1_IN = 10, ST_H_Q1_MID, ST_H_Q1_OUT, ST_H_Q2_IN, ST_H_Q2_MID, ST_H_Q2_OUT,
+    ST_H_Q3_IN, ST_H_Q3_MID, ST_H_Q3_OUT, ST_H_D1_IN, ST_H_D1_OUT, ST_H_PIPE0, ST_H_PIPE1,
+    ST_H_PIPE2, ST_H_PIPE3, ST_H_DC1, ST_H_DC2, ST_H_SCIN,
+    // SHMS
+    ST_S_HB_IN = 40, ST_S_HB_MEN, ST_S_HB_MEX, ST_S_HB_OUT, ST_S_Q1_IN, ST_S_Q1_MEN,
+    ST_S_Q1_MID, ST_S_Q1_MEX, ST_S_Q1_OUT, ST_S_Q2_IN, ST_S_Q2_MEN, ST_S_Q2_MID, ST_S_Q2_MEX,
+    ST_S_Q2_OUT, ST_S_Q3_IN, ST_S_Q3_MEN, ST_S_Q3_MID, ST_S_Q3_MEX, ST_S_Q3_OUT, ST_S_D1_IN,
+    ST_S_D1_FLR, ST_S_D1_MEN, ST_S_D1_MID1, ST_S_D1_MID2, ST_S_D1_MID3, ST_S_D1_MID4,
+    ST_S_D1_MID5, ST_S_D1_MID6, ST_S_D1_MID7, ST_S_D1_MEX, ST_S_D1_OUT, ST_S_DC1, ST_S_DC2,
+    ST_S_S1X, ST_S_S1Y, ST_S_S2X, ST_S_S2Y, ST_S_CAL, ST_S_CALFID,
+    // SOS
+    ST_SO_QUAD_IN = 100, ST_SO_QUAD_MID, ST_SO_QUAD_OUT, ST_SO_BM01_IN, ST_SO_BM01_OUT,
+    ST_SO_BM02_IN, ST_SO_BM02_OUT, ST_SO_EXIT_FLANGE, ST_SO_EXIT_APERTURE, ST_SO_EXIT_WINDOW,
+    ST_SO_DC1, ST_SO_DC2, ST_SO_SCIN,
+    // HRS (both arms)
+    ST_R_PIPE_65 = 120, ST_R_PIPE_80, ST_R_Q1_PRE, ST_R_Q1_IN, ST_R_Q1_MID, ST_R_Q1_OUT,
+    ST_R_Q1_POST, ST_R_Q2_PRE, ST_R_Q2_IN, ST_R_Q2_MID, ST_R_Q2_OUT, ST_R_Q2_POST1, ST_R_Q2_POST2,
+    ST_R_D1_PRE, ST_R_D1_IN, ST_R_D1_OUT, ST_R_D1_POST_R, ST_R_D1_POST_BOX, ST_R_Q3_PRE, ST_R_Q3_IN,
+    ST_R_Q3_MID, ST_R_Q3_OUT, ST_R_Q3_POST, ST_R_WINDOW, ST_R_DC1, ST_R_DC2, ST_R_S1, ST_R_S2,
+    ST_TAPE = 250, ST_BADOP = 251
+};
+
+// One step of a compiled detector hut: its op code, arg and stop code.
+struct HutStep {
+    int32_t code, arg, stop;
 };
 
 }  // namespace simcdev
