@@ -504,6 +504,24 @@ double or_grnd(or_rng *g)
     }
 }
 
+/* The detector hut's own stream of the counter-based source: the event's
+ * key and event id with the fourth counter word OR_HUT_TAG, from uniform 0.
+ * Every hut step draws one uniform (resmult) or Box-Muller pairs, so no
+ * normal is pending between steps.  A tape (the reference's own sequence)
+ * and the RANLUX source continue through the hut: *store is unused and g is
+ * returned. */
+static or_rng *or_hut_stream(or_rng *g, or_rng *store)
+{
+    if (g->kind != 1) return g;
+    *store = *g;
+    store->tag = OR_HUT_TAG;
+    store->pos = 0;
+    store->used = 0;
+    store->have = 0;
+    store->pending = 0.0;
+    return store;
+}
+
 /* ========================================================================= */
 /* Elementary routines                                                        */
 /* ========================================================================= */
@@ -1146,7 +1164,7 @@ static int hms_hit_dipole(double x, double y)
 
 /* mc_hms_hut, hms/mc_hms_hut.f:1-608.  Returns ok_hut; *fit_done is set once
  * x_fp..dy_fp have been written (:452-455). */
-static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+static int hms_hut(or_rng *g, or_rng *hg, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
                    double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                    int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
                    double zinit, double *pathlen, int *stop_code, int *fit_done)
@@ -1191,7 +1209,7 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     hut_hits h;
     int scincount;
 
-    double tmpran = or_grnd(g); /* :292-298 */
+    double tmpran = or_grnd(hg); /* :292-298 */
     double tmplim = 0.15;
     if (tmpran < tmplim) *resmult = 2.0;
     else *resmult = 1.0;
@@ -1201,12 +1219,12 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     drift = (hdc_1_zpos - 25.000) - zinit;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
     radw = hfoil_exit_thick / hfoil_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - (hdc_1_zpos - 25.000);
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
 
     for (int jchamber = 1; jchamber <= 2; jchamber++) {
         if (jchamber == 2) {
@@ -1214,24 +1232,24 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
                     (hdc_1_zpos + 0.5 * hdc_nr_plan * hdc_del_plane);
             radw = drift / hair_radlen;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-            if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+            if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
         }
         radw = hdc_entr_thick / hdc_entr_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         int npl_off = (jchamber - 1) * hdc_nr_plan;
         for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
             radw = hdc_cath_thick / hdc_cath_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_cath_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
             radw = hdc_wire_thick / hdc_wire_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             if (wcs_flag) {
-                tmpran1 = or_gauss1(g, nsig_max);
-                tmpran2 = or_gauss1(g, nsig_max);
+                tmpran1 = or_gauss1(hg, nsig_max);
+                tmpran2 = or_gauss1(hg, nsig_max);
             } else {
                 tmpran1 = 0.;
                 tmpran2 = 0.;
@@ -1242,12 +1260,12 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             else h.ydc[npl_off + iplane - 1] = 0.;
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_wire_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
         }
         radw = hdc_exit_thick / hdc_exit_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         if (jchamber == 1) {
             if (t->xs > (hdc_1_bot - hdc_1x_offset) || t->xs < (hdc_1_top - hdc_1x_offset) ||
                 t->ys > (hdc_1_left - hdc_1y_offset) || t->ys < (hdc_1_right - hdc_1y_offset)) {
@@ -1264,7 +1282,7 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             }
         }
         radw = hdc_cath_thick / hdc_cath_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     }
 
     for (int jchamber = 1; jchamber <= 2; jchamber++) {
@@ -1280,19 +1298,19 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     drift = haer_zentrance - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = haer_entr_thick / haer_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = haer_thick;
     radw = drift / haer_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     drift = haer_air_thick;
     radw = drift / haer_air_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = haer_exit_thick / haer_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
 #define HMS_SCIN_IN()                                                                          \
     (t->ys < (hscin_1x_left + hscin_1y_offset) && t->ys > (hscin_1x_right + hscin_1y_offset) && \
@@ -1301,53 +1319,53 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     drift = hscin_1x_zpos - haer_zexit;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (HMS_SCIN_IN()) scincount = scincount + 1;
     radw = hscin_1x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hscin_1y_zpos - hscin_1x_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (HMS_SCIN_IN()) scincount = scincount + 1;
     radw = hscin_1y_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hcer_zentrance - hscin_1y_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_entr_thick / hcer_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_zmirror - hcer_zentrance;
     radw = drift / hcer_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_mir_thick / hcer_mir_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_zexit - hcer_zmirror;
     radw = hcer_exit_thick / hcer_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
     radw = hcer_exit_thick / hcer_exit_radlen; /* exit window scattering a second time, :528-529 */
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hscin_2x_zpos - hcer_zexit;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (HMS_SCIN_IN()) scincount = scincount + 1; /* first-plane limits reused, :537-540 */
     radw = hscin_2x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hscin_2y_zpos - hscin_2x_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (HMS_SCIN_IN()) scincount = scincount + 1;
     radw = hscin_2y_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 #undef HMS_SCIN_IN
 
     if (scincount < scintrig) {
@@ -1358,7 +1376,7 @@ static int hms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     drift = hcal_4ta_zpos - hscin_2y_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     return 1;
 }
 
@@ -1476,8 +1494,11 @@ void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
 
     cnt[H_HUT]++;
     zdrift = F->driftdist[11] - z_dip3;
-    ok = hms_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+    or_rng hut_store;
+    or_rng *hg = or_hut_stream(g, &hut_store);
+    ok = hms_hut(g, hg, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
                  &dflag, resmult, -zdrift, pathlen, stop_code, fit_done);
+    if (hg != g) g->used += hg->used;
     if (!ok) return;
 
     t->xs = *x_fp;
@@ -1518,7 +1539,7 @@ static void recon_sos(const or_recon_map *m, const or_track *t, double fry, doub
 }
 
 /* mc_sos_hut, sos/mc_sos_hut.f:1-548 (constants :42-197) */
-static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+static int sos_hut(or_rng *g, or_rng *hg, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
                    double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                    int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
                    double zinit, double *pathlen, int *stop_code, int *fit_done)
@@ -1560,7 +1581,7 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     hut_hits h;
     int scincount;
 
-    double tmpran = or_grnd(g); /* :253-259 */
+    double tmpran = or_grnd(hg); /* :253-259 */
     double tmplim = 0.15;
     if (tmpran < tmplim) *resmult = 2.0;
     else *resmult = 1.0;
@@ -1573,7 +1594,7 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     if (drift <= 0.001) drift = 0.001;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
     radw = sfoil_exit_thick / sfoil_exit_radlen / cos(sfoil_exit_ang);
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     for (int jchamber = 1; jchamber <= sdc_nr_cham; jchamber++) {
         if (jchamber == 1) { /* :292-296 */
@@ -1584,21 +1605,21 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
         }
         radw = drift / sair_radlen;
         project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
         int npl_off = (jchamber - 1) * sdc_nr_plan;
         for (int iplane = 1; iplane <= sdc_nr_plan; iplane++) {
             radw = sdc_cath_thick / sdc_cath_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = 0.5 * sdc_thick;
             radw = drift / sdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + sdc_cath_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
             radw = sdc_wire_thick / sdc_wire_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             if (wcs_flag) {
-                tmpran1 = or_gauss1(g, nsig_max);
-                tmpran2 = or_gauss1(g, nsig_max);
+                tmpran1 = or_gauss1(hg, nsig_max);
+                tmpran2 = or_gauss1(hg, nsig_max);
             } else {
                 tmpran1 = 0.;
                 tmpran2 = 0.;
@@ -1609,7 +1630,7 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             else h.ydc[npl_off + iplane - 1] = 0.;
             drift = 0.5 * sdc_thick;
             radw = drift / sdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + sdc_wire_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
         }
@@ -1629,7 +1650,7 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             }
         }
         radw = sdc_cath_thick / sdc_cath_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     }
 
     for (int jchamber = 1; jchamber <= sdc_nr_cham; jchamber++) { /* :399-407 */
@@ -1650,38 +1671,38 @@ static int sos_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
         drift = (DRIFT);                                                                 \
         radw = drift / (RADLEN);                                                         \
         project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);      \
-        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
+        if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
     } while (0)
 
     SOS_STEP(sscin_1y_zpos - sdc_2_zpos - 0.5 * sdc_nr_plan * sdc_del_plane, sair_radlen); /* :418-428 */
     if (SOS_SCIN_IN(sscin_1_left, sscin_1_right, sscin_1y_offset, sscin_1_bot, sscin_1_top, sscin_1x_offset))
         scincount = scincount + 1;
     radw = sscin_1y_thick / sscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(sscin_1x_zpos - sscin_1y_zpos, sair_radlen); /* :430-439 */
     if (SOS_SCIN_IN(sscin_1_left, sscin_1_right, sscin_1y_offset, sscin_1_bot, sscin_1_top, sscin_1x_offset))
         scincount = scincount + 1;
     radw = sscin_1x_thick / sscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(scer_zentrance - sscin_1x_zpos, sair_radlen); /* :443-448 */
     radw = scer_entr_thick / scer_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(scer_zmirror - scer_zentrance, scer_radlen); /* :451-456 */
     radw = scer_mir_thick / scer_mir_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(scer_zexit - scer_zmirror, scer_radlen); /* :459-464 */
     radw = scer_exit_thick / scer_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(sscin_2y_zpos - scer_zexit, sair_radlen); /* :469-478 */
     if (SOS_SCIN_IN(sscin_2_left, sscin_2_right, sscin_2y_offset, sscin_2_bot, sscin_2_top, sscin_2x_offset))
         scincount = scincount + 1;
     radw = sscin_2y_thick / sscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     SOS_STEP(sscin_2x_zpos - sscin_2y_zpos, sair_radlen); /* :480-489 */
     if (SOS_SCIN_IN(sscin_2_left, sscin_2_right, sscin_2y_offset, sscin_2_bot, sscin_2_top, sscin_2x_offset))
         scincount = scincount + 1;
     radw = sscin_2x_thick / sscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     if (scincount < scintrig) { /* :491-494 */
         cnt[SO_SCIN]++;
         *stop_code = STOP_SO_SCIN;
@@ -1786,8 +1807,11 @@ void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
 #undef SO_STOP
 
     cnt[SO_HUT]++; /* :366-372 */
-    ok = sos_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+    or_rng hut_store;
+    or_rng *hg = or_hut_stream(g, &hut_store);
+    ok = sos_hut(g, hg, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
                  &dflag, resmult, -3.206267e0, pathlen, stop_code, fit_done);
+    if (hg != g) g->used += hg->used;
     if (!ok) return;
 
     t->xs = *x_fp;
@@ -1809,7 +1833,7 @@ void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
 
 /* mc_hrsl_hut / mc_hrsr_hut, hrsl/mc_hrsl_hut.f:1-498 (constants :37-185); the two
  * huts are the same code with their own counters.  resmult is 1 (:231). */
-static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+static int hrs_hut(or_rng *g, or_rng *hg, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
                    double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                    int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
                    double zinit, double *pathlen, int *stop_code, int *fit_done)
@@ -1847,7 +1871,7 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
     *resmult = 1.0;
 
     radw = hfoil_exit_thick / hfoil_exit_radlen; /* exit foil at zinit, :240-241 */
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     for (int jchamber = 1; jchamber <= hdc_nr_cham; jchamber++) {
         if (jchamber == 1) /* :247-250 */
@@ -1856,23 +1880,23 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;
         radw = drift / hair_radlen;
         project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+        if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
         radw = hdc_entr_thick / hdc_entr_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         int npl_off = (jchamber - 1) * hdc_nr_plan;
         for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) { /* :256-285, :319-348 */
             radw = hdc_cath_thick / hdc_cath_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_cath_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
             radw = hdc_wire_thick / hdc_wire_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             if (wcs_flag) {
-                tmpran1 = or_gauss1(g, nsig_max);
-                tmpran2 = or_gauss1(g, nsig_max);
+                tmpran1 = or_gauss1(hg, nsig_max);
+                tmpran2 = or_gauss1(hg, nsig_max);
             } else {
                 tmpran1 = 0.;
                 tmpran2 = 0.;
@@ -1883,12 +1907,12 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             else h.ydc[npl_off + iplane - 1] = 0.;
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_wire_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
         }
         radw = hdc_exit_thick / hdc_exit_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         /* the VDC frame, rotated by 45 degrees (:291-304, :354-367) */
         xt = t->xs;
         yt = t->ys;
@@ -1900,7 +1924,7 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
             return 0;
         }
         radw = hdc_cath_thick / hdc_cath_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     }
 
     for (int jchamber = 1; jchamber <= hdc_nr_cham; jchamber++) { /* :373-381 */
@@ -1918,7 +1942,7 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
         drift = (DRIFT);                                                                 \
         radw = drift / (RADLEN);                                                         \
         project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);      \
-        if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
+        if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs); \
     } while (0)
 
     HRS_STEP(hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane, hair_radlen); /* :393-405 */
@@ -1928,16 +1952,16 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
         return 0;
     }
     radw = hscin_1x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     HRS_STEP(hcer_zentrance - hscin_1x_zpos, hair_radlen); /* :410-416 */
     radw = hcer_entr_thick / hcer_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     HRS_STEP(hcer_zmirror - hcer_zentrance, hcer_radlen); /* :418-424 */
     radw = hcer_mir_thick / hcer_mir_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     HRS_STEP(hcer_zexit - hcer_zmirror, hcer_radlen); /* :426-432 */
     radw = hcer_exit_thick / hcer_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     HRS_STEP(hscin_2x_zpos - hcer_zexit, hair_radlen); /* :436-448 */
     if (t->ys > hscin_2x_left || t->ys < hscin_2x_right) {
         cnt[R_S2]++;
@@ -1945,7 +1969,7 @@ static int hrs_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, d
         return 0;
     }
     radw = hscin_2x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     HRS_STEP(hcal_4ta_zpos - hscin_2x_zpos, hair_radlen); /* :453-456 */
 #undef HRS_STEP
     return 1;
@@ -2094,8 +2118,11 @@ void or_mc_hrs(const or_maps *M, int arm, const or_settings *S, or_rng *g, or_tr
 
     cnt[R_HUT]++; /* :503-510 */
     zdrift = F->driftdist[11] - ztmp;
-    ok = hrs_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
+    or_rng hut_store;
+    or_rng *hg = or_hut_stream(g, &hut_store);
+    ok = hrs_hut(g, hg, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag, decay_flag,
                  &dflag, resmult, -zdrift, pathlen, stop_code, fit_done);
+    if (hg != g) g->used += hg->used;
     if (!ok) return;
 
     t->xs = *x_fp; /* :512-535: recon from the unshifted track, y_fp shifted by the VDC offset */
@@ -2118,7 +2145,7 @@ void or_mc_hrs(const or_maps *M, int arm, const or_settings *S, or_rng *g, or_tr
 
 /* mc_shms_hut, shms/mc_shms_hut.f:1-458 with shms/hut.inc (cer_flag forced
  * true, vac_flag false by mc_shms, shms/mc_shms.f:257-258; use_det_cut true) */
-static int shms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
+static int shms_hut(or_rng *g, or_rng *hg, or_track *t, const or_settings *S, int64_t *cnt, double *m2,
                     double *p, double *x_fp, double *dx_fp, double *y_fp, double *dy_fp,
                     int ms_flag, int wcs_flag, int decay_flag, int *dflag, double *resmult,
                     double *pathlen, int *stop_code, int *fit_done)
@@ -2167,50 +2194,50 @@ static int shms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, 
     for (int i = 0; i < 12; i++) { h.xdc[i] = 0.; h.ydc[i] = 0.; }
 
     radw = hfoil_exit_thick / hfoil_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     radw = hcer_entr_thick / hcer_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_1_zmirror - hcer_1_zentrance - hcer_mirglass_thick / 2;
     radw = drift / hcer_1_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_mirglass_thick / hcer_mirglass_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_1_zexit - hcer_1_zmirror - hcer_mirglass_thick / 2;
     radw = drift / hcer_1_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_exit_thick / hcer_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = (hdc_1_zpos - 0.5 * hdc_nr_plan * hdc_del_plane) - hcer_1_zexit;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
 
     for (int jchamber = 1; jchamber <= 2; jchamber++) {
         if (jchamber == 2) {
             drift = hdc_2_zpos - hdc_1_zpos - hdc_nr_plan * hdc_del_plane;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
             radw = drift / hair_radlen;
-            if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+            if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
         }
         radw = hdc_entr_thick / hdc_entr_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         int npl_off = (jchamber - 1) * hdc_nr_plan;
         for (int iplane = 1; iplane <= hdc_nr_plan; iplane++) {
             radw = hdc_cath_thick / hdc_cath_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_cath_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
             radw = hdc_wire_thick / hdc_wire_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             if (wcs_flag) {
-                tmpran1 = or_gauss1(g, nsig_max);
-                tmpran2 = or_gauss1(g, nsig_max);
+                tmpran1 = or_gauss1(hg, nsig_max);
+                tmpran2 = or_gauss1(hg, nsig_max);
             } else {
                 tmpran1 = 0.;
                 tmpran2 = 0.;
@@ -2221,12 +2248,12 @@ static int shms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, 
             else h.ydc[npl_off + iplane - 1] = 0.;
             drift = 0.5 * hdc_thick;
             radw = drift / hdc_radlen;
-            if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+            if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
             drift = drift + hdc_wire_thick;
             project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
         }
         radw = hdc_exit_thick / hdc_exit_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
         if (jchamber == 1) {
             if (t->xs > (hdc_1_bot - hdc_1x_offset) || t->xs < (hdc_1_top - hdc_1x_offset) ||
                 t->ys > (hdc_1_left - hdc_1y_offset) || t->ys < (hdc_1_right - hdc_1y_offset)) {
@@ -2243,78 +2270,78 @@ static int shms_hut(or_rng *g, or_track *t, const or_settings *S, int64_t *cnt, 
             }
         }
         radw = hdc_cath_thick / hdc_cath_radlen;
-        if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+        if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     }
 
     drift = hscin_1x_zpos - hdc_2_zpos - 0.5 * hdc_nr_plan * hdc_del_plane;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (t->ys > (hscin_1x_left + hscin_1y_offset) || t->ys < (hscin_1x_right + hscin_1y_offset)) {
         cnt[S_S1]++;
         *stop_code = STOP_S_S1X;
         return 0;
     }
     radw = hscin_1x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hscin_1y_zpos - hscin_1x_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (t->xs > (hscin_1y_bot + hscin_1x_offset) || t->xs < (hscin_1y_top + hscin_1x_offset)) {
         cnt[S_S1]++;
         *stop_code = STOP_S_S1Y;
         return 0;
     }
     radw = hscin_1y_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hcer_2_zentrance - hscin_1y_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_2_entr_thick / hcer_2_entr_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_2_zmirror - hcer_2_zentrance;
     radw = drift / hcer_2_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_mir_thick / hcer_mir_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hcer_2_zexit - hcer_2_zmirror;
     radw = drift / hcer_2_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     radw = hcer_2_exit_thick / hcer_2_exit_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hscin_2x_zpos - hcer_2_zexit;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (t->ys > (hscin_2x_left + hscin_2y_offset) || t->ys < (hscin_2x_right + hscin_2y_offset)) {
         cnt[S_S3]++;
         *stop_code = STOP_S_S2X;
         return 0;
     }
     radw = hscin_2x_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
     drift = hscin_2y_zpos - hscin_2x_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (t->xs > (hscin_2y_bot + hscin_2x_offset) || t->xs < (hscin_2y_top + hscin_2x_offset)) {
         cnt[S_S2]++;
         *stop_code = STOP_S_S2Y;
         return 0;
     }
     radw = hscin_2y_thick / hscin_radlen;
-    if (ms_flag) musc(g, *m2, *p, radw, &t->dydzs, &t->dxdzs);
+    if (ms_flag) musc(hg, *m2, *p, radw, &t->dydzs, &t->dxdzs);
 
     drift = hcal_4ta_zpos - hscin_2y_zpos;
     radw = drift / hair_radlen;
     project(g, t, S, &t->xs, &t->ys, drift, decay_flag, dflag, m2, p, pathlen);
-    if (ms_flag) musc_ext(g, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
+    if (ms_flag) musc_ext(hg, *m2, *p, radw, drift, &t->dydzs, &t->dxdzs, &t->ys, &t->xs);
     if (t->ys > hcal_left || t->ys < hcal_right || t->xs > hcal_bottom || t->xs < hcal_top) {
         cnt[S_CAL]++;
         *stop_code = STOP_S_CAL;
@@ -2530,8 +2557,11 @@ void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, 
     t->dxdzs_fp = t->dxdzs;
     t->dydzs_fp = t->dydzs;
 
-    int ok = shms_hut(g, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag,
+    or_rng hut_store;
+    or_rng *hg = or_hut_stream(g, &hut_store);
+    int ok = shms_hut(g, hg, t, S, cnt, m2, &p, x_fp, dx_fp, y_fp, dy_fp, ms_flag, wcs_flag,
                       decay_flag, &dflag, resmult, pathlen, stop_code, fit_done);
+    if (hg != g) g->used += hg->used;
     if (!ok) return;
     t->xs = *x_fp;
     t->ys = *y_fp;

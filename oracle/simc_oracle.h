@@ -359,6 +359,7 @@ typedef struct {
     uint8_t *ok;
 } or_hep_out;
 #define OR_GEN_TAG 0x48454E47u   /* philox stream of the generator, "GNEH" */
+#define OR_HUT_TAG 0x48555453u   /* philox stream of the detector huts, "HUTS" */
 void or_physics_angles(double theta0, double phi0, double dx, double dy, double *theta,
                        double *phi);
 void or_spectrometer_angles(double theta0, double phi0, double *dx, double *dy, double theta,

@@ -60,6 +60,7 @@ constexpr double kMk = 493.677;
 // mapped to (w + 1/2) 2^-32.
 constexpr uint32_t kPhiloxTag = 0x53494d43u;  // "SIMC"
 constexpr uint32_t kTargetTag = 0x54475453u;  // "TGTS"
+constexpr uint32_t kHutTag = 0x48555453u;     // "HUTS": the detector huts' draws
 
 template <bool LANE_KEY = false>
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -258,6 +259,58 @@ struct PhiloxStreamT {
 };
 using PhiloxStream = PhiloxStreamT<kPhiloxTag>;
 using PhiloxLaneStream = PhiloxStreamT<kPhiloxTag, true>;
+
+// The detector hut's own counter-based stream: uniform k of an event's hut
+// is word k%4 of the Philox block (k/4, event lo, event hi, kHutTag) under
+// the event's key.  Every hut step draws one uniform (resmult) or Box-Muller
+// pairs (both values used, cosine first), so nothing is pending between
+// steps.  The draws depend on launch-uniform flags only: the position is the
+// same in every lane of a wavefront, so block refills and word selection are
+// scalar branches.
+template <bool LANE_KEY>
+struct HutPhilox {
+    uint32_t k0, k1, e0, e1;
+    uint32_t pos;
+    uint32_t w[4];
+    template <uint32_t TAG>
+    __device__ __forceinline__ void init_from(const PhiloxStreamT<TAG, LANE_KEY>& g)
+    {
+        k0 = g.k0; k1 = g.k1; e0 = g.e0; e1 = g.e1;
+        pos = 0;
+    }
+    __device__ __forceinline__ double next()
+    {
+        const uint32_t j = pos & 3u;
+        if (j == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, kHutTag, k0, k1, w);
+        ++pos;
+        const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
+        return ((double)v + 0.5) * 2.3283064365386963e-10;
+    }
+    __device__ __forceinline__ void gauss_pair(double& a, double& b)
+    {
+        const double u1 = next();
+        const double u2 = next();
+        const double r = sqrt(-2.0 * bm_log(u1));
+        double sn, cs;
+        bm_sincos2pi(u2, sn, cs);
+        a = r * cs;
+        b = r * sn;
+    }
+    __device__ __forceinline__ uint32_t used() const { return pos; }
+};
+
+// The stream the detector hut draws from: its own for the counter-based
+// source; a tape (the reference's sequence) continues through the hut.
+template <class R>
+struct HutStreamOf {
+    using type = R;
+    static constexpr bool kOwn = false;
+};
+template <uint32_t TAG, bool LANE_KEY>
+struct HutStreamOf<PhiloxStreamT<TAG, LANE_KEY>> {
+    using type = HutPhilox<LANE_KEY>;
+    static constexpr bool kOwn = true;
+};
 
 template <class R>
 __device__ __forceinline__ double polar_gauss(R& g, double nsigmax);

@@ -416,16 +416,18 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
 // (the materials of musc / musc_ext, the wire planes, boxes, scintillators,
 // trigger, fit and fiducial cut of the mc_*_hut routines).
 //
-// Carried from step to step: the track, the random stream, resmult, the
-// scintillator count and the fitted track.  Wire hits go to LDS
-// (hits[plane][lane]).
+// Carried from step to step: the track, the random streams, resmult, the
+// scintillator count and the fitted track.  The steps draw from the hut's
+// stream hs (HutStreamOf); in-flight decays in the drifts draw from the arm's
+// stream g.  Wire hits go to LDS (hits[plane][lane]).
 // ---------------------------------------------------------------------------
 template <int>
 constexpr bool kNotHutStep = false;
 
-template <int ARM, class R, bool RARE>
+template <int ARM, class R, bool RARE, class S>
 struct Hut {
     R& g;
+    S& hs;
     Track& t;
     const KArgs& a;
     Flags F;
@@ -446,14 +448,14 @@ struct Hut {
         if constexpr (CODE == OP_PROJECT) {
             project(g, t, F, p0);
         } else if constexpr (CODE == OP_RESMULT_DRAW) {
-            resmult = g.next() < p0 ? 2.0 : 1.0;
+            resmult = hs.next() < p0 ? 2.0 : 1.0;
         } else if constexpr (CODE == OP_RESMULT_ONE) {
             resmult = 1.0;
         } else if constexpr (CODE == OP_MUSC) {
             // musc(m2,p,radw,dydzs,dxdzs): first draw to y', second to x'
             if (F.ms && p0 != 0.0) {
                 double d2, d3;
-                g.gauss_pair(d2, d3);
+                hs.gauss_pair(d2, d3);
                 const double ts = ms.theta(t, p1, p2);
                 t.dydzs = t.dydzs + ts * d2;
                 t.dxdzs = t.dxdzs + ts * d3;
@@ -462,8 +464,8 @@ struct Hut {
             // musc_ext(...,dydzs,dxdzs,ys,xs): (x', x) then (y', y)
             if (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) {
                 double d0, d1, d2, d3;
-                g.gauss_pair(d0, d1);
-                g.gauss_pair(d2, d3);
+                hs.gauss_pair(d0, d1);
+                hs.gauss_pair(d2, d3);
                 const double ts = ms.theta(t, p2, p3);
                 const double s12 = 3.4641016151377544;  // sqrt(12.)
                 t.dxdzs = t.dxdzs + ts * d0;
@@ -474,7 +476,7 @@ struct Hut {
         } else if constexpr (CODE == OP_WIRE) {
             // plane STOP (0-based); a y plane when ARG = 1
             double d2 = 0.0, d3 = 0.0;
-            if (F.wcs) g.gauss_pair(d2, d3);
+            if (F.wcs) hs.gauss_pair(d2, d3);
             const double h = ARG ? t.ys + F.sigma * d3 * resmult : t.xs + F.sigma * d2 * resmult;
             lds_hits[STOP * kBlock + threadIdx.x] = h;
         } else if constexpr (CODE == OP_BOX) {
@@ -510,20 +512,39 @@ struct Hut {
     }
 };
 
-// One event through its arm's compiled hut: ST_OK or the stop code.
-template <int ARM, class R, bool RARE>
-__device__ __forceinline__ int run_hut(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off,
-                                       double* lds_hits, double& resmult)
+template <int ARM, class H>
+__device__ __forceinline__ int hut_steps(H& h)
 {
-    Hut<ARM, R, RARE> h{g, t, a, FA, off, lds_hits, resmult, 0, ThetaCache{}, {0.0, 0.0, 0.0, 0.0}};
-    if (!RARE) {  // in-flight decay only in the RARE variant (as in the magnets)
-        h.F.decay = 0;
-        h.F.coll = 0;
-    }
     if constexpr (ARM == SIMC_ARM_HMS) return hut_steps_hms(h);
     else if constexpr (ARM == SIMC_ARM_SHMS) return hut_steps_shms(h);
     else if constexpr (ARM == SIMC_ARM_SOS) return hut_steps_sos(h);
     else return hut_steps_hrs(h);
+}
+
+// One event through its arm's compiled hut: ST_OK or the stop code; `drawn`
+// receives the uniforms taken from the hut's own stream.
+template <int ARM, class R, bool RARE>
+__device__ __forceinline__ int run_hut(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off,
+                                       double* lds_hits, double& resmult, uint32_t& drawn)
+{
+    Flags F = FA;
+    if (!RARE) {  // in-flight decay only in the RARE variant (as in the magnets)
+        F.decay = 0;
+        F.coll = 0;
+    }
+    using S = typename HutStreamOf<R>::type;
+    if constexpr (HutStreamOf<R>::kOwn) {
+        S hs;
+        hs.init_from(g);
+        Hut<ARM, R, RARE, S> h{g, hs, t, a, F, off, lds_hits, resmult, 0, ThetaCache{}, {0.0, 0.0, 0.0, 0.0}};
+        const int st = hut_steps<ARM>(h);
+        drawn = hs.used();
+        return st;
+    } else {
+        Hut<ARM, R, RARE, R> h{g, g, t, a, F, off, lds_hits, resmult, 0, ThetaCache{}, {0.0, 0.0, 0.0, 0.0}};
+        drawn = 0;
+        return hut_steps<ARM>(h);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -619,8 +640,8 @@ __device__ __forceinline__ void reconstruct_event(const KArgs& a, uint32_t off)
 // Outputs of a finished event (byte offset off in the event arrays); an
 // accepted event is reconstructed here unless RECON_LATER (the hut stage
 // hands accepted events to recon_list_kernel).
-template <int ARM, class R, bool RECON_LATER = false>
-__device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int st, const R& g, const Track& t,
+template <int ARM, bool RECON_LATER = false>
+__device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int st, uint32_t used, const Track& t,
                                              double resmult)
 {
     const uint32_t n = a.n_arr;
@@ -636,7 +657,7 @@ __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int s
     st64(a.pathlen, n, off, t.pathlen);
     if (a.resmult) st64(a.resmult, n, off, resmult);
     if (a.m2o) st64(a.m2o, n, off, t.m2);
-    if (a.used) st32(a.used, n, off >> 1, (uint32_t)g.used());
+    if (a.used) st32(a.used, n, off >> 1, used);
     st8(a.ok, n, off >> 3, st == ST_OK ? 1 : 0);
     st8(a.stop, n, off >> 3, (uint8_t)st);
 }
@@ -729,18 +750,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             resmult = a.resmult ? ld64(a.resmult, n, off) : 0.0;
         }
         int st;
-        if constexpr (STAGE == STAGE_HUT) st = run_hut<ARM, R, RARE>(g, t, a, F, e.off, lds_hits, resmult);
+        uint32_t hut_drawn = 0;
+        if constexpr (STAGE == STAGE_HUT) st = run_hut<ARM, R, RARE>(g, t, a, F, e.off, lds_hits, resmult, hut_drawn);
         else st = run_program<ARM, R, RARE>(g, t, a, F, coll_cnt, pc_begin, pc_end);
+        const uint32_t used = (uint32_t)g.used() + hut_drawn;
         if (g.overflow) st = ST_TAPE;
         if (STAGE == STAGE_HUT) {
             // TO_Q: accepted events go to the reconstruction list
-            finish_event<ARM, R, TO_Q>(a, e.off, st, g, t, resmult);
+            finish_event<ARM, TO_Q>(a, e.off, st, used, t, resmult);
             count_stop(a, hist, s_block, s, st, 1u);
             survivor = TO_Q && st == ST_OK;
         } else if (TO_Q && st == ST_OK) {
             survivor = true;
         } else {
-            finish_event<ARM>(a, e.off, st, g, t, resmult);
+            finish_event<ARM>(a, e.off, st, used, t, resmult);
             count_stop(a, hist, s_block, s, st, 1u);
         }
         if (coll_cnt[0] | coll_cnt[1] | coll_cnt[2]) {
@@ -889,20 +912,37 @@ __device__ bool adjoint_track(R& g, Track& t, double (&ta)[5], double (&tb)[5], 
     const ctptr tab = (ctptr)a.tab;
     double resmult = 0.0;
     ThetaCache ms;
+    // the detector hut draws from its own stream (as in run_hut)
+    using HS = typename HutStreamOf<R>::type;
+    HS hs_own;
+    bool in_hut = false;
     // fit tangents: slope / intercept of the x and y plane sets, two directions
     double sax = 0, sbx = 0, say = 0, sby = 0, iax = 0, ibx = 0, iay = 0, iby = 0;
     for (int pc = 0;; ++pc) {
         const int code = ops[pc].code;
         if (code == OP_END) break;
+        if constexpr (HutStreamOf<R>::kOwn) {
+            if (!in_hut && op_stages(code) == STAGE_HUT) {
+                in_hut = true;
+                hs_own.init_from(g);
+            }
+        }
         const double p0 = ops[pc].p[0], p1 = ops[pc].p[1];
         int ng = 0;
         if (code == OP_MUSC) ng = (F.ms && p0 != 0.0) ? 2 : 0;
         else if (code == OP_MUSC_EXT) ng = (F.ms && p0 != 0.0 && p1 > 0.0 && p0 >= 0.0) ? 4 : 0;
         else if (code == OP_WIRE) ng = F.wcs ? 2 : 0;
         double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-        for (int q = 0; q < ng; ++q) {
-            const double v = gauss1(g, 99.0);
-            d0 = d1; d1 = d2; d2 = d3; d3 = v;
+        if (in_hut) {
+            if constexpr (HutStreamOf<R>::kOwn) {
+                if (ng == 4) hs_own.gauss_pair(d0, d1);
+                if (ng) hs_own.gauss_pair(d2, d3);
+            }
+        } else {
+            for (int q = 0; q < ng; ++q) {
+                const double v = gauss1(g, 99.0);
+                d0 = d1; d1 = d2; d2 = d3; d3 = v;
+            }
         }
         switch (code) {
         case OP_PROJECT:
@@ -925,9 +965,13 @@ __device__ bool adjoint_track(R& g, Track& t, double (&ta)[5], double (&tb)[5], 
             tb[0] = ob[0]; tb[1] = ob[1] / 1000.0; tb[2] = ob[2]; tb[3] = ob[3] / 1000.0;
             break;
         }
-        case OP_RESMULT_DRAW:
-            resmult = g.next() < p0 ? 2.0 : 1.0;
+        case OP_RESMULT_DRAW: {
+            double u;
+            if constexpr (HutStreamOf<R>::kOwn) u = hs_own.next();
+            else u = g.next();
+            resmult = u < p0 ? 2.0 : 1.0;
             break;
+        }
         case OP_RESMULT_ONE:
             resmult = 1.0;
             break;
