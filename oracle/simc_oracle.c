@@ -15,6 +15,7 @@
  * gfortran -O on x86-64 does not contract either.
  */
 #include "simc_oracle.h"
+#include "../simc_gfortran_amd/csrc/simc_bm_tables.h"
 
 #include <ctype.h>
 #include <math.h>
@@ -542,74 +543,69 @@ double or_powi(double x, int m)
 }
 
 /* Elementary functions of the Box-Muller generator of the philox stream,
- * the same sequence of IEEE operations as the GPU library (explicit fma,
- * exact reductions), so both produce identical bits.
+ * the same sequence of IEEE operations over the same tables
+ * (simc_bm_tables.h, written by gen_bm_tables.py) as the GPU library, so
+ * both produce identical bits.
  *
- * or_bm_log: log u for u in (0, 1): u = m 2^e, m in [sqrt(1/2), sqrt(2)),
- * log m = 2 atanh s, s = (m-1)/(m+1), as 2s + 2s (z/3 + ... + z^11/23),
- * z = s^2; then e ln2 in two parts (ln2_hi has 21 significant bits). */
+ * or_bm_log: log u for u in (0, 1): u = m 2^e, m in [sqrt(1/2), sqrt(2));
+ * c = 1 + k/64 the nearest table point and invc = 1/c to 7 bits, so
+ * t = m invc - 1 is exact; log m = -log(invc) + log1p(t) (series to t^9);
+ * then e ln2 in two parts (ln2_hi has 21 significant bits). */
+typedef struct {
+    double nhi;
+    float nlo, invc;
+} bm_log_entry;
+static const bm_log_entry bm_log_tab[SIMC_BM_LOG_N] = SIMC_BM_LOG_TABLE;
+static const double bm_sincos_tab[SIMC_BM_SINCOS_N][2] = SIMC_BM_SINCOS_TABLE;
+
 double or_bm_log(double u)
 {
-    uint64_t b;
-    memcpy(&b, &u, 8);
-    int e = (int)(b >> 52) - 1023;
-    uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
-    double m;
-    memcpy(&m, &mb, 8);
-    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
-    double f = m - 1.0;
-    double s = f / (2.0 + f);
-    double z = s * s;
-    double q = 0x1.642c8590b2164p-4;
-    q = fma(z, q, 0x1.8618618618618p-4);
-    q = fma(z, q, 0x1.af286bca1af28p-4);
-    q = fma(z, q, 0x1.e1e1e1e1e1e1ep-4);
-    q = fma(z, q, 0x1.1111111111111p-3);
-    q = fma(z, q, 0x1.3b13b13b13b14p-3);
-    q = fma(z, q, 0x1.745d1745d1746p-3);
-    q = fma(z, q, 0x1.c71c71c71c71cp-3);
-    q = fma(z, q, 0x1.2492492492492p-2);
-    q = fma(z, q, 0x1.999999999999ap-2);
-    q = fma(z, q, 0x1.5555555555555p-1);
-    double t = z * q;
-    double logm = fma(s, t, 2.0 * s);
+    int e;
+    double m = frexp(u, &e); /* [1/2, 1) */
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double kd = nearbyint((m - 1.0) * 64.0);
+    const bm_log_entry *L = &bm_log_tab[(int)kd + SIMC_BM_LOG_K0];
+    double t = fma(m, (double)L->invc, -1.0);
+    double q = 0x1.c71c71c71c71cp-4;
+    q = fma(t, q, -0x1p-3);
+    q = fma(t, q, 0x1.2492492492492p-3);
+    q = fma(t, q, -0x1.5555555555555p-3);
+    q = fma(t, q, 0x1.999999999999ap-3);
+    q = fma(t, q, -0x1p-2);
+    q = fma(t, q, 0x1.5555555555555p-2);
+    q = fma(t, q, -0x1p-1);
+    double p = fma(t * t, q, t);
     double de = (double)e;
-    return fma(de, 0x1.62e42p-1, fma(de, 0x1.fdf473de6af28p-22, logm));
+    return fma(de, 0x1.62e42p-1, L->nhi + (p + fma(de, 0x1.fdf473de6af28p-22, (double)L->nlo)));
 }
 
-/* sin and cos of 2 pi u, u in [0, 1): x = 4u, q = rint(x), r = x - q in
- * [-1/2, 1/2]; Taylor series of sin(pi r/2), cos(pi r/2) in r^2, then the
- * quarter-turn rotation by q. */
+/* sin and cos of 2 pi u, u in [0, 1): x = 64u, k = rint(x), r = x - k in
+ * [-1/2, 1/2]; d = r pi/32, Taylor series of sin d (to d^9) and cos d (to
+ * d^8), then the rotation by the table's sin and cos of 2 pi k/64. */
 void or_bm_sincos2pi(double u, double *sn, double *cs)
 {
-    double x = 4.0 * u;
-    double qd = nearbyint(x);
-    double r = x - qd;
-    double z = r * r;
-    double ps = 0x1.aaec32af93359p-38;
-    ps = fma(z, ps, -0x1.6fadb9f155744p-31);
-    ps = fma(z, ps, 0x1.e8f434d018d63p-25);
-    ps = fma(z, ps, -0x1.e3074fde8871fp-19);
-    ps = fma(z, ps, 0x1.50783487ee782p-13);
-    ps = fma(z, ps, -0x1.32d2cce62bd86p-8);
-    ps = fma(z, ps, 0x1.466bc6775aae2p-4);
-    ps = fma(z, ps, -0x1.4abbce625be53p-1);
-    ps = fma(z, ps, 0x1.921fb54442d18p+0);
-    double sr = r * ps;
-    double pc = 0x1.20c62c2f2d7f5p-34;
-    pc = fma(z, pc, -0x1.b6e24f44b128fp-28);
-    pc = fma(z, pc, 0x1.f9d38a3763cc3p-22);
-    pc = fma(z, pc, -0x1.a6d1f2a204a8cp-16);
-    pc = fma(z, pc, 0x1.e1f506891babbp-11);
-    pc = fma(z, pc, -0x1.55d3c7e3cbffap-6);
-    pc = fma(z, pc, 0x1.03c1f081b5ac4p-2);
-    pc = fma(z, pc, -0x1.3bd3cc9be45dep+0);
-    double cr = fma(z, pc, 1.0);
-    int q = (int)qd & 3;
-    double s0 = (q & 1) ? cr : sr;
-    double c0 = (q & 1) ? sr : cr;
-    *sn = (q & 2) ? -s0 : s0;
-    *cs = ((q + 1) & 2) ? -c0 : c0;
+    double x = u * 64.0;
+    double kd = nearbyint(x);
+    double r = x - kd;
+    double d = r * 0x1.921fb54442d18p-4;
+    double z = d * d;
+    double ps = 0x1.71de3a556c734p-19;
+    ps = fma(z, ps, -0x1.a01a01a01a01ap-13);
+    ps = fma(z, ps, 0x1.1111111111111p-7);
+    ps = fma(z, ps, -0x1.5555555555555p-3);
+    double sd = fma(d * z, ps, d);
+    double pc = 0x1.a01a01a01a01ap-16;
+    pc = fma(z, pc, -0x1.6c16c16c16c17p-10);
+    pc = fma(z, pc, 0x1.5555555555555p-5);
+    pc = fma(z, pc, -0x1p-1);
+    double cd = fma(z, pc, 1.0);
+    int k = (int)kd & 63;
+    double sa = bm_sincos_tab[k][0], ca = bm_sincos_tab[k][1];
+    *sn = fma(sa, cd, ca * sd);
+    *cs = fma(ca, cd, -(sa * sd));
 }
 
 /* Normal deviates of the counter-based (philox) stream: Box-Muller pairs,

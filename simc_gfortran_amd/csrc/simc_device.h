@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "simc_bm_tables.h"
+
 namespace simcdev {
 
 // ---------------------------------------------------------------------------
@@ -99,76 +101,73 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// Elementary functions of the Box-Muller normal generator.  Both are written
-// as fixed sequences of IEEE operations (explicit fma, exact reductions) so
-// that the host restatement produces bit-identical values.
+// Elementary functions of the Box-Muller normal generator.  Both are fixed
+// sequences of IEEE operations (explicit fma, exact reductions) over the
+// tables of simc_bm_tables.h, so that the host restatement produces
+// bit-identical values.
 // ---------------------------------------------------------------------------
+struct __attribute__((aligned(16))) BmLogEntry {
+    double nhi;
+    float nlo, invc;
+};
+static __constant__ BmLogEntry kBmLogTab[SIMC_BM_LOG_N] = SIMC_BM_LOG_TABLE;
+static __constant__ double kBmSinCosTab[SIMC_BM_SINCOS_N][2] = SIMC_BM_SINCOS_TABLE;
 
 // Natural logarithm of u in (0, 1) (normal numbers): u = m 2^e with m in
-// [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), summed as
-// 2s + 2s (z/3 + z^2/5 + ... + z^11/23), z = s^2 <= 0.0295 (truncation
-// below 2^-58 relative), then e ln2 in two parts (ln2_hi has 21 significant
-// bits, so e ln2_hi is exact).
+// [sqrt(1/2), sqrt(2)); c = 1 + k/64 the nearest table point, invc = 1/c to
+// 7 bits, so t = m invc - 1 is exact (|t| < 0.0155); log m = -log(invc) +
+// log1p(t), log1p by its series to t^9 (truncation below 2^-56 relative);
+// then e ln2 in two parts (ln2_hi has 21 significant bits, so e ln2_hi is
+// exact) and the table's two-part -log(invc).
 __device__ __forceinline__ double bm_log(double u)
 {
-    const uint64_t b = __builtin_bit_cast(uint64_t, u);
-    int e = (int)(b >> 52) - 1023;
-    double m = __builtin_bit_cast(double, (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
-    if (m > 1.4142135623730951) { m = m * 0.5; e = e + 1; }
-    const double f = m - 1.0;
-    const double s = f / (2.0 + f);
-    const double z = s * s;
-    double q = 0x1.642c8590b2164p-4;             // 2/23
-    q = __builtin_fma(z, q, 0x1.8618618618618p-4);  // 2/21
-    q = __builtin_fma(z, q, 0x1.af286bca1af28p-4);  // 2/19
-    q = __builtin_fma(z, q, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
-    q = __builtin_fma(z, q, 0x1.1111111111111p-3);  // 2/15
-    q = __builtin_fma(z, q, 0x1.3b13b13b13b14p-3);  // 2/13
-    q = __builtin_fma(z, q, 0x1.745d1745d1746p-3);  // 2/11
-    q = __builtin_fma(z, q, 0x1.c71c71c71c71cp-3);  // 2/9
-    q = __builtin_fma(z, q, 0x1.2492492492492p-2);  // 2/7
-    q = __builtin_fma(z, q, 0x1.999999999999ap-2);  // 2/5
-    q = __builtin_fma(z, q, 0x1.5555555555555p-1);  // 2/3
-    const double t = z * q;
-    const double logm = __builtin_fma(s, t, 2.0 * s);
+    double m = __builtin_amdgcn_frexp_mant(u);  // [1/2, 1)
+    int e = __builtin_amdgcn_frexp_exp(u);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double kd = rint((m - 1.0) * 64.0);
+    const BmLogEntry L = kBmLogTab[(int)kd + SIMC_BM_LOG_K0];
+    const double t = __builtin_fma(m, (double)L.invc, -1.0);
+    double q = 0x1.c71c71c71c71cp-4;                 // 1/9
+    q = __builtin_fma(t, q, -0x1p-3);                // -1/8
+    q = __builtin_fma(t, q, 0x1.2492492492492p-3);   // 1/7
+    q = __builtin_fma(t, q, -0x1.5555555555555p-3);  // -1/6
+    q = __builtin_fma(t, q, 0x1.999999999999ap-3);   // 1/5
+    q = __builtin_fma(t, q, -0x1p-2);                // -1/4
+    q = __builtin_fma(t, q, 0x1.5555555555555p-2);   // 1/3
+    q = __builtin_fma(t, q, -0x1p-1);                // -1/2
+    const double p = __builtin_fma(t * t, q, t);
     const double de = (double)e;
-    return __builtin_fma(de, 0x1.62e42p-1, __builtin_fma(de, 0x1.fdf473de6af28p-22, logm));
+    return __builtin_fma(de, 0x1.62e42p-1, L.nhi + (p + __builtin_fma(de, 0x1.fdf473de6af28p-22, (double)L.nlo)));
 }
 
-// sin and cos of 2 pi u for u in [0, 1): x = 4u (exact), quadrant q =
-// rint(x), r = x - q in [-1/2, 1/2] (exact); sin(pi r/2) and cos(pi r/2) by
-// their Taylor series in r^2 (coefficients (pi/2)^k/k! rounded to double;
-// truncation below 2^-56), then the quarter-turn rotation by q.
+// sin and cos of 2 pi u for u in [0, 1): x = 64u (exact), k = rint(x),
+// r = x - k in [-1/2, 1/2] (exact), d = r pi/32; sin d and cos d by their
+// Taylor series to d^9 and d^8 (|d| <= pi/64, truncation below 2^-62), then
+// the rotation by the table's sin and cos of 2 pi k/64.
 __device__ __forceinline__ void bm_sincos2pi(double u, double& sn, double& cs)
 {
-    const double x = 4.0 * u;
-    const double qd = rint(x);
-    const double r = x - qd;
-    const double z = r * r;
-    double ps = 0x1.aaec32af93359p-38;
-    ps = __builtin_fma(z, ps, -0x1.6fadb9f155744p-31);
-    ps = __builtin_fma(z, ps, 0x1.e8f434d018d63p-25);
-    ps = __builtin_fma(z, ps, -0x1.e3074fde8871fp-19);
-    ps = __builtin_fma(z, ps, 0x1.50783487ee782p-13);
-    ps = __builtin_fma(z, ps, -0x1.32d2cce62bd86p-8);
-    ps = __builtin_fma(z, ps, 0x1.466bc6775aae2p-4);
-    ps = __builtin_fma(z, ps, -0x1.4abbce625be53p-1);
-    ps = __builtin_fma(z, ps, 0x1.921fb54442d18p+0);
-    const double sr = r * ps;
-    double pc = 0x1.20c62c2f2d7f5p-34;
-    pc = __builtin_fma(z, pc, -0x1.b6e24f44b128fp-28);
-    pc = __builtin_fma(z, pc, 0x1.f9d38a3763cc3p-22);
-    pc = __builtin_fma(z, pc, -0x1.a6d1f2a204a8cp-16);
-    pc = __builtin_fma(z, pc, 0x1.e1f506891babbp-11);
-    pc = __builtin_fma(z, pc, -0x1.55d3c7e3cbffap-6);
-    pc = __builtin_fma(z, pc, 0x1.03c1f081b5ac4p-2);
-    pc = __builtin_fma(z, pc, -0x1.3bd3cc9be45dep+0);
-    const double cr = __builtin_fma(z, pc, 1.0);
-    const int q = (int)qd & 3;
-    const double s0 = (q & 1) ? cr : sr;
-    const double c0 = (q & 1) ? sr : cr;
-    sn = (q & 2) ? -s0 : s0;
-    cs = ((q + 1) & 2) ? -c0 : c0;
+    const double x = u * 64.0;
+    const double kd = rint(x);
+    const double r = x - kd;
+    const double d = r * 0x1.921fb54442d18p-4;
+    const double z = d * d;
+    double ps = 0x1.71de3a556c734p-19;                 // 1/9!
+    ps = __builtin_fma(z, ps, -0x1.a01a01a01a01ap-13);  // -1/7!
+    ps = __builtin_fma(z, ps, 0x1.1111111111111p-7);    // 1/5!
+    ps = __builtin_fma(z, ps, -0x1.5555555555555p-3);   // -1/3!
+    const double sd = __builtin_fma(d * z, ps, d);
+    double pc = 0x1.a01a01a01a01ap-16;                 // 1/8!
+    pc = __builtin_fma(z, pc, -0x1.6c16c16c16c17p-10);  // -1/6!
+    pc = __builtin_fma(z, pc, 0x1.5555555555555p-5);    // 1/4!
+    pc = __builtin_fma(z, pc, -0x1p-1);                 // -1/2
+    const double cd = __builtin_fma(z, pc, 1.0);
+    const int k = (int)kd & 63;
+    const double sa = kBmSinCosTab[k][0], ca = kBmSinCosTab[k][1];
+    sn = __builtin_fma(sa, cd, ca * sd);
+    cs = __builtin_fma(ca, cd, -(sa * sd));
 }
 
 // TAG is the fourth counter word: streams of different tags are independent
@@ -209,7 +208,7 @@ struct PhiloxStreamT {
         if (j == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
         ++pos;
         const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
-        return ((double)v + 0.5) * 2.3283064365386963e-10;
+        return __builtin_fma((double)v, 0x1p-32, 0x1p-33);  // (v + 1/2) 2^-32, exact
     }
     // Normal deviates for the counter-based stream: Box-Muller pairs
     // (u1, u2) -> sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2), both used, the
@@ -284,7 +283,7 @@ struct HutPhilox {
         if (j == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, kHutTag, k0, k1, w);
         ++pos;
         const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
-        return ((double)v + 0.5) * 2.3283064365386963e-10;
+        return __builtin_fma((double)v, 0x1p-32, 0x1p-33);  // (v + 1/2) 2^-32, exact
     }
     __device__ __forceinline__ void gauss_pair(double& a, double& b)
     {

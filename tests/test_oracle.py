@@ -209,6 +209,36 @@ def test_bm_sincos2pi_against_libm(oracle):
     assert worst <= 2e-15, worst
 
 
+def test_bm_tables_match_their_generator():
+    """simc_bm_tables.h is what gen_bm_tables.py writes (mpmath at 60 digits,
+    one rounding; every log interval checked for an exact fma(m, invc, -1))."""
+    import importlib.util
+    csrc = os.path.join(os.path.dirname(HERE), "simc_gfortran_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen_bm_tables", os.path.join(csrc, "gen_bm_tables.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(csrc, "simc_bm_tables.h")) as f:
+        assert f.read() == gen.render()
+
+
+def test_bm_log_stream_values_within_one_ulp(oracle):
+    """On values of the stream, (w + 1/2) 2^-32, the logarithm is within one
+    unit in the last place of the correctly rounded one, and sin/cos of 2 pi u
+    within 1.2e-16 (50-digit references)."""
+    import mpmath
+    L = oracle.lib()
+    mpmath.mp.dps = 40
+    w = np.random.default_rng(21).integers(0, 2 ** 32, 4000, dtype=np.uint64)
+    us = list((w.astype(np.float64) + 0.5) * 2.0 ** -32) + [0.5 * 2.0 ** -32, 1.0 - 0.5 * 2.0 ** -32]
+    s, c = C.c_double(), C.c_double()
+    for u in us:
+        want = float(mpmath.log(mpmath.mpf(float(u))))
+        assert abs(L.or_bm_log(float(u)) - want) <= ulp(want), u
+        L.or_bm_sincos2pi(float(u), C.byref(s), C.byref(c))
+        a = 2 * mpmath.pi * mpmath.mpf(float(u))
+        assert abs(s.value - float(mpmath.sin(a))) <= 1.2e-16 and abs(c.value - float(mpmath.cos(a))) <= 1.2e-16
+
+
 def test_philox_normals_pairing(oracle):
     """First pair of event 9, seed 5: u1, u2 = words 0, 1 of Philox block 0;
     the cosine value comes first, the sine value second."""
