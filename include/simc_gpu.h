@@ -40,7 +40,7 @@ extern "C" {
 #define SIMC_E_TOOMANY (-5)   /* > 1000 terms or > 41 classes (transp.f:365,371)     */
 #define SIMC_E_ARG (-6)       /* invalid argument                                    */
 #define SIMC_E_TAPE (-7)      /* an event needed more uniforms than its tape slot    */
-#define SIMC_E_MAPSTRUCT (-8) /* map terms differ from the compiled polynomial structure */
+#define SIMC_E_MAPSTRUCT (-8) /* map terms differ from the compiled polynomial structure (adjoint form), or exceed order 8 */
 #define SIMC_E_HIP (-9)       /* HIP runtime error                                   */
 #define SIMC_E_NOMEM (-10)
 #define SIMC_E_INTERNAL (-11) /* inconsistent step program (library defect)          */
@@ -121,7 +121,12 @@ int simc_ctx_create(const char *map_dir, simc_ctx **out);
 int simc_ctx_create_files(const char *hms_forward, const char *hms_recon,
                           const char *shms_forward, const char *shms_recon, simc_ctx **out);
 /* Load (or replace) one arm's maps: transp_init(spectr) and the recon reader
- * of that arm (arm = SIMC_ARM_HMS, _SOS, _HRSR, _HRSL or _SHMS). */
+ * of that arm (arm = SIMC_ARM_HMS, _SOS, _HRSR, _HRSL or _SHMS).  Maps with
+ * the term structure the library was compiled for run its generated
+ * polynomials; any other tune or fit (other exponents or non-zero pattern,
+ * terms up to order 8) is evaluated from run-time tables built here, at a
+ * lower rate.  The adjoint form needs the compiled structure
+ * (SIMC_E_MAPSTRUCT otherwise). */
 int simc_ctx_load_arm(simc_ctx *ctx, int arm, const char *forward, const char *recon);
 void simc_ctx_destroy(simc_ctx *ctx);
 

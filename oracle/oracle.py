@@ -123,14 +123,50 @@ def lib():
     return _lib
 
 
-def maps():
+_tune_maps = {}
+
+
+def maps(map_dir=None):
+    """The oracle's maps: the default tree (data/maps), or a directory of the
+    same layout (tune_map_dir)."""
     global _maps
+    if map_dir is not None and map_dir != MAP_DIR:
+        if map_dir not in _tune_maps:
+            p = lib().or_maps_load(map_dir.encode())
+            if not p:
+                raise RuntimeError("oracle: cannot load COSY maps from " + map_dir)
+            _tune_maps[map_dir] = p
+        return _tune_maps[map_dir]
     if _maps is None:
         p = lib().or_maps_load(MAP_DIR.encode())
         if not p:
             raise RuntimeError("oracle: cannot load COSY maps from " + MAP_DIR)
         _maps = p
     return _maps
+
+
+TUNE_DIR = os.path.join(REPO, "data", "maps_tunes")
+
+
+def tune_files(tune):
+    """(forward, recon) paths of one of the reference's other SHMS tunes."""
+    return (os.path.join(TUNE_DIR, "shms", tune + "_forward.dat"),
+            os.path.join(TUNE_DIR, "shms", tune + "_recon.dat"))
+
+
+def tune_map_dir(root, tune):
+    """A map directory under `root` holding the default HMS maps and the
+    SHMS tune `tune` under the reference's file names (symbolic links)."""
+    d = os.path.join(root, "maps_" + tune)
+    os.makedirs(os.path.join(d, "shms"), exist_ok=True)
+    if not os.path.exists(os.path.join(d, "hms")):
+        os.symlink(os.path.join(MAP_DIR, "hms"), os.path.join(d, "hms"))
+    fwd, rec = tune_files(tune)
+    for src, name in ((fwd, "shms_forward.dat"), (rec, "shms_recon.dat")):
+        dst = os.path.join(d, "shms", name)
+        if not os.path.exists(dst):
+            os.symlink(src, dst)
+    return d
 
 
 def ref_lib():
@@ -167,9 +203,10 @@ def _prep_outputs(n, ev):
 
 def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, coll=False,
         fit_f32=True, wire_sigma=0.0, rng="philox", tape=None, stride=0, seed=0, event0=0,
-        nthreads=None):
+        nthreads=None, map_dir=None):
     """Run the oracle over SoA events (dict of float64 arrays: dpp, x, y, z, dxdz,
-    dydz, fry).  arm: 1 HMS, 2 SOS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'."""
+    dydz, fry).  arm: 1 HMS, 2 SOS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'.
+    map_dir: a map directory other than the default tree."""
     L = lib()
     n = len(events["dpp"])
     ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
@@ -189,7 +226,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
         tp = _p(tape)
     if nthreads is None:
         nthreads = default_threads()
-    rc = L.or_run(maps(), arm, C.byref(S), n, C.byref(ein), C.byref(eout), kind, tp,
+    rc = L.or_run(maps(map_dir), arm, C.byref(S), n, C.byref(ein), C.byref(eout), kind, tp,
                   int(stride), int(seed), int(event0), int(nthreads), _p(cnt, _i64p))
     out["rc"] = rc
     out["counters"] = cnt[:NCOUNT[arm]]
@@ -264,12 +301,12 @@ def philox_block(ctr, key):
     return list(o)
 
 
-def recon(arm, x_fp, dx_fp, y_fp, dy_fp, fry):
+def recon(arm, x_fp, dx_fp, y_fp, dy_fp, fry, map_dir=None):
     """Reconstruction only: returns delta (%), x'_tar, y'_tar, y_tar (cm)."""
     n = len(x_fp)
     a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_fp, dx_fp, y_fp, dy_fp, fry)]
     outs = [np.zeros(n) for _ in range(4)]
-    rc = lib().or_recon_run(maps(), arm, n, *[_p(v) for v in a], *[_p(v) for v in outs])
+    rc = lib().or_recon_run(maps(map_dir), arm, n, *[_p(v) for v in a], *[_p(v) for v in outs])
     if rc:
         raise RuntimeError(f"or_recon_run: {rc}")
     return tuple(outs)

@@ -80,6 +80,7 @@ struct KArgs {
     uint32_t n_arr;       // elements addressable from the base pointers
     const ArmTables* tab;
     const double* coef;
+    const uint32_t* gen;  // run-time polynomial tables of the arm's maps, or nullptr (compiled)
     const Op* ops;
     const double *dpp, *x, *y, *dxdz, *dydz, *fry;
     double *o_dpp, *o_y, *o_dxdz, *o_dydz, *x_fp, *dx_fp, *y_fp, *dy_fp, *pathlen, *resmult, *m2o;
@@ -94,6 +95,7 @@ enum : int32_t { KF_MS = 1, KF_WCS = 2, KF_DECAY = 4, KF_COLL = 8, KF_FIT_F32 = 
 typedef const double __attribute__((address_space(4)))* cdptr;
 typedef const Op __attribute__((address_space(4)))* coptr;
 typedef const ArmTables __attribute__((address_space(4)))* ctptr;
+typedef const uint32_t __attribute__((address_space(4)))* cuptr;
 
 // Results of one event besides the track.
 struct EvOut {
@@ -144,6 +146,54 @@ __device__ __forceinline__ void poly_fwd(int cls, cdptr c, double r0, double r1,
     }
 }
 
+// Run-time polynomial of class cls (0 = recon) from simc::build_generic's
+// tables, for maps whose term structure differs from the compiled one:
+// out_k = sum over the class's trie nodes of c_k * monomial, each node's
+// monomial one multiply from its parent's (kept per depth), one FMA per
+// non-zero coefficient (shared/transp.f:205-218, hms/mc_hms_recon.f:122-130).
+// Every index is wavefront-uniform: scalar loads and scalar branches.
+template <int NOUT>
+__device__ __forceinline__ void poly_generic(cuptr g, int cls, double r0, double r1, double r2, double r3, double r4,
+                                             double (&out)[NOUT])
+{
+    const uint32_t first = g[cls], n = g[33 + cls];
+    const cuptr rec = g + g[66];
+    const cdptr coef = (cdptr)g + g[67];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) out[o] = 0.0;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0, s8 = 0.0;
+    for (uint32_t k = first; k < first + n; ++k) {
+        const uint32_t h = rec[2 * k];
+        uint32_t c = rec[2 * k + 1];
+        double x;
+        switch ((h >> 4) & 15u) {
+        case 0: x = r0; break;
+        case 1: x = r1; break;
+        case 2: x = r2; break;
+        case 3: x = r3; break;
+        default: x = r4; break;
+        }
+        double m;
+        switch (h & 15u) {
+        case 0: m = 1.0; break;
+        case 1: s1 = x; m = s1; break;
+        case 2: s2 = s1 * x; m = s2; break;
+        case 3: s3 = s2 * x; m = s3; break;
+        case 4: s4 = s3 * x; m = s4; break;
+        case 5: s5 = s4 * x; m = s5; break;
+        case 6: s6 = s5 * x; m = s6; break;
+        case 7: s7 = s6 * x; m = s7; break;
+        default: s8 = s7 * x; m = s8; break;
+        }
+#pragma unroll
+        for (int o = 0; o < NOUT; ++o)
+            if (h & (256u << o)) {
+                out[o] = __builtin_fma(coef[c], m, out[o]);
+                ++c;
+            }
+    }
+}
+
 template <int ARM>
 __device__ __forceinline__ void poly_recon(cdptr c, double h0, double h1, double h2, double h3, double h4,
                                            double& s0, double& s1, double& s2, double& s3)
@@ -184,14 +234,20 @@ __device__ __forceinline__ void poly_fwd_tan(int cls, cdptr c, const double (&r)
 // hrsl/mc_hrsl_recon.f:113-137, sos/mc_sos_recon.f:113-137: the SOS keeps
 // every input away from 0, the others only x_tar)
 template <int ARM>
-__device__ __forceinline__ void recon(cdptr c, const EvOut& o, double fry, EvOut& r)
+__device__ __forceinline__ void recon(cdptr c, cuptr gen, const EvOut& o, double fry, EvOut& r)
 {
     double h[5] = {o.x_fp / 100.0, o.dx_fp, o.y_fp / 100.0, o.dy_fp, fry / 100.0};
 #pragma unroll
     for (int k = ARM == SIMC_ARM_SOS ? 0 : 4; k < 5; ++k)
         if (fabs(h[k]) <= 1.e-30) h[k] = 1.e-30;
     double s0, s1, s2, s3;
-    poly_recon<ARM>(c, h[0], h[1], h[2], h[3], h[4], s0, s1, s2, s3);
+    if (gen) {
+        double s[4];
+        poly_generic<4>(gen, 0, h[0], h[1], h[2], h[3], h[4], s);
+        s0 = s[0]; s1 = s[1]; s2 = s[2]; s3 = s[3];
+    } else {
+        poly_recon<ARM>(c, h[0], h[1], h[2], h[3], h[4], s0, s1, s2, s3);
+    }
     r.dxdz = s0;         // delta_phi
     r.y = s1 * 100.0;    // y_tgt
     r.dydz = s2;         // delta_t
@@ -293,9 +349,11 @@ __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, con
 // The step-program interpreter of the magnet stages: one event of arm ARM,
 // steps [pc_begin, pc_end) (drifts, COSY segments, apertures and the
 // collimator).  Only the track, the random stream and the tilted-plane point
-// (xt, yt) are carried from step to step.
+// (xt, yt) are carried from step to step.  GEN: the arm's maps run from
+// run-time polynomial tables (a copy of its own, so that the compiled
+// polynomials' code is not shaped by the other path).
 // ---------------------------------------------------------------------------
-template <int ARM, class R, bool RARE>
+template <int ARM, class R, bool RARE, bool GEN>
 __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, unsigned* coll_cnt,
                                            int pc_begin, int pc_end)
 {
@@ -324,7 +382,13 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
             const int cls = ops[pc].arg;
             transp(g, t, F, p0, [&](double r0, double r1, double r2, double r3, double r4, double& o0,
                                     double& o1, double& o2, double& o3, double& o4) {
-                poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
+                if constexpr (GEN) {
+                    double o[5];
+                    poly_generic<5>((cuptr)a.gen, cls, r0, r1, r2, r3, r4, o);
+                    o0 = o[0]; o1 = o[1]; o2 = o[2]; o3 = o[3]; o4 = o[4];
+                } else {
+                    poly_fwd<ARM>(cls, coef, r0, r1, r2, r3, r4, o0, o1, o2, o3, o4);
+                }
             });
             break;
         }
@@ -629,7 +693,7 @@ __device__ __forceinline__ void reconstruct_event(const KArgs& a, uint32_t off)
     o.dx_fp = ld64(a.dx_fp, n, off);
     o.y_fp = ld64(a.y_fp, n, off);
     o.dy_fp = ld64(a.dy_fp, n, off);
-    recon<ARM>((cdptr)a.coef, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
+    recon<ARM>((cdptr)a.coef, (cuptr)a.gen, o, a.fry ? ld64(a.fry, n, off) : ld64(a.x, n, off), r);
     if (ARM == SIMC_ARM_HRSR) st64(a.y_fp, n, off, o.y_fp - ((ctptr)a.tab)->yfp_shift);
     st64(a.o_dpp, n, off, r.dpp);
     st64(a.o_y, n, off, r.y);
@@ -752,7 +816,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
         int st;
         uint32_t hut_drawn = 0;
         if constexpr (STAGE == STAGE_HUT) st = run_hut<ARM, R, RARE>(g, t, a, F, e.off, lds_hits, resmult, hut_drawn);
-        else st = run_program<ARM, R, RARE>(g, t, a, F, coll_cnt, pc_begin, pc_end);
+        else if (a.gen) st = run_program<ARM, R, RARE, true>(g, t, a, F, coll_cnt, pc_begin, pc_end);
+        else st = run_program<ARM, R, RARE, false>(g, t, a, F, coll_cnt, pc_begin, pc_end);
         const uint32_t used = (uint32_t)g.used() + hut_drawn;
         if (g.overflow) st = ST_TAPE;
         if (STAGE == STAGE_HUT) {
@@ -1240,7 +1305,8 @@ __global__ void zero_u64(unsigned long long* p, uint32_t n)
 }
 
 template <int ARM>
-__global__ __launch_bounds__(256) void recon_kernel(int64_t n, const double* coef, const double* x_fp,
+__global__ __launch_bounds__(256) void recon_kernel(int64_t n, const double* coef, const uint32_t* gen,
+                                                     const double* x_fp,
                                                      const double* dx_fp, const double* y_fp,
                                                      const double* dy_fp, const double* fry, double* delta,
                                                      double* dxdz, double* dydz, double* ytar)
@@ -1249,7 +1315,7 @@ __global__ __launch_bounds__(256) void recon_kernel(int64_t n, const double* coe
     if (i < n) {
         EvOut o, r;
         o.x_fp = x_fp[i]; o.dx_fp = dx_fp[i]; o.y_fp = y_fp[i]; o.dy_fp = dy_fp[i];
-        recon<ARM>((cdptr)coef, o, fry ? fry[i] : 0.0, r);
+        recon<ARM>((cdptr)coef, (cuptr)gen, o, fry ? fry[i] : 0.0, r);
         delta[i] = r.dpp; dxdz[i] = r.dxdz; dydz[i] = r.dydz; ytar[i] = r.y;
     }
 }
@@ -1476,6 +1542,7 @@ struct FieldToArgs {
     double stht, ctht;
     uint32_t n;
     const double* coef;  // the arm's packed coefficients (reconstruction)
+    const uint32_t* gen;  // or its run-time polynomial tables
     const double *x_fp, *dx_fp, *y_fp, *dy_fp;
     double *delta, *y, *dx, *dy;
     const double *frx, *fry, *mom;
@@ -1571,7 +1638,7 @@ __global__ __launch_bounds__(256) void field_to_recon(FieldToArgs a)
     o.dx_fp = a.dx_fp[i];
     o.y_fp = a.y_fp[i];
     o.dy_fp = a.dy_fp[i];
-    recon<ARM>((cdptr)a.coef, o, a.xx[i], r);
+    recon<ARM>((cdptr)a.coef, (cuptr)a.gen, o, a.xx[i], r);
     a.delta[i] = r.dpp;
     a.dy[i] = r.dydz;
     a.dx[i] = r.dxdz;
@@ -1641,6 +1708,10 @@ struct simc_ctx {
     simc::ForwardMap fwd[kNumArms];
     simc::ReconMap rec[kNumArms];
     double* d_coef[kNumArms] = {};   // null: the arm's maps are not loaded
+    // run-time polynomial tables of an arm whose maps differ in structure from
+    // the compiled polynomials (null: the compiled ones serve)
+    uint32_t* d_gen[kNumArms] = {};
+    simc::GenericPoly gen[kNumArms];
     Op* d_ops[kNumArms] = {};
     ArmTables* d_tab[kNumArms] = {};
     std::vector<Op> ops[kNumArms];
@@ -1740,6 +1811,15 @@ ArmTables make_tables(const simc_ctx* ctx, int arm)
     return T;
 }
 
+// Algorithmic FP64 operations of one pass through a class's polynomial
+// (cls = 0: recon) as the arm's maps are evaluated.
+int64_t class_flops(const simc_ctx* ctx, int arm, int cls)
+{
+    const int ai = arm_index(arm);
+    if (ctx->d_gen[ai]) return cls >= 0 && cls <= 32 ? ctx->gen[ai].flops[cls] : 0;
+    return simc::poly_flops(arm, cls);
+}
+
 // The hut stage runs the compiled hut of the arm's kernel family
 // (hut_gen.inc): the loaded program's hut steps must be that sequence, with
 // the same literal parameters.
@@ -1778,7 +1858,16 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     if (rc) return fail(rc, msg);
     std::vector<double> packed;
     rc = simc::pack_arm(arm, ctx->fwd[ai], ctx->rec[ai], packed, msg);
-    if (rc) return fail(rc, msg);
+    std::vector<uint8_t> gen_bytes;
+    if (rc == SIMC_E_MAPSTRUCT) {
+        // another tune or fit: evaluate its polynomials from run-time tables
+        rc = simc::build_generic(ctx->fwd[ai], ctx->rec[ai], ctx->gen[ai], msg);
+        if (rc) return fail(rc, msg);
+        ctx->gen[ai].pack(gen_bytes);
+        packed.assign(1, 0.0);
+    } else if (rc) {
+        return fail(rc, msg);
+    }
     simcprog::Program P;
     simcprog::build_program(arm, ctx->fwd[ai], P, ctx->zdc[ai]);
     ctx->n_ops[ai] = (int)P.ops.size();
@@ -1807,6 +1896,10 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
     }
     HIPCHK(hipMalloc(&ctx->d_coef[ai], packed.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_coef[ai], packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!gen_bytes.empty()) {
+        HIPCHK(hipMalloc(&ctx->d_gen[ai], gen_bytes.size()));
+        HIPCHK(hipMemcpy(ctx->d_gen[ai], gen_bytes.data(), gen_bytes.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMalloc(&ctx->d_ops[ai], P.ops.size() * sizeof(Op)));
     HIPCHK(hipMemcpy(ctx->d_ops[ai], P.ops.data(), P.ops.size() * sizeof(Op), hipMemcpyHostToDevice));
     const ArmTables T = make_tables(ctx, arm);
@@ -2090,9 +2183,10 @@ int simc_ctx_load_arm(simc_ctx* ctx, int arm, const char* forward, const char* r
     if (ai < 0) return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), _SOS (2), _HRSR (3), _HRSL (4) or _SHMS (5)");
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipDeviceSynchronize());  // runs of the arm may still use its tables
-    void* old[5] = {ctx->d_coef[ai], ctx->d_ops[ai], ctx->d_tab[ai], ctx->d_rexp[ai], ctx->d_hist[ai]};
+    void* old[6] = {ctx->d_coef[ai], ctx->d_ops[ai], ctx->d_tab[ai], ctx->d_rexp[ai], ctx->d_hist[ai],
+                    ctx->d_gen[ai]};
     ctx->d_coef[ai] = nullptr; ctx->d_ops[ai] = nullptr; ctx->d_tab[ai] = nullptr;
-    ctx->d_rexp[ai] = nullptr; ctx->d_hist[ai] = nullptr;
+    ctx->d_rexp[ai] = nullptr; ctx->d_hist[ai] = nullptr; ctx->d_gen[ai] = nullptr;
     for (void* p : old)
         if (p) (void)hipFree(p);
     return load_arm(ctx, arm, forward, recon);
@@ -2103,6 +2197,7 @@ void simc_ctx_destroy(simc_ctx* ctx)
     if (!ctx) return;
     for (int i = 0; i < kNumArms; ++i) {
         if (ctx->d_coef[i]) (void)hipFree(ctx->d_coef[i]);
+        if (ctx->d_gen[i]) (void)hipFree(ctx->d_gen[i]);
         if (ctx->d_ops[i]) (void)hipFree(ctx->d_ops[i]);
         if (ctx->d_tab[i]) (void)hipFree(ctx->d_tab[i]);
         if (ctx->d_hist[i]) (void)hipFree(ctx->d_hist[i]);
@@ -2241,6 +2336,7 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
     a.seed = rng->seed;
     a.tab = ctx->d_tab[ai];
     a.coef = ctx->d_coef[ai];
+    a.gen = ctx->d_gen[ai];
     a.ops = ctx->d_ops[ai];
     a.hist = ctx->d_hist[ai];
     const bool tape = rng->kind == SIMC_RNG_TAPE;
@@ -2481,16 +2577,16 @@ int simc_recon_run(simc_ctx* ctx, int arm, int64_t n, const double* x_fp, const 
     if (blocks > 0x7fffffff) return fail(SIMC_E_ARG, "too many events for one launch");
     hipStream_t st = (hipStream_t)stream;
     if (arm == SIMC_ARM_HMS)
-        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], ctx->d_gen[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else if (arm == SIMC_ARM_SOS)
-        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SOS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SOS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], ctx->d_gen[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else if (is_hrs(arm))
-        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HRSR>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_HRSR>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], ctx->d_gen[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     else
-        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], x_fp,
+        hipLaunchKernelGGL(recon_kernel<SIMC_ARM_SHMS>, dim3(blocks), dim3(256), 0, st, n, ctx->d_coef[ai], ctx->d_gen[ai], x_fp,
                            dx_fp, y_fp, dy_fp, fry, delta, dxdz_tar, dydz_tar, y_tar);
     HIPCHK(hipGetLastError());
     return SIMC_OK;
@@ -3221,6 +3317,7 @@ int simc_track_to_tgt(simc_ctx* ctx, const simc_field* f, int arm, int spect, in
     a.map = f->d_map;
     a.n = (uint32_t)n;
     a.coef = ctx->d_coef[ai];
+    a.gen = ctx->d_gen[ai];
     a.x_fp = x_fp; a.dx_fp = dx_fp; a.y_fp = y_fp; a.dy_fp = dy_fp;
     a.delta = delta; a.y = y; a.dx = dx; a.dy = dy;
     a.frx = frx; a.fry = fry; a.mom = mom;
@@ -3373,13 +3470,13 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
     for (int pc = 0; pc < (int)ops.size(); ++pc) {
         const Op& o = ops[pc];
         const int in_stage = pc < ctx->coll_pc[ai] ? 1 : pc < ctx->hut_pc[ai] ? 2 : 3;
-        if (o.code == OP_TRANSP && (stage == 0 || stage == in_stage)) acc += simc::poly_flops(arm, o.arg);
+        if (o.code == OP_TRANSP && (stage == 0 || stage == in_stage)) acc += class_flops(ctx, arm, o.arg);
         if (o.code == OP_COLLIMATOR) {
             flops[ST_SLIT_HOR] = flops[ST_SLIT_VERT] = flops[ST_SLIT_OCT] = flops[ST_COLL] = acc;
         } else if (o.stop > 0 && o.code != OP_WIRE && o.stop < kHistBins) {
             flops[o.stop] = acc;
         }
-        if (o.code == OP_END) flops[ST_OK] = acc + (stage == 0 || stage == 3 ? simc::poly_flops(arm, 0) : 0);
+        if (o.code == OP_END) flops[ST_OK] = acc + (stage == 0 || stage == 3 ? class_flops(ctx, arm, 0) : 0);
     }
     return SIMC_OK;
 }
@@ -3415,6 +3512,8 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     if (arm != SIMC_ARM_HMS && arm != SIMC_ARM_SHMS)
         return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
     if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
+    if (ctx->d_coef[ai] && ctx->d_gen[ai])
+        return fail(SIMC_E_MAPSTRUCT, "the adjoint form needs maps of the compiled structure");
     if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");
     if (n <= 0 || n > ((int64_t)1 << 29)) return fail(SIMC_E_ARG, "simc_adjoint_run: n must be in [1, 2^29]");
     if (rng->kind != SIMC_RNG_PHILOX && rng->kind != SIMC_RNG_TAPE) return fail(SIMC_E_ARG, "unknown rng kind");
@@ -3608,6 +3707,7 @@ int simc_batch_run(simc_batch* b, const simc_events_in* in, simc_events_out* out
         a.n_arr = b->n_arr;
         a.tab = ctx->d_tab[ai];
         a.coef = ctx->d_coef[ai];
+        a.gen = ctx->d_gen[ai];
         a.ops = ctx->d_ops[ai];
         a.hist = b->d_hist;
         a.dpp = in->dpp; a.x = in->x; a.y = in->y; a.dxdz = in->dxdz; a.dydz = in->dydz; a.fry = in->fry;
@@ -3711,13 +3811,13 @@ int64_t simc_poly_flops_full(simc_ctx* ctx, int arm)
     int64_t f = 0;
     if (!ctx->d_coef[arm_index(arm)]) return 0;
     if (arm == SIMC_ARM_HMS)
-        for (int c : hms_cls) f += simc::poly_flops(arm, c);
+        for (int c : hms_cls) f += class_flops(ctx, arm, c);
     else if (arm == SIMC_ARM_SOS)
-        for (int c : sos_cls) f += simc::poly_flops(arm, c);
+        for (int c : sos_cls) f += class_flops(ctx, arm, c);
     else if (is_hrs(arm))
-        for (int c : hrs_cls) f += simc::poly_flops(arm, c);
+        for (int c : hrs_cls) f += class_flops(ctx, arm, c);
     else
-        for (int c : shms_cls) f += simc::poly_flops(arm, c);
+        for (int c : shms_cls) f += class_flops(ctx, arm, c);
     return f;
 }
 

@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
+#include <map>
 
 #include "../../include/simc_gpu.h"
 #include "cosy_gen.host.inc"
@@ -266,6 +268,72 @@ int pack_arm(int arm, const ForwardMap& f, const ReconMap& r, std::vector<double
                 }
     }
     return SIMC_OK;
+}
+
+namespace {
+
+// One class's monomial trie: the terms as sorted variable sequences (x0^2 x3
+// -> 0 0 3), every prefix present, in lexicographic (= depth-first) order.
+int add_generic_class(const std::vector<CosyTerm>& terms, int nout, GenericPoly& g, int cls, std::string& msg)
+{
+    std::map<std::vector<uint8_t>, std::vector<int>> nodes;  // sequence -> terms with that monomial
+    nodes[{}];
+    for (int t = 0; t < (int)terms.size(); ++t) {
+        std::vector<uint8_t> seq;
+        for (int v = 0; v < 5; ++v)
+            for (int k = 0; k < terms[t].e[v]; ++k) seq.push_back((uint8_t)v);
+        if ((int)seq.size() > kGenMaxDepth) {
+            msg = "class " + std::to_string(cls) + ": term of order " + std::to_string(seq.size()) +
+                  " beyond the run-time polynomial's " + std::to_string(kGenMaxDepth);
+            return SIMC_E_MAPSTRUCT;
+        }
+        nodes[seq].push_back(t);
+        for (size_t d = 1; d < seq.size(); ++d) nodes[std::vector<uint8_t>(seq.begin(), seq.begin() + d)];
+    }
+    g.words[cls] = (uint32_t)(g.words.size() - 68) / 2;
+    g.words[33 + cls] = (uint32_t)nodes.size();
+    int64_t flops = 0;
+    for (const auto& [seq, ts] : nodes) {
+        double c[5] = {0, 0, 0, 0, 0};
+        for (int t : ts)  // a monomial listed twice sums its coefficients
+            for (int o = 0; o < nout; ++o) c[o] += terms[t].c[o];
+        uint32_t mask = 0;
+        for (int o = 0; o < nout; ++o)
+            if (c[o] != 0.0) mask |= 1u << o;
+        const uint32_t d = (uint32_t)seq.size(), v = d ? seq.back() : 0u;
+        g.words.push_back(d | v << 4 | mask << 8);
+        g.words.push_back((uint32_t)g.coef.size());
+        for (int o = 0; o < nout; ++o)
+            if (mask & (1u << o)) g.coef.push_back(c[o]);
+        flops += (d > 1 ? 1 : 0) + 2 * __builtin_popcount(mask);
+    }
+    g.flops[cls] = flops;
+    g.terms[cls] = (int64_t)terms.size();
+    return SIMC_OK;
+}
+
+}  // namespace
+
+int build_generic(const ForwardMap& f, const ReconMap& r, GenericPoly& g, std::string& msg)
+{
+    g.words.assign(68, 0u);
+    g.coef.clear();
+    int rc = add_generic_class(r.terms, 4, g, 0, msg);
+    for (int c = 1; !rc && c <= f.n_classes && c <= 32; ++c) rc = add_generic_class(f.classes[c - 1], 5, g, c, msg);
+    if (rc) return rc;
+    g.words[66] = 68;
+    // coefficients start at the next 8-byte boundary after the words
+    g.words[67] = (uint32_t)((g.words.size() * 4 + 7) / 8);
+    return SIMC_OK;
+}
+
+size_t GenericPoly::bytes() const { return (size_t)words[67] * 8 + coef.size() * 8; }
+
+void GenericPoly::pack(std::vector<uint8_t>& out) const
+{
+    out.assign(bytes(), 0);
+    std::memcpy(out.data(), words.data(), words.size() * 4);
+    std::memcpy(out.data() + (size_t)words[67] * 8, coef.data(), coef.size() * 8);
 }
 
 int64_t poly_flops(int arm, int cls)

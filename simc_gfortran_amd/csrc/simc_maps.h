@@ -44,4 +44,28 @@ int pack_arm(int arm, const ForwardMap& f, const ReconMap& r, std::vector<double
 int64_t poly_flops(int arm, int cls);
 int64_t poly_terms(int arm, int cls);
 
+// Run-time polynomial tables, for maps whose term structure differs from the
+// compiled one (another tune or fit of the optics).  Each class (0 = recon)
+// is a list of monomial-trie nodes in depth-first order: node = (depth d,
+// variable v, output mask), its monomial = the monomial of the last node of
+// depth d-1 times input v, so every node costs one multiply; parents a map
+// lacks are inserted without coefficients.  Coefficients of each node's
+// non-zero outputs follow in output order.  One device buffer:
+//   uint32 words [0, 33)   first node of class c
+//   uint32 words [33, 66)  node count of class c
+//   uint32 word  66        word offset of the node records, 67: double offset
+//                          of the coefficients (in doubles from the start)
+//   node k: word rec + 2k = d | v << 4 | mask << 8, rec + 2k + 1 = its first
+//   coefficient (index into the coefficient doubles)
+constexpr int kGenMaxDepth = 8;
+struct GenericPoly {
+    std::vector<uint32_t> words;  // the header and node records
+    std::vector<double> coef;
+    int64_t flops[33] = {};      // multiplies + 2 per coefficient, per class
+    int64_t terms[33] = {};      // the map's terms, per class
+    size_t bytes() const;        // size of the device buffer
+    void pack(std::vector<uint8_t>& out) const;
+};
+int build_generic(const ForwardMap& f, const ReconMap& r, GenericPoly& g, std::string& msg);
+
 }  // namespace simc

@@ -410,10 +410,43 @@ MAP_SHA = {
 }
 
 
+# the reference tree's other SHMS tunes (shms/), copied verbatim: their term
+# structures differ from the default pair's
+TUNE_SHA = {
+    "shms/shms-2017-26cm-monte_quads_p18_forward.dat":
+        "3ff63af667d2bf557b90a349348c93eb1b601d1c343eecdaec3a2dc271280221",
+    "shms/shms-2017-26cm-monte_quads_p18_recon.dat":
+        "e22c8049dc359745e4237414c145d92983d17a5bc4260621e10feead5ab3f0b3",
+    "shms/shms-2011-26cm-monte_ideal_6ord_forward.dat":
+        "c757b6f3446dc20108186118f3ca0f16565a3e63672ed596cf38a7398c87c71b",
+    "shms/shms-2011-26cm-monte_ideal_6ord_recon.dat":
+        "015f8e891b9f7a81bd008d189e8afb7a4787bfd4bb1f8f3ecbe48859247090d3",
+}
+
+
 def test_map_files_checksums():
     for rel, sha in MAP_SHA.items():
         with open(os.path.join(REPO, "data", "maps", rel), "rb") as f:
             assert hashlib.sha256(f.read()).hexdigest() == sha, rel
+    for rel, sha in TUNE_SHA.items():
+        with open(os.path.join(REPO, "data", "maps_tunes", rel), "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == sha, rel
+
+
+@pytest.mark.parametrize("tune", ["shms-2017-26cm-monte_quads_p18", "shms-2011-26cm-monte_ideal_6ord"])
+def test_oracle_runs_other_tunes(oracle, tmp_path, tune):
+    """The oracle reads any tune (transp_init and the recon reader are
+    structure-free): a tune's map directory gives a run whose reconstruction
+    differs from the default tune's and accepts a plausible fraction."""
+    from simc_gfortran_amd import workloads as W
+    d = oracle.tune_map_dir(str(tmp_path), tune)
+    ev = W.generate(W.SHMS_C1, 4000)
+    kw = dict(p_spec=W.SHMS_C1.p_spec, m2=W.SHMS_C1.m2, rng="philox", seed=3, fit_f32=False)
+    a = oracle.run(5, ev, map_dir=d, **kw)
+    b = oracle.run(5, ev, **kw)
+    assert a["rc"] == 0 and 0.2 < a["ok"].mean() < 0.7
+    both = (a["ok"] == 1) & (b["ok"] == 1)
+    assert both.sum() > 500 and np.abs(a["dpp"][both] - b["dpp"][both]).max() > 1e-6
 
 
 class FwdMap(C.Structure):
