@@ -633,14 +633,51 @@ def bench_host_entry(ctx, wl, ev, reps=3):
     for r in range(reps + 1):
         t0 = time.perf_counter()
         out = ctx.run_host(wl.arm, ev, p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma,
-                           fit="fp64", seed=wl.seed)
+                           fit="fp64", seed=wl.seed, out=out)
         if r:
             ts.append(time.perf_counter() - t0)
     n = len(ev["dpp"])
     ms = float(np.mean(ts)) * 1e3
     return {"workload": wl.name, "events": n, "ms": ms, "events_per_s": n / (ms * 1e-3),
             "accepted_fraction": float(np.mean(out["ok"])),
-            "data": "host numpy arrays in and out (pageable memory), staging copies included"}
+            "data": "host numpy arrays in and out (pageable memory, allocated once), the library's chunked "
+                    "pinned staging and transfers included"}
+
+
+def bench_dropin(wl, calls=2000):
+    """The per-event drop-in (libsimc_dropin.so's mc_shms_, one event per call
+    as montecarlo() calls it, simc.f:1464-1488): microseconds per call, after
+    a warm-up that creates its context."""
+    import ctypes as C
+    from simc_gfortran_amd import workloads as W
+    os.environ.setdefault("SIMC_MAP_DIR", os.path.join(REPO, "data", "maps"))
+    L = C.CDLL(os.path.join(REPO, "simc_gfortran_amd", "libsimc_dropin.so"))
+    d = lambda v=0.0: C.c_double(v)
+    i = lambda v=0: C.c_int32(v)
+    ev = W.generate(wl, calls + 100, seed=5)
+    args_f = [d() for _ in range(13)]
+    p_spec, th, m2 = d(wl.p_spec), d(0.0), d(wl.m2)
+    ms, wcs, dec, ok, spectr, coll = i(1), i(1), i(0), i(0), i(5), i(0)
+    resmult, fry, pathlen = d(), d(), d()
+    dpp, x, y, z, dxdz, dydz, x_fp, dx_fp, y_fp, dy_fp = [d() for _ in range(10)]
+    ts = []
+    acc = 0
+    for k in range(calls + 100):
+        dpp.value, x.value, y.value = ev["dpp"][k], ev["x"][k], ev["y"][k]
+        z.value, dxdz.value, dydz.value = ev["z"][k], ev["dxdz"][k], ev["dydz"][k]
+        fry.value, pathlen.value, m2.value = ev["fry"][k], 0.0, wl.m2
+        t0 = time.perf_counter()
+        L.mc_shms_(C.byref(p_spec), C.byref(th), C.byref(dpp), C.byref(x), C.byref(y), C.byref(z), C.byref(dxdz),
+                   C.byref(dydz), C.byref(x_fp), C.byref(dx_fp), C.byref(y_fp), C.byref(dy_fp), C.byref(m2),
+                   C.byref(ms), C.byref(wcs), C.byref(dec), C.byref(resmult), C.byref(fry), C.byref(ok),
+                   C.byref(pathlen), C.byref(spectr), C.byref(coll))
+        if k >= 100:
+            ts.append(time.perf_counter() - t0)
+            acc += ok.value
+    us = float(np.median(ts)) * 1e6
+    return {"workload": wl.name, "calls": calls, "us_per_call_median": us,
+            "us_per_call_mean": float(np.mean(ts)) * 1e6, "accepted_fraction": acc / calls,
+            "data": "one event per mc_shms_ call through ctypes (the Fortran ABI), synchronous"}
 
 
 def bench_arms(ctx, n=10_000_000, reps=5):
@@ -883,6 +920,7 @@ def main():
         line["generator"] = bench_gen(ctx)
         line["c2_radiative_chain"] = bench_c2(ctx)
         line["host_entry"] = bench_host_entry(ctx, wl, ev)
+        line["dropin_per_event"] = bench_dropin(wl)
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         line["cpu_baseline"] = cb

@@ -171,12 +171,16 @@ int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_even
 int simc_arm_run_noalloc(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                          simc_events_out *out, int64_t n, const simc_rng *rng, void *stream);
 
-/* simc_arm_run on HOST arrays (the same structs, host pointers; the tape too):
- * the library copies the passed arrays to device staging memory it owns,
- * runs on the default stream and copies every passed output back before
- * returning (synchronous); ok and stop may be NULL here.  For callers without
- * device memory management, e.g. a Fortran batch driver of montecarlo()
- * (simc.f:1464-1488).  Calls on one context are serialised. */
+/* simc_arm_run on HOST arrays (the same structs, host pointers; the tape too),
+ * synchronous: every passed output is written before the call returns; ok
+ * and stop may be NULL here.  The events go through in chunks of up to 2^20
+ * on two library-owned streams with pinned staging buffers, so that the
+ * copies into and out of the caller's arrays (split over host threads), the
+ * transfers and the three stages of neighbouring chunks overlap.  Only the
+ * columns the kernels read are sent (the inputs, and x_fp..dy_fp and
+ * pathlen, which keep or accumulate the caller's values).  For callers
+ * without device memory management, e.g. a Fortran batch driver of
+ * montecarlo() (simc.f:1464-1488).  Calls on one context are serialised. */
 int simc_arm_run_host(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                       simc_events_out *out, int64_t n, const simc_rng *rng);
 

@@ -426,18 +426,25 @@ class Spectrometers:
 
     def run_host(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
                  coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
-                 tape=None, stride=0):
+                 tape=None, stride=0, out=None):
         """run() on numpy arrays through the synchronous host-pointer entry point.
-        Returns a dict of numpy arrays with the same fields as run()."""
+        Returns a dict of numpy arrays with the same fields as run(); `out`
+        (such a dict from an earlier call) is written in place instead (the
+        INOUT columns x_fp..dy_fp and pathlen start from its values)."""
         n = len(events["dpp"])
         ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
               ("dpp", "x", "y", "z", "dxdz", "dydz", "fry") if events.get(k) is not None}
-        out = {k: ev[k].copy() for k in ("dpp", "y", "dxdz", "dydz")}
-        for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2"):
-            out[k] = np.zeros(n)
-        out["ok"] = np.zeros(n, np.uint8)
-        out["stop"] = np.zeros(n, np.uint8)
-        out["used"] = np.zeros(n, np.int32)
+        if out is None:
+            out = {k: ev[k].copy() for k in ("dpp", "y", "dxdz", "dydz")}
+            for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2"):
+                out[k] = np.zeros(n)
+            out["ok"] = np.zeros(n, np.uint8)
+            out["stop"] = np.zeros(n, np.uint8)
+            out["used"] = np.zeros(n, np.int32)
+        for k, v in out.items():
+            if v is not None and not (isinstance(v, np.ndarray) and v.flags.c_contiguous and len(v) == n and
+                                      v.dtype == {"ok": np.uint8, "stop": np.uint8, "used": np.int32}.get(k, np.float64)):
+                raise TypeError(f"out[{k!r}]: expected a contiguous numpy array of {n} elements")
         S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
                      FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
         hp = lambda a: None if a is None else a.ctypes.data_as(_vp)

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/simc_gpu.h"
@@ -1734,6 +1735,16 @@ struct simc_ctx {
     std::mutex host_mu;
     void* d_stage = nullptr;
     size_t stage_bytes = 0;
+    // pipeline of the host-array arm entry point: two slots, each a pinned
+    // host buffer, a device buffer, a stream, its survivor queues and an event
+    struct HostPipe {
+        hipStream_t st[2] = {};
+        hipEvent_t done[2] = {};
+        void* pinned[2] = {};
+        void* dev[2] = {};
+        size_t bytes = 0;  // per slot
+        QueuePair qp[2];
+    } hp;
     int device = 0;
     int n_cu = 256;
 };
@@ -2204,6 +2215,13 @@ void simc_ctx_destroy(simc_ctx* ctx)
         free_pair(ctx->qp[i]);
     }
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    for (int j = 0; j < 2; ++j) {
+        if (ctx->hp.pinned[j]) (void)hipHostFree(ctx->hp.pinned[j]);
+        if (ctx->hp.dev[j]) (void)hipFree(ctx->hp.dev[j]);
+        if (ctx->hp.done[j]) (void)hipEventDestroy(ctx->hp.done[j]);
+        if (ctx->hp.st[j]) (void)hipStreamDestroy(ctx->hp.st[j]);
+        free_pair(ctx->hp.qp[j]);
+    }
     for (int i = 0; i < kNumArms; ++i)
         if (ctx->d_rexp[i]) (void)hipFree(ctx->d_rexp[i]);
     if (ctx->d_sel) (void)hipFree(ctx->d_sel);
@@ -2428,71 +2446,197 @@ int simc_arm_run_host(simc_ctx* ctx, int arm, const simc_settings* s, const simc
     if (n == 0) return SIMC_OK;
     if (rng->kind == SIMC_RNG_TAPE && (!rng->tape || rng->stride <= 0))
         return fail(SIMC_E_ARG, "tape rng needs a tape and a positive stride");
+    const int ai = arm_index(arm);
+    if (ai < 0 || !ctx->d_coef[ai]) return arm_run(ctx, arm, s, in, out, n, rng, nullptr, nullptr);  // its error
     std::lock_guard<std::mutex> lock(ctx->host_mu);
-    // device images of every array the caller passed (inputs, then outputs)
+    HIPCHK(hipSetDevice(ctx->device));
+    // Columns of a chunk, in the order of the slot's layout: read only (the
+    // inputs), read and written (the fitted track, kept where the fit is not
+    // reached, and the accumulated path length), written only.  One H2D copy
+    // covers the first two groups, one D2H copy the last two.
+    struct Col { const void* src; void* dst; size_t esz; };
+    std::vector<Col> cols;
     const double* hin[7] = {in->dpp, in->x, in->y, in->z, in->dxdz, in->dydz, in->fry};
-    double* hout_d[11] = {out->dpp, out->y, out->dxdz, out->dydz, out->x_fp, out->dx_fp, out->y_fp,
-                          out->dy_fp, out->pathlen, out->resmult, out->m2};
-    const size_t nd = (size_t)n * 8;
-    const size_t tape_bytes = rng->kind == SIMC_RNG_TAPE ? (size_t)n * (size_t)rng->stride * 8 : 0;
-    auto rounded = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
-    // every carve() below, each rounded up to 256 bytes
-    const size_t need = 18 * rounded(nd) + 2 * rounded((size_t)n) + rounded(4 * (size_t)n) + rounded(tape_bytes);
-    if (ctx->stage_bytes < need) {
-        if (ctx->d_stage) {
-            HIPCHK(hipDeviceSynchronize());
-            HIPCHK(hipFree(ctx->d_stage));
-            ctx->d_stage = nullptr;
-            ctx->stage_bytes = 0;
-        }
-        HIPCHK(hipMalloc(&ctx->d_stage, need));
-        ctx->stage_bytes = need;
+    for (const double* p : hin)
+        if (p) cols.push_back({p, nullptr, 8});
+    const size_t n_in = cols.size();
+    double* hio[5] = {out->x_fp, out->dx_fp, out->y_fp, out->dy_fp, out->pathlen};
+    for (double* p : hio)
+        if (p) cols.push_back({p, p, 8});
+    const size_t n_io = cols.size() - n_in;
+    double* hw[6] = {out->dpp, out->y, out->dxdz, out->dydz, out->resmult, out->m2};
+    for (double* p : hw)
+        if (p) cols.push_back({nullptr, p, 8});
+    cols.push_back({nullptr, out->ok, 1});    // always computed; copied back when asked for
+    cols.push_back({nullptr, out->stop, 1});
+    if (out->used) cols.push_back({nullptr, out->used, 4});
+    const bool tape = rng->kind == SIMC_RNG_TAPE;
+    const size_t tape_ev = tape ? (size_t)rng->stride * 8 : 0;
+    size_t per_ev = tape_ev;
+    for (const Col& c : cols) per_ev += c.esz;
+    // chunk: 2^20 events, fewer for long tapes (a slot stays near 256 MB)
+    int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 20);
+    chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, (int64_t)((256u << 20) / per_ev)));
+    auto rounded = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t slot = rounded(tape_ev * (size_t)chunk);
+    for (const Col& c : cols) slot += rounded(c.esz * (size_t)chunk);
+    auto& hp = ctx->hp;
+    for (int j = 0; j < 2; ++j) {
+        if (!hp.st[j]) HIPCHK(hipStreamCreateWithFlags(&hp.st[j], hipStreamNonBlocking));
+        if (!hp.done[j]) HIPCHK(hipEventCreateWithFlags(&hp.done[j], hipEventDisableTiming));
     }
-    char* base = (char*)ctx->d_stage;
-    size_t at = 0;
-    auto carve = [&](size_t bytes) {
-        char* p = base + at;
-        at += rounded(bytes);
-        return p;
+    if (hp.bytes < slot) {
+        HIPCHK(hipDeviceSynchronize());
+        for (int j = 0; j < 2; ++j) {
+            if (hp.pinned[j]) HIPCHK(hipHostFree(hp.pinned[j]));
+            if (hp.dev[j]) HIPCHK(hipFree(hp.dev[j]));
+            hp.pinned[j] = hp.dev[j] = nullptr;
+        }
+        hp.bytes = 0;
+        for (int j = 0; j < 2; ++j) {
+            HIPCHK(hipHostMalloc(&hp.pinned[j], slot, hipHostMallocDefault));
+            HIPCHK(hipMalloc(&hp.dev[j], slot));
+        }
+        hp.bytes = slot;
+    }
+    Queue q1[2], q2[2];
+    for (int j = 0; j < 2; ++j) {
+        int rc = ensure_pair(hp.qp[j], (uint32_t)chunk, q1[j], q2[j]);
+        if (rc) return rc;
+    }
+    // Copies between the caller's pageable arrays and a pinned slot, split
+    // over host threads (the pageable side is the bottleneck of this entry).
+    const int nthr = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    auto par_copy = [&](std::vector<std::pair<void*, const void*>>& dst_src, std::vector<size_t>& len) {
+        size_t total = 0;
+        for (size_t b : len) total += b;
+        if (total < ((size_t)4 << 20) || nthr == 1) {
+            for (size_t k = 0; k < len.size(); ++k) std::memcpy(dst_src[k].first, dst_src[k].second, len[k]);
+            return;
+        }
+        auto work = [&](int t) {
+            for (size_t k = 0; k < len.size(); ++k) {
+                const size_t lo = len[k] * t / nthr, hi = len[k] * (t + 1) / nthr;
+                if (hi > lo) std::memcpy((char*)dst_src[k].first + lo, (const char*)dst_src[k].second + lo, hi - lo);
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nthr; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
     };
-    const double* din[7];
-    for (int k = 0; k < 7; ++k) {
-        din[k] = nullptr;
-        if (hin[k]) {
-            double* d = (double*)carve(nd);
-            HIPCHK(hipMemcpy(d, hin[k], nd, hipMemcpyHostToDevice));
-            din[k] = d;
+    struct Pending { int64_t c0 = -1, m = 0; };
+    Pending pend[2];
+    // offsets of the columns in a slot for chunk size m (tape last)
+    auto layout = [&](int64_t m, std::vector<size_t>& off, size_t& tape_off, size_t& in_end, size_t& out_beg,
+                      size_t& out_end) {
+        off.resize(cols.size());
+        size_t at = 0;
+        for (size_t k = 0; k < cols.size(); ++k) {
+            off[k] = at;
+            at += rounded(cols[k].esz * (size_t)m);
+            if (k + 1 == n_in + n_io) in_end = at;
         }
-    }
-    double* dout_d[11];
-    for (int k = 0; k < 11; ++k) {
-        dout_d[k] = nullptr;
-        if (hout_d[k]) {  // INOUT semantics: the device copy starts from the caller's values
-            dout_d[k] = (double*)carve(nd);
-            HIPCHK(hipMemcpy(dout_d[k], hout_d[k], nd, hipMemcpyHostToDevice));
+        if (n_in + n_io == 0) in_end = 0;
+        out_beg = n_in < cols.size() ? off[n_in] : at;
+        out_end = at;
+        tape_off = at;
+    };
+    auto drain = [&](int j) -> int {
+        if (pend[j].c0 < 0) return SIMC_OK;
+        HIPCHK(hipEventSynchronize(hp.done[j]));
+        std::vector<size_t> off;
+        size_t tape_off, in_end, out_beg, out_end;
+        layout(pend[j].m, off, tape_off, in_end, out_beg, out_end);
+        std::vector<std::pair<void*, const void*>> ds;
+        std::vector<size_t> len;
+        for (size_t k = n_in; k < cols.size(); ++k)
+            if (cols[k].dst) {
+                ds.push_back({(char*)cols[k].dst + cols[k].esz * (size_t)pend[j].c0, (char*)hp.pinned[j] + off[k]});
+                len.push_back(cols[k].esz * (size_t)pend[j].m);
+            }
+        par_copy(ds, len);
+        pend[j].c0 = -1;
+        return SIMC_OK;
+    };
+    int rc = SIMC_OK;
+    int j = 0;
+    for (int64_t c0 = 0; c0 < n && rc == SIMC_OK; c0 += chunk, j ^= 1) {
+        const int64_t m = std::min<int64_t>(chunk, n - c0);
+        rc = drain(j);
+        if (rc) break;
+        std::vector<size_t> off;
+        size_t tape_off, in_end, out_beg, out_end;
+        layout(m, off, tape_off, in_end, out_beg, out_end);
+        std::vector<std::pair<void*, const void*>> ds;
+        std::vector<size_t> len;
+        for (size_t k = 0; k < n_in + n_io; ++k) {
+            ds.push_back({(char*)hp.pinned[j] + off[k], (const char*)cols[k].src + cols[k].esz * (size_t)c0});
+            len.push_back(cols[k].esz * (size_t)m);
         }
+        if (tape) {
+            ds.push_back({(char*)hp.pinned[j] + tape_off, rng->tape + (size_t)c0 * (size_t)rng->stride});
+            len.push_back(tape_ev * (size_t)m);
+        }
+        par_copy(ds, len);
+        char* d = (char*)hp.dev[j];
+        hipStream_t st = hp.st[j];
+        if (in_end) HIPCHK(hipMemcpyAsync(d, hp.pinned[j], in_end, hipMemcpyHostToDevice, st));
+        if (tape)
+            HIPCHK(hipMemcpyAsync(d + tape_off, (char*)hp.pinned[j] + tape_off, tape_ev * (size_t)m,
+                                  hipMemcpyHostToDevice, st));
+        auto dcol = [&](const void* host, bool is_out) -> void* {
+            if (!host) return nullptr;
+            for (size_t k = is_out ? n_in : 0; k < cols.size(); ++k)
+                if ((is_out ? (const void*)cols[k].dst : cols[k].src) == host) return d + off[k];
+            return nullptr;
+        };
+        simc_events_in ein;
+        ein.dpp = (const double*)dcol(in->dpp, false);
+        ein.x = (const double*)dcol(in->x, false);
+        ein.y = (const double*)dcol(in->y, false);
+        ein.z = (const double*)dcol(in->z, false);
+        ein.dxdz = (const double*)dcol(in->dxdz, false);
+        ein.dydz = (const double*)dcol(in->dydz, false);
+        ein.fry = (const double*)dcol(in->fry, false);
+        simc_events_out eout;
+        eout.dpp = (double*)dcol(out->dpp, true);
+        eout.y = (double*)dcol(out->y, true);
+        eout.dxdz = (double*)dcol(out->dxdz, true);
+        eout.dydz = (double*)dcol(out->dydz, true);
+        eout.x_fp = (double*)dcol(out->x_fp, true);
+        eout.dx_fp = (double*)dcol(out->dx_fp, true);
+        eout.y_fp = (double*)dcol(out->y_fp, true);
+        eout.dy_fp = (double*)dcol(out->dy_fp, true);
+        eout.pathlen = (double*)dcol(out->pathlen, true);
+        eout.resmult = (double*)dcol(out->resmult, true);
+        eout.m2 = (double*)dcol(out->m2, true);
+        // ok and stop are staged even when the caller does not ask for them
+        const size_t k_ok = cols.size() - (out->used ? 3 : 2);
+        eout.ok = (uint8_t*)(d + off[k_ok]);
+        eout.stop = (uint8_t*)(d + off[k_ok + 1]);
+        eout.used = out->used ? (int32_t*)(d + off[k_ok + 2]) : nullptr;
+        simc_rng drng = *rng;
+        if (tape) drng.tape = (const double*)(d + tape_off);
+        else drng.event0 = rng->event0 + (uint64_t)c0;
+        Queue fq[2] = {q1[j], q2[j]};  // the slot's own queues: two chunks are in flight at once
+        rc = arm_run(ctx, arm, s, &ein, &eout, m, &drng, (void*)st, nullptr, fq);
+        if (rc) break;
+        HIPCHK(hipMemcpyAsync((char*)hp.pinned[j] + out_beg, d + out_beg, out_end - out_beg, hipMemcpyDeviceToHost,
+                              st));
+        HIPCHK(hipEventRecord(hp.done[j], st));
+        pend[j].c0 = c0;
+        pend[j].m = m;
     }
-    uint8_t* d_ok = (uint8_t*)carve(n);    // always staged; copied back when asked for
-    uint8_t* d_stop = (uint8_t*)carve(n);
-    int32_t* d_used = out->used ? (int32_t*)carve(4 * (size_t)n) : nullptr;
-    simc_rng drng = *rng;
-    if (tape_bytes) {
-        double* d = (double*)carve(tape_bytes);
-        HIPCHK(hipMemcpy(d, rng->tape, tape_bytes, hipMemcpyHostToDevice));
-        drng.tape = d;
+    for (int k = 0; k < 2; ++k) {
+        const int r2 = drain(j ^ 1);
+        j ^= 1;
+        if (!rc) rc = r2;
     }
-    simc_events_in ein = {din[0], din[1], din[2], din[3], din[4], din[5], din[6]};
-    simc_events_out eout = {dout_d[0], dout_d[1], dout_d[2], dout_d[3], dout_d[4], dout_d[5],
-                            dout_d[6], dout_d[7], dout_d[8], dout_d[9], dout_d[10], d_ok, d_stop, d_used};
-    if (at > need) return fail(SIMC_E_INTERNAL, "simc_arm_run_host: staging layout exceeds its buffer");
-    int rc = arm_run(ctx, arm, s, &ein, &eout, n, &drng, nullptr, nullptr);
-    if (rc) return rc;
-    HIPCHK(hipDeviceSynchronize());
-    for (int k = 0; k < 11; ++k)
-        if (hout_d[k]) HIPCHK(hipMemcpy(hout_d[k], dout_d[k], nd, hipMemcpyDeviceToHost));
-    if (out->ok) HIPCHK(hipMemcpy(out->ok, d_ok, n, hipMemcpyDeviceToHost));
-    if (out->stop) HIPCHK(hipMemcpy(out->stop, d_stop, n, hipMemcpyDeviceToHost));
-    if (d_used) HIPCHK(hipMemcpy(out->used, d_used, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    if (rc) {
+        (void)hipDeviceSynchronize();
+        return rc;
+    }
     return SIMC_OK;
 }
 
