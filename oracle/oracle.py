@@ -53,7 +53,8 @@ _i64p = C.POINTER(C.c_int64)
 class Settings(C.Structure):
     _fields_ = [("p_spec", C.c_double), ("m2", C.c_double), ("ctau", C.c_double),
                 ("ms_flag", C.c_int), ("wcs_flag", C.c_int), ("decay_flag", C.c_int),
-                ("using_coll", C.c_int), ("fit_f32", C.c_int), ("wire_sigma", C.c_double)]
+                ("using_coll", C.c_int), ("fit_f32", C.c_int), ("wire_sigma", C.c_double),
+                ("mom_offset", C.c_double)]
 
 
 class EventsIn(C.Structure):
@@ -203,10 +204,11 @@ def _prep_outputs(n, ev):
 
 def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, coll=False,
         fit_f32=True, wire_sigma=0.0, rng="philox", tape=None, stride=0, seed=0, event0=0,
-        nthreads=None, map_dir=None):
+        nthreads=None, map_dir=None, mom_offset=0.0):
     """Run the oracle over SoA events (dict of float64 arrays: dpp, x, y, z, dxdz,
     dydz, fry).  arm: 1 HMS, 2 SOS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'.
-    map_dir: a map directory other than the default tree."""
+    map_dir: a map directory other than the default tree.  mom_offset: the
+    adjoint form's momentum scale s_p (HMS, SHMS)."""
     L = lib()
     n = len(events["dpp"])
     ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
@@ -217,7 +219,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
                                              "y_fp", "dy_fp", "pathlen", "resmult", "m2")],
                      _p(out["ok"], _u8p), _p(out["stop"], _u8p), _p(out["used"], _i32p))
     S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll), int(fit_f32),
-                 wire_sigma)
+                 wire_sigma, mom_offset)
     kind = {"tape": 0, "philox": 1, "ranlux": 2}[rng]
     cnt = np.zeros(64, np.int64)
     tp = None

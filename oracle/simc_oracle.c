@@ -1407,6 +1407,7 @@ void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
     t->dydzs = *dydz;
     t->dpps = *dpp;
     p = p_spec * (1. + t->dpps / 100.);
+    if (S->mom_offset != 0.0) t->dpps = 100. * ((1. + t->dpps / 100.) / (1. + S->mom_offset) - 1.);
 
     zdrift = z_entr;
     project(g, t, S, &t->xs, &t->ys, zdrift, decay_flag, &dflag, m2, &p, pathlen);
@@ -2403,6 +2404,7 @@ void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, 
 
     t->dpps = *dpp;
     p = p_spec * (1. + t->dpps / 100.);
+    if (S->mom_offset != 0.0) t->dpps = 100. * ((1. + t->dpps / 100.) / (1. + S->mom_offset) - 1.);
     t->xs = *x;
     t->ys = *y;
     t->zs = *z;

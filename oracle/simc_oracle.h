@@ -165,6 +165,9 @@ typedef struct {
     int ms_flag, wcs_flag, decay_flag, using_coll;
     int fit_f32;        /* 1: REAL*4 track fit as the reference; 0: FP64 fit */
     double wire_sigma;  /* cm; <= 0 keeps the reference table */
+    double mom_offset;  /* s_p of the adjoint form's momentum scale: the HMS and SHMS maps see
+                           delta' = 100((1 + delta/100)/(1 + s_p) - 1), scattering the true
+                           momentum; 0 = off (the reference) */
 } or_settings;
 
 typedef struct {

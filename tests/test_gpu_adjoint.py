@@ -10,6 +10,13 @@
   differences (h = 1e-6) of the adjoint's own loss (same draws, fixed mask).
 * The torch.autograd binding returns the same gradients, and descending them
   lowers the loss.
+* The s_p and dtheta gradients agree with Richardson-extrapolated central
+  differences of the ORACLE's loss (its own transport, with the maps seeing
+  delta' and y' - dtheta and the scattering the true momentum), on events that
+  all the perturbed oracle runs accept; the loss itself agrees to 1e-12.
+* At the adjoint configuration's 1e7 events: the six offset gradients and the
+  largest coefficient gradients against central differences of the adjoint's
+  own loss.
 """
 import os
 import sys
@@ -181,3 +188,74 @@ def test_autograd_binding_and_descent(ctx):
                 break
             step *= 0.5
     assert L < L0, (L, L0)
+
+
+@pytest.mark.parametrize("wl", [W.HMS_C0, W.SHMS_C1], ids=["hms", "shms"])
+def test_offset_gradients_against_oracle_differences(ctx, oracle, wl):
+    """An independent tangent for s_p and dtheta: differences of the oracle's
+    loss, the transport restated on the CPU (not the generated tangent
+    polynomials of the adjoint)."""
+    n = 20000
+    evn = W.generate(wl, n, seed=18)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma)
+    sp0, dth0 = 2e-3, 1e-3
+
+    def oracle_run(sp, dth):
+        e = dict(evn)
+        e["dydz"] = evn["dydz"] - dth
+        return oracle.run(wl.arm, e, rng="philox", seed=SEED, fit_f32=False, mom_offset=sp, **kw)
+
+    def loss(r, sel):
+        return float(np.mean(sum((r[k] - evn[k])[sel] ** 2 for k in ("dxdz", "y", "dydz", "dpp"))))
+
+    h = {"sp": 2e-6, "dth": 2e-6}
+    runs = {(0, 0): oracle_run(sp0, dth0)}
+    for name in ("sp", "dth"):
+        for m in (-2, -1, 1, 2):
+            d = m * h[name] / 2
+            runs[(name, m)] = oracle_run(sp0 + (d if name == "sp" else 0.0), dth0 + (d if name == "dth" else 0.0))
+    sel = np.ones(n, bool)
+    for r in runs.values():
+        sel &= r["ok"] == 1
+    assert sel.sum() > 0.25 * n
+    ev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in evn.items()}
+    coef = torch.from_numpy(ctx.recon_coefficients(wl.arm)).cuda()
+    mask = torch.from_numpy(sel.astype(np.uint8)).cuda()
+    g, o9 = adj(ctx, wl, ev, mask, coef, [0.0, 0.0, 0.0, 0.0, sp0, dth0], dict(kw, seed=SEED))
+    assert o9[7] == 0 and o9[8] == sel.sum()
+    L0 = loss(runs[(0, 0)], sel)
+    assert abs(o9[0] - L0) <= 1e-12 * L0, (o9[0], L0)
+    for j, name in ((5, "sp"), (6, "dth")):
+        d1 = (loss(runs[(name, 2)], sel) - loss(runs[(name, -2)], sel)) / (2 * h[name])
+        d2 = (loss(runs[(name, 1)], sel) - loss(runs[(name, -1)], sel)) / h[name]
+        rich = (4 * d2 - d1) / 3
+        print(wl.name, name, "adjoint", o9[j], "oracle differences", rich, flush=True)
+        assert abs(rich - o9[j]) <= 2e-6 * abs(o9[j]), (name, rich, o9[j])
+
+
+def test_gradients_at_1e7_events(ctx):
+    """The adjoint configuration's size: HMS, 1e7 events."""
+    wl = W.HMS_C0
+    n = 10_000_000
+    print("adjoint 1e7", flush=True)
+    ev, out, coef, kw = setup(ctx, wl, n)
+    mask = out["ok"]
+    base = [0.01, -0.02, 2e-4, -1e-4, 1e-3, 5e-4]
+    g, o9 = adj(ctx, wl, ev, mask, coef, base, kw)
+    assert o9[8] > 0.3 * n and o9[7] < 1e-4 * n
+    hstep = 1e-6
+    for j in range(6):
+        lo, hi = list(base), list(base)
+        lo[j] -= hstep
+        hi[j] += hstep
+        fd = (adj(ctx, wl, ev, mask, coef, hi, kw)[1][0] - adj(ctx, wl, ev, mask, coef, lo, kw)[1][0]) / (2 * hstep)
+        assert abs(fd - o9[1 + j]) <= 1e-5 * abs(o9[1 + j]) + 1e-8 * o9[0], (j, fd, o9[1 + j])
+    flat = np.abs(g).ravel()
+    for idx in np.argsort(flat)[-4:]:
+        t_, k = divmod(int(idx), 4)
+        hc = 1e-6 * max(1.0, abs(float(coef[t_, k])))
+        cp, cm = coef.clone(), coef.clone()
+        cp[t_, k] += hc
+        cm[t_, k] -= hc
+        fd = (adj(ctx, wl, ev, mask, cp, base, kw)[1][0] - adj(ctx, wl, ev, mask, cm, base, kw)[1][0]) / (2 * hc)
+        assert abs(fd - g[t_, k]) <= 1e-5 * abs(g[t_, k]) + 1e-9 * o9[0] / hc, (t_, k, fd, g[t_, k])
