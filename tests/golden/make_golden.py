@@ -19,6 +19,9 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         radc_init_ev) + complete_main on per-event tapes, one process per case
     python tests/golden/make_golden.py lfit     # track fit (lfit/ref_lfit.npz): the
         reference's LFIT (cern/lfit.f) over seeded REAL*4 point sets
+    python tests/golden/make_golden.py ntuple   # .bin ntuple (ntuple/*.npz): the file the
+        reference's NtupleInit / NtupleClose write for the selected rows of
+        seeded tables (elastic; pion with the target-field columns)
     python tests/golden/make_golden.py gen      # generator (gen/*.npz):
         the reference's physics_angles, spectrometer_angles (event.f), sigep and
         fofa_best_fit (physics_proton.f) over sampled arguments, and its
@@ -330,8 +333,35 @@ def main_lfit():
     print("lfit", LFIT_N, "sets; A = B = 0 for", int(((A == 0) & (B == 0)).sum()))
 
 
+# ntuple: the reference's NtupleInit / record writes / NtupleClose over the
+# selected rows of seeded tables, for the GPU writer to reproduce byte for byte.
+NTUPLE_CASES = [("elastic", False, 500, 61), ("pion", True, 400, 62)]
+
+
+def main_ntuple():
+    import tempfile
+    from simc_gfortran_amd import api
+    os.makedirs(os.path.join(HERE, "ntuple"), exist_ok=True)
+    for mode, tgt, n, seed in NTUPLE_CASES:
+        ncols = len(api.ntuple_tags(mode, tgt_field=tgt))
+        rng = np.random.default_rng(seed)
+        rows = rng.normal(size=(n, ncols)) * 10.0 ** rng.integers(-6, 6, size=(n, ncols))
+        rows[0, :4] = [0.0, -0.0, 1e-50, -3.5e38]
+        mask = (rng.random(n) < 0.6).astype(np.uint8)
+        with tempfile.TemporaryDirectory(prefix="nt", dir="/tmp") as d:
+            path = os.path.join(d, "r.bin")
+            O.ref_ntuple(path, mode, tgt, rows[mask == 1])
+            data = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+        name = f"{mode}{'_tgtfield' if tgt else ''}"
+        np.savez_compressed(os.path.join(HERE, "ntuple", name + ".npz"), mode=mode, tgt_field=tgt, rows=rows,
+                            mask=mask, ref_bin=data)
+        print("ntuple", name, n, "rows,", int(mask.sum()), "written,", len(data), "bytes")
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["gen"]:
+    if sys.argv[1:] == ["ntuple"]:
+        main_ntuple()
+    elif sys.argv[1:] == ["gen"]:
         main_gen()
     elif sys.argv[1:] == ["target"]:
         main_target()

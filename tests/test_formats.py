@@ -49,3 +49,15 @@ def test_ntuple_rejects_bad_shapes(short_dir):
     with api.Ntuple(os.path.join(short_dir, "y.bin"), ["a", "b"]) as nt:
         with pytest.raises(ValueError):
             nt.write(np.zeros((3, 3)))
+
+
+@pytest.mark.parametrize("case", ["elastic", "pion_tgtfield"])
+def test_ntuple_golden_files(short_dir, case):
+    """The committed reference-written files (tests/golden/ntuple/) are what
+    the host writer produces for the same selected rows."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ntuple", case + ".npz"))
+    tags = api.ntuple_tags(str(g["mode"]), tgt_field=bool(g["tgt_field"]))
+    ours = os.path.join(short_dir, "o.bin")
+    with api.Ntuple(ours, tags) as nt:
+        nt.write(g["rows"][g["mask"] == 1])
+    assert open(ours, "rb").read() == g["ref_bin"].tobytes()

@@ -88,3 +88,20 @@ def test_device_ntuple_empty_and_no_selection(ctx, tmp_path):
         assert nt.write_device([c, c], none) == 0
     t, v = api.read_ntuple(str(tmp_path / "e.bin"))
     assert t == ["a", "b"] and v.shape == (0, 2)
+
+
+@pytest.mark.parametrize("case", ["elastic", "pion_tgtfield"])
+def test_device_ntuple_equals_reference_file(ctx, tmp_path, case):
+    """The GPU writer against the file the reference's own NtupleInit /
+    NtupleClose wrote for the selected rows (tests/golden/ntuple/, made by
+    tests/golden/make_golden.py ntuple through oracle/_ref): byte for byte."""
+    from simc_gfortran_amd import api
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ntuple", case + ".npz"))
+    tags = api.ntuple_tags(str(g["mode"]), tgt_field=bool(g["tgt_field"]))
+    rows = g["rows"]
+    cols = [torch.from_numpy(np.ascontiguousarray(rows[:, k])).cuda() for k in range(rows.shape[1])]
+    mask = torch.from_numpy(g["mask"]).cuda()
+    path = tmp_path / "d.bin"
+    with api.Ntuple(str(path), tags) as nt:
+        assert nt.write_device(cols, mask) == int(g["mask"].sum())
+    assert open(path, "rb").read() == g["ref_bin"].tobytes()
