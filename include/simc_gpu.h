@@ -75,7 +75,12 @@ typedef struct {
     int32_t decay_flag;   /* in-flight decay                                  */
     int32_t using_coll;   /* step pions/muons through the collimator          */
     int32_t fit_mode;     /* SIMC_FIT_REF_F32 or SIMC_FIT_FP64                */
-    int32_t reserved;
+    int32_t sieve;        /* mock sieve slit, compile-time off in the reference
+                             (hms/mc_hms.f:155-168, shms/mc_shms.f:276-310,
+                             sos/mc_sos.f:122-137, hrs?/mc_hrs?.f:122-132):
+                             a bit mask: 0 off, 1 the arm's use_sieve,
+                             2 the SHMS's use_2sieve (in front of the
+                             bender), 3 both (SHMS)                       */
     double wire_sigma;    /* cm; <= 0 keeps the spectrometer's own table      */
 } simc_settings;
 

@@ -54,7 +54,7 @@ class Settings(C.Structure):
     _fields_ = [("p_spec", C.c_double), ("m2", C.c_double), ("ctau", C.c_double),
                 ("ms_flag", C.c_int), ("wcs_flag", C.c_int), ("decay_flag", C.c_int),
                 ("using_coll", C.c_int), ("fit_f32", C.c_int), ("wire_sigma", C.c_double),
-                ("mom_offset", C.c_double)]
+                ("mom_offset", C.c_double), ("sieve", C.c_int)]
 
 
 class EventsIn(C.Structure):
@@ -170,8 +170,26 @@ def tune_map_dir(root, tune):
     return d
 
 
-def ref_lib():
+_ref_variants = {}
+
+
+def ref_lib(variant=""):
+    """The reference build (oracle/_ref); variant "sieve1" / "sieve2" /
+    "sieve3": the builds with the mock sieve slit on (make -C oracle sieve)."""
     global _ref
+    if variant:
+        if variant not in _ref_variants:
+            path = os.path.join(HERE, "_ref", f"libsimcref_{variant}.so")
+            if not os.path.exists(path):
+                return None
+            L = C.CDLL(path)
+            L.ref_run.restype = C.c_int
+            L.ref_run.argtypes = ([C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_int, C.c_int64] + [_dp] * 7 + [_dp] * 11 +
+                                  [_u8p, _i32p, _dp, C.c_int64, C.c_char_p, _i64p,
+                                   C.POINTER(C.c_uint64)])
+            _ref_variants[variant] = L
+        return _ref_variants[variant]
     if _ref is None:
         if not os.path.exists(REF_LIB):
             return None
@@ -204,11 +222,12 @@ def _prep_outputs(n, ev):
 
 def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, coll=False,
         fit_f32=True, wire_sigma=0.0, rng="philox", tape=None, stride=0, seed=0, event0=0,
-        nthreads=None, map_dir=None, mom_offset=0.0):
+        nthreads=None, map_dir=None, mom_offset=0.0, sieve=0):
     """Run the oracle over SoA events (dict of float64 arrays: dpp, x, y, z, dxdz,
     dydz, fry).  arm: 1 HMS, 2 SOS, 5 SHMS.  rng: 'tape' | 'philox' | 'ranlux'.
     map_dir: a map directory other than the default tree.  mom_offset: the
-    adjoint form's momentum scale s_p (HMS, SHMS)."""
+    adjoint form's momentum scale s_p (HMS, SHMS).  sieve: the mock sieve slit,
+    a bit mask (1 the arm's, 2 the SHMS's in front of the bender, 3 both)."""
     L = lib()
     n = len(events["dpp"])
     ev = {k: np.ascontiguousarray(events[k], dtype=np.float64) for k in
@@ -219,7 +238,7 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
                                              "y_fp", "dy_fp", "pathlen", "resmult", "m2")],
                      _p(out["ok"], _u8p), _p(out["stop"], _u8p), _p(out["used"], _i32p))
     S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll), int(fit_f32),
-                 wire_sigma, mom_offset)
+                 wire_sigma, mom_offset, int(sieve))
     kind = {"tape": 0, "philox": 1, "ranlux": 2}[rng]
     cnt = np.zeros(64, np.int64)
     tp = None
@@ -236,9 +255,10 @@ def run(arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0, co
 
 
 def ref_run(arm, events, *, p_spec, m2, tape, stride, ms=True, wcs=True, decay=False,
-            ctau=0.0, coll=False):
-    """Run the reference Fortran (flang build) over the same events on a tape."""
-    L = ref_lib()
+            ctau=0.0, coll=False, sieve=0):
+    """Run the reference Fortran (flang build) over the same events on a tape;
+    sieve 1 / 2 / 3: its build with use_sieve / use_2sieve / both on."""
+    L = ref_lib({0: "", 1: "sieve1", 2: "sieve2", 3: "sieve3"}[sieve])
     if L is None:
         raise FileNotFoundError(REF_LIB)
     n = len(events["dpp"])

@@ -1376,6 +1376,36 @@ static int hms_hut(or_rng *g, or_rng *hg, or_track *t, const or_settings *S, int
     return 1;
 }
 
+/* The mock sieve slit (hms/mc_hms.f:155-168, shms/mc_shms.f:276-310,
+ * sos/mc_sos.f:122-137, hrsl/mc_hrsl.f:122-132): the track from the target
+ * point is forced through the nearest hole of a sieve at z_slit, at an
+ * offset drawn inside the hole (two uniforms), by changing the INOUT angles.
+ * kind 0: holes on a (px, py) grid; 1: the SOS's rows (middle three at
+ * 1.016 cm, outer ones on a 2.54 cm pitch from 1.016); 2: the HRS's (the
+ * offset is drawn but not applied).  NINT rounds half away from zero. */
+static void sieve_slit(or_rng *g, int kind, double z_slit, double px, double py, double r, double x, double y,
+                       double *dxdz, double *dydz)
+{
+    const double pi = 3.141592653589793; /* constants.inc:42 */
+    double xt = x + z_slit * *dxdz;
+    double yt = y + z_slit * *dydz;
+    xt = px * (double)lround(xt / px);
+    if (kind == 1) {
+        if (fabs(yt) < 1.524) yt = 1.016 * (double)lround(yt / 1.016);
+        else yt = yt / fabs(yt) * (2.54 * (double)lround((fabs(yt) - 1.016) / 2.54) + 1.016);
+    } else {
+        yt = py * (double)lround(yt / py);
+    }
+    double rt = r * sqrt(or_grnd(g));
+    double tht = 2 * pi * or_grnd(g);
+    if (kind != 2) {
+        xt = xt + rt * cos(tht);
+        yt = yt + rt * sin(tht);
+    }
+    *dxdz = (xt - x) / z_slit;
+    *dydz = (yt - y) / z_slit;
+}
+
 /* mc_hms, hms/mc_hms.f:1-441 */
 void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, int64_t *cnt,
                double p_spec, double *dpp, double *x, double *y, double *z, double *dxdz,
@@ -1399,6 +1429,7 @@ void or_mc_hms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
     *stop_code = STOP_OK;
     *fit_done = 0;
     cnt[H_TRIALS]++;
+    if (S->sieve == 1) sieve_slit(g, 0, z_entr, 2.54, 1.524, 0.254, *x, *y, dxdz, dydz); /* :155-168 */
 
     t->xs = *x;
     t->ys = *y;
@@ -1734,6 +1765,7 @@ void or_mc_sos(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, i
     *stop_code = STOP_OK;
     *fit_done = 0;
     cnt[SO_TRIALS]++;
+    if (S->sieve == 1) sieve_slit(g, 1, z_entr, 1.524, 0.0, 0.254, *x, *y, dxdz, dydz); /* sos/mc_sos.f:122-137 */
 
     t->xs = *x;
     t->ys = *y;
@@ -1999,6 +2031,7 @@ void or_mc_hrs(const or_maps *M, int arm, const or_settings *S, or_rng *g, or_tr
     *stop_code = STOP_OK;
     *fit_done = 0;
     cnt[R_TRIALS]++;
+    if (S->sieve == 1) sieve_slit(g, 2, z_exit, 2.50, 1.25, 0.1, *x, *y, dxdz, dydz); /* hrsl/mc_hrsl.f:122-132 */
 
     t->xs = *x;
     t->ys = *y;
@@ -2401,6 +2434,10 @@ void or_mc_shms(const or_maps *M, const or_settings *S, or_rng *g, or_track *t, 
     *stop_code = STOP_OK;
     *fit_done = 0;
     cnt[S_TRIALS]++;
+    if (S->sieve & 2) /* shms/mc_shms.f:276-290: in front of the bender */
+        sieve_slit(g, 0, zd_hbin, 2.5, 1.64, 0.3, *x, *y, dxdz, dydz);
+    if (S->sieve & 1) /* :293-310: at the collimator */
+        sieve_slit(g, 0, zd_hbin + zd_hbmen + zd_hbmex + zd_hbout + z_entr, 2.5, 1.64, 0.3, *x, *y, dxdz, dydz);
 
     t->dpps = *dpp;
     p = p_spec * (1. + t->dpps / 100.);

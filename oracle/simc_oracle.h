@@ -168,6 +168,8 @@ typedef struct {
     double mom_offset;  /* s_p of the adjoint form's momentum scale: the HMS and SHMS maps see
                            delta' = 100((1 + delta/100)/(1 + s_p) - 1), scattering the true
                            momentum; 0 = off (the reference) */
+    int sieve;          /* mock sieve slit, compile-time off in the reference: 0 off, 1 the arm's
+                           use_sieve, 2 the SHMS's use_2sieve (in front of the bender) */
 } or_settings;
 
 typedef struct {

@@ -81,7 +81,7 @@ _vp = C.c_void_p
 class Settings(C.Structure):
     _fields_ = [("p_spec", C.c_double), ("m2", C.c_double), ("ctau", C.c_double),
                 ("ms_flag", C.c_int32), ("wcs_flag", C.c_int32), ("decay_flag", C.c_int32),
-                ("using_coll", C.c_int32), ("fit_mode", C.c_int32), ("reserved", C.c_int32),
+                ("using_coll", C.c_int32), ("fit_mode", C.c_int32), ("sieve", C.c_int32),
                 ("wire_sigma", C.c_double)]
 
 
@@ -378,7 +378,7 @@ class Spectrometers:
     # -- transport + reconstruction -------------------------------------------------
     def run(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
             coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
-            tape=None, stride=0, out=None, stream=None, stage_events=None):
+            tape=None, stride=0, out=None, stream=None, stage_events=None, sieve=0):
         """Transport + reconstruct events through one arm.
 
         events: dict of float64 device tensors dpp, x, y, dxdz, dydz (+ z, fry).
@@ -386,7 +386,11 @@ class Spectrometers:
         dpp, y, dxdz, dydz (recon or input), x_fp, dx_fp, y_fp, dy_fp, pathlen,
         resmult, m2, ok (uint8), stop (uint8), used (int32).
         stage_events: optional four recorded torch.cuda.Event(enable_timing=True)
-        marking the boundaries of the three kernel stages (first launch chunk)."""
+        marking the boundaries of the three kernel stages (first launch chunk).
+        sieve: the mock sieve slit (the reference's compile-time use_sieve in
+        each mc_*.f), a bit mask: 0 off, 1 the arm's sieve (use_sieve), 2 the
+        SHMS's second sieve in front of the horizontal bender (mc_shms.f
+        use_2sieve), 3 both (SHMS)."""
         n = events["dpp"].numel()
         ev = {k: _dev_f64(events[k], n, k) for k in ("dpp", "x", "y", "dxdz", "dydz")}
         for k in ("z", "fry"):
@@ -402,7 +406,7 @@ class Spectrometers:
         else:
             _check_outputs(out, n, RUN_REQUIRED)
         S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
-                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
+                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, int(sieve), wire_sigma)
         ein = EventsIn(*[_ptr(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
         eout = EventsOut(*[_ptr(out.get(k)) for k in OUT_FIELDS])
         if rng == "tape":
@@ -426,7 +430,7 @@ class Spectrometers:
 
     def run_host(self, arm, events, *, p_spec, m2, ms=True, wcs=True, decay=False, ctau=0.0,
                  coll=False, fit="ref_f32", wire_sigma=0.0, rng="philox", seed=0, event0=0,
-                 tape=None, stride=0, out=None):
+                 tape=None, stride=0, out=None, sieve=0):
         """run() on numpy arrays through the synchronous host-pointer entry point.
         Returns a dict of numpy arrays with the same fields as run(); `out`
         (such a dict from an earlier call) is written in place instead (the
@@ -446,7 +450,7 @@ class Spectrometers:
                                       v.dtype == {"ok": np.uint8, "stop": np.uint8, "used": np.int32}.get(k, np.float64)):
                 raise TypeError(f"out[{k!r}]: expected a contiguous numpy array of {n} elements")
         S = Settings(p_spec, m2, ctau, int(ms), int(wcs), int(decay), int(coll),
-                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, 0, wire_sigma)
+                     FIT_REF_F32 if fit == "ref_f32" else FIT_FP64, int(sieve), wire_sigma)
         hp = lambda a: None if a is None else a.ctypes.data_as(_vp)
         ein = EventsIn(*[hp(ev.get(k)) for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")])
         eout = EventsOut(*[hp(out.get(k)) for k in OUT_FIELDS])
@@ -915,8 +919,8 @@ class Batch:
     """Many kinematic settings transported in one run (simc_batch_*).
 
     settings: list of dicts with keys arm, p_spec, m2, offset, count and
-    optionally ms, wcs, decay, ctau, coll, fit, wire_sigma, seed, event0 (the
-    run() keywords).  Event j of a setting is element offset + j of the event
+    optionally ms, wcs, decay, ctau, coll, fit, wire_sigma, sieve, seed, event0
+    (the run() keywords).  Event j of a setting is element offset + j of the event
     arrays and draws Philox stream (seed, event0 + j)."""
 
     def __init__(self, ctx: Spectrometers, settings):
@@ -924,7 +928,7 @@ class Batch:
         for i, d in enumerate(settings):
             S = Settings(d["p_spec"], d["m2"], d.get("ctau", 0.0), int(d.get("ms", True)),
                          int(d.get("wcs", True)), int(d.get("decay", False)), int(d.get("coll", False)),
-                         FIT_REF_F32 if d.get("fit", "ref_f32") == "ref_f32" else FIT_FP64, 0,
+                         FIT_REF_F32 if d.get("fit", "ref_f32") == "ref_f32" else FIT_FP64, int(d.get("sieve", 0)),
                          d.get("wire_sigma", 0.0))
             arr[i] = Setting(d["arm"], 0, S, int(d.get("seed", 0)), int(d.get("event0", 0)),
                              int(d["offset"]), int(d["count"]))

@@ -361,6 +361,7 @@ struct Flags {
     double p_spec;   // central momentum (MeV/c)
     double sigma;    // wire-chamber resolution (cm)
     int ms, wcs, decay, coll, fit_f32;
+    int sieve;       // the run's mock sieve options: bit 0 use_sieve, bit 1 the SHMS's use_2sieve
 };
 
 // Rotation constants tan/sin/cos of rotang * 0.017453292 (the truncated
@@ -426,6 +427,36 @@ struct ThetaCache {
         return a * sqrt_rl * (1.0 + epsilon * (log10_rl - lb));
     }
 };
+
+// The mock sieve slit (hms/mc_hms.f:155-168, shms/mc_shms.f:276-310,
+// sos/mc_sos.f:122-137, hrsl/mc_hrsl.f:122-132): the track from the target
+// point is forced through the nearest hole of a sieve at z, at an offset drawn
+// inside the hole (two uniforms), by changing its angles.  kind 0: holes on a
+// (px, py) grid; 1: the SOS's rows (middle three at 1.016 cm, outer ones on a
+// 2.54 cm pitch from 1.016); 2: the HRS's (the offset is drawn, not applied).
+// NINT is round half away from zero.
+template <class R>
+__device__ __forceinline__ void sieve_slit(R& g, Track& t, int kind, double z, double px, double py, double r)
+{
+    const double x = t.xs, y = t.ys;
+    double xt = x + z * t.dxdzs;
+    double yt = y + z * t.dydzs;
+    xt = px * round(xt / px);
+    if (kind == 1) {
+        if (fabs(yt) < 1.524) yt = 1.016 * round(yt / 1.016);
+        else yt = yt / fabs(yt) * (2.54 * round((fabs(yt) - 1.016) / 2.54) + 1.016);
+    } else {
+        yt = py * round(yt / py);
+    }
+    const double rt = r * sqrt(g.next());
+    const double tht = 2.0 * kPi * g.next();
+    if (kind != 2) {
+        xt = xt + rt * cos(tht);
+        yt = yt + rt * sin(tht);
+    }
+    t.dxdzs = (xt - x) / z;
+    t.dydzs = (yt - y) / z;
+}
 
 // musc(m2, p, rad_len, dth, dph), shared/musc.f:43-57
 template <class R>

@@ -29,6 +29,10 @@
 // (seed, call number); seed from SIMC_SEED (default 0) or simc_dropin_seed.
 // Maps are read from SIMC_MAP_DIR, default the current directory, where the
 // reference opens hms/forward_cosy.dat etc. (shared/transp.f:300-309).
+// The mock sieve slit, a compile-time DATA value in the reference
+// (hms/mc_hms.f use_sieve, shms/mc_shms.f use_sieve / use_2sieve, ...), is
+// chosen once per process by SIMC_SIEVE: unset or 0 off, 1 use_sieve,
+// 2 the SHMS's use_2sieve, 3 both (bit 1 applies to the SHMS only).
 // Errors are printed to stderr and leave ok_spec = .false.; they never end
 // the process.
 #include <cstdint>
@@ -56,6 +60,7 @@ simc_ctx* g_ctx = nullptr;
 bool g_failed = false;
 uint64_t g_seed = 0;
 uint64_t g_event = 0;
+int32_t g_sieve = 0;
 bool g_seed_set = false;
 simc_field* g_field = nullptr;
 
@@ -67,6 +72,8 @@ simc_ctx* context()
         const char* sd = std::getenv("SIMC_SEED");
         g_seed = sd ? std::strtoull(sd, nullptr, 10) : 0;
     }
+    const char* sv = std::getenv("SIMC_SIEVE");
+    g_sieve = sv ? (int32_t)std::strtol(sv, nullptr, 10) : 0;
     if (simc_ctx_create(dir ? dir : ".", &g_ctx) != SIMC_OK) {
         std::fprintf(stderr, "simc_dropin: %s\n", simc_last_error());
         g_failed = true;
@@ -116,6 +123,7 @@ void arm_event(int arm, double* p_spec, double* dpp, double* x, double* y, doubl
     s.decay_flag = *decay_flag != 0;
     s.using_coll = *using_coll != 0;
     s.fit_mode = SIMC_FIT_REF_F32;
+    s.sieve = arm == SIMC_ARM_SHMS ? g_sieve : (g_sieve & 1);
     s.wire_sigma = 0.0;
     simc_events_in in = {dpp, x, y, z, dxdz, dydz, fry};
     uint8_t ok = 0, stop = 0;

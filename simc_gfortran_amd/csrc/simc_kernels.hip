@@ -91,7 +91,7 @@ struct KArgs {
     uint32_t stride_bytes;
     unsigned long long* hist;   // per-setting histograms of kHistSlots bins
 };
-enum : int32_t { KF_MS = 1, KF_WCS = 2, KF_DECAY = 4, KF_COLL = 8, KF_FIT_F32 = 16 };
+enum : int32_t { KF_MS = 1, KF_WCS = 2, KF_DECAY = 4, KF_COLL = 8, KF_FIT_F32 = 16, KF_SIEVE = 32, KF_SIEVE2 = 64 };
 
 typedef const double __attribute__((address_space(4)))* cdptr;
 typedef const Op __attribute__((address_space(4)))* coptr;
@@ -355,8 +355,8 @@ __device__ __forceinline__ void fit_track(const Flags& F, const double* lds, con
 // polynomials' code is not shaped by the other path).
 // ---------------------------------------------------------------------------
 template <int ARM, class R, bool RARE, bool GEN>
-__device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, unsigned* coll_cnt,
-                                           int pc_begin, int pc_end)
+__device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const Flags& FA, uint32_t off,
+                                           unsigned* coll_cnt, int pc_begin, int pc_end)
 {
     // The in-flight decay and the collimator stepping of pions/muons are
     // compiled only into the RARE variant, so the common variant does not
@@ -462,6 +462,14 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
             break;
         case OP_CHECK_T_SLAB:
             if ((fabs(xt - p0) > p1) || ((fabs(yt) + ops[pc].p[2] * xt) > ops[pc].p[3])) st = ops[pc].stop;
+            break;
+        case OP_SIEVE:
+            // the forced angles are also the INOUT dxdz, dydz of the event
+            if (F.sieve & ops[pc].arg) {
+                sieve_slit(g, t, (int)ops[pc].p[4], p0, p1, ops[pc].p[2], ops[pc].p[3]);
+                st64(a.o_dxdz, a.n_arr, off, t.dxdzs);
+                st64(a.o_dydz, a.n_arr, off, t.dydzs);
+            }
             break;
         default:
             st = ST_BADOP;  // not a magnet step (excluded by the host's check)
@@ -663,6 +671,7 @@ __device__ __forceinline__ Flags flags_of(const KArgs& a, const SetDev& S)
     F.decay = (a.flags & KF_DECAY) != 0;
     F.coll = (a.flags & KF_COLL) != 0;
     F.fit_f32 = (a.flags & KF_FIT_F32) != 0;
+    F.sieve = ((a.flags & KF_SIEVE) ? 1 : 0) | ((a.flags & KF_SIEVE2) ? 2 : 0);
     return F;
 }
 
@@ -713,11 +722,14 @@ __device__ __forceinline__ void finish_event(const KArgs& a, uint32_t off, int s
     if (st == ST_OK) {
         if (!RECON_LATER) reconstruct_event<ARM>(a, off);
     } else {
-        // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440)
+        // failure leaves the INOUT target quantities at their inputs (hms/mc_hms.f:438-440);
+        // the mock sieve has already written its forced angles
         st64(a.o_dpp, n, off, ld64(a.dpp, n, off));
         st64(a.o_y, n, off, ld64(a.y, n, off));
-        st64(a.o_dxdz, n, off, ld64(a.dxdz, n, off));
-        st64(a.o_dydz, n, off, ld64(a.dydz, n, off));
+        if (!(a.flags & (KF_SIEVE | KF_SIEVE2))) {
+            st64(a.o_dxdz, n, off, ld64(a.dxdz, n, off));
+            st64(a.o_dydz, n, off, ld64(a.dydz, n, off));
+        }
     }
     st64(a.pathlen, n, off, t.pathlen);
     if (a.resmult) st64(a.resmult, n, off, resmult);
@@ -817,8 +829,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
         int st;
         uint32_t hut_drawn = 0;
         if constexpr (STAGE == STAGE_HUT) st = run_hut<ARM, R, RARE>(g, t, a, F, e.off, lds_hits, resmult, hut_drawn);
-        else if (a.gen) st = run_program<ARM, R, RARE, true>(g, t, a, F, coll_cnt, pc_begin, pc_end);
-        else st = run_program<ARM, R, RARE, false>(g, t, a, F, coll_cnt, pc_begin, pc_end);
+        else if (a.gen) st = run_program<ARM, R, RARE, true>(g, t, a, F, e.off, coll_cnt, pc_begin, pc_end);
+        else st = run_program<ARM, R, RARE, false>(g, t, a, F, e.off, coll_cnt, pc_begin, pc_end);
         const uint32_t used = (uint32_t)g.used() + hut_drawn;
         if (g.overflow) st = ST_TAPE;
         if (STAGE == STAGE_HUT) {
@@ -1111,6 +1123,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(AdjArgs a)
     F.decay = 0;
     F.coll = 0;
     F.fit_f32 = 0;
+    F.sieve = 0;
     const double sp = a.off[4], dth = a.off[5];
     const uint32_t ev = a.sel[i];
     const uint32_t n = a.n, off = ev * 8u;
@@ -2240,6 +2253,8 @@ int check_settings(int arm, const simc_settings* s)
         return fail(SIMC_E_ARG, "arm must be SIMC_ARM_HMS (1), _SOS (2), _HRSR (3), _HRSL (4) or _SHMS (5)");
     if (!(s->p_spec > 0.0) || !(s->m2 >= 0.0)) return fail(SIMC_E_ARG, "p_spec must be > 0 and m2 >= 0");
     if (s->decay_flag && !(s->ctau > 0.0)) return fail(SIMC_E_ARG, "decay_flag needs ctau > 0");
+    if (s->sieve < 0 || s->sieve > 3 || ((s->sieve & 2) && arm != SIMC_ARM_SHMS))
+        return fail(SIMC_E_ARG, "sieve must be 0 or 1, or for the SHMS 2 (in front of the bender) or 3 (both)");
     return SIMC_OK;
 }
 
@@ -2268,7 +2283,8 @@ SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t even
     d.sigma = s->wire_sigma > 0.0 ? s->wire_sigma
             : arm == SIMC_ARM_SHMS ? 0.020 : is_hrs(arm) ? 0.0225 : 0.030;
     d.flags = (s->ms_flag ? KF_MS : 0) | (s->wcs_flag ? KF_WCS : 0) | (s->decay_flag ? KF_DECAY : 0) |
-              (coll_stepping(arm, s) ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0);
+              (coll_stepping(arm, s) ? KF_COLL : 0) | (s->fit_mode == SIMC_FIT_REF_F32 ? KF_FIT_F32 : 0) |
+              ((s->sieve & 1) ? KF_SIEVE : 0) | ((s->sieve & 2) ? KF_SIEVE2 : 0);
     d.start = start;
     d.seed = seed;
     d.event0 = event0;
@@ -3656,6 +3672,7 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     if (arm != SIMC_ARM_HMS && arm != SIMC_ARM_SHMS)
         return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
     if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
+    if (s->sieve) return fail(SIMC_E_ARG, "the adjoint form has no mock sieve slit");
     if (ctx->d_coef[ai] && ctx->d_gen[ai])
         return fail(SIMC_E_MAPSTRUCT, "the adjoint form needs maps of the compiled structure");
     if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");

@@ -50,6 +50,9 @@ enum OpCode : int32_t {
     // hut steps of the HRS (hrsl/mc_hrsl_hut.f:289-304)
     OP_BOX_ROT_H,     // (xt,yt) = rotate_haxis(xs,ys; tan p0, sin p1, cos p2);
                       // stop if xt > p3 or xt < p4 or yt > p5 or yt < p6
+    // the mock sieve slit (the program's first steps)
+    OP_SIEVE,         // when the run's sieve option is arg: force the track through the nearest
+                      // hole of a sieve at z = p0 (pitch p1, p2; hole radius p3; layout p4)
     OP_NCODES
 };
 
@@ -61,7 +64,8 @@ enum : int { STAGE_MAGNETS = 1, STAGE_HUT = 2 };
 constexpr int op_stages(int code)
 {
     return code == OP_END || code == OP_PROJECT                                  ? (STAGE_MAGNETS | STAGE_HUT)
-         : code <= OP_COLLIMATOR || (code >= OP_CHECK_T_BOX && code <= OP_CHECK_T_SLAB) ? STAGE_MAGNETS
+         : code <= OP_COLLIMATOR || (code >= OP_CHECK_T_BOX && code <= OP_CHECK_T_SLAB) ||
+                   code == OP_SIEVE                                              ? STAGE_MAGNETS
                                                                                  : STAGE_HUT;
 }
 

@@ -83,6 +83,7 @@ inline void build_hms(const simc::ForwardMap& f, Program& P, double zdc[12])
     const double z_dip1 = 64.77e0, z_dip2 = z_dip1 + 297.18e0, z_dip3 = z_dip2 + 115.57e0;  // :92-94
     const double rq1 = 20.50 * 20.50, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;  // apertures_hms.inc:5-7
     const double* dd = f.driftdist.data();  // driftdist(spectr, k) = dd[k-1]
+    P.add(OP_SIEVE, {z_entr, 2.54, 1.524, 0.254, 0.0}, 0, 1);                          // :155-168
     P.add(OP_PROJECT, {z_entr});                                                        // :201-202
     P.add(OP_COLLIMATOR, {4.575, 11.646, 4.759, 12.114, 0.000, +0.028, z_exit - z_entr});  // :206-254
     P.add(OP_PROJECT, {dd[0] - z_exit});                                                // :259-260
@@ -211,6 +212,7 @@ inline void build_sos(const simc::ForwardMap& f, Program& P, double zdc[12])
     const double z_entr = 126.3e0, z_exit = z_entr + 6.3e0;  // :69-70
     const double r2_quad = 163.84, w_bm01 = 8.0, w_bm02 = 8.0;
     const double* dd = f.driftdist.data();
+    P.add(OP_SIEVE, {z_entr, 1.524, 0.0, 0.254, 1.0}, 0, 1);                              // :122-137
     P.add(OP_PROJECT, {z_entr});                                                          // :171-172
     P.add(OP_COLLIMATOR, {7.201, 4.696, 7.567, 4.935, 0.0, 0.0, z_exit - z_entr});       // :173-204
     P.add(OP_PROJECT, {dd[0] - z_exit});  P.add(OP_CHECK_R2, {r2_quad}, ST_SO_QUAD_IN);  // :209-215
@@ -336,6 +338,7 @@ inline void build_hrs(int arm, const simc::ForwardMap& f, Program& P, double zdc
     const double rq1 = 15.0 * 15.0, rq2 = 30.22 * 30.22, rq3 = 30.22 * 30.22;
     const double* dd = f.driftdist.data();
     double ztmp, zdrift;
+    P.add(OP_SIEVE, {z_exit, 2.50, 1.25, 0.1, 2.0}, 0, 1);  // hrsl/mc_hrsl.f:122-132, hrsr/mc_hrsr.f:122-132
     auto slit = [&](double h, double v) {  // |ys - y_off| > h, then |xs - x_off| > v (offsets 0)
         P.add(OP_CHECK_ABS, {kInf, h}, ST_SLIT_HOR);
         P.add(OP_CHECK_ABS, {v, kInf}, ST_SLIT_VERT);
@@ -460,6 +463,8 @@ inline void build_shms(Program& P, double zdc[12])
     // shms/apertures_shms.inc:9-28
     const double rhb = 11.2 * 11.2, rq1 = 20.00 * 20.00, rq2 = 30.00 * 30.00, rq3 = 30.00 * 30.00,
                  rd1 = 30.00 * 30.00;
+    P.add(OP_SIEVE, {zd_hbin, 2.5, 1.64, 0.3, 0.0}, 0, 2);                        // :276-290
+    P.add(OP_SIEVE, {zd_hbin + zd_hbmen + zd_hbmex + zd_hbout + z_entr, 2.5, 1.64, 0.3, 0.0}, 0, 1);  // :293-310
     P.add(OP_PROJECT, {zd_hbin});                                                 // :422-423
     P.rot(OP_ROT_V, 1.5, 0.0, 1.51);
     P.add(OP_CHECK_T_HB, {rhb, 11.75, -4.13}, ST_S_HB_IN);

@@ -23,7 +23,7 @@ module simc_gpu
   type, bind(C) :: simc_settings
     real(c_double)     :: p_spec = 0, m2 = 0, ctau = 0
     integer(c_int32_t) :: ms_flag = 1, wcs_flag = 1, decay_flag = 0, using_coll = 0
-    integer(c_int32_t) :: fit_mode = SIMC_FIT_REF_F32, reserved = 0
+    integer(c_int32_t) :: fit_mode = SIMC_FIT_REF_F32, sieve = 0
     real(c_double)     :: wire_sigma = 0
   end type
 

@@ -19,6 +19,8 @@ tapes are regenerated from their seed by the tests (sha256 recorded here).
         radc_init_ev) + complete_main on per-event tapes, one process per case
     python tests/golden/make_golden.py lfit     # track fit (lfit/ref_lfit.npz): the
         reference's LFIT (cern/lfit.f) over seeded REAL*4 point sets
+    python tests/golden/make_golden.py sieve    # mock sieve slit (sieve/*.npz): the
+        reference built with use_sieve / use_2sieve / both on (make -C oracle sieve)
     python tests/golden/make_golden.py ntuple   # .bin ntuple (ntuple/*.npz): the file the
         reference's NtupleInit / NtupleClose write for the selected rows of
         seeded tables (elastic; pion with the target-field columns)
@@ -83,6 +85,41 @@ def main(only=None):
             settings=np.array([args.get("p_spec"), args.get("m2"), float(args.get("ms", True)),
                                float(args.get("wcs", True)), float(args.get("decay", False)),
                                float(args.get("coll", False)), args.get("ctau", 0.0)]),
+            **{"in_" + k: ev[k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")},
+            **{"out_" + k: out[k] for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp",
+                                             "dy_fp", "pathlen", "resmult", "m2", "ok", "used")},
+            counters=out["counters"], stop_mask=out["stop_mask"])
+        print(name, "accepted", int(out["ok"].sum()), "of", N)
+
+
+# The mock sieve slit: the reference built with use_sieve, use_2sieve or both
+# on (oracle/_ref/libsimcref_sieve1.so, _sieve2.so, _sieve3.so,
+# `make -C oracle sieve`).
+SIEVE_CASES = [
+    ("hms_proton_sieve", 1, W.HMS_C0, dict(p_spec=3000.0, m2=W.MP ** 2, sieve=1)),
+    ("shms_electron_sieve", 5, W.SHMS_C1, dict(p_spec=6000.0, m2=W.ME ** 2, sieve=1)),
+    ("shms_electron_sieve2", 5, W.SHMS_C1, dict(p_spec=6000.0, m2=W.ME ** 2, sieve=2)),
+    ("sos_proton_sieve", 2, W.SOS_1p5, dict(p_spec=1500.0, m2=W.MP ** 2, sieve=1)),
+    ("hrsr_proton_sieve", 3, W.HRSR_2, dict(p_spec=2000.0, m2=W.MP ** 2, sieve=1)),
+    ("hrsl_electron_sieve", 4, W.HRSL_3, dict(p_spec=3000.0, m2=W.ME ** 2, sieve=1)),
+    ("shms_electron_sieve3", 5, W.SHMS_C1, dict(p_spec=6000.0, m2=W.ME ** 2, sieve=3)),
+]
+
+
+def main_sieve():
+    os.makedirs(os.path.join(HERE, "sieve"), exist_ok=True)
+    for i, (name, arm, wl, kw) in enumerate(SIEVE_CASES):
+        ev = W.generate(wl, N, seed=wl.seed + 31 * i)
+        tape = W.uniform_tape(N, STRIDE, 950 + i)
+        args = dict(ms=True, wcs=True)
+        args.update(kw)
+        out = O.ref_run(arm, ev, tape=tape, stride=STRIDE, **args)
+        assert out["rc"] == 0, (name, out["rc"])
+        np.savez_compressed(
+            os.path.join(HERE, "sieve", name + ".npz"),
+            arm=arm, tape_seed=950 + i, stride=STRIDE, sieve=args["sieve"],
+            tape_sha256=hashlib.sha256(tape.tobytes()).hexdigest(),
+            settings=np.array([args.get("p_spec"), args.get("m2"), 1.0, 1.0, 0.0, 0.0, 0.0]),
             **{"in_" + k: ev[k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")},
             **{"out_" + k: out[k] for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp",
                                              "dy_fp", "pathlen", "resmult", "m2", "ok", "used")},
@@ -359,7 +396,9 @@ def main_ntuple():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["ntuple"]:
+    if sys.argv[1:] == ["sieve"]:
+        main_sieve()
+    elif sys.argv[1:] == ["ntuple"]:
         main_ntuple()
     elif sys.argv[1:] == ["gen"]:
         main_gen()

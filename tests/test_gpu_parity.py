@@ -162,9 +162,47 @@ def test_recon_only(ctx, oracle, arm):
         assert np.all(np.abs(a - b) <= 1e-12 * (np.abs(b) + s))
 
 
+SIEVE_CASES = [(wl, 1) for _, wl in CASES] + [(W.SHMS_C1, 2), (W.SHMS_C1, 3)]
+
+
+@pytest.mark.parametrize("wl,sieve", SIEVE_CASES, ids=[f"{wl.name}-sieve{s}" for wl, s in SIEVE_CASES])
+def test_sieve_philox_parity(ctx, oracle, wl, sieve):
+    """The mock sieve slit (hms/mc_hms.f:155-168, shms/mc_shms.f:276-310,
+    sos/mc_sos.f:122-137, hrs?/mc_hrs?.f:122-132) on Philox streams: flags,
+    stop codes, uniforms drawn and counters identical, coordinates within
+    1e-12; the forced angles are returned for every event, accepted or not."""
+    n = 30000
+    ev = W.generate(wl, n, seed=61 + sieve)
+    kw = dict(p_spec=wl.p_spec, m2=wl.m2, ms=True, wcs=True, wire_sigma=wl.wire_sigma, sieve=sieve)
+    ref = oracle.run(wl.arm, ev, rng="philox", seed=71, event0=5, fit_f32=False, **kw)
+    ctx.reset_counters(wl.arm)
+    out = ctx.run(wl.arm, to_dev(ev), rng="philox", seed=71, event0=5, fit="fp64", **kw)
+    torch.cuda.synchronize()
+    g = to_host(out)
+    print(wl.name, "sieve", sieve, flush=True)
+    rep = parity_report(g, ref, 1e-12)
+    print(rep)
+    assert rep["flag_mismatch"] == 0
+    assert all(v == 0 for v in rep["bad"].values()), rep
+    np.testing.assert_array_equal(g["stop"], ref["stop"])
+    np.testing.assert_array_equal(g["used"], ref["used"])
+    rej = g["stop"] != 0
+    for k in ("dxdz", "dydz"):
+        assert np.all(np.abs(g[k][rej] - ref[k][rej]) <= 1e-12 * (np.abs(ref[k][rej]) + 1e-2)), k
+        assert (g[k][rej] != ev[k][rej]).mean() > 0.9  # rejected events return the sieve's angles
+    cnt = ctx.counters(wl.arm)
+    for i, k in enumerate(cnt):
+        if k != "stop_id":
+            assert cnt[k] == ref["counters"][i], (k, cnt[k], ref["counters"][i])
+
+
 def test_errors(ctx):
     from simc_gfortran_amd import api
     ev = to_dev(W.generate(W.HMS_C0, 16))
+    for arm, sieve in ((1, 2), (1, 3), (2, 2), (5, 4), (5, -1)):
+        with pytest.raises(api.SimcError) as e:
+            ctx.run(arm, ev, p_spec=3000.0, m2=1.0, sieve=sieve)
+        assert e.value.code == -6
     with pytest.raises(api.SimcError) as e:
         ctx.run(6, ev, p_spec=3000.0, m2=1.0)
     assert e.value.code == -6
@@ -187,8 +225,9 @@ def test_tape_overflow_is_reported(ctx, oracle):
     assert (g["stop"] == 250).any()
 
 
-GOLDEN = sorted(__import__("glob").glob(__import__("os").path.join(
-    __import__("os").path.dirname(__file__), "golden", "*.npz")))
+_GDIR = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+# the default builds' vectors, then those of the builds with the mock sieve slit on
+GOLDEN = sorted(__import__("glob").glob(_GDIR + "/*.npz")) + sorted(__import__("glob").glob(_GDIR + "/sieve/*.npz"))
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
@@ -205,7 +244,8 @@ def test_gpu_matches_reference_golden(ctx, oracle, path):
     p_spec, m2, ms, wcs, decay, coll, ctau = g["settings"]
     arm = int(g["arm"])
     ev = {k: g["in_" + k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
-    kw = dict(p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs), decay=bool(decay), coll=bool(coll), ctau=ctau)
+    kw = dict(p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs), decay=bool(decay), coll=bool(coll), ctau=ctau,
+              sieve=int(g["sieve"]) if "sieve" in g.files else 0)
     ctx.reset_counters(arm)
     out = ctx.run(arm, to_dev(ev), rng="tape", tape=torch.from_numpy(tape).cuda(), stride=stride,
                   fit="ref_f32", **kw)

@@ -21,7 +21,13 @@ from simc_gfortran_amd import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+# the default builds' vectors, then those of the builds with the mock sieve slit
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz"))) + \
+    sorted(glob.glob(os.path.join(HERE, "golden", "sieve", "*.npz")))
+
+
+def _sieve(g):
+    return int(g["sieve"]) if "sieve" in g.files else 0
 
 
 # ---------------------------------------------------------------------------
@@ -38,7 +44,7 @@ def test_oracle_matches_reference_golden(oracle, path):
     ev = {k: g["in_" + k] for k in ("dpp", "x", "y", "z", "dxdz", "dydz", "fry")}
     out = oracle.run(int(g["arm"]), ev, p_spec=p_spec, m2=m2, ms=bool(ms), wcs=bool(wcs),
                      decay=bool(decay), coll=bool(coll), ctau=ctau, rng="tape", tape=tape,
-                     stride=stride, fit_f32=True)
+                     stride=stride, fit_f32=True, sieve=_sieve(g))
     assert out["rc"] == 0
     for k in ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen",
               "resmult", "m2", "ok", "used"):
@@ -68,7 +74,7 @@ def test_oracle_exit_points_match_reference_per_event(oracle, path):
     for i in range(n):
         out = oracle.run(int(g["arm"]), {k: v[i:i + 1] for k, v in ev.items()}, p_spec=p_spec, m2=m2,
                          ms=bool(ms), wcs=bool(wcs), decay=bool(decay), coll=bool(coll), ctau=ctau,
-                         rng="tape", tape=tape[i], stride=stride, fit_f32=True, nthreads=1)
+                         rng="tape", tape=tape[i], stride=stride, fit_f32=True, nthreads=1, sieve=_sieve(g))
         m = 0
         for c, v in enumerate(out["counters"]):
             if v and c != skip:
