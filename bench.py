@@ -133,7 +133,7 @@ def cpu_c0(seconds=8.0):
             "cpu_events_per_s_1_thread": rates[1], "cpu_events_per_s": rates[threads], "cores": threads}
 
 
-STAGE_NAMES = ["stage1_target_to_collimator", "stage2_magnets", "stage3_hut_fit_recon"]
+STAGE_NAMES = ["stage1_target_to_collimator", "stage2_magnets", "stage3_hut_fit", "recon_accepted"]
 
 
 def pmc_summary(fetch_csv, write_csv, events):
@@ -152,7 +152,9 @@ def pmc_summary(fetch_csv, write_csv, events):
             if "stage_kernel" in name:
                 key = STAGE_NAMES[0] if "false, 1, false, true" in name else \
                     STAGE_NAMES[1] if "false, 1, true, true" in name else \
-                    STAGE_NAMES[2] if "false, 2, true, false" in name else None
+                    STAGE_NAMES[2] if "false, 2, true, true" in name else None
+            elif "recon_list_kernel" in name:
+                key = STAGE_NAMES[3]
             elif "at::native" in name and "Mul" in name:
                 key = "copy"
             if key is None:
@@ -693,14 +695,14 @@ def bench_arms(ctx, n=10_000_000, reps=5):
         out = None
         st = []
         for r in range(reps + 1):
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
             for x in e:
                 x.record()
             out = ctx.run(wl.arm, ev, p_spec=wl.p_spec, m2=wl.m2, rng="philox", seed=11 + r, fit="fp64", out=out,
                           stage_events=e)
             torch.cuda.synchronize()
             if r:
-                st.append([e[k].elapsed_time(e[k + 1]) for k in range(3)])
+                st.append([e[k].elapsed_time(e[k + 1]) for k in range(4)])
         ms = np.mean(np.array(st), axis=0)
         res[wl.name] = {"arm": wl.arm, "events": n, "ms": float(ms.sum()), "events_per_s": n / (ms.sum() * 1e-3),
                         "stage_ms": [float(x) for x in ms], "accepted_fraction": float(out["ok"].float().mean())}
@@ -848,7 +850,7 @@ def main():
     ev_stop = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     # stage boundaries of every timed step (recorded once here so that the
     # underlying HIP events exist before their handles are passed down)
-    ev_stage = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    ev_stage = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
     for evs in ev_stage:
         for e in evs:
             e.record(stream)
@@ -863,16 +865,20 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_start, ev_stop)]))
-    stage_ms = [float(np.mean([evs[k].elapsed_time(evs[k + 1]) for evs in ev_stage])) for k in range(3)]
+    stage_ms = [float(np.mean([evs[k].elapsed_time(evs[k + 1]) for evs in ev_stage])) for k in range(4)]
 
     # algorithmic work of the last launch per kernel stage, from its stop codes
     stop = out["stop"].to(torch.int64)
     hist = torch.bincount(stop, minlength=256).cpu().numpy()
-    stage_flops = [float((hist * np.array(ctx.stop_flops(wl.arm, k + 1), dtype=np.float64)).sum())
-                   for k in range(3)]
+    # (the library's stage 3 count is the reconstruction of the accepted
+    # events, done by its own kernel; the hut holds no polynomial: its count
+    # is the FP64 arithmetic of the reference's hut routines)
+    fl = {k: float((hist * np.array(ctx.stop_flops(wl.arm, k), dtype=np.float64)).sum()) for k in (1, 2, 3)}
+    hut = float((hist * np.array(ctx.hut_flops(wl.arm, ms=True, wcs=True), dtype=np.float64)).sum())
+    stage_flops = [fl[1], fl[2], hut, fl[3]]
     names = STAGE_NAMES
     stages = {names[k]: {"ms": stage_ms[k], "algorithmic_flop": stage_flops[k],
-                         "TFLOPs": stage_flops[k] / (stage_ms[k] * 1e-3) / 1e12} for k in range(3)}
+                         "TFLOPs": stage_flops[k] / (stage_ms[k] * 1e-3) / 1e12} for k in range(4)}
     dom = int(np.argmax(stage_ms))
     accepted = int(hist[0])
     tflops = stage_flops[dom] / (stage_ms[dom] * 1e-3) / 1e12
@@ -903,7 +909,10 @@ def main():
         "roofline": {"bound": "fp64_valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": names[dom], "kernel_ms": stage_ms[dom],
-                     "algorithmic_flop_per_launch": stage_flops[dom]},
+                     "algorithmic_flop_per_launch": stage_flops[dom],
+                     "counted": "COSY polynomials (stages 1, 2 and the reconstruction); the hut's "
+                                "steps: the FP64 arithmetic of the reference's hut routines",
+                     "step_frac": sum(stage_flops) / (step_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "stages": stages,
         "step_ms_hip_events": step_ms,
         "accepted_fraction": accepted / n,

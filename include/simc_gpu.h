@@ -94,8 +94,10 @@ typedef struct {
  * mc_hms semantics and may alias the inputs: on success they receive the
  * reconstructed delta (%), y_tar (cm), x'_tar, y'_tar; on failure the input
  * values (hms/mc_hms.f:427-440).  pathlen accumulates.  x_fp..dy_fp are
- * written when the track fit was reached, else left untouched.  resmult, m2,
- * used may be NULL.  stop receives the exit point (SIMC_STOP_*), ok 1/0. */
+ * written when the track fit was reached, else left untouched.  resmult is
+ * set in the hut (hms/mc_hms_hut.f:292-297) and left untouched by events
+ * that stop before it.  resmult, m2, used may be NULL.  stop receives the
+ * exit point (SIMC_STOP_*), ok 1/0. */
 typedef struct {
     double *dpp, *y, *dxdz, *dydz;
     double *x_fp, *dx_fp, *y_fp, *dy_fp;
@@ -137,7 +139,7 @@ void simc_ctx_destroy(simc_ctx *ctx);
 
 /* Transport + reconstruct n events through one arm (mc_hms / mc_shms).
  * stream: a hipStream_t (NULL = default stream).  Asynchronous: returns after
- * the launch.  The context keeps one survivor queue per arm (about 100 bytes
+ * the launch.  The context keeps one survivor queue per arm (92 bytes
  * per event of a launch, at most 2^24 events, allocated on first use): runs of
  * the same arm on one context must be ordered (same stream, or synchronised
  * streams). */
@@ -165,7 +167,7 @@ int simc_arm_run(simc_ctx *ctx, int arm, const simc_settings *s, const simc_even
 /* simc_arm_run without any allocation (no counterpart in the reference: for
  * callers that cannot allocate device memory, such as code under HIP graph
  * capture or a tight serving loop).  The survivor queues live in a static
- * device array of the library (about 100 MiB per device, held until the
+ * device array of the library (about 92 MiB per device, held until the
  * process ends, its address set up by simc_ctx_create); the call allocates
  * nothing and only launches kernels on `stream`.  At most
  * SIMC_NOALLOC_MAX_EVENTS events per call (larger n: SIMC_E_ARG).  Every call
@@ -195,11 +197,11 @@ int simc_recon_run_host(simc_ctx *ctx, int arm, int64_t n, const double *x_fp, c
                         double *dxdz_tar, double *dydz_tar, double *y_tar);
 
 /* simc_arm_run with stage timing (no counterpart in the reference: a
- * measurement hook).  stage_events[0..3] are hipEvent_t handles created by
+ * measurement hook).  stage_events[0..4] are hipEvent_t handles created by
  * the caller; they are recorded on `stream` before stage 1 (target to
- * collimator), after stage 1, after stage 2 (the magnets) and after stage 3
- * (detector hut, fit, reconstruction) of the first launch chunk (2^24
- * events, fewer for long tapes). */
+ * collimator), after stage 1, after stage 2 (the magnets), after stage 3
+ * (detector hut and fit) and after the reconstruction of the accepted
+ * events, in the first launch chunk (2^24 events, fewer for long tapes). */
 int simc_arm_run_timed(simc_ctx *ctx, int arm, const simc_settings *s, const simc_events_in *in,
                        simc_events_out *out, int64_t n, const simc_rng *rng, void *stream,
                        void *const *stage_events);
@@ -539,8 +541,24 @@ int64_t simc_poly_flops_full(simc_ctx *ctx, int arm);
  * flops[c] for c = 0..255 (nflops >= 256): the segments it passes through. */
 int simc_stop_flops(simc_ctx *ctx, int arm, int64_t *flops, int nflops);
 /* The same restricted to the segments of one kernel stage (1: target to
- * collimator, 2: the magnets, 3: detector hut and reconstruction; 0: all). */
+ * collimator, 2: the magnets, 3: the reconstruction of accepted events;
+ * 0: all). */
 int simc_stop_flops_stage(simc_ctx *ctx, int arm, int stage, int64_t *flops, int nflops);
+
+/* FP64 operations of the detector hut for an event that ends with stop code
+ * c, written to flops[c] (no counterpart in the reference: a measurement
+ * hook).  The hut holds no COSY polynomial, so the counts above hold none of
+ * its work; this one counts the arithmetic of the reference's hut routines
+ * for the steps the event executes: a drift 11 (shared/project.f:39-41), a
+ * scattering 18 (shared/musc.f:48-56) or 36 (shared/musc_ext.f:36-51) plus
+ * its normal deviates, a wire plane 3 plus its two deviates
+ * (shms/mc_shms_hut.f:183-192), the two track fits 11 per plane + 18
+ * (cern/lfit.f:11-59), a rotated box 13 (shared/rotate_haxis.f:56-61), the
+ * calorimeter fiducial cut 4; +, -, *, /, sqrt, log and sin/cos one each,
+ * 4 per normal deviate (one Box-Muller pair: ln, *-2, sqrt, 2 pi u, sin,
+ * cos, two products).  Scattering and wire deviates count as ms_flag and
+ * wcs_flag say.  Stop codes before the hut: 0.  nflops >= 256. */
+int simc_stop_flops_hut(simc_ctx *ctx, int arm, int ms_flag, int wcs_flag, int64_t *flops, int nflops);
 
 const char *simc_last_error(void);
 const char *simc_version(void);

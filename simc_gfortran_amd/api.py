@@ -64,7 +64,7 @@ EXPORTS = [
     "simc_hms_run", "simc_shms_run", "simc_sos_run", "simc_hrsr_run", "simc_hrsl_run", "simc_arm_run",
     "simc_arm_run_noalloc", "simc_arm_run_timed", "simc_recon_run", "simc_counters_get",
     "simc_counters_reset", "simc_map_class_info", "simc_poly_flops_full", "simc_stop_flops",
-    "simc_stop_flops_stage", "simc_arm_run_host", "simc_recon_run_host",
+    "simc_stop_flops_stage", "simc_stop_flops_hut", "simc_arm_run_host", "simc_recon_run_host",
     "simc_batch_create", "simc_batch_run", "simc_batch_counters", "simc_batch_set_chunk",
     "simc_batch_tune", "simc_batch_destroy", "simc_batch_passes", "simc_adjoint_run", "simc_recon_coefficients",
     "simc_target_prep", "simc_target_prep_host", "simc_gen_hep", "simc_gen_hep_host", "simc_gen_hep_rad",
@@ -288,6 +288,7 @@ def library(path: str = LIB_PATH):
     L.simc_poly_flops_full.restype = C.c_int64
     L.simc_stop_flops.argtypes = [_vp, C.c_int, C.POINTER(C.c_int64), C.c_int]
     L.simc_stop_flops_stage.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int]
+    L.simc_stop_flops_hut.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int]
     L.simc_last_error.restype = C.c_char_p
     L.simc_version.restype = C.c_char_p
     _lib = L
@@ -385,8 +386,9 @@ class Spectrometers:
         Returns a dict of device tensors (see include/simc_gpu.h for semantics):
         dpp, y, dxdz, dydz (recon or input), x_fp, dx_fp, y_fp, dy_fp, pathlen,
         resmult, m2, ok (uint8), stop (uint8), used (int32).
-        stage_events: optional four recorded torch.cuda.Event(enable_timing=True)
-        marking the boundaries of the three kernel stages (first launch chunk).
+        stage_events: optional five recorded torch.cuda.Event(enable_timing=True)
+        marking the boundaries of the three kernel stages and the
+        reconstruction kernel (first launch chunk).
         sieve: the mock sieve slit (the reference's compile-time use_sieve in
         each mc_*.f), a bit mask: 0 off, 1 the arm's sieve (use_sieve), 2 the
         SHMS's second sieve in front of the horizontal bender (mc_shms.f
@@ -421,9 +423,9 @@ class Spectrometers:
             _check(library().simc_arm_run(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout), n,
                                           C.byref(R), C.c_void_p(st)))
         else:
-            if len(stage_events) != 4 or any(e.cuda_event == 0 for e in stage_events):
-                raise ValueError("stage_events: four events, each recorded once beforehand")
-            evs = (_vp * 4)(*[e.cuda_event for e in stage_events])
+            if len(stage_events) != 5 or any(e.cuda_event == 0 for e in stage_events):
+                raise ValueError("stage_events: five events, each recorded once beforehand")
+            evs = (_vp * 5)(*[e.cuda_event for e in stage_events])
             _check(library().simc_arm_run_timed(self._h, arm, C.byref(S), C.byref(ein), C.byref(eout),
                                                 n, C.byref(R), C.c_void_p(st), evs))
         return out
@@ -662,9 +664,17 @@ class Spectrometers:
 
     def stop_flops(self, arm, stage=0):
         """Algorithmic polynomial FP64 operations of an event, by stop code
-        (stage 0: the whole path; 1, 2, 3: one kernel stage)."""
+        (stage 0: the whole path; 1, 2: the first two kernel stages; 3: the
+        reconstruction of an accepted event)."""
         buf = (C.c_int64 * 256)()
         _check(library().simc_stop_flops_stage(self._h, arm, stage, buf, 256))
+        return list(buf)
+
+    def hut_flops(self, arm, ms=True, wcs=True):
+        """FP64 operations of the detector hut's steps for an event, by stop
+        code (simc_stop_flops_hut)."""
+        buf = (C.c_int64 * 256)()
+        _check(library().simc_stop_flops_hut(self._h, arm, int(ms), int(wcs), buf, 256))
         return list(buf)
 
 

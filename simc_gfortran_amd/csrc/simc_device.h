@@ -193,6 +193,15 @@ struct PhiloxStreamT {
     // Stream position for a later resume: uniforms drawn, and the pending
     // Box-Muller value.
     __device__ __forceinline__ uint32_t state_word() const { return pos | ((uint32_t)have << 31); }
+    // The block's words are consumed from w[0], shifting the rest down: an
+    // index by the position would put w in scratch memory (one round trip
+    // per uniform), the shift is three register moves.
+    __device__ __forceinline__ void shift()
+    {
+        w[0] = w[1];
+        w[1] = w[2];
+        w[2] = w[3];
+    }
     // Resume at a saved position (re-forms the current Philox block when the
     // position is inside one).
     __device__ __forceinline__ void resume(uint32_t word, double pend)
@@ -200,14 +209,19 @@ struct PhiloxStreamT {
         pos = word & 0x7fffffffu;
         have = (int)(word >> 31);
         pending = pend;
-        if (pos & 3u) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
+        if (pos & 3u) {
+            philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
+            if ((pos & 3u) >= 1u) shift();
+            if ((pos & 3u) >= 2u) shift();
+            if ((pos & 3u) >= 3u) shift();
+        }
     }
     __device__ __forceinline__ double next()
     {
-        const uint32_t j = pos & 3u;
-        if (j == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
+        if ((pos & 3u) == 0) philox4x32_10<LANE_KEY>(pos >> 2, e0, e1, TAG, k0, k1, w);
         ++pos;
-        const uint32_t v = j == 0 ? w[0] : j == 1 ? w[1] : j == 2 ? w[2] : w[3];
+        const uint32_t v = w[0];
+        shift();
         return __builtin_fma((double)v, 0x1p-32, 0x1p-33);  // (v + 1/2) 2^-32, exact
     }
     // Normal deviates for the counter-based stream: Box-Muller pairs

@@ -464,11 +464,15 @@ __device__ __forceinline__ int run_program(R& g, Track& t, const KArgs& a, const
             if ((fabs(xt - p0) > p1) || ((fabs(yt) + ops[pc].p[2] * xt) > ops[pc].p[3])) st = ops[pc].stop;
             break;
         case OP_SIEVE:
+            // compiled into the RARE variants only (a run with the sieve on
+            // launches them), keeping its registers out of the common ones;
             // the forced angles are also the INOUT dxdz, dydz of the event
-            if (F.sieve & ops[pc].arg) {
-                sieve_slit(g, t, (int)ops[pc].p[4], p0, p1, ops[pc].p[2], ops[pc].p[3]);
-                st64(a.o_dxdz, a.n_arr, off, t.dxdzs);
-                st64(a.o_dydz, a.n_arr, off, t.dydzs);
+            if constexpr (RARE) {
+                if (F.sieve & ops[pc].arg) {
+                    sieve_slit(g, t, (int)ops[pc].p[4], p0, p1, ops[pc].p[2], ops[pc].p[3]);
+                    st64(a.o_dxdz, a.n_arr, off, t.dxdzs);
+                    st64(a.o_dydz, a.n_arr, off, t.dydzs);
+                }
             }
             break;
         default:
@@ -638,7 +642,7 @@ __device__ __forceinline__ int run_hut(R& g, Track& t, const KArgs& a, const Fla
 struct Queue {
     uint32_t* count;     // entries appended (device counter, zeroed per launch)
     uint32_t *idx, *rng, *bits;   // launch event index, stream word, dflag | setting << 1
-    double *pend, *xs, *ys, *dxdzs, *dydzs, *dpps, *p, *m2, *pathlen, *decdist, *resmult;
+    double *pend, *xs, *ys, *dxdzs, *dydzs, *dpps, *p, *m2, *pathlen, *decdist;
     uint32_t cap;
 };
 
@@ -792,18 +796,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             const SetDev S = setting_of(a, s);
             const EvId e = event_id(a, S, s, li);
             g.init(R::kLaneKey ? S.seed : a.seed, e.event, a.tape, li * a.stride_bytes, a.stride_bytes);
-            g.resume(ld32(qin.rng, c, o4), ld64(qin.pend, c, o8));
+            // the pending normal is stored only when there is one (bit 31)
+            const uint32_t word = ld32(qin.rng, c, o4);
+            g.resume(word, (word >> 31) ? ld64(qin.pend, c, o8) : 0.0);
             t.xs = ld64(qin.xs, c, o8);
             t.ys = ld64(qin.ys, c, o8);
             t.dxdzs = ld64(qin.dxdzs, c, o8);
             t.dydzs = ld64(qin.dydzs, c, o8);
             t.dpps = ld64(qin.dpps, c, o8);
-            t.p = ld64(qin.p, c, o8);
-            t.m2 = ld64(qin.m2, c, o8);
             t.pathlen = ld64(qin.pathlen, c, o8);
-            t.decdist = ld64(qin.decdist, c, o8);
+            if constexpr (RARE) {
+                // decay and collimator stepping change the momentum and mass
+                t.p = ld64(qin.p, c, o8);
+                t.m2 = ld64(qin.m2, c, o8);
+                t.decdist = ld64(qin.decdist, c, o8);
+            } else {
+                // as stage 1 forms them: the same bits
+                t.p = S.p_spec * (1.0 + t.dpps / 100.0);
+                t.m2 = S.m2;
+                t.decdist = 0.0;
+            }
             t.dflag = (int)(bits & 1u);
-            resmult = ld64(qin.resmult, c, o8);
+            // resmult is drawn in the hut only: until then it is the
+            // caller's INOUT value, still in place at the event's index
+            resmult = a.resmult ? ld64(a.resmult, a.n_arr, e.off) : 0.0;
         } else {
             li = i;
             s = find_setting(a, a.pass_base + li);
@@ -871,18 +887,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1
             st32(qout.idx, c, o4, li);
             st32(qout.bits, c, o4, (uint32_t)t.dflag | (s << 1));
             if (STAGE != STAGE_HUT) {
-                st32(qout.rng, c, o4, g.state_word());
-                st64(qout.pend, c, o8, g.pending);
+                // 60 bytes per entry (84 in the RARE variants), plus 8 when
+                // a Box-Muller normal is pending
+                const uint32_t word = g.state_word();
+                st32(qout.rng, c, o4, word);
+                if (word >> 31) st64(qout.pend, c, o8, g.pending);
                 st64(qout.xs, c, o8, t.xs);
                 st64(qout.ys, c, o8, t.ys);
                 st64(qout.dxdzs, c, o8, t.dxdzs);
                 st64(qout.dydzs, c, o8, t.dydzs);
                 st64(qout.dpps, c, o8, t.dpps);
-                st64(qout.p, c, o8, t.p);
-                st64(qout.m2, c, o8, t.m2);
                 st64(qout.pathlen, c, o8, t.pathlen);
-                st64(qout.decdist, c, o8, t.decdist);
-                st64(qout.resmult, c, o8, resmult);
+                if constexpr (RARE) {
+                    st64(qout.p, c, o8, t.p);
+                    st64(qout.m2, c, o8, t.m2);
+                    st64(qout.decdist, c, o8, t.decdist);
+                }
             }
         }
     }
@@ -1732,6 +1752,7 @@ struct simc_ctx {
     int n_ops[kNumArms] = {};
     double zdc[kNumArms][12];
     unsigned long long* d_hist[kNumArms] = {};
+    int lead_pc[kNumArms] = {};      // first step after the leading sieve steps
     int coll_pc[kNumArms] = {};      // first step after the collimator
     int hut_pc[kNumArms] = {};       // first step of the detector hut
     // survivor queues between the stages of plain runs (per arm)
@@ -1904,6 +1925,9 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
             return fail(SIMC_E_INTERNAL, "step " + std::to_string(k) + " outside its stage");
     }
     ctx->hut_pc[ai] = hut;
+    int lead = 0;
+    while (lead < hut && P.ops[lead].code == OP_SIEVE) ++lead;
+    ctx->lead_pc[ai] = lead;
     rc = check_compiled_hut(arm, P, hut);
     if (rc) return rc;
     // Stage 1 ends after the collimator; an arm without one (the HRS: slit
@@ -1943,15 +1967,15 @@ int load_arm(simc_ctx* ctx, int arm, const std::string& fwd_path, const std::str
 }
 
 // The two survivor queues of arm index ai, room for `cap` entries each.
-constexpr size_t kQueueEntryBytes = 11 * 8 + 3 * 4;
+constexpr size_t kQueueEntryBytes = 10 * 8 + 3 * 4;
 
 void queue_view(void* base, uint32_t c, uint32_t* count, Queue& q)
 {
     double* d = (double*)base;
-    double** dp[11] = {&q.pend, &q.xs, &q.ys, &q.dxdzs, &q.dydzs, &q.dpps, &q.p, &q.m2, &q.pathlen,
-                       &q.decdist, &q.resmult};
-    for (int k = 0; k < 11; ++k) *dp[k] = d + (size_t)k * c;
-    uint32_t* u = (uint32_t*)(d + (size_t)11 * c);
+    double** dp[10] = {&q.pend, &q.xs, &q.ys, &q.dxdzs, &q.dydzs, &q.dpps, &q.p, &q.m2, &q.pathlen,
+                       &q.decdist};
+    for (int k = 0; k < 10; ++k) *dp[k] = d + (size_t)k * c;
+    uint32_t* u = (uint32_t*)(d + (size_t)10 * c);
     q.idx = u;
     q.rng = u + c;
     q.bits = u + 2 * (size_t)c;
@@ -2001,7 +2025,7 @@ int ensure_queues(simc_ctx* ctx, int ai, uint32_t cap, Queue& q1, Queue& q2)
 // survivor queues of one launch of up to SIMC_NOALLOC_MAX_EVENTS events, after
 // 256 bytes that hold their two counters.  Each device holds its own copy for
 // the life of the process; every call on that device shares it.
-constexpr size_t kScratchQueueBytes = (size_t)SIMC_NOALLOC_MAX_EVENTS * (11 * 8 + 3 * 4);
+constexpr size_t kScratchQueueBytes = (size_t)SIMC_NOALLOC_MAX_EVENTS * (10 * 8 + 3 * 4);
 static_assert(kScratchQueueBytes % 256 == 0, "queue arrays stay 256-byte aligned");
 static __device__ __align__(256) unsigned char scratch[256 + 2 * kScratchQueueBytes];
 
@@ -2267,7 +2291,8 @@ bool coll_stepping(int arm, const simc_settings* s)
 }
 bool needs_rare(int arm, const simc_settings* s)
 {
-    return s->decay_flag || (coll_stepping(arm, s) && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0);
+    return s->decay_flag || (coll_stepping(arm, s) && s->m2 > 100.0 * 100.0 && s->m2 < 200.0 * 200.0) ||
+           s->sieve != 0;
 }
 
 SetDev make_setdev(int arm, const simc_settings* s, uint64_t seed, uint64_t event0, uint64_t start,
@@ -2300,13 +2325,15 @@ bool outputs_complete(const simc_events_in* in, const simc_events_out* out)
            out->stop;
 }
 
-// One launch chunk: counter reset and the three stages on `st`; `ev` (four
-// events or nullptr) marks the stage boundaries.
+// One launch chunk: counter reset, the three stages and the reconstruction on
+// `st`; `ev` (five events or nullptr) marks the kernel boundaries.
 int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, const Queue& q1, const Queue& q2,
                   hipStream_t st, hipEvent_t* ev, bool lane_key = false)
 {
     const int ai = arm_index(arm);
     const int32_t coll = ctx->coll_pc[ai], hut = ctx->hut_pc[ai];
+    // the leading sieve steps are no-ops in the common variants
+    const int32_t first = rare ? 0 : ctx->lead_pc[ai];
     const dim3 G((a.count + kBlock - 1) / kBlock), B(kBlock);
     // the accepted list of the hut stage reuses queue 1 (consumed by then)
     // with the third counter
@@ -2317,15 +2344,16 @@ int launch_stages(simc_ctx* ctx, int arm, bool rare, bool tape, const KArgs& a, 
 #define LAUNCH3(ARMV, RV, RARE)                                                                             \
     do {                                                                                                    \
         hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, false, true>), G, B, 0, st, a, q2, q1, \
-                           0, coll);                                                                        \
+                           first, coll);                                                                    \
         if (ev) HIPCHK(hipEventRecord(ev[1], st));                                                          \
         hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_MAGNETS, true, true>), G, B, 0, st, a, q1, q2,  \
                            coll, hut);                                                                      \
         if (ev) HIPCHK(hipEventRecord(ev[2], st));                                                          \
         hipLaunchKernelGGL((stage_kernel<ARMV, RV, RARE, STAGE_HUT, true, true>), G, B, 0, st, a, q2, acc,     \
                            hut, 0x7fffffff);                                                                \
-        hipLaunchKernelGGL((recon_list_kernel<ARMV>), G, B, 0, st, a, acc);                                 \
         if (ev) HIPCHK(hipEventRecord(ev[3], st));                                                          \
+        hipLaunchKernelGGL((recon_list_kernel<ARMV>), G, B, 0, st, a, acc);                                 \
+        if (ev) HIPCHK(hipEventRecord(ev[4], st));                                                          \
     } while (0)
 #define LAUNCH1(ARMV, RV) do { if (rare) LAUNCH3(ARMV, RV, true); else LAUNCH3(ARMV, RV, false); } while (0)
 #define LAUNCHR(ARMV)                                                                                       \
@@ -2413,7 +2441,7 @@ int arm_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_events_in
         a.stop = out->stop + b;
         a.used = out->used ? out->used + b : nullptr;
         a.tape = tape ? rng->tape + b * rng->stride : nullptr;
-        // stage_events: boundaries of the first chunk's three stages
+        // stage_events: boundaries of the first chunk's four kernels
         hipEvent_t* ev = (stage_events && b == 0) ? (hipEvent_t*)stage_events : nullptr;
         rc = launch_stages(ctx, arm, rare, tape, a, q1, q2, st, ev);
         if (rc) return rc;
@@ -2448,8 +2476,10 @@ int simc_arm_run_timed(simc_ctx* ctx, int arm, const simc_settings* s, const sim
                        simc_events_out* out, int64_t n, const simc_rng* rng, void* stream,
                        void* const* stage_events)
 {
-    if (!stage_events || !stage_events[0] || !stage_events[1] || !stage_events[2] || !stage_events[3])
-        return fail(SIMC_E_ARG, "simc_arm_run_timed: four events required");
+    if (!stage_events)
+        return fail(SIMC_E_ARG, "simc_arm_run_timed: five events required");
+    for (int k = 0; k < 5; ++k)
+        if (!stage_events[k]) return fail(SIMC_E_ARG, "simc_arm_run_timed: five events required");
     return arm_run(ctx, arm, s, in, out, n, rng, stream, stage_events);
 }
 
@@ -2468,7 +2498,8 @@ int simc_arm_run_host(simc_ctx* ctx, int arm, const simc_settings* s, const simc
     HIPCHK(hipSetDevice(ctx->device));
     // Columns of a chunk, in the order of the slot's layout: read only (the
     // inputs), read and written (the fitted track, kept where the fit is not
-    // reached, and the accumulated path length), written only.  One H2D copy
+    // reached, the accumulated path length, and resmult, kept by events that
+    // stop before the hut), written only.  One H2D copy
     // covers the first two groups, one D2H copy the last two.
     struct Col { const void* src; void* dst; size_t esz; };
     std::vector<Col> cols;
@@ -2476,11 +2507,11 @@ int simc_arm_run_host(simc_ctx* ctx, int arm, const simc_settings* s, const simc
     for (const double* p : hin)
         if (p) cols.push_back({p, nullptr, 8});
     const size_t n_in = cols.size();
-    double* hio[5] = {out->x_fp, out->dx_fp, out->y_fp, out->dy_fp, out->pathlen};
+    double* hio[6] = {out->x_fp, out->dx_fp, out->y_fp, out->dy_fp, out->pathlen, out->resmult};
     for (double* p : hio)
         if (p) cols.push_back({p, p, 8});
     const size_t n_io = cols.size() - n_in;
-    double* hw[6] = {out->dpp, out->y, out->dxdz, out->dydz, out->resmult, out->m2};
+    double* hw[5] = {out->dpp, out->y, out->dxdz, out->dydz, out->m2};
     for (double* p : hw)
         if (p) cols.push_back({nullptr, p, 8});
     cols.push_back({nullptr, out->ok, 1});    // always computed; copied back when asked for
@@ -3622,7 +3653,7 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
     const int ai = arm_index(arm);
     if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
     if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
-    if (stage < 0 || stage > 3) return fail(SIMC_E_ARG, "stage must be 0 (all), 1, 2 or 3");
+    if (stage < 0 || stage > 3) return fail(SIMC_E_ARG, "stage must be 0 (all), 1, 2 or 3 (reconstruction)");
     if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
     for (int i = 0; i < nflops; ++i) flops[i] = 0;
     int64_t acc = 0;
@@ -3637,6 +3668,41 @@ int simc_stop_flops_stage(simc_ctx* ctx, int arm, int stage, int64_t* flops, int
             flops[o.stop] = acc;
         }
         if (o.code == OP_END) flops[ST_OK] = acc + (stage == 0 || stage == 3 ? class_flops(ctx, arm, 0) : 0);
+    }
+    return SIMC_OK;
+}
+
+int simc_stop_flops_hut(simc_ctx* ctx, int arm, int ms_flag, int wcs_flag, int64_t* flops, int nflops)
+{
+    if (!ctx || !flops) return fail(SIMC_E_ARG, "simc_stop_flops_hut: NULL argument");
+    const int ai = arm_index(arm);
+    if (ai < 0) return fail(SIMC_E_ARG, "arm must be 1 (HMS), 2 (SOS), 3 (HRS-R), 4 (HRS-L) or 5 (SHMS)");
+    if (!ctx->d_coef[ai]) return fail(SIMC_E_ARG, "the arm's maps are not loaded (simc_ctx_load_arm)");
+    if (nflops < kHistBins) return fail(SIMC_E_ARG, "flops array needs 256 entries");
+    for (int i = 0; i < nflops; ++i) flops[i] = 0;
+    const std::vector<Op>& ops = ctx->ops[ai];
+    const int64_t normal = 4;  // per deviate: half a Box-Muller pair
+    int planes = 0;
+    for (int pc = ctx->hut_pc[ai]; pc < (int)ops.size(); ++pc) planes += ops[pc].code == OP_WIRE;
+    int64_t acc = 0;
+    for (int pc = ctx->hut_pc[ai]; pc < (int)ops.size(); ++pc) {
+        const Op& o = ops[pc];
+        switch (o.code) {
+        case OP_PROJECT: acc += 11; break;
+        case OP_MUSC:
+            if (ms_flag && o.p[0] != 0.0) acc += 18 + 2 * normal;
+            break;
+        case OP_MUSC_EXT:
+            if (ms_flag && o.p[0] != 0.0 && o.p[1] > 0.0) acc += 36 + 4 * normal;
+            break;
+        case OP_WIRE: acc += 3 + (wcs_flag ? 2 * normal : 0); break;
+        case OP_FIT: acc += 11 * planes + 18; break;
+        case OP_BOX_ROT_H: acc += 13; break;
+        case OP_FIDUCIAL: acc += 4; break;
+        default: break;  // comparisons and counters
+        }
+        if (o.code == OP_END) flops[ST_OK] = acc;
+        else if (o.stop > 0 && o.code != OP_WIRE && o.stop < kHistBins) flops[o.stop] = acc;
     }
     return SIMC_OK;
 }
@@ -3671,8 +3737,8 @@ int simc_adjoint_run(simc_ctx* ctx, int arm, const simc_settings* s, const simc_
     const int ai = arm_index(arm);
     if (arm != SIMC_ARM_HMS && arm != SIMC_ARM_SHMS)
         return fail(SIMC_E_ARG, "the adjoint form covers the HMS and the SHMS");
-    if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
     if (s->sieve) return fail(SIMC_E_ARG, "the adjoint form has no mock sieve slit");
+    if (needs_rare(arm, s)) return fail(SIMC_E_ARG, "the adjoint form has no decay or collimator stepping");
     if (ctx->d_coef[ai] && ctx->d_gen[ai])
         return fail(SIMC_E_MAPSTRUCT, "the adjoint form needs maps of the compiled structure");
     if (!in->dpp || !in->x || !in->y || !in->dxdz || !in->dydz) return fail(SIMC_E_ARG, "missing event input array");

@@ -410,7 +410,7 @@ module simc_gpu
       type(simc_events_out), intent(in) :: evout
       integer(c_int64_t), value :: n
       type(simc_rng), intent(in) :: rng
-      type(c_ptr), intent(in) :: stage_events(4)
+      type(c_ptr), intent(in) :: stage_events(5)
     end function
 
     integer(c_int) function simc_map_class_info(ctx, arm, cls, driftdist, adrift, n_terms) &
@@ -440,6 +440,14 @@ module simc_gpu
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: arm, stage, nflops
+      integer(c_int64_t), intent(out) :: flops(*)
+    end function
+
+    integer(c_int) function simc_stop_flops_hut(ctx, arm, ms_flag, wcs_flag, flops, nflops) &
+        bind(C, name="simc_stop_flops_hut")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: arm, ms_flag, wcs_flag, nflops
       integer(c_int64_t), intent(out) :: flops(*)
     end function
 

@@ -10,7 +10,8 @@ from simc_gfortran_amd import workloads as W
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-FIELDS = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "ok", "stop", "used")
+FIELDS = ("dpp", "y", "dxdz", "dydz", "x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult", "m2", "ok", "stop",
+          "used")
 
 
 @pytest.fixture(scope="module")
@@ -49,10 +50,11 @@ def test_host_entry_tape_and_inout(ctx):
     tape = W.uniform_tape(n, stride, 99)
     d = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in ev.items()}
     # INOUT: the fitted track keeps the caller's values where the fit is not
-    # reached, and pathlen accumulates onto them
-    init = {k: np.full(n, 7.25) for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen")}
+    # reached, pathlen accumulates onto them, resmult is kept by events that
+    # stop before the hut
+    init = {k: np.full(n, 7.25) for k in ("x_fp", "dx_fp", "y_fp", "dy_fp", "pathlen", "resmult")}
     dev_out = {k: torch.from_numpy(v.copy()).cuda() for k, v in init.items()}
-    for k in ("dpp", "y", "dxdz", "dydz", "resmult", "m2"):
+    for k in ("dpp", "y", "dxdz", "dydz", "m2"):
         dev_out[k] = torch.empty(n, dtype=torch.float64, device="cuda")
     dev_out["ok"] = torch.empty(n, dtype=torch.uint8, device="cuda")
     dev_out["stop"] = torch.empty(n, dtype=torch.uint8, device="cuda")
@@ -62,7 +64,7 @@ def test_host_entry_tape_and_inout(ctx):
     torch.cuda.synchronize()
     ref = {k: v.cpu().numpy() for k, v in ref.items() if v is not None}
     out = {k: v.copy() for k, v in init.items()}
-    for k in ("dpp", "y", "dxdz", "dydz", "resmult", "m2"):
+    for k in ("dpp", "y", "dxdz", "dydz", "m2"):
         out[k] = np.zeros(n)
     out["ok"] = np.zeros(n, np.uint8)
     out["stop"] = np.zeros(n, np.uint8)
@@ -72,6 +74,8 @@ def test_host_entry_tape_and_inout(ctx):
         np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
     before_hut = (got["stop"] >= 1) & (got["stop"] <= 70)  # SHMS stops up to the dipole exit
     assert before_hut.sum() > 100 and (got["x_fp"][before_hut] == 7.25).all()  # kept: no fit reached
+    in_hut = (got["stop"] == 0) | ((got["stop"] > 70) & (got["stop"] < 250))
+    assert (got["resmult"][before_hut] == 7.25).all() and np.isin(got["resmult"][in_hut], (1.0, 2.0)).all()
     assert (got["pathlen"] > 7.25).all()
 
 
