@@ -757,10 +757,11 @@ __device__ __forceinline__ void count_stop(const KArgs& a, unsigned* hist, uint3
 // TO_Q: survivors go to queue qout (else every event is finished here); in
 // the hut stage, TO_Q lists the accepted events (index and setting) for
 // recon_list_kernel instead of reconstructing them here.
-// The common variants are held to 168 VGPRs (three wavefronts per SIMD),
-// which the hut stage would exceed slightly on its own.
+// The common variants are held to three wavefronts per SIMD (168 VGPRs),
+// the hut stage to four (128 VGPRs: a few values spill to scratch, and the
+// extra wavefront hides more of the hut's latency than the spills cost).
 template <int ARM, class R, bool RARE, int STAGE, bool FROM_Q, bool TO_Q>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : 3))) void stage_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : (STAGE == STAGE_HUT ? 4 : 3)))) void stage_kernel(
     KArgs a, Queue qin, Queue qout, int32_t pc_begin, int32_t pc_end)
 {
     __shared__ unsigned hist[kHistSlots];
