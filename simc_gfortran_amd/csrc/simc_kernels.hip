@@ -759,9 +759,11 @@ __device__ __forceinline__ void count_stop(const KArgs& a, unsigned* hist, uint3
 // recon_list_kernel instead of reconstructing them here.
 // The common variants are held to three wavefronts per SIMD (168 VGPRs),
 // the hut stage to four (128 VGPRs: a few values spill to scratch, and the
-// extra wavefront hides more of the hut's latency than the spills cost).
+// extra wavefront hides more of the hut's latency than the spills cost), the
+// RARE variants to two (256 VGPRs, a few spills; at one wavefront a pion run
+// with decay took 1.45 x as long).
 template <int ARM, class R, bool RARE, int STAGE, bool FROM_Q, bool TO_Q>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 1 : (STAGE == STAGE_HUT ? 4 : 3)))) void stage_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RARE ? 2 : (STAGE == STAGE_HUT ? 4 : 3)))) void stage_kernel(
     KArgs a, Queue qin, Queue qout, int32_t pc_begin, int32_t pc_end)
 {
     __shared__ unsigned hist[kHistSlots];
@@ -1493,8 +1495,11 @@ struct GenRadArgs {
     int32_t* used;
 };
 
+// Held to three wavefronts per SIMD (168 VGPRs, about 260 values spilled):
+// at two (its unconstrained 276 registers) the kernel waited on its
+// elementary functions' latency and took 1.8 x as long.
 template <class R>
-__global__ __launch_bounds__(256) void gen_hep_rad_kernel(GenRadArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gen_hep_rad_kernel(GenRadArgs a)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.n) return;
